@@ -1,0 +1,166 @@
+/*
+ * C ABI of the MI355X delta-SPH engine (libsph_hip.so).
+ *
+ * The engine replaces the device side of pySurf's SPH backend, sphGpu.PairSums
+ * (src/pysurf/sphGpu.py:1492-2323), which WcsphSolver selects through SimulationConfig.backend
+ * (src/pysurf/sphSolver.py:425, 1499-1518). Every entry point below names the PairSums method it
+ * replaces. The Python mirror of PairSums (pysurf_amd/pair_sums.py) binds these through ctypes.
+ *
+ * Conventions:
+ *  - Every pointer into particle data is a device pointer that the caller owns (PyTorch-ROCm
+ *    tensors in the Python mirror). The library never allocates or frees caller memory; its own
+ *    scratch (cell lists, sorted float32 records, per-run clocks) lives in the context.
+ *  - Particle arrays are structure-of-arrays: a vector field is `dims` rows of `stride` elements.
+ *  - "Real" arrays are float for precision 0 (float32 pair sums over float64 state, the
+ *    reference's own split, sphGpu.py:28-38) and double for precision 1 (float64 throughout).
+ *  - All work is stream-ordered on the context's stream; nothing synchronizes except
+ *    sph_get_clock, sph_synchronize and the neighbor-list export.
+ *  - Every entry point returns SPH_OK or an error status; sph_last_error gives the message.
+ *    SPH_ERR_VALUE corresponds to the reference's ValueError, SPH_ERR_HIP to a HIP runtime error.
+ */
+#ifndef SPH_HIP_H
+#define SPH_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPH_OK 0
+#define SPH_ERR_VALUE 1
+#define SPH_ERR_HIP 2
+
+typedef struct sph_context sph_context;
+
+/* The boundary a run's device step enforces: PairSums.setBoundary (sphGpu.py:2121-2171). Table
+ * pointers are device pointers the caller keeps alive while the run is configured. */
+typedef struct {
+    int32_t kind;          /* 0 none, 1 container, 2 bed, 3 height field */
+    double clearance;      /* a quarter spacing [m] */
+    double lower[3], upper[3];
+    int32_t openTop;
+    int32_t nBed;
+    const double* bedX;
+    const double* bedZ;
+    int32_t sideWalls;
+    double width;
+    int32_t offshoreWall;
+    double wallX, pistonSpeed, endLimit;
+    int32_t fieldNx, fieldNy;
+    const double* table;   /* fieldNy rows of fieldNx bed elevations */
+    double originX, originY, stepX, stepY;
+    double lowY, highY, lowX, highX;
+} sph_boundary;
+
+/* One run: its slots, grid and physics. Replaces the PairSums constructor and its set* calls
+ * (sphGpu.py:1537-1570, setDiffusion :1879, setWallPressure :1906, setNegativePressure :1916,
+ * setContinuityCorrection :1696, setViscosityLimiter :1734, setPhysics :2173). */
+typedef struct {
+    int64_t first;         /* first slot of the run's particles */
+    int64_t count;         /* the run's particles, fluid and boundary */
+    int32_t kernel;        /* 0 cubic spline, 1 Wendland C2 */
+    int32_t grid[3];       /* cells along x, y, z (z = 1 in 2D): gridDimensions, sphGpu.py:1466-1478 */
+    double cell;           /* cell width, 2 h_max [m] */
+    double hmax, hmin;     /* longest and shortest smoothing length [m] */
+    double period;         /* periodic y length [m], 0 for none (3D only) */
+    double rho0, c0, alpha, boundaryFactor;
+    double gravity[3];
+    int32_t diffusion;     /* 0 none, 1 molteni, 2 hydrostatic */
+    double delta;
+    int32_t extrapolatedWalls, negativePressure, correction, limiter;
+    double densityFloor;   /* smallest fluid density [kg/m^3], 0 for none */
+    double cfl, maxTimeStep;
+    sph_boundary boundary;
+} sph_run;
+
+/* The state of every run, slot order. Replaces PairSums' device arrays (sphGpu.py:111-116). */
+typedef struct {
+    double* position;      /* dims rows */
+    double* velocity;      /* dims rows */
+    double* density;       /* 1 row */
+    void* pressure;        /* Real, 1 row */
+    void* acceleration;    /* Real, dims rows */
+    const void* mass;      /* Real */
+    const void* length;    /* Real, each particle's smoothing length */
+    const uint8_t* fluid;  /* 1 for fluid, 0 for boundary */
+    int64_t stride;        /* elements between rows */
+} sph_state;
+
+/* A context for `dims` (2 or 3) and `precision` (0 float32 pairs, 1 float64) on `stream`
+ * (a hipStream_t; null for the default stream). Replaces PairSums.__init__ (sphGpu.py:1537). */
+int sph_create(sph_context** out, int32_t dims, int32_t precision, void* stream);
+int sph_destroy(sph_context* ctx);
+const char* sph_last_error(const sph_context* ctx);
+
+/* The runs the context steps together, and the slots they cover: PairSums._reserve and the set*
+ * calls (sphGpu.py:1580-1594). Allocates the context's scratch for `capacity` slots. */
+int sph_configure(sph_context* ctx, const sph_run* runs, int32_t count, int64_t capacity);
+
+/* Binds the state arrays: PairSums.push/update (sphGpu.py:1998-2042, 2207-2221). */
+int sph_bind(sph_context* ctx, const sph_state* state);
+
+/* The cell list over the bound positions: PairSums._rebuild (sphGpu.py:1635-1641). */
+int sph_build_cells(sph_context* ctx);
+
+/* Host-step sums over the cells of the last sph_build_cells, each reading caller arrays in slot
+ * order and writing one result per slot. */
+int sph_wall_pressure(sph_context* ctx, const double* density, const void* pressure, void* pressureOut,
+                      double* densityOut);                                        /* sphGpu.py:2090 */
+int sph_velocity_gradients(sph_context* ctx, const double* velocity, const double* density,
+                           void* out);                                            /* sphGpu.py:1827 */
+int sph_correction_matrices(sph_context* ctx, const double* density, void* out);  /* sphGpu.py:1855 */
+int sph_pair_accelerations(sph_context* ctx, const double* velocity, const double* density,
+                           const void* pressure, void* out);                      /* sphGpu.py:2044 */
+int sph_continuity_rate(sph_context* ctx, const double* velocity, const double* density,
+                        void* out);                                               /* sphGpu.py:2061 */
+
+/* Each slot's neighbor count, then (offsets from the caller's prefix sum of the counts) its
+ * neighbors' slots in visit order; the predicate is the sums' own. Synchronizes. */
+int sph_neighbor_counts(sph_context* ctx, int32_t* counts);
+int sph_neighbor_rows(sph_context* ctx, const int64_t* offsets, int32_t* rows);
+
+/* The bound boundary enforcement on the bound state: PairSums.enforceBoundary (sphGpu.py:2317). */
+int sph_enforce(sph_context* ctx);
+
+/* Each run's clock: WcsphSolver.advance's loop condition (sphSolver.py:1983). A run steps while
+ * time < untilTime and steps < maxSteps. */
+int sph_set_clock(sph_context* ctx, int32_t run, double time, double untilTime, int64_t steps, int64_t maxSteps);
+/* Copies every run's time, last step size, steps taken and maxima (vmax, amax) to host arrays
+ * (any may be null). Synchronizes. */
+int sph_get_clock(sph_context* ctx, double* time, double* lastDt, int64_t* steps, double* maxima);
+
+/* `steps` device-resident steps of every active run: PairSums.residentRates + residentIntegrate
+ * (sphGpu.py:2237-2315) with the time step reduced on the device instead of cflTimeStep on the
+ * host (sphSolver.py:1980-1986). With `useGraph` the steps are captured once into a HIP graph
+ * and replayed. Asynchronous. */
+int sph_advance(sph_context* ctx, int32_t steps, int32_t useGraph);
+
+/* The two halves of one step with the time step from the host, for step-for-step parity with the
+ * reference's loop: sph_rates is PairSums.residentRates (sphGpu.py:2237-2273) and copies every
+ * run's (|v|max, |a|max) to `maxima` (may be null; then it does not synchronize);
+ * sph_integrate is PairSums.residentIntegrate(dt) (sphGpu.py:2275-2315). Both step every run
+ * whatever its clock, as the host loop decides when to stop; sph_integrate advances the clocks. */
+int sph_rates(sph_context* ctx, double* maxima);
+int sph_integrate(sph_context* ctx, double timeStep);
+
+int sph_synchronize(sph_context* ctx);
+
+/* Per-kernel timing with HIP events recorded on the context's stream around every launch of a
+ * step (no counterpart in the reference, which times whole runs only). sph_profile(1) clears the
+ * totals and starts recording (steps then run as stream launches, not graphs); sph_kernel_times
+ * synchronizes and copies, per kernel id, the accumulated milliseconds and launch counts:
+ * 0 clock, 1 cell keys, 2 cell-count scan, 3 cell scatter, 4 cell order, 5 record gather,
+ * 6 sweep 1 (wall pressure, G), 7 sweep 2 (accelerations, L), 8 kick, 9 sweep 3 (continuity,
+ * density advance, drift, enforcement). */
+int sph_profile(sph_context* ctx, int32_t enable);
+int sph_kernel_times(sph_context* ctx, double* milliseconds, int64_t* launches, int32_t count);
+
+/* Launch shape of the pair sweeps: threads per workgroup (64 to 1024, a multiple of 64). */
+int sph_set_launch(sph_context* ctx, int32_t blockSize);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPH_HIP_H */

@@ -1,0 +1,297 @@
+'''
+
+The array formulation of pySurf's SPH equations, restated in numpy over explicit pair lists.
+
+TEST INFRASTRUCTURE, NOT PRODUCT CODE: only tests/, `__graft_entry__.smoke()` and bench.py's
+cpu_baseline leg may import anything under oracle/, and only as the checker.
+
+Each function restates one function of the reference's numpy backend, which the survey names the
+float64 oracle of the step. Every pair appears once as (i, j) with i < j and is scattered to both
+members, the structure of sphSolver.py, which differs from the per-particle gathers of the C++
+restatement (oracle/sph_oracle.cpp) and of the device. Comparing the two therefore checks index,
+sign and symmetry mistakes that a retyped formula would repeat.
+
+'''
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+from scipy.spatial import cKDTree
+
+taitExponent = 7.0           # sphSolver.py:225
+viscositySoftening = 0.01    # sphSolver.py:229
+limiterCloseDistance = 0.3   # sphSolver.py:234
+limiterFoldWidth = 0.2       # sphSolver.py:235
+correctionFloor = 0.25       # sphSolver.py:262
+
+
+def sigmaUnit(kernelType: str, dimensions: int) -> float:
+    '''Kernel normalization at h = 1 (sphKernels.py:439-442, sphGpu.py:1424-1444).'''
+    if kernelType == 'wendlandC2':
+        return 7.0 / (4.0 * math.pi) if dimensions == 2 else 21.0 / (16.0 * math.pi)
+    return 10.0 / (7.0 * math.pi) if dimensions == 2 else 1.0 / math.pi
+
+
+def kernelValues(kernelType: str, r: np.ndarray, h, dimensions: int) -> np.ndarray:
+    '''W(r, h), h one value or one per pair (sphKernels.py:387-644).'''
+    r = np.asarray(r, dtype = float)
+    h = np.broadcast_to(np.asarray(h, dtype = float), r.shape)
+    q = r / h
+    sigma = sigmaUnit(kernelType, dimensions) / h**dimensions
+    if kernelType == 'wendlandC2':
+        rest = np.maximum(1.0 - 0.5 * q, 0.0)
+        return np.where(q < 2.0, sigma * rest**4 * (2.0 * q + 1.0), 0.0)
+    inner = sigma * (1.0 - 1.5 * q**2 + 0.75 * q**3)
+    outer = sigma * 0.25 * np.maximum(2.0 - q, 0.0)**3
+    return np.where(q < 1.0, inner, np.where(q < 2.0, outer, 0.0))
+
+
+def kernelSlopes(kernelType: str, r: np.ndarray, h, dimensions: int) -> np.ndarray:
+    '''dW/dr (sphKernels.py:387-644).'''
+    r = np.asarray(r, dtype = float)
+    h = np.broadcast_to(np.asarray(h, dtype = float), r.shape)
+    q = r / h
+    sigma = sigmaUnit(kernelType, dimensions) / h**dimensions
+    if kernelType == 'wendlandC2':
+        rest = np.maximum(1.0 - 0.5 * q, 0.0)
+        return np.where(q < 2.0, -5.0 * sigma * q * rest**3 / h, 0.0)
+    inner = sigma * (-3.0 * q + 2.25 * q**2) / h
+    outer = sigma * (-0.75 * np.maximum(2.0 - q, 0.0)**2) / h
+    return np.where(q < 1.0, inner, np.where(q < 2.0, outer, 0.0))
+
+
+def kernelGradients(kernelType: str, separation: np.ndarray, distance: np.ndarray, h, dimensions: int) -> np.ndarray:
+    '''grad_i W_ij = dW/dr (r_i - r_j) / r (sphKernels.py:607-644).'''
+    slope = kernelSlopes(kernelType, distance, h, dimensions)
+    safe = np.where(distance > 1e-12, distance, 1.0)
+    return (slope / safe)[:, None] * separation
+
+
+def neighborPairs(positions: np.ndarray, radius: float, period: float | None = None) -> tuple[np.ndarray, np.ndarray]:
+    '''Every pair i < j closer than `radius`, y periodic when a period is given (sphNeighbors.py:28-78).'''
+    if len(positions) < 2:
+        empty = np.array([], dtype = np.intp)
+        return empty, empty.copy()
+    if period is None:
+        tree = cKDTree(positions)
+    else:
+        data = positions - positions.min(axis = 0)
+        box = 2.0 * data.max(axis = 0) + 4.0 * radius + 1.0
+        wrapped = np.mod(positions[:, 1], period)
+        wrapped[wrapped >= period] = 0.0
+        data[:, 1] = wrapped
+        box[1] = period
+        tree = cKDTree(data, boxsize = box)
+    pairs = tree.query_pairs(radius, output_type = 'ndarray')
+    if pairs.size == 0:
+        empty = np.array([], dtype = np.intp)
+        return empty, empty.copy()
+    return pairs[:, 0].astype(np.intp), pairs[:, 1].astype(np.intp)
+
+
+def bruteForcePairs(positions: np.ndarray, lengths: np.ndarray, period: float | None = None) -> set:
+    '''Every ordered pair (i, j), j != i, with r_ij < h_i + h_j, by brute force in float64.'''
+    n = len(positions)
+    result = set()
+    for i in range(n):
+        d = positions[i] - positions
+        if period is not None:
+            d[:, 1] -= period * np.round(d[:, 1] / period)
+        r = np.sqrt(np.sum(d * d, axis = 1))
+        hits = np.flatnonzero(r < lengths[i] + lengths)
+        result.update((i, int(j)) for j in hits if j != i)
+    return result
+
+
+def scatterAdd(target: np.ndarray, index: np.ndarray, values: np.ndarray) -> np.ndarray:
+    '''target[index[k]] += values[k] (sphSolver.py:573-592).'''
+    if np.ndim(values) == 1:
+        np.add.at(target, index, values)
+    else:
+        for column in range(values.shape[1]):
+            target[:, column] += np.bincount(index, weights = values[:, column], minlength = target.shape[0])
+    return target
+
+
+def taitEquationOfState(density, referenceDensity: float, soundSpeed: float, clampNegative: bool = True):
+    '''p = B ((rho / rho0)^7 - 1), B = rho0 c0^2 / 7 (sphSolver.py:541-557).'''
+    stiffness = referenceDensity * soundSpeed**2 / taitExponent
+    pressure = stiffness * ((np.asarray(density) / referenceDensity)**taitExponent - 1.0)
+    return np.maximum(pressure, 0.0) if clampNegative else pressure
+
+
+def taitDensity(pressure, referenceDensity: float, soundSpeed: float):
+    '''The inverse of the Tait equation (sphSolver.py:559-566).'''
+    stiffness = referenceDensity * soundSpeed**2 / taitExponent
+    return referenceDensity * (1.0 + np.asarray(pressure) / stiffness)**(1.0 / taitExponent)
+
+
+def continuityRate(masses, velocities, i, j, gradients, count, active = None, corrections = None):
+    '''sum_j m_j (v_i - v_j) . grad W_ij, optionally with L_i (sphSolver.py:623-654).'''
+    difference = velocities[i] - velocities[j]
+    if corrections is None:
+        rateI = rateJ = np.sum(difference * gradients, axis = 1)
+    else:
+        rateI = np.einsum('pa,pab,pb->p', difference, corrections[i], gradients)
+        rateJ = np.einsum('pa,pab,pb->p', difference, corrections[j], gradients)
+    if active is not None:
+        rateI, rateJ = np.where(active, rateI, 0.0), np.where(active, rateJ, 0.0)
+    rate = np.zeros(count)
+    scatterAdd(rate, i, masses[j] * rateI)
+    scatterAdd(rate, j, masses[i] * rateJ)
+    return rate
+
+
+def correctionMatrices(masses, densities, i, j, separation, gradients, isFluid):
+    '''L_i = M_i^-1, M_i = sum_j V_j (x_j - x_i) (x) grad_i W_ij (sphSolver.py:656-689).'''
+    count, dimensions = len(densities), separation.shape[1]
+    volumes = masses / densities
+    outer = -separation[:, :, None] * gradients[:, None, :]
+    matrices = np.zeros((count, dimensions, dimensions))
+    for a in range(dimensions):
+        for b in range(dimensions):
+            column = np.zeros(count)
+            scatterAdd(column, i, volumes[j] * outer[:, a, b])
+            scatterAdd(column, j, volumes[i] * outer[:, a, b])
+            matrices[:, a, b] = column
+    corrections = np.broadcast_to(np.eye(dimensions), matrices.shape).copy()
+    applies = isFluid & (np.linalg.det(matrices) >= correctionFloor**dimensions)
+    corrections[applies] = np.linalg.inv(matrices[applies])
+    return corrections
+
+
+def hydrostaticDensityDifference(vertical, meanDensity, referenceDensity, soundSpeed, gravity):
+    '''rho0 (g / c0^2) (z_i - z_j) (rho_bar / rho0)^(2 - gamma) (sphSolver.py:718-737).'''
+    ratio = np.asarray(meanDensity) / referenceDensity
+    return referenceDensity * gravity / soundSpeed**2 * np.asarray(vertical) * ratio**(2.0 - taitExponent)
+
+
+def densityDiffusionRate(masses, densities, i, j, separation, distance, gradients, count, h, soundSpeed, delta,
+                         referenceDensity, gravity = None, active = None):
+    '''The delta-SPH diffusive term, Molteni's or with the hydrostatic part removed (sphSolver.py:739-822).'''
+    if active is not None:
+        i, j = i[active], j[active]
+        separation, distance, gradients = separation[active], distance[active], gradients[active]
+        if np.ndim(h):
+            h = h[active]
+    difference = densities[j] - densities[i]
+    if gravity is not None:
+        difference = difference - hydrostaticDensityDifference(separation[:, -1], 0.5 * (densities[i] + densities[j]),
+                                                               referenceDensity, soundSpeed, gravity)
+    outward = -np.sum(separation * gradients, axis = 1)
+    weight = 2.0 * delta * h * soundSpeed * difference * outward / (distance**2 + viscositySoftening * h**2)
+    volumes = masses / densities
+    rate = np.zeros(count)
+    scatterAdd(rate, i, volumes[j] * weight)
+    scatterAdd(rate, j, -volumes[i] * weight)
+    return rate
+
+
+def extrapolatedWallPressure(densities, pressures, i, j, separation, values, isFluid, gravity):
+    '''Adami et al.'s wall pressure on every boundary particle (sphSolver.py:1045-1106).'''
+    count = len(pressures)
+    wallFirst = ~isFluid[i] & isFluid[j]
+    fluidFirst = isFluid[i] & ~isFluid[j]
+    wall = np.concatenate([i[wallFirst], j[fluidFirst]])
+    water = np.concatenate([j[wallFirst], i[fluidFirst]])
+    toWall = np.concatenate([separation[wallFirst], -separation[fluidFirst]])
+    weights = np.concatenate([values[wallFirst], values[fluidFirst]])
+    weight = np.zeros(count)
+    total = np.zeros(count)
+    scatterAdd(weight, wall, weights)
+    scatterAdd(total, wall, weights * (pressures[water] + densities[water] * (toWall @ np.asarray(gravity))))
+    result = pressures.copy()
+    boundary = ~isFluid
+    reached = boundary & (weight > 0.0)
+    result[boundary] = 0.0
+    result[reached] = np.maximum(total[reached] / weight[reached], 0.0)
+    return result
+
+
+def artificialViscosity(relative, separation, distance, meanDensity, h, soundSpeed, alpha, reconstructed = None):
+    '''Monaghan's Pi_ij, acting on approaching pairs (sphSolver.py:1122-1149).'''
+    approach = np.sum(relative * separation, axis = 1)
+    acting = approach < 0.0
+    if reconstructed is not None:
+        approach = np.sum(reconstructed * separation, axis = 1)
+        acting &= approach < 0.0
+    mu = h * approach / (distance**2 + viscositySoftening * h**2)
+    return np.where(acting, -alpha * soundSpeed * mu / meanDensity, 0.0)
+
+
+def velocityGradients(masses, densities, velocities, i, j, separation, gradients, isFluid):
+    '''G_i = S_i M_i^-1 over fluid pairs (sphSolver.py:1151-1190).'''
+    count, dimensions = len(densities), separation.shape[1]
+    volumes = masses / densities
+    fluidPairs = isFluid[i] & isFluid[j]
+    i, j = i[fluidPairs], j[fluidPairs]
+    gradient = gradients[fluidPairs]
+    rise = (velocities[j] - velocities[i])[:, :, None] * gradient[:, None, :]
+    offset = -separation[fluidPairs][:, :, None] * gradient[:, None, :]
+    sums = np.zeros((count, dimensions, dimensions))
+    matrices = np.zeros((count, dimensions, dimensions))
+    for a in range(dimensions):
+        for b in range(dimensions):
+            for target, outer in ((sums, rise), (matrices, offset)):
+                column = np.zeros(count)
+                scatterAdd(column, i, volumes[j] * outer[:, a, b])
+                scatterAdd(column, j, volumes[i] * outer[:, a, b])
+                target[:, a, b] = column
+    result = np.zeros((count, dimensions, dimensions))
+    applies = isFluid & (np.linalg.det(matrices) >= correctionFloor**dimensions)
+    result[applies] = sums[applies] @ np.linalg.inv(matrices[applies])
+    return result
+
+
+def limitedVelocityDifference(relative, separation, distance, h, gradientsI, gradientsJ):
+    '''v_ij less (phi_ij / 2) (G_i + G_j) r_ij under MAGMA2's limiter (sphSolver.py:1192-1230).'''
+    rampI = np.einsum('pab,pb->pa', gradientsI, separation)
+    rampJ = np.einsum('pab,pb->pa', gradientsJ, separation)
+    slopeI = np.sum(separation * rampI, axis = 1)
+    slopeJ = np.sum(separation * rampJ, axis = 1)
+    agree = slopeI * slopeJ > 0.0
+    ratio = np.divide(slopeI, slopeJ, out = np.zeros_like(slopeI), where = agree)
+    limiter = np.where(agree, np.minimum(1.0, 4.0 * ratio / (1.0 + ratio)**2), 0.0)
+    eta = distance / h
+    limiter *= np.where(eta < limiterCloseDistance, np.exp(-((eta - limiterCloseDistance) / limiterFoldWidth)**2), 1.0)
+    return relative - 0.5 * limiter[:, None] * (rampI + rampJ)
+
+
+def pairAccelerations(masses, densities, pressures, i, j, gradients, viscosity, count):
+    '''-sum_j m_j (p_i / rho_i^2 + p_j / rho_j^2 + Pi_ij) grad W_ij (sphSolver.py:1328-1356).'''
+    coefficient = -(pressures[i] / densities[i]**2 + pressures[j] / densities[j]**2 + viscosity)
+    accelerations = np.zeros((count, gradients.shape[1]))
+    scatterAdd(accelerations, i, (masses[j] * coefficient)[:, None] * gradients)
+    scatterAdd(accelerations, j, -(masses[i] * coefficient)[:, None] * gradients)
+    return accelerations
+
+
+def cellOrderKey(positions, cellWidth: float, shape) -> np.ndarray:
+    '''Cells from the origin, wrapped onto the grid, x fastest (sphSolver.py:1379-1408).'''
+    cells = np.floor(np.asarray(positions, dtype = float) / cellWidth).astype(np.int64)
+    nx, ny, nz = (int(count) for count in shape)
+    x, y = np.mod(cells[:, 0], nx), np.mod(cells[:, 1], ny)
+    z = np.mod(cells[:, 2], nz) if cells.shape[1] == 3 else np.zeros_like(x)
+    return (z * ny + y) * nx + x
+
+
+def cflTimeStep(h, soundSpeed, maxSpeed, maxAcceleration, cflNumber, maxTimeStep) -> float:
+    '''min(CFL h / (c0 + |v|max), CFL sqrt(h / |a|max), maxTimeStep) (sphSolver.py:1410-1424).'''
+    timeStep = cflNumber * h / (soundSpeed + maxSpeed)
+    if maxAcceleration > 1e-12:
+        timeStep = min(timeStep, cflNumber * math.sqrt(h / maxAcceleration))
+    return min(timeStep, maxTimeStep)
+
+
+def gridDimensions(domainMin, domainMax, cellSize: float, dimensions: int, maxCells: int = 2**25) -> tuple:
+    '''Cells along x, y, z covering the domain, one support across (sphGpu.py:1466-1478).'''
+    extent = np.zeros(3)
+    extent[:dimensions] = np.asarray(domainMax, dtype = float) - np.asarray(domainMin, dtype = float)
+    cells = np.maximum(np.ceil(extent / cellSize).astype(int) + 3, 1)
+    if dimensions == 2:
+        cells[2] = 1
+    while int(np.prod(cells)) > maxCells:
+        cells = np.maximum((cells * 0.8).astype(int), 1)
+    return tuple(int(count) for count in cells)

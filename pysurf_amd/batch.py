@@ -1,0 +1,148 @@
+'''
+
+Many runs in one engine: every run's particles packed into consecutive slots of one context, each
+with its own cell grid, physics, boundary and clock, all stepped by the same kernel launches and,
+with `graph`, replayed from one captured HIP graph. The reference runs one `PairSums` per solver
+and one run per process (sphSolver.py:1499-1502); packing is what fills an MI355X with 2D runs of
+a few thousand to a hundred thousand particles.
+
+A run's results do not depend on its neighbors in the batch: every sum is a gather over the run's
+own cells, in the same order as when the run is alone, and each run's time step is its own.
+
+'''
+
+from __future__ import annotations
+
+import numpy as np
+
+from pysurf_amd import _native
+from pysurf_amd.pair_sums import Engine, boundaryOf, gridDimensions
+from pysurf_amd.particles import ParticleSystem
+
+diffusionModes = {'none': 0, 'molteni': 1, 'hydrostatic': 2}
+
+
+def runOf(case, first: int, boundary: _native.Boundary) -> _native.Run:
+    '''The sph_run of a scenarios.Case whose particles start at slot `first`.'''
+    run = _native.Run()
+    run.first, run.count = int(first), int(case.count)
+    run.kernel = 1 if case.kernelType == 'wendlandC2' else 0
+    grid = gridDimensions(case.domainMin, case.domainMax, 2.0 * case.smoothingLength, case.dimensions)
+    run.grid[:] = list(grid)
+    run.cell = 2.0 * case.smoothingLength
+    lengths = case.particles.lengths
+    run.hmax = case.smoothingLength
+    run.hmin = float(lengths.min()) if lengths is not None else case.smoothingLength
+    run.period = 0.0
+    run.rho0, run.c0, run.alpha = case.referenceDensity, case.soundSpeed, case.alpha
+    run.boundaryFactor = case.boundaryViscosityFactor
+    g = np.zeros(3)
+    g[:case.dimensions] = case.gravity
+    run.gravity[:] = list(g)
+    run.diffusion = diffusionModes[case.diffusion]
+    run.delta = case.delta
+    run.extrapolatedWalls = int(case.extrapolatedWalls)
+    run.negativePressure = int(case.negativePressure)
+    run.correction = int(case.correction)
+    run.limiter = int(case.limiter)
+    run.densityFloor = case.densityFloor
+    run.cfl, run.maxTimeStep = case.cfl, case.maxTimeStep
+    run.boundary = boundary
+    return run
+
+
+class BatchEngine:
+
+    '''
+
+    Runs (scenarios.Case) packed into one context of `precision` pair arithmetic.
+
+    Parameters:
+    -----------
+    cases : list
+        The runs, all of one dimension
+    precision : str
+        'float32' pairs over float64 state, or 'float64'
+    device : str | None
+        'cuda:N'; None for cuda:0
+    blockSize : int
+        Threads per workgroup of the sweeps
+
+    '''
+
+    def __init__(self, cases: list, precision: str = 'float32', device = None, blockSize: int = 256) -> None:
+        if not cases:
+            raise ValueError('A batch needs at least one run.')
+        dimensions = {case.dimensions for case in cases}
+        if len(dimensions) != 1:
+            raise ValueError('Every run of a batch has the same dimensions.')
+        self.dimensions = dimensions.pop()
+        self.cases = list(cases)
+        self.engine = Engine(self.dimensions, precision, device)
+        self.device = self.engine.device
+        self.precision = precision
+        self._keep: list = []
+        runs, first = [], 0
+        self.offsets = []
+        for case in self.cases:
+            boundary, keep = boundaryOf(case.boundary, self.dimensions, self.device)
+            self._keep += keep
+            runs.append(runOf(case, first, boundary))
+            self.offsets.append(first)
+            first += case.count
+        self.offsets.append(first)
+        self.engine.configure(runs)
+        self.engine.setLaunch(blockSize)
+        self.load()
+
+    @property
+    def count(self) -> int:
+        return self.offsets[-1]
+
+    @property
+    def fluidCount(self) -> int:
+        return sum(case.fluidCount for case in self.cases)
+
+    def load(self) -> None:
+        '''Copy every run's particle state into the engine.'''
+        engine = self.engine
+        parts = [case.particles for case in self.cases]
+        engine.upload('position', np.vstack([p.positions for p in parts]))
+        engine.upload('velocity', np.vstack([p.velocities for p in parts]))
+        engine.upload('density', np.concatenate([p.densities for p in parts]))
+        engine.upload('mass', np.concatenate([p.masses for p in parts]))
+        lengths = [p.lengths if p.lengths is not None else np.full(p.nParticles, case.smoothingLength)
+                   for p, case in zip(parts, self.cases)]
+        engine.upload('length', np.concatenate(lengths))
+        engine.upload('fluid', np.concatenate([p.isFluid for p in parts]).astype(np.uint8))
+        engine.synchronize()
+        for r in range(len(self.cases)):
+            engine.setClock(r, 0.0, 0.0, 0, 0)
+
+    def setClocks(self, untilTime: float = np.inf, maxSteps: int = 2**62, times = None, steps = None) -> None:
+        '''Every run steps until its time reaches `untilTime` or it has taken `maxSteps` steps.'''
+        for r in range(len(self.cases)):
+            self.engine.setClock(r, 0.0 if times is None else times[r], untilTime, 0 if steps is None else steps[r],
+                                 maxSteps)
+
+    def advance(self, steps: int, graph: bool = True) -> None:
+        '''Enqueue `steps` steps of every active run (asynchronous).'''
+        self.engine.advance(steps, graph)
+
+    def synchronize(self) -> None:
+        self.engine.synchronize()
+
+    def clocks(self) -> dict:
+        return self.engine.clock(len(self.cases))
+
+    def state(self, index: int) -> ParticleSystem:
+        '''Run `index`'s state as a ParticleSystem, copied from the device.'''
+        a, b = self.offsets[index], self.offsets[index + 1]
+        engine, d = self.engine, self.dimensions
+        particles = self.cases[index].particles.copy()
+        particles.positions = engine.download('position', d, b)[a:b]
+        particles.velocities = engine.download('velocity', d, b)[a:b]
+        particles.densities = engine.download('density', None, b)[a:b]
+        particles.pressures = engine.download('pressure', None, b)[a:b]
+        particles.accelerations = engine.download('acceleration', d, b)[a:b]
+        return particles

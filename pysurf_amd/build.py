@@ -1,0 +1,51 @@
+'''
+
+Builds libsph_hip.so, the engine's HIP library, in-tree for gfx950 with hipcc.
+
+Packed float32 instructions are switched off (-packed-fp32-ops) and float32 division and square
+root stay correctly rounded, so the float32 neighbor predicate is the one the float32 restatement
+evaluates on the host.
+
+'''
+
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+
+here = os.path.dirname(os.path.abspath(__file__))
+root = os.path.dirname(here)
+library = os.path.join(here, 'libsph_hip.so')
+sources = [os.path.join(here, 'csrc', 'sph_hip.hip')]
+headers = [os.path.join(here, 'csrc', 'sph_device.hpp'), os.path.join(root, 'include', 'sph_hip.h')]
+
+flags = ['--offload-arch=gfx950', '-O3', '-std=c++17', '-fPIC', '-shared', '-Wall',
+         '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
+         '-fhip-fp32-correctly-rounded-divide-sqrt', '-ffp-contract=fast',
+         '-I', os.path.join(root, 'include')]
+
+
+def hipcc() -> str:
+    '''The hipcc compiler driver.'''
+    found = shutil.which('hipcc') or '/opt/rocm/bin/hipcc'
+    if not os.path.exists(found):
+        raise RuntimeError('hipcc was not found; the HIP library cannot be built.')
+    return found
+
+
+def build(verbose: bool = False) -> str:
+    '''Compile the HIP library; returns its path.'''
+    temporary = library + '.tmp'
+    command = [hipcc()] + flags + ['-o', temporary] + sources
+    result = subprocess.run(command, capture_output = True, text = True)
+    if result.returncode != 0:
+        raise RuntimeError('hipcc failed:\n' + result.stdout + result.stderr)
+    if verbose and result.stderr:
+        print(result.stderr)
+    os.replace(temporary, library)
+    return library
+
+
+if __name__ == '__main__':
+    print(build(verbose = True))

@@ -1,0 +1,1616 @@
+// The MI355X delta-SPH engine: kernels of the device-resident step and the C ABI of
+// include/sph_hip.h.
+//
+// A step of every active run (sphGpu.py:2237-2315, reordered for one device pass per phase):
+//
+//   clock      each run's active flag (time < until, steps < maxSteps), maxima cleared
+//   cells      cellOrderKey per slot (sphSolver.py:1379-1408), a counting sort by cell whose
+//              cells are then ordered by slot, so the order is the stable sort's and repeatable
+//   gather     float32 records in cell order: point + remainder (sphGpu.py:1010-1019), velocity,
+//              density, rho - rho0, Tait pressure (sphGpu.py:1058-1070); |v|max per run
+//   sweep 1    boundary lanes: extrapolated wall pressure (sphGpu.py:804-854);
+//              fluid lanes: velocity gradient G (sphGpu.py:643-697)
+//   sweep 2    fluid lanes: accelerations with the limited viscosity (sphGpu.py:303-414) and the
+//              correction matrix L (sphGpu.py:593-641, positions and densities only, so it can
+//              share this sweep); |a|max per run
+//   kick       dt from the run's maxima (sphSolver.py:1410-1424), v += a dt (sphGpu.py:1084-1109)
+//   sweep 3    continuity with the delta-SPH term (sphGpu.py:416-545), then, per particle, the
+//              density advance with its floor (:1111-1140), the drift (:1142-1186) and the
+//              boundary enforcement (:1197-1229, :1246-1310, :1332-1419)
+//
+// Every per-particle result is a gather over the particle's own neighbors, written by its own
+// thread: no atomics touch a sum, so a run's results do not depend on the other runs it is
+// batched with or on the launch shape. The maxima use atomicMax on float64 bit patterns, which
+// is order-independent.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "sph_device.hpp"
+
+using namespace sph;
+
+namespace {
+
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 4;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+// ---------------------------------------------------------------------------------------------
+// Wave and block scans
+// ---------------------------------------------------------------------------------------------
+
+__device__ __forceinline__ int32_t waveInclusiveScan(int32_t v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int32_t t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+// Exclusive scan over a block of kScanThreads threads; `total` receives the block's sum.
+__device__ __forceinline__ int32_t blockExclusiveScan(int32_t v, int32_t* waves, int32_t& total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    constexpr int kWaves = kScanThreads / 64;
+    const int32_t inclusive = waveInclusiveScan(v);
+    if (lane == 63) waves[wave] = inclusive;
+    __syncthreads();
+    if (wave == 0) {
+        int32_t w = lane < kWaves ? waves[lane] : 0;
+        w = waveInclusiveScan(w);
+        if (lane < kWaves) waves[lane] = w;
+    }
+    __syncthreads();
+    const int32_t prefix = wave > 0 ? waves[wave - 1] : 0;
+    total = waves[kWaves - 1];
+    __syncthreads();
+    return prefix + inclusive - v;
+}
+
+__global__ void __launch_bounds__(kScanThreads) scanTiles(const int32_t* __restrict__ in, int32_t* __restrict__ out,
+                                                          int32_t* __restrict__ partial, int64_t n) {
+    __shared__ int32_t waves[kScanThreads / 64];
+    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+    int32_t v[kScanItems];
+    int32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        v[k] = base + k < n ? in[base + k] : 0;
+        sum += v[k];
+    }
+    int32_t total;
+    int32_t running = blockExclusiveScan(sum, waves, total);
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        if (base + k < n) out[base + k] = running;
+        running += v[k];
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = total;
+}
+
+// One block: exclusive scan of the tile totals in place; out[n] takes the grand total.
+__global__ void __launch_bounds__(kScanThreads) scanPartials(int32_t* __restrict__ partial, int64_t tiles,
+                                                             int32_t* __restrict__ out, int64_t n) {
+    __shared__ int32_t waves[kScanThreads / 64];
+    int32_t carry = 0;
+    for (int64_t base = 0; base < tiles; base += kScanThreads) {
+        const int64_t i = base + threadIdx.x;
+        const int32_t v = i < tiles ? partial[i] : 0;
+        int32_t total;
+        const int32_t prefix = blockExclusiveScan(v, waves, total);
+        if (i < tiles) partial[i] = carry + prefix;
+        carry += total;
+    }
+    if (threadIdx.x == 0) out[n] = carry;
+}
+
+__global__ void __launch_bounds__(kScanThreads) scanAdd(int32_t* __restrict__ out, const int32_t* __restrict__ partial,
+                                                        int64_t n) {
+    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+    const int32_t add = partial[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k)
+        if (base + k < n) out[base + k] += add;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Clock
+// ---------------------------------------------------------------------------------------------
+
+__global__ void clockBegin(Clock clock, int32_t runs, int32_t useClock) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= runs) return;
+    clock.active[r] = useClock ? (clock.time[r] < clock.until[r] && clock.steps[r] < clock.maxSteps[r]) : 1;
+    clock.maxima[2 * r] = 0ull;
+    clock.maxima[2 * r + 1] = 0ull;
+}
+
+// cflTimeStep (sphSolver.py:1410-1424) from the run's maxima.
+__device__ __forceinline__ double runTimeStep(const RunDev& run, const Clock& clock, int r) {
+    const double vmax = __longlong_as_double((long long)clock.maxima[2 * r]);
+    const double amax = __longlong_as_double((long long)clock.maxima[2 * r + 1]);
+    double dt = run.cfl * run.hmin / (run.c0 + vmax);
+    if (amax > 1e-12) dt = fmin(dt, run.cfl * sqrt(run.hmin / amax));
+    return fmin(dt, run.maxTimeStep);
+}
+
+// The largest value of a wave's lanes of one run, folded into maxima[slot] with one atomic when
+// the wave holds one run, lane by lane otherwise. Every lane of the wave must call it.
+__device__ __forceinline__ void waveMaxInto(unsigned long long* maxima, int run, bool valid, double value) {
+    unsigned long long bits = valid ? (unsigned long long)__double_as_longlong(value) : 0ull;
+    const int first = __shfl(run, 0, 64);
+    const bool uniform = __all(run == first || !valid);
+    if (uniform) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const unsigned long long other = __shfl_xor(bits, o, 64);
+            bits = other > bits ? other : bits;
+        }
+        if ((threadIdx.x & 63) == 0 && bits != 0ull) atomicMax(maxima + 2 * first, bits);
+    } else if (valid && bits != 0ull) {
+        atomicMax(maxima + 2 * run, bits);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Cell list
+// ---------------------------------------------------------------------------------------------
+
+template <int D>
+__global__ void cellKeys(const RunDev* __restrict__ runs, const int32_t* __restrict__ runOf, const double* __restrict__ x,
+                         const double* __restrict__ y, const double* __restrict__ z, int64_t n, int32_t* __restrict__ key,
+                         int32_t* __restrict__ count) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const RunDev& run = runs[runOf[i]];
+    const int cx = wrapIndex(cellCoordinate(x[i], run.cell), run.nx);
+    const int cy = wrapIndex(cellCoordinate(y[i], run.cell), run.ny);
+    const int cz = D == 3 ? wrapIndex(cellCoordinate(z[i], run.cell), run.nz) : 0;
+    const int32_t k = (int32_t)(run.cellFirst + ((int64_t)cz * run.ny + cy) * run.nx + cx);
+    key[i] = k;
+    atomicAdd(count + k, 1);
+}
+
+__global__ void cellScatter(const int32_t* __restrict__ key, const int32_t* __restrict__ start, int32_t* __restrict__ fill,
+                            int32_t* __restrict__ order, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t k = key[i];
+    order[start[k] + atomicAdd(fill + k, 1)] = (int32_t)i;
+}
+
+// Orders each cell's slots ascending, the order a stable counting sort gives, and clears the
+// cell's counters for the next build.
+__global__ void cellOrder(const int32_t* __restrict__ start, int32_t* __restrict__ order, int32_t* __restrict__ count,
+                          int32_t* __restrict__ fill, int64_t cells) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= cells) return;
+    const int32_t begin = start[c], end = start[c + 1];
+    for (int32_t a = begin + 1; a < end; ++a) {
+        const int32_t v = order[a];
+        int32_t b = a - 1;
+        while (b >= begin && order[b] > v) {
+            order[b + 1] = order[b];
+            --b;
+        }
+        order[b + 1] = v;
+    }
+    count[c] = 0;
+    fill[c] = 0;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Records
+// ---------------------------------------------------------------------------------------------
+
+// Slot-order arrays a gather reads; `pressure` null takes the Tait pressure of `density`.
+template <typename Real>
+struct Sources {
+    const double* pos[3];
+    const double* vel[3];
+    const double* density;
+    const Real* pressure;
+    const Real* mass;
+    const Real* len;
+    const uint8_t* fluid;
+    const int32_t* key;
+    Real* pressureOut;       // the Tait pressure per slot, written when non-null
+};
+
+template <int D, typename Real>
+__global__ void gatherRecords(const RunDev* __restrict__ runs, const int32_t* __restrict__ runOf,
+                              const int32_t* __restrict__ order, Sources<Real> src, Records<Real> rec, Clock clock,
+                              int32_t resident, int64_t n) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool inRange = s < n;
+    const int r = inRange ? runOf[s] : 0;
+    bool valid = inRange && (!resident || clock.active[r]);
+    double speed = 0.0;
+    if (valid) {
+        const RunDev& run = runs[r];
+        const int32_t i = order[s];
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            const double x = src.pos[a][i];
+            if constexpr (sizeof(Real) == 4) {
+                const float point = (float)x;
+                rec.pos[a][s] = point;
+                rec.lo[a][s] = (float)(x - (double)point);
+            } else {
+                rec.pos[a][s] = x;
+            }
+        }
+        double v2 = 0.0;
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            const double v = src.vel[a][i];
+            rec.vel[a][s] = (Real)v;
+            v2 += v * v;
+        }
+        const double rho = src.density[i];
+        const bool isFluid = src.fluid[i] != 0;
+        rec.rho[s] = (Real)rho;
+        rec.dev[s] = (Real)(rho - run.rho0);
+        Real p;
+        if (src.pressure) {
+            p = src.pressure[i];
+        } else {
+            const double ratio = rho / run.rho0, sq = ratio * ratio;
+            double pressure = run.stiffness * (sq * sq * sq * ratio - 1.0);
+            if (!run.negativePressure || !isFluid) pressure = fmax(pressure, 0.0);
+            p = (Real)pressure;
+            if (src.pressureOut) src.pressureOut[i] = p;
+        }
+        rec.press[s] = p;
+        rec.mass[s] = src.mass[i];
+        rec.len[s] = src.len[i];
+        rec.fluid[s] = isFluid ? 1 : 0;
+        rec.key[s] = src.key[i];
+        speed = sqrt(v2);
+        valid = isFluid;
+    }
+    if (resident) waveMaxInto(clock.maxima, r, valid, speed);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Pair sweeps
+// ---------------------------------------------------------------------------------------------
+
+template <int D, typename Real>
+__device__ __forceinline__ Self<D, Real> loadSelf(const RunDev& run, const Records<Real>& rec, int64_t s, int& cx,
+                                                  int& cy, int& cz) {
+    Self<D, Real> me;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        me.p[a] = a < D ? rec.pos[a][s] : Real(0);
+        me.l[a] = 0.0f;
+        if constexpr (sizeof(Real) == 4)
+            if (a < D) me.l[a] = rec.lo[a][s];
+    }
+    me.h = rec.len[s];
+    me.reach = me.h + (Real)run.hmax;
+    const int64_t local = (int64_t)rec.key[s] - run.cellFirst;
+    cx = (int)(local % run.nx);
+    cy = (int)((local / run.nx) % run.ny);
+    cz = D == 3 ? (int)(local / ((int64_t)run.nx * run.ny)) : 0;
+    return me;
+}
+
+// In-place inverse of a D x D matrix given its determinant (2D: the reference's third axis carries
+// a one on the diagonal, sphGpu.py:636-641, which leaves the 2 x 2 block's inverse).
+template <int D, typename Real>
+__device__ __forceinline__ Real determinant(const Real (&m)[3][3]) {
+    if (D == 2) return m[0][0] * m[1][1] - m[0][1] * m[1][0];
+    return m[0][0] * (m[1][1] * m[2][2] - m[1][2] * m[2][1]) - m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0]) +
+           m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]);
+}
+template <int D, typename Real>
+__device__ __forceinline__ void inverse(const Real (&m)[3][3], Real det, Real (&out)[3][3]) {
+    const Real inv = Real(1) / det;
+    if (D == 2) {
+        out[0][0] = m[1][1] * inv;
+        out[0][1] = -m[0][1] * inv;
+        out[1][0] = -m[1][0] * inv;
+        out[1][1] = m[0][0] * inv;
+        return;
+    }
+    out[0][0] = (m[1][1] * m[2][2] - m[1][2] * m[2][1]) * inv;
+    out[0][1] = (m[0][2] * m[2][1] - m[0][1] * m[2][2]) * inv;
+    out[0][2] = (m[0][1] * m[1][2] - m[0][2] * m[1][1]) * inv;
+    out[1][0] = (m[1][2] * m[2][0] - m[1][0] * m[2][2]) * inv;
+    out[1][1] = (m[0][0] * m[2][2] - m[0][2] * m[2][0]) * inv;
+    out[1][2] = (m[0][2] * m[1][0] - m[0][0] * m[1][2]) * inv;
+    out[2][0] = (m[1][0] * m[2][1] - m[1][1] * m[2][0]) * inv;
+    out[2][1] = (m[0][1] * m[2][0] - m[0][0] * m[2][1]) * inv;
+    out[2][2] = (m[0][0] * m[1][1] - m[0][1] * m[1][0]) * inv;
+}
+
+template <int D>
+__host__ __device__ constexpr double correctionFloorPower() {
+    return D == 2 ? 0.0625 : 0.015625;   // 0.25^D, sphSolver.py:262
+}
+
+template <typename Real>
+struct SweepOut {
+    Real* pressure;        // sweep 1: every slot's pressure (boundary from the fluid)
+    double* density;       // sweep 1: boundary densities that carry it
+    Real* gradients;       // sweep 1: G per slot, D*D rows (optional)
+    Real* acceleration;    // sweep 2: D rows per slot
+    Real* corrections;     // sweep 2: L per slot, D*D rows (optional)
+    Real* rate;            // sweep 3: d rho / dt per slot (host steps)
+    double* pos[3];        // sweep 3 resident epilogue: the float64 state
+    double* vel[3];
+    double* density64;
+    int64_t stride;
+};
+
+// Sweep 1: wall pressure on boundary lanes, G on fluid lanes.
+template <int D, typename Real>
+__global__ void sweepWallAndGradients(const RunDev* __restrict__ runs, const int32_t* __restrict__ runOf,
+                                      const int32_t* __restrict__ order, const int32_t* __restrict__ start,
+                                      Records<Real> rec, SweepOut<Real> out, Clock clock, const double* __restrict__ y64,
+                                      int32_t resident, int32_t doWall, int32_t doGradients, int64_t n) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int r = runOf[s];
+    if (resident && !clock.active[r]) return;
+    const RunDev& run = runs[r];
+    const bool fluidI = rec.fluid[s] != 0;
+    const bool wall = !fluidI && doWall && (!resident || run.extrapolatedWalls);
+    const bool gradients = fluidI && doGradients && (!resident || run.limiter);
+    if (!wall && !gradients) {
+        if (!fluidI && doWall && out.pressure) out.pressure[order[s]] = rec.press[s];
+        if (fluidI && doWall && out.pressure && !resident) out.pressure[order[s]] = rec.press[s];
+        if (doGradients && out.gradients && !fluidI) {
+            const int32_t i = order[s];
+            for (int e = 0; e < D * D; ++e) out.gradients[e * out.stride + i] = Real(0);
+        }
+        return;
+    }
+    int cx, cy, cz;
+    const Self<D, Real> me = loadSelf<D, Real>(run, rec, s, cx, cy, cz);
+    const Smoothing<Real, D> kernel{run.kernel, (Real)run.sigmaUnit};
+    const int32_t i = order[s];
+    const double yy = run.period > 0.0 ? y64[i] : 0.0;
+    if (wall) {
+        const Real g[3] = {(Real)run.gravity[0], (Real)run.gravity[1], (Real)run.gravity[2]};
+        Real weight = Real(0), total = Real(0);
+        forNeighbors<D, Real, false>(run, start, rec, s, me, cx, cy, cz, yy,
+                                     [&](int32_t k, const Real* d, Real r, Real h) {
+                                         if (!rec.fluid[k]) return;
+                                         const Real w = kernel.value(r, h);
+                                         Real gd = g[0] * d[0] + g[1] * d[1];
+                                         if (D == 3) gd += g[2] * d[2];
+                                         weight += w;
+                                         total += w * (rec.press[k] + rec.rho[k] * gd);
+                                     });
+        Real pressure = Real(0);
+        if (weight > Real(0)) pressure = fmax(total / weight, Real(0));
+        const double rho = run.rho0 * pow(1.0 + (double)pressure / run.stiffness, 1.0 / 7.0);
+        rec.press[s] = pressure;
+        rec.rho[s] = (Real)rho;
+        if (out.pressure) out.pressure[i] = pressure;
+        if (out.density) out.density[i] = rho;
+        return;
+    }
+    Real vi[3];
+#pragma unroll
+    for (int a = 0; a < D; ++a) vi[a] = rec.vel[a][s];
+    Real rises[3][3] = {}, offsets[3][3] = {};
+    forNeighbors<D, Real, true>(run, start, rec, s, me, cx, cy, cz, yy, [&](int32_t k, const Real* d, Real r, Real h) {
+        if (!rec.fluid[k]) return;
+        const Real w = rec.mass[k] / rec.rho[k] * kernel.slopeOverR(r, h);
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            const Real dv = rec.vel[a][k] - vi[a];
+#pragma unroll
+            for (int b = 0; b < D; ++b) {
+                rises[a][b] += w * dv * d[b];
+                offsets[a][b] -= w * d[a] * d[b];
+            }
+        }
+    });
+    Real G[3][3] = {};
+    const Real det = determinant<D, Real>(offsets);
+    if (det >= (Real)correctionFloorPower<D>()) {
+        Real inv[3][3];
+        inverse<D, Real>(offsets, det, inv);
+#pragma unroll
+        for (int a = 0; a < D; ++a)
+#pragma unroll
+            for (int b = 0; b < D; ++b) {
+                Real sum = Real(0);
+#pragma unroll
+                for (int c = 0; c < D; ++c) sum += rises[a][c] * inv[c][b];
+                G[a][b] = sum;
+            }
+    }
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+#pragma unroll
+        for (int b = 0; b < D; ++b) {
+            rec.G[(a * D + b) * rec.stride + s] = G[a][b];
+            if (out.gradients) out.gradients[(a * D + b) * out.stride + i] = G[a][b];
+        }
+}
+
+// Sweep 2: accelerations, and L when asked, on fluid lanes.
+template <int D, typename Real>
+__global__ void sweepAccelerations(const RunDev* __restrict__ runs, const int32_t* __restrict__ runOf,
+                                   const int32_t* __restrict__ order, const int32_t* __restrict__ start,
+                                   Records<Real> rec, SweepOut<Real> out, Clock clock, const double* __restrict__ y64,
+                                   int32_t resident, int32_t doAcceleration, int32_t doCorrection,
+                                   int32_t hostLimiter, int64_t n) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool inRange = s < n;
+    const int r = inRange ? runOf[s] : 0;
+    const bool activeRun = inRange && (!resident || clock.active[r]);
+    bool fluidI = false;
+    double magnitude = 0.0;
+    if (activeRun) {
+        const RunDev& run = runs[r];
+        const int32_t i = order[s];
+        fluidI = rec.fluid[s] != 0;
+        const bool correction = doCorrection && (!resident || run.correction);
+        const bool limiter = resident ? run.limiter != 0 : hostLimiter != 0;
+        if (!fluidI) {
+            if (doAcceleration)
+#pragma unroll
+                for (int a = 0; a < D; ++a) out.acceleration[a * out.stride + i] = Real(0);
+            if (correction)
+#pragma unroll
+                for (int a = 0; a < D; ++a)
+#pragma unroll
+                    for (int b = 0; b < D; ++b) {
+                        const Real e = a == b ? Real(1) : Real(0);
+                        rec.L[(a * D + b) * rec.stride + s] = e;
+                        if (out.corrections) out.corrections[(a * D + b) * out.stride + i] = e;
+                    }
+        } else {
+            int cx, cy, cz;
+            const Self<D, Real> me = loadSelf<D, Real>(run, rec, s, cx, cy, cz);
+            const Smoothing<Real, D> kernel{run.kernel, (Real)run.sigmaUnit};
+            const double yy = run.period > 0.0 ? y64[i] : 0.0;
+            Real vi[3];
+#pragma unroll
+            for (int a = 0; a < D; ++a) vi[a] = rec.vel[a][s];
+            const Real rhoI = rec.rho[s];
+            const Real own = rec.press[s] / (rhoI * rhoI);
+            Real Gi[3][3] = {};
+            if (limiter && doAcceleration)
+#pragma unroll
+                for (int a = 0; a < D; ++a)
+#pragma unroll
+                    for (int b = 0; b < D; ++b) Gi[a][b] = rec.G[(a * D + b) * rec.stride + s];
+            const Real alpha = (Real)run.alpha, c0 = (Real)run.c0, bf = (Real)run.boundaryFactor;
+            Real total[3] = {};
+            Real M[3][3] = {};
+            forNeighbors<D, Real, true>(run, start, rec, s, me, cx, cy, cz, yy, [&](int32_t k, const Real* d, Real rr, Real h) {
+                const Real rhoJ = rec.rho[k];
+                const Real F = kernel.slopeOverR(rr, h);
+                if (correction) {
+                    const Real w = -rec.mass[k] / rhoJ * F;
+#pragma unroll
+                    for (int a = 0; a < D; ++a)
+#pragma unroll
+                        for (int b = 0; b < D; ++b) M[a][b] += w * d[a] * d[b];
+                }
+                if (!doAcceleration) return;
+                const bool fluidJ = rec.fluid[k] != 0;
+                Real dv[3];
+                Real approach = Real(0);
+#pragma unroll
+                for (int a = 0; a < D; ++a) {
+                    dv[a] = vi[a] - rec.vel[a][k];
+                    approach += dv[a] * d[a];
+                }
+                Real viscosity = Real(0);
+                if (approach < Real(0) && limiter && fluidJ) {
+                    Real rampI[3], rampJ[3];
+                    Real slopeI = Real(0), slopeJ = Real(0);
+#pragma unroll
+                    for (int a = 0; a < D; ++a) {
+                        Real ti = Real(0), tj = Real(0);
+#pragma unroll
+                        for (int b = 0; b < D; ++b) {
+                            ti += Gi[a][b] * d[b];
+                            tj += rec.G[(a * D + b) * rec.stride + k] * d[b];
+                        }
+                        rampI[a] = ti;
+                        rampJ[a] = tj;
+                        slopeI += d[a] * ti;
+                        slopeJ += d[a] * tj;
+                    }
+                    Real limit = Real(0);
+                    if (slopeI * slopeJ > Real(0)) {
+                        const Real ratio = slopeI / slopeJ;
+                        limit = fmin(Real(1), Real(4) * ratio / ((Real(1) + ratio) * (Real(1) + ratio)));
+                        const Real eta = rr / h;
+                        if (eta < Real(0.3)) {
+                            const Real fade = (eta - Real(0.3)) / Real(0.2);
+                            limit = limit * exp(-fade * fade);
+                        }
+                    }
+                    approach = Real(0);
+#pragma unroll
+                    for (int a = 0; a < D; ++a) approach += (dv[a] - Real(0.5) * limit * (rampI[a] + rampJ[a])) * d[a];
+                }
+                if (approach < Real(0)) {
+                    const Real mu = h * approach / (rr * rr + Real(0.01) * h * h);
+                    viscosity = -alpha * c0 * mu / (Real(0.5) * (rhoI + rhoJ));
+                    if (!fluidJ) viscosity *= bf;
+                }
+                const Real coefficient = -(own + rec.press[k] / (rhoJ * rhoJ) + viscosity) * rec.mass[k] * F;
+#pragma unroll
+                for (int a = 0; a < D; ++a) total[a] += coefficient * d[a];
+            });
+            if (doAcceleration) {
+                Real a2 = Real(0);
+#pragma unroll
+                for (int a = 0; a < D; ++a) {
+                    const Real v = total[a] + (Real)run.gravity[a];
+                    out.acceleration[a * out.stride + i] = v;
+                    a2 += v * v;
+                }
+                magnitude = (double)sqrt(a2);
+            }
+            if (correction) {
+                Real Lm[3][3] = {};
+                const Real det = determinant<D, Real>(M);
+                if (det >= (Real)correctionFloorPower<D>()) {
+                    inverse<D, Real>(M, det, Lm);
+                } else {
+#pragma unroll
+                    for (int a = 0; a < D; ++a) Lm[a][a] = Real(1);
+                }
+#pragma unroll
+                for (int a = 0; a < D; ++a)
+#pragma unroll
+                    for (int b = 0; b < D; ++b) {
+                        rec.L[(a * D + b) * rec.stride + s] = Lm[a][b];
+                        if (out.corrections) out.corrections[(a * D + b) * out.stride + i] = Lm[a][b];
+                    }
+            }
+        }
+    }
+    if (resident && doAcceleration) waveMaxInto(clock.maxima + 1, r, activeRun && fluidI, magnitude);
+}
+
+// Clears what a neighbor-free run would leave stale and applies v += a dt on the fluid, refreshing
+// the float32 velocity records; the run's first lane advances its clock.
+template <int D, typename Real>
+__global__ void kick(const RunDev* __restrict__ runs, const int32_t* __restrict__ runOf, const int32_t* __restrict__ order,
+                     Records<Real> rec, const Real* __restrict__ acceleration, double* const* __restrict__ vel64p,
+                     Clock clock, double forcedDt, int64_t stride, int64_t n) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int r = runOf[s];
+    if (!clock.active[r]) return;
+    const RunDev& run = runs[r];
+    const double dt = forcedDt > 0.0 ? forcedDt : runTimeStep(run, clock, r);
+    if (s == run.first) {
+        clock.dt[r] = dt;
+        clock.time[r] += dt;
+        clock.steps[r] += 1;
+    }
+    if (!rec.fluid[s]) return;
+    const int32_t i = order[s];
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        double* v = vel64p[a];
+        const double value = v[i] + (double)acceleration[a * stride + i] * dt;
+        v[i] = value;
+        rec.vel[a][s] = (Real)value;
+    }
+}
+
+// The bed's boundary enforcement on one fluid particle (sphGpu.py:1197-1229, 1246-1310,
+// 1332-1419; sphBoundaries.py:295-344, 597-640).
+template <int D>
+__device__ __forceinline__ void enforceBoundary(const sph_boundary& b, double* x, double* v) {
+    const double clearance = b.clearance;
+    if (b.kind == 1) {
+#pragma unroll
+        for (int axis = 0; axis < D; ++axis) {
+            if (x[axis] < b.lower[axis]) { x[axis] = b.lower[axis]; v[axis] = fmax(v[axis], 0.0); }
+            if (axis < D - 1 || b.openTop == 0)
+                if (x[axis] > b.upper[axis]) { x[axis] = b.upper[axis]; v[axis] = fmin(v[axis], 0.0); }
+        }
+    } else if (b.kind == 2) {
+        constexpr int vertical = D - 1;
+        int low = 0, high = b.nBed - 1;
+        while (high - low > 1) {
+            const int middle = (low + high) / 2;
+            if (b.bedX[middle] <= x[0]) low = middle; else high = middle;
+        }
+        const double run = b.bedX[low + 1] - b.bedX[low], rise = b.bedZ[low + 1] - b.bedZ[low];
+        const double share = fmin(fmax((x[0] - b.bedX[low]) / run, 0.0), 1.0);
+        const double bed = b.bedZ[low] + share * rise;
+        if (x[vertical] < bed + clearance) {
+            x[vertical] = bed + clearance;
+            const double slope = rise / run, length = sqrt(1.0 + slope * slope);
+            const double nx = -slope / length, nz = 1.0 / length;
+            const double inward = v[0] * nx + v[vertical] * nz;
+            if (inward < 0.0) { v[0] -= inward * nx; v[vertical] -= inward * nz; }
+        }
+        if (D == 3 && b.sideWalls) {
+            if (x[1] < clearance) { x[1] = clearance; v[1] = fmax(v[1], 0.0); }
+            if (x[1] > b.width - clearance) { x[1] = b.width - clearance; v[1] = fmin(v[1], 0.0); }
+        }
+        if (b.offshoreWall && x[0] < b.wallX + clearance) { x[0] = b.wallX + clearance; v[0] = fmax(v[0], b.pistonSpeed); }
+        if (x[0] > b.endLimit) { x[0] = b.endLimit; v[0] = fmin(v[0], 0.0); }
+    } else if (b.kind == 3 && D == 3) {
+        const int nx = b.fieldNx, ny = b.fieldNy;
+        const double rawX = (x[0] - b.originX) / b.stepX, rawY = (x[1] - b.originY) / b.stepY;
+        const double fx = fmin(fmax(rawX, 0.0), (double)(nx - 1)), fy = fmin(fmax(rawY, 0.0), (double)(ny - 1));
+        const int ix = (int)floor(fx), iy = (int)floor(fy);
+        const double ux = fx - ix, uy = fy - iy;
+        const double ux2 = ux * ux, ux3 = ux2 * ux, uy2 = uy * uy, uy3 = uy2 * uy;
+        const double wx[4] = {-0.5 * ux3 + ux2 - 0.5 * ux, 1.5 * ux3 - 2.5 * ux2 + 1.0, -1.5 * ux3 + 2.0 * ux2 + 0.5 * ux,
+                              0.5 * ux3 - 0.5 * ux2};
+        const double wy[4] = {-0.5 * uy3 + uy2 - 0.5 * uy, 1.5 * uy3 - 2.5 * uy2 + 1.0, -1.5 * uy3 + 2.0 * uy2 + 0.5 * uy,
+                              0.5 * uy3 - 0.5 * uy2};
+        const double sx[4] = {-1.5 * ux2 + 2.0 * ux - 0.5, 4.5 * ux2 - 5.0 * ux, -4.5 * ux2 + 4.0 * ux + 0.5, 1.5 * ux2 - ux};
+        const double sy[4] = {-1.5 * uy2 + 2.0 * uy - 0.5, 4.5 * uy2 - 5.0 * uy, -4.5 * uy2 + 4.0 * uy + 0.5, 1.5 * uy2 - uy};
+        double bed = 0.0, slopeX = 0.0, slopeY = 0.0;
+        for (int a = 0; a < 4; ++a) {
+            const int row = min(max(iy - 1 + a, 0), ny - 1);
+            double along = 0.0, alongSlope = 0.0;
+            for (int c = 0; c < 4; ++c) {
+                const double value = b.table[(int64_t)row * nx + min(max(ix - 1 + c, 0), nx - 1)];
+                along += wx[c] * value;
+                alongSlope += sx[c] * value;
+            }
+            bed += wy[a] * along;
+            slopeX += wy[a] * alongSlope;
+            slopeY += sy[a] * along;
+        }
+        slopeX /= b.stepX;
+        slopeY /= b.stepY;
+        if (rawX < 0.0 || rawX > (double)(nx - 1)) slopeX = 0.0;
+        if (rawY < 0.0 || rawY > (double)(ny - 1)) slopeY = 0.0;
+        if (x[2] < bed + clearance) {
+            x[2] = bed + clearance;
+            const double length = sqrt(slopeX * slopeX + slopeY * slopeY + 1.0);
+            const double nrm[3] = {-slopeX / length, -slopeY / length, 1.0 / length};
+            const double inward = v[0] * nrm[0] + v[1] * nrm[1] + v[2] * nrm[2];
+            if (inward < 0.0)
+                for (int a = 0; a < 3; ++a) v[a] -= inward * nrm[a];
+        }
+        if (x[1] < b.lowY) { x[1] = b.lowY; v[1] = fmax(v[1], 0.0); }
+        if (x[1] > b.highY) { x[1] = b.highY; v[1] = fmin(v[1], 0.0); }
+        if (b.offshoreWall && x[0] < b.lowX) { x[0] = b.lowX; v[0] = fmax(v[0], 0.0); }
+        if (x[0] > b.highX) { x[0] = b.highX; v[0] = fmin(v[0], 0.0); }
+    }
+}
+
+// Sweep 3: continuity with the delta-SPH term; resident, the particle's density advance, drift and
+// boundary enforcement follow on its float64 state.
+template <int D, typename Real>
+__global__ void sweepContinuity(const RunDev* __restrict__ runs, const int32_t* __restrict__ runOf,
+                                const int32_t* __restrict__ order, const int32_t* __restrict__ start, Records<Real> rec,
+                                SweepOut<Real> out, Clock clock, int32_t resident, int32_t hostCorrection,
+                                int32_t hostDiffusion, int64_t n) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int r = runOf[s];
+    if (resident && !clock.active[r]) return;
+    const RunDev& run = runs[r];
+    const int32_t i = order[s];
+    const bool boundaryI = rec.fluid[s] == 0;
+    const int diffusion = resident ? run.diffusion : hostDiffusion;
+    const bool correction = resident ? run.correction != 0 : hostCorrection != 0;
+    const bool diffuses = diffusion != 0 && !boundaryI;
+    int cx, cy, cz;
+    const Self<D, Real> me = loadSelf<D, Real>(run, rec, s, cx, cy, cz);
+    const Smoothing<Real, D> kernel{run.kernel, (Real)run.sigmaUnit};
+    const double yy = run.period > 0.0 ? out.pos[1][i] : 0.0;
+    Real vi[3];
+#pragma unroll
+    for (int a = 0; a < D; ++a) vi[a] = rec.vel[a][s];
+    Real Li[3][3] = {};
+    if (correction) {
+#pragma unroll
+        for (int a = 0; a < D; ++a)
+#pragma unroll
+            for (int b = 0; b < D; ++b) Li[a][b] = rec.L[(a * D + b) * rec.stride + s];
+    }
+    const Real devI = rec.dev[s];
+    const Real factor = (Real)(2.0 * run.delta * run.c0);
+    const Real rho0 = (Real)run.rho0;
+    const Real gOverC2 = (Real)(fabs(run.gravity[D - 1]) / (run.c0 * run.c0));
+    Real total = Real(0);
+    forNeighbors<D, Real, true>(run, start, rec, s, me, cx, cy, cz, yy, [&](int32_t k, const Real* d, Real rr, Real h) {
+        const bool fluidJ = rec.fluid[k] != 0;
+        if (boundaryI && !fluidJ) return;
+        const Real F = kernel.slopeOverR(rr, h);
+        Real dot = Real(0);
+        if (correction && !boundaryI) {
+#pragma unroll
+            for (int a = 0; a < D; ++a) {
+                Real ld = Real(0);
+#pragma unroll
+                for (int b = 0; b < D; ++b) ld += Li[a][b] * d[b];
+                dot += (vi[a] - rec.vel[a][k]) * ld;
+            }
+        } else {
+#pragma unroll
+            for (int a = 0; a < D; ++a) dot += (vi[a] - rec.vel[a][k]) * d[a];
+        }
+        total += rec.mass[k] * dot * F;
+        if (diffuses && fluidJ) {
+            const Real devJ = rec.dev[k];
+            Real difference = devJ - devI;
+            if (diffusion == 2) {
+                const Real ratio = Real(1) + Real(0.5) * (devI + devJ) / rho0;
+                const Real sq = ratio * ratio;
+                difference -= rho0 * gOverC2 * d[D - 1] / (sq * sq * ratio);
+            }
+            total += factor * h * rec.mass[k] / rec.rho[k] * difference * (-F * rr * rr) / (rr * rr + Real(0.01) * h * h);
+        }
+    });
+    if (!resident) {
+        out.rate[i] = total;
+        return;
+    }
+    const double dt = clock.dt[r];
+    double rho = out.density64[i] + (double)total * dt;
+    if (run.densityFloor > 0.0 && !boundaryI) rho = fmax(rho, run.densityFloor);
+    out.density64[i] = rho;
+    if (boundaryI) return;
+    double x[3] = {0.0, 0.0, 0.0}, v[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        v[a] = out.vel[a][i];
+        x[a] = out.pos[a][i] + v[a] * dt;
+    }
+    if (run.period > 0.0) x[1] = x[1] - run.period * floor(x[1] / run.period);
+    enforceBoundary<D>(run.boundary, x, v);
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        out.pos[a][i] = x[a];
+        out.vel[a][i] = v[a];
+    }
+}
+
+template <int D>
+__global__ void enforceKernel(const RunDev* __restrict__ runs, const int32_t* __restrict__ runOf,
+                              const uint8_t* __restrict__ fluid, double* px, double* py, double* pz, double* vx, double* vy,
+                              double* vz, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !fluid[i]) return;
+    const RunDev& run = runs[runOf[i]];
+    double* P[3] = {px, py, pz};
+    double* V[3] = {vx, vy, vz};
+    double x[3] = {0.0, 0.0, 0.0}, v[3] = {0.0, 0.0, 0.0};
+    for (int a = 0; a < D; ++a) { x[a] = P[a][i]; v[a] = V[a][i]; }
+    enforceBoundary<D>(run.boundary, x, v);
+    for (int a = 0; a < D; ++a) { P[a][i] = x[a]; V[a][i] = v[a]; }
+}
+
+// Neighbor counts (rows null) or rows, each slot's neighbors in visit order, as slots.
+template <int D, typename Real>
+__global__ void neighborList(const RunDev* __restrict__ runs, const int32_t* __restrict__ runOf,
+                             const int32_t* __restrict__ order, const int32_t* __restrict__ start, Records<Real> rec,
+                             const double* __restrict__ y64, int32_t* counts, const int64_t* offsets, int32_t* rows,
+                             int64_t n) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const RunDev& run = runs[runOf[s]];
+    int cx, cy, cz;
+    const Self<D, Real> me = loadSelf<D, Real>(run, rec, s, cx, cy, cz);
+    const int32_t i = order[s];
+    const double yy = run.period > 0.0 ? y64[i] : 0.0;
+    int32_t count = 0;
+    int64_t base = rows ? offsets[i] : 0;
+    forNeighbors<D, Real, false>(run, start, rec, s, me, cx, cy, cz, yy, [&](int32_t k, const Real*, Real, Real) {
+        if (rows) rows[base + count] = order[k];
+        ++count;
+    });
+    if (counts) counts[i] = count;
+}
+
+inline unsigned gridFor(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
+
+}  // namespace
+
+// -------------------------------------------------------------------------------------------------
+// Context
+// -------------------------------------------------------------------------------------------------
+
+struct sph_context {
+    int32_t dims = 2, precision = 0;
+    hipStream_t stream = nullptr;
+    bool ownsStream = false;
+    std::string error;
+    int block = 256;
+
+    std::vector<sph_run> runs;
+    std::vector<RunDev> runsHost;
+    int64_t capacity = 0, total = 0, cells = 0;
+
+    RunDev* dRuns = nullptr;
+    int32_t* runOf = nullptr;
+    int32_t* key = nullptr;
+    int32_t* order = nullptr;
+    int32_t* cellCount = nullptr;
+    int32_t* cellFill = nullptr;
+    int32_t* cellStart = nullptr;
+    int32_t* scanPartial = nullptr;
+    int64_t scanTiles = 0;
+
+    std::vector<void*> allocations;
+    void* rec = nullptr;   // Records<float> or Records<double>, host copy of device pointers
+    Records<float> rec32{};
+    Records<double> rec64{};
+    double** dVel = nullptr;   // device array of the bound velocity row pointers
+
+    Clock clock{};
+    sph_state state{};
+    bool bound = false;
+
+    hipGraphExec_t graphExec = nullptr;
+    hipGraph_t graph = nullptr;
+    int32_t graphSteps = 0;
+
+    // Per-kernel timing with HIP events on the context's stream, while profiling
+    bool profiling = false;
+    std::vector<hipEvent_t> events;            // pool; pairs (start, stop) in use up to eventsUsed
+    size_t eventsUsed = 0;
+    std::vector<int32_t> eventKernel;          // kernel id of each pair in use
+    double kernelMs[16] = {};
+    int64_t kernelLaunches[16] = {};
+};
+
+namespace {
+
+int fail(sph_context* ctx, int code, const std::string& message) {
+    if (ctx) ctx->error = message;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                                  \
+    do {                                                                                               \
+        hipError_t status_ = (expr);                                                                   \
+        if (status_ != hipSuccess)                                                                     \
+            return fail(ctx, SPH_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(status_));    \
+    } while (0)
+
+template <typename T>
+int allocate(sph_context* ctx, T** out, size_t count) {
+    void* p = nullptr;
+    if (count == 0) count = 1;
+    HIP_TRY(hipMalloc(&p, count * sizeof(T)));
+    ctx->allocations.push_back(p);
+    *out = static_cast<T*>(p);
+    return SPH_OK;
+}
+
+void dropGraph(sph_context* ctx) {
+    if (ctx->graphExec) (void)hipGraphExecDestroy(ctx->graphExec);
+    if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
+    ctx->graphExec = nullptr;
+    ctx->graph = nullptr;
+    ctx->graphSteps = 0;
+}
+
+void releaseScratch(sph_context* ctx) {
+    dropGraph(ctx);
+    for (void* p : ctx->allocations) (void)hipFree(p);
+    ctx->allocations.clear();
+    ctx->dRuns = nullptr;
+}
+
+double sigmaUnitOf(int kernel, int dims) {
+    const double pi = 3.14159265358979323846;
+    if (kernel == 1) return dims == 2 ? 7.0 / (4.0 * pi) : 21.0 / (16.0 * pi);
+    return dims == 2 ? 10.0 / (7.0 * pi) : 1.0 / pi;
+}
+
+template <typename Real>
+int allocateRecords(sph_context* ctx, Records<Real>& rec, int64_t cap) {
+    const int D = ctx->dims;
+    int status = SPH_OK;
+    for (int a = 0; a < 3; ++a) {
+        rec.pos[a] = nullptr;
+        rec.lo[a] = nullptr;
+        rec.vel[a] = nullptr;
+    }
+    for (int a = 0; a < D && status == SPH_OK; ++a) {
+        status = allocate(ctx, &rec.pos[a], cap);
+        if (status == SPH_OK && sizeof(Real) == 4) status = allocate(ctx, &rec.lo[a], cap);
+        if (status == SPH_OK) status = allocate(ctx, &rec.vel[a], cap);
+    }
+    if (status == SPH_OK) status = allocate(ctx, &rec.rho, cap);
+    if (status == SPH_OK) status = allocate(ctx, &rec.press, cap);
+    if (status == SPH_OK) status = allocate(ctx, &rec.dev, cap);
+    if (status == SPH_OK) status = allocate(ctx, &rec.mass, cap);
+    if (status == SPH_OK) status = allocate(ctx, &rec.len, cap);
+    if (status == SPH_OK) status = allocate(ctx, &rec.fluid, cap);
+    if (status == SPH_OK) status = allocate(ctx, &rec.key, cap);
+    if (status == SPH_OK) status = allocate(ctx, &rec.G, cap * D * D);
+    if (status == SPH_OK) status = allocate(ctx, &rec.L, cap * D * D);
+    rec.stride = cap;
+    return status;
+}
+
+int checkRuns(sph_context* ctx, const sph_run* runs, int32_t count, int64_t capacity) {
+    int64_t next = 0;
+    for (int32_t r = 0; r < count; ++r) {
+        const sph_run& run = runs[r];
+        if (run.first != next) return fail(ctx, SPH_ERR_VALUE, "Runs must cover consecutive slots from 0.");
+        if (run.count < 0) return fail(ctx, SPH_ERR_VALUE, "A run's particle count cannot be negative.");
+        next += run.count;
+        if (run.kernel != 0 && run.kernel != 1) return fail(ctx, SPH_ERR_VALUE, "The kernel is not 0 (cubic spline) or 1 (Wendland C2).");
+        if (run.grid[0] < 1 || run.grid[1] < 1 || run.grid[2] < 1) return fail(ctx, SPH_ERR_VALUE, "Grid dimensions must be positive.");
+        if (ctx->dims == 2 && run.grid[2] != 1) return fail(ctx, SPH_ERR_VALUE, "A 2D grid has one cell along z.");
+        if (!(run.cell > 0.0) || !(run.hmax > 0.0) || run.cell < 2.0 * run.hmax * (1.0 - 1e-12))
+            return fail(ctx, SPH_ERR_VALUE, "The cell width must be positive and at least twice the longest smoothing length.");
+        if (run.period > 0.0 && ctx->dims != 3) return fail(ctx, SPH_ERR_VALUE, "A periodic axis needs three dimensions: it is y.");
+        if (run.period > 0.0 && run.period <= 4.0 * run.hmax)
+            return fail(ctx, SPH_ERR_VALUE, "The periodic width must exceed twice the kernel support.");
+        if (run.diffusion < 0 || run.diffusion > 2)
+            return fail(ctx, SPH_ERR_VALUE, "Density diffusion is not 0 (none), 1 (molteni) or 2 (hydrostatic).");
+        const sph_boundary& b = run.boundary;
+        if (b.kind < 0 || b.kind > 3) return fail(ctx, SPH_ERR_VALUE, "Unknown boundary kind.");
+        if (b.kind == 2 && (b.nBed < 2 || !b.bedX || !b.bedZ)) return fail(ctx, SPH_ERR_VALUE, "A bed needs at least two vertices.");
+        if (b.kind == 3 && (ctx->dims != 3 || b.fieldNx < 1 || b.fieldNy < 1 || !b.table))
+            return fail(ctx, SPH_ERR_VALUE, "A height field needs three dimensions and a table.");
+    }
+    if (next > capacity) return fail(ctx, SPH_ERR_VALUE, "The runs cover more slots than the capacity.");
+    if (capacity >= (int64_t)1 << 31) return fail(ctx, SPH_ERR_VALUE, "At most 2^31 - 1 slots per context.");
+    return SPH_OK;
+}
+
+template <typename Real>
+Records<Real>& recordsOf(sph_context* ctx);
+template <>
+Records<float>& recordsOf<float>(sph_context* ctx) { return ctx->rec32; }
+template <>
+Records<double>& recordsOf<double>(sph_context* ctx) { return ctx->rec64; }
+
+int launchCheck(sph_context* ctx, const char* what) {
+    const hipError_t status = hipGetLastError();
+    if (status != hipSuccess) return fail(ctx, SPH_ERR_HIP, std::string(what) + ": " + hipGetErrorString(status));
+    return SPH_OK;
+}
+
+// Kernel ids of the per-kernel timing: include/sph_hip.h lists them for sph_kernel_times.
+enum KernelId { kClock = 0, kCellKeys, kScan, kCellScatter, kCellOrder, kGather, kSweepWall, kSweepAcceleration,
+                kKick, kSweepContinuity, kEnforce, kKernelKinds };
+
+// Records a start event before a timed launch; returns its slot, or -1 while not profiling.
+int timeBegin(sph_context* ctx, int kernel) {
+    if (!ctx->profiling) return -1;
+    if (ctx->eventsUsed + 2 > ctx->events.size()) {
+        for (int k = 0; k < 64; ++k) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return -1;
+            ctx->events.push_back(e);
+        }
+    }
+    const size_t slot = ctx->eventsUsed;
+    ctx->eventsUsed += 2;
+    ctx->eventKernel.push_back(kernel);
+    (void)hipEventRecord(ctx->events[slot], ctx->stream);
+    return (int)slot;
+}
+
+void timeEnd(sph_context* ctx, int slot) {
+    if (slot >= 0) (void)hipEventRecord(ctx->events[slot + 1], ctx->stream);
+}
+
+// Folds the recorded pairs into the per-kernel totals; synchronizes.
+int collectTimes(sph_context* ctx) {
+    if (ctx->eventsUsed == 0) return SPH_OK;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    for (size_t k = 0; k < ctx->eventKernel.size(); ++k) {
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, ctx->events[2 * k], ctx->events[2 * k + 1]));
+        ctx->kernelMs[ctx->eventKernel[k]] += ms;
+        ctx->kernelLaunches[ctx->eventKernel[k]] += 1;
+    }
+    ctx->eventsUsed = 0;
+    ctx->eventKernel.clear();
+    return SPH_OK;
+}
+
+#define TIMED(ctx, id, launch)                    \
+    do {                                          \
+        const int slot_ = timeBegin((ctx), (id)); \
+        launch;                                   \
+        timeEnd((ctx), slot_);                    \
+    } while (0)
+
+// The cell list over the bound positions.
+template <int D>
+int buildCells(sph_context* ctx) {
+    const int64_t n = ctx->total;
+    const int b = 256;
+    double* P = ctx->state.position;
+    const int64_t st = ctx->state.stride;
+    if (n > 0)
+        TIMED(ctx, kCellKeys, (cellKeys<D><<<gridFor(n, b), b, 0, ctx->stream>>>(ctx->dRuns, ctx->runOf, P, P + st,
+                                                                                 D == 3 ? P + 2 * st : P, n, ctx->key,
+                                                                                 ctx->cellCount)));
+    const int64_t m = ctx->cells;
+    {
+        const int slot = timeBegin(ctx, kScan);
+        scanTiles<<<(unsigned)ctx->scanTiles, kScanThreads, 0, ctx->stream>>>(ctx->cellCount, ctx->cellStart,
+                                                                               ctx->scanPartial, m);
+        scanPartials<<<1, kScanThreads, 0, ctx->stream>>>(ctx->scanPartial, ctx->scanTiles, ctx->cellStart, m);
+        scanAdd<<<(unsigned)ctx->scanTiles, kScanThreads, 0, ctx->stream>>>(ctx->cellStart, ctx->scanPartial, m);
+        timeEnd(ctx, slot);
+    }
+    if (n > 0)
+        TIMED(ctx, kCellScatter, (cellScatter<<<gridFor(n, b), b, 0, ctx->stream>>>(ctx->key, ctx->cellStart,
+                                                                                    ctx->cellFill, ctx->order, n)));
+    TIMED(ctx, kCellOrder, (cellOrder<<<gridFor(m, b), b, 0, ctx->stream>>>(ctx->cellStart, ctx->order, ctx->cellCount,
+                                                                            ctx->cellFill, m)));
+    return launchCheck(ctx, "cell build");
+}
+
+template <typename Real>
+Sources<Real> sourcesOf(sph_context* ctx, const double* velocity, const double* density, const Real* pressure,
+                        Real* pressureOut) {
+    Sources<Real> src{};
+    const int64_t st = ctx->state.stride;
+    for (int a = 0; a < 3; ++a) {
+        src.pos[a] = a < ctx->dims ? ctx->state.position + a * st : nullptr;
+        src.vel[a] = a < ctx->dims ? velocity + a * st : nullptr;
+    }
+    src.density = density;
+    src.pressure = pressure;
+    src.mass = static_cast<const Real*>(ctx->state.mass);
+    src.len = static_cast<const Real*>(ctx->state.length);
+    src.fluid = ctx->state.fluid;
+    src.key = ctx->key;
+    src.pressureOut = pressureOut;
+    return src;
+}
+
+template <typename Real>
+SweepOut<Real> emptyOut(sph_context* ctx) {
+    SweepOut<Real> out{};
+    out.stride = ctx->state.stride;
+    return out;
+}
+
+int requireBound(sph_context* ctx) {
+    if (!ctx) return SPH_ERR_VALUE;
+    if (!ctx->bound) return fail(ctx, SPH_ERR_VALUE, "No state is bound; call sph_configure and sph_bind first.");
+    return SPH_OK;
+}
+
+// The two halves of a device-resident step of every active run. `phase` 1 is the rates
+// (residentRates), 2 the integration (residentIntegrate) with `forcedDt` when positive and the
+// run's own time step otherwise, 3 both. `useClock` 0 steps every run whatever its clock.
+template <int D, typename Real>
+int stepOnce(sph_context* ctx, int phase, int useClock, double forcedDt) {
+    const int64_t n = ctx->total;
+    const int b = ctx->block;
+    const int32_t nruns = (int32_t)ctx->runs.size();
+    Records<Real>& rec = recordsOf<Real>(ctx);
+    const int64_t st = ctx->state.stride;
+    if (phase & 1) {
+        TIMED(ctx, kClock, (clockBegin<<<gridFor(nruns, 64), 64, 0, ctx->stream>>>(ctx->clock, nruns, useClock)));
+        int status = buildCells<D>(ctx);
+        if (status != SPH_OK) return status;
+    }
+    if (n == 0) return launchCheck(ctx, "device step");
+    Sources<Real> src = sourcesOf<Real>(ctx, ctx->state.velocity, ctx->state.density, nullptr,
+                                        static_cast<Real*>(ctx->state.pressure));
+    SweepOut<Real> out = emptyOut<Real>(ctx);
+    out.pressure = static_cast<Real*>(ctx->state.pressure);
+    out.density = ctx->state.density;
+    out.acceleration = static_cast<Real*>(ctx->state.acceleration);
+    for (int a = 0; a < 3; ++a) {
+        out.pos[a] = a < D ? ctx->state.position + a * st : nullptr;
+        out.vel[a] = a < D ? ctx->state.velocity + a * st : nullptr;
+    }
+    out.density64 = ctx->state.density;
+    const double* y64 = ctx->state.position + st;
+    if (phase & 1) {
+        TIMED(ctx, kGather,
+              (gatherRecords<D, Real><<<gridFor(n, b), b, 0, ctx->stream>>>(ctx->dRuns, ctx->runOf, ctx->order, src, rec,
+                                                                             ctx->clock, 1, n)));
+        TIMED(ctx, kSweepWall,
+              (sweepWallAndGradients<D, Real><<<gridFor(n, b), b, 0, ctx->stream>>>(
+                  ctx->dRuns, ctx->runOf, ctx->order, ctx->cellStart, rec, out, ctx->clock, y64, 1, 1, 1, n)));
+        TIMED(ctx, kSweepAcceleration,
+              (sweepAccelerations<D, Real><<<gridFor(n, b), b, 0, ctx->stream>>>(
+                  ctx->dRuns, ctx->runOf, ctx->order, ctx->cellStart, rec, out, ctx->clock, y64, 1, 1, 1, 0, n)));
+    }
+    if (phase & 2) {
+        TIMED(ctx, kKick,
+              (kick<D, Real><<<gridFor(n, b), b, 0, ctx->stream>>>(ctx->dRuns, ctx->runOf, ctx->order, rec,
+                                                                    static_cast<const Real*>(ctx->state.acceleration),
+                                                                    ctx->dVel, ctx->clock, forcedDt, st, n)));
+        TIMED(ctx, kSweepContinuity,
+              (sweepContinuity<D, Real><<<gridFor(n, b), b, 0, ctx->stream>>>(ctx->dRuns, ctx->runOf, ctx->order,
+                                                                               ctx->cellStart, rec, out, ctx->clock, 1,
+                                                                               0, 0, n)));
+    }
+    return launchCheck(ctx, "device step");
+}
+
+template <int D, typename Real>
+int advanceSteps(sph_context* ctx, int32_t steps) {
+    for (int32_t k = 0; k < steps; ++k) {
+        const int status = stepOnce<D, Real>(ctx, 3, 1, 0.0);
+        if (status != SPH_OK) return status;
+    }
+    return SPH_OK;
+}
+
+int dispatchAdvance(sph_context* ctx, int32_t steps) {
+    if (ctx->dims == 2) return ctx->precision == 0 ? advanceSteps<2, float>(ctx, steps) : advanceSteps<2, double>(ctx, steps);
+    return ctx->precision == 0 ? advanceSteps<3, float>(ctx, steps) : advanceSteps<3, double>(ctx, steps);
+}
+
+int dispatchPhase(sph_context* ctx, int phase, double forcedDt) {
+    if (ctx->dims == 2)
+        return ctx->precision == 0 ? stepOnce<2, float>(ctx, phase, 0, forcedDt) : stepOnce<2, double>(ctx, phase, 0, forcedDt);
+    return ctx->precision == 0 ? stepOnce<3, float>(ctx, phase, 0, forcedDt) : stepOnce<3, double>(ctx, phase, 0, forcedDt);
+}
+
+// Host-step sums: gather records from caller arrays, then one sweep.
+template <int D, typename Real>
+int hostSweep(sph_context* ctx, int which, const double* velocity, const double* density, const void* pressure,
+              void* out0, double* out1) {
+    const int64_t n = ctx->total;
+    if (n == 0) return SPH_OK;
+    const int b = ctx->block;
+    const int32_t nruns = (int32_t)ctx->runs.size();
+    Records<Real>& rec = recordsOf<Real>(ctx);
+    clockBegin<<<gridFor(nruns, 64), 64, 0, ctx->stream>>>(ctx->clock, nruns, 0);
+    const double* vel = velocity ? velocity : ctx->state.velocity;
+    Sources<Real> src = sourcesOf<Real>(ctx, vel, density, static_cast<const Real*>(pressure), nullptr);
+    gatherRecords<D, Real><<<gridFor(n, b), b, 0, ctx->stream>>>(ctx->dRuns, ctx->runOf, ctx->order, src, rec, ctx->clock, 0, n);
+    SweepOut<Real> out = emptyOut<Real>(ctx);
+    const double* y64 = ctx->state.position + ctx->state.stride;
+    const RunDev& run0 = ctx->runsHost[0];
+    switch (which) {
+        case 0:   // wall pressure
+            out.pressure = static_cast<Real*>(out0);
+            out.density = out1;
+            sweepWallAndGradients<D, Real><<<gridFor(n, b), b, 0, ctx->stream>>>(ctx->dRuns, ctx->runOf, ctx->order,
+                                                                                  ctx->cellStart, rec, out, ctx->clock,
+                                                                                  y64, 0, 1, 0, n);
+            break;
+        case 1:   // velocity gradients
+            out.gradients = static_cast<Real*>(out0);
+            sweepWallAndGradients<D, Real><<<gridFor(n, b), b, 0, ctx->stream>>>(ctx->dRuns, ctx->runOf, ctx->order,
+                                                                                  ctx->cellStart, rec, out, ctx->clock,
+                                                                                  y64, 0, 0, 1, n);
+            break;
+        case 2:   // correction matrices
+            out.corrections = static_cast<Real*>(out0);
+            sweepAccelerations<D, Real><<<gridFor(n, b), b, 0, ctx->stream>>>(ctx->dRuns, ctx->runOf, ctx->order,
+                                                                               ctx->cellStart, rec, out, ctx->clock, y64,
+                                                                               0, 0, 1, 0, n);
+            break;
+        case 3: {  // accelerations, with the limiter's gradients first when the run has it
+            const int limiter = run0.limiter;
+            if (limiter)
+                sweepWallAndGradients<D, Real><<<gridFor(n, b), b, 0, ctx->stream>>>(ctx->dRuns, ctx->runOf, ctx->order,
+                                                                                      ctx->cellStart, rec, out, ctx->clock,
+                                                                                      y64, 0, 0, 1, n);
+            out.acceleration = static_cast<Real*>(out0);
+            sweepAccelerations<D, Real><<<gridFor(n, b), b, 0, ctx->stream>>>(ctx->dRuns, ctx->runOf, ctx->order,
+                                                                               ctx->cellStart, rec, out, ctx->clock, y64,
+                                                                               0, 1, 0, limiter, n);
+            break;
+        }
+        case 4: {  // continuity, with the correction matrices first when the run has them
+            const int correction = run0.correction;
+            if (correction)
+                sweepAccelerations<D, Real><<<gridFor(n, b), b, 0, ctx->stream>>>(ctx->dRuns, ctx->runOf, ctx->order,
+                                                                                   ctx->cellStart, rec, out, ctx->clock,
+                                                                                   y64, 0, 0, 1, 0, n);
+            out.rate = static_cast<Real*>(out0);
+            for (int a = 0; a < 3; ++a) out.pos[a] = a < D ? ctx->state.position + a * ctx->state.stride : nullptr;
+            sweepContinuity<D, Real><<<gridFor(n, b), b, 0, ctx->stream>>>(ctx->dRuns, ctx->runOf, ctx->order,
+                                                                            ctx->cellStart, rec, out, ctx->clock, 0,
+                                                                            correction, run0.diffusion, n);
+            break;
+        }
+        default:
+            return fail(ctx, SPH_ERR_VALUE, "Unknown sum.");
+    }
+    return launchCheck(ctx, "host-step sum");
+}
+
+int dispatchHostSweep(sph_context* ctx, int which, const double* velocity, const double* density, const void* pressure,
+                      void* out0, double* out1) {
+    int status = requireBound(ctx);
+    if (status != SPH_OK) return status;
+    if (!density) return fail(ctx, SPH_ERR_VALUE, "The sums need the densities.");
+    if (ctx->dims == 2)
+        return ctx->precision == 0 ? hostSweep<2, float>(ctx, which, velocity, density, pressure, out0, out1)
+                                   : hostSweep<2, double>(ctx, which, velocity, density, pressure, out0, out1);
+    return ctx->precision == 0 ? hostSweep<3, float>(ctx, which, velocity, density, pressure, out0, out1)
+                               : hostSweep<3, double>(ctx, which, velocity, density, pressure, out0, out1);
+}
+
+template <int D, typename Real>
+int neighborsImpl(sph_context* ctx, int32_t* counts, const int64_t* offsets, int32_t* rows) {
+    const int64_t n = ctx->total;
+    if (n == 0) return SPH_OK;
+    const int b = ctx->block;
+    Records<Real>& rec = recordsOf<Real>(ctx);
+    const int32_t nruns = (int32_t)ctx->runs.size();
+    clockBegin<<<gridFor(nruns, 64), 64, 0, ctx->stream>>>(ctx->clock, nruns, 0);
+    Sources<Real> src = sourcesOf<Real>(ctx, ctx->state.velocity, ctx->state.density, nullptr, nullptr);
+    gatherRecords<D, Real><<<gridFor(n, b), b, 0, ctx->stream>>>(ctx->dRuns, ctx->runOf, ctx->order, src, rec, ctx->clock, 0, n);
+    neighborList<D, Real><<<gridFor(n, b), b, 0, ctx->stream>>>(ctx->dRuns, ctx->runOf, ctx->order, ctx->cellStart, rec,
+                                                                 ctx->state.position + ctx->state.stride, counts, offsets,
+                                                                 rows, n);
+    int status = launchCheck(ctx, "neighbor list");
+    if (status != SPH_OK) return status;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return SPH_OK;
+}
+
+}  // namespace
+
+// -------------------------------------------------------------------------------------------------
+// C ABI
+// -------------------------------------------------------------------------------------------------
+
+extern "C" {
+
+int sph_create(sph_context** out, int32_t dims, int32_t precision, void* stream) {
+    if (!out) return SPH_ERR_VALUE;
+    *out = nullptr;
+    if (dims != 2 && dims != 3) return SPH_ERR_VALUE;
+    if (precision != 0 && precision != 1) return SPH_ERR_VALUE;
+    int device = 0;
+    if (hipGetDevice(&device) != hipSuccess) return SPH_ERR_HIP;
+    sph_context* ctx = new sph_context();
+    ctx->dims = dims;
+    ctx->precision = precision;
+    if (stream) {
+        ctx->stream = static_cast<hipStream_t>(stream);
+    } else {
+        if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+            delete ctx;
+            return SPH_ERR_HIP;
+        }
+        ctx->ownsStream = true;
+    }
+    *out = ctx;
+    return SPH_OK;
+}
+
+int sph_destroy(sph_context* ctx) {
+    if (!ctx) return SPH_OK;
+    (void)hipStreamSynchronize(ctx->stream);
+    releaseScratch(ctx);
+    for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
+    if (ctx->ownsStream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return SPH_OK;
+}
+
+const char* sph_last_error(const sph_context* ctx) { return ctx ? ctx->error.c_str() : "no context"; }
+
+int sph_configure(sph_context* ctx, const sph_run* runs, int32_t count, int64_t capacity) {
+    if (!ctx) return SPH_ERR_VALUE;
+    if (count < 1 || !runs) return fail(ctx, SPH_ERR_VALUE, "At least one run is needed.");
+    int status = checkRuns(ctx, runs, count, capacity);
+    if (status != SPH_OK) return status;
+    // The same slots and grids: only the physics changed, so the scratch, the bound state and a
+    // captured graph (which reads the descriptors through their device copy) all stay
+    bool sameLayout = ctx->dRuns && (int32_t)ctx->runs.size() == count && ctx->capacity == std::max<int64_t>(capacity, 1);
+    for (int32_t r = 0; sameLayout && r < count; ++r) {
+        const sph_run& a = ctx->runs[r];
+        const sph_run& b = runs[r];
+        sameLayout = a.first == b.first && a.count == b.count && a.grid[0] == b.grid[0] && a.grid[1] == b.grid[1] &&
+                     a.grid[2] == b.grid[2];
+    }
+    if (!sameLayout) {
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        releaseScratch(ctx);
+        ctx->bound = false;
+    }
+    ctx->runs.assign(runs, runs + count);
+    ctx->runsHost.clear();
+    int64_t cells = 0, total = 0;
+    for (const sph_run& run : ctx->runs) {
+        RunDev d{};
+        d.first = run.first;
+        d.count = run.count;
+        d.cellFirst = cells;
+        d.nx = run.grid[0];
+        d.ny = run.grid[1];
+        d.nz = ctx->dims == 3 ? run.grid[2] : 1;
+        d.kernel = run.kernel;
+        d.diffusion = run.diffusion;
+        d.extrapolatedWalls = run.extrapolatedWalls;
+        d.negativePressure = run.negativePressure;
+        d.correction = run.correction;
+        d.limiter = run.limiter;
+        d.uniform = run.hmin == run.hmax;
+        d.cell = run.cell;
+        d.hmax = run.hmax;
+        d.hmin = run.hmin;
+        d.period = run.period;
+        d.rho0 = run.rho0;
+        d.c0 = run.c0;
+        d.alpha = run.alpha;
+        d.boundaryFactor = run.boundaryFactor;
+        for (int a = 0; a < 3; ++a) d.gravity[a] = run.gravity[a];
+        d.delta = run.delta;
+        d.densityFloor = run.densityFloor;
+        d.cfl = run.cfl;
+        d.maxTimeStep = run.maxTimeStep;
+        d.sigmaUnit = sigmaUnitOf(run.kernel, ctx->dims);
+        d.stiffness = run.rho0 * run.c0 * run.c0 / 7.0;
+        d.boundary = run.boundary;
+        ctx->runsHost.push_back(d);
+        cells += (int64_t)d.nx * d.ny * d.nz;
+        total += run.count;
+    }
+    if (cells >= ((int64_t)1 << 31) - 1) return fail(ctx, SPH_ERR_VALUE, "Too many cells across the runs.");
+    if (sameLayout) {
+        HIP_TRY(hipMemcpyAsync(ctx->dRuns, ctx->runsHost.data(), count * sizeof(RunDev), hipMemcpyHostToDevice,
+                               ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        return SPH_OK;
+    }
+    ctx->cells = cells;
+    ctx->total = total;
+    ctx->capacity = std::max<int64_t>(capacity, 1);
+    ctx->scanTiles = (cells + kScanTile - 1) / kScanTile;
+    const int64_t cap = ctx->capacity;
+
+    status = allocate(ctx, &ctx->dRuns, count);
+    if (status == SPH_OK) status = allocate(ctx, &ctx->runOf, cap);
+    if (status == SPH_OK) status = allocate(ctx, &ctx->key, cap);
+    if (status == SPH_OK) status = allocate(ctx, &ctx->order, cap);
+    if (status == SPH_OK) status = allocate(ctx, &ctx->cellCount, cells);
+    if (status == SPH_OK) status = allocate(ctx, &ctx->cellFill, cells);
+    if (status == SPH_OK) status = allocate(ctx, &ctx->cellStart, cells + 1);
+    if (status == SPH_OK) status = allocate(ctx, &ctx->scanPartial, ctx->scanTiles);
+    if (status == SPH_OK) status = allocate(ctx, &ctx->clock.time, count);
+    if (status == SPH_OK) status = allocate(ctx, &ctx->clock.until, count);
+    if (status == SPH_OK) status = allocate(ctx, &ctx->clock.dt, count);
+    if (status == SPH_OK) status = allocate(ctx, &ctx->clock.steps, count);
+    if (status == SPH_OK) status = allocate(ctx, &ctx->clock.maxSteps, count);
+    if (status == SPH_OK) status = allocate(ctx, &ctx->clock.maxima, 2 * (size_t)count);
+    if (status == SPH_OK) status = allocate(ctx, &ctx->clock.active, count);
+    if (status == SPH_OK) status = allocate(ctx, &ctx->dVel, 3);
+    if (status == SPH_OK)
+        status = ctx->precision == 0 ? allocateRecords(ctx, ctx->rec32, cap) : allocateRecords(ctx, ctx->rec64, cap);
+    if (status != SPH_OK) {
+        releaseScratch(ctx);
+        return status;
+    }
+    std::vector<int32_t> runOf(cap, count - 1);
+    for (int32_t r = 0; r < count; ++r)
+        std::fill(runOf.begin() + runs[r].first, runOf.begin() + runs[r].first + runs[r].count, r);
+    HIP_TRY(hipMemcpyAsync(ctx->dRuns, ctx->runsHost.data(), count * sizeof(RunDev), hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ctx->runOf, runOf.data(), cap * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemsetAsync(ctx->cellCount, 0, std::max<int64_t>(cells, 1) * sizeof(int32_t), ctx->stream));
+    HIP_TRY(hipMemsetAsync(ctx->cellFill, 0, std::max<int64_t>(cells, 1) * sizeof(int32_t), ctx->stream));
+    std::vector<double> zeros(count, 0.0);
+    std::vector<int64_t> zeroSteps(count, 0);
+    HIP_TRY(hipMemcpyAsync(ctx->clock.time, zeros.data(), count * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ctx->clock.until, zeros.data(), count * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ctx->clock.dt, zeros.data(), count * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ctx->clock.steps, zeroSteps.data(), count * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ctx->clock.maxSteps, zeroSteps.data(), count * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemsetAsync(ctx->clock.maxima, 0, 2 * count * sizeof(unsigned long long), ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return SPH_OK;
+}
+
+int sph_bind(sph_context* ctx, const sph_state* state) {
+    if (!ctx) return SPH_ERR_VALUE;
+    if (ctx->runs.empty()) return fail(ctx, SPH_ERR_VALUE, "Configure the runs before binding a state.");
+    if (!state || !state->position || !state->velocity || !state->density || !state->pressure || !state->acceleration ||
+        !state->mass || !state->length || !state->fluid)
+        return fail(ctx, SPH_ERR_VALUE, "Every state array is needed.");
+    if (state->stride < ctx->total) return fail(ctx, SPH_ERR_VALUE, "The state's row stride is shorter than the runs.");
+    if (state->stride > ctx->capacity) return fail(ctx, SPH_ERR_VALUE, "The state's row stride exceeds the capacity.");
+    ctx->state = *state;
+    double* rows[3] = {state->velocity, state->velocity + state->stride,
+                       ctx->dims == 3 ? state->velocity + 2 * state->stride : nullptr};
+    HIP_TRY(hipMemcpyAsync(ctx->dVel, rows, sizeof rows, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    dropGraph(ctx);
+    ctx->bound = true;
+    return SPH_OK;
+}
+
+int sph_build_cells(sph_context* ctx) {
+    int status = requireBound(ctx);
+    if (status != SPH_OK) return status;
+    return ctx->dims == 2 ? buildCells<2>(ctx) : buildCells<3>(ctx);
+}
+
+int sph_wall_pressure(sph_context* ctx, const double* density, const void* pressure, void* pressureOut,
+                      double* densityOut) {
+    if (!pressure || !pressureOut || !densityOut) return fail(ctx, SPH_ERR_VALUE, "The wall pressure needs its arrays.");
+    return dispatchHostSweep(ctx, 0, nullptr, density, pressure, pressureOut, densityOut);
+}
+
+int sph_velocity_gradients(sph_context* ctx, const double* velocity, const double* density, void* out) {
+    if (!velocity || !out) return fail(ctx, SPH_ERR_VALUE, "The velocity gradients need velocities and an output.");
+    return dispatchHostSweep(ctx, 1, velocity, density, nullptr, out, nullptr);
+}
+
+int sph_correction_matrices(sph_context* ctx, const double* density, void* out) {
+    if (!out) return fail(ctx, SPH_ERR_VALUE, "The correction matrices need an output.");
+    return dispatchHostSweep(ctx, 2, nullptr, density, nullptr, out, nullptr);
+}
+
+int sph_pair_accelerations(sph_context* ctx, const double* velocity, const double* density, const void* pressure,
+                           void* out) {
+    if (!velocity || !pressure || !out) return fail(ctx, SPH_ERR_VALUE, "The accelerations need velocities, pressures and an output.");
+    return dispatchHostSweep(ctx, 3, velocity, density, pressure, out, nullptr);
+}
+
+int sph_continuity_rate(sph_context* ctx, const double* velocity, const double* density, void* out) {
+    if (!velocity || !out) return fail(ctx, SPH_ERR_VALUE, "The continuity rate needs velocities and an output.");
+    return dispatchHostSweep(ctx, 4, velocity, density, nullptr, out, nullptr);
+}
+
+int sph_neighbor_counts(sph_context* ctx, int32_t* counts) {
+    int status = requireBound(ctx);
+    if (status != SPH_OK) return status;
+    if (!counts) return fail(ctx, SPH_ERR_VALUE, "No counts array.");
+    if (ctx->dims == 2)
+        return ctx->precision == 0 ? neighborsImpl<2, float>(ctx, counts, nullptr, nullptr)
+                                   : neighborsImpl<2, double>(ctx, counts, nullptr, nullptr);
+    return ctx->precision == 0 ? neighborsImpl<3, float>(ctx, counts, nullptr, nullptr)
+                               : neighborsImpl<3, double>(ctx, counts, nullptr, nullptr);
+}
+
+int sph_neighbor_rows(sph_context* ctx, const int64_t* offsets, int32_t* rows) {
+    int status = requireBound(ctx);
+    if (status != SPH_OK) return status;
+    if (!offsets || !rows) return fail(ctx, SPH_ERR_VALUE, "No offsets or rows array.");
+    if (ctx->dims == 2)
+        return ctx->precision == 0 ? neighborsImpl<2, float>(ctx, nullptr, offsets, rows)
+                                   : neighborsImpl<2, double>(ctx, nullptr, offsets, rows);
+    return ctx->precision == 0 ? neighborsImpl<3, float>(ctx, nullptr, offsets, rows)
+                               : neighborsImpl<3, double>(ctx, nullptr, offsets, rows);
+}
+
+int sph_enforce(sph_context* ctx) {
+    int status = requireBound(ctx);
+    if (status != SPH_OK) return status;
+    const int64_t n = ctx->total, st = ctx->state.stride;
+    if (n == 0) return SPH_OK;
+    double* P = ctx->state.position;
+    double* V = ctx->state.velocity;
+    if (ctx->dims == 2)
+        enforceKernel<2><<<gridFor(n, 256), 256, 0, ctx->stream>>>(ctx->dRuns, ctx->runOf, ctx->state.fluid, P, P + st, P,
+                                                                    V, V + st, V, n);
+    else
+        enforceKernel<3><<<gridFor(n, 256), 256, 0, ctx->stream>>>(ctx->dRuns, ctx->runOf, ctx->state.fluid, P, P + st,
+                                                                    P + 2 * st, V, V + st, V + 2 * st, n);
+    return launchCheck(ctx, "enforce");
+}
+
+int sph_set_clock(sph_context* ctx, int32_t run, double time, double untilTime, int64_t steps, int64_t maxSteps) {
+    if (!ctx) return SPH_ERR_VALUE;
+    if (run < 0 || run >= (int32_t)ctx->runs.size()) return fail(ctx, SPH_ERR_VALUE, "No such run.");
+    HIP_TRY(hipMemcpyAsync(ctx->clock.time + run, &time, sizeof time, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ctx->clock.until + run, &untilTime, sizeof untilTime, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ctx->clock.steps + run, &steps, sizeof steps, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ctx->clock.maxSteps + run, &maxSteps, sizeof maxSteps, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return SPH_OK;
+}
+
+int sph_get_clock(sph_context* ctx, double* time, double* lastDt, int64_t* steps, double* maxima) {
+    if (!ctx) return SPH_ERR_VALUE;
+    const size_t count = ctx->runs.size();
+    if (count == 0) return SPH_OK;
+    if (time) HIP_TRY(hipMemcpyAsync(time, ctx->clock.time, count * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    if (lastDt) HIP_TRY(hipMemcpyAsync(lastDt, ctx->clock.dt, count * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    if (steps) HIP_TRY(hipMemcpyAsync(steps, ctx->clock.steps, count * sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+    if (maxima) HIP_TRY(hipMemcpyAsync(maxima, ctx->clock.maxima, 2 * count * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return SPH_OK;
+}
+
+int sph_advance(sph_context* ctx, int32_t steps, int32_t useGraph) {
+    int status = requireBound(ctx);
+    if (status != SPH_OK) return status;
+    if (steps < 0) return fail(ctx, SPH_ERR_VALUE, "The step count cannot be negative.");
+    if (steps == 0) return SPH_OK;
+    if (!useGraph || ctx->profiling) return dispatchAdvance(ctx, steps);
+    if (ctx->ownsStream == false && ctx->stream == nullptr)
+        return fail(ctx, SPH_ERR_VALUE, "A graph needs a stream other than the null stream.");
+    if (ctx->graphSteps != steps) {
+        dropGraph(ctx);
+        HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        status = dispatchAdvance(ctx, steps);
+        hipGraph_t graph = nullptr;
+        const hipError_t end = hipStreamEndCapture(ctx->stream, &graph);
+        if (status != SPH_OK) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return status;
+        }
+        if (end != hipSuccess) return fail(ctx, SPH_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(end));
+        ctx->graph = graph;
+        HIP_TRY(hipGraphInstantiate(&ctx->graphExec, graph, nullptr, nullptr, 0));
+        ctx->graphSteps = steps;
+    }
+    HIP_TRY(hipGraphLaunch(ctx->graphExec, ctx->stream));
+    return SPH_OK;
+}
+
+int sph_rates(sph_context* ctx, double* maxima) {
+    int status = requireBound(ctx);
+    if (status != SPH_OK) return status;
+    status = dispatchPhase(ctx, 1, 0.0);
+    if (status != SPH_OK) return status;
+    if (maxima) {
+        HIP_TRY(hipMemcpyAsync(maxima, ctx->clock.maxima, 2 * ctx->runs.size() * sizeof(double), hipMemcpyDeviceToHost,
+                               ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+    }
+    return SPH_OK;
+}
+
+int sph_integrate(sph_context* ctx, double timeStep) {
+    int status = requireBound(ctx);
+    if (status != SPH_OK) return status;
+    if (!(timeStep > 0.0)) return fail(ctx, SPH_ERR_VALUE, "The time step must be positive.");
+    return dispatchPhase(ctx, 2, timeStep);
+}
+
+int sph_synchronize(sph_context* ctx) {
+    if (!ctx) return SPH_ERR_VALUE;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return SPH_OK;
+}
+
+int sph_profile(sph_context* ctx, int32_t enable) {
+    if (!ctx) return SPH_ERR_VALUE;
+    int status = collectTimes(ctx);
+    if (status != SPH_OK) return status;
+    ctx->profiling = enable != 0;
+    for (int k = 0; k < 16; ++k) {
+        ctx->kernelMs[k] = 0.0;
+        ctx->kernelLaunches[k] = 0;
+    }
+    return SPH_OK;
+}
+
+int sph_kernel_times(sph_context* ctx, double* milliseconds, int64_t* launches, int32_t count) {
+    if (!ctx) return SPH_ERR_VALUE;
+    int status = collectTimes(ctx);
+    if (status != SPH_OK) return status;
+    for (int32_t k = 0; k < count && k < kKernelKinds; ++k) {
+        if (milliseconds) milliseconds[k] = ctx->kernelMs[k];
+        if (launches) launches[k] = ctx->kernelLaunches[k];
+    }
+    return SPH_OK;
+}
+
+int sph_set_launch(sph_context* ctx, int32_t blockSize) {
+    if (!ctx) return SPH_ERR_VALUE;
+    if (blockSize < 64 || blockSize > 1024 || blockSize % 64 != 0)
+        return fail(ctx, SPH_ERR_VALUE, "The workgroup size must be a multiple of 64 from 64 to 1024.");
+    if (blockSize != ctx->block) {
+        ctx->block = blockSize;
+        dropGraph(ctx);
+    }
+    return SPH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,684 @@
+'''
+
+The MI355X counterpart of pySurf's `sphGpu.PairSums` (src/pysurf/sphGpu.py:1492-2323): the same
+method surface over the HIP engine (include/sph_hip.h), with device state held in PyTorch-ROCm
+tensors and every sum, step and enforcement run by hand-written HIP kernels.
+
+    host steps      `update`, then `pairAccelerations`, `continuityRate`, `wallPressure`,
+                    `velocityGradients`, `correctionMatrices`: host float64 arrays in, host
+                    float64 arrays out, shaped as the reference returns them
+    device steps    `setPhysics`, `push`, `setBoundary`, then `residentRates` and
+                    `residentIntegrate(dt)` each step, or `advance`, which keeps the time step on
+                    the device and replays the steps as a HIP graph; `pull` brings the state back
+
+Pair arithmetic is float32 over float64 state by default, each float64 position carried as a
+float32 point and remainder as the reference does (sphGpu.py:28-38); `pairPrecision = 'float64'`
+runs the plain float64 kernels of the same source.
+
+The options the delta-SPH scheme leaves off (XSPH, the Shepard filter, the renormalized 'antuono'
+diffusion, particle shifting, the delta-LES closure and the energy budget's ledger) are not
+built: their setters accept them switched off and raise ValueError when asked to switch one on.
+
+'''
+
+from __future__ import annotations
+
+import ctypes
+import math
+
+import numpy as np
+import torch
+
+from pysurf_amd import _native
+from pysurf_amd.errors import EngineError
+
+taitExponent = 7.0
+diffusionModes = {'none': 0, 'molteni': 1, 'hydrostatic': 2}
+maxGridCells = 2**25
+
+
+def cubicSplineNormalization(smoothingLength: float, dimensions: int) -> float:
+    '''The cubic spline's sigma: 10 / (7 pi h^2) in 2D, 1 / (pi h^3) in 3D (sphGpu.py:1424-1429).'''
+    h = smoothingLength
+    return 10.0 / (7.0 * math.pi * h * h) if dimensions == 2 else 1.0 / (math.pi * h * h * h)
+
+
+def wendlandNormalization(smoothingLength: float, dimensions: int) -> float:
+    '''The Wendland C2's sigma: 7 / (4 pi h^2) in 2D, 21 / (16 pi h^3) in 3D (sphGpu.py:1431-1436).'''
+    h = smoothingLength
+    return 7.0 / (4.0 * math.pi * h * h) if dimensions == 2 else 21.0 / (16.0 * math.pi * h * h * h)
+
+
+def kernelNormalization(kernelType: str, smoothingLength: float, dimensions: int) -> float:
+    '''The sigma of 'cubicSpline' or 'wendlandC2' (sphGpu.py:1438-1444).'''
+    if kernelType == 'wendlandC2':
+        return wendlandNormalization(smoothingLength, dimensions)
+    return cubicSplineNormalization(smoothingLength, dimensions)
+
+
+def gridDimensions(domainMin, domainMax, cellSize: float, dimensions: int) -> tuple:
+    '''Cells along x, y and z covering the domain, one support across, capped at 2^25 (sphGpu.py:1466-1478).'''
+    extent = np.zeros(3)
+    extent[:dimensions] = np.asarray(domainMax, dtype = float) - np.asarray(domainMin, dtype = float)
+    cells = np.maximum(np.ceil(extent / cellSize).astype(int) + 3, 1)
+    if dimensions == 2:
+        cells[2] = 1
+    while int(np.prod(cells)) > maxGridCells:
+        cells = np.maximum((cells * 0.8).astype(int), 1)
+    return tuple(int(count) for count in cells)
+
+
+def deviceOf(device) -> torch.device:
+    '''The HIP device of a Warp-style device name; the engine has no CPU path.'''
+    if device is None:
+        device = 'cuda:0'
+    device = torch.device(device)
+    if device.type != 'cuda':
+        raise ValueError(f'The HIP engine runs on a GPU, not on {device}.')
+    if not torch.cuda.is_available():
+        raise EngineError('No HIP device is present; the engine has no CPU path.')
+    return device
+
+
+class Engine:
+
+    '''
+
+    A context of the HIP library over one or more runs, with the runs' state in torch tensors.
+    `PairSums` holds one; the batch engine (pysurf_amd.batch) holds one over many runs.
+
+    '''
+
+    def __init__(self, dimensions: int, precision: str, device = None) -> None:
+        if dimensions not in (2, 3):
+            raise ValueError(f'Dimensions must be 2 or 3, not {dimensions}.')
+        if precision not in ('float32', 'float64'):
+            raise ValueError(f"Pair precision '{precision}' is not 'float32' or 'float64'.")
+        self.dimensions = dimensions
+        self.precision = precision
+        self.device = deviceOf(device)
+        self.real = torch.float32 if precision == 'float32' else torch.float64
+        self.library = _native.load()
+        with torch.cuda.device(self.device):
+            self.stream = torch.cuda.Stream(device = self.device)
+            handle = ctypes.c_void_p()
+            status = self.library.sph_create(ctypes.byref(handle), dimensions, 0 if precision == 'float32' else 1,
+                                             ctypes.c_void_p(self.stream.cuda_stream))
+        if status != _native.SPH_OK:
+            raise EngineError('sph_create failed: no HIP device or a bad argument.')
+        self.handle = handle
+        self.capacity = 0
+        self.count = 0
+        self.tensors: dict[str, torch.Tensor] = {}
+        self._runs = None
+        self._layout = None
+        self._keep: list = []
+
+    def __del__(self) -> None:
+        handle = getattr(self, 'handle', None)
+        if handle is not None and handle.value:
+            try:
+                self.library.sph_destroy(handle)
+            except Exception:
+                pass
+            self.handle = None
+
+    def check(self, status: int, what: str) -> None:
+        _native.check(status, self.handle, what)
+
+    # -- State -- #
+    def reserve(self, count: int) -> None:
+        '''State tensors for at least `count` slots, growing by 1.25 as the reference's _reserve does (sphGpu.py:1580-1594).'''
+        if count <= self.capacity and self.tensors:
+            return
+        capacity = max(count, int(1.25 * self.capacity), 1)
+        d, dev, real = self.dimensions, self.device, self.real
+        old, oldCount = self.tensors, self.count
+        with torch.cuda.device(dev), torch.cuda.stream(self.stream):
+            fresh = {
+                'position': torch.zeros((d, capacity), dtype = torch.float64, device = dev),
+                'velocity': torch.zeros((d, capacity), dtype = torch.float64, device = dev),
+                'density': torch.zeros(capacity, dtype = torch.float64, device = dev),
+                'pressure': torch.zeros(capacity, dtype = real, device = dev),
+                'acceleration': torch.zeros((d, capacity), dtype = real, device = dev),
+                'mass': torch.zeros(capacity, dtype = real, device = dev),
+                'length': torch.zeros(capacity, dtype = real, device = dev),
+                'fluid': torch.zeros(capacity, dtype = torch.uint8, device = dev),
+                # host-step inputs and outputs
+                'velocityIn': torch.zeros((d, capacity), dtype = torch.float64, device = dev),
+                'densityIn': torch.zeros(capacity, dtype = torch.float64, device = dev),
+                'pressureIn': torch.zeros(capacity, dtype = real, device = dev),
+                'vector': torch.zeros((d, capacity), dtype = real, device = dev),
+                'scalar': torch.zeros(capacity, dtype = real, device = dev),
+                'scalar64': torch.zeros(capacity, dtype = torch.float64, device = dev),
+                'matrix': torch.zeros((d * d, capacity), dtype = real, device = dev),
+            }
+            if old and oldCount:
+                for name, tensor in fresh.items():
+                    tensor[..., :oldCount].copy_(old[name][..., :oldCount])
+        self.tensors = fresh
+        self.capacity = capacity
+        self._layout = None
+
+    def configure(self, runs: list) -> None:
+        '''Hand the runs (a list of _native.Run) to the library and bind the state tensors.'''
+        count = sum(int(run.count) for run in runs)
+        self.reserve(count)
+        self.count = count
+        array = (_native.Run * len(runs))(*runs)
+        self._runs = array
+        self.check(self.library.sph_configure(self.handle, array, len(runs), self.capacity), 'sph_configure')
+        t = self.tensors
+        state = _native.State(t['position'].data_ptr(), t['velocity'].data_ptr(), t['density'].data_ptr(),
+                              t['pressure'].data_ptr(), t['acceleration'].data_ptr(), t['mass'].data_ptr(),
+                              t['length'].data_ptr(), t['fluid'].data_ptr(), self.capacity)
+        self.check(self.library.sph_bind(self.handle, ctypes.byref(state)), 'sph_bind')
+
+    def upload(self, name: str, values, count: int | None = None) -> torch.Tensor:
+        '''Copy host values (N,), or (N, dim) into the rows of the field, into a state tensor.'''
+        count = self.count if count is None else count
+        tensor = self.tensors[name]
+        values = np.asarray(values)
+        with torch.cuda.device(self.device), torch.cuda.stream(self.stream):
+            source = torch.from_numpy(np.ascontiguousarray(values.T if values.ndim == 2 else values))
+            if tensor.dim() == 2:
+                tensor[:values.shape[1] if values.ndim == 2 else 1, :count].copy_(source.to(tensor.dtype), non_blocking = False)
+            else:
+                tensor[:count].copy_(source.to(tensor.dtype), non_blocking = False)
+        return tensor
+
+    def download(self, name: str, rows: int | None = None, count: int | None = None) -> np.ndarray:
+        '''A state tensor's first `count` slots as float64, (N, rows) for a field of rows, (N,) otherwise.'''
+        count = self.count if count is None else count
+        self.synchronize()
+        tensor = self.tensors[name]
+        if tensor.dim() == 2:
+            rows = tensor.shape[0] if rows is None else rows
+            return tensor[:rows, :count].to(torch.float64).cpu().numpy().T.copy()
+        return tensor[:count].to(torch.float64).cpu().numpy()
+
+    def synchronize(self) -> None:
+        self.check(self.library.sph_synchronize(self.handle), 'sph_synchronize')
+
+    def pointer(self, name: str) -> ctypes.c_void_p:
+        return ctypes.c_void_p(self.tensors[name].data_ptr())
+
+    # -- Clock -- #
+    def setClock(self, run: int, time: float, untilTime: float, steps: int, maxSteps: int) -> None:
+        self.check(self.library.sph_set_clock(self.handle, run, float(time), float(untilTime), int(steps),
+                                              int(maxSteps)), 'sph_set_clock')
+
+    def clock(self, runs: int) -> dict:
+        time, dt = np.zeros(runs), np.zeros(runs)
+        steps = np.zeros(runs, dtype = np.int64)
+        maxima = np.zeros(2 * runs)
+        self.check(self.library.sph_get_clock(self.handle, time.ctypes.data_as(ctypes.c_void_p),
+                                              dt.ctypes.data_as(ctypes.c_void_p),
+                                              steps.ctypes.data_as(ctypes.c_void_p),
+                                              maxima.ctypes.data_as(ctypes.c_void_p)), 'sph_get_clock')
+        return {'time': time, 'dt': dt, 'steps': steps, 'maxima': maxima.reshape(runs, 2)}
+
+    def advance(self, steps: int, graph: bool = True) -> None:
+        self.check(self.library.sph_advance(self.handle, int(steps), 1 if graph else 0), 'sph_advance')
+
+    def setLaunch(self, blockSize: int) -> None:
+        self.check(self.library.sph_set_launch(self.handle, int(blockSize)), 'sph_set_launch')
+
+
+def boundaryOf(boundary, dimensions: int, device: torch.device, pistonSpeed: float = 0.0) -> tuple:
+    '''
+
+    The sph_boundary of a ContainerBoundary, BedBoundary or HeightFieldBoundary, as
+    PairSums.setBoundary lays it out (sphGpu.py:2121-2171); returns it with the device tables it
+    points into, which the caller keeps alive.
+
+    '''
+
+    from pysurf_amd import boundaries
+
+    BedBoundary, ContainerBoundary = boundaries.BedBoundary, boundaries.ContainerBoundary
+    HeightFieldBoundary = getattr(boundaries, 'HeightFieldBoundary', type(None))
+    b = _native.Boundary()
+    keep = []
+    if boundary is None:
+        b.kind = 0
+    elif isinstance(boundary, ContainerBoundary):
+        b.kind = 1
+        lower, upper = np.zeros(3), np.zeros(3)
+        lower[:dimensions] = boundary.containerMin
+        upper[:dimensions] = boundary.containerMax
+        b.lower[:] = list(lower)
+        b.upper[:] = list(upper)
+        b.openTop = 1 if boundary.openTop else 0
+    elif isinstance(boundary, BedBoundary):
+        bed = boundary.bathymetry
+        b.kind = 2
+        b.clearance = 0.25 * boundary.spacing
+        bedX = torch.tensor(bed.x, dtype = torch.float64, device = device)
+        bedZ = torch.tensor(bed.z, dtype = torch.float64, device = device)
+        keep += [bedX, bedZ]
+        b.nBed = len(bed.x)
+        b.bedX, b.bedZ = bedX.data_ptr(), bedZ.data_ptr()
+        b.sideWalls = 1 if (boundary.width is not None and not boundary.periodic) else 0
+        b.width = float(boundary.width or 0.0)
+        b.offshoreWall = 1 if boundary.wallTop is not None else 0
+        b.wallX = float(boundary.wallX)
+        b.pistonSpeed = float(pistonSpeed)
+        b.endLimit = bed.endX - b.clearance if boundary.endWallTop is not None else bed.endX
+    elif isinstance(boundary, HeightFieldBoundary):
+        field = boundary.field
+        b.kind = 3
+        b.clearance = 0.25 * boundary.spacing
+        table = torch.tensor(np.ascontiguousarray(field.z), dtype = torch.float64, device = device)
+        keep.append(table)
+        b.fieldNy, b.fieldNx = field.z.shape
+        b.table = table.data_ptr()
+        startX, endX, startY, endY = field.bounds
+        b.originX, b.originY = float(field.x[0]), float(field.y[0])
+        b.stepX, b.stepY = float(field.spacingX), float(field.spacingY)
+        b.lowY, b.highY = startY + b.clearance, endY - b.clearance
+        b.offshoreWall = 1 if boundary.offshoreWall else 0
+        b.lowX = startX + b.clearance
+        b.highX = endX - b.clearance if boundary.endWall else endX
+    else:
+        raise ValueError(f'The device cannot enforce a {type(boundary).__name__}; step from the host.')
+    return b, keep
+
+
+class PairSums:
+
+    '''
+
+    The SPH pair sums and the time step of one particle system on an MI355X
+    (sphGpu.PairSums, sphGpu.py:1492-1570).
+
+    Parameters:
+    -----------
+    dimensions : int
+        2 or 3
+    smoothingLength : float
+        h [m]; the support is 2h
+    domainMin, domainMax : np.ndarray
+        Corners of the region the particles occupy [m], which sizes the cell grid
+    period : float
+        Length of a periodic y axis in 3D [m]; 0 for none
+    separationPrecision : str
+        'float64', separations from each float64 position's float32 point and remainder, the
+        only form the engine builds
+    device : str | None
+        'cuda:N'; None for cuda:0
+    kernelType : str
+        'cubicSpline' or 'wendlandC2'
+    pairPrecision : str
+        'float32' pair sums over float64 state, or 'float64' throughout (the plain kernel)
+
+    '''
+
+    def __init__(self, dimensions: int, smoothingLength: float, domainMin: np.ndarray, domainMax: np.ndarray,
+                 period: float = 0.0, separationPrecision: str = 'float64', device: str | None = None,
+                 kernelType: str = 'cubicSpline', pairPrecision: str = 'float32') -> None:
+
+        if separationPrecision not in ('float64', 'float32'):
+            raise ValueError(f"Separation precision '{separationPrecision}' is not 'float64' or 'float32'.")
+        if separationPrecision == 'float32':
+            raise ValueError("The engine takes separations from the float64 positions only ('float64').")
+        if kernelType not in ('cubicSpline', 'wendlandC2'):
+            raise ValueError(f"Kernel '{kernelType}' is not 'cubicSpline' or 'wendlandC2'.")
+        if period > 0.0 and dimensions != 3:
+            raise ValueError('A periodic axis needs three dimensions: it is y.')
+        self.dimensions = dimensions
+        self.smoothingLength = float(smoothingLength)
+        self.kernelType = kernelType
+        self.kind = 1 if kernelType == 'wendlandC2' else 0
+        self.sigma = kernelNormalization(kernelType, smoothingLength, dimensions)
+        self.sigmaUnit = kernelNormalization(kernelType, 1.0, dimensions)
+        self.maxLength = self.smoothingLength
+        self.minLength = self.smoothingLength
+        self.uniformLengths = True
+        self.period = float(period)
+        self.precise = 1
+        self.pairPrecision = pairPrecision
+        self.gridShape = gridDimensions(domainMin, domainMax, 2.0 * smoothingLength, dimensions)
+        self.count = 0
+        self.physics: dict = {}
+        self.diffusion = {'mode': 0, 'factor': 0.0, 'delta': 0.1, 'referenceDensity': 1000.0,
+                          'gravityOverSound': 0.0, 'soundSpeed': 1.0}
+        self.extrapolateWalls = False
+        self.negativePressure = False
+        self.correctContinuity = False
+        self.limitViscosity = False
+        self.trackEnergy = False
+        self.engine = Engine(dimensions, pairPrecision, device)
+        self.device = self.engine.device
+        self._boundary = _native.Boundary()
+        self._boundaryKeep: list = []
+        self._configured = None
+
+    # -- Configuration -- #
+    def _run(self) -> _native.Run:
+        '''This system's sph_run from its settings.'''
+        run = _native.Run()
+        run.first, run.count = 0, self.count
+        run.kernel = self.kind
+        run.grid[:] = list(self.gridShape)
+        run.cell = 2.0 * self.smoothingLength
+        run.hmax, run.hmin = self.maxLength, self.minLength
+        run.period = self.period
+        physics = self.physics
+        run.rho0 = physics.get('referenceDensity', self.diffusion['referenceDensity'])
+        run.c0 = physics.get('soundSpeed', self.diffusion['soundSpeed'])
+        run.alpha = physics.get('alpha', 0.0)
+        run.boundaryFactor = physics.get('boundaryViscosityFactor', 0.0)
+        g = np.zeros(3)
+        g[:self.dimensions] = physics.get('gravity64', np.zeros(self.dimensions))
+        run.gravity[:] = list(g)
+        run.diffusion = self.diffusion['mode']
+        run.delta = self.diffusion['delta']
+        run.extrapolatedWalls = int(self.extrapolateWalls)
+        run.negativePressure = int(self.negativePressure)
+        run.correction = int(self.correctContinuity)
+        run.limiter = int(self.limitViscosity)
+        run.densityFloor = physics.get('densityFloor', 0.0)
+        run.cfl = physics.get('cfl', 0.25)
+        run.maxTimeStep = physics.get('maxTimeStep', 1.0)
+        run.boundary = self._boundary
+        return run
+
+    def _configure(self) -> None:
+        '''Hand the current settings to the engine (the cells are rebuilt by the next update or step).'''
+        self.engine.configure([self._run()])
+        self._configured = True
+
+    def setDiffusion(self, mode: str, delta: float, soundSpeed: float, referenceDensity: float, gravity: float) -> None:
+        '''The delta-SPH term the continuity sum carries (sphGpu.py:1879-1904).'''
+        if mode == 'antuono':
+            raise ValueError("The renormalized 'antuono' diffusion is not built; use 'molteni' or 'hydrostatic'.")
+        if mode not in diffusionModes:
+            raise ValueError(f"Density diffusion '{mode}' is not 'none', 'molteni' or 'hydrostatic'.")
+        self.diffusion = {'mode': diffusionModes[mode], 'factor': 2.0 * delta * soundSpeed, 'delta': float(delta),
+                          'referenceDensity': float(referenceDensity), 'gravityOverSound': gravity / soundSpeed**2,
+                          'soundSpeed': float(soundSpeed)}
+        self._configured = None
+
+    def setWallPressure(self, extrapolated: bool) -> None:
+        '''Whether boundary particles take their pressure from the fluid (sphGpu.py:1906-1914).'''
+        self.extrapolateWalls = bool(extrapolated)
+        self._configured = None
+
+    def setNegativePressure(self, enabled: bool) -> None:
+        '''Whether the fluid may carry tension (sphGpu.py:1916-1920).'''
+        self.negativePressure = bool(enabled)
+        self._configured = None
+
+    def setContinuityCorrection(self, enabled: bool) -> None:
+        '''Whether the continuity sum takes the kernel gradient correction (sphGpu.py:1696-1702).'''
+        self.correctContinuity = bool(enabled)
+        self._configured = None
+
+    def setViscosityLimiter(self, enabled: bool) -> None:
+        '''Whether the artificial viscosity takes the limited velocity difference (sphGpu.py:1734-1740).'''
+        self.limitViscosity = bool(enabled)
+        self._configured = None
+
+    def setShifting(self, enabled: bool, mach: float = 0.1, spacing: float = 0.0, soundSpeed: float = 1.0,
+                    local: bool = False) -> None:
+        '''Particle shifting (sphGpu.py:1922-1955), off under the delta-SPH scheme and not built.'''
+        if enabled:
+            raise ValueError('Particle shifting is not built; the delta-SPH scheme runs without it.')
+
+    def setClosure(self, enabled: bool, soundSpeed: float = 1.0, referenceDensity: float = 1000.0,
+                   kinematicViscosity: float = 1e-6) -> None:
+        '''The delta-LES-SPH closure (sphGpu.py:1742-1775), off under the delta-SPH scheme and not built.'''
+        if enabled:
+            raise ValueError('The delta-LES-SPH closure is not built; the delta-SPH scheme runs without it.')
+
+    def setEnergyBudget(self, enabled: bool) -> None:
+        '''The per-particle energy ledger (sphGpu.py:1777-1789), not built: use `energy` for the global sums.'''
+        if enabled:
+            raise ValueError('The per-particle energy ledger is not built.')
+
+    # -- Host steps -- #
+    def update(self, positions: np.ndarray, masses: np.ndarray, isFluid: np.ndarray,
+               lengths: np.ndarray | None = None) -> None:
+
+        '''
+
+        Copy the positions, masses, fluid flags and smoothing lengths to the device and rebuild the
+        cell list (sphGpu.py:1998-2042).
+
+        Raises:
+        -------
+        ValueError : If a smoothing length exceeds the grid's.
+
+        '''
+
+        count = len(positions)
+        if lengths is None:
+            lengths = np.full(count, self.smoothingLength)
+        lengths = np.asarray(lengths, dtype = float)
+        if count and lengths.max() > self.maxLength * (1.0 + 1e-9):
+            raise ValueError(f'A smoothing length of {lengths.max()} m exceeds the {self.maxLength} m the cell grid '
+                             f'is built for.')
+        self.count = count
+        if count:
+            self.minLength = float(lengths.min())
+            self.uniformLengths = bool(np.all(lengths == lengths[0]))
+        self._configure()
+        if count == 0:
+            return
+        engine = self.engine
+        engine.upload('position', np.asarray(positions, dtype = float))
+        engine.upload('mass', np.asarray(masses, dtype = float))
+        engine.upload('length', lengths)
+        engine.upload('fluid', np.asarray(isFluid, dtype = bool).astype(np.uint8))
+        self.check(engine.library.sph_build_cells(engine.handle), 'sph_build_cells')
+
+    def check(self, status: int, what: str) -> None:
+        self.engine.check(status, what)
+
+    def _sync(self) -> None:
+        """Push changed settings to the engine: a descriptor upload, the layout and cells unchanged."""
+        if self._configured is None and self.count:
+            self.check(self.engine.library.sph_configure(self.engine.handle, ctypes.byref(self._run()), 1,
+                                                         self.engine.capacity), 'sph_configure')
+            self._configured = True
+
+    def _gravityPhysics(self, soundSpeed, alpha, boundaryFactor, gravity, referenceDensity = None) -> None:
+        physics = dict(self.physics)
+        physics.update(soundSpeed = float(soundSpeed), alpha = float(alpha),
+                       boundaryViscosityFactor = float(boundaryFactor),
+                       gravity64 = np.asarray(gravity, dtype = float))
+        if referenceDensity is not None:
+            physics['referenceDensity'] = float(referenceDensity)
+        self.physics = physics
+        self._configured = None
+        self._sync()
+
+    def pairAccelerations(self, velocities: np.ndarray, densities: np.ndarray, pressures: np.ndarray,
+                          soundSpeed: float, alpha: float, boundaryViscosityFactor: float,
+                          gravity: np.ndarray) -> np.ndarray:
+        '''Pressure gradient, artificial viscosity and gravity on the fluid, zero on the boundary (sphGpu.py:2044-2059).'''
+        self._gravityPhysics(soundSpeed, alpha, boundaryViscosityFactor, gravity)
+        engine = self.engine
+        engine.upload('velocityIn', velocities)
+        engine.upload('densityIn', densities)
+        engine.upload('pressureIn', pressures)
+        self.check(engine.library.sph_pair_accelerations(engine.handle, engine.pointer('velocityIn'),
+                                                         engine.pointer('densityIn'), engine.pointer('pressureIn'),
+                                                         engine.pointer('vector')), 'sph_pair_accelerations')
+        return engine.download('vector')
+
+    def continuityRate(self, velocities: np.ndarray, densities: np.ndarray | None = None) -> np.ndarray:
+        '''d rho / dt for every particle, with the diffusive term setDiffusion set (sphGpu.py:2061-2078).'''
+        if densities is None:
+            if self.diffusion['mode'] != 0 or self.correctContinuity:
+                raise ValueError('The diffusive term and the correction need the densities.')
+            densities = np.full(self.count, self.diffusion['referenceDensity'])
+        if self.diffusion['mode'] != 0 and not self.physics:
+            self._gravityPhysics(self.diffusion['soundSpeed'], 0.0, 0.0,
+                                 self._diffusionGravity(), self.diffusion['referenceDensity'])
+        self._sync()
+        engine = self.engine
+        engine.upload('velocityIn', velocities)
+        engine.upload('densityIn', densities)
+        self.check(engine.library.sph_continuity_rate(engine.handle, engine.pointer('velocityIn'),
+                                                      engine.pointer('densityIn'), engine.pointer('scalar')),
+                   'sph_continuity_rate')
+        return engine.download('scalar')
+
+    def _diffusionGravity(self) -> np.ndarray:
+        '''Gravity along the last axis, from setDiffusion's g / c0^2.'''
+        g = np.zeros(self.dimensions)
+        g[-1] = -self.diffusion['gravityOverSound'] * self.diffusion['soundSpeed']**2
+        return g
+
+    def wallPressure(self, densities: np.ndarray, pressures: np.ndarray, gravity: np.ndarray, referenceDensity: float,
+                     soundSpeed: float) -> np.ndarray:
+        '''Every pressure with the boundary's taken from the fluid (sphGpu.py:2090-2102).'''
+        physics = self.physics
+        self._gravityPhysics(soundSpeed, physics.get('alpha', 0.0), physics.get('boundaryViscosityFactor', 0.0),
+                             gravity, referenceDensity)
+        engine = self.engine
+        engine.upload('densityIn', densities)
+        engine.upload('pressureIn', pressures)
+        self.check(engine.library.sph_wall_pressure(engine.handle, engine.pointer('densityIn'),
+                                                    engine.pointer('pressureIn'), engine.pointer('scalar'),
+                                                    engine.pointer('scalar64')), 'sph_wall_pressure')
+        self.wallDensities = engine.download('scalar64')
+        return engine.download('scalar')
+
+    def velocityGradients(self, velocities: np.ndarray, densities: np.ndarray) -> np.ndarray:
+        '''G for every particle, shape (N, dim, dim) (sphGpu.py:1827-1839).'''
+        if not self.limitViscosity:
+            raise ValueError('The viscosity limiter is not on; see setViscosityLimiter.')
+        self._sync()
+        engine = self.engine
+        engine.upload('velocityIn', velocities)
+        engine.upload('densityIn', densities)
+        self.check(engine.library.sph_velocity_gradients(engine.handle, engine.pointer('velocityIn'),
+                                                         engine.pointer('densityIn'), engine.pointer('matrix')),
+                   'sph_velocity_gradients')
+        dim = self.dimensions
+        return engine.download('matrix').reshape(self.count, dim, dim)
+
+    def correctionMatrices(self, densities: np.ndarray) -> np.ndarray:
+        '''L for every particle, shape (N, dim, dim) (sphGpu.py:1855-1866).'''
+        if not self.correctContinuity:
+            raise ValueError('The continuity correction is not on; see setContinuityCorrection.')
+        self._sync()
+        engine = self.engine
+        engine.upload('densityIn', densities)
+        self.check(engine.library.sph_correction_matrices(engine.handle, engine.pointer('densityIn'),
+                                                          engine.pointer('matrix')), 'sph_correction_matrices')
+        dim = self.dimensions
+        return engine.download('matrix').reshape(self.count, dim, dim)
+
+    def neighborLists(self) -> tuple[np.ndarray, np.ndarray]:
+        '''Each particle's neighbors {j != i : r_ij < h_i + h_j}, as CSR (offsets, rows), each row ascending.'''
+        self._sync()
+        engine, n = self.engine, self.count
+        counts = torch.zeros(max(n, 1), dtype = torch.int32, device = self.device)
+        self.check(engine.library.sph_neighbor_counts(engine.handle, ctypes.c_void_p(counts.data_ptr())),
+                   'sph_neighbor_counts')
+        counts = counts[:n].cpu().numpy().astype(np.int64)
+        offsets = np.zeros(n + 1, dtype = np.int64)
+        np.cumsum(counts, out = offsets[1:])
+        deviceOffsets = torch.from_numpy(offsets).to(self.device)
+        rows = torch.zeros(max(int(offsets[-1]), 1), dtype = torch.int32, device = self.device)
+        self.check(engine.library.sph_neighbor_rows(engine.handle, ctypes.c_void_p(deviceOffsets.data_ptr()),
+                                                    ctypes.c_void_p(rows.data_ptr())), 'sph_neighbor_rows')
+        rows = rows[:int(offsets[-1])].cpu().numpy()
+        owner = np.repeat(np.arange(n), counts)
+        order = np.lexsort((rows, owner))
+        return offsets, rows[order]
+
+    # -- Device steps -- #
+    def canEnforce(self, boundary) -> bool:
+        '''Whether the device can enforce this boundary (sphGpu.py:2115-2119).'''
+        from pysurf_amd import boundaries
+        kinds = tuple(getattr(boundaries, name) for name in ('ContainerBoundary', 'BedBoundary', 'HeightFieldBoundary')
+                      if hasattr(boundaries, name))
+        return boundary is None or isinstance(boundary, kinds)
+
+    def setBoundary(self, boundary, particles) -> None:
+        '''The boundary the device steps enforce, a piston held at its speed (sphGpu.py:2121-2171).'''
+        pistonSpeed = 0.0
+        mask = getattr(boundary, '_pistonMask', None)
+        if mask is not None and mask.any():
+            pistonSpeed = float(particles.velocities[mask, 0][0])
+        self._boundary, self._boundaryKeep = boundaryOf(boundary, self.dimensions, self.device, pistonSpeed)
+        self._configured = None
+        self._sync()
+
+    def setPhysics(self, referenceDensity: float, soundSpeed: float, alpha: float, boundaryViscosityFactor: float,
+                   gravity: np.ndarray, xsphEpsilon: float, densityFloor: float, cflNumber: float = 0.25,
+                   maxTimeStep: float = 1.0) -> None:
+        '''The constants of the device step (sphGpu.py:2173-2205); XSPH is off under delta-SPH and not built.'''
+        if xsphEpsilon > 0.0:
+            raise ValueError('XSPH is not built; the delta-SPH scheme moves the fluid with its velocity.')
+        self.physics = {'referenceDensity': float(referenceDensity), 'soundSpeed': float(soundSpeed),
+                        'stiffness': float(referenceDensity * soundSpeed**2 / taitExponent), 'alpha': float(alpha),
+                        'boundaryViscosityFactor': float(boundaryViscosityFactor),
+                        'gravity64': np.asarray(gravity, dtype = float), 'xsphEpsilon': 0.0,
+                        'densityFloor': float(densityFloor), 'cfl': float(cflNumber),
+                        'maxTimeStep': float(maxTimeStep)}
+        self._configured = None
+
+    def push(self, particles) -> None:
+        '''Copy a particle system's state to the device (sphGpu.py:2207-2221).'''
+        self.update(particles.positions, particles.masses, particles.isFluid, particles.lengths)
+        if self.count == 0:
+            return
+        self.engine.upload('velocity', np.asarray(particles.velocities, dtype = float))
+        self.engine.upload('density', np.asarray(particles.densities, dtype = float))
+        self.engine.synchronize()
+
+    def pull(self, particles) -> None:
+        '''Copy the device state back into a particle system, in place (sphGpu.py:2223-2235).'''
+        if self.count == 0:
+            return
+        engine, dim = self.engine, self.dimensions
+        particles.positions[:] = engine.download('position')[:, :dim]
+        particles.velocities[:] = engine.download('velocity')[:, :dim]
+        particles.densities[:] = engine.download('density')
+        particles.pressures[:] = engine.download('pressure')
+        particles.accelerations[:] = engine.download('acceleration')[:, :dim]
+
+    def residentRates(self, filterDensity: bool = False) -> tuple[float, float]:
+        '''The first half of a device step; returns the largest fluid speed and acceleration (sphGpu.py:2237-2273).'''
+        if filterDensity:
+            raise ValueError('The Shepard filter is not built; the delta-SPH scheme runs without it.')
+        self._sync()
+        maxima = np.zeros(2)
+        self.check(self.engine.library.sph_rates(self.engine.handle, maxima.ctypes.data_as(ctypes.c_void_p)),
+                   'sph_rates')
+        return float(maxima[0]), float(maxima[1])
+
+    def residentIntegrate(self, timeStep: float) -> None:
+        '''The second half of a device step with the host's time step (sphGpu.py:2275-2315).'''
+        self.check(self.engine.library.sph_integrate(self.engine.handle, float(timeStep)), 'sph_integrate')
+
+    def enforceBoundary(self) -> None:
+        '''Enforce the boundary set by setBoundary on the device state (sphGpu.py:2317-2323).'''
+        self._sync()
+        self.check(self.engine.library.sph_enforce(self.engine.handle), 'sph_enforce')
+
+    def advance(self, time: float, untilTime: float, steps: int, maxSteps: int, chunk: int = 64,
+                graph: bool = True) -> dict:
+
+        '''
+
+        Device steps until the run's time reaches `untilTime` or it has taken `maxSteps` steps, the
+        time step reduced on the device, in graphs of `chunk` steps; returns the clock.
+
+        '''
+
+        self._sync()
+        engine = self.engine
+        engine.setClock(0, time, untilTime, steps, maxSteps)
+        while True:
+            engine.advance(chunk, graph)
+            clock = engine.clock(1)
+            if not (clock['time'][0] < untilTime and clock['steps'][0] < maxSteps):
+                return {'time': float(clock['time'][0]), 'steps': int(clock['steps'][0]),
+                        'dt': float(clock['dt'][0])}

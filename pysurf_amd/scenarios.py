@@ -1,0 +1,126 @@
+'''
+
+The runs the engine is measured and checked on, built as pySurf builds its 2D breaking waves
+(BreakingWave.build, src/pysurf/BreakingWave.py:658-767): a plane beach, a bed boundary with an
+offshore wall, a first-order solitary wave laid on the lattice, jittered, its densities
+hydrostatic beneath the wave (WcsphSolver.initialize, sphSolver.py:1869-1897), and the
+delta-SPH scheme's options (BreakingWave.py:722-725).
+
+'''
+
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from pysurf_amd.bathymetry import Bathymetry
+from pysurf_amd.boundaries import BedBoundary
+from pysurf_amd.particles import ParticleSystem, jitterPoints
+from pysurf_amd.waves import freshwaterDensity, gravity, solitaryWaveFluid, solitaryWaveProfile
+
+
+def taitDensity(pressure, referenceDensity: float, soundSpeed: float):
+    '''rho0 (1 + p / B)^(1/7), B = rho0 c0^2 / 7 (sphSolver.py:559-566).'''
+    stiffness = referenceDensity * soundSpeed**2 / 7.0
+    return referenceDensity * (1.0 + np.asarray(pressure) / stiffness)**(1.0 / 7.0)
+
+
+@dataclass
+class Case:
+
+    '''One run ready for the engine: particles, boundary, grid and the scheme's physics.'''
+
+    name: str
+    particles: ParticleSystem
+    boundary: BedBoundary
+    dimensions: int
+    spacing: float
+    smoothingLength: float
+    domainMin: np.ndarray
+    domainMax: np.ndarray
+    kernelType: str = 'wendlandC2'
+    referenceDensity: float = freshwaterDensity
+    soundSpeed: float = 10.0 * math.sqrt(gravity)
+    alpha: float = 0.01
+    boundaryViscosityFactor: float = 0.0
+    gravity: np.ndarray = field(default_factory = lambda: np.array([0.0, -gravity]))
+    diffusion: str = 'hydrostatic'
+    delta: float = 0.1
+    extrapolatedWalls: bool = True
+    negativePressure: bool = False
+    correction: bool = True
+    limiter: bool = True
+    densityFloor: float = 0.9 * freshwaterDensity
+    cfl: float = 0.25
+    maxTimeStep: float = 1.0
+    parameters: dict = field(default_factory = dict)
+
+    @property
+    def fluidCount(self) -> int:
+        return self.particles.nFluid
+
+    @property
+    def count(self) -> int:
+        return self.particles.nParticles
+
+
+def solitaryOnSlope(particlesPerDepth: int = 16, depth: float = 1.0, waveHeight: float = 0.45, crestX: float = 4.0,
+                    slope: float = 1.0 / 15.0, toeX: float = 8.0, length: float = 25.0, jitter: float = 1e-3,
+                    seed: int = 0, smoothingRatio: float = 1.3, kernelType: str = 'wendlandC2',
+                    soundSpeedFactor: float = 10.0, name: str | None = None) -> Case:
+
+    '''
+
+    A first-order sech^2 solitary wave of H = `waveHeight` at x = `crestX` over a bed flat to
+    `toeX`, then a plane slope, `length` long; the sound speed `soundSpeedFactor` sqrt(g d); each
+    fluid point jittered by up to `jitter` spacings with `seed`.
+
+    '''
+
+    d, s = depth, depth / particlesPerDepth
+    layers = max(3, math.ceil(2.0 * smoothingRatio - 1e-9))          # BreakingWave.py:584-586
+    wallTop = d + 3.0 * waveHeight                                   # BreakingWave.py:667
+    bed = Bathymetry.planeBeach(depth = d, toeX = toeX, slope = slope, endX = length)
+    endWallTop = wallTop if bed.elevation(bed.endX) < d else None
+    boundary = BedBoundary(bed, s, nLayers = layers, wallTop = wallTop, endWallTop = endWallTop)
+    fluid, velocities = solitaryWaveFluid(bed, waveHeight, crestX, s)
+    if jitter > 0.0:
+        surface = lambda x: d + solitaryWaveProfile(x, waveHeight, d, crestX)
+        fluid = jitterPoints(fluid, s, jitter, seed, [bed.startX], [bed.endX], bed.elevation, surface)
+    particles = ParticleSystem.assemble(fluid, boundary.positions(), s, freshwaterDensity, velocities)
+    c0 = soundSpeedFactor * math.sqrt(gravity * d)
+    x = particles.positions[:, 0]
+    surfaceElevation = d + solitaryWaveProfile(x, waveHeight, d, crestX)
+    depthBelow = np.maximum(surfaceElevation - particles.positions[:, -1], 0.0)
+    particles.densities = taitDensity(freshwaterDensity * gravity * depthBelow, freshwaterDensity, c0)
+    h = smoothingRatio * s
+    return Case(name = name or f'solitary-{particlesPerDepth}ppd', particles = particles, boundary = boundary,
+                dimensions = 2, spacing = s, smoothingLength = h,
+                domainMin = np.array([-(layers + 1.0) * s, -(layers + 1.0) * s]),
+                domainMax = np.array([bed.endX + (layers + 1.0) * s, wallTop]), kernelType = kernelType,
+                soundSpeed = c0, densityFloor = 0.9 * freshwaterDensity,
+                parameters = dict(particlesPerDepth = particlesPerDepth, waveHeight = waveHeight, slope = slope,
+                                  crestX = crestX, toeX = toeX, length = length, seed = seed, depth = d))
+
+
+def singleRun(seed: int = 0, particlesPerDepth: int = 64) -> Case:
+    '''The 2D solitary wave at 64 particles per depth with H/d ~ U[0.3, 0.6] and slope ~ U[1/35, 1/10] from `seed`.'''
+    rng = np.random.default_rng(seed)
+    waveHeight = float(rng.uniform(0.3, 0.6))
+    slope = float(rng.uniform(1.0 / 35.0, 1.0 / 10.0))
+    return solitaryOnSlope(particlesPerDepth = particlesPerDepth, waveHeight = waveHeight, slope = slope, seed = seed,
+                           name = f'solitary-{particlesPerDepth}ppd-seed{seed}')
+
+
+def sweepRun(seed: int, resolutions = (16, 24, 32, 48)) -> Case:
+    '''One run of the batched sweep: slope U[1/35, 1/8], H/d U[0.2, 0.6], ppd from `resolutions`, length U[20, 30] d.'''
+    rng = np.random.default_rng(1000 + seed)
+    slope = float(rng.uniform(1.0 / 35.0, 1.0 / 8.0))
+    waveHeight = float(rng.uniform(0.2, 0.6))
+    ppd = int(rng.choice(resolutions))
+    length = float(rng.uniform(20.0, 30.0))
+    toeX = min(8.0, 0.5 * length)
+    return solitaryOnSlope(particlesPerDepth = ppd, waveHeight = waveHeight, slope = slope, length = length,
+                           toeX = toeX, seed = seed, name = f'sweep-{seed}')

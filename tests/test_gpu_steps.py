@@ -1,0 +1,177 @@
+'''
+
+Device-resident steps of the HIP engine against the float64 restatement's steps (oracle/), on a
+2D solitary wave over a plane beach with the delta-SPH scheme (extrapolated walls, L-corrected
+continuity, hydrostatic diffusion, limited viscosity, density floor) and its bed boundary.
+
+Tolerances: the float64 kernels follow the restatement to near rounding; the float32 pair sums
+follow it as the reference's float32 device follows its float64 array solver
+(tests/testSphGpu.py:503-514: positions 1e-6, velocities 1e-3 over 30 steps). Graph and stream
+launches, split and fused steps, and a run alone and in a batch must agree bit for bit.
+
+'''
+
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def physicsOf(case) -> oracle.Physics:
+    return oracle.Physics(dims = case.dimensions, h = case.smoothingLength, domainMin = case.domainMin,
+                          domainMax = case.domainMax, kernelType = case.kernelType, rho0 = case.referenceDensity,
+                          c0 = case.soundSpeed, alpha = case.alpha, boundaryFactor = case.boundaryViscosityFactor,
+                          gravity = case.gravity, diffusion = case.diffusion, delta = case.delta,
+                          extrapolatedWalls = case.extrapolatedWalls, negativePressure = case.negativePressure,
+                          correction = case.correction, limiter = case.limiter, densityFloor = case.densityFloor,
+                          cfl = case.cfl, maxTimeStep = case.maxTimeStep)
+
+
+def smallCase(seed = 0, ppd = 8, **options):
+    from pysurf_amd.scenarios import solitaryOnSlope
+    return solitaryOnSlope(particlesPerDepth = ppd, seed = seed, **options)
+
+
+def oracleRun(case, steps):
+    p = case.particles
+    run = oracle.OracleRun(physicsOf(case), case.boundary.description(), p.positions, p.velocities, p.densities,
+                           p.masses, p.isFluid)
+    taken = run.advance(np.inf, steps)
+    assert taken == steps
+    return run
+
+
+def engineRun(cases, steps, precision, graph = True, chunk = None):
+    from pysurf_amd.batch import BatchEngine
+    batch = BatchEngine(cases, precision)
+    batch.setClocks(maxSteps = steps)
+    batch.advance(chunk or steps, graph)
+    if chunk:
+        for _ in range(steps // chunk - 1):
+            batch.advance(chunk, graph)
+    batch.synchronize()
+    return batch
+
+
+@pytest.mark.parametrize('precision, tolerances', [('float64', (1e-10, 1e-8, 1e-9)), ('float32', (1e-6, 1e-3, 1e-3))])
+def testDeviceStepsFollowTheOracle(precision, tolerances):
+    steps = 60
+    print(f'device steps {precision} vs oracle, {steps} steps', flush = True)
+    case = smallCase()
+    expected = oracleRun(case, steps)
+    batch = engineRun([case], steps, precision)
+    clock = batch.clocks()
+    state = batch.state(0)
+    positionTol, velocityTol, densityTol = tolerances
+    assert int(clock['steps'][0]) == steps
+    assert clock['time'][0] == pytest.approx(expected.time.value, rel = 1e-12 if precision == 'float64' else 1e-5)
+    assert state.positions == pytest.approx(expected.positions, abs = positionTol)
+    assert state.velocities == pytest.approx(expected.velocities, abs = velocityTol)
+    assert state.densities == pytest.approx(expected.dens, abs = densityTol)
+
+
+def testGraphAndStreamLaunchesAgreeBitForBit():
+    case = smallCase(seed = 1)
+    print('graph vs stream launches', flush = True)
+    a = engineRun([case], 40, 'float32', graph = True, chunk = 8).state(0)
+    b = engineRun([case], 40, 'float32', graph = False, chunk = 8).state(0)
+    for name in ('positions', 'velocities', 'densities', 'pressures', 'accelerations'):
+        assert np.array_equal(getattr(a, name), getattr(b, name)), name
+
+
+def testARunAloneAndInABatchAgreeBitForBit():
+    # The same run stepped alone and packed between two others of different sizes and physics
+    print('batched vs solo', flush = True)
+    cases = [smallCase(seed = 2, ppd = 6, waveHeight = 0.3), smallCase(seed = 3, ppd = 8),
+             smallCase(seed = 4, ppd = 10, slope = 1.0 / 25.0)]
+    steps = 30
+    together = engineRun(cases, steps, 'float32')
+    for index, case in enumerate(cases):
+        alone = engineRun([case], steps, 'float32').state(0)
+        mixed = together.state(index)
+        for name in ('positions', 'velocities', 'densities'):
+            assert np.array_equal(getattr(alone, name), getattr(mixed, name)), (index, name)
+
+
+def testRunsStopAtTheirOwnClocks():
+    from pysurf_amd.batch import BatchEngine
+    cases = [smallCase(seed = 5, ppd = 6), smallCase(seed = 6, ppd = 6)]
+    batch = BatchEngine(cases, 'float32')
+    batch.engine.setClock(0, 0.0, np.inf, 0, 7)
+    batch.engine.setClock(1, 0.0, np.inf, 0, 12)
+    batch.advance(20)
+    clock = batch.clocks()
+    assert list(clock['steps']) == [7, 12]
+
+
+def testSplitStepsWithTheHostTimeStepMatchTheDeviceTimeStep():
+    # residentRates, cflTimeStep on the host and residentIntegrate(dt), against the fused device step
+    from pysurf_amd.pair_sums import PairSums
+    case = smallCase(seed = 7)
+    steps = 20
+
+    def make():
+        sums = PairSums(2, case.smoothingLength, case.domainMin, case.domainMax, kernelType = case.kernelType)
+        sums.setDiffusion(case.diffusion, case.delta, case.soundSpeed, case.referenceDensity, 9.80665)
+        sums.setWallPressure(case.extrapolatedWalls)
+        sums.setContinuityCorrection(case.correction)
+        sums.setViscosityLimiter(case.limiter)
+        sums.setPhysics(case.referenceDensity, case.soundSpeed, case.alpha, case.boundaryViscosityFactor, case.gravity,
+                        0.0, case.densityFloor)
+        particles = case.particles.copy()
+        sums.push(particles)
+        sums.setBoundary(case.boundary, particles)
+        return sums, particles
+
+    split, splitParticles = make()
+    time = 0.0
+    for _ in range(steps):
+        vmax, amax = split.residentRates(False)
+        dt = oracle.cflTimeStep(case.smoothingLength, case.soundSpeed, vmax, amax, 0.25, 1.0)
+        split.residentIntegrate(dt)
+        time += dt
+    split.pull(splitParticles)
+    fused, fusedParticles = make()
+    clock = fused.advance(0.0, np.inf, 0, steps, chunk = steps)
+    fused.pull(fusedParticles)
+    assert clock['steps'] == steps and clock['time'] == time
+    assert np.array_equal(splitParticles.positions, fusedParticles.positions)
+    assert np.array_equal(splitParticles.densities, fusedParticles.densities)
+
+
+@pytest.mark.parametrize('dimensions', [2, 3])
+def testBedEnforcementMatchesTheOracle(dimensions):
+    # tests/testSphGpu.py:645-680: fluid scattered around a reef profile, a moving piston and both end walls
+    from pysurf_amd.bathymetry import Bathymetry
+    from pysurf_amd.boundaries import BedBoundary
+    from pysurf_amd.pair_sums import PairSums
+    from pysurf_amd.particles import ParticleSystem
+    rng = np.random.default_rng(7)
+    bed = Bathymetry([0.0, 2.0, 5.5, 7.5], [0.0, 0.0, 0.7, 0.7], 1.0)
+    s, width = 0.05, (0.6 if dimensions == 3 else None)
+    boundary = BedBoundary(bed, s, nLayers = 3, wallTop = 1.5, width = width,
+                           endWallTop = 1.5 if dimensions == 2 else None)
+    count = 4000
+    fluid = np.column_stack([rng.uniform(bed.startX - 0.2, bed.endX + 0.2, count)]
+                            + ([rng.uniform(-0.1, 0.7, count)] if dimensions == 3 else [])
+                            + [rng.uniform(-1.2, 1.2, count)])
+    particles = ParticleSystem.assemble(fluid, boundary.positions(), s, 1000.0, rng.normal(0.0, 1.0, (count, dimensions)))
+    boundary.attachPiston(particles)
+    boundary.movePiston(particles, bed.startX + 0.03, 0.4)
+    moved = particles.copy()
+    sums = PairSums(dimensions, 1.3 * s, particles.positions.min(axis = 0), particles.positions.max(axis = 0))
+    sums.push(moved)
+    sums.setBoundary(boundary, moved)
+    sums.enforceBoundary()
+    sums.pull(moved)
+    physics = oracle.Physics(dims = dimensions, h = 1.3 * s, domainMin = particles.positions.min(axis = 0),
+                             domainMax = particles.positions.max(axis = 0))
+    x, v = oracle.enforce(physics, boundary.description(pistonSpeed = 0.4), particles.positions, particles.velocities,
+                          particles.isFluid)
+    assert np.count_nonzero(np.any(x != particles.positions, axis = 1)) > 0.2 * count
+    assert moved.positions == pytest.approx(x, abs = 1e-12)
+    assert moved.velocities == pytest.approx(v, abs = 1e-12)

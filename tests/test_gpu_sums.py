@@ -1,0 +1,167 @@
+'''
+
+The HIP engine's host-step sums against the float64 restatement (oracle/), on the reference's
+test clouds (tests/testSphGpu.py:32-62: 600 or 900 particles 25 m from the origin, 80% fluid), in
+2D, 3D and periodic 3D, with both kernels, in both pair precisions. The float32 tolerance is the
+reference's own, 1e-5 of the largest expected value (tests/testSphGpu.py:101); the float64 kernels
+are held to 1e-10. Neighbor sets are compared bit for bit with the restatement of each precision's
+predicate.
+
+'''
+
+import numpy as np
+import pytest
+
+from clouds import alpha, c0, cloud, physicsFor, rho0, strainedFlow
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+cases = [(2, False), (3, False), (3, True)]
+kernelTypes = ['cubicSpline', 'wendlandC2']
+precisions = ['float32', 'float64']
+tolerance = {'float32': 1e-5, 'float64': 1e-10}
+
+
+def pairSums(case, precision, lengths = None):
+    from pysurf_amd.pair_sums import PairSums
+    positions = case['positions']
+    sums = PairSums(positions.shape[1], case['h'], positions.min(axis = 0), positions.max(axis = 0),
+                    case['period'] or 0.0, 'float64', 'cuda:0', case['kernelType'], pairPrecision = precision)
+    sums.update(positions, case['masses'], case['isFluid'], lengths)
+    return sums
+
+
+def gravityOf(dimensions):
+    g = np.zeros(dimensions)
+    g[-1] = -9.81
+    return g
+
+
+@pytest.mark.parametrize('precision', precisions)
+@pytest.mark.parametrize('dimensions, periodic', cases)
+@pytest.mark.parametrize('kernelType', kernelTypes)
+def testAccelerationsMatchTheOracle(precision, dimensions, periodic, kernelType):
+    print(f'accelerations {precision} {dimensions}D periodic={periodic} {kernelType}', flush = True)
+    case = cloud(dimensions, periodic, kernelType = kernelType)
+    g = gravityOf(dimensions)
+    physics = physicsFor(case, boundaryFactor = 0.3)
+    expected = oracle.accelerations(physics, case['positions'], case['velocities'], case['densities'],
+                                    case['pressures'], case['masses'], case['isFluid'])
+    result = pairSums(case, precision).pairAccelerations(case['velocities'], case['densities'], case['pressures'], c0,
+                                                         alpha, 0.3, g)
+    assert result == pytest.approx(expected, abs = tolerance[precision] * np.abs(expected).max())
+
+
+@pytest.mark.parametrize('precision', precisions)
+@pytest.mark.parametrize('dimensions, periodic', cases)
+@pytest.mark.parametrize('kernelType', kernelTypes)
+def testLimitedAccelerationsMatchTheOracle(precision, dimensions, periodic, kernelType):
+    print(f'limited accelerations {precision} {dimensions}D periodic={periodic} {kernelType}', flush = True)
+    case = cloud(dimensions, periodic, kernelType = kernelType)
+    velocities = strainedFlow(case)
+    g = gravityOf(dimensions)
+    physics = physicsFor(case, boundaryFactor = 0.3, limiter = True)
+    gradients = oracle.velocityGradients(physics, case['positions'], velocities, case['densities'], case['masses'],
+                                         case['isFluid'])
+    expected = oracle.accelerations(physics, case['positions'], velocities, case['densities'], case['pressures'],
+                                    case['masses'], case['isFluid'], gradients = gradients)
+    sums = pairSums(case, precision)
+    sums.setViscosityLimiter(True)
+    matrices = sums.velocityGradients(velocities, case['densities'])
+    result = sums.pairAccelerations(velocities, case['densities'], case['pressures'], c0, alpha, 0.3, g)
+    assert matrices == pytest.approx(gradients, abs = 2 * tolerance[precision] * np.abs(gradients).max())
+    assert result == pytest.approx(expected, abs = tolerance[precision] * np.abs(expected).max())
+
+
+@pytest.mark.parametrize('precision', precisions)
+@pytest.mark.parametrize('dimensions, periodic', cases)
+@pytest.mark.parametrize('kernelType', kernelTypes)
+@pytest.mark.parametrize('corrected', [False, True])
+@pytest.mark.parametrize('mode', ['none', 'molteni', 'hydrostatic'])
+def testContinuityMatchesTheOracle(precision, dimensions, periodic, kernelType, corrected, mode):
+    print(f'continuity {precision} {dimensions}D periodic={periodic} {kernelType} L={corrected} {mode}', flush = True)
+    case = cloud(dimensions, periodic, kernelType = kernelType)
+    physics = physicsFor(case, correction = corrected, diffusion = mode, delta = 0.1)
+    corrections = None
+    if corrected:
+        corrections = oracle.correctionMatrices(physics, case['positions'], case['densities'], case['masses'],
+                                                case['isFluid'])
+    expected = oracle.continuityRate(physics, case['positions'], case['velocities'], case['densities'], case['masses'],
+                                     case['isFluid'], corrections = corrections)
+    sums = pairSums(case, precision)
+    sums.setDiffusion(mode, 0.1, c0, rho0, 9.81)
+    sums.setContinuityCorrection(corrected)
+    if corrected:
+        matrices = sums.correctionMatrices(case['densities'])
+        assert matrices == pytest.approx(corrections, abs = 2 * tolerance[precision] * np.abs(corrections).max())
+    result = sums.continuityRate(case['velocities'], case['densities'])
+    assert result == pytest.approx(expected, abs = tolerance[precision] * np.abs(expected).max())
+
+
+@pytest.mark.parametrize('precision', precisions)
+@pytest.mark.parametrize('dimensions, periodic', cases)
+@pytest.mark.parametrize('kernelType', kernelTypes)
+def testWallPressureMatchesTheOracle(precision, dimensions, periodic, kernelType):
+    print(f'wall pressure {precision} {dimensions}D periodic={periodic} {kernelType}', flush = True)
+    case = cloud(dimensions, periodic, kernelType = kernelType)
+    g = gravityOf(dimensions)
+    expected, density = oracle.wallPressure(physicsFor(case), case['positions'], case['densities'], case['pressures'],
+                                            case['masses'], case['isFluid'])
+    sums = pairSums(case, precision)
+    result = sums.wallPressure(case['densities'], case['pressures'], g, rho0, c0)
+    assert result == pytest.approx(expected, abs = tolerance[precision] * np.abs(expected).max())
+    boundary = ~case['isFluid']
+    assert sums.wallDensities[boundary] == pytest.approx(density[boundary], rel = tolerance[precision])
+
+
+@pytest.mark.parametrize('precision', precisions)
+def testDiffusionDifferencesKeepTheirDigits(precision):
+    # tests/testSphGpu.py:284-297: a density gradient of 1e-4 kg/m^3 a spacing survives float32
+    case = cloud(2, False)
+    densities = rho0 + 1e-4 * case['positions'][:, 0] / 0.04
+    physics = physicsFor(case, diffusion = 'molteni', delta = 0.1)
+    still = np.zeros_like(case['velocities'])
+    expected = oracle.continuityRate(physics, case['positions'], still, densities, case['masses'], case['isFluid'])
+    sums = pairSums(case, precision)
+    sums.setDiffusion('molteni', 0.1, c0, rho0, 9.81)
+    result = sums.continuityRate(still, densities)
+    assert result == pytest.approx(expected, abs = 1e-3 * np.abs(expected).max())
+
+
+@pytest.mark.parametrize('precision', precisions)
+@pytest.mark.parametrize('dimensions, periodic', cases)
+def testNeighborSetsAreBitExact(precision, dimensions, periodic):
+    # Each precision's predicate, evaluated by the device and by the host restatement, selects the
+    # same neighbors; the float64 set is also the brute-force set
+    print(f'neighbors {precision} {dimensions}D periodic={periodic}', flush = True)
+    case = cloud(dimensions, periodic)
+    lengths = np.full(len(case['positions']), case['h'])
+    offsets, rows = pairSums(case, precision).neighborLists()
+    expectedOffsets, expectedRows = oracle.neighbors(physicsFor(case), case['positions'], lengths, precision)
+    assert np.array_equal(offsets, expectedOffsets)
+    assert np.array_equal(rows, expectedRows)
+
+
+@pytest.mark.parametrize('precision', precisions)
+def testMixedLengthsMatchTheOracle(precision):
+    # Particles of two smoothing lengths pair at h_ij = (h_i + h_j) / 2 (sphGpu.py:52-56)
+    case = cloud(2, False, kernelType = 'wendlandC2')
+    rng = np.random.default_rng(5)
+    lengths = case['h'] * np.where(rng.uniform(size = len(case['positions'])) < 0.3, 0.5, 1.0)
+    physics = physicsFor(case, boundaryFactor = 0.3, hmin = float(lengths.min()))
+    expected = oracle.accelerations(physics, case['positions'], case['velocities'], case['densities'],
+                                    case['pressures'], case['masses'], case['isFluid'], lengths = lengths)
+    sums = pairSums(case, precision, lengths)
+    result = sums.pairAccelerations(case['velocities'], case['densities'], case['pressures'], c0, alpha, 0.3,
+                                    gravityOf(2))
+    assert result == pytest.approx(expected, abs = tolerance[precision] * np.abs(expected).max())
+    offsets, rows = sums.neighborLists()
+    expectedOffsets, expectedRows = oracle.neighbors(physics, case['positions'], lengths, precision)
+    assert np.array_equal(rows, expectedRows) and np.array_equal(offsets, expectedOffsets)
+
+
+def testTheEngineFailsLoudlyWithoutADevicePath():
+    from pysurf_amd.pair_sums import PairSums
+    with pytest.raises(ValueError):
+        PairSums(2, 0.05, np.zeros(2), np.ones(2), device = 'cpu')

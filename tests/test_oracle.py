@@ -1,0 +1,432 @@
+'''
+
+Pins the float64 CPU restatement (oracle/) before anything is checked against it: the reference's
+own known answers, closed forms and conservation laws (tests/testSphSolver.py,
+tests/testSphKernels.py, tests/testSphNeighbors.py of the reference), brute force, and the numpy
+pair-list restatement of the reference's array functions, whose scatter structure differs from the
+C++ gathers. CPU only.
+
+'''
+
+import math
+
+import numpy as np
+import pytest
+
+from clouds import alpha, c0, cloud, physicsFor, rho0, strainedFlow
+from oracle import array_formulation as af
+from oracle import oracle
+
+cases = [(2, False), (3, False), (3, True)]
+kernelTypes = ['cubicSpline', 'wendlandC2']
+
+
+# -- Cells -- #
+def testTheCellKeyFollowsTheHashGridsOrder():
+    # tests/testSphSolver.py:818-826: x fastest, then y, then z, cells wrapped onto the grid
+    positions = np.array([[0.5, 0.5, 0.5], [1.5, 0.5, 0.5], [0.5, 1.5, 0.5], [0.5, 0.5, 1.5], [4.5, 0.5, 0.5],
+                          [-0.5, 0.5, 0.5]])
+    physics3 = oracle.Physics(dims = 3, h = 0.5, domainMin = np.zeros(3), domainMax = np.ones(3), grid = (4, 3, 2))
+    physics2 = oracle.Physics(dims = 2, h = 0.5, domainMin = np.zeros(2), domainMax = np.ones(2), grid = (4, 3, 1))
+    assert list(oracle.cellKeys(physics3, positions)) == [0, 1, 4, 12, 0, 3]
+    assert list(oracle.cellKeys(physics2, positions[:3, :2])) == [0, 1, 4]
+    assert list(af.cellOrderKey(positions, 1.0, (4, 3, 2))) == [0, 1, 4, 12, 0, 3]
+
+
+def testGridDimensionsCoverTheDomainPlusThreeCells():
+    assert af.gridDimensions([0.0, 0.0], [1.0, 0.5], 0.1, 2) == (13, 8, 1)
+    assert af.gridDimensions([0.0, 0.0, 0.0], [1.0, 0.5, 0.2], 0.1, 3) == (13, 8, 5)
+
+
+# -- Kernels -- #
+@pytest.mark.parametrize('kernelType', kernelTypes)
+@pytest.mark.parametrize('dimensions', [2, 3])
+def testKernelsIntegrateToOne(kernelType, dimensions):
+    # tests/testSphKernels.py:27-40: unit integral over the support
+    h = 0.7
+    r = np.linspace(0.0, 2.0 * h, 200001)
+    w = af.kernelValues(kernelType, r, h, dimensions)
+    shell = 2.0 * math.pi * r if dimensions == 2 else 4.0 * math.pi * r**2
+    assert np.trapezoid(w * shell, r) == pytest.approx(1.0, abs = 1e-6)
+
+
+@pytest.mark.parametrize('kernelType', kernelTypes)
+def testKernelSlopeIsTheDerivative(kernelType):
+    # tests/testSphKernels.py:55-68: dW/dr against a central difference, zero beyond 2h
+    h, step = 0.3, 1e-6
+    r = np.linspace(0.01, 0.59, 117)
+    numeric = (af.kernelValues(kernelType, r + step, h, 3) - af.kernelValues(kernelType, r - step, h, 3)) / (2 * step)
+    assert af.kernelSlopes(kernelType, r, h, 3) == pytest.approx(numeric, abs = 1e-6 * np.abs(numeric).max())
+    assert af.kernelValues(kernelType, np.array([0.6, 0.7]), h, 2) == pytest.approx([0.0, 0.0])
+
+
+# -- Equation of state and time step -- #
+def testTaitPressureAndItsInverse():
+    # tests/testSphSolver.py:61-91
+    density = np.array([990.0, 1000.0, 1010.0])
+    pressure = af.taitEquationOfState(density, rho0, c0, clampNegative = False)
+    stiffness = rho0 * c0**2 / 7.0
+    assert pressure == pytest.approx(stiffness * ((density / rho0)**7 - 1.0), rel = 1e-14)
+    assert af.taitEquationOfState(density, rho0, c0)[0] == 0.0
+    assert af.taitDensity(pressure[1:], rho0, c0) == pytest.approx(density[1:], rel = 1e-14)
+    # slope c0^2 at rho0
+    above = af.taitEquationOfState(1000.001, rho0, c0, clampNegative = False)
+    below = af.taitEquationOfState(999.999, rho0, c0, clampNegative = False)
+    assert (above - below) / 0.002 == pytest.approx(c0**2, rel = 1e-6)
+
+
+def testCflTimeStepTakesTheSmallestLimit():
+    # tests/testSphSolver.py:689-700
+    assert oracle.cflTimeStep(0.01, 20.0, 0.0, 0.0, 0.25, 1.0) == pytest.approx(0.25 * 0.01 / 20.0)
+    assert oracle.cflTimeStep(0.01, 20.0, 0.0, 1e6, 0.25, 1.0) == pytest.approx(0.25 * math.sqrt(0.01 / 1e6))
+    assert oracle.cflTimeStep(0.01, 20.0, 0.0, 0.0, 0.25, 1e-5) == 1e-5
+    assert oracle.cflTimeStep(0.01, 20.0, 5.0, 0.0, 0.25, 1.0) == af.cflTimeStep(0.01, 20.0, 5.0, 0.0, 0.25, 1.0)
+
+
+# -- Neighbors -- #
+@pytest.mark.parametrize('dimensions, periodic', cases)
+def testNeighborSetsMatchBruteForce(dimensions, periodic):
+    # tests/testSphNeighbors.py:16-22: every pair r < 2h and no other, periodic images included
+    case = cloud(dimensions, periodic)
+    physics = physicsFor(case)
+    lengths = np.full(len(case['positions']), case['h'])
+    offsets, rows = oracle.neighbors(physics, case['positions'], lengths)
+    found = {(i, int(j)) for i in range(len(lengths)) for j in rows[offsets[i]:offsets[i + 1]]}
+    assert found == af.bruteForcePairs(case['positions'], lengths, case['period'])
+
+
+def testMixedLengthsPairWithinTheMeanSupport():
+    # sphGpu.py:52-56: h_ij = (h_i + h_j) / 2, support 2 h_ij
+    case = cloud(2, False)
+    rng = np.random.default_rng(5)
+    lengths = case['h'] * np.where(rng.uniform(size = len(case['positions'])) < 0.3, 0.5, 1.0)
+    physics = physicsFor(case)
+    offsets, rows = oracle.neighbors(physics, case['positions'], lengths)
+    found = {(i, int(j)) for i in range(len(lengths)) for j in rows[offsets[i]:offsets[i + 1]]}
+    assert found == af.bruteForcePairs(case['positions'], lengths)
+
+
+def testFloat32NeighborSetDiffersOnlyAtTheSupportEdge():
+    # The float32 predicate of the fast path agrees with float64 but for pairs within rounding of r = 2h
+    case = cloud(3, False)
+    physics = physicsFor(case)
+    lengths = np.full(len(case['positions']), case['h'])
+    o64, r64 = oracle.neighbors(physics, case['positions'], lengths, 'float64')
+    o32, r32 = oracle.neighbors(physics, case['positions'], lengths, 'float32')
+    set64 = {(i, int(j)) for i in range(len(lengths)) for j in r64[o64[i]:o64[i + 1]]}
+    set32 = {(i, int(j)) for i in range(len(lengths)) for j in r32[o32[i]:o32[i + 1]]}
+    for i, j in set64 ^ set32:
+        r = np.linalg.norm(case['positions'][i] - case['positions'][j])
+        assert abs(r - 2.0 * case['h']) < 1e-6 * case['h']
+
+
+# -- Sums against the array formulation -- #
+def _gravity(dimensions):
+    g = np.zeros(dimensions)
+    g[-1] = -9.81
+    return g
+
+
+@pytest.mark.parametrize('dimensions, periodic', cases)
+@pytest.mark.parametrize('kernelType', kernelTypes)
+@pytest.mark.parametrize('limited', [False, True])
+def testAccelerationsMatchTheArrayFormulation(dimensions, periodic, kernelType, limited):
+    # tests/testSphGpu.py:75-137: boundary pairs' viscosity scaled by 0.3, limiter on a strained flow
+    case = cloud(dimensions, periodic, kernelType = kernelType)
+    i, j, fluid = case['i'], case['j'], case['isFluid']
+    velocities = strainedFlow(case) if limited else case['velocities']
+    relative = velocities[i] - velocities[j]
+    reconstructed, gradients = None, None
+    boundaryPairs = ~(fluid[i] & fluid[j])
+    if limited:
+        gradients = af.velocityGradients(case['masses'], case['densities'], velocities, i, j, case['separation'],
+                                         case['gradients'], fluid)
+        reconstructed = af.limitedVelocityDifference(relative, case['separation'], case['distance'], case['h'],
+                                                     gradients[i], gradients[j])
+        reconstructed[boundaryPairs] = relative[boundaryPairs]
+    viscosity = af.artificialViscosity(relative, case['separation'], case['distance'],
+                                       0.5 * (case['densities'][i] + case['densities'][j]), case['h'], c0, alpha,
+                                       reconstructed)
+    viscosity[boundaryPairs] *= 0.3
+    expected = af.pairAccelerations(case['masses'], case['densities'], case['pressures'], i, j, case['gradients'],
+                                    viscosity, len(fluid))
+    expected[fluid] += _gravity(dimensions)
+    expected[~fluid] = 0.0
+    physics = physicsFor(case, boundaryFactor = 0.3, limiter = limited)
+    if limited:
+        g = oracle.velocityGradients(physics, case['positions'], velocities, case['densities'], case['masses'], fluid)
+        assert g == pytest.approx(gradients, abs = 1e-10 * np.abs(gradients).max())
+    result = oracle.accelerations(physics, case['positions'], velocities, case['densities'], case['pressures'],
+                                  case['masses'], fluid, gradients = gradients)
+    assert result == pytest.approx(expected, abs = 1e-12 * np.abs(expected).max())
+
+
+@pytest.mark.parametrize('dimensions, periodic', cases)
+@pytest.mark.parametrize('kernelType', kernelTypes)
+@pytest.mark.parametrize('corrected', [False, True])
+@pytest.mark.parametrize('mode', ['none', 'molteni', 'hydrostatic'])
+def testContinuityMatchesTheArrayFormulation(dimensions, periodic, kernelType, corrected, mode):
+    # tests/testSphGpu.py:137-182: pairs with a fluid member, L_i, the diffusive term over fluid pairs
+    case = cloud(dimensions, periodic, kernelType = kernelType)
+    i, j, fluid = case['i'], case['j'], case['isFluid']
+    corrections = (af.correctionMatrices(case['masses'], case['densities'], i, j, case['separation'],
+                                         case['gradients'], fluid) if corrected else None)
+    expected = af.continuityRate(case['masses'], case['velocities'], i, j, case['gradients'], len(fluid),
+                                 fluid[i] | fluid[j], corrections)
+    if mode != 'none':
+        expected += af.densityDiffusionRate(case['masses'], case['densities'], i, j, case['separation'],
+                                            case['distance'], case['gradients'], len(fluid), case['h'], c0, 0.1, rho0,
+                                            9.81 if mode == 'hydrostatic' else None, fluid[i] & fluid[j])
+    physics = physicsFor(case, correction = corrected, diffusion = mode, delta = 0.1)
+    if corrected:
+        matrices = oracle.correctionMatrices(physics, case['positions'], case['densities'], case['masses'], fluid)
+        assert matrices == pytest.approx(corrections, abs = 1e-10 * np.abs(corrections).max())
+        assert np.sum(np.all(np.isclose(corrections, np.eye(dimensions)), axis = (1, 2)) & fluid) > 0
+    result = oracle.continuityRate(physics, case['positions'], case['velocities'], case['densities'], case['masses'],
+                                   fluid, corrections = corrections)
+    assert result == pytest.approx(expected, abs = 1e-12 * np.abs(expected).max())
+
+
+@pytest.mark.parametrize('dimensions, periodic', cases)
+@pytest.mark.parametrize('kernelType', kernelTypes)
+def testWallPressureMatchesTheArrayFormulation(dimensions, periodic, kernelType):
+    # tests/testSphGpu.py:272-282
+    case = cloud(dimensions, periodic, kernelType = kernelType)
+    g = _gravity(dimensions)
+    expected = af.extrapolatedWallPressure(case['densities'], case['pressures'], case['i'], case['j'],
+                                           case['separation'], case['values'], case['isFluid'], g)
+    pressure, density = oracle.wallPressure(physicsFor(case), case['positions'], case['densities'], case['pressures'],
+                                            case['masses'], case['isFluid'])
+    assert pressure == pytest.approx(expected, abs = 1e-12 * np.abs(expected).max())
+    boundary = ~case['isFluid']
+    assert density[boundary] == pytest.approx(af.taitDensity(expected[boundary], rho0, c0), rel = 1e-14)
+
+
+# -- Physics the formulas must satisfy -- #
+def testPairForcesConserveMomentum():
+    # tests/testSphSolver.py:126-145: a gravity-free all-fluid cloud keeps sum m a = 0 to rounding
+    case = cloud(3, False)
+    fluid = np.ones(len(case['positions']), dtype = bool)
+    physics = physicsFor(case, gravity = np.zeros(3))
+    a = oracle.accelerations(physics, case['positions'], case['velocities'], case['densities'], case['pressures'],
+                             case['masses'], fluid)
+    momentum = np.sum(case['masses'][:, None] * a, axis = 0)
+    scale = np.sum(case['masses'][:, None] * np.abs(a), axis = 0)
+    assert np.all(np.abs(momentum) < 1e-12 * scale)
+
+
+def testViscosityActsOnApproachingPairsOnly():
+    # tests/testSphSolver.py:147-155: separating pairs feel no viscosity
+    case = cloud(2, False)
+    fluid = np.ones(len(case['positions']), dtype = bool)
+    centre = case['positions'].mean(axis = 0)
+    expanding = 0.5 * (case['positions'] - centre)
+    pressures = np.zeros(len(fluid))
+    physics = physicsFor(case, gravity = np.zeros(2))
+    withViscosity = oracle.accelerations(physics, case['positions'], expanding, case['densities'], pressures,
+                                         case['masses'], fluid)
+    assert np.abs(withViscosity).max() == 0.0
+
+
+def testRigidTranslationLeavesTheDensity():
+    # tests/testSphSolver.py:274-280
+    case = cloud(2, False)
+    v = np.tile([0.7, -0.2], (len(case['positions']), 1))
+    rate = oracle.continuityRate(physicsFor(case), case['positions'], v, case['densities'], case['masses'],
+                                 case['isFluid'])
+    assert np.abs(rate).max() < 1e-9
+
+
+def _lattice(dimensions, spacing, count):
+    axes = [np.arange(count) * spacing] * dimensions
+    grids = np.meshgrid(*axes, indexing = 'ij')
+    return np.column_stack([g.ravel() for g in grids])
+
+
+@pytest.mark.parametrize('dimensions', [2, 3])
+def testVelocityGradientsAndCorrectionAreExactForLinearFields(dimensions):
+    # tests/testSphSolver.py:197-206, 401-411: G = A for v = A x wherever M inverts, a free surface included
+    s = 0.05
+    positions = _lattice(dimensions, s, 12 if dimensions == 2 else 7) + 3.0
+    positions += np.random.default_rng(2).uniform(-0.1, 0.1, positions.shape) * s
+    n = len(positions)
+    A = np.random.default_rng(3).normal(size = (dimensions, dimensions))
+    velocities = positions @ A.T
+    masses = np.full(n, rho0 * s**dimensions)
+    densities = np.full(n, rho0)
+    fluid = np.ones(n, dtype = bool)
+    physics = oracle.Physics(dims = dimensions, h = 1.3 * s, domainMin = positions.min(axis = 0),
+                             domainMax = positions.max(axis = 0), kernelType = 'wendlandC2', limiter = True)
+    G = oracle.velocityGradients(physics, positions, velocities, densities, masses, fluid)
+    applied = np.any(G != 0.0, axis = (1, 2))
+    assert applied.sum() > 0.9 * n
+    assert G[applied] == pytest.approx(np.broadcast_to(A, G[applied].shape), abs = 1e-9 * np.abs(A).max())
+    # L-corrected continuity of a linear field is -rho tr(A) sum_j V_j ... exact: sum_j m_j (v_i - v_j) . L grad W = rho tr A
+    physics.correction = True
+    L = oracle.correctionMatrices(physics, positions, densities, masses, fluid)
+    rate = oracle.continuityRate(physics, positions, velocities, densities, masses, fluid, corrections = L)
+    corrected = ~np.all(np.isclose(L, np.eye(dimensions)), axis = (1, 2))
+    assert rate[corrected] == pytest.approx(np.full(corrected.sum(), -rho0 * np.trace(A)), rel = 1e-9)
+
+
+def testDiffusionLeavesHydrostaticWaterAlone():
+    # tests/testSphSolver.py:311-371: a column at the Tait equilibrium rho0 (1 + 6 g z / c0^2)^(1/6);
+    # Molteni's term diffuses it at the free surface, the hydrostatic subtraction leaves < 1e-6 of that
+    s, surface, g = 0.01, 0.1, 9.80665
+    positions = np.array([[x, z] for z in np.arange(s / 2, surface, s) for x in np.arange(s / 2, 0.2, s)])
+    n = len(positions)
+    densities = rho0 * (1.0 + 6.0 * g * (surface - positions[:, 1]) / c0**2)**(1.0 / 6.0)
+    masses = np.full(n, rho0 * s * s)
+    fluid = np.ones(n, dtype = bool)
+    physics = oracle.Physics(dims = 2, h = 1.3 * s, domainMin = positions.min(axis = 0),
+                             domainMax = positions.max(axis = 0), kernelType = 'cubicSpline', c0 = c0,
+                             gravity = np.array([0.0, -g]), diffusion = 'hydrostatic', delta = 0.1)
+    still = np.zeros((n, 2))
+    hydrostatic = oracle.continuityRate(physics, positions, still, densities, masses, fluid)
+    physics.diffusion = 'molteni'
+    molteni = oracle.continuityRate(physics, positions, still, densities, masses, fluid)
+    assert np.abs(molteni).max() > 1e-3 * rho0
+    assert np.abs(hydrostatic).max() < 1e-6 * np.abs(molteni).max()
+
+
+def testDiffusionLeavesAUniformDensityAlone():
+    # tests/testSphSolver.py:326-336
+    s = 0.01
+    positions = np.array([[x, z] for z in np.arange(s / 2, 0.05, s) for x in np.arange(s / 2, 0.1, s)])
+    n = len(positions)
+    physics = oracle.Physics(dims = 2, h = 1.3 * s, domainMin = positions.min(axis = 0),
+                             domainMax = positions.max(axis = 0), kernelType = 'cubicSpline', c0 = c0,
+                             diffusion = 'molteni', delta = 0.1)
+    rate = oracle.continuityRate(physics, positions, np.zeros((n, 2)), np.full(n, 1003.0), np.full(n, rho0 * s * s),
+                                 np.ones(n, dtype = bool))
+    assert np.abs(rate).max() < 1e-9
+
+
+def _waterOnABed(s = 0.01, width = 0.3, depth = 0.1, g = 9.80665):
+    water = np.array([[x, z] for z in np.arange(s / 2, depth, s) for x in np.arange(s / 2, width, s)])
+    bed = np.array([[x, -(k + 0.5) * s] for k in range(3) for x in np.arange(s / 2, width, s)])
+    positions = np.vstack([water, bed])
+    fluid = np.concatenate([np.ones(len(water), bool), np.zeros(len(bed), bool)])
+    densities = rho0 * (1.0 + 6.0 * g * (depth - positions[:, 1]) / c0**2)**(1.0 / 6.0)
+    return positions, fluid, densities
+
+
+def testWallPressureIsHydrostaticAtRest():
+    # tests/testSphSolver.py:663-675: each bed particle within reach of the water takes the pressure
+    # water at rest holds at its depth; beyond reach zero; the fluid's pressures come back as given
+    s, g = 0.01, 9.80665
+    positions, fluid, densities = _waterOnABed()
+    pressures = af.taitEquationOfState(densities, rho0, c0)
+    masses = np.full(len(positions), rho0 * s * s)
+    physics = oracle.Physics(dims = 2, h = 1.3 * s, domainMin = positions.min(axis = 0),
+                             domainMax = positions.max(axis = 0), kernelType = 'cubicSpline', c0 = c0,
+                             gravity = np.array([0.0, -g]))
+    result, _ = oracle.wallPressure(physics, positions, densities, pressures, masses, fluid)
+    h = 1.3 * s
+    inner = ~fluid & (positions[:, 1] > -2.0 * s) & (positions[:, 0] > 2.0 * h) & (positions[:, 0] < 0.3 - 2.0 * h)
+    assert result[inner] == pytest.approx(pressures[inner], rel = 2e-3)
+    assert np.all(result[~fluid & (positions[:, 1] < -2.0 * s)] == 0.0)
+    assert np.array_equal(result[fluid], pressures[fluid])
+
+
+def testWallPressureFollowsTheWaterAlongTheBed():
+    # tests/testSphSolver.py:677-687: without gravity a pressure linear along the water reaches the bed exactly
+    s = 0.01
+    positions, fluid, densities = _waterOnABed()
+    pressures = np.where(fluid, 1000.0 + 5000.0 * positions[:, 0], 0.0)
+    masses = np.full(len(positions), rho0 * s * s)
+    physics = oracle.Physics(dims = 2, h = 1.3 * s, domainMin = positions.min(axis = 0),
+                             domainMax = positions.max(axis = 0), kernelType = 'cubicSpline', c0 = c0,
+                             gravity = np.zeros(2))
+    result, _ = oracle.wallPressure(physics, positions, densities, pressures, masses, fluid)
+    h = 1.3 * s
+    inner = ~fluid & (positions[:, 1] > -2.0 * s) & (positions[:, 0] > 2.0 * h) & (positions[:, 0] < 0.3 - 2.0 * h)
+    assert result[inner] == pytest.approx(1000.0 + 5000.0 * positions[inner, 0], rel = 1e-9)
+
+
+# -- Boundary enforcement -- #
+def _hostBedEnforce(description, positions, velocities, fluid, dims):
+    '''BedBoundary.enforce as the reference's host code writes it (sphBoundaries.py:295-344), in numpy.'''
+    x, v = positions.copy(), velocities.copy()
+    bedX, bedZ, clearance = description['bedX'], description['bedZ'], description['clearance']
+    idx = np.flatnonzero(fluid)
+    bz = np.interp(x[idx, 0], bedX, bedZ)
+    under = idx[x[idx, -1] < bz + clearance]
+    if under.size:
+        x[under, -1] = np.interp(x[under, 0], bedX, bedZ) + clearance
+        slopes = np.diff(bedZ) / np.diff(bedX)
+        k = np.clip(np.searchsorted(bedX, x[under, 0], side = 'right') - 1, 0, slopes.size - 1)
+        length = np.sqrt(1.0 + slopes[k]**2)
+        normal = np.stack([-slopes[k] / length, 1.0 / length], axis = -1)
+        inward = v[under, 0] * normal[:, 0] + v[under, -1] * normal[:, 1]
+        into = inward < 0.0
+        v[under[into], 0] -= inward[into] * normal[into, 0]
+        v[under[into], -1] -= inward[into] * normal[into, 1]
+    if description['sideWalls']:
+        low = idx[x[idx, 1] < clearance]
+        v[low, 1] = np.maximum(v[low, 1], 0.0)
+        x[low, 1] = clearance
+        high = idx[x[idx, 1] > description['width'] - clearance]
+        v[high, 1] = np.minimum(v[high, 1], 0.0)
+        x[high, 1] = description['width'] - clearance
+    if description['offshoreWall']:
+        behind = idx[x[idx, 0] < description['wallX'] + clearance]
+        x[behind, 0] = description['wallX'] + clearance
+        v[behind, 0] = np.maximum(v[behind, 0], description['pistonSpeed'])
+    beyond = idx[x[idx, 0] > description['endLimit']]
+    x[beyond, 0] = description['endLimit']
+    v[beyond, 0] = np.minimum(v[beyond, 0], 0.0)
+    return x, v
+
+
+@pytest.mark.parametrize('dimensions', [2, 3])
+def testBedEnforcementMatchesTheHostFormula(dimensions):
+    # tests/testSphGpu.py:645-680: a reef profile, a moving piston and both end walls, to 1e-12
+    rng = np.random.default_rng(7)
+    bedX = np.array([0.0, 2.0, 5.5, 7.5])
+    bedZ = np.array([0.0, 0.0, 0.7, 0.7])
+    description = {'kind': 'bed', 'clearance': 0.0125, 'bedX': bedX, 'bedZ': bedZ, 'sideWalls': dimensions == 3,
+                   'width': 0.6, 'offshoreWall': True, 'wallX': 0.03, 'pistonSpeed': 0.4, 'endLimit': 7.5 - 0.0125}
+    count = 4000
+    positions = np.column_stack([rng.uniform(-0.2, 7.7, count)] + ([rng.uniform(-0.1, 0.7, count)] if dimensions == 3
+                                                                   else []) + [rng.uniform(-1.2, 1.2, count)])
+    velocities = rng.normal(0.0, 1.0, (count, dimensions))
+    fluid = rng.uniform(size = count) < 0.9
+    physics = oracle.Physics(dims = dimensions, h = 0.05, domainMin = positions.min(axis = 0),
+                             domainMax = positions.max(axis = 0))
+    x, v = oracle.enforce(physics, description, positions, velocities, fluid)
+    ex, ev = _hostBedEnforce(description, positions, velocities, fluid, dimensions)
+    assert np.count_nonzero(np.any(ex != positions, axis = 1)) > 0.2 * count
+    assert x == pytest.approx(ex, abs = 1e-12)
+    assert v == pytest.approx(ev, abs = 1e-12)
+
+
+# -- Whole steps -- #
+def testADeltaSphStillTankStaysAtRest():
+    # tests/testSphGpu.py:557-575: the delta-SPH scheme's still tank, 0.2 s, max speed under 1% of
+    # sqrt(g d_fill), no particle moving a quarter spacing
+    s, width, fill, g = 0.005, 0.2, 0.075, 9.81
+    xs = np.arange(s / 2, width, s)
+    zs = np.arange(s / 2, fill, s)
+    water = np.array([[x, z] for z in zs for x in xs])
+    offsets = (np.arange(3) + 0.5) * s
+    walls = [np.array([[x, -o] for x in np.concatenate([-offsets[::-1], xs, width + offsets]) for o in offsets])]
+    for side in (-offsets, width + offsets):
+        walls.append(np.array([[x, z] for x in side for z in np.arange(s / 2, 0.15, s)]))
+    bed = np.vstack(walls)
+    positions = np.vstack([water, bed])
+    fluid = np.concatenate([np.ones(len(water), bool), np.zeros(len(bed), bool)])
+    c = 10.0 * math.sqrt(g * 0.15)
+    densities = af.taitDensity(rho0 * g * np.maximum(fill - positions[:, 1], 0.0), rho0, c)
+    physics = oracle.Physics(dims = 2, h = 1.3 * s, domainMin = np.array([-4 * s, -4 * s]),
+                             domainMax = np.array([width + 4 * s, 0.15]), kernelType = 'cubicSpline', c0 = c,
+                             alpha = 0.1, gravity = np.array([0.0, -g]), diffusion = 'hydrostatic', delta = 0.1,
+                             extrapolatedWalls = True, correction = True, densityFloor = 0.9 * rho0,
+                             maxTimeStep = 1e-3)
+    boundary = {'kind': 'container', 'clearance': 0.0, 'lower': [0.0, 0.0], 'upper': [width, 0.15], 'openTop': True}
+    run = oracle.OracleRun(physics, boundary, positions, np.zeros_like(positions), densities,
+                           np.full(len(positions), rho0 * s * s), fluid)
+    run.advance(0.2, 100000)
+    speed = np.linalg.norm(run.velocities[fluid], axis = 1).max()
+    assert speed < 0.01 * math.sqrt(g * fill)
+    assert np.abs(run.positions[fluid] - positions[fluid]).max() < 0.25 * s
