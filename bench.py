@@ -49,6 +49,7 @@ def kernelBytes(dimensions: int, real: int) -> dict:
         'sweepAccelerations': index + 1 + d * r + lo + r + 4 + d * r + 3 * r + d * d * r + d * r + d * d * r,
         'kick': index + 1 + d * r + 2 * d * 8 + d * r,
         'sweepContinuity': index + 1 + d * r + lo + r + 4 + d * r + r + d * d * r + 2 * 8 + 4 * d * 8,
+        'buildNeighbors': index + d * r + lo + r + 4 + 4,
     }
 
 
@@ -165,9 +166,11 @@ def main() -> int:
     print(f'# float32: {msPerStep:.4f} ms/step, {value:.4e} fluid updates/s', file = sys.stderr, flush = True)
 
     profile = kernelProfile(batch, args.profile_steps)
-    dominant = max((name for name in profile if profile[name]['launches']), key = lambda name: profile[name]['ms'])
+    table = kernelBytes(case.dimensions, 4)
+    dominant = max((name for name in profile if profile[name]['launches'] and name in table),
+                   key = lambda name: profile[name]['ms'])
     launchMs = profile[dominant]['ms'] / profile[dominant]['launches']
-    perParticle = kernelBytes(case.dimensions, 4)[dominant]
+    perParticle = table[dominant]
     traffic = perParticle * case.count
     achieved = traffic / (launchMs / 1000.0) / 1e9
     stepMs = sum(entry['ms'] for entry in profile.values()) / args.profile_steps

@@ -152,12 +152,16 @@ int sph_synchronize(sph_context* ctx);
  * synchronizes and copies, per kernel id, the accumulated milliseconds and launch counts:
  * 0 clock, 1 cell keys, 2 cell-count scan, 3 cell scatter, 4 cell order, 5 record gather,
  * 6 sweep 1 (wall pressure, G), 7 sweep 2 (accelerations, L), 8 kick, 9 sweep 3 (continuity,
- * density advance, drift, enforcement). */
+ * density advance, drift, enforcement), 10 neighbor list. */
 int sph_profile(sph_context* ctx, int32_t enable);
 int sph_kernel_times(sph_context* ctx, double* milliseconds, int64_t* launches, int32_t count);
 
-/* Launch shape of the pair sweeps: threads per workgroup (64 to 1024, a multiple of 64). */
-int sph_set_launch(sph_context* ctx, int32_t blockSize);
+/* Launch shape of the pair sweeps: threads per workgroup (64, 128, 192 or 256), lanes per
+ * particle (1, 2, 4, 8 or 16; 0 picks 8 in 2D, 16 in 3D), the lanes of a particle sharing its
+ * neighbors, and the neighbor list's entries per particle (0 walks the cells in every sweep; -1
+ * picks 64 in 2D, 160 in 3D; a particle with more neighbors walks its cells). The summation order
+ * depends on the lanes, so runs compared bit for bit are stepped with the same lanes. */
+int sph_set_launch(sph_context* ctx, int32_t blockSize, int32_t lanes, int32_t listCapacity);
 
 #ifdef __cplusplus
 }

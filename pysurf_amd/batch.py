@@ -67,10 +67,15 @@ class BatchEngine:
         'cuda:N'; None for cuda:0
     blockSize : int
         Threads per workgroup of the sweeps
+    lanes : int
+        Lanes per particle of the sweeps; 0 picks by dimension
+    listCapacity : int
+        Neighbor-list entries per particle; 0 walks the cells in every sweep, -1 picks by dimension
 
     '''
 
-    def __init__(self, cases: list, precision: str = 'float32', device = None, blockSize: int = 256) -> None:
+    def __init__(self, cases: list, precision: str = 'float32', device = None, blockSize: int = 256,
+                 lanes: int = 0, listCapacity: int = -1) -> None:
         if not cases:
             raise ValueError('A batch needs at least one run.')
         dimensions = {case.dimensions for case in cases}
@@ -92,7 +97,7 @@ class BatchEngine:
             first += case.count
         self.offsets.append(first)
         self.engine.configure(runs)
-        self.engine.setLaunch(blockSize)
+        self.engine.setLaunch(blockSize, lanes, listCapacity)
         self.load()
 
     @property

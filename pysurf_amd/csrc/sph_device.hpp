@@ -48,6 +48,8 @@ struct Records {
     Real* vel[3];
     Real* rho;       // density copy
     Real* press;     // pressure
+    Real* pr2;       // p / rho^2
+    Real* vol;       // m / rho
     Real* dev;       // rho - rho0, taken in float64 (sphGpu.py:1031-1038)
     Real* mass;
     Real* len;       // smoothing length
@@ -107,50 +109,6 @@ __device__ __forceinline__ void forStencil(const RunDev& run, const int32_t* __r
     }
 }
 
-// The smoothing kernels (sphGpu.py:139-186): W and dW/dr / r at smoothing length h.
-template <typename Real, int D>
-struct Smoothing {
-    int kind;
-    Real unit;      // sigma at h = 1
-
-    __device__ __forceinline__ Real sigma(Real h) const {
-        return D == 2 ? unit / (h * h) : unit / (h * h * h);
-    }
-    __device__ __forceinline__ Real value(Real r, Real h) const {
-        const Real q = r / h, s = sigma(h);
-        if (kind == 1) {
-            if (q < Real(2)) {
-                const Real rest = Real(1) - Real(0.5) * q, sq = rest * rest;
-                return s * sq * sq * (Real(2) * q + Real(1));
-            }
-            return Real(0);
-        }
-        if (q < Real(1)) return s * (Real(1) - Real(1.5) * q * q + Real(0.75) * q * q * q);
-        if (q < Real(2)) {
-            const Real rest = Real(2) - q;
-            return s * Real(0.25) * rest * rest * rest;
-        }
-        return Real(0);
-    }
-    // dW/dr divided by r: the factor grad_i W_ij = factor * (r_i - r_j) takes.
-    __device__ __forceinline__ Real slopeOverR(Real r, Real h) const {
-        const Real q = r / h, s = sigma(h);
-        if (kind == 1) {
-            if (q < Real(2)) {
-                const Real rest = Real(1) - Real(0.5) * q;
-                return Real(-5) * s * rest * rest * rest / (h * h);
-            }
-            return Real(0);
-        }
-        if (q < Real(1)) return s * (Real(-3) + Real(2.25) * q) / (h * h);
-        if (q < Real(2)) {
-            const Real rest = Real(2) - q;
-            return s * (Real(-0.75) * rest * rest) / (h * r);
-        }
-        return Real(0);
-    }
-};
-
 template <int D, typename Real>
 struct Self {
     Real p[3];
@@ -159,31 +117,101 @@ struct Self {
     Real reach;
 };
 
-// sqrt(fma(dz, dz, fma(dy, dy, dx * dx))), the neighbor predicate's distance.
-__device__ __forceinline__ float pairDistance(const float* d, int dims) {
-    float r2 = d[0] * d[0];
-    r2 = fmaf(d[1], d[1], r2);
-    if (dims == 3) r2 = fmaf(d[2], d[2], r2);
+// sqrt(fma(dz, dz, fma(dy, dy, dx * dx))), the neighbor predicate's distance, correctly rounded.
+__device__ __forceinline__ float pairDistance(float dx, float dy, float dz, int dims) {
+    float r2 = dx * dx;
+    r2 = fmaf(dy, dy, r2);
+    if (dims == 3) r2 = fmaf(dz, dz, r2);
     return sqrtf(r2);
 }
-__device__ __forceinline__ double pairDistance(const double* d, int dims) {
-    double r2 = d[0] * d[0];
-    r2 = fma(d[1], d[1], r2);
-    if (dims == 3) r2 = fma(d[2], d[2], r2);
+__device__ __forceinline__ double pairDistance(double dx, double dy, double dz, int dims) {
+    double r2 = dx * dx;
+    r2 = fma(dy, dy, r2);
+    if (dims == 3) r2 = fma(dz, dz, r2);
     return sqrt(r2);
 }
 
+// a / b: the fast reciprocal-based quotient in float32, the exact one in float64.
+__device__ __forceinline__ float quotient(float a, float b) { return __fdividef(a, b); }
+__device__ __forceinline__ double quotient(double a, double b) { return a / b; }
+
+// The smoothing kernels (sphGpu.py:139-186) at one run's constants: W and dW/dr / r, with a fast
+// path for runs whose particles share one smoothing length, where every pair is at h = h_max.
+template <typename Real, int D>
+struct PairKernel {
+    int kind;
+    bool uniform;
+    Real unit;       // sigma at h = 1
+    Real h;          // h_max
+    Real twoH;       // the support at h_max
+    Real invH;
+    Real sigma;      // sigma at h_max
+    Real slopeC;     // Wendland: -5 sigma / h^2; cubic: sigma / h^2
+    Real soft;       // 0.01 h^2
+
+    __device__ __forceinline__ void init(int kernelKind, double sigmaUnit, double hmax, bool uniformLengths) {
+        kind = kernelKind;
+        uniform = uniformLengths;
+        unit = (Real)sigmaUnit;
+        h = (Real)hmax;
+        twoH = Real(2) * h;
+        invH = Real(1) / h;
+        sigma = D == 2 ? unit / (h * h) : unit / (h * h * h);
+        slopeC = (kind == 1 ? Real(-5) : Real(1)) * sigma / (h * h);
+        soft = Real(0.01) * h * h;
+    }
+    __device__ __forceinline__ Real sigmaAt(Real hij) const {
+        return D == 2 ? unit / (hij * hij) : unit / (hij * hij * hij);
+    }
+    __device__ __forceinline__ Real value(Real r, Real hij) const {
+        Real q, s;
+        if (uniform) { q = r * invH; s = sigma; } else { q = r / hij; s = sigmaAt(hij); }
+        if (kind == 1) {
+            const Real rest = Real(1) - Real(0.5) * q, sq = rest * rest;
+            return q < Real(2) ? s * sq * sq * (Real(2) * q + Real(1)) : Real(0);
+        }
+        if (q < Real(1)) return s * (Real(1) - Real(1.5) * q * q + Real(0.75) * q * q * q);
+        if (q < Real(2)) {
+            const Real rest = Real(2) - q;
+            return s * Real(0.25) * rest * rest * rest;
+        }
+        return Real(0);
+    }
+    // dW/dr divided by r: grad_i W_ij = slopeOverR * (r_i - r_j).
+    __device__ __forceinline__ Real slopeOverR(Real r, Real hij) const {
+        Real q, c, inv;
+        if (uniform) { q = r * invH; c = slopeC; inv = invH; } else {
+            inv = Real(1) / hij;
+            q = r * inv;
+            c = (kind == 1 ? Real(-5) : Real(1)) * sigmaAt(hij) * inv * inv;
+        }
+        if (kind == 1) {
+            const Real rest = Real(1) - Real(0.5) * q;
+            return q < Real(2) ? c * rest * rest * rest : Real(0);
+        }
+        if (q < Real(1)) return c * (Real(-3) + Real(2.25) * q);
+        if (q < Real(2)) {
+            const Real rest = Real(2) - q;
+            return c * Real(-0.75) * rest * rest * quotient(Real(1), q);
+        }
+        return Real(0);
+    }
+};
+
 // Walks every neighbor j != i of sorted particle s with r < 2 h_ij (sphGpu.py:352-363), calling
-// f(k, d, r, h) with k the neighbor's sorted position, d = r_i - r_j at the nearest periodic
-// image, r = |d| and h = h_ij = (h_i + h_j) / 2. With `skipCoincident` pairs closer than 1e-12 are
-// skipped too, as every sum but the wall pressure skips them. `y64` is the particle's float64 y,
-// read only for a search from its periodic image.
-template <int D, typename Real, bool SkipCoincident, class F>
+// f(k, dx, dy, dz, r, h) with k the neighbor's sorted position, d = r_i - r_j at the nearest
+// periodic image, r = |d| and h = h_ij = (h_i + h_j) / 2. Each cell range's candidates are dealt to
+// `Lanes` lanes in turn, lane `lane` taking the range's candidates lane, lane + Lanes, ..., so a
+// group of lanes shares one particle's walk. With `skipCoincident` pairs closer than 1e-12 are skipped too, as every sum but
+// the wall pressure skips them. `y64` is the particle's float64 y, read only for a search from
+// its periodic image.
+template <int D, typename Real, int Lanes, bool SkipCoincident, class F>
 __device__ __forceinline__ void forNeighbors(const RunDev& run, const int32_t* __restrict__ start,
                                              const Records<Real>& rec, int64_t s, const Self<D, Real>& me,
-                                             int cx, int cy, int cz, double y64, F&& f) {
+                                             int cx, int cy, int cz, double y64, int lane, bool uniform, F&& f) {
     constexpr bool single = sizeof(Real) == 4;
     const Real period = (Real)run.period;
+    const Real twoHmax = Real(2) * (Real)run.hmax;
     int images = 1;
     if (run.period > 0.0 && (me.p[1] < me.reach || me.p[1] > period - me.reach)) images = 2;
     for (int image = 0; image < images; ++image) {
@@ -194,27 +222,111 @@ __device__ __forceinline__ void forNeighbors(const RunDev& run, const int32_t* _
             oy = wrapIndex(cellCoordinate(y64 + (double)shift, run.cell), run.ny);
         }
         forStencil<D>(run, start, cx, oy, cz, [&](int32_t begin, int32_t end) {
-            for (int32_t k = begin; k < end; ++k) {
+            for (int32_t k = begin + lane; k < end; k += Lanes) {
                 if (k == s) continue;
-                Real d[3];
+                Real dx, dy, dz = Real(0);
                 if constexpr (single) {
-                    d[0] = (me.p[0] - rec.pos[0][k]) + (me.l[0] - rec.lo[0][k]);
-                    d[1] = (me.p[1] - rec.pos[1][k]) + shift + (me.l[1] - rec.lo[1][k]);
-                    if (D == 3) d[2] = (me.p[2] - rec.pos[2][k]) + (me.l[2] - rec.lo[2][k]);
+                    dx = (me.p[0] - rec.pos[0][k]) + (me.l[0] - rec.lo[0][k]);
+                    dy = (me.p[1] - rec.pos[1][k]) + shift + (me.l[1] - rec.lo[1][k]);
+                    if (D == 3) dz = (me.p[2] - rec.pos[2][k]) + (me.l[2] - rec.lo[2][k]);
                 } else {
-                    d[0] = me.p[0] - rec.pos[0][k];
-                    d[1] = (me.p[1] + shift) - rec.pos[1][k];
-                    if (D == 3) d[2] = me.p[2] - rec.pos[2][k];
+                    dx = me.p[0] - rec.pos[0][k];
+                    dy = (me.p[1] + shift) - rec.pos[1][k];
+                    if (D == 3) dz = me.p[2] - rec.pos[2][k];
                 }
-                if (D == 2) d[2] = Real(0);
-                const Real r = pairDistance(d, D);
-                const Real h = Real(0.5) * (me.h + rec.len[k]);
-                if (!(r < Real(2) * h)) continue;
+                const Real r = pairDistance(dx, dy, dz, D);
+                Real h, support;
+                if (uniform) {
+                    h = (Real)run.hmax;
+                    support = twoHmax;
+                } else {
+                    h = Real(0.5) * (me.h + rec.len[k]);
+                    support = Real(2) * h;
+                }
+                if (!(r < support)) continue;
                 if (SkipCoincident && r < Real(1.0e-12)) continue;
-                f(k, d, r, h);
+                f(k, dx, dy, dz, r, h);
             }
         });
     }
+}
+
+// Each sorted particle's neighbors for one step, column-major: entry m of particle s at
+// index[m * stride + s], the neighbor's sorted position with bit 31 set when it is reached through
+// the periodic image. count[s] is the particle's whole count, which may exceed the capacity; such
+// a particle's sums walk its cells instead.
+struct NeighborList {
+    int32_t* index;
+    int32_t* count;
+    int64_t stride;
+    int32_t capacity;
+};
+
+constexpr int32_t kImageBit = (int32_t)0x80000000u;
+
+// Calls f(k, dx, dy, dz, r, h) for the neighbors of sorted particle s that lane `lane` of its
+// group takes: neighbor m, in the order forNeighbors visits them, goes to lane m % Lanes. From the
+// list when the particle's neighbors fit it, from its cells otherwise; the same pairs reach the
+// same lanes in the same order either way.
+template <int D, typename Real, int Lanes, bool SkipCoincident, class F>
+__device__ __forceinline__ void forEachNeighbor(const RunDev& run, const int32_t* __restrict__ start,
+                                                const Records<Real>& rec, const NeighborList& list, int64_t s,
+                                                const Self<D, Real>& me, int cx, int cy, int cz, double y64, int lane,
+                                                bool uniform, F&& f) {
+    const int32_t count = list.capacity > 0 ? list.count[s] : -1;
+    if (count >= 0 && count <= list.capacity) {
+        constexpr bool single = sizeof(Real) == 4;
+        const Real period = (Real)run.period;
+        const Real shiftImage = me.p[1] < Real(0.5) * period ? period : -period;
+        for (int32_t m = lane; m < count; m += Lanes) {
+            const int32_t entry = list.index[(int64_t)m * list.stride + s];
+            const int32_t k = entry & 0x7fffffff;
+            const Real shift = entry < 0 ? shiftImage : Real(0);
+            Real dx, dy, dz = Real(0);
+            if constexpr (single) {
+                dx = (me.p[0] - rec.pos[0][k]) + (me.l[0] - rec.lo[0][k]);
+                dy = (me.p[1] - rec.pos[1][k]) + shift + (me.l[1] - rec.lo[1][k]);
+                if (D == 3) dz = (me.p[2] - rec.pos[2][k]) + (me.l[2] - rec.lo[2][k]);
+            } else {
+                dx = me.p[0] - rec.pos[0][k];
+                dy = (me.p[1] + shift) - rec.pos[1][k];
+                if (D == 3) dz = me.p[2] - rec.pos[2][k];
+            }
+            const Real r = pairDistance(dx, dy, dz, D);
+            if (SkipCoincident && r < Real(1.0e-12)) continue;
+            const Real h = uniform ? (Real)run.hmax : Real(0.5) * (me.h + rec.len[k]);
+            f(k, dx, dy, dz, r, h);
+        }
+        return;
+    }
+    int32_t m = 0;
+    forNeighbors<D, Real, 1, false>(run, start, rec, s, me, cx, cy, cz, y64, 0, uniform,
+                                    [&](int32_t k, Real dx, Real dy, Real dz, Real r, Real h) {
+                                        const bool mine = (m % Lanes) == lane;
+                                        ++m;
+                                        if (!mine || (SkipCoincident && r < Real(1.0e-12))) return;
+                                        f(k, dx, dy, dz, r, h);
+                                    });
+}
+
+// The workgroup's place in the launch with the blocks of each XCD made contiguous: an MI355X deals
+// workgroups to its 8 XCDs in turn, each XCD with its own L2, while neighboring blocks of sorted
+// particles read the same neighbor records. Renumbering so XCD x runs the x-th contiguous slice
+// keeps those shared reads in one L2.
+__device__ __forceinline__ int64_t xcdBlock() {
+    constexpr unsigned kXcds = 8;
+    const unsigned blocks = gridDim.x, b = blockIdx.x;
+    const unsigned xcd = b % kXcds, index = b / kXcds;
+    const unsigned per = blocks / kXcds, extra = blocks % kXcds;
+    return (int64_t)xcd * per + (xcd < extra ? xcd : extra) + index;
+}
+
+// The sum of v over a group of `Lanes` consecutive lanes; every lane of the wave must call it.
+template <int Lanes, typename T>
+__device__ __forceinline__ T groupSum(T v) {
+#pragma unroll
+    for (int o = Lanes / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
 }
 
 }  // namespace sph

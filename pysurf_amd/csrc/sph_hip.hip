@@ -142,6 +142,15 @@ __device__ __forceinline__ double runTimeStep(const RunDev& run, const Clock& cl
     return fmin(dt, run.maxTimeStep);
 }
 
+// max(*target, bits), as an atomic only when bits exceed the value already there: once a run's
+// maximum is reached the waves that follow read it and leave, rather than queueing atomics on one
+// address. A stale read can only cost an atomic that changes nothing.
+__device__ __forceinline__ void raiseMax(unsigned long long* target, unsigned long long bits) {
+    if (bits == 0ull) return;
+    const unsigned long long seen = __hip_atomic_load(target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (bits > seen) atomicMax(target, bits);
+}
+
 // The largest value of a wave's lanes of one run, folded into maxima[slot] with one atomic when
 // the wave holds one run, lane by lane otherwise. Every lane of the wave must call it.
 __device__ __forceinline__ void waveMaxInto(unsigned long long* maxima, int run, bool valid, double value) {
@@ -154,9 +163,9 @@ __device__ __forceinline__ void waveMaxInto(unsigned long long* maxima, int run,
             const unsigned long long other = __shfl_xor(bits, o, 64);
             bits = other > bits ? other : bits;
         }
-        if ((threadIdx.x & 63) == 0 && bits != 0ull) atomicMax(maxima + 2 * first, bits);
-    } else if (valid && bits != 0ull) {
-        atomicMax(maxima + 2 * run, bits);
+        if ((threadIdx.x & 63) == 0) raiseMax(maxima + 2 * first, bits);
+    } else if (valid) {
+        raiseMax(maxima + 2 * run, bits);
     }
 }
 
@@ -164,13 +173,22 @@ __device__ __forceinline__ void waveMaxInto(unsigned long long* maxima, int run,
 // Cell list
 // ---------------------------------------------------------------------------------------------
 
+// Each slot's cell and the cells' counts. With `clockMode` the first slot of each run also opens the
+// run's step: 1 sets its active flag from its clock, 2 makes it active whatever the clock; both
+// clear its maxima.
 template <int D>
 __global__ void cellKeys(const RunDev* __restrict__ runs, const int32_t* __restrict__ runOf, const double* __restrict__ x,
                          const double* __restrict__ y, const double* __restrict__ z, int64_t n, int32_t* __restrict__ key,
-                         int32_t* __restrict__ count) {
+                         int32_t* __restrict__ count, Clock clock, int32_t clockMode) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const RunDev& run = runs[runOf[i]];
+    const int r = runOf[i];
+    const RunDev& run = runs[r];
+    if (clockMode && i == run.first) {
+        clock.active[r] = clockMode == 2 || (clock.time[r] < clock.until[r] && clock.steps[r] < clock.maxSteps[r]);
+        clock.maxima[2 * r] = 0ull;
+        clock.maxima[2 * r + 1] = 0ull;
+    }
     const int cx = wrapIndex(cellCoordinate(x[i], run.cell), run.nx);
     const int cy = wrapIndex(cellCoordinate(y[i], run.cell), run.ny);
     const int cz = D == 3 ? wrapIndex(cellCoordinate(z[i], run.cell), run.nz) : 0;
@@ -270,7 +288,10 @@ __global__ void gatherRecords(const RunDev* __restrict__ runs, const int32_t* __
             if (src.pressureOut) src.pressureOut[i] = p;
         }
         rec.press[s] = p;
-        rec.mass[s] = src.mass[i];
+        const Real rhoR = (Real)rho, massR = src.mass[i];
+        rec.pr2[s] = p / (rhoR * rhoR);
+        rec.vol[s] = massR / rhoR;
+        rec.mass[s] = massR;
         rec.len[s] = src.len[i];
         rec.fluid[s] = isFluid ? 1 : 0;
         rec.key[s] = src.key[i];
@@ -304,8 +325,8 @@ __device__ __forceinline__ Self<D, Real> loadSelf(const RunDev& run, const Recor
     return me;
 }
 
-// In-place inverse of a D x D matrix given its determinant (2D: the reference's third axis carries
-// a one on the diagonal, sphGpu.py:636-641, which leaves the 2 x 2 block's inverse).
+// Determinant and inverse of the leading D x D block (2D: the reference's third axis carries a one
+// on the diagonal, sphGpu.py:636-641, which leaves the 2 x 2 block's).
 template <int D, typename Real>
 __device__ __forceinline__ Real determinant(const Real (&m)[3][3]) {
     if (D == 2) return m[0][0] * m[1][1] - m[0][1] * m[1][0];
@@ -352,72 +373,192 @@ struct SweepOut {
     int64_t stride;
 };
 
-// Sweep 1: wall pressure on boundary lanes, G on fluid lanes.
+// Sweep 1's sums: the extrapolated wall pressure of a boundary particle (sphGpu.py:804-854, no
+// coincident skip) or the velocity gradient of a fluid particle (sphGpu.py:643-697), each over
+// fluid neighbors. One accumulator serves the standalone sweep and the one fused with the list
+// build, so both add the same pairs in the same order.
 template <int D, typename Real>
-__global__ void sweepWallAndGradients(const RunDev* __restrict__ runs, const int32_t* __restrict__ runOf,
-                                      const int32_t* __restrict__ order, const int32_t* __restrict__ start,
-                                      Records<Real> rec, SweepOut<Real> out, Clock clock, const double* __restrict__ y64,
-                                      int32_t resident, int32_t doWall, int32_t doGradients, int64_t n) {
-    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    const int r = runOf[s];
-    if (resident && !clock.active[r]) return;
-    const RunDev& run = runs[r];
-    const bool fluidI = rec.fluid[s] != 0;
-    const bool wall = !fluidI && doWall && (!resident || run.extrapolatedWalls);
-    const bool gradients = fluidI && doGradients && (!resident || run.limiter);
-    if (!wall && !gradients) {
-        if (!fluidI && doWall && out.pressure) out.pressure[order[s]] = rec.press[s];
-        if (fluidI && doWall && out.pressure && !resident) out.pressure[order[s]] = rec.press[s];
-        if (doGradients && out.gradients && !fluidI) {
-            const int32_t i = order[s];
-            for (int e = 0; e < D * D; ++e) out.gradients[e * out.stride + i] = Real(0);
+struct Sweep1 {
+    static constexpr int K = 2 * D * D;   // wall: weight, total; gradients: rises then offsets
+    Real acc[K];
+    int mode;                             // 0 none, 1 wall pressure, 2 gradients
+    Real vi[3];
+    Real g[3];
+
+    __device__ __forceinline__ void begin(const RunDev& run, const Records<Real>& rec, int64_t s, int chosen) {
+#pragma unroll
+        for (int e = 0; e < K; ++e) acc[e] = Real(0);
+        mode = chosen;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            vi[a] = (mode == 2 && a < D) ? rec.vel[a][s] : Real(0);
+            g[a] = (Real)run.gravity[a];
         }
+    }
+    __device__ __forceinline__ void add(const Records<Real>& rec, const PairKernel<Real, D>& kernel, int32_t k, Real dx,
+                                        Real dy, Real dz, Real rr, Real h) {
+        if (mode == 0 || !rec.fluid[k]) return;
+        if (mode == 1) {
+            const Real w = kernel.value(rr, h);
+            Real gd = g[0] * dx + g[1] * dy;
+            if (D == 3) gd += g[2] * dz;
+            acc[0] += w;
+            acc[1] += w * (rec.press[k] + rec.rho[k] * gd);
+            return;
+        }
+        if (rr < Real(1.0e-12)) return;
+        const Real w = rec.vol[k] * kernel.slopeOverR(rr, h);
+        const Real d[3] = {dx, dy, dz};
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            const Real dv = w * (rec.vel[a][k] - vi[a]);
+            const Real da = w * d[a];
+#pragma unroll
+            for (int b = 0; b < D; ++b) {
+                acc[a * D + b] += dv * d[b];
+                acc[D * D + a * D + b] -= da * d[b];
+            }
+        }
+    }
+};
+
+template <typename Real>
+struct SweepOut;
+
+// The mode of sweep 1 for sorted particle s.
+template <typename Real>
+__device__ __forceinline__ int sweep1Mode(const RunDev& run, bool fluidI, int32_t resident, int32_t doWall,
+                                          int32_t doGradients) {
+    if (!fluidI && doWall && (!resident || run.extrapolatedWalls)) return 1;
+    if (fluidI && doGradients && (!resident || run.limiter)) return 2;
+    return 0;
+}
+
+template <int D, typename Real>
+__device__ __forceinline__ void sweep1Finish(const RunDev& run, const Sweep1<D, Real>& sweep, Records<Real>& rec,
+                                             const SweepOut<Real>& out, int64_t s, int32_t i, bool fluidI,
+                                             int32_t resident, int32_t doWall, int32_t doGradients);
+
+// The step's neighbor list: each particle's neighbors {j != i : r_ij < h_i + h_j} in the order the
+// cell walk visits them. A group of `Lanes` lanes tests a particle's candidates `Lanes` at a time
+// and compacts the hits with a ballot, so the list keeps the walk's order. With `WithSweep1` the
+// same walk accumulates sweep 1's sums, each lane over the candidates it tested.
+template <int D, typename Real, int Lanes, bool WithSweep1>
+__global__ void __launch_bounds__(256)
+    buildNeighbors(const RunDev* __restrict__ runs, const int32_t* __restrict__ runOf,
+                   const int32_t* __restrict__ order, const int32_t* __restrict__ start, Records<Real> rec,
+                   NeighborList list, SweepOut<Real> out, Clock clock, const double* __restrict__ y64,
+                   int32_t resident, int64_t n) {
+    const int64_t t = xcdBlock() * blockDim.x + threadIdx.x;
+    const int64_t s = t / Lanes;
+    const int lane = (int)(t % Lanes);
+    const int groupBase = (int)(threadIdx.x & 63) & ~(Lanes - 1);
+    bool valid = s < n;
+    int r = 0;
+    if (valid) {
+        r = runOf[s];
+        valid = !resident || clock.active[r];
+    }
+    Sweep1<D, Real> sweep;
+    sweep.mode = 0;
+    bool fluidI = false;
+    if (valid) {
+        const RunDev& run = runs[r];
+        int cx, cy, cz;
+        const Self<D, Real> me = loadSelf<D, Real>(run, rec, s, cx, cy, cz);
+        const bool uniform = run.uniform != 0;
+        PairKernel<Real, D> kernel;
+        kernel.init(run.kernel, run.sigmaUnit, run.hmax, uniform);
+        fluidI = rec.fluid[s] != 0;
+        sweep.begin(run, rec, s, WithSweep1 ? sweep1Mode<Real>(run, fluidI, resident, 1, 1) : 0);
+        const double yy = run.period > 0.0 ? y64[order[s]] : 0.0;
+        constexpr bool single = sizeof(Real) == 4;
+        const Real period = (Real)run.period;
+        const Real twoHmax = Real(2) * (Real)run.hmax;
+        const uint32_t groupMask = Lanes >= 32 ? 0xffffffffu : ((1u << Lanes) - 1u);
+        int images = 1;
+        if (run.period > 0.0 && (me.p[1] < me.reach || me.p[1] > period - me.reach)) images = 2;
+        int32_t count = 0;
+        for (int image = 0; image < images; ++image) {
+            Real shift = Real(0);
+            int oy = cy;
+            if (image == 1) {
+                shift = me.p[1] < Real(0.5) * period ? period : -period;
+                oy = wrapIndex(cellCoordinate(yy + (double)shift, run.cell), run.ny);
+            }
+            const int32_t tag = image ? kImageBit : 0;
+            forStencil<D>(run, start, cx, oy, cz, [&](int32_t begin, int32_t end) {
+                for (int32_t base = begin; base < end; base += Lanes) {
+                    const int32_t k = base + lane;
+                    bool hit = false;
+                    Real dx = Real(0), dy = Real(0), dz = Real(0), rr = Real(0), h = Real(0);
+                    if (k < end && k != s) {
+                        if constexpr (single) {
+                            dx = (me.p[0] - rec.pos[0][k]) + (me.l[0] - rec.lo[0][k]);
+                            dy = (me.p[1] - rec.pos[1][k]) + shift + (me.l[1] - rec.lo[1][k]);
+                            if (D == 3) dz = (me.p[2] - rec.pos[2][k]) + (me.l[2] - rec.lo[2][k]);
+                        } else {
+                            dx = me.p[0] - rec.pos[0][k];
+                            dy = (me.p[1] + shift) - rec.pos[1][k];
+                            if (D == 3) dz = me.p[2] - rec.pos[2][k];
+                        }
+                        rr = pairDistance(dx, dy, dz, D);
+                        h = uniform ? (Real)run.hmax : Real(0.5) * (me.h + rec.len[k]);
+                        hit = rr < (uniform ? twoHmax : Real(2) * h);
+                    }
+                    const uint32_t mine = (uint32_t)(__ballot(hit) >> groupBase) & groupMask;
+                    if (hit) {
+                        const int32_t slot = count + __popc(mine & ((1u << lane) - 1u));
+                        if (slot < list.capacity) list.index[(int64_t)slot * list.stride + s] = k | tag;
+                        if (WithSweep1) sweep.add(rec, kernel, k, dx, dy, dz, rr, h);
+                    }
+                    count += __popc(mine);
+                }
+            });
+        }
+        if (lane == 0) list.count[s] = count;
+    }
+    if (!WithSweep1) return;
+#pragma unroll
+    for (int e = 0; e < Sweep1<D, Real>::K; ++e) sweep.acc[e] = groupSum<Lanes>(sweep.acc[e]);
+    if (!valid || lane != 0) return;
+    sweep1Finish<D, Real>(runs[r], sweep, rec, out, s, order[s], fluidI, resident, 1, 1);
+}
+
+
+// Sweep 1's results for sorted particle s: a boundary particle's pressure and the density that
+// carries it, or a fluid particle's G; writes the pass-through outputs of a particle with neither.
+template <int D, typename Real>
+__device__ __forceinline__ void sweep1Finish(const RunDev& run, const Sweep1<D, Real>& sweep, Records<Real>& rec,
+                                             const SweepOut<Real>& out, int64_t s, int32_t i, bool fluidI,
+                                             int32_t resident, int32_t doWall, int32_t doGradients) {
+    if (sweep.mode == 0) {
+        if (doWall && out.pressure && (!fluidI || !resident)) out.pressure[i] = rec.press[s];
+        if (doGradients && out.gradients && !fluidI)
+            for (int e = 0; e < D * D; ++e) out.gradients[e * out.stride + i] = Real(0);
         return;
     }
-    int cx, cy, cz;
-    const Self<D, Real> me = loadSelf<D, Real>(run, rec, s, cx, cy, cz);
-    const Smoothing<Real, D> kernel{run.kernel, (Real)run.sigmaUnit};
-    const int32_t i = order[s];
-    const double yy = run.period > 0.0 ? y64[i] : 0.0;
-    if (wall) {
-        const Real g[3] = {(Real)run.gravity[0], (Real)run.gravity[1], (Real)run.gravity[2]};
-        Real weight = Real(0), total = Real(0);
-        forNeighbors<D, Real, false>(run, start, rec, s, me, cx, cy, cz, yy,
-                                     [&](int32_t k, const Real* d, Real r, Real h) {
-                                         if (!rec.fluid[k]) return;
-                                         const Real w = kernel.value(r, h);
-                                         Real gd = g[0] * d[0] + g[1] * d[1];
-                                         if (D == 3) gd += g[2] * d[2];
-                                         weight += w;
-                                         total += w * (rec.press[k] + rec.rho[k] * gd);
-                                     });
+    if (sweep.mode == 1) {
         Real pressure = Real(0);
-        if (weight > Real(0)) pressure = fmax(total / weight, Real(0));
+        if (sweep.acc[0] > Real(0)) pressure = fmax(sweep.acc[1] / sweep.acc[0], Real(0));
         const double rho = run.rho0 * pow(1.0 + (double)pressure / run.stiffness, 1.0 / 7.0);
+        const Real rhoR = (Real)rho;
         rec.press[s] = pressure;
-        rec.rho[s] = (Real)rho;
+        rec.rho[s] = rhoR;
+        rec.pr2[s] = pressure / (rhoR * rhoR);
+        rec.vol[s] = rec.mass[s] / rhoR;
         if (out.pressure) out.pressure[i] = pressure;
         if (out.density) out.density[i] = rho;
         return;
     }
-    Real vi[3];
+    Real offsets[3][3] = {}, rises[3][3] = {};
 #pragma unroll
-    for (int a = 0; a < D; ++a) vi[a] = rec.vel[a][s];
-    Real rises[3][3] = {}, offsets[3][3] = {};
-    forNeighbors<D, Real, true>(run, start, rec, s, me, cx, cy, cz, yy, [&](int32_t k, const Real* d, Real r, Real h) {
-        if (!rec.fluid[k]) return;
-        const Real w = rec.mass[k] / rec.rho[k] * kernel.slopeOverR(r, h);
+    for (int a = 0; a < D; ++a)
 #pragma unroll
-        for (int a = 0; a < D; ++a) {
-            const Real dv = rec.vel[a][k] - vi[a];
-#pragma unroll
-            for (int b = 0; b < D; ++b) {
-                rises[a][b] += w * dv * d[b];
-                offsets[a][b] -= w * d[a] * d[b];
-            }
+        for (int b = 0; b < D; ++b) {
+            rises[a][b] = sweep.acc[a * D + b];
+            offsets[a][b] = sweep.acc[D * D + a * D + b];
         }
-    });
     Real G[3][3] = {};
     const Real det = determinant<D, Real>(offsets);
     if (det >= (Real)correctionFloorPower<D>()) {
@@ -442,25 +583,165 @@ __global__ void sweepWallAndGradients(const RunDev* __restrict__ runs, const int
         }
 }
 
-// Sweep 2: accelerations, and L when asked, on fluid lanes.
-template <int D, typename Real>
-__global__ void sweepAccelerations(const RunDev* __restrict__ runs, const int32_t* __restrict__ runOf,
-                                   const int32_t* __restrict__ order, const int32_t* __restrict__ start,
-                                   Records<Real> rec, SweepOut<Real> out, Clock clock, const double* __restrict__ y64,
-                                   int32_t resident, int32_t doAcceleration, int32_t doCorrection,
-                                   int32_t hostLimiter, int64_t n) {
-    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool inRange = s < n;
-    const int r = inRange ? runOf[s] : 0;
-    const bool activeRun = inRange && (!resident || clock.active[r]);
-    bool fluidI = false;
-    double magnitude = 0.0;
+// Sweep 1 on its own: wall pressure on boundary particles, G on fluid particles, over the cells.
+template <int D, typename Real, int Lanes>
+__global__ void __launch_bounds__(256)
+    sweepWallAndGradients(const RunDev* __restrict__ runs, const int32_t* __restrict__ runOf,
+                          const int32_t* __restrict__ order, const int32_t* __restrict__ start, Records<Real> rec,
+                          NeighborList list, SweepOut<Real> out, Clock clock, const double* __restrict__ y64,
+                          int32_t resident, int32_t doWall, int32_t doGradients, int64_t n) {
+    const int64_t t = xcdBlock() * blockDim.x + threadIdx.x;
+    const int64_t s = t / Lanes;
+    const int lane = (int)(t % Lanes);
+    Sweep1<D, Real> sweep;
+    sweep.mode = 0;
+    bool valid = s < n, fluidI = false;
+    int r = 0;
+    if (valid) {
+        r = runOf[s];
+        valid = !resident || clock.active[r];
+    }
+    if (valid) {
+        const RunDev& run = runs[r];
+        fluidI = rec.fluid[s] != 0;
+        sweep.begin(run, rec, s, sweep1Mode<Real>(run, fluidI, resident, doWall, doGradients));
+        if (sweep.mode) {
+            int cx, cy, cz;
+            const Self<D, Real> me = loadSelf<D, Real>(run, rec, s, cx, cy, cz);
+            PairKernel<Real, D> kernel;
+            kernel.init(run.kernel, run.sigmaUnit, run.hmax, run.uniform != 0);
+            const double yy = run.period > 0.0 ? y64[order[s]] : 0.0;
+            forNeighbors<D, Real, Lanes, false>(run, start, rec, s, me, cx, cy, cz, yy, lane, kernel.uniform,
+                                                [&](int32_t k, Real dx, Real dy, Real dz, Real rr, Real h) {
+                                                    sweep.add(rec, kernel, k, dx, dy, dz, rr, h);
+                                                });
+        }
+    }
+    (void)list;
+#pragma unroll
+    for (int e = 0; e < Sweep1<D, Real>::K; ++e) sweep.acc[e] = groupSum<Lanes>(sweep.acc[e]);
+    if (!valid || lane != 0) return;
+    sweep1Finish<D, Real>(runs[r], sweep, rec, out, s, order[s], fluidI, resident, doWall, doGradients);
+}
+
+// Sweep 2: accelerations with the limited viscosity (sphGpu.py:303-414) and, when asked, the
+// correction matrix L (sphGpu.py:593-641) on fluid particles.
+template <int D, typename Real, int Lanes>
+__global__ void __launch_bounds__(256)
+    sweepAccelerations(const RunDev* __restrict__ runs, const int32_t* __restrict__ runOf,
+                       const int32_t* __restrict__ order, const int32_t* __restrict__ start, Records<Real> rec,
+                       NeighborList list, SweepOut<Real> out, Clock clock, const double* __restrict__ y64, int32_t resident,
+                       int32_t doAcceleration, int32_t doCorrection, int32_t hostLimiter, int64_t n) {
+    const int64_t t = xcdBlock() * blockDim.x + threadIdx.x;
+    const int64_t s = t / Lanes;
+    const int lane = (int)(t % Lanes);
+    constexpr int K = D + D * D;          // acceleration, then M
+    Real acc[K];
+#pragma unroll
+    for (int e = 0; e < K; ++e) acc[e] = Real(0);
+    bool activeRun = false, fluidI = false, correction = false;
+    int r = 0;
+    if (s < n) {
+        r = runOf[s];
+        activeRun = !resident || clock.active[r];
+    }
     if (activeRun) {
         const RunDev& run = runs[r];
-        const int32_t i = order[s];
         fluidI = rec.fluid[s] != 0;
-        const bool correction = doCorrection && (!resident || run.correction);
+        correction = doCorrection && (!resident || run.correction);
         const bool limiter = resident ? run.limiter != 0 : hostLimiter != 0;
+        if (fluidI) {
+            int cx, cy, cz;
+            const Self<D, Real> me = loadSelf<D, Real>(run, rec, s, cx, cy, cz);
+            PairKernel<Real, D> kernel;
+            kernel.init(run.kernel, run.sigmaUnit, run.hmax, run.uniform != 0);
+            const double yy = run.period > 0.0 ? y64[order[s]] : 0.0;
+            Real vi[3] = {};
+#pragma unroll
+            for (int a = 0; a < D; ++a) vi[a] = rec.vel[a][s];
+            const Real rhoI = rec.rho[s];
+            const Real own = rec.pr2[s];
+            Real Gi[3][3] = {};
+            if (limiter && doAcceleration)
+#pragma unroll
+                for (int a = 0; a < D; ++a)
+#pragma unroll
+                    for (int b = 0; b < D; ++b) Gi[a][b] = rec.G[(a * D + b) * rec.stride + s];
+            const Real alphaC0 = (Real)(run.alpha * run.c0), bf = (Real)run.boundaryFactor;
+            forEachNeighbor<D, Real, Lanes, true>(
+                run, start, rec, list, s, me, cx, cy, cz, yy, lane, kernel.uniform,
+                [&](int32_t k, Real dx, Real dy, Real dz, Real rr, Real h) {
+                    const Real F = kernel.slopeOverR(rr, h);
+                    const Real d[3] = {dx, dy, dz};
+                    if (correction) {
+                        const Real w = -rec.vol[k] * F;
+#pragma unroll
+                        for (int a = 0; a < D; ++a) {
+                            const Real wa = w * d[a];
+#pragma unroll
+                            for (int b = 0; b < D; ++b) acc[D + a * D + b] += wa * d[b];
+                        }
+                    }
+                    if (!doAcceleration) return;
+                    const bool fluidJ = rec.fluid[k] != 0;
+                    Real dv[3] = {};
+                    Real approach = Real(0);
+#pragma unroll
+                    for (int a = 0; a < D; ++a) {
+                        dv[a] = vi[a] - rec.vel[a][k];
+                        approach += dv[a] * d[a];
+                    }
+                    const Real rhoJ = rec.rho[k];
+                    Real viscosity = Real(0);
+                    if (approach < Real(0) && limiter && fluidJ) {
+                        Real rampI[3] = {}, rampJ[3] = {};
+                        Real slopeI = Real(0), slopeJ = Real(0);
+#pragma unroll
+                        for (int a = 0; a < D; ++a) {
+                            Real ti = Real(0), tj = Real(0);
+#pragma unroll
+                            for (int b = 0; b < D; ++b) {
+                                ti += Gi[a][b] * d[b];
+                                tj += rec.G[(a * D + b) * rec.stride + k] * d[b];
+                            }
+                            rampI[a] = ti;
+                            rampJ[a] = tj;
+                            slopeI += d[a] * ti;
+                            slopeJ += d[a] * tj;
+                        }
+                        Real limit = Real(0);
+                        if (slopeI * slopeJ > Real(0)) {
+                            const Real ratio = quotient(slopeI, slopeJ);
+                            limit = fmin(Real(1), quotient(Real(4) * ratio, (Real(1) + ratio) * (Real(1) + ratio)));
+                            const Real eta = kernel.uniform ? rr * kernel.invH : quotient(rr, h);
+                            if (eta < Real(0.3)) {
+                                const Real fade = (eta - Real(0.3)) * Real(5);
+                                limit = limit * exp(-fade * fade);
+                            }
+                        }
+                        approach = Real(0);
+#pragma unroll
+                        for (int a = 0; a < D; ++a)
+                            approach += (dv[a] - Real(0.5) * limit * (rampI[a] + rampJ[a])) * d[a];
+                    }
+                    if (approach < Real(0)) {
+                        const Real soft = kernel.uniform ? kernel.soft : Real(0.01) * h * h;
+                        const Real mu = quotient(h * approach, rr * rr + soft);
+                        viscosity = quotient(-alphaC0 * mu, Real(0.5) * (rhoI + rhoJ));
+                        if (!fluidJ) viscosity *= bf;
+                    }
+                    const Real coefficient = -(own + rec.pr2[k] + viscosity) * rec.mass[k] * F;
+#pragma unroll
+                    for (int a = 0; a < D; ++a) acc[a] += coefficient * d[a];
+                });
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < K; ++e) acc[e] = groupSum<Lanes>(acc[e]);
+    double magnitude = 0.0;
+    if (activeRun && lane == 0) {
+        const RunDev& run = runs[r];
+        const int32_t i = order[s];
         if (!fluidI) {
             if (doAcceleration)
 #pragma unroll
@@ -475,95 +756,22 @@ __global__ void sweepAccelerations(const RunDev* __restrict__ runs, const int32_
                         if (out.corrections) out.corrections[(a * D + b) * out.stride + i] = e;
                     }
         } else {
-            int cx, cy, cz;
-            const Self<D, Real> me = loadSelf<D, Real>(run, rec, s, cx, cy, cz);
-            const Smoothing<Real, D> kernel{run.kernel, (Real)run.sigmaUnit};
-            const double yy = run.period > 0.0 ? y64[i] : 0.0;
-            Real vi[3];
-#pragma unroll
-            for (int a = 0; a < D; ++a) vi[a] = rec.vel[a][s];
-            const Real rhoI = rec.rho[s];
-            const Real own = rec.press[s] / (rhoI * rhoI);
-            Real Gi[3][3] = {};
-            if (limiter && doAcceleration)
-#pragma unroll
-                for (int a = 0; a < D; ++a)
-#pragma unroll
-                    for (int b = 0; b < D; ++b) Gi[a][b] = rec.G[(a * D + b) * rec.stride + s];
-            const Real alpha = (Real)run.alpha, c0 = (Real)run.c0, bf = (Real)run.boundaryFactor;
-            Real total[3] = {};
-            Real M[3][3] = {};
-            forNeighbors<D, Real, true>(run, start, rec, s, me, cx, cy, cz, yy, [&](int32_t k, const Real* d, Real rr, Real h) {
-                const Real rhoJ = rec.rho[k];
-                const Real F = kernel.slopeOverR(rr, h);
-                if (correction) {
-                    const Real w = -rec.mass[k] / rhoJ * F;
-#pragma unroll
-                    for (int a = 0; a < D; ++a)
-#pragma unroll
-                        for (int b = 0; b < D; ++b) M[a][b] += w * d[a] * d[b];
-                }
-                if (!doAcceleration) return;
-                const bool fluidJ = rec.fluid[k] != 0;
-                Real dv[3];
-                Real approach = Real(0);
-#pragma unroll
-                for (int a = 0; a < D; ++a) {
-                    dv[a] = vi[a] - rec.vel[a][k];
-                    approach += dv[a] * d[a];
-                }
-                Real viscosity = Real(0);
-                if (approach < Real(0) && limiter && fluidJ) {
-                    Real rampI[3], rampJ[3];
-                    Real slopeI = Real(0), slopeJ = Real(0);
-#pragma unroll
-                    for (int a = 0; a < D; ++a) {
-                        Real ti = Real(0), tj = Real(0);
-#pragma unroll
-                        for (int b = 0; b < D; ++b) {
-                            ti += Gi[a][b] * d[b];
-                            tj += rec.G[(a * D + b) * rec.stride + k] * d[b];
-                        }
-                        rampI[a] = ti;
-                        rampJ[a] = tj;
-                        slopeI += d[a] * ti;
-                        slopeJ += d[a] * tj;
-                    }
-                    Real limit = Real(0);
-                    if (slopeI * slopeJ > Real(0)) {
-                        const Real ratio = slopeI / slopeJ;
-                        limit = fmin(Real(1), Real(4) * ratio / ((Real(1) + ratio) * (Real(1) + ratio)));
-                        const Real eta = rr / h;
-                        if (eta < Real(0.3)) {
-                            const Real fade = (eta - Real(0.3)) / Real(0.2);
-                            limit = limit * exp(-fade * fade);
-                        }
-                    }
-                    approach = Real(0);
-#pragma unroll
-                    for (int a = 0; a < D; ++a) approach += (dv[a] - Real(0.5) * limit * (rampI[a] + rampJ[a])) * d[a];
-                }
-                if (approach < Real(0)) {
-                    const Real mu = h * approach / (rr * rr + Real(0.01) * h * h);
-                    viscosity = -alpha * c0 * mu / (Real(0.5) * (rhoI + rhoJ));
-                    if (!fluidJ) viscosity *= bf;
-                }
-                const Real coefficient = -(own + rec.press[k] / (rhoJ * rhoJ) + viscosity) * rec.mass[k] * F;
-#pragma unroll
-                for (int a = 0; a < D; ++a) total[a] += coefficient * d[a];
-            });
             if (doAcceleration) {
                 Real a2 = Real(0);
 #pragma unroll
                 for (int a = 0; a < D; ++a) {
-                    const Real v = total[a] + (Real)run.gravity[a];
+                    const Real v = acc[a] + (Real)run.gravity[a];
                     out.acceleration[a * out.stride + i] = v;
                     a2 += v * v;
                 }
                 magnitude = (double)sqrt(a2);
             }
             if (correction) {
-                Real Lm[3][3] = {};
+                Real M[3][3] = {}, Lm[3][3] = {};
+#pragma unroll
+                for (int a = 0; a < D; ++a)
+#pragma unroll
+                    for (int b = 0; b < D; ++b) M[a][b] = acc[D + a * D + b];
                 const Real det = determinant<D, Real>(M);
                 if (det >= (Real)correctionFloorPower<D>()) {
                     inverse<D, Real>(M, det, Lm);
@@ -581,7 +789,7 @@ __global__ void sweepAccelerations(const RunDev* __restrict__ runs, const int32_
             }
         }
     }
-    if (resident && doAcceleration) waveMaxInto(clock.maxima + 1, r, activeRun && fluidI, magnitude);
+    if (resident && doAcceleration) waveMaxInto(clock.maxima + 1, r, activeRun && fluidI && lane == 0, magnitude);
 }
 
 // Clears what a neighbor-free run would leave stale and applies v += a dt on the fluid, refreshing
@@ -692,71 +900,91 @@ __device__ __forceinline__ void enforceBoundary(const sph_boundary& b, double* x
     }
 }
 
-// Sweep 3: continuity with the delta-SPH term; resident, the particle's density advance, drift and
-// boundary enforcement follow on its float64 state.
-template <int D, typename Real>
-__global__ void sweepContinuity(const RunDev* __restrict__ runs, const int32_t* __restrict__ runOf,
-                                const int32_t* __restrict__ order, const int32_t* __restrict__ start, Records<Real> rec,
-                                SweepOut<Real> out, Clock clock, int32_t resident, int32_t hostCorrection,
-                                int32_t hostDiffusion, int64_t n) {
-    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    const int r = runOf[s];
-    if (resident && !clock.active[r]) return;
+// Sweep 3: continuity with the delta-SPH term (sphGpu.py:416-545); resident, the particle's
+// density advance with its floor (:1111-1140), drift (:1142-1186) and boundary enforcement
+// follow on its float64 state.
+template <int D, typename Real, int Lanes>
+__global__ void __launch_bounds__(256)
+    sweepContinuity(const RunDev* __restrict__ runs, const int32_t* __restrict__ runOf,
+                    const int32_t* __restrict__ order, const int32_t* __restrict__ start, Records<Real> rec,
+                    NeighborList list, SweepOut<Real> out, Clock clock, int32_t resident, int32_t hostCorrection,
+                    int32_t hostDiffusion, int64_t n) {
+    const int64_t t = xcdBlock() * blockDim.x + threadIdx.x;
+    const int64_t s = t / Lanes;
+    const int lane = (int)(t % Lanes);
+    Real total = Real(0);
+    bool activeRun = false;
+    int r = 0;
+    if (s < n) {
+        r = runOf[s];
+        activeRun = !resident || clock.active[r];
+    }
+    if (activeRun) {
+        const RunDev& run = runs[r];
+        const bool boundaryI = rec.fluid[s] == 0;
+        const int diffusion = resident ? run.diffusion : hostDiffusion;
+        const bool correction = (resident ? run.correction != 0 : hostCorrection != 0) && !boundaryI;
+        const bool diffuses = diffusion != 0 && !boundaryI;
+        int cx, cy, cz;
+        const Self<D, Real> me = loadSelf<D, Real>(run, rec, s, cx, cy, cz);
+        PairKernel<Real, D> kernel;
+        kernel.init(run.kernel, run.sigmaUnit, run.hmax, run.uniform != 0);
+        const double yy = run.period > 0.0 ? out.pos[1][order[s]] : 0.0;
+        Real vi[3] = {};
+#pragma unroll
+        for (int a = 0; a < D; ++a) vi[a] = rec.vel[a][s];
+        Real Li[3][3] = {};
+        if (correction) {
+#pragma unroll
+            for (int a = 0; a < D; ++a)
+#pragma unroll
+                for (int b = 0; b < D; ++b) Li[a][b] = rec.L[(a * D + b) * rec.stride + s];
+        }
+        const Real devI = rec.dev[s];
+        const Real factor = (Real)(2.0 * run.delta * run.c0);
+        const Real rho0 = (Real)run.rho0, invRho0 = (Real)(1.0 / run.rho0);
+        const Real head = (Real)(run.rho0 * fabs(run.gravity[D - 1]) / (run.c0 * run.c0));
+        forEachNeighbor<D, Real, Lanes, true>(
+            run, start, rec, list, s, me, cx, cy, cz, yy, lane, kernel.uniform,
+            [&](int32_t k, Real dx, Real dy, Real dz, Real rr, Real h) {
+                const bool fluidJ = rec.fluid[k] != 0;
+                if (boundaryI && !fluidJ) return;
+                const Real F = kernel.slopeOverR(rr, h);
+                const Real d[3] = {dx, dy, dz};
+                Real dot = Real(0);
+                if (correction) {
+#pragma unroll
+                    for (int a = 0; a < D; ++a) {
+                        Real ld = Real(0);
+#pragma unroll
+                        for (int b = 0; b < D; ++b) ld += Li[a][b] * d[b];
+                        dot += (vi[a] - rec.vel[a][k]) * ld;
+                    }
+                } else {
+#pragma unroll
+                    for (int a = 0; a < D; ++a) dot += (vi[a] - rec.vel[a][k]) * d[a];
+                }
+                total += rec.mass[k] * dot * F;
+                if (diffuses && fluidJ) {
+                    const Real devJ = rec.dev[k];
+                    Real difference = devJ - devI;
+                    if (diffusion == 2) {
+                        const Real ratio = Real(1) + Real(0.5) * (devI + devJ) * invRho0;
+                        const Real sq = ratio * ratio;
+                        difference -= quotient(head * d[D - 1], sq * sq * ratio);
+                    }
+                    const Real soft = kernel.uniform ? kernel.soft : Real(0.01) * h * h;
+                    const Real r2 = rr * rr;
+                    total += factor * h * rec.vol[k] * difference * quotient(-F * r2, r2 + soft);
+                }
+            });
+        (void)rho0;
+    }
+    total = groupSum<Lanes>(total);
+    if (!activeRun || lane != 0) return;
     const RunDev& run = runs[r];
     const int32_t i = order[s];
     const bool boundaryI = rec.fluid[s] == 0;
-    const int diffusion = resident ? run.diffusion : hostDiffusion;
-    const bool correction = resident ? run.correction != 0 : hostCorrection != 0;
-    const bool diffuses = diffusion != 0 && !boundaryI;
-    int cx, cy, cz;
-    const Self<D, Real> me = loadSelf<D, Real>(run, rec, s, cx, cy, cz);
-    const Smoothing<Real, D> kernel{run.kernel, (Real)run.sigmaUnit};
-    const double yy = run.period > 0.0 ? out.pos[1][i] : 0.0;
-    Real vi[3];
-#pragma unroll
-    for (int a = 0; a < D; ++a) vi[a] = rec.vel[a][s];
-    Real Li[3][3] = {};
-    if (correction) {
-#pragma unroll
-        for (int a = 0; a < D; ++a)
-#pragma unroll
-            for (int b = 0; b < D; ++b) Li[a][b] = rec.L[(a * D + b) * rec.stride + s];
-    }
-    const Real devI = rec.dev[s];
-    const Real factor = (Real)(2.0 * run.delta * run.c0);
-    const Real rho0 = (Real)run.rho0;
-    const Real gOverC2 = (Real)(fabs(run.gravity[D - 1]) / (run.c0 * run.c0));
-    Real total = Real(0);
-    forNeighbors<D, Real, true>(run, start, rec, s, me, cx, cy, cz, yy, [&](int32_t k, const Real* d, Real rr, Real h) {
-        const bool fluidJ = rec.fluid[k] != 0;
-        if (boundaryI && !fluidJ) return;
-        const Real F = kernel.slopeOverR(rr, h);
-        Real dot = Real(0);
-        if (correction && !boundaryI) {
-#pragma unroll
-            for (int a = 0; a < D; ++a) {
-                Real ld = Real(0);
-#pragma unroll
-                for (int b = 0; b < D; ++b) ld += Li[a][b] * d[b];
-                dot += (vi[a] - rec.vel[a][k]) * ld;
-            }
-        } else {
-#pragma unroll
-            for (int a = 0; a < D; ++a) dot += (vi[a] - rec.vel[a][k]) * d[a];
-        }
-        total += rec.mass[k] * dot * F;
-        if (diffuses && fluidJ) {
-            const Real devJ = rec.dev[k];
-            Real difference = devJ - devI;
-            if (diffusion == 2) {
-                const Real ratio = Real(1) + Real(0.5) * (devI + devJ) / rho0;
-                const Real sq = ratio * ratio;
-                difference -= rho0 * gOverC2 * d[D - 1] / (sq * sq * ratio);
-            }
-            total += factor * h * rec.mass[k] / rec.rho[k] * difference * (-F * rr * rr) / (rr * rr + Real(0.01) * h * h);
-        }
-    });
     if (!resident) {
         out.rate[i] = total;
         return;
@@ -796,12 +1024,13 @@ __global__ void enforceKernel(const RunDev* __restrict__ runs, const int32_t* __
     for (int a = 0; a < D; ++a) { P[a][i] = x[a]; V[a][i] = v[a]; }
 }
 
-// Neighbor counts (rows null) or rows, each slot's neighbors in visit order, as slots.
+// Neighbor counts (rows null) or rows as slots, in walk order: from the list where the particle's
+// neighbors fit it, from its cells otherwise.
 template <int D, typename Real>
-__global__ void neighborList(const RunDev* __restrict__ runs, const int32_t* __restrict__ runOf,
-                             const int32_t* __restrict__ order, const int32_t* __restrict__ start, Records<Real> rec,
-                             const double* __restrict__ y64, int32_t* counts, const int64_t* offsets, int32_t* rows,
-                             int64_t n) {
+__global__ void __launch_bounds__(256)
+    neighborExport(const RunDev* __restrict__ runs, const int32_t* __restrict__ runOf, const int32_t* __restrict__ order,
+                   const int32_t* __restrict__ start, Records<Real> rec, NeighborList list,
+                   const double* __restrict__ y64, int32_t* counts, const int64_t* offsets, int32_t* rows, int64_t n) {
     const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const RunDev& run = runs[runOf[s]];
@@ -810,15 +1039,26 @@ __global__ void neighborList(const RunDev* __restrict__ runs, const int32_t* __r
     const int32_t i = order[s];
     const double yy = run.period > 0.0 ? y64[i] : 0.0;
     int32_t count = 0;
-    int64_t base = rows ? offsets[i] : 0;
-    forNeighbors<D, Real, false>(run, start, rec, s, me, cx, cy, cz, yy, [&](int32_t k, const Real*, Real, Real) {
-        if (rows) rows[base + count] = order[k];
-        ++count;
-    });
+    const int64_t base = rows ? offsets[i] : 0;
+    forEachNeighbor<D, Real, 1, false>(run, start, rec, list, s, me, cx, cy, cz, yy, 0, run.uniform != 0,
+                                       [&](int32_t k, Real, Real, Real, Real, Real) {
+                                           if (rows) rows[base + count] = order[k];
+                                           ++count;
+                                       });
     if (counts) counts[i] = count;
 }
 
 inline unsigned gridFor(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
+
+// Runs BODY with the compile-time constant L set to the context's lanes per particle.
+#define WITH_LANES(ctx, BODY)                            \
+    switch (lanesOf(ctx)) {                              \
+        case 1: { constexpr int L = 1; BODY; } break;    \
+        case 2: { constexpr int L = 2; BODY; } break;    \
+        case 4: { constexpr int L = 4; BODY; } break;    \
+        case 8: { constexpr int L = 8; BODY; } break;    \
+        default: { constexpr int L = 16; BODY; } break;  \
+    }
 
 }  // namespace
 
@@ -832,6 +1072,9 @@ struct sph_context {
     bool ownsStream = false;
     std::string error;
     int block = 256;
+    int lanes = 0;           // lanes per particle in the sweeps; 0 picks by dimension
+    int listCapacity = -1;   // neighbor-list entries per particle; -1 picks by dimension, 0 walks the cells
+    NeighborList list{};
 
     std::vector<sph_run> runs;
     std::vector<RunDev> runsHost;
@@ -872,6 +1115,16 @@ struct sph_context {
 
 namespace {
 
+int lanesOf(const sph_context* ctx) {
+    if (ctx->lanes > 0) return ctx->lanes;
+    return ctx->dims == 2 ? 8 : 16;
+}
+
+int listCapacityOf(const sph_context* ctx) {
+    if (ctx->listCapacity >= 0) return ctx->listCapacity;
+    return ctx->dims == 2 ? 64 : 160;
+}
+
 int fail(sph_context* ctx, int code, const std::string& message) {
     if (ctx) ctx->error = message;
     return code;
@@ -904,6 +1157,9 @@ void dropGraph(sph_context* ctx) {
 
 void releaseScratch(sph_context* ctx) {
     dropGraph(ctx);
+    if (ctx->list.index) (void)hipFree(ctx->list.index);
+    if (ctx->list.count) (void)hipFree(ctx->list.count);
+    ctx->list = NeighborList{};
     for (void* p : ctx->allocations) (void)hipFree(p);
     ctx->allocations.clear();
     ctx->dRuns = nullptr;
@@ -931,6 +1187,8 @@ int allocateRecords(sph_context* ctx, Records<Real>& rec, int64_t cap) {
     }
     if (status == SPH_OK) status = allocate(ctx, &rec.rho, cap);
     if (status == SPH_OK) status = allocate(ctx, &rec.press, cap);
+    if (status == SPH_OK) status = allocate(ctx, &rec.pr2, cap);
+    if (status == SPH_OK) status = allocate(ctx, &rec.vol, cap);
     if (status == SPH_OK) status = allocate(ctx, &rec.dev, cap);
     if (status == SPH_OK) status = allocate(ctx, &rec.mass, cap);
     if (status == SPH_OK) status = allocate(ctx, &rec.len, cap);
@@ -970,6 +1228,21 @@ int checkRuns(sph_context* ctx, const sph_run* runs, int32_t count, int64_t capa
     return SPH_OK;
 }
 
+// The neighbor list for the context's capacity of slots; its entries are allocated apart from the
+// rest of the scratch so a new list capacity does not reallocate the state's scratch.
+int allocateList(sph_context* ctx) {
+    if (ctx->list.index) (void)hipFree(ctx->list.index);
+    if (ctx->list.count) (void)hipFree(ctx->list.count);
+    ctx->list = NeighborList{};
+    const int32_t capacity = listCapacityOf(ctx);
+    const int64_t cap = std::max<int64_t>(ctx->capacity, 1);
+    ctx->list.stride = cap;
+    ctx->list.capacity = capacity;
+    HIP_TRY(hipMalloc(&ctx->list.count, cap * sizeof(int32_t)));
+    if (capacity > 0) HIP_TRY(hipMalloc(&ctx->list.index, (size_t)capacity * cap * sizeof(int32_t)));
+    return SPH_OK;
+}
+
 template <typename Real>
 Records<Real>& recordsOf(sph_context* ctx);
 template <>
@@ -985,7 +1258,7 @@ int launchCheck(sph_context* ctx, const char* what) {
 
 // Kernel ids of the per-kernel timing: include/sph_hip.h lists them for sph_kernel_times.
 enum KernelId { kClock = 0, kCellKeys, kScan, kCellScatter, kCellOrder, kGather, kSweepWall, kSweepAcceleration,
-                kKick, kSweepContinuity, kEnforce, kKernelKinds };
+                kKick, kSweepContinuity, kNeighbors, kEnforce, kKernelKinds };
 
 // Records a start event before a timed launch; returns its slot, or -1 while not profiling.
 int timeBegin(sph_context* ctx, int kernel) {
@@ -1032,7 +1305,7 @@ int collectTimes(sph_context* ctx) {
 
 // The cell list over the bound positions.
 template <int D>
-int buildCells(sph_context* ctx) {
+int buildCells(sph_context* ctx, int clockMode = 0) {
     const int64_t n = ctx->total;
     const int b = 256;
     double* P = ctx->state.position;
@@ -1040,7 +1313,8 @@ int buildCells(sph_context* ctx) {
     if (n > 0)
         TIMED(ctx, kCellKeys, (cellKeys<D><<<gridFor(n, b), b, 0, ctx->stream>>>(ctx->dRuns, ctx->runOf, P, P + st,
                                                                                  D == 3 ? P + 2 * st : P, n, ctx->key,
-                                                                                 ctx->cellCount)));
+                                                                                 ctx->cellCount, ctx->clock,
+                                                                                 clockMode)));
     const int64_t m = ctx->cells;
     {
         const int slot = timeBegin(ctx, kScan);
@@ -1101,10 +1375,11 @@ int stepOnce(sph_context* ctx, int phase, int useClock, double forcedDt) {
     Records<Real>& rec = recordsOf<Real>(ctx);
     const int64_t st = ctx->state.stride;
     if (phase & 1) {
-        TIMED(ctx, kClock, (clockBegin<<<gridFor(nruns, 64), 64, 0, ctx->stream>>>(ctx->clock, nruns, useClock)));
-        int status = buildCells<D>(ctx);
+        // A run without particles has no slot to open its step; the kernels skip it either way
+        int status = buildCells<D>(ctx, useClock ? 1 : 2);
         if (status != SPH_OK) return status;
     }
+    (void)nruns;
     if (n == 0) return launchCheck(ctx, "device step");
     Sources<Real> src = sourcesOf<Real>(ctx, ctx->state.velocity, ctx->state.density, nullptr,
                                         static_cast<Real*>(ctx->state.pressure));
@@ -1122,22 +1397,31 @@ int stepOnce(sph_context* ctx, int phase, int useClock, double forcedDt) {
         TIMED(ctx, kGather,
               (gatherRecords<D, Real><<<gridFor(n, b), b, 0, ctx->stream>>>(ctx->dRuns, ctx->runOf, ctx->order, src, rec,
                                                                              ctx->clock, 1, n)));
-        TIMED(ctx, kSweepWall,
-              (sweepWallAndGradients<D, Real><<<gridFor(n, b), b, 0, ctx->stream>>>(
-                  ctx->dRuns, ctx->runOf, ctx->order, ctx->cellStart, rec, out, ctx->clock, y64, 1, 1, 1, n)));
-        TIMED(ctx, kSweepAcceleration,
-              (sweepAccelerations<D, Real><<<gridFor(n, b), b, 0, ctx->stream>>>(
-                  ctx->dRuns, ctx->runOf, ctx->order, ctx->cellStart, rec, out, ctx->clock, y64, 1, 1, 1, 0, n)));
+        if (ctx->list.capacity > 0) {
+            WITH_LANES(ctx, TIMED(ctx, kNeighbors,
+                                  (buildNeighbors<D, Real, L, true><<<gridFor(n * L, b), b, 0, ctx->stream>>>(
+                                      ctx->dRuns, ctx->runOf, ctx->order, ctx->cellStart, rec, ctx->list, out,
+                                      ctx->clock, y64, 1, n))));
+        } else {
+            WITH_LANES(ctx, TIMED(ctx, kSweepWall,
+                                  (sweepWallAndGradients<D, Real, L><<<gridFor(n * L, b), b, 0, ctx->stream>>>(
+                                      ctx->dRuns, ctx->runOf, ctx->order, ctx->cellStart, rec, ctx->list, out,
+                                      ctx->clock, y64, 1, 1, 1, n))));
+        }
+        WITH_LANES(ctx, TIMED(ctx, kSweepAcceleration,
+                              (sweepAccelerations<D, Real, L><<<gridFor(n * L, b), b, 0, ctx->stream>>>(
+                                  ctx->dRuns, ctx->runOf, ctx->order, ctx->cellStart, rec, ctx->list, out, ctx->clock,
+                                  y64, 1, 1, 1, 0, n))));
     }
     if (phase & 2) {
         TIMED(ctx, kKick,
               (kick<D, Real><<<gridFor(n, b), b, 0, ctx->stream>>>(ctx->dRuns, ctx->runOf, ctx->order, rec,
                                                                     static_cast<const Real*>(ctx->state.acceleration),
                                                                     ctx->dVel, ctx->clock, forcedDt, st, n)));
-        TIMED(ctx, kSweepContinuity,
-              (sweepContinuity<D, Real><<<gridFor(n, b), b, 0, ctx->stream>>>(ctx->dRuns, ctx->runOf, ctx->order,
-                                                                               ctx->cellStart, rec, out, ctx->clock, 1,
-                                                                               0, 0, n)));
+        WITH_LANES(ctx, TIMED(ctx, kSweepContinuity,
+                              (sweepContinuity<D, Real, L><<<gridFor(n * L, b), b, 0, ctx->stream>>>(
+                                  ctx->dRuns, ctx->runOf, ctx->order, ctx->cellStart, rec, ctx->list, out, ctx->clock,
+                                  1, 0, 0, n))));
     }
     return launchCheck(ctx, "device step");
 }
@@ -1178,49 +1462,50 @@ int hostSweep(sph_context* ctx, int which, const double* velocity, const double*
     SweepOut<Real> out = emptyOut<Real>(ctx);
     const double* y64 = ctx->state.position + ctx->state.stride;
     const RunDev& run0 = ctx->runsHost[0];
+    const int32_t* order = ctx->order;
+    const int32_t* start = ctx->cellStart;
+    const NeighborList list = ctx->list;
+    if (list.capacity > 0)
+        WITH_LANES(ctx, (buildNeighbors<D, Real, L, false><<<gridFor(n * L, b), b, 0, ctx->stream>>>(
+                            ctx->dRuns, ctx->runOf, order, start, rec, list, emptyOut<Real>(ctx), ctx->clock, y64, 0,
+                            n)));
     switch (which) {
         case 0:   // wall pressure
             out.pressure = static_cast<Real*>(out0);
             out.density = out1;
-            sweepWallAndGradients<D, Real><<<gridFor(n, b), b, 0, ctx->stream>>>(ctx->dRuns, ctx->runOf, ctx->order,
-                                                                                  ctx->cellStart, rec, out, ctx->clock,
-                                                                                  y64, 0, 1, 0, n);
+            WITH_LANES(ctx, (sweepWallAndGradients<D, Real, L><<<gridFor(n * L, b), b, 0, ctx->stream>>>(
+                                ctx->dRuns, ctx->runOf, order, start, rec, list, out, ctx->clock, y64, 0, 1, 0, n)));
             break;
         case 1:   // velocity gradients
             out.gradients = static_cast<Real*>(out0);
-            sweepWallAndGradients<D, Real><<<gridFor(n, b), b, 0, ctx->stream>>>(ctx->dRuns, ctx->runOf, ctx->order,
-                                                                                  ctx->cellStart, rec, out, ctx->clock,
-                                                                                  y64, 0, 0, 1, n);
+            WITH_LANES(ctx, (sweepWallAndGradients<D, Real, L><<<gridFor(n * L, b), b, 0, ctx->stream>>>(
+                                ctx->dRuns, ctx->runOf, order, start, rec, list, out, ctx->clock, y64, 0, 0, 1, n)));
             break;
         case 2:   // correction matrices
             out.corrections = static_cast<Real*>(out0);
-            sweepAccelerations<D, Real><<<gridFor(n, b), b, 0, ctx->stream>>>(ctx->dRuns, ctx->runOf, ctx->order,
-                                                                               ctx->cellStart, rec, out, ctx->clock, y64,
-                                                                               0, 0, 1, 0, n);
+            WITH_LANES(ctx, (sweepAccelerations<D, Real, L><<<gridFor(n * L, b), b, 0, ctx->stream>>>(
+                                ctx->dRuns, ctx->runOf, order, start, rec, list, out, ctx->clock, y64, 0, 0, 1, 0, n)));
             break;
         case 3: {  // accelerations, with the limiter's gradients first when the run has it
             const int limiter = run0.limiter;
             if (limiter)
-                sweepWallAndGradients<D, Real><<<gridFor(n, b), b, 0, ctx->stream>>>(ctx->dRuns, ctx->runOf, ctx->order,
-                                                                                      ctx->cellStart, rec, out, ctx->clock,
-                                                                                      y64, 0, 0, 1, n);
+                WITH_LANES(ctx, (sweepWallAndGradients<D, Real, L><<<gridFor(n * L, b), b, 0, ctx->stream>>>(
+                                    ctx->dRuns, ctx->runOf, order, start, rec, list, out, ctx->clock, y64, 0, 0, 1, n)));
             out.acceleration = static_cast<Real*>(out0);
-            sweepAccelerations<D, Real><<<gridFor(n, b), b, 0, ctx->stream>>>(ctx->dRuns, ctx->runOf, ctx->order,
-                                                                               ctx->cellStart, rec, out, ctx->clock, y64,
-                                                                               0, 1, 0, limiter, n);
+            WITH_LANES(ctx, (sweepAccelerations<D, Real, L><<<gridFor(n * L, b), b, 0, ctx->stream>>>(
+                                ctx->dRuns, ctx->runOf, order, start, rec, list, out, ctx->clock, y64, 0, 1, 0, limiter, n)));
             break;
         }
         case 4: {  // continuity, with the correction matrices first when the run has them
             const int correction = run0.correction;
             if (correction)
-                sweepAccelerations<D, Real><<<gridFor(n, b), b, 0, ctx->stream>>>(ctx->dRuns, ctx->runOf, ctx->order,
-                                                                                   ctx->cellStart, rec, out, ctx->clock,
-                                                                                   y64, 0, 0, 1, 0, n);
+                WITH_LANES(ctx, (sweepAccelerations<D, Real, L><<<gridFor(n * L, b), b, 0, ctx->stream>>>(
+                                    ctx->dRuns, ctx->runOf, order, start, rec, list, out, ctx->clock, y64, 0, 0, 1, 0, n)));
             out.rate = static_cast<Real*>(out0);
             for (int a = 0; a < 3; ++a) out.pos[a] = a < D ? ctx->state.position + a * ctx->state.stride : nullptr;
-            sweepContinuity<D, Real><<<gridFor(n, b), b, 0, ctx->stream>>>(ctx->dRuns, ctx->runOf, ctx->order,
-                                                                            ctx->cellStart, rec, out, ctx->clock, 0,
-                                                                            correction, run0.diffusion, n);
+            WITH_LANES(ctx, (sweepContinuity<D, Real, L><<<gridFor(n * L, b), b, 0, ctx->stream>>>(
+                                ctx->dRuns, ctx->runOf, order, start, rec, list, out, ctx->clock, 0, correction,
+                                run0.diffusion, n)));
             break;
         }
         default:
@@ -1251,9 +1536,13 @@ int neighborsImpl(sph_context* ctx, int32_t* counts, const int64_t* offsets, int
     clockBegin<<<gridFor(nruns, 64), 64, 0, ctx->stream>>>(ctx->clock, nruns, 0);
     Sources<Real> src = sourcesOf<Real>(ctx, ctx->state.velocity, ctx->state.density, nullptr, nullptr);
     gatherRecords<D, Real><<<gridFor(n, b), b, 0, ctx->stream>>>(ctx->dRuns, ctx->runOf, ctx->order, src, rec, ctx->clock, 0, n);
-    neighborList<D, Real><<<gridFor(n, b), b, 0, ctx->stream>>>(ctx->dRuns, ctx->runOf, ctx->order, ctx->cellStart, rec,
-                                                                 ctx->state.position + ctx->state.stride, counts, offsets,
-                                                                 rows, n);
+    const double* y64 = ctx->state.position + ctx->state.stride;
+    if (ctx->list.capacity > 0)
+        WITH_LANES(ctx, (buildNeighbors<D, Real, L, false><<<gridFor(n * L, b), b, 0, ctx->stream>>>(
+                            ctx->dRuns, ctx->runOf, ctx->order, ctx->cellStart, rec, ctx->list, emptyOut<Real>(ctx),
+                            ctx->clock, y64, 0, n)));
+    neighborExport<D, Real><<<gridFor(n, b), b, 0, ctx->stream>>>(ctx->dRuns, ctx->runOf, ctx->order, ctx->cellStart, rec,
+                                                                   ctx->list, y64, counts, offsets, rows, n);
     int status = launchCheck(ctx, "neighbor list");
     if (status != SPH_OK) return status;
     HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -1391,6 +1680,7 @@ int sph_configure(sph_context* ctx, const sph_run* runs, int32_t count, int64_t 
     if (status == SPH_OK) status = allocate(ctx, &ctx->dVel, 3);
     if (status == SPH_OK)
         status = ctx->precision == 0 ? allocateRecords(ctx, ctx->rec32, cap) : allocateRecords(ctx, ctx->rec64, cap);
+    if (status == SPH_OK) status = allocateList(ctx);
     if (status != SPH_OK) {
         releaseScratch(ctx);
         return status;
@@ -1602,13 +1892,26 @@ int sph_kernel_times(sph_context* ctx, double* milliseconds, int64_t* launches, 
     return SPH_OK;
 }
 
-int sph_set_launch(sph_context* ctx, int32_t blockSize) {
+int sph_set_launch(sph_context* ctx, int32_t blockSize, int32_t lanes, int32_t listCapacity) {
     if (!ctx) return SPH_ERR_VALUE;
-    if (blockSize < 64 || blockSize > 1024 || blockSize % 64 != 0)
-        return fail(ctx, SPH_ERR_VALUE, "The workgroup size must be a multiple of 64 from 64 to 1024.");
-    if (blockSize != ctx->block) {
+    if (listCapacity < -1 || listCapacity > 4096)
+        return fail(ctx, SPH_ERR_VALUE, "The neighbor-list capacity must be from 0 to 4096 (-1 picks by dimension).");
+    if (blockSize < 64 || blockSize > 256 || blockSize % 64 != 0)
+        return fail(ctx, SPH_ERR_VALUE, "The workgroup size must be 64, 128, 192 or 256.");
+    if (lanes != 0 && lanes != 1 && lanes != 2 && lanes != 4 && lanes != 8 && lanes != 16)
+        return fail(ctx, SPH_ERR_VALUE, "Lanes per particle must be 1, 2, 4, 8 or 16 (0 picks by dimension).");
+    if (blockSize != ctx->block || lanes != ctx->lanes) {
         ctx->block = blockSize;
+        ctx->lanes = lanes;
         dropGraph(ctx);
+    }
+    if (listCapacity != ctx->listCapacity) {
+        ctx->listCapacity = listCapacity;
+        dropGraph(ctx);
+        if (ctx->dRuns) {
+            HIP_TRY(hipStreamSynchronize(ctx->stream));
+            return allocateList(ctx);
+        }
     }
     return SPH_OK;
 }

@@ -221,8 +221,16 @@ class Engine:
     def advance(self, steps: int, graph: bool = True) -> None:
         self.check(self.library.sph_advance(self.handle, int(steps), 1 if graph else 0), 'sph_advance')
 
-    def setLaunch(self, blockSize: int) -> None:
-        self.check(self.library.sph_set_launch(self.handle, int(blockSize)), 'sph_set_launch')
+    def setLaunch(self, blockSize: int = 256, lanes: int = 0, listCapacity: int = -1) -> None:
+        '''
+
+        Threads per workgroup and lanes per particle of the pair sweeps (0 lanes: 8 in 2D, 16 in 3D),
+        and the neighbor list's entries per particle (0: every sweep walks the cells; -1: 64 in 2D,
+        160 in 3D).
+
+        '''
+        self.check(self.library.sph_set_launch(self.handle, int(blockSize), int(lanes), int(listCapacity)),
+                   'sph_set_launch')
 
 
 def boundaryOf(boundary, dimensions: int, device: torch.device, pistonSpeed: float = 0.0) -> tuple:
@@ -673,6 +681,8 @@ class PairSums:
 
         '''
 
+        if self.count == 0:
+            return {'time': float(time), 'steps': int(steps), 'dt': 0.0}
         self._sync()
         engine = self.engine
         engine.setClock(0, time, untilTime, steps, maxSteps)

@@ -45,9 +45,9 @@ def oracleRun(case, steps):
     return run
 
 
-def engineRun(cases, steps, precision, graph = True, chunk = None):
+def engineRun(cases, steps, precision, graph = True, chunk = None, listCapacity = -1):
     from pysurf_amd.batch import BatchEngine
-    batch = BatchEngine(cases, precision)
+    batch = BatchEngine(cases, precision, listCapacity = listCapacity)
     batch.setClocks(maxSteps = steps)
     batch.advance(chunk or steps, graph)
     if chunk:
@@ -81,6 +81,19 @@ def testGraphAndStreamLaunchesAgreeBitForBit():
     b = engineRun([case], 40, 'float32', graph = False, chunk = 8).state(0)
     for name in ('positions', 'velocities', 'densities', 'pressures', 'accelerations'):
         assert np.array_equal(getattr(a, name), getattr(b, name)), name
+
+
+@pytest.mark.parametrize('precision', ['float32', 'float64'])
+def testTheNeighborListAndTheCellWalkAgreeBitForBit(precision):
+    # A particle whose neighbors overflow the list walks its cells, dealing the same neighbors to the
+    # same lanes: a full list, a list of 4 entries (nearly every particle overflows) and no list
+    # step identically
+    print(f'neighbor list vs cell walk {precision}', flush = True)
+    case = smallCase(seed = 8)
+    states = [engineRun([case], 25, precision, listCapacity = capacity).state(0) for capacity in (-1, 4, 0)]
+    for other in states[1:]:
+        for name in ('positions', 'velocities', 'densities', 'pressures'):
+            assert np.array_equal(states[0].__dict__[name], other.__dict__[name]), name
 
 
 def testARunAloneAndInABatchAgreeBitForBit():
