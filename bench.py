@@ -64,6 +64,11 @@ def parse():
     parser.add_argument('--cpu-seconds', type = float, default = 15.0)
     parser.add_argument('--seed', type = int, default = 0)
     parser.add_argument('--no-cpu', action = 'store_true', help = 'skip the CPU baseline')
+    parser.add_argument('--no-batched', action = 'store_true', help = 'skip the batched sweep')
+    parser.add_argument('--runs', type = int, default = 256, help = 'runs of the batched sweep')
+    parser.add_argument('--batched-steps', type = int, default = 200)
+    parser.add_argument('--plain-sample', type = int, default = 16, help = 'runs the plain float64 kernel steps alone')
+    parser.add_argument('--plain-steps', type = int, default = 200)
     return parser.parse_args()
 
 
@@ -114,6 +119,58 @@ def kernelProfile(batch, steps: int) -> dict:
                                                  launches.ctypes.data_as(ctypes.c_void_p), kinds), 'sph_kernel_times')
     engine.check(engine.library.sph_profile(engine.handle, 0), 'sph_profile')
     return {name: {'ms': float(ms[k]), 'launches': int(launches[k])} for k, name in enumerate(_native.kernelNames)}
+
+
+def batchedSweep(args) -> dict:
+
+    '''
+
+    The batched sweep: `args.runs` uneven 2D runs (slope, height, resolution and length drawn per
+    seed) packed into one context and stepped by the same graph-replayed launches, against the plain
+    float64 kernel stepping a sample of the same runs, each alone, one after another.
+
+    '''
+
+    import numpy as np
+
+    from pysurf_amd.batch import BatchEngine
+    from pysurf_amd.scenarios import sweepRun
+    cases = [sweepRun(seed) for seed in range(args.runs)]
+    fluid = sum(case.fluidCount for case in cases)
+    particles = sum(case.count for case in cases)
+    print(f'# batched: {len(cases)} runs, {particles} particles ({fluid} fluid)', file = sys.stderr, flush = True)
+    batch = BatchEngine(cases, 'float32')
+    batch.setClocks(maxSteps = 2**62)
+    batch.advance(args.chunk)
+    batch.synchronize()
+    seconds = timedSteps(batch, args.batched_steps, args.chunk)
+    value = fluid * args.batched_steps / seconds
+    clock = batch.clocks()
+    del batch
+    print(f'# batched float32: {1000 * seconds / args.batched_steps:.3f} ms/step, {value:.4e} fluid updates/s',
+          file = sys.stderr, flush = True)
+    updates, elapsed = 0, 0.0
+    sample = cases[:args.plain_sample]
+    for case in sample:
+        solo = BatchEngine([case], 'float64')
+        solo.setClocks(maxSteps = 2**62)
+        solo.advance(args.chunk)
+        solo.synchronize()
+        elapsed += timedSteps(solo, args.plain_steps, args.chunk)
+        updates += case.fluidCount * args.plain_steps
+        del solo
+    plain = updates / elapsed
+    print(f'# plain float64, {len(sample)} runs alone: {plain:.4e} fluid updates/s', file = sys.stderr, flush = True)
+    sizes = np.array([case.count for case in cases])
+    return {'value': value, 'unit': 'particle-updates/s', 'runs': len(cases), 'particles': particles,
+            'fluid_particles': fluid, 'run_particles_min': int(sizes.min()), 'run_particles_max': int(sizes.max()),
+            'steps': args.batched_steps, 'ms_per_step': 1000.0 * seconds / args.batched_steps,
+            'simulated_seconds_min': float(clock['time'].min()), 'simulated_seconds_max': float(clock['time'].max()),
+            'plain_fp64': {'value': plain, 'unit': 'particle-updates/s', 'runs_sampled': len(sample),
+                           'steps_each': args.plain_steps, 'how': 'each run alone in its own context, one after another'},
+            'speedup_over_plain_fp64': value / plain,
+            'workload': 'batched sweep: runs seeded 0..N-1 with slope U[1/35, 1/8], H/d U[0.2, 0.6], '
+                        'resolution from {16, 24, 32, 48} particles per depth, length U[20, 30] d, float32 pairs'}
 
 
 def cpuBaseline(case, seconds: float) -> dict:
@@ -185,6 +242,10 @@ def main() -> int:
           file = sys.stderr, flush = True)
     del plain
 
+    batched = None
+    if not args.no_batched:
+        batched = batchedSweep(args)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpuBaseline(case, args.cpu_seconds)
@@ -217,6 +278,7 @@ def main() -> int:
                      'kernel_ms_per_step': {name: entry['ms'] / args.profile_steps for name, entry in profile.items()}},
         'plain_fp64': {'value': fp64Value, 'unit': 'particle-updates/s', 'steps': args.fp64_steps,
                        'ms_per_step': 1000.0 * fp64Seconds / args.fp64_steps, 'speedup_of_f32': value / fp64Value},
+        'batched': batched,
         'cpu_baseline': cpu,
     }
     print(json.dumps(line))

@@ -38,26 +38,31 @@ struct Clock {
     int32_t* active;
 };
 
-// The float32 (or float64) records of the sorted particles the sweeps gather from. For float32
-// pairs each float64 position is carried as a float32 point and a float32 remainder
-// (sphGpu.py:1010-1019); for float64, `pos` is the position itself and `lo` is unused.
+template <typename Real>
+struct alignas(16) Quad {
+    Real x, y, z, w;
+};
+
+// One sorted particle as the pair sweeps read it: four quads, 64 bytes in float32 (one cache
+// line, so a pair reads one line) and 128 in float64. For float32 pairs each float64 position is
+// carried as a float32 point and a float32 remainder (sphGpu.py:1010-1019); in float64 the
+// remainder is zero and never read.
+template <typename Real>
+struct Record {
+    Quad<Real> a;   // point x, y, z; the smoothing length, negated on boundary particles
+    Quad<Real> b;   // remainder x, y, z; p / rho^2
+    Quad<Real> c;   // velocity x, y, z; density
+    Quad<Real> e;   // mass; m / rho; rho - rho0 taken in float64 (sphGpu.py:1031-1038); pressure
+};
+
+// The sorted records, with the particle's cell and the per-particle matrices each sweep writes
+// for the next, D*D values per particle, row-major.
 template <typename Real>
 struct Records {
-    Real* pos[3];
-    float* lo[3];
-    Real* vel[3];
-    Real* rho;       // density copy
-    Real* press;     // pressure
-    Real* pr2;       // p / rho^2
-    Real* vol;       // m / rho
-    Real* dev;       // rho - rho0, taken in float64 (sphGpu.py:1031-1038)
-    Real* mass;
-    Real* len;       // smoothing length
-    uint8_t* fluid;
+    Record<Real>* r;
     int32_t* key;    // the particle's cell, in the context's cell array
-    Real* G;         // velocity gradient, D*D rows
-    Real* L;         // kernel gradient correction, D*D rows
-    int64_t stride;
+    Real* G;         // velocity gradient
+    Real* L;         // kernel gradient correction
 };
 
 __device__ __forceinline__ int wrapIndex(int64_t a, int32_t n) {
@@ -115,7 +120,25 @@ struct Self {
     float l[3];
     Real h;
     Real reach;
+    bool fluid;
 };
+
+// d = r_i - r_j for particle i against record quads (a, b) of j, with `shift` added along y for
+// j's periodic image: (p_i - p_j) + shift + (l_i - l_j) on float32 points and remainders, the
+// plain float64 difference otherwise.
+template <int D, typename Real>
+__device__ __forceinline__ void separation(const Self<D, Real>& me, const Quad<Real>& a, const Quad<Real>& b,
+                                           Real shift, Real& dx, Real& dy, Real& dz) {
+    if constexpr (sizeof(Real) == 4) {
+        dx = (me.p[0] - a.x) + (me.l[0] - b.x);
+        dy = (me.p[1] - a.y) + shift + (me.l[1] - b.y);
+        dz = D == 3 ? (me.p[2] - a.z) + (me.l[2] - b.z) : 0.0f;
+    } else {
+        dx = me.p[0] - a.x;
+        dy = (me.p[1] + shift) - a.y;
+        dz = D == 3 ? me.p[2] - a.z : 0.0;
+    }
+}
 
 // sqrt(fma(dz, dz, fma(dy, dy, dx * dx))), the neighbor predicate's distance, correctly rounded.
 __device__ __forceinline__ float pairDistance(float dx, float dy, float dz, int dims) {
@@ -199,17 +222,16 @@ struct PairKernel {
 };
 
 // Walks every neighbor j != i of sorted particle s with r < 2 h_ij (sphGpu.py:352-363), calling
-// f(k, dx, dy, dz, r, h) with k the neighbor's sorted position, d = r_i - r_j at the nearest
-// periodic image, r = |d| and h = h_ij = (h_i + h_j) / 2. Each cell range's candidates are dealt to
-// `Lanes` lanes in turn, lane `lane` taking the range's candidates lane, lane + Lanes, ..., so a
-// group of lanes shares one particle's walk. With `skipCoincident` pairs closer than 1e-12 are skipped too, as every sum but
-// the wall pressure skips them. `y64` is the particle's float64 y, read only for a search from
-// its periodic image.
+// f(k, fluidJ, dx, dy, dz, r, h) with k the neighbor's sorted position, d = r_i - r_j at the
+// nearest periodic image, r = |d| and h = h_ij = (h_i + h_j) / 2. Each cell range's candidates are
+// dealt to `Lanes` lanes in turn, lane `lane` taking the range's candidates lane, lane + Lanes, ...,
+// so a group of lanes shares one particle's walk. With `SkipCoincident` pairs closer than 1e-12 are
+// skipped too, as every sum but the wall pressure skips them. `y64` is the particle's float64 y,
+// read only for a search from its periodic image.
 template <int D, typename Real, int Lanes, bool SkipCoincident, class F>
 __device__ __forceinline__ void forNeighbors(const RunDev& run, const int32_t* __restrict__ start,
                                              const Records<Real>& rec, int64_t s, const Self<D, Real>& me,
                                              int cx, int cy, int cz, double y64, int lane, bool uniform, F&& f) {
-    constexpr bool single = sizeof(Real) == 4;
     const Real period = (Real)run.period;
     const Real twoHmax = Real(2) * (Real)run.hmax;
     int images = 1;
@@ -224,28 +246,21 @@ __device__ __forceinline__ void forNeighbors(const RunDev& run, const int32_t* _
         forStencil<D>(run, start, cx, oy, cz, [&](int32_t begin, int32_t end) {
             for (int32_t k = begin + lane; k < end; k += Lanes) {
                 if (k == s) continue;
-                Real dx, dy, dz = Real(0);
-                if constexpr (single) {
-                    dx = (me.p[0] - rec.pos[0][k]) + (me.l[0] - rec.lo[0][k]);
-                    dy = (me.p[1] - rec.pos[1][k]) + shift + (me.l[1] - rec.lo[1][k]);
-                    if (D == 3) dz = (me.p[2] - rec.pos[2][k]) + (me.l[2] - rec.lo[2][k]);
-                } else {
-                    dx = me.p[0] - rec.pos[0][k];
-                    dy = (me.p[1] + shift) - rec.pos[1][k];
-                    if (D == 3) dz = me.p[2] - rec.pos[2][k];
-                }
+                const Quad<Real> a = rec.r[k].a;
+                Real dx, dy, dz;
+                separation<D, Real>(me, a, rec.r[k].b, shift, dx, dy, dz);
                 const Real r = pairDistance(dx, dy, dz, D);
                 Real h, support;
                 if (uniform) {
                     h = (Real)run.hmax;
                     support = twoHmax;
                 } else {
-                    h = Real(0.5) * (me.h + rec.len[k]);
+                    h = Real(0.5) * (me.h + fabs(a.w));
                     support = Real(2) * h;
                 }
                 if (!(r < support)) continue;
                 if (SkipCoincident && r < Real(1.0e-12)) continue;
-                f(k, dx, dy, dz, r, h);
+                f(k, a.w > Real(0), dx, dy, dz, r, h);
             }
         });
     }
@@ -264,9 +279,9 @@ struct NeighborList {
 
 constexpr int32_t kImageBit = (int32_t)0x80000000u;
 
-// Calls f(k, dx, dy, dz, r, h) for the neighbors of sorted particle s that lane `lane` of its
-// group takes: neighbor m, in the order forNeighbors visits them, goes to lane m % Lanes. From the
-// list when the particle's neighbors fit it, from its cells otherwise; the same pairs reach the
+// Calls f(k, fluidJ, dx, dy, dz, r, h) for the neighbors of sorted particle s that lane `lane` of
+// its group takes: neighbor m, in the order forNeighbors visits them, goes to lane m % Lanes. From
+// the list when the particle's neighbors fit it, from its cells otherwise; the same pairs reach the
 // same lanes in the same order either way.
 template <int D, typename Real, int Lanes, bool SkipCoincident, class F>
 __device__ __forceinline__ void forEachNeighbor(const RunDev& run, const int32_t* __restrict__ start,
@@ -275,37 +290,29 @@ __device__ __forceinline__ void forEachNeighbor(const RunDev& run, const int32_t
                                                 bool uniform, F&& f) {
     const int32_t count = list.capacity > 0 ? list.count[s] : -1;
     if (count >= 0 && count <= list.capacity) {
-        constexpr bool single = sizeof(Real) == 4;
         const Real period = (Real)run.period;
         const Real shiftImage = me.p[1] < Real(0.5) * period ? period : -period;
         for (int32_t m = lane; m < count; m += Lanes) {
             const int32_t entry = list.index[(int64_t)m * list.stride + s];
             const int32_t k = entry & 0x7fffffff;
             const Real shift = entry < 0 ? shiftImage : Real(0);
-            Real dx, dy, dz = Real(0);
-            if constexpr (single) {
-                dx = (me.p[0] - rec.pos[0][k]) + (me.l[0] - rec.lo[0][k]);
-                dy = (me.p[1] - rec.pos[1][k]) + shift + (me.l[1] - rec.lo[1][k]);
-                if (D == 3) dz = (me.p[2] - rec.pos[2][k]) + (me.l[2] - rec.lo[2][k]);
-            } else {
-                dx = me.p[0] - rec.pos[0][k];
-                dy = (me.p[1] + shift) - rec.pos[1][k];
-                if (D == 3) dz = me.p[2] - rec.pos[2][k];
-            }
+            const Quad<Real> a = rec.r[k].a;
+            Real dx, dy, dz;
+            separation<D, Real>(me, a, rec.r[k].b, shift, dx, dy, dz);
             const Real r = pairDistance(dx, dy, dz, D);
             if (SkipCoincident && r < Real(1.0e-12)) continue;
-            const Real h = uniform ? (Real)run.hmax : Real(0.5) * (me.h + rec.len[k]);
-            f(k, dx, dy, dz, r, h);
+            const Real h = uniform ? (Real)run.hmax : Real(0.5) * (me.h + fabs(a.w));
+            f(k, a.w > Real(0), dx, dy, dz, r, h);
         }
         return;
     }
     int32_t m = 0;
     forNeighbors<D, Real, 1, false>(run, start, rec, s, me, cx, cy, cz, y64, 0, uniform,
-                                    [&](int32_t k, Real dx, Real dy, Real dz, Real r, Real h) {
+                                    [&](int32_t k, bool fluidJ, Real dx, Real dy, Real dz, Real r, Real h) {
                                         const bool mine = (m % Lanes) == lane;
                                         ++m;
                                         if (!mine || (SkipCoincident && r < Real(1.0e-12))) return;
-                                        f(k, dx, dy, dz, r, h);
+                                        f(k, fluidJ, dx, dy, dz, r, h);
                                     });
 }
 

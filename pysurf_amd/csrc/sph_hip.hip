@@ -243,6 +243,10 @@ struct Sources {
     Real* pressureOut;       // the Tait pressure per slot, written when non-null
 };
 
+// Each sorted particle's record from its slot's state: the float32 point and remainder of its
+// float64 position (sphGpu.py:1010-1019), its velocity and density as the pair sums read them
+// (sphGpu.py:1021-1029), rho - rho0 taken in float64 (sphGpu.py:1031-1038), the Tait pressure
+// (sphGpu.py:1058-1070), p / rho^2 and m / rho; resident, each run's |v|max.
 template <int D, typename Real>
 __global__ void gatherRecords(const RunDev* __restrict__ runs, const int32_t* __restrict__ runOf,
                               const int32_t* __restrict__ order, Sources<Real> src, Records<Real> rec, Clock clock,
@@ -255,45 +259,41 @@ __global__ void gatherRecords(const RunDev* __restrict__ runs, const int32_t* __
     if (valid) {
         const RunDev& run = runs[r];
         const int32_t i = order[s];
+        Record<Real> out;
+        Real p[3] = {Real(0), Real(0), Real(0)}, l[3] = {Real(0), Real(0), Real(0)}, v[3] = {Real(0), Real(0), Real(0)};
+        double v2 = 0.0;
 #pragma unroll
         for (int a = 0; a < D; ++a) {
             const double x = src.pos[a][i];
             if constexpr (sizeof(Real) == 4) {
                 const float point = (float)x;
-                rec.pos[a][s] = point;
-                rec.lo[a][s] = (float)(x - (double)point);
+                p[a] = point;
+                l[a] = (float)(x - (double)point);
             } else {
-                rec.pos[a][s] = x;
+                p[a] = x;
             }
-        }
-        double v2 = 0.0;
-#pragma unroll
-        for (int a = 0; a < D; ++a) {
-            const double v = src.vel[a][i];
-            rec.vel[a][s] = (Real)v;
-            v2 += v * v;
+            const double u = src.vel[a][i];
+            v[a] = (Real)u;
+            v2 += u * u;
         }
         const double rho = src.density[i];
         const bool isFluid = src.fluid[i] != 0;
-        rec.rho[s] = (Real)rho;
-        rec.dev[s] = (Real)(rho - run.rho0);
-        Real p;
+        Real pressure;
         if (src.pressure) {
-            p = src.pressure[i];
+            pressure = src.pressure[i];
         } else {
             const double ratio = rho / run.rho0, sq = ratio * ratio;
-            double pressure = run.stiffness * (sq * sq * sq * ratio - 1.0);
-            if (!run.negativePressure || !isFluid) pressure = fmax(pressure, 0.0);
-            p = (Real)pressure;
-            if (src.pressureOut) src.pressureOut[i] = p;
+            double value = run.stiffness * (sq * sq * sq * ratio - 1.0);
+            if (!run.negativePressure || !isFluid) value = fmax(value, 0.0);
+            pressure = (Real)value;
+            if (src.pressureOut) src.pressureOut[i] = pressure;
         }
-        rec.press[s] = p;
-        const Real rhoR = (Real)rho, massR = src.mass[i];
-        rec.pr2[s] = p / (rhoR * rhoR);
-        rec.vol[s] = massR / rhoR;
-        rec.mass[s] = massR;
-        rec.len[s] = src.len[i];
-        rec.fluid[s] = isFluid ? 1 : 0;
+        const Real rhoR = (Real)rho, massR = src.mass[i], h = src.len[i];
+        out.a = {p[0], p[1], p[2], isFluid ? h : -h};
+        out.b = {l[0], l[1], l[2], pressure / (rhoR * rhoR)};
+        out.c = {v[0], v[1], v[2], rhoR};
+        out.e = {massR, massR / rhoR, (Real)(rho - run.rho0), pressure};
+        rec.r[s] = out;
         rec.key[s] = src.key[i];
         speed = sqrt(v2);
         valid = isFluid;
@@ -309,14 +309,19 @@ template <int D, typename Real>
 __device__ __forceinline__ Self<D, Real> loadSelf(const RunDev& run, const Records<Real>& rec, int64_t s, int& cx,
                                                   int& cy, int& cz) {
     Self<D, Real> me;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        me.p[a] = a < D ? rec.pos[a][s] : Real(0);
-        me.l[a] = 0.0f;
-        if constexpr (sizeof(Real) == 4)
-            if (a < D) me.l[a] = rec.lo[a][s];
+    const Quad<Real> a = rec.r[s].a;
+    me.p[0] = a.x;
+    me.p[1] = a.y;
+    me.p[2] = D == 3 ? a.z : Real(0);
+    me.l[0] = me.l[1] = me.l[2] = 0.0f;
+    if constexpr (sizeof(Real) == 4) {
+        const Quad<Real> b = rec.r[s].b;
+        me.l[0] = b.x;
+        me.l[1] = b.y;
+        me.l[2] = D == 3 ? b.z : 0.0f;
     }
-    me.h = rec.len[s];
+    me.fluid = a.w > Real(0);
+    me.h = fabs(a.w);
     me.reach = me.h + (Real)run.hmax;
     const int64_t local = (int64_t)rec.key[s] - run.cellFirst;
     cx = (int)(local % run.nx);
@@ -389,29 +394,32 @@ struct Sweep1 {
 #pragma unroll
         for (int e = 0; e < K; ++e) acc[e] = Real(0);
         mode = chosen;
+        const Quad<Real> c = rec.r[s].c;
+        vi[0] = c.x;
+        vi[1] = c.y;
+        vi[2] = c.z;
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            vi[a] = (mode == 2 && a < D) ? rec.vel[a][s] : Real(0);
-            g[a] = (Real)run.gravity[a];
-        }
+        for (int a = 0; a < 3; ++a) g[a] = (Real)run.gravity[a];
     }
-    __device__ __forceinline__ void add(const Records<Real>& rec, const PairKernel<Real, D>& kernel, int32_t k, Real dx,
-                                        Real dy, Real dz, Real rr, Real h) {
-        if (mode == 0 || !rec.fluid[k]) return;
+    __device__ __forceinline__ void add(const Records<Real>& rec, const PairKernel<Real, D>& kernel, int32_t k,
+                                        bool fluidJ, Real dx, Real dy, Real dz, Real rr, Real h) {
+        if (mode == 0 || !fluidJ) return;
         if (mode == 1) {
             const Real w = kernel.value(rr, h);
             Real gd = g[0] * dx + g[1] * dy;
             if (D == 3) gd += g[2] * dz;
             acc[0] += w;
-            acc[1] += w * (rec.press[k] + rec.rho[k] * gd);
+            acc[1] += w * (rec.r[k].e.w + rec.r[k].c.w * gd);
             return;
         }
         if (rr < Real(1.0e-12)) return;
-        const Real w = rec.vol[k] * kernel.slopeOverR(rr, h);
+        const Quad<Real> cj = rec.r[k].c;
+        const Real w = rec.r[k].e.y * kernel.slopeOverR(rr, h);
         const Real d[3] = {dx, dy, dz};
+        const Real vj[3] = {cj.x, cj.y, cj.z};
 #pragma unroll
         for (int a = 0; a < D; ++a) {
-            const Real dv = w * (rec.vel[a][k] - vi[a]);
+            const Real dv = w * (vj[a] - vi[a]);
             const Real da = w * d[a];
 #pragma unroll
             for (int b = 0; b < D; ++b) {
@@ -422,109 +430,13 @@ struct Sweep1 {
     }
 };
 
-template <typename Real>
-struct SweepOut;
-
-// The mode of sweep 1 for sorted particle s.
-template <typename Real>
+// The mode of sweep 1 for a particle.
 __device__ __forceinline__ int sweep1Mode(const RunDev& run, bool fluidI, int32_t resident, int32_t doWall,
                                           int32_t doGradients) {
     if (!fluidI && doWall && (!resident || run.extrapolatedWalls)) return 1;
     if (fluidI && doGradients && (!resident || run.limiter)) return 2;
     return 0;
 }
-
-template <int D, typename Real>
-__device__ __forceinline__ void sweep1Finish(const RunDev& run, const Sweep1<D, Real>& sweep, Records<Real>& rec,
-                                             const SweepOut<Real>& out, int64_t s, int32_t i, bool fluidI,
-                                             int32_t resident, int32_t doWall, int32_t doGradients);
-
-// The step's neighbor list: each particle's neighbors {j != i : r_ij < h_i + h_j} in the order the
-// cell walk visits them. A group of `Lanes` lanes tests a particle's candidates `Lanes` at a time
-// and compacts the hits with a ballot, so the list keeps the walk's order. With `WithSweep1` the
-// same walk accumulates sweep 1's sums, each lane over the candidates it tested.
-template <int D, typename Real, int Lanes, bool WithSweep1>
-__global__ void __launch_bounds__(256)
-    buildNeighbors(const RunDev* __restrict__ runs, const int32_t* __restrict__ runOf,
-                   const int32_t* __restrict__ order, const int32_t* __restrict__ start, Records<Real> rec,
-                   NeighborList list, SweepOut<Real> out, Clock clock, const double* __restrict__ y64,
-                   int32_t resident, int64_t n) {
-    const int64_t t = xcdBlock() * blockDim.x + threadIdx.x;
-    const int64_t s = t / Lanes;
-    const int lane = (int)(t % Lanes);
-    const int groupBase = (int)(threadIdx.x & 63) & ~(Lanes - 1);
-    bool valid = s < n;
-    int r = 0;
-    if (valid) {
-        r = runOf[s];
-        valid = !resident || clock.active[r];
-    }
-    Sweep1<D, Real> sweep;
-    sweep.mode = 0;
-    bool fluidI = false;
-    if (valid) {
-        const RunDev& run = runs[r];
-        int cx, cy, cz;
-        const Self<D, Real> me = loadSelf<D, Real>(run, rec, s, cx, cy, cz);
-        const bool uniform = run.uniform != 0;
-        PairKernel<Real, D> kernel;
-        kernel.init(run.kernel, run.sigmaUnit, run.hmax, uniform);
-        fluidI = rec.fluid[s] != 0;
-        sweep.begin(run, rec, s, WithSweep1 ? sweep1Mode<Real>(run, fluidI, resident, 1, 1) : 0);
-        const double yy = run.period > 0.0 ? y64[order[s]] : 0.0;
-        constexpr bool single = sizeof(Real) == 4;
-        const Real period = (Real)run.period;
-        const Real twoHmax = Real(2) * (Real)run.hmax;
-        const uint32_t groupMask = Lanes >= 32 ? 0xffffffffu : ((1u << Lanes) - 1u);
-        int images = 1;
-        if (run.period > 0.0 && (me.p[1] < me.reach || me.p[1] > period - me.reach)) images = 2;
-        int32_t count = 0;
-        for (int image = 0; image < images; ++image) {
-            Real shift = Real(0);
-            int oy = cy;
-            if (image == 1) {
-                shift = me.p[1] < Real(0.5) * period ? period : -period;
-                oy = wrapIndex(cellCoordinate(yy + (double)shift, run.cell), run.ny);
-            }
-            const int32_t tag = image ? kImageBit : 0;
-            forStencil<D>(run, start, cx, oy, cz, [&](int32_t begin, int32_t end) {
-                for (int32_t base = begin; base < end; base += Lanes) {
-                    const int32_t k = base + lane;
-                    bool hit = false;
-                    Real dx = Real(0), dy = Real(0), dz = Real(0), rr = Real(0), h = Real(0);
-                    if (k < end && k != s) {
-                        if constexpr (single) {
-                            dx = (me.p[0] - rec.pos[0][k]) + (me.l[0] - rec.lo[0][k]);
-                            dy = (me.p[1] - rec.pos[1][k]) + shift + (me.l[1] - rec.lo[1][k]);
-                            if (D == 3) dz = (me.p[2] - rec.pos[2][k]) + (me.l[2] - rec.lo[2][k]);
-                        } else {
-                            dx = me.p[0] - rec.pos[0][k];
-                            dy = (me.p[1] + shift) - rec.pos[1][k];
-                            if (D == 3) dz = me.p[2] - rec.pos[2][k];
-                        }
-                        rr = pairDistance(dx, dy, dz, D);
-                        h = uniform ? (Real)run.hmax : Real(0.5) * (me.h + rec.len[k]);
-                        hit = rr < (uniform ? twoHmax : Real(2) * h);
-                    }
-                    const uint32_t mine = (uint32_t)(__ballot(hit) >> groupBase) & groupMask;
-                    if (hit) {
-                        const int32_t slot = count + __popc(mine & ((1u << lane) - 1u));
-                        if (slot < list.capacity) list.index[(int64_t)slot * list.stride + s] = k | tag;
-                        if (WithSweep1) sweep.add(rec, kernel, k, dx, dy, dz, rr, h);
-                    }
-                    count += __popc(mine);
-                }
-            });
-        }
-        if (lane == 0) list.count[s] = count;
-    }
-    if (!WithSweep1) return;
-#pragma unroll
-    for (int e = 0; e < Sweep1<D, Real>::K; ++e) sweep.acc[e] = groupSum<Lanes>(sweep.acc[e]);
-    if (!valid || lane != 0) return;
-    sweep1Finish<D, Real>(runs[r], sweep, rec, out, s, order[s], fluidI, resident, 1, 1);
-}
-
 
 // Sweep 1's results for sorted particle s: a boundary particle's pressure and the density that
 // carries it, or a fluid particle's G; writes the pass-through outputs of a particle with neither.
@@ -533,7 +445,7 @@ __device__ __forceinline__ void sweep1Finish(const RunDev& run, const Sweep1<D, 
                                              const SweepOut<Real>& out, int64_t s, int32_t i, bool fluidI,
                                              int32_t resident, int32_t doWall, int32_t doGradients) {
     if (sweep.mode == 0) {
-        if (doWall && out.pressure && (!fluidI || !resident)) out.pressure[i] = rec.press[s];
+        if (doWall && out.pressure && (!fluidI || !resident)) out.pressure[i] = rec.r[s].e.w;
         if (doGradients && out.gradients && !fluidI)
             for (int e = 0; e < D * D; ++e) out.gradients[e * out.stride + i] = Real(0);
         return;
@@ -543,10 +455,11 @@ __device__ __forceinline__ void sweep1Finish(const RunDev& run, const Sweep1<D, 
         if (sweep.acc[0] > Real(0)) pressure = fmax(sweep.acc[1] / sweep.acc[0], Real(0));
         const double rho = run.rho0 * pow(1.0 + (double)pressure / run.stiffness, 1.0 / 7.0);
         const Real rhoR = (Real)rho;
-        rec.press[s] = pressure;
-        rec.rho[s] = rhoR;
-        rec.pr2[s] = pressure / (rhoR * rhoR);
-        rec.vol[s] = rec.mass[s] / rhoR;
+        Record<Real>& own = rec.r[s];
+        own.b.w = pressure / (rhoR * rhoR);
+        own.c.w = rhoR;
+        own.e.y = own.e.x / rhoR;
+        own.e.w = pressure;
         if (out.pressure) out.pressure[i] = pressure;
         if (out.density) out.density[i] = rho;
         return;
@@ -578,9 +491,88 @@ __device__ __forceinline__ void sweep1Finish(const RunDev& run, const Sweep1<D, 
     for (int a = 0; a < D; ++a)
 #pragma unroll
         for (int b = 0; b < D; ++b) {
-            rec.G[(a * D + b) * rec.stride + s] = G[a][b];
+            rec.G[s * (D * D) + a * D + b] = G[a][b];
             if (out.gradients) out.gradients[(a * D + b) * out.stride + i] = G[a][b];
         }
+}
+
+// The step's neighbor list: each particle's neighbors {j != i : r_ij < h_i + h_j} in the order the
+// cell walk visits them. A group of `Lanes` lanes tests a particle's candidates `Lanes` at a time
+// and compacts the hits with a ballot, so the list keeps the walk's order. With `WithSweep1` the
+// same walk accumulates sweep 1's sums, each lane over the candidates it tested.
+template <int D, typename Real, int Lanes, bool WithSweep1>
+__global__ void __launch_bounds__(256)
+    buildNeighbors(const RunDev* __restrict__ runs, const int32_t* __restrict__ runOf,
+                   const int32_t* __restrict__ order, const int32_t* __restrict__ start, Records<Real> rec,
+                   NeighborList list, SweepOut<Real> out, Clock clock, const double* __restrict__ y64,
+                   int32_t resident, int64_t n) {
+    const int64_t t = xcdBlock() * blockDim.x + threadIdx.x;
+    const int64_t s = t / Lanes;
+    const int lane = (int)(t % Lanes);
+    const int groupBase = (int)(threadIdx.x & 63) & ~(Lanes - 1);
+    bool valid = s < n;
+    int r = 0;
+    if (valid) {
+        r = runOf[s];
+        valid = !resident || clock.active[r];
+    }
+    Sweep1<D, Real> sweep;
+    sweep.mode = 0;
+    bool fluidI = false;
+    if (valid) {
+        const RunDev& run = runs[r];
+        int cx, cy, cz;
+        const Self<D, Real> me = loadSelf<D, Real>(run, rec, s, cx, cy, cz);
+        const bool uniform = run.uniform != 0;
+        PairKernel<Real, D> kernel;
+        kernel.init(run.kernel, run.sigmaUnit, run.hmax, uniform);
+        fluidI = me.fluid;
+        sweep.begin(run, rec, s, WithSweep1 ? sweep1Mode(run, fluidI, resident, 1, 1) : 0);
+        const double yy = run.period > 0.0 ? y64[order[s]] : 0.0;
+        const Real period = (Real)run.period;
+        const Real twoHmax = Real(2) * (Real)run.hmax;
+        const uint32_t groupMask = Lanes >= 32 ? 0xffffffffu : ((1u << Lanes) - 1u);
+        int images = 1;
+        if (run.period > 0.0 && (me.p[1] < me.reach || me.p[1] > period - me.reach)) images = 2;
+        int32_t count = 0;
+        for (int image = 0; image < images; ++image) {
+            Real shift = Real(0);
+            int oy = cy;
+            if (image == 1) {
+                shift = me.p[1] < Real(0.5) * period ? period : -period;
+                oy = wrapIndex(cellCoordinate(yy + (double)shift, run.cell), run.ny);
+            }
+            const int32_t tag = image ? kImageBit : 0;
+            forStencil<D>(run, start, cx, oy, cz, [&](int32_t begin, int32_t end) {
+                for (int32_t base = begin; base < end; base += Lanes) {
+                    const int32_t k = base + lane;
+                    bool hit = false, fluidJ = false;
+                    Real dx = Real(0), dy = Real(0), dz = Real(0), rr = Real(0), h = Real(0);
+                    if (k < end && k != s) {
+                        const Quad<Real> a = rec.r[k].a;
+                        separation<D, Real>(me, a, rec.r[k].b, shift, dx, dy, dz);
+                        rr = pairDistance(dx, dy, dz, D);
+                        fluidJ = a.w > Real(0);
+                        h = uniform ? (Real)run.hmax : Real(0.5) * (me.h + fabs(a.w));
+                        hit = rr < (uniform ? twoHmax : Real(2) * h);
+                    }
+                    const uint32_t mine = (uint32_t)(__ballot(hit) >> groupBase) & groupMask;
+                    if (hit) {
+                        const int32_t slot = count + __popc(mine & ((1u << lane) - 1u));
+                        if (slot < list.capacity) list.index[(int64_t)slot * list.stride + s] = k | tag;
+                        if (WithSweep1) sweep.add(rec, kernel, k, fluidJ, dx, dy, dz, rr, h);
+                    }
+                    count += __popc(mine);
+                }
+            });
+        }
+        if (lane == 0) list.count[s] = count;
+    }
+    if (!WithSweep1) return;
+#pragma unroll
+    for (int e = 0; e < Sweep1<D, Real>::K; ++e) sweep.acc[e] = groupSum<Lanes>(sweep.acc[e]);
+    if (!valid || lane != 0) return;
+    sweep1Finish<D, Real>(runs[r], sweep, rec, out, s, order[s], fluidI, resident, 1, 1);
 }
 
 // Sweep 1 on its own: wall pressure on boundary particles, G on fluid particles, over the cells.
@@ -603,18 +595,17 @@ __global__ void __launch_bounds__(256)
     }
     if (valid) {
         const RunDev& run = runs[r];
-        fluidI = rec.fluid[s] != 0;
-        sweep.begin(run, rec, s, sweep1Mode<Real>(run, fluidI, resident, doWall, doGradients));
+        int cx, cy, cz;
+        const Self<D, Real> me = loadSelf<D, Real>(run, rec, s, cx, cy, cz);
+        fluidI = me.fluid;
+        sweep.begin(run, rec, s, sweep1Mode(run, fluidI, resident, doWall, doGradients));
         if (sweep.mode) {
-            int cx, cy, cz;
-            const Self<D, Real> me = loadSelf<D, Real>(run, rec, s, cx, cy, cz);
             PairKernel<Real, D> kernel;
             kernel.init(run.kernel, run.sigmaUnit, run.hmax, run.uniform != 0);
             const double yy = run.period > 0.0 ? y64[order[s]] : 0.0;
             forNeighbors<D, Real, Lanes, false>(run, start, rec, s, me, cx, cy, cz, yy, lane, kernel.uniform,
-                                                [&](int32_t k, Real dx, Real dy, Real dz, Real rr, Real h) {
-                                                    sweep.add(rec, kernel, k, dx, dy, dz, rr, h);
-                                                });
+                                                [&](int32_t k, bool fluidJ, Real dx, Real dy, Real dz, Real rr,
+                                                    Real h) { sweep.add(rec, kernel, k, fluidJ, dx, dy, dz, rr, h); });
         }
     }
     (void)list;
@@ -630,8 +621,8 @@ template <int D, typename Real, int Lanes>
 __global__ void __launch_bounds__(256)
     sweepAccelerations(const RunDev* __restrict__ runs, const int32_t* __restrict__ runOf,
                        const int32_t* __restrict__ order, const int32_t* __restrict__ start, Records<Real> rec,
-                       NeighborList list, SweepOut<Real> out, Clock clock, const double* __restrict__ y64, int32_t resident,
-                       int32_t doAcceleration, int32_t doCorrection, int32_t hostLimiter, int64_t n) {
+                       NeighborList list, SweepOut<Real> out, Clock clock, const double* __restrict__ y64,
+                       int32_t resident, int32_t doAcceleration, int32_t doCorrection, int32_t hostLimiter, int64_t n) {
     const int64_t t = xcdBlock() * blockDim.x + threadIdx.x;
     const int64_t s = t / Lanes;
     const int lane = (int)(t % Lanes);
@@ -647,34 +638,34 @@ __global__ void __launch_bounds__(256)
     }
     if (activeRun) {
         const RunDev& run = runs[r];
-        fluidI = rec.fluid[s] != 0;
         correction = doCorrection && (!resident || run.correction);
         const bool limiter = resident ? run.limiter != 0 : hostLimiter != 0;
+        int cx, cy, cz;
+        const Self<D, Real> me = loadSelf<D, Real>(run, rec, s, cx, cy, cz);
+        fluidI = me.fluid;
         if (fluidI) {
-            int cx, cy, cz;
-            const Self<D, Real> me = loadSelf<D, Real>(run, rec, s, cx, cy, cz);
             PairKernel<Real, D> kernel;
             kernel.init(run.kernel, run.sigmaUnit, run.hmax, run.uniform != 0);
             const double yy = run.period > 0.0 ? y64[order[s]] : 0.0;
-            Real vi[3] = {};
-#pragma unroll
-            for (int a = 0; a < D; ++a) vi[a] = rec.vel[a][s];
-            const Real rhoI = rec.rho[s];
-            const Real own = rec.pr2[s];
+            const Quad<Real> ci = rec.r[s].c;
+            const Real vi[3] = {ci.x, ci.y, ci.z};
+            const Real rhoI = ci.w;
+            const Real own = rec.r[s].b.w;
             Real Gi[3][3] = {};
             if (limiter && doAcceleration)
 #pragma unroll
                 for (int a = 0; a < D; ++a)
 #pragma unroll
-                    for (int b = 0; b < D; ++b) Gi[a][b] = rec.G[(a * D + b) * rec.stride + s];
+                    for (int b = 0; b < D; ++b) Gi[a][b] = rec.G[s * (D * D) + a * D + b];
             const Real alphaC0 = (Real)(run.alpha * run.c0), bf = (Real)run.boundaryFactor;
             forEachNeighbor<D, Real, Lanes, true>(
                 run, start, rec, list, s, me, cx, cy, cz, yy, lane, kernel.uniform,
-                [&](int32_t k, Real dx, Real dy, Real dz, Real rr, Real h) {
+                [&](int32_t k, bool fluidJ, Real dx, Real dy, Real dz, Real rr, Real h) {
                     const Real F = kernel.slopeOverR(rr, h);
                     const Real d[3] = {dx, dy, dz};
+                    const Quad<Real> ej = rec.r[k].e;
                     if (correction) {
-                        const Real w = -rec.vol[k] * F;
+                        const Real w = -ej.y * F;
 #pragma unroll
                         for (int a = 0; a < D; ++a) {
                             const Real wa = w * d[a];
@@ -683,26 +674,28 @@ __global__ void __launch_bounds__(256)
                         }
                     }
                     if (!doAcceleration) return;
-                    const bool fluidJ = rec.fluid[k] != 0;
+                    const Quad<Real> cj = rec.r[k].c;
+                    const Real vj[3] = {cj.x, cj.y, cj.z};
                     Real dv[3] = {};
                     Real approach = Real(0);
 #pragma unroll
                     for (int a = 0; a < D; ++a) {
-                        dv[a] = vi[a] - rec.vel[a][k];
+                        dv[a] = vi[a] - vj[a];
                         approach += dv[a] * d[a];
                     }
-                    const Real rhoJ = rec.rho[k];
+                    const Real rhoJ = cj.w;
                     Real viscosity = Real(0);
                     if (approach < Real(0) && limiter && fluidJ) {
                         Real rampI[3] = {}, rampJ[3] = {};
                         Real slopeI = Real(0), slopeJ = Real(0);
+                        const Real* Gj = rec.G + (int64_t)k * (D * D);
 #pragma unroll
                         for (int a = 0; a < D; ++a) {
                             Real ti = Real(0), tj = Real(0);
 #pragma unroll
                             for (int b = 0; b < D; ++b) {
                                 ti += Gi[a][b] * d[b];
-                                tj += rec.G[(a * D + b) * rec.stride + k] * d[b];
+                                tj += Gj[a * D + b] * d[b];
                             }
                             rampI[a] = ti;
                             rampJ[a] = tj;
@@ -730,7 +723,7 @@ __global__ void __launch_bounds__(256)
                         viscosity = quotient(-alphaC0 * mu, Real(0.5) * (rhoI + rhoJ));
                         if (!fluidJ) viscosity *= bf;
                     }
-                    const Real coefficient = -(own + rec.pr2[k] + viscosity) * rec.mass[k] * F;
+                    const Real coefficient = -(own + rec.r[k].b.w + viscosity) * ej.x * F;
 #pragma unroll
                     for (int a = 0; a < D; ++a) acc[a] += coefficient * d[a];
                 });
@@ -752,7 +745,7 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
                     for (int b = 0; b < D; ++b) {
                         const Real e = a == b ? Real(1) : Real(0);
-                        rec.L[(a * D + b) * rec.stride + s] = e;
+                        rec.L[s * (D * D) + a * D + b] = e;
                         if (out.corrections) out.corrections[(a * D + b) * out.stride + i] = e;
                     }
         } else {
@@ -783,7 +776,7 @@ __global__ void __launch_bounds__(256)
                 for (int a = 0; a < D; ++a)
 #pragma unroll
                     for (int b = 0; b < D; ++b) {
-                        rec.L[(a * D + b) * rec.stride + s] = Lm[a][b];
+                        rec.L[s * (D * D) + a * D + b] = Lm[a][b];
                         if (out.corrections) out.corrections[(a * D + b) * out.stride + i] = Lm[a][b];
                     }
             }
@@ -792,8 +785,8 @@ __global__ void __launch_bounds__(256)
     if (resident && doAcceleration) waveMaxInto(clock.maxima + 1, r, activeRun && fluidI && lane == 0, magnitude);
 }
 
-// Clears what a neighbor-free run would leave stale and applies v += a dt on the fluid, refreshing
-// the float32 velocity records; the run's first lane advances its clock.
+// Applies v += a dt on the fluid (sphGpu.py:1084-1109), refreshing the velocity records; the run's
+// first lane advances its clock.
 template <int D, typename Real>
 __global__ void kick(const RunDev* __restrict__ runs, const int32_t* __restrict__ runOf, const int32_t* __restrict__ order,
                      Records<Real> rec, const Real* __restrict__ acceleration, double* const* __restrict__ vel64p,
@@ -809,15 +802,20 @@ __global__ void kick(const RunDev* __restrict__ runs, const int32_t* __restrict_
         clock.time[r] += dt;
         clock.steps[r] += 1;
     }
-    if (!rec.fluid[s]) return;
+    if (!(rec.r[s].a.w > Real(0))) return;
     const int32_t i = order[s];
+    Real v[3] = {Real(0), Real(0), Real(0)};
 #pragma unroll
     for (int a = 0; a < D; ++a) {
-        double* v = vel64p[a];
-        const double value = v[i] + (double)acceleration[a * stride + i] * dt;
-        v[i] = value;
-        rec.vel[a][s] = (Real)value;
+        double* u = vel64p[a];
+        const double value = u[i] + (double)acceleration[a * stride + i] * dt;
+        u[i] = value;
+        v[a] = (Real)value;
     }
+    Quad<Real>& c = rec.r[s].c;
+    c.x = v[0];
+    c.y = v[1];
+    if (D == 3) c.z = v[2];
 }
 
 // The bed's boundary enforcement on one fluid particle (sphGpu.py:1197-1229, 1246-1310,
@@ -913,7 +911,7 @@ __global__ void __launch_bounds__(256)
     const int64_t s = t / Lanes;
     const int lane = (int)(t % Lanes);
     Real total = Real(0);
-    bool activeRun = false;
+    bool activeRun = false, boundaryI = false;
     int r = 0;
     if (s < n) {
         r = runOf[s];
@@ -921,36 +919,37 @@ __global__ void __launch_bounds__(256)
     }
     if (activeRun) {
         const RunDev& run = runs[r];
-        const bool boundaryI = rec.fluid[s] == 0;
+        int cx, cy, cz;
+        const Self<D, Real> me = loadSelf<D, Real>(run, rec, s, cx, cy, cz);
+        boundaryI = !me.fluid;
         const int diffusion = resident ? run.diffusion : hostDiffusion;
         const bool correction = (resident ? run.correction != 0 : hostCorrection != 0) && !boundaryI;
         const bool diffuses = diffusion != 0 && !boundaryI;
-        int cx, cy, cz;
-        const Self<D, Real> me = loadSelf<D, Real>(run, rec, s, cx, cy, cz);
         PairKernel<Real, D> kernel;
         kernel.init(run.kernel, run.sigmaUnit, run.hmax, run.uniform != 0);
         const double yy = run.period > 0.0 ? out.pos[1][order[s]] : 0.0;
-        Real vi[3] = {};
-#pragma unroll
-        for (int a = 0; a < D; ++a) vi[a] = rec.vel[a][s];
+        const Quad<Real> ci = rec.r[s].c;
+        const Real vi[3] = {ci.x, ci.y, ci.z};
         Real Li[3][3] = {};
         if (correction) {
 #pragma unroll
             for (int a = 0; a < D; ++a)
 #pragma unroll
-                for (int b = 0; b < D; ++b) Li[a][b] = rec.L[(a * D + b) * rec.stride + s];
+                for (int b = 0; b < D; ++b) Li[a][b] = rec.L[s * (D * D) + a * D + b];
         }
-        const Real devI = rec.dev[s];
+        const Real devI = rec.r[s].e.z;
         const Real factor = (Real)(2.0 * run.delta * run.c0);
-        const Real rho0 = (Real)run.rho0, invRho0 = (Real)(1.0 / run.rho0);
+        const Real invRho0 = (Real)(1.0 / run.rho0);
         const Real head = (Real)(run.rho0 * fabs(run.gravity[D - 1]) / (run.c0 * run.c0));
         forEachNeighbor<D, Real, Lanes, true>(
             run, start, rec, list, s, me, cx, cy, cz, yy, lane, kernel.uniform,
-            [&](int32_t k, Real dx, Real dy, Real dz, Real rr, Real h) {
-                const bool fluidJ = rec.fluid[k] != 0;
+            [&](int32_t k, bool fluidJ, Real dx, Real dy, Real dz, Real rr, Real h) {
                 if (boundaryI && !fluidJ) return;
                 const Real F = kernel.slopeOverR(rr, h);
                 const Real d[3] = {dx, dy, dz};
+                const Quad<Real> cj = rec.r[k].c;
+                const Quad<Real> ej = rec.r[k].e;
+                const Real vj[3] = {cj.x, cj.y, cj.z};
                 Real dot = Real(0);
                 if (correction) {
 #pragma unroll
@@ -958,15 +957,15 @@ __global__ void __launch_bounds__(256)
                         Real ld = Real(0);
 #pragma unroll
                         for (int b = 0; b < D; ++b) ld += Li[a][b] * d[b];
-                        dot += (vi[a] - rec.vel[a][k]) * ld;
+                        dot += (vi[a] - vj[a]) * ld;
                     }
                 } else {
 #pragma unroll
-                    for (int a = 0; a < D; ++a) dot += (vi[a] - rec.vel[a][k]) * d[a];
+                    for (int a = 0; a < D; ++a) dot += (vi[a] - vj[a]) * d[a];
                 }
-                total += rec.mass[k] * dot * F;
+                total += ej.x * dot * F;
                 if (diffuses && fluidJ) {
-                    const Real devJ = rec.dev[k];
+                    const Real devJ = ej.z;
                     Real difference = devJ - devI;
                     if (diffusion == 2) {
                         const Real ratio = Real(1) + Real(0.5) * (devI + devJ) * invRho0;
@@ -975,16 +974,14 @@ __global__ void __launch_bounds__(256)
                     }
                     const Real soft = kernel.uniform ? kernel.soft : Real(0.01) * h * h;
                     const Real r2 = rr * rr;
-                    total += factor * h * rec.vol[k] * difference * quotient(-F * r2, r2 + soft);
+                    total += factor * h * ej.y * difference * quotient(-F * r2, r2 + soft);
                 }
             });
-        (void)rho0;
     }
     total = groupSum<Lanes>(total);
     if (!activeRun || lane != 0) return;
     const RunDev& run = runs[r];
     const int32_t i = order[s];
-    const bool boundaryI = rec.fluid[s] == 0;
     if (!resident) {
         out.rate[i] = total;
         return;
@@ -1041,7 +1038,7 @@ __global__ void __launch_bounds__(256)
     int32_t count = 0;
     const int64_t base = rows ? offsets[i] : 0;
     forEachNeighbor<D, Real, 1, false>(run, start, rec, list, s, me, cx, cy, cz, yy, 0, run.uniform != 0,
-                                       [&](int32_t k, Real, Real, Real, Real, Real) {
+                                       [&](int32_t k, bool, Real, Real, Real, Real, Real) {
                                            if (rows) rows[base + count] = order[k];
                                            ++count;
                                        });
@@ -1174,29 +1171,10 @@ double sigmaUnitOf(int kernel, int dims) {
 template <typename Real>
 int allocateRecords(sph_context* ctx, Records<Real>& rec, int64_t cap) {
     const int D = ctx->dims;
-    int status = SPH_OK;
-    for (int a = 0; a < 3; ++a) {
-        rec.pos[a] = nullptr;
-        rec.lo[a] = nullptr;
-        rec.vel[a] = nullptr;
-    }
-    for (int a = 0; a < D && status == SPH_OK; ++a) {
-        status = allocate(ctx, &rec.pos[a], cap);
-        if (status == SPH_OK && sizeof(Real) == 4) status = allocate(ctx, &rec.lo[a], cap);
-        if (status == SPH_OK) status = allocate(ctx, &rec.vel[a], cap);
-    }
-    if (status == SPH_OK) status = allocate(ctx, &rec.rho, cap);
-    if (status == SPH_OK) status = allocate(ctx, &rec.press, cap);
-    if (status == SPH_OK) status = allocate(ctx, &rec.pr2, cap);
-    if (status == SPH_OK) status = allocate(ctx, &rec.vol, cap);
-    if (status == SPH_OK) status = allocate(ctx, &rec.dev, cap);
-    if (status == SPH_OK) status = allocate(ctx, &rec.mass, cap);
-    if (status == SPH_OK) status = allocate(ctx, &rec.len, cap);
-    if (status == SPH_OK) status = allocate(ctx, &rec.fluid, cap);
+    int status = allocate(ctx, &rec.r, cap);
     if (status == SPH_OK) status = allocate(ctx, &rec.key, cap);
     if (status == SPH_OK) status = allocate(ctx, &rec.G, cap * D * D);
     if (status == SPH_OK) status = allocate(ctx, &rec.L, cap * D * D);
-    rec.stride = cap;
     return status;
 }
 
