@@ -1,19 +1,26 @@
 '''
 
-Throughput of the MI355X delta-SPH step: fluid particle-updates per second on one GPU.
+Throughput of the MI355X delta-SPH step: fluid particle-updates per second on one GPU, batched
+float32 pairs against the plain float64 kernel.
 
-The workload is the single 2D run of the benchmark's second configuration: a first-order solitary
-wave over a plane beach at 64 particles per depth, H/d and slope drawn from seed 0, Wendland C2 at
-h = 1.3 dx, c0 = 10 sqrt(g d), the delta-SPH scheme (hydrostatic diffusion with delta = 0.1,
-extrapolated walls, L-corrected continuity, limited viscosity with alpha = 0.01, density floor),
-synthetic input laid as the reference lays it. A step is one full device-resident step: cell build,
-the three pair sweeps, kick, density advance, drift and bed enforcement, with the time step reduced
-on the device. Steps are replayed as HIP graphs of `--chunk` steps; inputs are resident in HBM
-before the timed region starts.
+The headline workload is the batched sweep: 256 independent 2D runs (seeds 0..255: slope
+U[1/35, 1/8], H/d U[0.2, 0.6], 16, 24, 32 or 48 particles per depth, length U[20, 30] d), each a
+first-order solitary wave over a plane beach with the delta-SPH scheme (Wendland C2 at h = 1.3 dx,
+c0 = 10 sqrt(g d), hydrostatic diffusion with delta = 0.1, extrapolated walls, L-corrected
+continuity, limited viscosity with alpha = 0.01, density floor), packed into one context and
+stepped by the same graph-replayed launches, with the launch shape chosen by the autotuner. A step
+is one full device-resident step of every run: cell build, the three pair sweeps, kick, density
+advance, drift and bed enforcement, each run with its own time step reduced on the device. Inputs
+are resident in HBM before the timed region starts; one particle-update is one fluid particle
+advanced one step.
 
-Beside the headline the line carries the plain float64 kernel's rate on the same run, the
-dominant kernel's roofline from HIP events recorded around its launches during a profiled pass,
-and the float64 CPU restatement timed on the host's cores on a bounded sample of the same run.
+Beside the headline the line carries:
+  - the plain float64 kernel stepping a sample of the same runs, each alone, one after another;
+  - the single 64-particles-per-depth run (float32 pairs and plain float64);
+  - the dominant kernel's roofline, from HIP events on the engine's stream around each launch of a
+    stream-launched pass over the same batch right after the timed region;
+  - the float64 CPU restatement (oracle/) timed on the host's cores on a bounded sample of the
+    sweep's runs.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 
@@ -23,8 +30,8 @@ from __future__ import annotations
 
 import argparse
 import json
-import math
 import os
+import statistics
 import sys
 import time
 
@@ -35,49 +42,77 @@ if ROOT not in sys.path:
 METRIC = 'particle-updates/s on 1 MI355X, batched FP32 vs plain FP64; run p50/p99 latency'
 HBM_PEAK_GBS = 8000.0
 
-# Algorithmic bytes per particle of each kernel: the particle's own inputs read once and its outputs
-# written once (neighbors' records are reused from cache and not counted), by dimension and pair
-# precision. Counted from the kernels in pysurf_amd/csrc/sph_hip.hip.
-def kernelBytes(dimensions: int, real: int) -> dict:
-    d, r = dimensions, real
-    lo = d * 4 if r == 4 else 0                                        # float32 remainders
-    record = d * r + lo + d * r + 5 * r + 1 + 4                        # pos, lo, vel, rho, press, dev, mass, len, fluid, key
-    index = 8                                                          # run id and sorted order
+# HBM bytes per launch of float32 kernels on the 256-run batched sweep (5,073,680 particles), keyed by
+# (kernel, lanes per particle, particles): rocprofv3 FETCH_SIZE and WRITE_SIZE passes, each scaled by
+# what the counter reads for a 1 GiB device copy (x 2.000 and x 1.000), as recorded in
+# profiles/batched_sweep_256runs_counters.md. A kernel, shape or size without a measurement has none.
+MEASURED_TRAFFIC = {
+    ('sweepAccelerations', 2, 5073680): 9.7034e+08,
+    ('buildNeighbors', 2, 5073680): 1.1358e+09,
+    ('sweepContinuity', 2, 5073680): 1.2554e+09,
+    ('gatherRecords', 2, 5073680): 7.6598e+08,
+}
+
+
+def kernelBytes(dimensions: int, real: int, entries: float) -> dict:
+
+    '''
+
+    Algorithmic bytes per particle of each kernel of the step: the particle's own inputs read once
+    and its outputs written once, with `entries` neighbor-list entries per particle (its mean).
+    Neighbor records read through the caches are not counted. Counted from the kernels in
+    pysurf_amd/csrc/sph_hip.hip: a record is four quads of `real`-byte values.
+
+    '''
+
+    d, r, k = dimensions, real, entries
+    record = 16 * r
+    ids = 8                                      # run id and sorted order
     return {
-        'gatherRecords': index + d * 8 * 2 + 8 + 2 * r + 1 + 4 + record + r,
-        'sweepWallAndGradients': index + 1 + d * r + lo + r + 4 + d * r + 2 * r + d * d * r,
-        'sweepAccelerations': index + 1 + d * r + lo + r + 4 + d * r + 3 * r + d * d * r + d * r + d * d * r,
-        'kick': index + 1 + d * r + 2 * d * 8 + d * r,
-        'sweepContinuity': index + 1 + d * r + lo + r + 4 + d * r + r + d * d * r + 2 * 8 + 4 * d * 8,
-        'buildNeighbors': index + d * r + lo + r + 4 + 4,
+        'cellKeys': 8 * d + 4 + 4 + 4,           # position; run id; key; the cell count's atomic
+        'cellScatter': 4 + 4 + 4 + 4,            # key; cell start; the fill atomic; order
+        'cellOrder': 8,                          # the order read and written in place
+        'gatherRecords': ids + 16 * d + 8 + 2 * r + 1 + 4 + record + 4 + r,
+        'buildNeighbors': ids + record + 4 + 4 + 4 * k + d * d * r,
+        'sweepAccelerations': ids + record + 4 + d * d * r + 4 + 4 * k + d * r + d * d * r,
+        'kick': ids + 16 + 16 * d + d * r + 4 * r,
+        'sweepContinuity': ids + record + 4 + d * d * r + 4 + 4 * k + 16 + 32 * d,
     }
 
 
 def parse():
     parser = argparse.ArgumentParser(description = __doc__.split('\n\n')[1])
     parser.add_argument('--gpus', type = int, default = 1)
-    parser.add_argument('--steps', type = int, default = 2000)
-    parser.add_argument('--warmup', type = int, default = 100)
+    parser.add_argument('--steps', type = int, default = 500, help = 'timed steps of the batched sweep')
+    parser.add_argument('--warmup', type = int, default = 50)
     parser.add_argument('--chunk', type = int, default = 50, help = 'steps per captured graph')
-    parser.add_argument('--fp64-steps', type = int, default = 400)
-    parser.add_argument('--profile-steps', type = int, default = 50)
-    parser.add_argument('--cpu-seconds', type = float, default = 15.0)
-    parser.add_argument('--seed', type = int, default = 0)
-    parser.add_argument('--no-cpu', action = 'store_true', help = 'skip the CPU baseline')
-    parser.add_argument('--no-batched', action = 'store_true', help = 'skip the batched sweep')
     parser.add_argument('--runs', type = int, default = 256, help = 'runs of the batched sweep')
-    parser.add_argument('--batched-steps', type = int, default = 200)
+    parser.add_argument('--no-tune', action = 'store_true', help = 'the default launch shape instead of the tuned one')
+    parser.add_argument('--lanes', type = int, default = 0, help = 'lanes per particle of the batched sweep (0: tuned)')
+    parser.add_argument('--block', type = int, default = 256, help = 'workgroup threads with --lanes')
+    parser.add_argument('--profile-steps', type = int, default = 20, help = 'stream-launched steps timed per kernel')
     parser.add_argument('--plain-sample', type = int, default = 16, help = 'runs the plain float64 kernel steps alone')
     parser.add_argument('--plain-steps', type = int, default = 200)
+    parser.add_argument('--single-steps', type = int, default = 1000, help = 'steps of the single 64 ppd run')
+    parser.add_argument('--no-single', action = 'store_true', help = 'skip the single 64 ppd run')
+    parser.add_argument('--cpu-seconds', type = float, default = 15.0)
+    parser.add_argument('--no-cpu', action = 'store_true', help = 'skip the CPU baseline')
+    parser.add_argument('--copy-calibration', type = int, default = 0, metavar = 'MIB',
+                        help = 'only copy a MIB-mebibyte device buffer 10 times (the counter calibration) and exit')
     return parser.parse_args()
 
 
-def timedSteps(batch, steps: int, chunk: int) -> float:
-    '''Seconds for `steps` graph-replayed steps, between events on the engine's stream.'''
+def log(message: str) -> None:
+    print(f'# {message}', file = sys.stderr, flush = True)
+
+
+def timedSteps(batch, steps: int, chunk: int) -> tuple[float, float]:
+    '''Wall seconds between device synchronizations, and event seconds on the engine's stream, for `steps` graph-replayed steps.'''
     import torch
     stream = batch.engine.stream
-    torch.cuda.synchronize()
     start, end = torch.cuda.Event(enable_timing = True), torch.cuda.Event(enable_timing = True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter()
     start.record(stream)
     done = 0
     while done < steps:
@@ -86,20 +121,18 @@ def timedSteps(batch, steps: int, chunk: int) -> float:
         done += take
     end.record(stream)
     torch.cuda.synchronize()
-    return start.elapsed_time(end) / 1000.0
+    wall = time.perf_counter() - wall
+    return wall, start.elapsed_time(end) / 1000.0
 
 
-def prepare(case, precision: str, steps: int, chunk: int, warmup: int):
-    from pysurf_amd.batch import BatchEngine
-    batch = BatchEngine([case], precision)
+def warm(batch, steps: int, chunk: int) -> None:
     batch.setClocks(maxSteps = 2**62)
     done = 0
-    while done < warmup:
-        take = min(chunk, warmup - done)
+    while done < steps:
+        take = min(chunk, steps - done)
         batch.advance(take)
         done += take
     batch.synchronize()
-    return batch
 
 
 def kernelProfile(batch, steps: int) -> dict:
@@ -121,83 +154,150 @@ def kernelProfile(batch, steps: int) -> dict:
     return {name: {'ms': float(ms[k]), 'launches': int(launches[k])} for k, name in enumerate(_native.kernelNames)}
 
 
-def batchedSweep(args) -> dict:
+def meanNeighbors(batch) -> float:
+    '''Mean neighbor count per particle of the batch's current state.'''
+    import ctypes
 
-    '''
+    import torch
+    engine = batch.engine
+    counts = torch.zeros(max(batch.count, 1), dtype = torch.int32, device = batch.device)
+    engine.check(engine.library.sph_neighbor_counts(engine.handle, ctypes.c_void_p(counts.data_ptr())),
+                 'sph_neighbor_counts')
+    return float(counts[:batch.count].double().mean().item())
 
-    The batched sweep: `args.runs` uneven 2D runs (slope, height, resolution and length drawn per
-    seed) packed into one context and stepped by the same graph-replayed launches, against the plain
-    float64 kernel stepping a sample of the same runs, each alone, one after another.
 
-    '''
+def roofline(batch, profileSteps: int, precision: str) -> dict:
+    '''The dominant kernel's algorithmic bytes over its mean launch time, from a profiled pass.'''
+    profile = kernelProfile(batch, profileSteps)
+    entries = meanNeighbors(batch)
+    table = kernelBytes(batch.dimensions, 4 if precision == 'float32' else 8, entries)
+    dominant = max((name for name in profile if profile[name]['launches'] and name in table),
+                   key = lambda name: profile[name]['ms'])
+    launchMs = profile[dominant]['ms'] / profile[dominant]['launches']
+    perParticle = table[dominant]
+    traffic = perParticle * batch.count
+    achieved = traffic / (launchMs / 1000.0) / 1e9
+    stepBytes = sum(table.values()) * batch.count
+    stepMs = sum(entry['ms'] for entry in profile.values()) / profileSteps
+    log(f'dominant kernel {dominant}: {launchMs * 1000:.1f} us/launch, {achieved:.1f} GB/s of algorithmic bytes')
+    measured = MEASURED_TRAFFIC.get((dominant, batch.launch.get('lanes'), batch.count)) if precision == 'float32' else None
+    return {'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS,
+            'traffic': measured, 'kernel': dominant, 'launch_us': launchMs * 1000.0,
+            'algorithmic_bytes_per_launch': traffic, 'bytes_per_particle': perParticle,
+            'mean_list_entries': entries, 'particles_per_launch': batch.count,
+            'step_algorithmic_bytes_per_particle': sum(table.values()),
+            'step_achieved_GBps': stepBytes / (stepMs / 1000.0) / 1e9, 'profiled_step_ms': stepMs,
+            'kernel_us_per_step': {name: 1000.0 * entry['ms'] / profileSteps for name, entry in profile.items()
+                                   if entry['launches']}}
 
-    import numpy as np
 
+def plainSample(cases, steps: int, chunk: int, tune: bool) -> dict:
+    '''The plain float64 kernel stepping each run alone in its own context, one after another.'''
     from pysurf_amd.batch import BatchEngine
-    from pysurf_amd.scenarios import sweepRun
-    cases = [sweepRun(seed) for seed in range(args.runs)]
-    fluid = sum(case.fluidCount for case in cases)
-    particles = sum(case.count for case in cases)
-    print(f'# batched: {len(cases)} runs, {particles} particles ({fluid} fluid)', file = sys.stderr, flush = True)
-    batch = BatchEngine(cases, 'float32')
-    batch.setClocks(maxSteps = 2**62)
-    batch.advance(args.chunk)
-    batch.synchronize()
-    seconds = timedSteps(batch, args.batched_steps, args.chunk)
-    value = fluid * args.batched_steps / seconds
-    clock = batch.clocks()
-    del batch
-    print(f'# batched float32: {1000 * seconds / args.batched_steps:.3f} ms/step, {value:.4e} fluid updates/s',
-          file = sys.stderr, flush = True)
     updates, elapsed = 0, 0.0
-    sample = cases[:args.plain_sample]
-    for case in sample:
-        solo = BatchEngine([case], 'float64')
-        solo.setClocks(maxSteps = 2**62)
-        solo.advance(args.chunk)
-        solo.synchronize()
-        elapsed += timedSteps(solo, args.plain_steps, args.chunk)
-        updates += case.fluidCount * args.plain_steps
+    for case in cases:
+        solo = BatchEngine([case], 'float64', tune = tune)
+        warm(solo, chunk, chunk)
+        elapsed += timedSteps(solo, steps, chunk)[0]
+        updates += case.fluidCount * steps
         del solo
-    plain = updates / elapsed
-    print(f'# plain float64, {len(sample)} runs alone: {plain:.4e} fluid updates/s', file = sys.stderr, flush = True)
-    sizes = np.array([case.count for case in cases])
-    return {'value': value, 'unit': 'particle-updates/s', 'runs': len(cases), 'particles': particles,
-            'fluid_particles': fluid, 'run_particles_min': int(sizes.min()), 'run_particles_max': int(sizes.max()),
-            'steps': args.batched_steps, 'ms_per_step': 1000.0 * seconds / args.batched_steps,
-            'simulated_seconds_min': float(clock['time'].min()), 'simulated_seconds_max': float(clock['time'].max()),
-            'plain_fp64': {'value': plain, 'unit': 'particle-updates/s', 'runs_sampled': len(sample),
-                           'steps_each': args.plain_steps, 'how': 'each run alone in its own context, one after another'},
-            'speedup_over_plain_fp64': value / plain,
-            'workload': 'batched sweep: runs seeded 0..N-1 with slope U[1/35, 1/8], H/d U[0.2, 0.6], '
-                        'resolution from {16, 24, 32, 48} particles per depth, length U[20, 30] d, float32 pairs'}
+    value = updates / elapsed
+    log(f'plain float64, {len(cases)} runs alone: {value:.4e} fluid updates/s')
+    return {'value': value, 'unit': 'particle-updates/s', 'runs_sampled': len(cases), 'steps_each': steps,
+            'how': 'each run alone in its own context, one after another, graph-replayed, tuned launch shape'}
 
 
-def cpuBaseline(case, seconds: float) -> dict:
-    '''The float64 CPU restatement on the host's cores, over as many steps of the same run as fit `seconds`.'''
+def singleRun(args) -> dict:
+    '''The single 64-particles-per-depth run: float32 pairs against the plain float64 kernel.'''
+    from pysurf_amd.batch import BatchEngine
+    from pysurf_amd.scenarios import singleRun as single
+    case = single(seed = 0, particlesPerDepth = 64)
+    result = {'workload': f'{case.name}: H/d {case.parameters["waveHeight"]:.3f}, slope 1/'
+                          f'{1.0 / case.parameters["slope"]:.1f}', 'particles': case.count,
+              'fluid_particles': case.fluidCount, 'steps': args.single_steps}
+    for precision in ('float32', 'float64'):
+        batch = BatchEngine([case], precision, tune = not args.no_tune)
+        warm(batch, args.warmup, args.chunk)
+        wall, _ = timedSteps(batch, args.single_steps, args.chunk)
+        result[precision] = {'value': case.fluidCount * args.single_steps / wall,
+                             'ms_per_step': 1000.0 * wall / args.single_steps, 'lanes': batch.launch['lanes']}
+        log(f'single run {precision}: {result[precision]["ms_per_step"]:.4f} ms/step')
+        del batch
+    result['speedup_of_f32'] = result['float32']['value'] / result['float64']['value']
+    return result
+
+
+def cpuModel() -> str:
+    try:
+        with open('/proc/cpuinfo') as f:
+            for line in f:
+                if line.startswith('model name'):
+                    return line.split(':', 1)[1].strip()
+    except OSError:
+        pass
+    return 'unknown'
+
+
+def cpuBaseline(cases, seconds: float) -> dict:
+
+    '''
+
+    The float64 CPU restatement (oracle/sph_oracle.cpp) on the host's cores: the sample's runs
+    stepped in turn after a warm-up step each, three repeats of about `seconds` / 3, the median.
+
+    '''
+
     import numpy as np
 
     from oracle import oracle
     threads = min(16, os.cpu_count() or 1)
     oracle.threads(threads)
-    p = case.particles
-    physics = oracle.Physics(dims = case.dimensions, h = case.smoothingLength, domainMin = case.domainMin,
-                             domainMax = case.domainMax, kernelType = case.kernelType, rho0 = case.referenceDensity,
-                             c0 = case.soundSpeed, alpha = case.alpha, boundaryFactor = case.boundaryViscosityFactor,
-                             gravity = case.gravity, diffusion = case.diffusion, delta = case.delta,
-                             extrapolatedWalls = case.extrapolatedWalls, correction = case.correction,
-                             limiter = case.limiter, densityFloor = case.densityFloor)
-    run = oracle.OracleRun(physics, case.boundary.description(), p.positions, p.velocities, p.densities, p.masses,
-                           p.isFluid)
-    run.advance(np.inf, 1)                      # warm-up step
-    steps, start = 0, time.perf_counter()
-    while time.perf_counter() - start < seconds or steps < 2:
+    runs = []
+    for case in cases:
+        p = case.particles
+        physics = oracle.Physics(dims = case.dimensions, h = case.smoothingLength, domainMin = case.domainMin,
+                                 domainMax = case.domainMax, kernelType = case.kernelType,
+                                 rho0 = case.referenceDensity, c0 = case.soundSpeed, alpha = case.alpha,
+                                 boundaryFactor = case.boundaryViscosityFactor, gravity = case.gravity,
+                                 diffusion = case.diffusion, delta = case.delta,
+                                 extrapolatedWalls = case.extrapolatedWalls, correction = case.correction,
+                                 limiter = case.limiter, densityFloor = case.densityFloor)
+        run = oracle.OracleRun(physics, case.boundary.description(), p.positions, p.velocities, p.densities,
+                               p.masses, p.isFluid)
         run.advance(np.inf, 1)
-        steps += 1
-    elapsed = time.perf_counter() - start
-    return {'value': case.fluidCount * steps / elapsed, 'unit': 'particle-updates/s', 'cores': threads, 'kind': 'port',
-            'sample': f'{steps} steps of the same {case.count}-particle run in the float64 C++ OpenMP restatement '
-                      f'(oracle/sph_oracle.cpp), {elapsed:.1f} s, {threads} threads'}
+        runs.append((run, case.fluidCount))
+    start = time.perf_counter()
+    for run, _ in runs:
+        run.advance(np.inf, 1)
+    perRound = max(time.perf_counter() - start, 1e-6)
+    rounds = max(1, int(seconds / 3.0 / perRound))
+    rates = []
+    for _ in range(3):
+        start = time.perf_counter()
+        for run, _ in runs:
+            run.advance(np.inf, rounds)
+        elapsed = time.perf_counter() - start
+        rates.append(sum(fluid for _, fluid in runs) * rounds / elapsed)
+    value = statistics.median(rates)
+    sizes = ', '.join(f'{case.name} ({case.count} particles)' for case in cases)
+    log(f'cpu: {value:.4e} fluid updates/s')
+    return {'value': value, 'unit': 'particle-updates/s', 'cores': threads, 'kind': 'port',
+            'sample': f'the float64 C++ OpenMP restatement (oracle/sph_oracle.cpp) stepping {sizes} '
+                      f'{rounds} steps each, median of 3 repeats',
+            'repeats': rates, 'nproc': os.cpu_count(), 'cpu_model': cpuModel()}
+
+
+def copyCalibration(mebibytes: int) -> int:
+    '''Copies a device buffer of `mebibytes` MiB ten times: the known-size kernel the HBM counters are scaled by.'''
+    import torch
+    count = mebibytes * (1 << 20) // 4
+    source = torch.ones(count, dtype = torch.float32, device = 'cuda')
+    target = torch.empty_like(source)
+    for _ in range(10):
+        target.copy_(source)
+    torch.cuda.synchronize()
+    print(json.dumps({'copy_bytes_read': count * 4, 'copy_bytes_written': count * 4, 'copies': 10}))
+    return 0
 
 
 def main() -> int:
@@ -207,50 +307,41 @@ def main() -> int:
               f'(asked for --gpus {args.gpus}).', file = sys.stderr)
         return 2
     import torch
-
-    from pysurf_amd.scenarios import singleRun
     if not torch.cuda.is_available():
         print('bench.py needs a HIP device.', file = sys.stderr)
         return 1
-    case = singleRun(seed = args.seed, particlesPerDepth = 64)
-    print(f'# workload {case.name}: {case.count} particles ({case.fluidCount} fluid)', file = sys.stderr, flush = True)
+    if args.copy_calibration:
+        return copyCalibration(args.copy_calibration)
 
-    batch = prepare(case, 'float32', args.steps, args.chunk, args.warmup)
-    print('# float32 warm-up done', file = sys.stderr, flush = True)
-    seconds = timedSteps(batch, args.steps, args.chunk)
-    value = case.fluidCount * args.steps / seconds
-    msPerStep = 1000.0 * seconds / args.steps
-    print(f'# float32: {msPerStep:.4f} ms/step, {value:.4e} fluid updates/s', file = sys.stderr, flush = True)
+    import numpy as np
 
-    profile = kernelProfile(batch, args.profile_steps)
-    table = kernelBytes(case.dimensions, 4)
-    dominant = max((name for name in profile if profile[name]['launches'] and name in table),
-                   key = lambda name: profile[name]['ms'])
-    launchMs = profile[dominant]['ms'] / profile[dominant]['launches']
-    perParticle = table[dominant]
-    traffic = perParticle * case.count
-    achieved = traffic / (launchMs / 1000.0) / 1e9
-    stepMs = sum(entry['ms'] for entry in profile.values()) / args.profile_steps
-    print(f'# dominant kernel {dominant}: {launchMs * 1000:.2f} us/launch, {achieved:.1f} GB/s', file = sys.stderr,
-          flush = True)
+    from pysurf_amd.batch import BatchEngine
+    from pysurf_amd.scenarios import sweepRun
+    cases = [sweepRun(seed) for seed in range(args.runs)]
+    fluid = sum(case.fluidCount for case in cases)
+    particles = sum(case.count for case in cases)
+    log(f'batched sweep: {len(cases)} runs, {particles} particles ({fluid} fluid)')
+    if args.lanes:
+        batch = BatchEngine(cases, 'float32', blockSize = args.block, lanes = args.lanes)
+    else:
+        batch = BatchEngine(cases, 'float32', tune = not args.no_tune)
+    log(f'launch shape: {batch.launch.get("lanes")} lanes, {batch.launch.get("block")} threads')
+    warm(batch, args.warmup, args.chunk)
+    wall, events = timedSteps(batch, args.steps, args.chunk)
+    value = fluid * args.steps / wall
+    msPerStep = 1000.0 * wall / args.steps
+    clock = batch.clocks()
+    log(f'batched float32: {msPerStep:.3f} ms/step, {value:.4e} fluid updates/s')
+    roof = roofline(batch, args.profile_steps, 'float32')
+    launch = dict(batch.launch)
     del batch
 
-    plain = prepare(case, 'float64', args.fp64_steps, args.chunk, min(args.warmup, 50))
-    fp64Seconds = timedSteps(plain, args.fp64_steps, args.chunk)
-    fp64Value = case.fluidCount * args.fp64_steps / fp64Seconds
-    print(f'# float64: {1000.0 * fp64Seconds / args.fp64_steps:.4f} ms/step, {fp64Value:.4e} fluid updates/s',
-          file = sys.stderr, flush = True)
-    del plain
+    plain = plainSample(cases[:args.plain_sample], args.plain_steps, args.chunk, not args.no_tune) \
+        if args.plain_sample > 0 else None
+    single = None if args.no_single else singleRun(args)
+    cpu = None if args.no_cpu else cpuBaseline(cases[:2], args.cpu_seconds)
 
-    batched = None
-    if not args.no_batched:
-        batched = batchedSweep(args)
-
-    cpu = None
-    if not args.no_cpu:
-        cpu = cpuBaseline(case, args.cpu_seconds)
-        print(f'# cpu: {cpu["value"]:.4e} fluid updates/s ({cpu["sample"]})', file = sys.stderr, flush = True)
-
+    sizes = np.array([case.count for case in cases])
     line = {
         'metric': METRIC,
         'value': value,
@@ -263,22 +354,22 @@ def main() -> int:
         'scaling': 'weak',
         'vs_baseline': None,
         'dtype': 'f32',
-        'data': 'synthetic: first-order solitary wave on a plane beach laid on the lattice with 1e-3 dx jitter, '
-                'seeded as the benchmark configuration states',
-        'config': {'workload': f'2D delta-SPH solitary wave on a slope, 64 particles per depth, single run '
-                               f'({case.name}: H/d {case.parameters["waveHeight"]:.3f}, slope 1/'
-                               f'{1.0 / case.parameters["slope"]:.1f})',
-                   'particles': case.count, 'fluid_particles': case.fluidCount, 'state': 'f64 positions, '
-                   'velocities, densities; f32 pair sums', 'kernel': 'Wendland C2, h = 1.3 dx',
-                   'graph_chunk': args.chunk},
-        'roofline': {'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-                     'frac': achieved / HBM_PEAK_GBS, 'traffic': None, 'kernel': dominant,
-                     'launch_us': launchMs * 1000.0, 'algorithmic_bytes_per_launch': traffic,
-                     'bytes_per_particle': perParticle, 'profiled_step_ms': stepMs,
-                     'kernel_ms_per_step': {name: entry['ms'] / args.profile_steps for name, entry in profile.items()}},
-        'plain_fp64': {'value': fp64Value, 'unit': 'particle-updates/s', 'steps': args.fp64_steps,
-                       'ms_per_step': 1000.0 * fp64Seconds / args.fp64_steps, 'speedup_of_f32': value / fp64Value},
-        'batched': batched,
+        'data': 'synthetic: first-order solitary waves on plane beaches laid on the lattice with 1e-3 dx jitter, '
+                'drawn per seed as the batched-sweep configuration states',
+        'config': {'workload': f'batched sweep of {len(cases)} 2D delta-SPH solitary-wave runs (seeds 0..'
+                               f'{len(cases) - 1}: slope U[1/35, 1/8], H/d U[0.2, 0.6], 16/24/32/48 particles per '
+                               f'depth, length U[20, 30] d) in one context, float32 pairs, float64 state',
+                   'runs': len(cases), 'particles': particles, 'fluid_particles': fluid,
+                   'run_particles_min': int(sizes.min()), 'run_particles_max': int(sizes.max()),
+                   'kernel': 'Wendland C2, h = 1.3 dx', 'graph_chunk': args.chunk,
+                   'launch': launch,
+                   'event_ms_per_step': 1000.0 * events / args.steps,
+                   'simulated_seconds_min': float(clock['time'].min()),
+                   'simulated_seconds_max': float(clock['time'].max())},
+        'roofline': roof,
+        'plain_fp64': plain,
+        'speedup_over_plain_fp64': value / plain['value'] if plain else None,
+        'single_run': single,
         'cpu_baseline': cpu,
     }
     print(json.dumps(line))

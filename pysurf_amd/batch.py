@@ -71,11 +71,14 @@ class BatchEngine:
         Lanes per particle of the sweeps; 0 picks by dimension
     listCapacity : int
         Neighbor-list entries per particle; 0 walks the cells in every sweep, -1 picks by dimension
+    tune : bool
+        Time the candidate launch shapes on these runs and keep the fastest (autotune.tune); the
+        three arguments above are then ignored
 
     '''
 
     def __init__(self, cases: list, precision: str = 'float32', device = None, blockSize: int = 256,
-                 lanes: int = 0, listCapacity: int = -1) -> None:
+                 lanes: int = 0, listCapacity: int = -1, tune: bool = False) -> None:
         if not cases:
             raise ValueError('A batch needs at least one run.')
         dimensions = {case.dimensions for case in cases}
@@ -99,6 +102,10 @@ class BatchEngine:
         self.engine.configure(runs)
         self.engine.setLaunch(blockSize, lanes, listCapacity)
         self.load()
+        self.launch = {'lanes': lanes, 'block': blockSize, 'listCapacity': listCapacity}
+        if tune:
+            from pysurf_amd import autotune
+            self.launch = autotune.tune(self)
 
     @property
     def count(self) -> int:
