@@ -144,6 +144,14 @@ int sph_advance(sph_context* ctx, int32_t steps, int32_t useGraph);
 int sph_rates(sph_context* ctx, double* maxima);
 int sph_integrate(sph_context* ctx, double timeStep);
 
+/* Each run's float64 sums over its fluid particles of the bound state, 7 per run in run order:
+ * mass, kinetic energy sum m |v|^2 / 2 and potential energy sum m g z (ParticleSystem.kineticEnergy
+ * and potentialEnergy, sphParticles.py:236-255, g the downward gravity along the last axis), the
+ * Tait elastic energy sum m e(rho) (sphEnergy.elasticEnergy, sphEnergy.py:120-160; the device form
+ * sphGpu.py:271-282) and momentum sum m v along x, y, z (z zero in 2D). The summation order is fixed,
+ * so the sums repeat bit for bit. Synchronizes. */
+int sph_global_sums(sph_context* ctx, double* sums);
+
 int sph_synchronize(sph_context* ctx);
 
 /* Per-kernel timing with HIP events recorded on the context's stream around every launch of a

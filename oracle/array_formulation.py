@@ -295,3 +295,39 @@ def gridDimensions(domainMin, domainMax, cellSize: float, dimensions: int, maxCe
     while int(np.prod(cells)) > maxCells:
         cells = np.maximum((cells * 0.8).astype(int), 1)
     return tuple(int(count) for count in cells)
+
+
+def elasticEnergy(densities, referenceDensity: float, soundSpeed: float, tension: bool = False) -> np.ndarray:
+
+    '''
+
+    The elastic energy per unit mass the Tait equation stores at a density, the integral of p / s^2
+    from rho0 to rho (sphEnergy.py:120-160): (c0^2 / 7) x^2 (21 + 35x + 35x^2 + 21x^3 + 7x^4 + x^5)
+    / (6 (1 + x)) with x = rho / rho0 - 1; zero below rho0 unless the fluid carries tension.
+
+    '''
+
+    x = np.asarray(densities, dtype = float) / referenceDensity - 1.0
+    polynomial = 21.0 + x * (35.0 + x * (35.0 + x * (21.0 + x * (7.0 + x))))
+    energy = soundSpeed**2 / 7.0 * x * x * polynomial / (6.0 * (1.0 + x))
+    return energy if tension else np.where(x < 0.0, 0.0, energy)
+
+
+def globalSums(positions, velocities, densities, masses, isFluid, gravity: float, referenceDensity: float,
+               soundSpeed: float, tension: bool = False) -> dict:
+
+    '''
+
+    Float64 sums over the fluid: mass, kinetic energy sum m |v|^2 / 2 and potential energy sum m g z
+    (sphParticles.py:236-255, z the last axis), elastic energy sum m e(rho) and momentum sum m v.
+
+    '''
+
+    fluid = np.asarray(isFluid, dtype = bool)
+    m = np.asarray(masses, dtype = float)[fluid]
+    v = np.asarray(velocities, dtype = float)[fluid]
+    return {'mass': float(m.sum()), 'kinetic': float(0.5 * np.sum(m * np.sum(v * v, axis = 1))),
+            'potential': float(np.sum(m * gravity * np.asarray(positions, dtype = float)[fluid, -1])),
+            'elastic': float(np.sum(m * elasticEnergy(np.asarray(densities)[fluid], referenceDensity, soundSpeed,
+                                                      tension))),
+            'momentum': (m[:, None] * v).sum(axis = 0)}

@@ -78,10 +78,14 @@ signatures = {
     'sph_rates': [ctypes.c_void_p, ctypes.c_void_p],
     'sph_integrate': [ctypes.c_void_p, ctypes.c_double],
     'sph_synchronize': [ctypes.c_void_p],
+    'sph_global_sums': [ctypes.c_void_p, ctypes.c_void_p],
     'sph_set_launch': [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32],
     'sph_profile': [ctypes.c_void_p, ctypes.c_int32],
     'sph_kernel_times': [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32],
 }
+
+# The columns of sph_global_sums, per run
+globalSumNames = ['mass', 'kinetic', 'potential', 'elastic', 'momentumX', 'momentumY', 'momentumZ']
 
 # Kernel ids of sph_kernel_times
 kernelNames = ['clock', 'cellKeys', 'cellScan', 'cellScatter', 'cellOrder', 'gatherRecords', 'sweepWallAndGradients',

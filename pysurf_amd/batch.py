@@ -147,6 +147,10 @@ class BatchEngine:
     def clocks(self) -> dict:
         return self.engine.clock(len(self.cases))
 
+    def globalSums(self) -> np.ndarray:
+        '''Every run's float64 sums over its fluid, shape (runs, 7), columns as _native.globalSumNames.'''
+        return self.engine.globalSums(len(self.cases))
+
     def state(self, index: int) -> ParticleSystem:
         '''Run `index`'s state as a ParticleSystem, copied from the device.'''
         a, b = self.offsets[index], self.offsets[index + 1]

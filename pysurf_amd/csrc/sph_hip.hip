@@ -1045,6 +1045,75 @@ __global__ void __launch_bounds__(256)
     if (counts) counts[i] = count;
 }
 
+// ---------------------------------------------------------------------------------------------
+// Global float64 sums
+// ---------------------------------------------------------------------------------------------
+
+constexpr int kSums = 7;              // fluid mass, kinetic, potential, elastic energy, momentum x, y, z
+constexpr int64_t kSumChunk = 8192;   // slots per partial sum
+
+// The elastic energy per unit mass the Tait equation stores at density rho (sphEnergy.py:120-160,
+// sphGpu.py:271-282): e = (c0^2 / 7) x^2 (21 + 35x + 35x^2 + 21x^3 + 7x^4 + x^5) / (6 (1 + x)) with
+// x = rho / rho0 - 1, zero below rho0 unless the fluid carries tension.
+__device__ __forceinline__ double elasticEnergy(double rho, double rho0, double scale, int tension) {
+    const double x = rho / rho0 - 1.0;
+    const double polynomial = 21.0 + x * (35.0 + x * (35.0 + x * (21.0 + x * (7.0 + x))));
+    const double energy = scale * x * x * polynomial / (6.0 * (1.0 + x));
+    return (!tension && x < 0.0) ? 0.0 : energy;
+}
+
+// One partial of every sum over chunk b's slots, fluid only, summed in a fixed order: each thread
+// over a fixed stride, then a fixed tree. K = m |v|^2 / 2 and P = m g z with g the downward
+// gravity along the last axis (sphParticles.py:236-255); masses as the device stores them.
+template <int D, typename Real>
+__global__ void __launch_bounds__(256)
+    sumChunks(const RunDev* __restrict__ runs, const int32_t* __restrict__ chunkRun,
+              const int64_t* __restrict__ chunkFirst, sph_state state, double* __restrict__ partial) {
+    __shared__ double tree[kSums][256];
+    const int64_t b = blockIdx.x;
+    const RunDev& run = runs[chunkRun[b]];
+    const int64_t first = chunkFirst[b];
+    const int64_t end = min(first + kSumChunk, run.first + run.count);
+    const double scale = run.c0 * run.c0 / 7.0;
+    const double g = -run.gravity[D - 1];
+    const Real* mass = static_cast<const Real*>(state.mass);
+    double acc[kSums] = {};
+    for (int64_t i = first + threadIdx.x; i < end; i += blockDim.x) {
+        if (!state.fluid[i]) continue;
+        const double m = (double)mass[i];
+        double v2 = 0.0;
+        for (int a = 0; a < D; ++a) {
+            const double v = state.velocity[a * state.stride + i];
+            v2 += v * v;
+            acc[4 + a] += m * v;
+        }
+        acc[0] += m;
+        acc[1] += 0.5 * m * v2;
+        acc[2] += m * g * state.position[(D - 1) * state.stride + i];
+        acc[3] += m * elasticEnergy(state.density[i], run.rho0, scale, run.negativePressure);
+    }
+    for (int e = 0; e < kSums; ++e) tree[e][threadIdx.x] = acc[e];
+    __syncthreads();
+    for (int width = blockDim.x / 2; width > 0; width >>= 1) {
+        if ((int)threadIdx.x < width)
+            for (int e = 0; e < kSums; ++e) tree[e][threadIdx.x] += tree[e][threadIdx.x + width];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int e = 0; e < kSums; ++e) partial[b * kSums + e] = tree[e][0];
+}
+
+// Each run's sums from its chunks' partials, in chunk order.
+__global__ void sumRuns(const int64_t* __restrict__ runChunk, const double* __restrict__ partial, double* out,
+                        int32_t runs) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= runs) return;
+    double acc[kSums] = {};
+    for (int64_t b = runChunk[r]; b < runChunk[r + 1]; ++b)
+        for (int e = 0; e < kSums; ++e) acc[e] += partial[b * kSums + e];
+    for (int e = 0; e < kSums; ++e) out[(int64_t)r * kSums + e] = acc[e];
+}
+
 inline unsigned gridFor(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
 
 // Runs BODY with the compile-time constant L set to the context's lanes per particle.
@@ -1092,6 +1161,14 @@ struct sph_context {
     Records<float> rec32{};
     Records<double> rec64{};
     double** dVel = nullptr;   // device array of the bound velocity row pointers
+
+    // The global sums' chunks: each run's slots in chunks of kSumChunk, in run order
+    int64_t sumChunkCount = 0;
+    int32_t* sumChunkRun = nullptr;
+    int64_t* sumChunkFirst = nullptr;
+    int64_t* sumRunChunk = nullptr;   // runs + 1 offsets into the chunks
+    double* sumPartial = nullptr;
+    double* sumOut = nullptr;
 
     Clock clock{};
     sph_state state{};
@@ -1656,6 +1733,21 @@ int sph_configure(sph_context* ctx, const sph_run* runs, int32_t count, int64_t 
     if (status == SPH_OK) status = allocate(ctx, &ctx->clock.maxima, 2 * (size_t)count);
     if (status == SPH_OK) status = allocate(ctx, &ctx->clock.active, count);
     if (status == SPH_OK) status = allocate(ctx, &ctx->dVel, 3);
+    std::vector<int32_t> chunkRun;
+    std::vector<int64_t> chunkFirst, runChunk(1, 0);
+    for (int32_t r = 0; r < count; ++r) {
+        for (int64_t first = runs[r].first; first < runs[r].first + runs[r].count; first += kSumChunk) {
+            chunkRun.push_back(r);
+            chunkFirst.push_back(first);
+        }
+        runChunk.push_back((int64_t)chunkRun.size());
+    }
+    ctx->sumChunkCount = (int64_t)chunkRun.size();
+    if (status == SPH_OK) status = allocate(ctx, &ctx->sumChunkRun, chunkRun.size());
+    if (status == SPH_OK) status = allocate(ctx, &ctx->sumChunkFirst, chunkFirst.size());
+    if (status == SPH_OK) status = allocate(ctx, &ctx->sumRunChunk, runChunk.size());
+    if (status == SPH_OK) status = allocate(ctx, &ctx->sumPartial, chunkRun.size() * kSums);
+    if (status == SPH_OK) status = allocate(ctx, &ctx->sumOut, (size_t)count * kSums);
     if (status == SPH_OK)
         status = ctx->precision == 0 ? allocateRecords(ctx, ctx->rec32, cap) : allocateRecords(ctx, ctx->rec64, cap);
     if (status == SPH_OK) status = allocateList(ctx);
@@ -1668,6 +1760,14 @@ int sph_configure(sph_context* ctx, const sph_run* runs, int32_t count, int64_t 
         std::fill(runOf.begin() + runs[r].first, runOf.begin() + runs[r].first + runs[r].count, r);
     HIP_TRY(hipMemcpyAsync(ctx->dRuns, ctx->runsHost.data(), count * sizeof(RunDev), hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(hipMemcpyAsync(ctx->runOf, runOf.data(), cap * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
+    if (!chunkRun.empty()) {
+        HIP_TRY(hipMemcpyAsync(ctx->sumChunkRun, chunkRun.data(), chunkRun.size() * sizeof(int32_t),
+                               hipMemcpyHostToDevice, ctx->stream));
+        HIP_TRY(hipMemcpyAsync(ctx->sumChunkFirst, chunkFirst.data(), chunkFirst.size() * sizeof(int64_t),
+                               hipMemcpyHostToDevice, ctx->stream));
+    }
+    HIP_TRY(hipMemcpyAsync(ctx->sumRunChunk, runChunk.data(), runChunk.size() * sizeof(int64_t), hipMemcpyHostToDevice,
+                           ctx->stream));
     HIP_TRY(hipMemsetAsync(ctx->cellCount, 0, std::max<int64_t>(cells, 1) * sizeof(int32_t), ctx->stream));
     HIP_TRY(hipMemsetAsync(ctx->cellFill, 0, std::max<int64_t>(cells, 1) * sizeof(int32_t), ctx->stream));
     std::vector<double> zeros(count, 0.0);
@@ -1831,6 +1931,37 @@ int sph_rates(sph_context* ctx, double* maxima) {
                                ctx->stream));
         HIP_TRY(hipStreamSynchronize(ctx->stream));
     }
+    return SPH_OK;
+}
+
+int sph_global_sums(sph_context* ctx, double* sums) {
+    int status = requireBound(ctx);
+    if (status != SPH_OK) return status;
+    if (!sums) return fail(ctx, SPH_ERR_VALUE, "The sums need an output array.");
+    const int32_t runs = (int32_t)ctx->runs.size();
+    if (ctx->sumChunkCount > 0) {
+        const unsigned chunks = (unsigned)ctx->sumChunkCount;
+        if (ctx->dims == 2) {
+            if (ctx->precision == 0)
+                sumChunks<2, float><<<chunks, 256, 0, ctx->stream>>>(ctx->dRuns, ctx->sumChunkRun, ctx->sumChunkFirst,
+                                                                       ctx->state, ctx->sumPartial);
+            else
+                sumChunks<2, double><<<chunks, 256, 0, ctx->stream>>>(ctx->dRuns, ctx->sumChunkRun, ctx->sumChunkFirst,
+                                                                        ctx->state, ctx->sumPartial);
+        } else {
+            if (ctx->precision == 0)
+                sumChunks<3, float><<<chunks, 256, 0, ctx->stream>>>(ctx->dRuns, ctx->sumChunkRun, ctx->sumChunkFirst,
+                                                                       ctx->state, ctx->sumPartial);
+            else
+                sumChunks<3, double><<<chunks, 256, 0, ctx->stream>>>(ctx->dRuns, ctx->sumChunkRun, ctx->sumChunkFirst,
+                                                                        ctx->state, ctx->sumPartial);
+        }
+    }
+    sumRuns<<<gridFor(runs, 64), 64, 0, ctx->stream>>>(ctx->sumRunChunk, ctx->sumPartial, ctx->sumOut, runs);
+    status = launchCheck(ctx, "global sums");
+    if (status != SPH_OK) return status;
+    HIP_TRY(hipMemcpyAsync(sums, ctx->sumOut, (size_t)runs * kSums * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
     return SPH_OK;
 }
 

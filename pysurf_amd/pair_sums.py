@@ -221,6 +221,12 @@ class Engine:
     def advance(self, steps: int, graph: bool = True) -> None:
         self.check(self.library.sph_advance(self.handle, int(steps), 1 if graph else 0), 'sph_advance')
 
+    def globalSums(self, runs: int) -> np.ndarray:
+        '''Each run's float64 sums over its fluid (sph_global_sums), shape (runs, 7) in _native.globalSumNames order.'''
+        sums = np.zeros((runs, len(_native.globalSumNames)))
+        self.check(self.library.sph_global_sums(self.handle, sums.ctypes.data_as(ctypes.c_void_p)), 'sph_global_sums')
+        return sums
+
     def setLaunch(self, blockSize: int = 256, lanes: int = 0, listCapacity: int = -1) -> None:
         '''
 
@@ -441,7 +447,7 @@ class PairSums:
             raise ValueError('The delta-LES-SPH closure is not built; the delta-SPH scheme runs without it.')
 
     def setEnergyBudget(self, enabled: bool) -> None:
-        '''The per-particle energy ledger (sphGpu.py:1777-1789), not built: use `energy` for the global sums.'''
+        '''The per-particle energy ledger (sphGpu.py:1777-1789), not built: `globalSums` gives the float64 totals.'''
         if enabled:
             raise ValueError('The per-particle energy ledger is not built.')
 
@@ -651,6 +657,26 @@ class PairSums:
         particles.densities[:] = engine.download('density')
         particles.pressures[:] = engine.download('pressure')
         particles.accelerations[:] = engine.download('acceleration')[:, :dim]
+
+    def energySums(self) -> np.ndarray:
+        '''The per-particle energy ledger's sums (sphGpu.py:1805-1825); the ledger is not built, so always raises.'''
+        raise ValueError('The energy budget is off; see setEnergyBudget.')
+
+    def globalSums(self) -> dict:
+
+        '''
+
+        Float64 sums over the fluid of the device-resident state, reduced on the device in a fixed
+        order: 'mass', 'kinetic' (sum m |v|^2 / 2, ParticleSystem.kineticEnergy), 'potential'
+        (sum m g z, ParticleSystem.potentialEnergy with g the downward gravity), 'elastic' (sum m e(rho),
+        sphEnergy.elasticEnergy) and 'momentum' (sum m v).
+
+        '''
+
+        sums = self.engine.globalSums(1)[0]
+        values = dict(zip(_native.globalSumNames[:4], map(float, sums[:4])))
+        values['momentum'] = sums[4:4 + self.dimensions].copy()
+        return values
 
     def residentRates(self, filterDensity: bool = False) -> tuple[float, float]:
         '''The first half of a device step; returns the largest fluid speed and acceleration (sphGpu.py:2237-2273).'''

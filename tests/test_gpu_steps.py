@@ -188,3 +188,44 @@ def testBedEnforcementMatchesTheOracle(dimensions):
     assert np.count_nonzero(np.any(x != particles.positions, axis = 1)) > 0.2 * count
     assert moved.positions == pytest.approx(x, abs = 1e-12)
     assert moved.velocities == pytest.approx(v, abs = 1e-12)
+
+
+@pytest.mark.parametrize('precision, tolerances', [('float64', (1e-12, 1e-13, 1e-11)), ('float32', (1e-6, 1e-7, 1e-5))])
+def testGlobalSumsFollowTheOracle(precision, tolerances):
+    # The device's float64 sums of its own state against numpy's sums of the same state (the order of
+    # summation differs: 1e-12), then the run's energies against the float64 oracle run's after 40 steps
+    from oracle import array_formulation as af
+    print(f'global sums {precision}', flush = True)
+    case = smallCase(seed = 9)
+    steps = 40
+    batch = engineRun([case], steps, precision)
+    sums = batch.globalSums()[0]
+    state = batch.state(0)
+    masses = case.particles.masses.astype(np.float32 if precision == 'float32' else np.float64)
+    g = -case.gravity[-1]
+    own = af.globalSums(state.positions, state.velocities, state.densities, masses, state.isFluid, g,
+                        case.referenceDensity, case.soundSpeed)
+    for k, name in enumerate(('mass', 'kinetic', 'potential', 'elastic')):
+        assert sums[k] == pytest.approx(own[name], rel = 1e-12), name
+    assert sums[4:6] == pytest.approx(own['momentum'], rel = 1e-12, abs = 1e-12 * own['mass'])
+    assert sums[6] == 0.0
+    again = batch.globalSums()[0]
+    assert np.array_equal(sums, again)
+    expected = oracleRun(case, steps)
+    reference = af.globalSums(expected.positions, expected.velocities, expected.dens, case.particles.masses,
+                              case.particles.isFluid, g, case.referenceDensity, case.soundSpeed)
+    kinetic, potential, elastic = tolerances
+    print('relative energy differences', sums[1] / reference['kinetic'] - 1, sums[2] / reference['potential'] - 1,
+          sums[3] / reference['elastic'] - 1, flush = True)
+    assert sums[0] == pytest.approx(reference['mass'], rel = 1e-7)
+    assert sums[1] == pytest.approx(reference['kinetic'], rel = kinetic)
+    assert sums[2] == pytest.approx(reference['potential'], rel = potential)
+    assert sums[3] == pytest.approx(reference['elastic'], rel = elastic)
+
+
+def testGlobalSumsOfARunAloneAndInABatchAgreeBitForBit():
+    cases = [smallCase(seed = 10, ppd = 6), smallCase(seed = 11, ppd = 8)]
+    together = engineRun(cases, 10, 'float32').globalSums()
+    for index, case in enumerate(cases):
+        alone = engineRun([case], 10, 'float32').globalSums()[0]
+        assert np.array_equal(alone, together[index]), index

@@ -430,3 +430,44 @@ def testADeltaSphStillTankStaysAtRest():
     speed = np.linalg.norm(run.velocities[fluid], axis = 1).max()
     assert speed < 0.01 * math.sqrt(g * fill)
     assert np.abs(run.positions[fluid] - positions[fluid]).max() < 0.25 * s
+
+
+@pytest.mark.parametrize('ratio', [0.97, 0.999, 1.0, 1.0004, 1.01, 1.2])
+def testElasticEnergyIsTheIntegralOfTheTaitPressure(ratio):
+    # e(rho) = integral from rho0 to rho of p(s) / s^2 ds with the unclamped Tait pressure (tension)
+    from scipy.integrate import quad
+    stiffness = rho0 * c0**2 / 7.0
+    density = ratio * rho0
+    exact, _ = quad(lambda s: stiffness * ((s / rho0)**7 - 1.0) / s**2, rho0, density, epsabs = 0.0, epsrel = 1e-13)
+    energy = float(af.elasticEnergy(density, rho0, c0, tension = True))
+    assert energy == pytest.approx(exact, rel = 1e-10, abs = 1e-14)
+    # Linear acoustics near rho0: c0^2 x^2 / 2
+    if abs(ratio - 1.0) < 1e-3 and ratio != 1.0:
+        assert energy == pytest.approx(0.5 * c0**2 * (ratio - 1.0)**2, rel = 2e-3)
+    # Without tension the clamped pressure stores nothing below rho0
+    clamped = float(af.elasticEnergy(density, rho0, c0))
+    assert clamped == (0.0 if ratio < 1.0 else energy)
+
+
+def testGlobalSumsOfARigidMotion():
+    # A fluid translating rigidly at v: K = M |v|^2 / 2 and momentum = M v; a boundary particle adds nothing
+    rng = np.random.default_rng(5)
+    count = 200
+    positions = rng.uniform(0.0, 1.0, (count, 2))
+    masses = rng.uniform(0.5, 1.5, count)
+    isFluid = np.ones(count, dtype = bool)
+    isFluid[:20] = False
+    velocity = np.array([0.3, -1.2])
+    velocities = np.tile(velocity, (count, 1))
+    velocities[:20] = 50.0
+    sums = af.globalSums(positions, velocities, np.full(count, rho0), masses, isFluid, 9.81, rho0, c0)
+    total = masses[isFluid].sum()
+    assert sums['mass'] == pytest.approx(total, rel = 1e-14)
+    assert sums['kinetic'] == pytest.approx(0.5 * total * velocity @ velocity, rel = 1e-13)
+    assert sums['momentum'] == pytest.approx(total * velocity, rel = 1e-13)
+    assert sums['elastic'] == 0.0
+    # Lifting the fluid by dz raises P by M g dz
+    lifted = positions.copy()
+    lifted[:, 1] += 0.25
+    raised = af.globalSums(lifted, velocities, np.full(count, rho0), masses, isFluid, 9.81, rho0, c0)
+    assert raised['potential'] - sums['potential'] == pytest.approx(total * 9.81 * 0.25, rel = 1e-12)
