@@ -10,7 +10,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from pysurf_amd.bathymetry import Bathymetry
+from pysurf_amd.bathymetry import Bathymetry, HeightField
 
 
 class ContainerBoundary:
@@ -132,3 +132,120 @@ class BedBoundary:
                 'sideWalls': self.width is not None and not self.periodic, 'width': float(self.width or 0.0),
                 'offshoreWall': self.wallTop is not None, 'wallX': float(self.wallX), 'pistonSpeed': pistonSpeed,
                 'endLimit': bed.endX - clearance if self.endWallTop is not None else bed.endX}
+
+
+class HeightFieldBoundary:
+
+    '''
+
+    A height-field bed in a rectangular tank whose walls stand on the field's bounds: offshore at
+    x = startX, shoreward at x = endX and at y = startY and y = endY (sphBoundaries.py:498-640). Bed
+    particles sit on the lattice's columns, (i + 1/2) s from the walls, in layers below the bed along
+    its local normal at (layer + 1/2) s; each wall's layers stand (layer + 1/2) s behind its plane on
+    the fluid lattice's heights, from below the bed's deepest layer up to `wallTop`, the side walls
+    running past the end walls to close the corners.
+
+    '''
+
+    def __init__(self, field: HeightField, spacing: float, nLayers: int = 3, wallTop: float = 1.0,
+                 offshoreWall: bool = True, endWall: bool = True) -> None:
+        self.field = field
+        self.spacing = spacing
+        self.nLayers = nLayers
+        self.wallTop = wallTop
+        self.offshoreWall = offshoreWall
+        self.endWall = endWall
+
+    @property
+    def dimensions(self) -> int:
+        return 3
+
+    @property
+    def latticeFloor(self) -> float:
+        '''The level the fluid lattice's heights count from: the bed's lowest point (sphBoundaries.py:545-548).'''
+        return self.field.lowestElevation()
+
+    def columns(self) -> tuple[np.ndarray, np.ndarray]:
+        '''The lattice's column positions along x and y inside the walls (sphBoundaries.py:550-556).'''
+        s = self.spacing
+        startX, endX, startY, endY = self.field.bounds
+        return np.arange(startX + 0.5 * s, endX, s), np.arange(startY + 0.5 * s, endY, s)
+
+    def positions(self) -> np.ndarray:
+        '''Boundary particle positions, (N, 3): the bed's layers, then the walls (sphBoundaries.py:558-595).'''
+        s, layers, field = self.spacing, self.nLayers, self.field
+        startX, endX, startY, endY = field.bounds
+        xs, ys = self.columns()
+        gridX, gridY = np.meshgrid(xs, ys, indexing = 'xy')
+        gridX, gridY = gridX.ravel(), gridY.ravel()
+        surface = np.column_stack([gridX, gridY, field.elevation(gridX, gridY)])
+        normal = field.normal(gridX, gridY)
+        blocks = [surface - (layer + 0.5) * s * normal for layer in range(layers)]
+        heights = np.arange(self.latticeFloor - (layers + 1) * s + 0.5 * s, self.wallTop, s)
+
+        def wall(lineX, lineY):
+            bottom = field.elevation(np.clip(lineX, startX, endX), np.clip(lineY, startY, endY)) - layers * s
+            x, y = np.repeat(lineX, heights.size), np.repeat(lineY, heights.size)
+            z = np.tile(heights, lineX.size)
+            keep = z > np.repeat(bottom, heights.size)
+            return np.column_stack([x[keep], y[keep], z[keep]])
+
+        offsets = (np.arange(layers) + 0.5) * s
+        for offset in offsets:
+            if self.offshoreWall:
+                blocks.append(wall(np.full(ys.size, startX - offset), ys))
+            if self.endWall:
+                blocks.append(wall(np.full(ys.size, endX + offset), ys))
+        alongAll = np.concatenate([startX - offsets[::-1], xs, endX + offsets])
+        for offset in offsets:
+            blocks.append(wall(alongAll, np.full(alongAll.size, startY - offset)))
+            blocks.append(wall(alongAll, np.full(alongAll.size, endY + offset)))
+        return np.vstack(blocks)
+
+    def enforce(self, particles) -> None:
+
+        '''
+
+        Return fluid that crossed the bed or a wall a quarter spacing clear of it, removing its
+        velocity into the solid: below the bed a particle rises vertically and loses its velocity
+        into the bed along the bed's normal there (sphBoundaries.py:597-640).
+
+        '''
+
+        fluid = np.nonzero(particles.isFluid)[0]
+        positions, velocities = particles.positions, particles.velocities
+        clearance = 0.25 * self.spacing
+        field = self.field
+        startX, endX, startY, endY = field.bounds
+        bed = field.elevation(positions[fluid, 0], positions[fluid, 1])
+        below = positions[fluid, 2] < bed + clearance
+        under = fluid[below]
+        if under.size:
+            positions[under, 2] = bed[below] + clearance
+            normal = field.normal(positions[under, 0], positions[under, 1])
+            inward = np.sum(velocities[under] * normal, axis = 1)
+            into = inward < 0.0
+            velocities[under[into]] -= inward[into, None] * normal[into]
+        low = fluid[positions[fluid, 1] < startY + clearance]
+        positions[low, 1] = startY + clearance
+        velocities[low, 1] = np.maximum(velocities[low, 1], 0.0)
+        high = fluid[positions[fluid, 1] > endY - clearance]
+        positions[high, 1] = endY - clearance
+        velocities[high, 1] = np.minimum(velocities[high, 1], 0.0)
+        if self.offshoreWall:
+            behind = fluid[positions[fluid, 0] < startX + clearance]
+            positions[behind, 0] = startX + clearance
+            velocities[behind, 0] = np.maximum(velocities[behind, 0], 0.0)
+        limit = endX - clearance if self.endWall else endX
+        beyond = fluid[positions[fluid, 0] > limit]
+        positions[beyond, 0] = limit
+        velocities[beyond, 0] = np.minimum(velocities[beyond, 0], 0.0)
+
+    def description(self) -> dict:
+        '''The clamp as the device applies it (sphGpu.py:2159-2171), as a plain dict.'''
+        field, clearance = self.field, 0.25 * self.spacing
+        startX, endX, startY, endY = field.bounds
+        return {'kind': 'heightField', 'clearance': clearance, 'table': field.z.copy(), 'originX': float(field.x[0]),
+                'originY': float(field.y[0]), 'stepX': field.spacingX, 'stepY': field.spacingY,
+                'lowY': startY + clearance, 'highY': endY - clearance, 'offshoreWall': self.offshoreWall,
+                'lowX': startX + clearance, 'highX': endX - clearance if self.endWall else endX}

@@ -15,10 +15,11 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from pysurf_amd.bathymetry import Bathymetry
-from pysurf_amd.boundaries import BedBoundary
+from pysurf_amd.bathymetry import Bathymetry, HeightField
+from pysurf_amd.boundaries import BedBoundary, HeightFieldBoundary
 from pysurf_amd.particles import ParticleSystem, jitterPoints
-from pysurf_amd.waves import freshwaterDensity, gravity, solitaryWaveFluid, solitaryWaveProfile
+from pysurf_amd.refinement import splitParticles
+from pysurf_amd.waves import freshwaterDensity, gravity, solitaryWaveFluid, solitaryWaveFluid3D, solitaryWaveProfile
 
 
 def taitDensity(pressure, referenceDensity: float, soundSpeed: float):
@@ -124,3 +125,52 @@ def sweepRun(seed: int, resolutions = (16, 24, 32, 48)) -> Case:
     toeX = min(8.0, 0.5 * length)
     return solitaryOnSlope(particlesPerDepth = ppd, waveHeight = waveHeight, slope = slope, length = length,
                            toeX = toeX, seed = seed, name = f'sweep-{seed}')
+
+
+def ridge3D(particlesPerDepth: int = 32, depth: float = 1.0, waveHeight: float = 0.5, crestX: float = 4.0,
+            ridgeHeight: float = 0.5, ridgeWidth: float = 2.0, slope: float = 1.0 / 15.0, toeX: float = 6.0,
+            length: float = 20.0, width: float = 10.0, obliquity: float = 25.0, zoneHalfWidth: float | None = 1.0,
+            lengthRatio: float = 1.0, jitter: float = 1e-3, seed: int = 0, smoothingRatio: float = 1.3,
+            kernelType: str = 'wendlandC2', soundSpeedFactor: float = 10.0, name: str | None = None) -> Case:
+
+    '''
+
+    A 3D solitary wave of H = `waveHeight` with its crest line along y at x = `crestX`, over a
+    Gaussian ridge (HeightField.gaussianRidge: `ridgeHeight` tall, `ridgeWidth` wide, rising
+    shoreward from `toeX` at `slope`, its axis turned `obliquity` degrees from the wave's direction)
+    in a walled tank `length` x `width` depths. Fluid within `zoneHalfWidth` depths of the crest is
+    split once into 8 daughters (keeping h with `lengthRatio` 1); None leaves it whole. Lengths are
+    in units of `depth`.
+
+    '''
+
+    d, s = depth, depth / particlesPerDepth
+    layers = max(3, math.ceil(2.0 * smoothingRatio - 1e-9))
+    wallTop = d + 2.0 * waveHeight
+    field = HeightField.gaussianRidge(depth = d, height = ridgeHeight, width = ridgeWidth, slope = slope, toeX = toeX,
+                                      length = length, halfWidth = 0.5 * width, direction = obliquity)
+    boundary = HeightFieldBoundary(field, s, nLayers = layers, wallTop = wallTop)
+    fluid, velocities = solitaryWaveFluid3D(field, waveHeight, crestX * d, s)
+    startX, endX, startY, endY = field.bounds
+    if jitter > 0.0:
+        surface = lambda x, y: d + solitaryWaveProfile(x, waveHeight, d, crestX * d)
+        fluid = jitterPoints(fluid, s, jitter, seed, [startX, startY], [endX, endY], field.elevation, surface)
+    particles = ParticleSystem.assemble(fluid, boundary.positions(), s, freshwaterDensity, velocities)
+    h = smoothingRatio * s
+    if zoneHalfWidth is not None:
+        zone = np.abs(particles.positions[:, 0] - crestX * d) < zoneHalfWidth * d
+        splitParticles(particles, zone, freshwaterDensity, h, lengthRatio)
+    c0 = soundSpeedFactor * math.sqrt(gravity * d)
+    surfaceElevation = d + solitaryWaveProfile(particles.positions[:, 0], waveHeight, d, crestX * d)
+    depthBelow = np.maximum(surfaceElevation - particles.positions[:, -1], 0.0)
+    particles.densities = taitDensity(freshwaterDensity * gravity * depthBelow, freshwaterDensity, c0)
+    margin = (layers + 1.0) * s
+    return Case(name = name or f'ridge3d-{particlesPerDepth}ppd', particles = particles, boundary = boundary,
+                dimensions = 3, spacing = s, smoothingLength = h,
+                domainMin = np.array([startX - margin, startY - margin, field.lowestElevation() - (layers + 2.0) * s]),
+                domainMax = np.array([endX + margin, endY + margin, wallTop]), kernelType = kernelType,
+                soundSpeed = c0, gravity = np.array([0.0, 0.0, -gravity]), densityFloor = 0.9 * freshwaterDensity,
+                parameters = dict(particlesPerDepth = particlesPerDepth, waveHeight = waveHeight, crestX = crestX,
+                                  ridgeHeight = ridgeHeight, ridgeWidth = ridgeWidth, slope = slope, toeX = toeX,
+                                  length = length, width = width, obliquity = obliquity,
+                                  zoneHalfWidth = zoneHalfWidth, lengthRatio = lengthRatio, seed = seed, depth = d))

@@ -65,3 +65,31 @@ def solitaryWaveFluid(bathymetry, height: float, crestX: float, spacing: float) 
     velocities = solitaryWaveVelocityField(positions[:, 0], positions[:, 1], height, depth, crestX,
                                            bathymetry.elevation(positions[:, 0]))
     return positions, velocities
+
+
+def solitaryWaveFluid3D(field, height: float, crestX: float, spacing: float) -> tuple[np.ndarray, np.ndarray]:
+
+    '''
+
+    The 3D start over a height field: lattice points from half a spacing above the bed to
+    SWL + eta(x), the crest line along y, with the long-wave velocity of the 2D start in each x-z
+    plane (waveMakers.py:282-326 extended across y as BreakingWave3D lays its wave).
+
+    '''
+
+    depth = field.stillWaterLevel - float(field.elevation(field.bounds[0], 0.5 * (field.bounds[2] + field.bounds[3])))
+    swl = field.stillWaterLevel
+    startX, endX, startY, endY = field.bounds
+
+    def inside(points):
+        bed = field.elevation(points[:, 0], points[:, 1])
+        surface = swl + solitaryWaveProfile(points[:, 0], height, depth, crestX)
+        return (points[:, 2] >= bed + (0.5 - 1e-9) * spacing) & (points[:, 2] < surface)
+
+    lower = [startX, startY, field.lowestElevation()]
+    upper = [endX, endY, swl + height + spacing]
+    positions = fillRegion(inside, lower, upper, spacing)
+    planar = solitaryWaveVelocityField(positions[:, 0], positions[:, 2], height, depth, crestX,
+                                       field.elevation(positions[:, 0], positions[:, 1]))
+    velocities = np.column_stack([planar[:, 0], np.zeros(len(positions)), planar[:, 1]])
+    return positions, velocities

@@ -28,7 +28,8 @@ def physicsOf(case) -> oracle.Physics:
                           gravity = case.gravity, diffusion = case.diffusion, delta = case.delta,
                           extrapolatedWalls = case.extrapolatedWalls, negativePressure = case.negativePressure,
                           correction = case.correction, limiter = case.limiter, densityFloor = case.densityFloor,
-                          cfl = case.cfl, maxTimeStep = case.maxTimeStep)
+                          cfl = case.cfl, maxTimeStep = case.maxTimeStep,
+                          hmin = None if case.particles.lengths is None else float(case.particles.lengths.min()))
 
 
 def smallCase(seed = 0, ppd = 8, **options):
@@ -39,7 +40,7 @@ def smallCase(seed = 0, ppd = 8, **options):
 def oracleRun(case, steps):
     p = case.particles
     run = oracle.OracleRun(physicsOf(case), case.boundary.description(), p.positions, p.velocities, p.densities,
-                           p.masses, p.isFluid)
+                           p.masses, p.isFluid, lengths = p.lengths)
     taken = run.advance(np.inf, steps)
     assert taken == steps
     return run
@@ -229,3 +230,57 @@ def testGlobalSumsOfARunAloneAndInABatchAgreeBitForBit():
     for index, case in enumerate(cases):
         alone = engineRun([case], 10, 'float32').globalSums()[0]
         assert np.array_equal(alone, together[index]), index
+
+
+def smallRidge(seed = 0, zone = 0.4, lengthRatio = 0.5, ppd = 8):
+    from pysurf_amd.scenarios import ridge3D
+    return ridge3D(particlesPerDepth = ppd, length = 6.0, width = 1.5, toeX = 2.0, crestX = 2.5, ridgeWidth = 0.6,
+                   zoneHalfWidth = zone, lengthRatio = lengthRatio, seed = seed)
+
+
+@pytest.mark.parametrize('precision, tolerances', [('float64', (1e-10, 1e-8, 1e-9)), ('float32', (1e-6, 1e-3, 1e-3))])
+@pytest.mark.parametrize('split', ['none', 'half-length'])
+def testDeviceStepsFollowTheOracleIn3D(precision, tolerances, split):
+    # A solitary wave over a turned Gaussian ridge in a walled tank: the height-field clamp, and with
+    # 'half-length' a crest zone split into daughters of half the smoothing length (mixed h)
+    steps = 30
+    print(f'3D device steps {precision} split {split}, {steps} steps', flush = True)
+    case = smallRidge(zone = None if split == 'none' else 0.4)
+    assert (case.particles.lengths is None) == (split == 'none')
+    expected = oracleRun(case, steps)
+    batch = engineRun([case], steps, precision)
+    clock = batch.clocks()
+    state = batch.state(0)
+    positionTol, velocityTol, densityTol = tolerances
+    assert int(clock['steps'][0]) == steps
+    assert clock['time'][0] == pytest.approx(expected.time.value, rel = 1e-12 if precision == 'float64' else 1e-5)
+    assert state.positions == pytest.approx(expected.positions, abs = positionTol)
+    assert state.velocities == pytest.approx(expected.velocities, abs = velocityTol)
+    assert state.densities == pytest.approx(expected.dens, abs = densityTol)
+
+
+def testHeightFieldEnforcementMatchesTheOracle():
+    from pysurf_amd.bathymetry import HeightField
+    from pysurf_amd.boundaries import HeightFieldBoundary
+    from pysurf_amd.pair_sums import PairSums
+    from pysurf_amd.particles import ParticleSystem
+    field = HeightField.sechRidge(0.7, toeX = 1.0, endX = 6.0, halfWidth = 1.5, gridSpacing = 0.1)
+    boundary = HeightFieldBoundary(field, 0.05, wallTop = 1.5)
+    rng = np.random.default_rng(11)
+    count = 5000
+    positions = np.column_stack([rng.uniform(-0.2, 6.2, count), rng.uniform(-1.7, 1.7, count),
+                                 rng.uniform(-0.2, 1.2, count)])
+    velocities = rng.normal(0.0, 1.0, (count, 3))
+    fluid = rng.uniform(size = count) < 0.9
+    particles = ParticleSystem(positions.copy(), velocities.copy(), np.zeros((count, 3)), np.full(count, 1000.0),
+                               np.zeros(count), np.ones(count), fluid)
+    sums = PairSums(3, 0.065, positions.min(axis = 0), positions.max(axis = 0))
+    sums.push(particles)
+    sums.setBoundary(boundary, particles)
+    sums.enforceBoundary()
+    sums.pull(particles)
+    physics = oracle.Physics(dims = 3, h = 0.065, domainMin = positions.min(axis = 0), domainMax = positions.max(axis = 0))
+    x, v = oracle.enforce(physics, boundary.description(), positions, velocities, fluid)
+    assert np.count_nonzero(np.any(x != positions, axis = 1)) > 0.2 * count
+    assert particles.positions == pytest.approx(x, abs = 1e-12)
+    assert particles.velocities == pytest.approx(v, abs = 1e-12)

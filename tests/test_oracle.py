@@ -471,3 +471,85 @@ def testGlobalSumsOfARigidMotion():
     lifted[:, 1] += 0.25
     raised = af.globalSums(lifted, velocities, np.full(count, rho0), masses, isFluid, 9.81, rho0, c0)
     assert raised['potential'] - sums['potential'] == pytest.approx(total * 9.81 * 0.25, rel = 1e-12)
+
+
+# -- Height-field bed, split -- #
+def testHeightFieldReproducesQuadraticsAndTheirGradients():
+    # Cubic convolution with a = -1/2 reproduces any quadratic exactly (bathymetry.py:372-380)
+    from pysurf_amd.bathymetry import HeightField
+    quadratic = lambda x, y: 0.1 + 0.2 * x - 0.05 * y + 0.03 * x * x + 0.02 * x * y - 0.04 * y * y
+    field = HeightField.fromFunction(quadratic, (0.0, 4.0, -1.0, 1.0), 0.1, stillWaterLevel = 1.0)
+    rng = np.random.default_rng(2)
+    x, y = rng.uniform(0.2, 3.8, 500), rng.uniform(-0.8, 0.8, 500)
+    assert field.elevation(x, y) == pytest.approx(quadratic(x, y), abs = 1e-12)
+    gradient = field.gradient(x, y)
+    assert gradient[:, 0] == pytest.approx(0.2 + 0.06 * x + 0.02 * y, abs = 1e-11)
+    assert gradient[:, 1] == pytest.approx(-0.05 + 0.02 * x - 0.08 * y, abs = 1e-11)
+    normal = field.normal(x, y)
+    assert np.linalg.norm(normal, axis = 1) == pytest.approx(1.0, abs = 1e-14)
+    assert np.einsum('ij,ij->i', normal[:, :2], np.ones((500, 2))) == pytest.approx(
+        -(gradient[:, 0] + gradient[:, 1]) * normal[:, 2], abs = 1e-14)
+
+
+def testTheGaussianRidgeFollowsItsFormula():
+    from pysurf_amd.bathymetry import HeightField
+    field = HeightField.gaussianRidge(height = 0.5, width = 2.0, slope = 1.0 / 15.0, toeX = 6.0, direction = 0.0,
+                                      gridSpacing = 1.0 / 32.0)
+    # Along the axis: flat offshore of the toe, 1:15 up to the plateau at 0.5; across: exp(-y^2 / 8)
+    assert field.elevation(np.array([3.0, 9.0, 18.0]), np.zeros(3)) == pytest.approx([0.0, 0.2, 0.5], abs = 1e-6)
+    assert float(field.elevation(18.0, 2.0)) == pytest.approx(0.5 * math.exp(-0.5), rel = 1e-5)
+    turned = HeightField.gaussianRidge(direction = 25.0, gridSpacing = 1.0 / 32.0)
+    t = math.radians(25.0)
+    # A point 6 depths along the turned axis from the toe stands at min(6 / 15, 0.5)
+    assert float(turned.elevation(6.0 + 6.0 * math.cos(t), 6.0 * math.sin(t) - 2.0 * math.sin(t) * 0.0)) \
+        == pytest.approx(0.4, rel = 1e-4)
+
+
+def testHeightFieldEnforcementMatchesTheHostFormula():
+    # tests/testSphGpu.py:682-712: fluid scattered above and below a ridge, every wall, to 1e-12
+    from pysurf_amd.bathymetry import HeightField
+    from pysurf_amd.boundaries import HeightFieldBoundary
+    from pysurf_amd.particles import ParticleSystem
+    field = HeightField.sechRidge(0.7, toeX = 1.0, endX = 6.0, halfWidth = 1.5, gridSpacing = 0.1)
+    boundary = HeightFieldBoundary(field, 0.05, wallTop = 1.5)
+    rng = np.random.default_rng(11)
+    count = 5000
+    positions = np.column_stack([rng.uniform(-0.2, 6.2, count), rng.uniform(-1.7, 1.7, count),
+                                 rng.uniform(-0.2, 1.2, count)])
+    velocities = rng.normal(0.0, 1.0, (count, 3))
+    fluid = rng.uniform(size = count) < 0.9
+    particles = ParticleSystem(positions.copy(), velocities.copy(), np.zeros((count, 3)), np.full(count, 1000.0),
+                               np.zeros(count), np.ones(count), fluid)
+    boundary.enforce(particles)
+    physics = oracle.Physics(dims = 3, h = 0.065, domainMin = positions.min(axis = 0), domainMax = positions.max(axis = 0))
+    x, v = oracle.enforce(physics, boundary.description(), positions, velocities, fluid)
+    assert np.count_nonzero(np.any(particles.positions != positions, axis = 1)) > 0.2 * count
+    assert x == pytest.approx(particles.positions, abs = 1e-12)
+    assert v == pytest.approx(particles.velocities, abs = 1e-12)
+
+
+def testSplittingConservesMassMomentumAndTheCentreOfMass():
+    from pysurf_amd.particles import ParticleSystem
+    from pysurf_amd.refinement import splitParticles
+    rng = np.random.default_rng(4)
+    for dimensions in (2, 3):
+        count = 300
+        s = 0.02
+        particles = ParticleSystem.assemble(rng.uniform(0.0, 1.0, (count, dimensions)), np.zeros((0, dimensions)), s,
+                                            1000.0, rng.normal(0.0, 1.0, (count, dimensions)))
+        mask = rng.uniform(size = count) < 0.4
+        before = particles.copy()
+        split = splitParticles(particles, mask, 1000.0, 1.3 * s, lengthRatio = 0.5)
+        assert split == int(mask.sum())
+        assert particles.nParticles == count + split * (2**dimensions - 1)
+        assert particles.masses.sum() == pytest.approx(before.masses.sum(), rel = 1e-14)
+        momentum = (particles.masses[:, None] * particles.velocities).sum(axis = 0)
+        assert momentum == pytest.approx((before.masses[:, None] * before.velocities).sum(axis = 0), rel = 1e-12)
+        centre = (particles.masses[:, None] * particles.positions).sum(axis = 0)
+        assert centre == pytest.approx((before.masses[:, None] * before.positions).sum(axis = 0), rel = 1e-12)
+        # Daughters sit at the corners of a cube s / 2 across around their mother, with half her length
+        mothers = np.flatnonzero(mask)
+        assert particles.positions[mothers] == pytest.approx(before.positions[mothers] - 0.25 * s, abs = 1e-15)
+        assert np.all(particles.lengths[mothers] == 0.65 * s)
+        assert np.all(particles.lengths[:count][~mask] == 1.3 * s)
+        assert np.all(particles.lengths[count:] == 0.65 * s)
