@@ -17,6 +17,7 @@ advanced one step.
 Beside the headline the line carries:
   - the plain float64 kernel stepping a sample of the same runs, each alone, one after another;
   - the single 64-particles-per-depth run (float32 pairs and plain float64);
+  - the 3D run: 12.4 M particles over a turned Gaussian ridge with a split crest zone;
   - the dominant kernel's roofline, from HIP events on the engine's stream around each launch of a
     stream-launched pass over the same batch right after the timed region;
   - the float64 CPU restatement (oracle/) timed on the host's cores on a bounded sample of the
@@ -95,6 +96,8 @@ def parse():
     parser.add_argument('--plain-steps', type = int, default = 200)
     parser.add_argument('--single-steps', type = int, default = 1000, help = 'steps of the single 64 ppd run')
     parser.add_argument('--no-single', action = 'store_true', help = 'skip the single 64 ppd run')
+    parser.add_argument('--ridge-steps', type = int, default = 10, help = 'timed steps of the 3D ridge run')
+    parser.add_argument('--no-ridge', action = 'store_true', help = 'skip the 3D ridge run')
     parser.add_argument('--cpu-seconds', type = float, default = 15.0)
     parser.add_argument('--no-cpu', action = 'store_true', help = 'skip the CPU baseline')
     parser.add_argument('--copy-calibration', type = int, default = 0, metavar = 'MIB',
@@ -227,6 +230,38 @@ def singleRun(args) -> dict:
     return result
 
 
+def ridgeRun(args) -> dict:
+
+    '''
+
+    The 3D run: a solitary wave (H/d 0.5) over a Gaussian ridge turned 25 degrees in a 20 x 10 depth
+    tank at 32 particles per depth, the crest zone split into daughters that keep h, float32 pairs,
+    the launch shape tuned among neighbor lists long enough for the split zone.
+
+    '''
+
+    from pysurf_amd import autotune
+    from pysurf_amd.batch import BatchEngine
+    from pysurf_amd.scenarios import ridge3D
+    case = ridge3D()
+    log(f'3D ridge: {case.count} particles ({case.fluidCount} fluid)')
+    batch = BatchEngine([case], 'float32')
+    launch = autotune.tune(batch, steps = 3, candidates = [(lanes, 256, 768) for lanes in (4, 8, 16)])
+    warm(batch, 2, args.chunk)
+    wall, events = timedSteps(batch, args.ridge_steps, args.chunk)
+    profile = kernelProfile(batch, 2)
+    entries = meanNeighbors(batch)
+    value = case.fluidCount * args.ridge_steps / wall
+    log(f'3D ridge float32: {1000.0 * wall / args.ridge_steps:.1f} ms/step, {value:.4e} fluid updates/s')
+    del batch
+    return {'workload': f'{case.name}: Gaussian ridge 0.5 d tall, 2 d wide, 1:15, turned 25 degrees; 20 x 10 d tank; '
+                        f'crest zone |x - 4 d| < 0.75 d split once, daughters keep h', 'particles': case.count,
+            'fluid_particles': case.fluidCount, 'steps': args.ridge_steps, 'value': value,
+            'unit': 'particle-updates/s', 'ms_per_step': 1000.0 * wall / args.ridge_steps,
+            'mean_neighbors': entries, 'launch': {key: launch[key] for key in ('lanes', 'block', 'listCapacity')},
+            'kernel_ms_per_step': {name: entry['ms'] / 2 for name, entry in profile.items() if entry['launches']}}
+
+
 def cpuModel() -> str:
     try:
         with open('/proc/cpuinfo') as f:
@@ -339,6 +374,7 @@ def main() -> int:
     plain = plainSample(cases[:args.plain_sample], args.plain_steps, args.chunk, not args.no_tune) \
         if args.plain_sample > 0 else None
     single = None if args.no_single else singleRun(args)
+    ridge = None if args.no_ridge else ridgeRun(args)
     cpu = None if args.no_cpu else cpuBaseline(cases[:2], args.cpu_seconds)
 
     sizes = np.array([case.count for case in cases])
@@ -370,6 +406,7 @@ def main() -> int:
         'plain_fp64': plain,
         'speedup_over_plain_fp64': value / plain['value'] if plain else None,
         'single_run': single,
+        'ridge_3d': ridge,
         'cpu_baseline': cpu,
     }
     print(json.dumps(line))

@@ -129,7 +129,7 @@ def sweepRun(seed: int, resolutions = (16, 24, 32, 48)) -> Case:
 
 def ridge3D(particlesPerDepth: int = 32, depth: float = 1.0, waveHeight: float = 0.5, crestX: float = 4.0,
             ridgeHeight: float = 0.5, ridgeWidth: float = 2.0, slope: float = 1.0 / 15.0, toeX: float = 6.0,
-            length: float = 20.0, width: float = 10.0, obliquity: float = 25.0, zoneHalfWidth: float | None = 1.0,
+            length: float = 20.0, width: float = 10.0, obliquity: float = 25.0, zoneHalfWidth: float | None = 0.75,
             lengthRatio: float = 1.0, jitter: float = 1e-3, seed: int = 0, smoothingRatio: float = 1.3,
             kernelType: str = 'wendlandC2', soundSpeedFactor: float = 10.0, name: str | None = None) -> Case:
 
