@@ -233,3 +233,10 @@ class HeightField:
         inX = (self.x >= startX - 1e-9) & (self.x <= endX + 1e-9)
         inY = (self.y >= startY - 1e-9) & (self.y <= endY + 1e-9)
         return float(np.min(self.z[np.ix_(inY, inX)]))
+
+    def section(self, y: float) -> Bathymetry:
+        '''The bed along x at fixed y over the bounds, through the grid's x positions and the bounds' ends (bathymetry.py:719-731).'''
+        startX, endX = self.bounds[0], self.bounds[1]
+        inside = self.x[(self.x > startX + 1e-9) & (self.x < endX - 1e-9)]
+        x = np.concatenate([[startX], inside, [endX]])
+        return Bathymetry(x, self.elevation(x, np.full_like(x, float(y))), self.stillWaterLevel)

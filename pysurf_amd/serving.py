@@ -1,0 +1,356 @@
+'''
+
+Runs stepped to their breaking point, alone or many at once, and the request loop that serves
+them: Poisson arrivals, admission under an HBM budget, continuous batching.
+
+A fleet holds the runs of one dimension in one BatchEngine. It steps them one tracking interval
+at a time (0.01 sqrt(d / g) of each run's simulated time, BreakingWave.py:514, 881): each run's
+device clock stops at its next reading, the positions come back once for all runs, and each run's
+breaking tracker reads its own (a 3D run's tracker reads the strip a quarter depth wide along the
+tank's centre line, in x-z, over the bed's section there). A run is done when its breaking point is
+recorded or its time runs out.
+
+Membership changes between intervals: finished runs leave and admitted runs join, and the fleet
+rebuilds its engine from the runs' current states, each run's particles in the same slot order and
+its clock carried over. Because a run's step depends only on its own state and clock (its cells,
+its time step and the fixed launch shape), a run's history is the same bit for bit alone and in any
+mix.
+
+'''
+
+from __future__ import annotations
+
+import math
+import time as timeModule
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from pysurf_amd.batch import BatchEngine
+from pysurf_amd.breaking import BreakingTracker, trackInterval
+
+# HBM per particle the admission counts: state 81 B (f64 position, velocity, density; f32 pressure,
+# acceleration, mass, length; a flag), the sorted record and its key, G and L, and the neighbor list
+def bytesPerParticle(dimensions: int, listCapacity: int) -> int:
+    d = dimensions
+    state = 8 * d * 2 + 8 + 4 + 4 * d + 4 + 4 + 1
+    return state + 64 + 4 + 8 * d * d + 4 * (listCapacity + 1) + 12
+
+
+def trackerOf(case) -> BreakingTracker:
+    '''The run's breaking tracker: face reach one depth, crests offshore of the bed's toe ignored (BreakingWave.py:763-765).'''
+    depth = case.parameters.get('depth', 1.0)
+    toe = case.parameters.get('toeX', 0.0) * (depth if case.dimensions == 3 else 1.0)
+    if case.dimensions == 2:
+        bed = case.boundary.bathymetry
+    else:
+        startY, endY = case.boundary.field.bounds[2:]
+        bed = case.boundary.field.section(0.5 * (startY + endY))
+    return BreakingTracker(bed, case.spacing, faceReach = depth, searchFromX = toe)
+
+
+def trackedPositions(case, positions: np.ndarray, isFluid: np.ndarray) -> np.ndarray:
+    '''The fluid positions a run's tracker reads: (x, z); in 3D the centre strip a quarter depth wide.'''
+    fluid = positions[isFluid]
+    if case.dimensions == 2:
+        return fluid
+    startY, endY = case.boundary.field.bounds[2:]
+    strip = np.abs(fluid[:, 1] - 0.5 * (startY + endY)) < 0.125 * case.parameters.get('depth', 1.0)
+    return fluid[strip][:, [0, 2]]
+
+
+@dataclass
+class Member:
+
+    '''One run in a fleet: its case, its current state and clock, its tracker and its deadlines.'''
+
+    key: object
+    case: object
+    endTime: float
+    particles: object = None
+    time: float = 0.0
+    steps: int = 0
+    tracker: BreakingTracker = None
+    nextTrack: float = 0.0
+    interval: float = 0.0
+    done: bool = False
+    finishedWall: float = float('nan')
+
+    def __post_init__(self) -> None:
+        if self.particles is None:
+            self.particles = self.case.particles.copy()
+        if self.tracker is None:
+            self.tracker = trackerOf(self.case)
+        self.interval = trackInterval(self.case.parameters.get('depth', 1.0))
+        self.nextTrack = self.time + self.interval
+
+
+class Fleet:
+
+    '''
+
+    The runs of one dimension in one BatchEngine, stepped a tracking interval at a time.
+
+    Parameters:
+    -----------
+    dimensions : int
+        2 or 3
+    precision : str
+        'float32' or 'float64'
+    lanes, block, listCapacity : int
+        The launch shape, fixed for every engine the fleet builds so a run's sums do not depend on
+        the mix
+    chunk : int
+        Steps per graph-replayed advance
+
+    '''
+
+    def __init__(self, dimensions: int, precision: str = 'float32', lanes: int = 2, block: int = 256,
+                 listCapacity: int = -1, chunk: int = 8, device = None) -> None:
+        self.dimensions = dimensions
+        self.precision = precision
+        self.shape = (block, lanes, listCapacity)
+        self.chunk = chunk
+        self.device = device
+        self.members: list[Member] = []
+        self.batch: BatchEngine | None = None
+        self.updates = 0
+
+    def _harvest(self) -> None:
+        '''Copy every member's state and clock back from the engine.'''
+        if self.batch is None:
+            return
+        batch = self.batch
+        positions = batch.engine.download('position', self.dimensions)
+        velocities = batch.engine.download('velocity', self.dimensions)
+        densities = batch.engine.download('density')
+        clock = batch.clocks()
+        for index, member in enumerate(self.members):
+            a, b = batch.offsets[index], batch.offsets[index + 1]
+            member.particles.positions = positions[a:b].copy()
+            member.particles.velocities = velocities[a:b].copy()
+            member.particles.densities = densities[a:b].copy()
+            member.time, member.steps = float(clock['time'][index]), int(clock['steps'][index])
+
+    def rebuild(self, leaving: list, joining: list) -> None:
+        '''Drop `leaving` members and add `joining` ones, rebuilding the engine from the members' current states.'''
+        self._harvest()
+        keep = [m for m in self.members if m not in leaving]
+        self.members = keep + list(joining)
+        self.batch = None
+        if not self.members:
+            return
+        block, lanes, capacity = self.shape
+        cases = []
+        for member in self.members:
+            case = member.case
+            snapshot = type(case)(**{**case.__dict__, 'particles': member.particles})
+            cases.append(snapshot)
+        self.batch = BatchEngine(cases, self.precision, self.device, blockSize = block, lanes = lanes,
+                                 listCapacity = capacity)
+        for index, member in enumerate(self.members):
+            self.batch.engine.setClock(index, member.time, member.nextTrack, member.steps, 2**62)
+
+    @property
+    def particles(self) -> int:
+        return sum(member.case.count for member in self.members)
+
+    def enqueue(self) -> None:
+        '''Point every member's clock at its next reading (asynchronous launches follow in `advance`).'''
+        for index, member in enumerate(self.members):
+            self.batch.engine.setClock(index, member.time, member.nextTrack if not member.done else member.time,
+                                       member.steps, 2**62)
+
+    def advanceChunk(self) -> None:
+        self.batch.advance(self.chunk)
+
+    def reached(self) -> bool:
+        '''Whether every member's clock has reached its reading; refreshes the members' clocks.'''
+        clock = self.batch.clocks()
+        ready = True
+        for index, member in enumerate(self.members):
+            member.time, member.steps = float(clock['time'][index]), int(clock['steps'][index])
+            if not member.done and member.time < member.nextTrack:
+                ready = False
+        return ready
+
+    def read(self, wall: float) -> list:
+        '''Each member's tracker reads its positions; returns the members that finished at this reading.'''
+        batch = self.batch
+        positions = batch.engine.download('position', self.dimensions)
+        finished = []
+        for index, member in enumerate(self.members):
+            if member.done:
+                continue
+            a, b = batch.offsets[index], batch.offsets[index + 1]
+            tracked = trackedPositions(member.case, positions[a:b], member.case.particles.isFluid)
+            if len(tracked):
+                member.tracker.update(member.time, tracked)
+            self.updates += 1
+            member.nextTrack += member.interval
+            if member.tracker.breakingFinal or member.time >= member.endTime:
+                member.done = True
+                member.finishedWall = wall
+                finished.append(member)
+        return finished
+
+    def interval(self, wall = timeModule.perf_counter) -> list:
+        '''Step every member to its next reading and read it; returns the members that finished.'''
+        if self.batch is None:
+            return []
+        self.enqueue()
+        while True:
+            self.advanceChunk()
+            if self.reached():
+                break
+        return self.read(wall())
+
+
+def runToBreaking(cases: list, precision: str = 'float32', endTime: float = 10.0, lanes: int = 2,
+                  listCapacity: int = -1, chunk: int = 8) -> list:
+    '''Step `cases` together to their breaking points; returns each run's tracker, in order.'''
+    dimensions = {case.dimensions for case in cases}
+    if len(dimensions) != 1:
+        raise ValueError('One fleet steps runs of one dimension.')
+    fleet = Fleet(dimensions.pop(), precision, lanes = lanes, listCapacity = listCapacity, chunk = chunk)
+    members = [Member(key = index, case = case, endTime = endTime) for index, case in enumerate(cases)]
+    fleet.rebuild([], members)
+    while not all(member.done for member in members):
+        fleet.interval()
+    return [member.tracker for member in members]
+
+
+@dataclass
+class Request:
+
+    '''A run to serve: arriving `arrival` seconds into the stream.'''
+
+    key: int
+    arrival: float
+    case: object
+    endTime: float = 10.0
+
+
+@dataclass
+class Served:
+
+    '''What became of a request: arrival, admission and finish on the server's wall clock [s].'''
+
+    key: int
+    dimensions: int
+    particles: int
+    fluid: int
+    arrival: float
+    admitted: float
+    finished: float
+    simulated: float
+    steps: int
+    breaking: object
+    tracker: object = None
+
+    @property
+    def latency(self) -> float:
+        return self.finished - self.arrival
+
+
+@dataclass
+class Server:
+
+    '''
+
+    The request loop: requests arrive on the wall clock, wait until their state fits the HBM
+    budget, join their dimension's fleet at the next interval boundary, and leave it when their
+    breaking point is recorded or their time runs out. Both fleets' launches are enqueued before
+    either is waited on, so their work overlaps on the device.
+
+    '''
+
+    budgetBytes: float = 200e9
+    precision: str = 'float32'
+    lanes: int = 2
+    chunk: int = 8
+    listCapacity2D: int = 64
+    listCapacity3D: int = 768
+    fleets: dict = field(default_factory = dict)
+    served: list = field(default_factory = list)
+    queueDepth: list = field(default_factory = list)
+
+    def footprint(self, case) -> float:
+        capacity = self.listCapacity2D if case.dimensions == 2 else self.listCapacity3D
+        return case.count * bytesPerParticle(case.dimensions, capacity)
+
+    def fleet(self, dimensions: int) -> Fleet:
+        if dimensions not in self.fleets:
+            capacity = self.listCapacity2D if dimensions == 2 else self.listCapacity3D
+            self.fleets[dimensions] = Fleet(dimensions, self.precision, lanes = self.lanes, listCapacity = capacity,
+                                            chunk = self.chunk)
+        return self.fleets[dimensions]
+
+    def serve(self, requests: list, wallLimit: float = float('inf')) -> list:
+        '''Serve `requests` (sorted by arrival) until all finish or `wallLimit` seconds pass; returns the Served records.'''
+        start = timeModule.perf_counter()
+        clock = lambda: timeModule.perf_counter() - start
+        waiting = list(requests)
+        queued: list = []
+        admittedAt: dict = {}
+        inUse = 0.0
+        pending = len(requests)
+        while pending and clock() < wallLimit:
+            now = clock()
+            while waiting and waiting[0].arrival <= now:
+                queued.append(waiting.pop(0))
+            joining: dict = {}
+            while queued and inUse + self.footprint(queued[0].case) <= self.budgetBytes:
+                request = queued.pop(0)
+                inUse += self.footprint(request.case)
+                admittedAt[request.key] = now
+                joining.setdefault(request.case.dimensions, []).append(
+                    Member(key = request, case = request.case, endTime = request.endTime))
+            self.queueDepth.append((now, len(queued)))
+            for dimensions, members in joining.items():
+                self.fleet(dimensions).rebuild([], members)
+            active = [fleet for fleet in self.fleets.values() if fleet.batch is not None]
+            if not active:
+                if waiting:
+                    timeModule.sleep(max(0.0, min(waiting[0].arrival - clock(), 0.05)))
+                continue
+            for fleet in active:
+                fleet.enqueue()
+            done = {id(fleet): False for fleet in active}
+            while not all(done.values()):
+                for fleet in active:
+                    if not done[id(fleet)]:
+                        fleet.advanceChunk()
+                for fleet in active:
+                    if not done[id(fleet)]:
+                        done[id(fleet)] = fleet.reached()
+            for fleet in active:
+                finished = fleet.read(clock())
+                for member in finished:
+                    request = member.key
+                    inUse -= self.footprint(request.case)
+                    pending -= 1
+                    self.served.append(Served(key = request.key, dimensions = request.case.dimensions,
+                                              particles = request.case.count, fluid = request.case.fluidCount,
+                                              arrival = request.arrival, admitted = admittedAt[request.key],
+                                              finished = member.finishedWall, simulated = member.time,
+                                              steps = member.steps, breaking = member.tracker.breakingPoint,
+                                              tracker = member.tracker))
+                if finished:
+                    fleet.rebuild(finished, [])
+        return self.served
+
+
+def poissonRequests(cases: list, rate: float, seed: int = 0, endTime: float = 10.0) -> list:
+    '''Requests for `cases` arriving as a Poisson stream of `rate` per second.'''
+    gaps = np.random.default_rng(seed).exponential(1.0 / rate, len(cases))
+    arrivals = np.cumsum(gaps) - gaps[0]
+    return [Request(key = index, arrival = float(arrival), case = case, endTime = endTime)
+            for index, (arrival, case) in enumerate(zip(arrivals, cases))]
+
+
+def latencyPercentiles(served: list) -> dict:
+    latencies = np.array([record.latency for record in served])
+    if latencies.size == 0:
+        return {'p50': float('nan'), 'p99': float('nan')}
+    return {'p50': float(np.percentile(latencies, 50)), 'p99': float(np.percentile(latencies, 99))}

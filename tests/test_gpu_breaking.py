@@ -1,0 +1,59 @@
+'''
+
+Breaking on the device against the float64 restatement, and the request loop's promise that a run
+served in a mix is the run served alone.
+
+The 16-particles-per-depth solitary wave on a 1:15 beach is stepped on the GPU to its breaking point
+with float32 pairs and with the plain float64 kernel, read by the same tracker every 0.01 sqrt(d/g)
+as the oracle run in tests/golden/breaking_c1_16ppd.json (tests/golden/make_breaking.py): the
+breaking time and the crest's position at breaking must agree within 1%.
+
+'''
+
+import json
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'golden', 'breaking_c1_16ppd.json')
+
+
+@pytest.mark.parametrize('precision', ['float32', 'float64'])
+def testBreakingTimeAndLocationFollowTheOracle(precision):
+    from pysurf_amd.scenarios import solitaryOnSlope
+    from pysurf_amd.serving import runToBreaking
+    expected = json.load(open(GOLDEN))
+    print(f'breaking {precision} vs oracle: t {expected["time"]:.4f} s at x {expected["crestX"]:.4f}', flush = True)
+    tracker = runToBreaking([solitaryOnSlope(particlesPerDepth = 16)], precision, endTime = 10.0)[0]
+    point = tracker.breakingPoint
+    assert point is not None
+    print(f'device breaks at t {point.time:.4f} s, x {point.crestX:.4f}', flush = True)
+    assert point.time == pytest.approx(expected['time'], rel = 0.01)
+    assert point.crestX == pytest.approx(expected['crestX'], rel = 0.01)
+
+
+def testARunServedInAMixIsTheRunServedAlone():
+    # Two 2D runs and a 3D ridge run arrive together, one 2D run finishes early and leaves, so the
+    # fleets rebuild mid-run; every run's tracker history must match the run's own, alone, bit for bit
+    from pysurf_amd.scenarios import ridge3D, solitaryOnSlope
+    from pysurf_amd.serving import Request, Server, runToBreaking
+    cases = [solitaryOnSlope(particlesPerDepth = 8, seed = 1), solitaryOnSlope(particlesPerDepth = 10, seed = 2),
+             ridge3D(particlesPerDepth = 8, length = 6.0, width = 1.5, toeX = 2.0, crestX = 2.5, ridgeWidth = 0.6,
+                     zoneHalfWidth = 0.4, lengthRatio = 0.5)]
+    ends = [0.15, 0.3, 0.2]
+    requests = [Request(key = k, arrival = 0.0, case = case, endTime = end) for k, (case, end) in enumerate(zip(cases, ends))]
+    server = Server(budgetBytes = 1e12, listCapacity3D = 160)
+    served = {record.key: record for record in server.serve(requests)}
+    assert sorted(served) == [0, 1, 2]
+    for k, (case, end) in enumerate(zip(cases, ends)):
+        alone = runToBreaking([case], 'float32', endTime = end, lanes = 2,
+                              listCapacity = 64 if case.dimensions == 2 else 160)[0]
+        mixed = served[k].tracker
+        print(f'run {k}: {len(alone.times)} readings alone, {len(mixed.times)} in the mix', flush = True)
+        assert served[k].steps > 0 and len(alone.times) > 10
+        for name in ('times', 'crestX', 'crestHeight', 'lean', 'waterline'):
+            assert np.array_equal(np.asarray(getattr(alone, name)), np.asarray(getattr(mixed, name)), equal_nan = True), \
+                (k, name)
