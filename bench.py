@@ -18,6 +18,8 @@ Beside the headline the line carries:
   - the plain float64 kernel stepping a sample of the same runs, each alone, one after another;
   - the single 64-particles-per-depth run (float32 pairs and plain float64);
   - the 3D run: 12.4 M particles over a turned Gaussian ridge with a split crest zone;
+  - the serving stream: Poisson arrivals, HBM admission, continuous batching, p50/p99 time to
+    the breaking point;
   - the dominant kernel's roofline, from HIP events on the engine's stream around each launch of a
     stream-launched pass over the same batch right after the timed region;
   - the float64 CPU restatement (oracle/) timed on the host's cores on a bounded sample of the
@@ -98,6 +100,9 @@ def parse():
     parser.add_argument('--no-single', action = 'store_true', help = 'skip the single 64 ppd run')
     parser.add_argument('--ridge-steps', type = int, default = 10, help = 'timed steps of the 3D ridge run')
     parser.add_argument('--no-ridge', action = 'store_true', help = 'skip the 3D ridge run')
+    parser.add_argument('--serving-requests', type = int, default = 24, help = 'requests of the serving stream')
+    parser.add_argument('--serving-wall', type = float, default = 150.0, help = 'wall seconds the serving stream may take')
+    parser.add_argument('--no-serving', action = 'store_true', help = 'skip the serving stream')
     parser.add_argument('--cpu-seconds', type = float, default = 15.0)
     parser.add_argument('--no-cpu', action = 'store_true', help = 'skip the CPU baseline')
     parser.add_argument('--copy-calibration', type = int, default = 0, metavar = 'MIB',
@@ -262,6 +267,45 @@ def ridgeRun(args) -> dict:
             'kernel_ms_per_step': {name: entry['ms'] / 2 for name, entry in profile.items() if entry['launches']}}
 
 
+def servingRun(args) -> dict:
+
+    '''
+
+    The serving stream: `args.serving_requests` requests arriving as a Poisson stream of 40 per
+    second, 90% 2D runs of the batched sweep and 10% small 3D ridge runs at 16 particles per depth,
+    admitted under a 200 GB HBM budget and served by continuous batching until each run's breaking
+    point is recorded (2D runs, at most 8 s simulated) or its time runs out (3D runs: 1 s simulated).
+    Latency is each request's arrival to its breaking point's recording on the server's wall clock.
+
+    '''
+
+    import numpy as np
+
+    from pysurf_amd.scenarios import servingMix
+    from pysurf_amd.serving import Server, latencyPercentiles, poissonRequests
+    cases = servingMix(args.serving_requests, seed = 0)
+    requests = poissonRequests(cases, 40.0, seed = 0, endTime = 8.0)
+    for request in requests:
+        if request.case.dimensions == 3:
+            request.endTime = 1.0
+    server = Server(budgetBytes = 200e9)
+    start = time.perf_counter()
+    served = server.serve(requests, wallLimit = args.serving_wall)
+    wall = time.perf_counter() - start
+    broken = [record for record in served if record.breaking is not None]
+    updates = sum(record.fluid * record.steps for record in served)
+    latency = latencyPercentiles(broken)
+    log(f'serving: {len(served)} of {len(requests)} served in {wall:.1f} s, {len(broken)} broke; time to breaking '
+        f'p50 {latency["p50"]:.2f} s, p99 {latency["p99"]:.2f} s')
+    return {'requests': len(requests), 'served': len(served), 'broke': len(broken), 'rate_per_s': 40.0,
+            'share_3d': float(np.mean([case.dimensions == 3 for case in cases])), 'budget_GB': 200.0,
+            'wall_s': wall, 'time_to_breaking_p50_s': latency['p50'], 'time_to_breaking_p99_s': latency['p99'],
+            'completion_p50_s': latencyPercentiles(served)['p50'], 'completion_p99_s': latencyPercentiles(served)['p99'],
+            'throughput': updates / wall, 'unit': 'particle-updates/s',
+            'max_queue_depth': max((depth for _, depth in server.queueDepth), default = 0),
+            'particles_max': max(case.count for case in cases), 'particles_min': min(case.count for case in cases)}
+
+
 def cpuModel() -> str:
     try:
         with open('/proc/cpuinfo') as f:
@@ -375,6 +419,7 @@ def main() -> int:
         if args.plain_sample > 0 else None
     single = None if args.no_single else singleRun(args)
     ridge = None if args.no_ridge else ridgeRun(args)
+    serving = None if args.no_serving else servingRun(args)
     cpu = None if args.no_cpu else cpuBaseline(cases[:2], args.cpu_seconds)
 
     sizes = np.array([case.count for case in cases])
@@ -407,6 +452,7 @@ def main() -> int:
         'speedup_over_plain_fp64': value / plain['value'] if plain else None,
         'single_run': single,
         'ridge_3d': ridge,
+        'serving': serving,
         'cpu_baseline': cpu,
     }
     print(json.dumps(line))

@@ -174,3 +174,21 @@ def ridge3D(particlesPerDepth: int = 32, depth: float = 1.0, waveHeight: float =
                                   ridgeHeight = ridgeHeight, ridgeWidth = ridgeWidth, slope = slope, toeX = toeX,
                                   length = length, width = width, obliquity = obliquity,
                                   zoneHalfWidth = zoneHalfWidth, lengthRatio = lengthRatio, seed = seed, depth = d))
+
+
+def servingRidge(seed: int) -> Case:
+    '''A small 3D ridge run of the serving mix at 16 ppd: tank 16-24 d long, 6-12 d wide; ridge 0.3-0.6 d tall, 1.5-3 d wide, turned 0-30 degrees.'''
+    rng = np.random.default_rng(5000 + seed)
+    length = float(rng.uniform(16.0, 24.0))
+    return ridge3D(particlesPerDepth = 16, waveHeight = float(rng.uniform(0.3, 0.5)), crestX = 4.0,
+                   ridgeHeight = float(rng.uniform(0.3, 0.6)), ridgeWidth = float(rng.uniform(1.5, 3.0)),
+                   toeX = float(rng.uniform(6.0, 8.0)), length = length, width = float(rng.uniform(6.0, 12.0)),
+                   obliquity = float(rng.uniform(0.0, 30.0)), zoneHalfWidth = None, seed = seed,
+                   name = f'serving-ridge-{seed}')
+
+
+def servingMix(count: int, seed: int = 0, share3D: float = 0.1) -> list:
+    '''`count` runs of the serving stream: a `share3D` share of small 3D ridge runs, the rest 2D runs of the batched sweep.'''
+    rng = np.random.default_rng(7000 + seed)
+    kinds = rng.uniform(size = count) < share3D
+    return [servingRidge(1000 * seed + k) if three else sweepRun(1000 * seed + k) for k, three in enumerate(kinds)]

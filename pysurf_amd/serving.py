@@ -75,6 +75,8 @@ class Member:
     interval: float = 0.0
     done: bool = False
     finishedWall: float = float('nan')
+    doneTime: float = float('nan')
+    doneSteps: int = 0
 
     def __post_init__(self) -> None:
         if self.particles is None:
@@ -155,14 +157,45 @@ class Fleet:
     def particles(self) -> int:
         return sum(member.case.count for member in self.members)
 
-    def enqueue(self) -> None:
-        '''Point every member's clock at its next reading (asynchronous launches follow in `advance`).'''
-        for index, member in enumerate(self.members):
-            self.batch.engine.setClock(index, member.time, member.nextTrack if not member.done else member.time,
-                                       member.steps, 2**62)
+    def launch(self) -> None:
 
-    def advanceChunk(self) -> None:
-        self.batch.advance(self.chunk)
+        '''
+
+        Point every member's clock at its next reading and enqueue enough graph-replayed steps to
+        reach it at the last step sizes (a run's clock stops it at its reading, so surplus steps
+        leave it alone); returns without waiting.
+
+        '''
+
+        clock = self.batch.clocks()
+        needed = 1
+        for index, member in enumerate(self.members):
+            until = member.time if member.done else member.nextTrack
+            self.batch.engine.setClock(index, member.time, until, member.steps, 2**62)
+            dt = float(clock['dt'][index])
+            if not member.done and dt > 0.0:
+                needed = max(needed, int(math.ceil((until - member.time) / dt)) + 1)
+        self.batch.advance(self.chunk * max(1, -(-needed // self.chunk)))
+
+    def collect(self) -> dict:
+
+        '''
+
+        Wait until every member's clock reaches its reading (stepping on where one falls short),
+        then copy the positions back and advance each member's next reading; returns the reading.
+
+        '''
+
+        while not self.reached():
+            self.batch.advance(self.chunk)
+        positions = self.batch.engine.download('position', self.dimensions)
+        reading = {'positions': positions, 'offsets': list(self.batch.offsets), 'members': []}
+        for index, member in enumerate(self.members):
+            if member.done:
+                continue
+            reading['members'].append((index, member, member.time, member.steps))
+            member.nextTrack += member.interval
+        return reading
 
     def reached(self) -> bool:
         '''Whether every member's clock has reached its reading; refreshes the members' clocks.'''
@@ -174,36 +207,24 @@ class Fleet:
                 ready = False
         return ready
 
-    def read(self, wall: float) -> list:
-        '''Each member's tracker reads its positions; returns the members that finished at this reading.'''
-        batch = self.batch
-        positions = batch.engine.download('position', self.dimensions)
+    def track(self, reading: dict, wall: float) -> list:
+        '''Each member's tracker reads its positions of `reading`; returns the members that finished at it.'''
+        positions, offsets = reading['positions'], reading['offsets']
         finished = []
-        for index, member in enumerate(self.members):
+        for index, member, time, steps in reading['members']:
             if member.done:
                 continue
-            a, b = batch.offsets[index], batch.offsets[index + 1]
+            a, b = offsets[index], offsets[index + 1]
             tracked = trackedPositions(member.case, positions[a:b], member.case.particles.isFluid)
             if len(tracked):
-                member.tracker.update(member.time, tracked)
+                member.tracker.update(time, tracked)
             self.updates += 1
-            member.nextTrack += member.interval
-            if member.tracker.breakingFinal or member.time >= member.endTime:
+            if member.tracker.breakingFinal or time >= member.endTime:
                 member.done = True
+                member.doneTime, member.doneSteps = time, steps
                 member.finishedWall = wall
                 finished.append(member)
         return finished
-
-    def interval(self, wall = timeModule.perf_counter) -> list:
-        '''Step every member to its next reading and read it; returns the members that finished.'''
-        if self.batch is None:
-            return []
-        self.enqueue()
-        while True:
-            self.advanceChunk()
-            if self.reached():
-                break
-        return self.read(wall())
 
 
 def runToBreaking(cases: list, precision: str = 'float32', endTime: float = 10.0, lanes: int = 2,
@@ -215,9 +236,14 @@ def runToBreaking(cases: list, precision: str = 'float32', endTime: float = 10.0
     fleet = Fleet(dimensions.pop(), precision, lanes = lanes, listCapacity = listCapacity, chunk = chunk)
     members = [Member(key = index, case = case, endTime = endTime) for index, case in enumerate(cases)]
     fleet.rebuild([], members)
-    while not all(member.done for member in members):
-        fleet.interval()
-    return [member.tracker for member in members]
+    fleet.launch()
+    while True:
+        reading = fleet.collect()
+        if not all(member.done for member in members):
+            fleet.launch()
+        fleet.track(reading, timeModule.perf_counter())
+        if all(member.done for member in members):
+            return [member.tracker for member in members]
 
 
 @dataclass
@@ -287,7 +313,15 @@ class Server:
         return self.fleets[dimensions]
 
     def serve(self, requests: list, wallLimit: float = float('inf')) -> list:
-        '''Serve `requests` (sorted by arrival) until all finish or `wallLimit` seconds pass; returns the Served records.'''
+
+        '''
+
+        Serve `requests` (sorted by arrival) until all finish or `wallLimit` seconds pass; returns
+        the Served records. Each round launches every fleet's next interval, then reads the last
+        interval's positions on the host while the device steps, so the trackers overlap the GPU.
+
+        '''
+
         start = timeModule.perf_counter()
         clock = lambda: timeModule.perf_counter() - start
         waiting = list(requests)
@@ -295,10 +329,12 @@ class Server:
         admittedAt: dict = {}
         inUse = 0.0
         pending = len(requests)
+        readings: dict = {}
         while pending and clock() < wallLimit:
             now = clock()
             while waiting and waiting[0].arrival <= now:
                 queued.append(waiting.pop(0))
+            # Membership changes while every fleet is idle: finished runs leave, admitted runs join
             joining: dict = {}
             while queued and inUse + self.footprint(queued[0].case) <= self.budgetBytes:
                 request = queued.pop(0)
@@ -307,37 +343,34 @@ class Server:
                 joining.setdefault(request.case.dimensions, []).append(
                     Member(key = request, case = request.case, endTime = request.endTime))
             self.queueDepth.append((now, len(queued)))
-            for dimensions, members in joining.items():
-                self.fleet(dimensions).rebuild([], members)
-            active = [fleet for fleet in self.fleets.values() if fleet.batch is not None]
-            if not active:
+            for dimensions in set(joining) | set(self.fleets):
+                fleet = self.fleet(dimensions)
+                leaving = [member for member in fleet.members if member.done]
+                if leaving or joining.get(dimensions):
+                    fleet.rebuild(leaving, joining.get(dimensions, []))
+            active = [fleet for fleet in self.fleets.values() if fleet.batch is not None
+                      and not all(member.done for member in fleet.members)]
+            if not active and not readings:
                 if waiting:
                     timeModule.sleep(max(0.0, min(waiting[0].arrival - clock(), 0.05)))
                 continue
             for fleet in active:
-                fleet.enqueue()
-            done = {id(fleet): False for fleet in active}
-            while not all(done.values()):
-                for fleet in active:
-                    if not done[id(fleet)]:
-                        fleet.advanceChunk()
-                for fleet in active:
-                    if not done[id(fleet)]:
-                        done[id(fleet)] = fleet.reached()
-            for fleet in active:
-                finished = fleet.read(clock())
-                for member in finished:
+                fleet.launch()
+            # The readings of the last round, tracked on the host while the device steps this one
+            for fleet, reading in readings.items():
+                for member in fleet.track(reading, clock()):
                     request = member.key
                     inUse -= self.footprint(request.case)
                     pending -= 1
                     self.served.append(Served(key = request.key, dimensions = request.case.dimensions,
                                               particles = request.case.count, fluid = request.case.fluidCount,
                                               arrival = request.arrival, admitted = admittedAt[request.key],
-                                              finished = member.finishedWall, simulated = member.time,
-                                              steps = member.steps, breaking = member.tracker.breakingPoint,
+                                              finished = member.finishedWall, simulated = member.doneTime,
+                                              steps = member.doneSteps, breaking = member.tracker.breakingPoint,
                                               tracker = member.tracker))
-                if finished:
-                    fleet.rebuild(finished, [])
+            readings = {fleet: fleet.collect() for fleet in active}
+            if not pending:
+                break
         return self.served
 
 
