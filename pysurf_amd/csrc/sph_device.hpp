@@ -154,8 +154,8 @@ __device__ __forceinline__ double pairDistance(double dx, double dy, double dz, 
     return sqrt(r2);
 }
 
-// a / b: the fast reciprocal-based quotient in float32, the exact one in float64.
-__device__ __forceinline__ float quotient(float a, float b) { return __fdividef(a, b); }
+// a / b, correctly rounded in both precisions, as the reference's float32 kernels divide.
+__device__ __forceinline__ float quotient(float a, float b) { return a / b; }
 __device__ __forceinline__ double quotient(double a, double b) { return a / b; }
 
 // The smoothing kernels (sphGpu.py:139-186) at one run's constants: W and dW/dr / r, with a fast

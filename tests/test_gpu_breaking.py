@@ -57,3 +57,24 @@ def testARunServedInAMixIsTheRunServedAlone():
         for name in ('times', 'crestX', 'crestHeight', 'lean', 'waterline'):
             assert np.array_equal(np.asarray(getattr(alone, name)), np.asarray(getattr(mixed, name)), equal_nan = True), \
                 (k, name)
+
+
+@pytest.mark.timeout(400)
+def testTheSingle64ppdRunBreaksInFloat32WhereItBreaksInFloat64():
+    # The single-run configuration (64 particles per depth, H/d and slope drawn from seed 0): the
+    # float32-pair engine against the plain float64 kernel, which follows the oracle step for step.
+    # At this resolution the breaking time read in cells moves by about 1.5% under a change of
+    # summation order alone, in float64 too, so each precision is run under three orders (1, 2 and
+    # 4 lanes per particle) and the means are compared: within 1%.
+    from pysurf_amd.scenarios import singleRun
+    from pysurf_amd.serving import runToBreaking
+    case = singleRun(seed = 0, particlesPerDepth = 64)
+    means = {}
+    for precision in ('float64', 'float32'):
+        points = [runToBreaking([case], precision, endTime = 10.0, lanes = lanes)[0].breakingPoint for lanes in (1, 2, 4)]
+        assert all(point is not None for point in points)
+        times, places = [point.time for point in points], [point.crestX for point in points]
+        print(f'{precision}: breaking times {times}, crest x {places}', flush = True)
+        means[precision] = (np.mean(times), np.mean(places))
+    assert means['float32'][0] == pytest.approx(means['float64'][0], rel = 0.01)
+    assert means['float32'][1] == pytest.approx(means['float64'][1], rel = 0.01)
