@@ -46,14 +46,105 @@ struct alignas(16) Quad {
 // One sorted particle as the pair sweeps read it: four quads, 64 bytes in float32 (one cache
 // line, so a pair reads one line) and 128 in float64. For float32 pairs each float64 position is
 // carried as a float32 point and a float32 remainder (sphGpu.py:1010-1019); in float64 the
-// remainder is zero and never read.
+// remainder is zero and never read. The fields sit so that a pair's reads take whole quads:
+//
+//   3D: q0 point xyz, signed length   q1 remainder xyz, p / rho^2   q2 velocity xyz, density
+//       q3 mass, m / rho, rho - rho0, pressure
+//   2D: q0 point xy, remainder xy     q1 velocity xy, density, signed length
+//       q2 mass, m / rho, rho - rho0, p / rho^2                      q3 pressure
+//
+// The smoothing length is negated on boundary particles; rho - rho0 is taken in float64
+// (sphGpu.py:1031-1038).
 template <typename Real>
 struct Record {
-    Quad<Real> a;   // point x, y, z; the smoothing length, negated on boundary particles
-    Quad<Real> b;   // remainder x, y, z; p / rho^2
-    Quad<Real> c;   // velocity x, y, z; density
-    Quad<Real> e;   // mass; m / rho; rho - rho0 taken in float64 (sphGpu.py:1031-1038); pressure
+    Quad<Real> q0, q1, q2, q3;
 };
+
+// Every field of a record, unpacked.
+template <typename Real>
+struct Fields {
+    Real p[3];
+    float l[3];
+    Real sh;      // signed smoothing length
+    Real v[3];
+    Real rho, mass, vol, dev, pr2, press;
+};
+
+template <int D, typename Real>
+__host__ __device__ __forceinline__ Record<Real> pack(const Fields<Real>& f) {
+    Record<Real> r;
+    if constexpr (D == 2) {
+        r.q0 = {f.p[0], f.p[1], (Real)f.l[0], (Real)f.l[1]};
+        r.q1 = {f.v[0], f.v[1], f.rho, f.sh};
+        r.q2 = {f.mass, f.vol, f.dev, f.pr2};
+        r.q3 = {f.press, Real(0), Real(0), Real(0)};
+    } else {
+        r.q0 = {f.p[0], f.p[1], f.p[2], f.sh};
+        r.q1 = {(Real)f.l[0], (Real)f.l[1], (Real)f.l[2], f.pr2};
+        r.q2 = {f.v[0], f.v[1], f.v[2], f.rho};
+        r.q3 = {f.mass, f.vol, f.dev, f.press};
+    }
+    return r;
+}
+
+// Field readers of one record, by dimension.
+template <int D, typename Real>
+__device__ __forceinline__ Real signedLength(const Record<Real>& r) { return D == 2 ? r.q1.w : r.q0.w; }
+template <int D, typename Real>
+__device__ __forceinline__ Real densityOf(const Record<Real>& r) { return D == 2 ? r.q1.z : r.q2.w; }
+template <int D, typename Real>
+__device__ __forceinline__ Real massOf(const Record<Real>& r) { return D == 2 ? r.q2.x : r.q3.x; }
+template <int D, typename Real>
+__device__ __forceinline__ Real volumeOf(const Record<Real>& r) { return D == 2 ? r.q2.y : r.q3.y; }
+template <int D, typename Real>
+__device__ __forceinline__ Real deviationOf(const Record<Real>& r) { return D == 2 ? r.q2.z : r.q3.z; }
+template <int D, typename Real>
+__device__ __forceinline__ Real pressureRatioOf(const Record<Real>& r) { return D == 2 ? r.q2.w : r.q1.w; }
+template <int D, typename Real>
+__device__ __forceinline__ Real pressureOf(const Record<Real>& r) { return D == 2 ? r.q3.x : r.q3.w; }
+template <int D, typename Real>
+__device__ __forceinline__ void velocityOf(const Record<Real>& r, Real (&v)[3]) {
+    if constexpr (D == 2) {
+        const Quad<Real> q = r.q1;
+        v[0] = q.x;
+        v[1] = q.y;
+        v[2] = Real(0);
+    } else {
+        const Quad<Real> q = r.q2;
+        v[0] = q.x;
+        v[1] = q.y;
+        v[2] = q.z;
+    }
+}
+
+// Field writers of one record: the wall pressure's results on a boundary particle, and the
+// velocity the kick refreshes.
+template <int D, typename Real>
+__device__ __forceinline__ void setWall(Record<Real>& r, Real pressure, Real rho) {
+    const Real pr2 = pressure / (rho * rho);
+    if constexpr (D == 2) {
+        r.q1.z = rho;
+        r.q2.y = r.q2.x / rho;
+        r.q2.w = pr2;
+        r.q3.x = pressure;
+    } else {
+        r.q1.w = pr2;
+        r.q2.w = rho;
+        r.q3.y = r.q3.x / rho;
+        r.q3.w = pressure;
+    }
+}
+template <int D, typename Real>
+__device__ __forceinline__ void setVelocity(Record<Real>& r, const Real (&v)[3]) {
+    if constexpr (D == 2) {
+        r.q1.x = v[0];
+        r.q1.y = v[1];
+    } else {
+        r.q2.x = v[0];
+        r.q2.y = v[1];
+        r.q2.z = v[2];
+    }
+}
 
 // The sorted records, with the particle's cell and the per-particle matrices each sweep writes
 // for the next, D*D values per particle, row-major.
@@ -123,20 +214,34 @@ struct Self {
     bool fluid;
 };
 
-// d = r_i - r_j for particle i against record quads (a, b) of j, with `shift` added along y for
-// j's periodic image: (p_i - p_j) + shift + (l_i - l_j) on float32 points and remainders, the
-// plain float64 difference otherwise.
+// d = r_i - r_j for particle i against record j, with `shift` added along y for j's periodic
+// image: (p_i - p_j) + shift + (l_i - l_j) on float32 points and remainders, the plain float64
+// difference otherwise.
 template <int D, typename Real>
-__device__ __forceinline__ void separation(const Self<D, Real>& me, const Quad<Real>& a, const Quad<Real>& b,
-                                           Real shift, Real& dx, Real& dy, Real& dz) {
-    if constexpr (sizeof(Real) == 4) {
-        dx = (me.p[0] - a.x) + (me.l[0] - b.x);
-        dy = (me.p[1] - a.y) + shift + (me.l[1] - b.y);
-        dz = D == 3 ? (me.p[2] - a.z) + (me.l[2] - b.z) : 0.0f;
+__device__ __forceinline__ void separation(const Self<D, Real>& me, const Record<Real>& r, Real shift, Real& dx, Real& dy,
+                                           Real& dz) {
+    if constexpr (D == 2) {
+        const Quad<Real> q = r.q0;
+        if constexpr (sizeof(Real) == 4) {
+            dx = (me.p[0] - q.x) + (me.l[0] - q.z);
+            dy = (me.p[1] - q.y) + shift + (me.l[1] - q.w);
+        } else {
+            dx = me.p[0] - q.x;
+            dy = (me.p[1] + shift) - q.y;
+        }
+        dz = Real(0);
     } else {
-        dx = me.p[0] - a.x;
-        dy = (me.p[1] + shift) - a.y;
-        dz = D == 3 ? me.p[2] - a.z : 0.0;
+        const Quad<Real> a = r.q0;
+        if constexpr (sizeof(Real) == 4) {
+            const Quad<Real> b = r.q1;
+            dx = (me.p[0] - a.x) + (me.l[0] - b.x);
+            dy = (me.p[1] - a.y) + shift + (me.l[1] - b.y);
+            dz = (me.p[2] - a.z) + (me.l[2] - b.z);
+        } else {
+            dx = me.p[0] - a.x;
+            dy = (me.p[1] + shift) - a.y;
+            dz = me.p[2] - a.z;
+        }
     }
 }
 
@@ -222,7 +327,7 @@ struct PairKernel {
 };
 
 // Walks every neighbor j != i of sorted particle s with r < 2 h_ij (sphGpu.py:352-363), calling
-// f(k, fluidJ, dx, dy, dz, r, h) with k the neighbor's sorted position, d = r_i - r_j at the
+// f(k, dx, dy, dz, r, h) with k the neighbor's sorted position, d = r_i - r_j at the
 // nearest periodic image, r = |d| and h = h_ij = (h_i + h_j) / 2. Each cell range's candidates are
 // dealt to `Lanes` lanes in turn, lane `lane` taking the range's candidates lane, lane + Lanes, ...,
 // so a group of lanes shares one particle's walk. With `SkipCoincident` pairs closer than 1e-12 are
@@ -246,21 +351,21 @@ __device__ __forceinline__ void forNeighbors(const RunDev& run, const int32_t* _
         forStencil<D>(run, start, cx, oy, cz, [&](int32_t begin, int32_t end) {
             for (int32_t k = begin + lane; k < end; k += Lanes) {
                 if (k == s) continue;
-                const Quad<Real> a = rec.r[k].a;
+                const Record<Real>& other = rec.r[k];
                 Real dx, dy, dz;
-                separation<D, Real>(me, a, rec.r[k].b, shift, dx, dy, dz);
+                separation<D, Real>(me, other, shift, dx, dy, dz);
                 const Real r = pairDistance(dx, dy, dz, D);
                 Real h, support;
                 if (uniform) {
                     h = (Real)run.hmax;
                     support = twoHmax;
                 } else {
-                    h = Real(0.5) * (me.h + fabs(a.w));
+                    h = Real(0.5) * (me.h + fabs(signedLength<D, Real>(other)));
                     support = Real(2) * h;
                 }
                 if (!(r < support)) continue;
                 if (SkipCoincident && r < Real(1.0e-12)) continue;
-                f(k, a.w > Real(0), dx, dy, dz, r, h);
+                f(k, dx, dy, dz, r, h);
             }
         });
     }
@@ -279,7 +384,7 @@ struct NeighborList {
 
 constexpr int32_t kImageBit = (int32_t)0x80000000u;
 
-// Calls f(k, fluidJ, dx, dy, dz, r, h) for the neighbors of sorted particle s that lane `lane` of
+// Calls f(k, dx, dy, dz, r, h) for the neighbors of sorted particle s that lane `lane` of
 // its group takes: neighbor m, in the order forNeighbors visits them, goes to lane m % Lanes. From
 // the list when the particle's neighbors fit it, from its cells otherwise; the same pairs reach the
 // same lanes in the same order either way.
@@ -296,25 +401,33 @@ __device__ __forceinline__ void forEachNeighbor(const RunDev& run, const int32_t
             const int32_t entry = list.index[(int64_t)m * list.stride + s];
             const int32_t k = entry & 0x7fffffff;
             const Real shift = entry < 0 ? shiftImage : Real(0);
-            const Quad<Real> a = rec.r[k].a;
+            const Record<Real>& other = rec.r[k];
             Real dx, dy, dz;
-            separation<D, Real>(me, a, rec.r[k].b, shift, dx, dy, dz);
+            separation<D, Real>(me, other, shift, dx, dy, dz);
             const Real r = pairDistance(dx, dy, dz, D);
             if (SkipCoincident && r < Real(1.0e-12)) continue;
-            const Real h = uniform ? (Real)run.hmax : Real(0.5) * (me.h + fabs(a.w));
-            f(k, a.w > Real(0), dx, dy, dz, r, h);
+            const Real h = uniform ? (Real)run.hmax : Real(0.5) * (me.h + fabs(signedLength<D, Real>(other)));
+            f(k, dx, dy, dz, r, h);
         }
         return;
     }
     int32_t m = 0;
     forNeighbors<D, Real, 1, false>(run, start, rec, s, me, cx, cy, cz, y64, 0, uniform,
-                                    [&](int32_t k, bool fluidJ, Real dx, Real dy, Real dz, Real r, Real h) {
+                                    [&](int32_t k, Real dx, Real dy, Real dz, Real r, Real h) {
                                         const bool mine = (m % Lanes) == lane;
                                         ++m;
                                         if (!mine || (SkipCoincident && r < Real(1.0e-12))) return;
-                                        f(k, fluidJ, dx, dy, dz, r, h);
+                                        f(k, dx, dy, dz, r, h);
                                     });
 }
+
+// The pair sweeps' workgroups: each covers up to blockDim / Lanes consecutive sorted particles of
+// one run, so the run and everything read from its description are the same across the
+// workgroup and live in scalar registers.
+struct Tiles {
+    const int32_t* run;     // run of each workgroup
+    const int64_t* first;   // first sorted particle of each workgroup
+};
 
 // The workgroup's place in the launch with the blocks of each XCD made contiguous: an MI355X deals
 // workgroups to its 8 XCDs in turn, each XCD with its own L2, while neighboring blocks of sorted

@@ -36,6 +36,12 @@
 
 using namespace sph;
 
+// Waves each SIMD keeps resident in the pair sweeps: the register budget the compiler plans for
+// (512 VGPRs / waves). The sweeps wait on dependent gathers, so more waves hide more latency.
+#ifndef SPH_WAVES_PER_EU
+#define SPH_WAVES_PER_EU 4
+#endif
+
 namespace {
 
 constexpr int kScanThreads = 256;
@@ -259,8 +265,9 @@ __global__ void gatherRecords(const RunDev* __restrict__ runs, const int32_t* __
     if (valid) {
         const RunDev& run = runs[r];
         const int32_t i = order[s];
-        Record<Real> out;
-        Real p[3] = {Real(0), Real(0), Real(0)}, l[3] = {Real(0), Real(0), Real(0)}, v[3] = {Real(0), Real(0), Real(0)};
+        Fields<Real> f;
+        Real p[3] = {Real(0), Real(0), Real(0)}, v[3] = {Real(0), Real(0), Real(0)};
+        float l[3] = {0.0f, 0.0f, 0.0f};
         double v2 = 0.0;
 #pragma unroll
         for (int a = 0; a < D; ++a) {
@@ -289,11 +296,19 @@ __global__ void gatherRecords(const RunDev* __restrict__ runs, const int32_t* __
             if (src.pressureOut) src.pressureOut[i] = pressure;
         }
         const Real rhoR = (Real)rho, massR = src.mass[i], h = src.len[i];
-        out.a = {p[0], p[1], p[2], isFluid ? h : -h};
-        out.b = {l[0], l[1], l[2], pressure / (rhoR * rhoR)};
-        out.c = {v[0], v[1], v[2], rhoR};
-        out.e = {massR, massR / rhoR, (Real)(rho - run.rho0), pressure};
-        rec.r[s] = out;
+        for (int a = 0; a < 3; ++a) {
+            f.p[a] = p[a];
+            f.l[a] = l[a];
+            f.v[a] = v[a];
+        }
+        f.sh = isFluid ? h : -h;
+        f.rho = rhoR;
+        f.mass = massR;
+        f.vol = massR / rhoR;
+        f.dev = (Real)(rho - run.rho0);
+        f.pr2 = pressure / (rhoR * rhoR);
+        f.press = pressure;
+        rec.r[s] = pack<D, Real>(f);
         rec.key[s] = src.key[i];
         speed = sqrt(v2);
         valid = isFluid;
@@ -309,19 +324,31 @@ template <int D, typename Real>
 __device__ __forceinline__ Self<D, Real> loadSelf(const RunDev& run, const Records<Real>& rec, int64_t s, int& cx,
                                                   int& cy, int& cz) {
     Self<D, Real> me;
-    const Quad<Real> a = rec.r[s].a;
-    me.p[0] = a.x;
-    me.p[1] = a.y;
-    me.p[2] = D == 3 ? a.z : Real(0);
+    const Record<Real>& own = rec.r[s];
+    const Quad<Real> a = own.q0;
     me.l[0] = me.l[1] = me.l[2] = 0.0f;
-    if constexpr (sizeof(Real) == 4) {
-        const Quad<Real> b = rec.r[s].b;
-        me.l[0] = b.x;
-        me.l[1] = b.y;
-        me.l[2] = D == 3 ? b.z : 0.0f;
+    if constexpr (D == 2) {
+        me.p[0] = a.x;
+        me.p[1] = a.y;
+        me.p[2] = Real(0);
+        if constexpr (sizeof(Real) == 4) {
+            me.l[0] = a.z;
+            me.l[1] = a.w;
+        }
+    } else {
+        me.p[0] = a.x;
+        me.p[1] = a.y;
+        me.p[2] = a.z;
+        if constexpr (sizeof(Real) == 4) {
+            const Quad<Real> b = own.q1;
+            me.l[0] = b.x;
+            me.l[1] = b.y;
+            me.l[2] = b.z;
+        }
     }
-    me.fluid = a.w > Real(0);
-    me.h = fabs(a.w);
+    const Real sh = signedLength<D, Real>(own);
+    me.fluid = sh > Real(0);
+    me.h = fabs(sh);
     me.reach = me.h + (Real)run.hmax;
     const int64_t local = (int64_t)rec.key[s] - run.cellFirst;
     cx = (int)(local % run.nx);
@@ -394,29 +421,28 @@ struct Sweep1 {
 #pragma unroll
         for (int e = 0; e < K; ++e) acc[e] = Real(0);
         mode = chosen;
-        const Quad<Real> c = rec.r[s].c;
-        vi[0] = c.x;
-        vi[1] = c.y;
-        vi[2] = c.z;
+        velocityOf<D, Real>(rec.r[s], vi);
 #pragma unroll
         for (int a = 0; a < 3; ++a) g[a] = (Real)run.gravity[a];
     }
     __device__ __forceinline__ void add(const Records<Real>& rec, const PairKernel<Real, D>& kernel, int32_t k,
-                                        bool fluidJ, Real dx, Real dy, Real dz, Real rr, Real h) {
-        if (mode == 0 || !fluidJ) return;
+                                        Real dx, Real dy, Real dz, Real rr, Real h) {
+        if (mode == 0) return;
+        const Record<Real>& other = rec.r[k];
+        if (!(signedLength<D, Real>(other) > Real(0))) return;
         if (mode == 1) {
             const Real w = kernel.value(rr, h);
             Real gd = g[0] * dx + g[1] * dy;
             if (D == 3) gd += g[2] * dz;
             acc[0] += w;
-            acc[1] += w * (rec.r[k].e.w + rec.r[k].c.w * gd);
+            acc[1] += w * (pressureOf<D, Real>(other) + densityOf<D, Real>(other) * gd);
             return;
         }
         if (rr < Real(1.0e-12)) return;
-        const Quad<Real> cj = rec.r[k].c;
-        const Real w = rec.r[k].e.y * kernel.slopeOverR(rr, h);
+        const Real w = volumeOf<D, Real>(other) * kernel.slopeOverR(rr, h);
         const Real d[3] = {dx, dy, dz};
-        const Real vj[3] = {cj.x, cj.y, cj.z};
+        Real vj[3];
+        velocityOf<D, Real>(other, vj);
 #pragma unroll
         for (int a = 0; a < D; ++a) {
             const Real dv = w * (vj[a] - vi[a]);
@@ -445,7 +471,7 @@ __device__ __forceinline__ void sweep1Finish(const RunDev& run, const Sweep1<D, 
                                              const SweepOut<Real>& out, int64_t s, int32_t i, bool fluidI,
                                              int32_t resident, int32_t doWall, int32_t doGradients) {
     if (sweep.mode == 0) {
-        if (doWall && out.pressure && (!fluidI || !resident)) out.pressure[i] = rec.r[s].e.w;
+        if (doWall && out.pressure && (!fluidI || !resident)) out.pressure[i] = pressureOf<D, Real>(rec.r[s]);
         if (doGradients && out.gradients && !fluidI)
             for (int e = 0; e < D * D; ++e) out.gradients[e * out.stride + i] = Real(0);
         return;
@@ -454,12 +480,7 @@ __device__ __forceinline__ void sweep1Finish(const RunDev& run, const Sweep1<D, 
         Real pressure = Real(0);
         if (sweep.acc[0] > Real(0)) pressure = fmax(sweep.acc[1] / sweep.acc[0], Real(0));
         const double rho = run.rho0 * pow(1.0 + (double)pressure / run.stiffness, 1.0 / 7.0);
-        const Real rhoR = (Real)rho;
-        Record<Real>& own = rec.r[s];
-        own.b.w = pressure / (rhoR * rhoR);
-        own.c.w = rhoR;
-        own.e.y = own.e.x / rhoR;
-        own.e.w = pressure;
+        setWall<D, Real>(rec.r[s], pressure, (Real)rho);
         if (out.pressure) out.pressure[i] = pressure;
         if (out.density) out.density[i] = rho;
         return;
@@ -501,21 +522,17 @@ __device__ __forceinline__ void sweep1Finish(const RunDev& run, const Sweep1<D, 
 // and compacts the hits with a ballot, so the list keeps the walk's order. With `WithSweep1` the
 // same walk accumulates sweep 1's sums, each lane over the candidates it tested.
 template <int D, typename Real, int Lanes, bool WithSweep1>
-__global__ void __launch_bounds__(256)
-    buildNeighbors(const RunDev* __restrict__ runs, const int32_t* __restrict__ runOf,
-                   const int32_t* __restrict__ order, const int32_t* __restrict__ start, Records<Real> rec,
+__global__ void __launch_bounds__(256, SPH_WAVES_PER_EU)
+    buildNeighbors(const RunDev* __restrict__ runs, Tiles tiles, const int32_t* __restrict__ order, const int32_t* __restrict__ start, Records<Real> rec,
                    NeighborList list, SweepOut<Real> out, Clock clock, const double* __restrict__ y64,
                    int32_t resident, int64_t n) {
-    const int64_t t = xcdBlock() * blockDim.x + threadIdx.x;
-    const int64_t s = t / Lanes;
-    const int lane = (int)(t % Lanes);
+    const int64_t tile = xcdBlock();
+    const int r = tiles.run[tile];
+    const int64_t s = tiles.first[tile] + threadIdx.x / Lanes;
+    const int lane = (int)(threadIdx.x % Lanes);
     const int groupBase = (int)(threadIdx.x & 63) & ~(Lanes - 1);
-    bool valid = s < n;
-    int r = 0;
-    if (valid) {
-        r = runOf[s];
-        valid = !resident || clock.active[r];
-    }
+    bool valid = s < runs[r].first + runs[r].count && (!resident || clock.active[r]);
+    (void)n;
     Sweep1<D, Real> sweep;
     sweep.mode = 0;
     bool fluidI = false;
@@ -546,21 +563,20 @@ __global__ void __launch_bounds__(256)
             forStencil<D>(run, start, cx, oy, cz, [&](int32_t begin, int32_t end) {
                 for (int32_t base = begin; base < end; base += Lanes) {
                     const int32_t k = base + lane;
-                    bool hit = false, fluidJ = false;
+                    bool hit = false;
                     Real dx = Real(0), dy = Real(0), dz = Real(0), rr = Real(0), h = Real(0);
                     if (k < end && k != s) {
-                        const Quad<Real> a = rec.r[k].a;
-                        separation<D, Real>(me, a, rec.r[k].b, shift, dx, dy, dz);
+                        const Record<Real>& other = rec.r[k];
+                        separation<D, Real>(me, other, shift, dx, dy, dz);
                         rr = pairDistance(dx, dy, dz, D);
-                        fluidJ = a.w > Real(0);
-                        h = uniform ? (Real)run.hmax : Real(0.5) * (me.h + fabs(a.w));
+                        h = uniform ? (Real)run.hmax : Real(0.5) * (me.h + fabs(signedLength<D, Real>(other)));
                         hit = rr < (uniform ? twoHmax : Real(2) * h);
                     }
                     const uint32_t mine = (uint32_t)(__ballot(hit) >> groupBase) & groupMask;
                     if (hit) {
                         const int32_t slot = count + __popc(mine & ((1u << lane) - 1u));
                         if (slot < list.capacity) list.index[(int64_t)slot * list.stride + s] = k | tag;
-                        if (WithSweep1) sweep.add(rec, kernel, k, fluidJ, dx, dy, dz, rr, h);
+                        if (WithSweep1) sweep.add(rec, kernel, k, dx, dy, dz, rr, h);
                     }
                     count += __popc(mine);
                 }
@@ -577,22 +593,19 @@ __global__ void __launch_bounds__(256)
 
 // Sweep 1 on its own: wall pressure on boundary particles, G on fluid particles, over the cells.
 template <int D, typename Real, int Lanes>
-__global__ void __launch_bounds__(256)
-    sweepWallAndGradients(const RunDev* __restrict__ runs, const int32_t* __restrict__ runOf,
-                          const int32_t* __restrict__ order, const int32_t* __restrict__ start, Records<Real> rec,
+__global__ void __launch_bounds__(256, SPH_WAVES_PER_EU)
+    sweepWallAndGradients(const RunDev* __restrict__ runs, Tiles tiles, const int32_t* __restrict__ order, const int32_t* __restrict__ start, Records<Real> rec,
                           NeighborList list, SweepOut<Real> out, Clock clock, const double* __restrict__ y64,
                           int32_t resident, int32_t doWall, int32_t doGradients, int64_t n) {
-    const int64_t t = xcdBlock() * blockDim.x + threadIdx.x;
-    const int64_t s = t / Lanes;
-    const int lane = (int)(t % Lanes);
+    const int64_t tile = xcdBlock();
+    const int r = tiles.run[tile];
+    const int64_t s = tiles.first[tile] + threadIdx.x / Lanes;
+    const int lane = (int)(threadIdx.x % Lanes);
     Sweep1<D, Real> sweep;
     sweep.mode = 0;
-    bool valid = s < n, fluidI = false;
-    int r = 0;
-    if (valid) {
-        r = runOf[s];
-        valid = !resident || clock.active[r];
-    }
+    bool fluidI = false;
+    const bool valid = s < runs[r].first + runs[r].count && (!resident || clock.active[r]);
+    (void)n;
     if (valid) {
         const RunDev& run = runs[r];
         int cx, cy, cz;
@@ -604,8 +617,9 @@ __global__ void __launch_bounds__(256)
             kernel.init(run.kernel, run.sigmaUnit, run.hmax, run.uniform != 0);
             const double yy = run.period > 0.0 ? y64[order[s]] : 0.0;
             forNeighbors<D, Real, Lanes, false>(run, start, rec, s, me, cx, cy, cz, yy, lane, kernel.uniform,
-                                                [&](int32_t k, bool fluidJ, Real dx, Real dy, Real dz, Real rr,
-                                                    Real h) { sweep.add(rec, kernel, k, fluidJ, dx, dy, dz, rr, h); });
+                                                [&](int32_t k, Real dx, Real dy, Real dz, Real rr, Real h) {
+                                                    sweep.add(rec, kernel, k, dx, dy, dz, rr, h);
+                                                });
         }
     }
     (void)list;
@@ -618,24 +632,21 @@ __global__ void __launch_bounds__(256)
 // Sweep 2: accelerations with the limited viscosity (sphGpu.py:303-414) and, when asked, the
 // correction matrix L (sphGpu.py:593-641) on fluid particles.
 template <int D, typename Real, int Lanes>
-__global__ void __launch_bounds__(256)
-    sweepAccelerations(const RunDev* __restrict__ runs, const int32_t* __restrict__ runOf,
-                       const int32_t* __restrict__ order, const int32_t* __restrict__ start, Records<Real> rec,
+__global__ void __launch_bounds__(256, SPH_WAVES_PER_EU)
+    sweepAccelerations(const RunDev* __restrict__ runs, Tiles tiles, const int32_t* __restrict__ order, const int32_t* __restrict__ start, Records<Real> rec,
                        NeighborList list, SweepOut<Real> out, Clock clock, const double* __restrict__ y64,
                        int32_t resident, int32_t doAcceleration, int32_t doCorrection, int32_t hostLimiter, int64_t n) {
-    const int64_t t = xcdBlock() * blockDim.x + threadIdx.x;
-    const int64_t s = t / Lanes;
-    const int lane = (int)(t % Lanes);
+    const int64_t tile = xcdBlock();
+    const int r = tiles.run[tile];
+    const int64_t s = tiles.first[tile] + threadIdx.x / Lanes;
+    const int lane = (int)(threadIdx.x % Lanes);
     constexpr int K = D + D * D;          // acceleration, then M
     Real acc[K];
 #pragma unroll
     for (int e = 0; e < K; ++e) acc[e] = Real(0);
-    bool activeRun = false, fluidI = false, correction = false;
-    int r = 0;
-    if (s < n) {
-        r = runOf[s];
-        activeRun = !resident || clock.active[r];
-    }
+    bool fluidI = false, correction = false;
+    const bool activeRun = s < runs[r].first + runs[r].count && (!resident || clock.active[r]);
+    (void)n;
     if (activeRun) {
         const RunDev& run = runs[r];
         correction = doCorrection && (!resident || run.correction);
@@ -647,10 +658,10 @@ __global__ void __launch_bounds__(256)
             PairKernel<Real, D> kernel;
             kernel.init(run.kernel, run.sigmaUnit, run.hmax, run.uniform != 0);
             const double yy = run.period > 0.0 ? y64[order[s]] : 0.0;
-            const Quad<Real> ci = rec.r[s].c;
-            const Real vi[3] = {ci.x, ci.y, ci.z};
-            const Real rhoI = ci.w;
-            const Real own = rec.r[s].b.w;
+            Real vi[3];
+            velocityOf<D, Real>(rec.r[s], vi);
+            const Real rhoI = densityOf<D, Real>(rec.r[s]);
+            const Real own = pressureRatioOf<D, Real>(rec.r[s]);
             Real Gi[3][3] = {};
             if (limiter && doAcceleration)
 #pragma unroll
@@ -660,12 +671,12 @@ __global__ void __launch_bounds__(256)
             const Real alphaC0 = (Real)(run.alpha * run.c0), bf = (Real)run.boundaryFactor;
             forEachNeighbor<D, Real, Lanes, true>(
                 run, start, rec, list, s, me, cx, cy, cz, yy, lane, kernel.uniform,
-                [&](int32_t k, bool fluidJ, Real dx, Real dy, Real dz, Real rr, Real h) {
+                [&](int32_t k, Real dx, Real dy, Real dz, Real rr, Real h) {
+                    const Record<Real>& other = rec.r[k];
                     const Real F = kernel.slopeOverR(rr, h);
                     const Real d[3] = {dx, dy, dz};
-                    const Quad<Real> ej = rec.r[k].e;
                     if (correction) {
-                        const Real w = -ej.y * F;
+                        const Real w = -volumeOf<D, Real>(other) * F;
 #pragma unroll
                         for (int a = 0; a < D; ++a) {
                             const Real wa = w * d[a];
@@ -674,8 +685,9 @@ __global__ void __launch_bounds__(256)
                         }
                     }
                     if (!doAcceleration) return;
-                    const Quad<Real> cj = rec.r[k].c;
-                    const Real vj[3] = {cj.x, cj.y, cj.z};
+                    Real vj[3];
+                    velocityOf<D, Real>(other, vj);
+                    const bool fluidJ = signedLength<D, Real>(other) > Real(0);
                     Real dv[3] = {};
                     Real approach = Real(0);
 #pragma unroll
@@ -683,7 +695,7 @@ __global__ void __launch_bounds__(256)
                         dv[a] = vi[a] - vj[a];
                         approach += dv[a] * d[a];
                     }
-                    const Real rhoJ = cj.w;
+                    const Real rhoJ = densityOf<D, Real>(other);
                     Real viscosity = Real(0);
                     if (approach < Real(0) && limiter && fluidJ) {
                         Real rampI[3] = {}, rampJ[3] = {};
@@ -723,7 +735,7 @@ __global__ void __launch_bounds__(256)
                         viscosity = quotient(-alphaC0 * mu, Real(0.5) * (rhoI + rhoJ));
                         if (!fluidJ) viscosity *= bf;
                     }
-                    const Real coefficient = -(own + rec.r[k].b.w + viscosity) * ej.x * F;
+                    const Real coefficient = -(own + pressureRatioOf<D, Real>(other) + viscosity) * massOf<D, Real>(other) * F;
 #pragma unroll
                     for (int a = 0; a < D; ++a) acc[a] += coefficient * d[a];
                 });
@@ -802,7 +814,7 @@ __global__ void kick(const RunDev* __restrict__ runs, const int32_t* __restrict_
         clock.time[r] += dt;
         clock.steps[r] += 1;
     }
-    if (!(rec.r[s].a.w > Real(0))) return;
+    if (!(signedLength<D, Real>(rec.r[s]) > Real(0))) return;
     const int32_t i = order[s];
     Real v[3] = {Real(0), Real(0), Real(0)};
 #pragma unroll
@@ -812,10 +824,7 @@ __global__ void kick(const RunDev* __restrict__ runs, const int32_t* __restrict_
         u[i] = value;
         v[a] = (Real)value;
     }
-    Quad<Real>& c = rec.r[s].c;
-    c.x = v[0];
-    c.y = v[1];
-    if (D == 3) c.z = v[2];
+    setVelocity<D, Real>(rec.r[s], v);
 }
 
 // The bed's boundary enforcement on one fluid particle (sphGpu.py:1197-1229, 1246-1310,
@@ -902,21 +911,18 @@ __device__ __forceinline__ void enforceBoundary(const sph_boundary& b, double* x
 // density advance with its floor (:1111-1140), drift (:1142-1186) and boundary enforcement
 // follow on its float64 state.
 template <int D, typename Real, int Lanes>
-__global__ void __launch_bounds__(256)
-    sweepContinuity(const RunDev* __restrict__ runs, const int32_t* __restrict__ runOf,
-                    const int32_t* __restrict__ order, const int32_t* __restrict__ start, Records<Real> rec,
+__global__ void __launch_bounds__(256, SPH_WAVES_PER_EU)
+    sweepContinuity(const RunDev* __restrict__ runs, Tiles tiles, const int32_t* __restrict__ order, const int32_t* __restrict__ start, Records<Real> rec,
                     NeighborList list, SweepOut<Real> out, Clock clock, int32_t resident, int32_t hostCorrection,
                     int32_t hostDiffusion, int64_t n) {
-    const int64_t t = xcdBlock() * blockDim.x + threadIdx.x;
-    const int64_t s = t / Lanes;
-    const int lane = (int)(t % Lanes);
+    const int64_t tile = xcdBlock();
+    const int r = tiles.run[tile];
+    const int64_t s = tiles.first[tile] + threadIdx.x / Lanes;
+    const int lane = (int)(threadIdx.x % Lanes);
     Real total = Real(0);
-    bool activeRun = false, boundaryI = false;
-    int r = 0;
-    if (s < n) {
-        r = runOf[s];
-        activeRun = !resident || clock.active[r];
-    }
+    bool boundaryI = false;
+    const bool activeRun = s < runs[r].first + runs[r].count && (!resident || clock.active[r]);
+    (void)n;
     if (activeRun) {
         const RunDev& run = runs[r];
         int cx, cy, cz;
@@ -928,8 +934,8 @@ __global__ void __launch_bounds__(256)
         PairKernel<Real, D> kernel;
         kernel.init(run.kernel, run.sigmaUnit, run.hmax, run.uniform != 0);
         const double yy = run.period > 0.0 ? out.pos[1][order[s]] : 0.0;
-        const Quad<Real> ci = rec.r[s].c;
-        const Real vi[3] = {ci.x, ci.y, ci.z};
+        Real vi[3];
+        velocityOf<D, Real>(rec.r[s], vi);
         Real Li[3][3] = {};
         if (correction) {
 #pragma unroll
@@ -937,19 +943,20 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
                 for (int b = 0; b < D; ++b) Li[a][b] = rec.L[s * (D * D) + a * D + b];
         }
-        const Real devI = rec.r[s].e.z;
+        const Real devI = deviationOf<D, Real>(rec.r[s]);
         const Real factor = (Real)(2.0 * run.delta * run.c0);
         const Real invRho0 = (Real)(1.0 / run.rho0);
         const Real head = (Real)(run.rho0 * fabs(run.gravity[D - 1]) / (run.c0 * run.c0));
         forEachNeighbor<D, Real, Lanes, true>(
             run, start, rec, list, s, me, cx, cy, cz, yy, lane, kernel.uniform,
-            [&](int32_t k, bool fluidJ, Real dx, Real dy, Real dz, Real rr, Real h) {
+            [&](int32_t k, Real dx, Real dy, Real dz, Real rr, Real h) {
+                const Record<Real>& other = rec.r[k];
+                const bool fluidJ = signedLength<D, Real>(other) > Real(0);
                 if (boundaryI && !fluidJ) return;
                 const Real F = kernel.slopeOverR(rr, h);
                 const Real d[3] = {dx, dy, dz};
-                const Quad<Real> cj = rec.r[k].c;
-                const Quad<Real> ej = rec.r[k].e;
-                const Real vj[3] = {cj.x, cj.y, cj.z};
+                Real vj[3];
+                velocityOf<D, Real>(other, vj);
                 Real dot = Real(0);
                 if (correction) {
 #pragma unroll
@@ -963,9 +970,9 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
                     for (int a = 0; a < D; ++a) dot += (vi[a] - vj[a]) * d[a];
                 }
-                total += ej.x * dot * F;
+                total += massOf<D, Real>(other) * dot * F;
                 if (diffuses && fluidJ) {
-                    const Real devJ = ej.z;
+                    const Real devJ = deviationOf<D, Real>(other);
                     Real difference = devJ - devI;
                     if (diffusion == 2) {
                         const Real ratio = Real(1) + Real(0.5) * (devI + devJ) * invRho0;
@@ -974,7 +981,7 @@ __global__ void __launch_bounds__(256)
                     }
                     const Real soft = kernel.uniform ? kernel.soft : Real(0.01) * h * h;
                     const Real r2 = rr * rr;
-                    total += factor * h * ej.y * difference * quotient(-F * r2, r2 + soft);
+                    total += factor * h * volumeOf<D, Real>(other) * difference * quotient(-F * r2, r2 + soft);
                 }
             });
     }
@@ -1024,7 +1031,7 @@ __global__ void enforceKernel(const RunDev* __restrict__ runs, const int32_t* __
 // Neighbor counts (rows null) or rows as slots, in walk order: from the list where the particle's
 // neighbors fit it, from its cells otherwise.
 template <int D, typename Real>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256, SPH_WAVES_PER_EU)
     neighborExport(const RunDev* __restrict__ runs, const int32_t* __restrict__ runOf, const int32_t* __restrict__ order,
                    const int32_t* __restrict__ start, Records<Real> rec, NeighborList list,
                    const double* __restrict__ y64, int32_t* counts, const int64_t* offsets, int32_t* rows, int64_t n) {
@@ -1038,7 +1045,7 @@ __global__ void __launch_bounds__(256)
     int32_t count = 0;
     const int64_t base = rows ? offsets[i] : 0;
     forEachNeighbor<D, Real, 1, false>(run, start, rec, list, s, me, cx, cy, cz, yy, 0, run.uniform != 0,
-                                       [&](int32_t k, bool, Real, Real, Real, Real, Real) {
+                                       [&](int32_t k, Real, Real, Real, Real, Real) {
                                            if (rows) rows[base + count] = order[k];
                                            ++count;
                                        });
@@ -1066,7 +1073,7 @@ __device__ __forceinline__ double elasticEnergy(double rho, double rho0, double 
 // over a fixed stride, then a fixed tree. K = m |v|^2 / 2 and P = m g z with g the downward
 // gravity along the last axis (sphParticles.py:236-255); masses as the device stores them.
 template <int D, typename Real>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256, SPH_WAVES_PER_EU)
     sumChunks(const RunDev* __restrict__ runs, const int32_t* __restrict__ chunkRun,
               const int64_t* __restrict__ chunkFirst, sph_state state, double* __restrict__ partial) {
     __shared__ double tree[kSums][256];
@@ -1141,6 +1148,11 @@ struct sph_context {
     int lanes = 0;           // lanes per particle in the sweeps; 0 picks by dimension
     int listCapacity = -1;   // neighbor-list entries per particle; -1 picks by dimension, 0 walks the cells
     NeighborList list{};
+    // The sweeps' tiles for the current workgroup size and lanes (tilePer particles each)
+    int32_t* tileRun = nullptr;
+    int64_t* tileFirst = nullptr;
+    int64_t tileCount = 0;
+    int tilePer = 0;
 
     std::vector<sph_run> runs;
     std::vector<RunDev> runsHost;
@@ -1234,6 +1246,12 @@ void releaseScratch(sph_context* ctx) {
     if (ctx->list.index) (void)hipFree(ctx->list.index);
     if (ctx->list.count) (void)hipFree(ctx->list.count);
     ctx->list = NeighborList{};
+    if (ctx->tileRun) (void)hipFree(ctx->tileRun);
+    if (ctx->tileFirst) (void)hipFree(ctx->tileFirst);
+    ctx->tileRun = nullptr;
+    ctx->tileFirst = nullptr;
+    ctx->tileCount = 0;
+    ctx->tilePer = 0;
     for (void* p : ctx->allocations) (void)hipFree(p);
     ctx->allocations.clear();
     ctx->dRuns = nullptr;
@@ -1297,6 +1315,41 @@ int allocateList(sph_context* ctx) {
     if (capacity > 0) HIP_TRY(hipMalloc(&ctx->list.index, (size_t)capacity * cap * sizeof(int32_t)));
     return SPH_OK;
 }
+
+// The sweeps' tiles: every run's sorted particles cut into workgroups of block / lanes particles.
+// Rebuilt when the workgroup size or the lanes change; never during graph capture, because every
+// entry point that launches sweeps calls it first.
+int ensureTiles(sph_context* ctx) {
+    const int per = std::max(1, ctx->block / lanesOf(ctx));
+    if (per == ctx->tilePer && ctx->tileRun) return SPH_OK;
+    std::vector<int32_t> run;
+    std::vector<int64_t> first;
+    for (size_t r = 0; r < ctx->runs.size(); ++r)
+        for (int64_t f = ctx->runs[r].first; f < ctx->runs[r].first + ctx->runs[r].count; f += per) {
+            run.push_back((int32_t)r);
+            first.push_back(f);
+        }
+    if (run.empty()) {
+        run.push_back(0);
+        first.push_back(0);
+    }
+    if (ctx->tileRun) (void)hipFree(ctx->tileRun);
+    if (ctx->tileFirst) (void)hipFree(ctx->tileFirst);
+    ctx->tileRun = nullptr;
+    ctx->tileFirst = nullptr;
+    HIP_TRY(hipMalloc(&ctx->tileRun, run.size() * sizeof(int32_t)));
+    HIP_TRY(hipMalloc(&ctx->tileFirst, first.size() * sizeof(int64_t)));
+    HIP_TRY(hipMemcpyAsync(ctx->tileRun, run.data(), run.size() * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ctx->tileFirst, first.data(), first.size() * sizeof(int64_t), hipMemcpyHostToDevice,
+                           ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    ctx->tileCount = (int64_t)run.size();
+    ctx->tilePer = per;
+    return SPH_OK;
+}
+
+inline Tiles tilesOf(const sph_context* ctx) { return Tiles{ctx->tileRun, ctx->tileFirst}; }
+inline unsigned tileGrid(const sph_context* ctx) { return (unsigned)ctx->tileCount; }
 
 template <typename Real>
 Records<Real>& recordsOf(sph_context* ctx);
@@ -1416,7 +1469,7 @@ SweepOut<Real> emptyOut(sph_context* ctx) {
 int requireBound(sph_context* ctx) {
     if (!ctx) return SPH_ERR_VALUE;
     if (!ctx->bound) return fail(ctx, SPH_ERR_VALUE, "No state is bound; call sph_configure and sph_bind first.");
-    return SPH_OK;
+    return ensureTiles(ctx);
 }
 
 // The two halves of a device-resident step of every active run. `phase` 1 is the rates
@@ -1454,18 +1507,18 @@ int stepOnce(sph_context* ctx, int phase, int useClock, double forcedDt) {
                                                                              ctx->clock, 1, n)));
         if (ctx->list.capacity > 0) {
             WITH_LANES(ctx, TIMED(ctx, kNeighbors,
-                                  (buildNeighbors<D, Real, L, true><<<gridFor(n * L, b), b, 0, ctx->stream>>>(
-                                      ctx->dRuns, ctx->runOf, ctx->order, ctx->cellStart, rec, ctx->list, out,
+                                  (buildNeighbors<D, Real, L, true><<<tileGrid(ctx), b, 0, ctx->stream>>>(
+                            ctx->dRuns, tilesOf(ctx), ctx->order, ctx->cellStart, rec, ctx->list, out,
                                       ctx->clock, y64, 1, n))));
         } else {
             WITH_LANES(ctx, TIMED(ctx, kSweepWall,
-                                  (sweepWallAndGradients<D, Real, L><<<gridFor(n * L, b), b, 0, ctx->stream>>>(
-                                      ctx->dRuns, ctx->runOf, ctx->order, ctx->cellStart, rec, ctx->list, out,
+                                  (sweepWallAndGradients<D, Real, L><<<tileGrid(ctx), b, 0, ctx->stream>>>(
+                            ctx->dRuns, tilesOf(ctx), ctx->order, ctx->cellStart, rec, ctx->list, out,
                                       ctx->clock, y64, 1, 1, 1, n))));
         }
         WITH_LANES(ctx, TIMED(ctx, kSweepAcceleration,
-                              (sweepAccelerations<D, Real, L><<<gridFor(n * L, b), b, 0, ctx->stream>>>(
-                                  ctx->dRuns, ctx->runOf, ctx->order, ctx->cellStart, rec, ctx->list, out, ctx->clock,
+                              (sweepAccelerations<D, Real, L><<<tileGrid(ctx), b, 0, ctx->stream>>>(
+                            ctx->dRuns, tilesOf(ctx), ctx->order, ctx->cellStart, rec, ctx->list, out, ctx->clock,
                                   y64, 1, 1, 1, 0, n))));
     }
     if (phase & 2) {
@@ -1474,8 +1527,8 @@ int stepOnce(sph_context* ctx, int phase, int useClock, double forcedDt) {
                                                                     static_cast<const Real*>(ctx->state.acceleration),
                                                                     ctx->dVel, ctx->clock, forcedDt, st, n)));
         WITH_LANES(ctx, TIMED(ctx, kSweepContinuity,
-                              (sweepContinuity<D, Real, L><<<gridFor(n * L, b), b, 0, ctx->stream>>>(
-                                  ctx->dRuns, ctx->runOf, ctx->order, ctx->cellStart, rec, ctx->list, out, ctx->clock,
+                              (sweepContinuity<D, Real, L><<<tileGrid(ctx), b, 0, ctx->stream>>>(
+                            ctx->dRuns, tilesOf(ctx), ctx->order, ctx->cellStart, rec, ctx->list, out, ctx->clock,
                                   1, 0, 0, n))));
     }
     return launchCheck(ctx, "device step");
@@ -1521,45 +1574,45 @@ int hostSweep(sph_context* ctx, int which, const double* velocity, const double*
     const int32_t* start = ctx->cellStart;
     const NeighborList list = ctx->list;
     if (list.capacity > 0)
-        WITH_LANES(ctx, (buildNeighbors<D, Real, L, false><<<gridFor(n * L, b), b, 0, ctx->stream>>>(
-                            ctx->dRuns, ctx->runOf, order, start, rec, list, emptyOut<Real>(ctx), ctx->clock, y64, 0,
+        WITH_LANES(ctx, (buildNeighbors<D, Real, L, false><<<tileGrid(ctx), b, 0, ctx->stream>>>(
+                            ctx->dRuns, tilesOf(ctx), order, start, rec, list, emptyOut<Real>(ctx), ctx->clock, y64, 0,
                             n)));
     switch (which) {
         case 0:   // wall pressure
             out.pressure = static_cast<Real*>(out0);
             out.density = out1;
-            WITH_LANES(ctx, (sweepWallAndGradients<D, Real, L><<<gridFor(n * L, b), b, 0, ctx->stream>>>(
-                                ctx->dRuns, ctx->runOf, order, start, rec, list, out, ctx->clock, y64, 0, 1, 0, n)));
+            WITH_LANES(ctx, (sweepWallAndGradients<D, Real, L><<<tileGrid(ctx), b, 0, ctx->stream>>>(
+                            ctx->dRuns, tilesOf(ctx), order, start, rec, list, out, ctx->clock, y64, 0, 1, 0, n)));
             break;
         case 1:   // velocity gradients
             out.gradients = static_cast<Real*>(out0);
-            WITH_LANES(ctx, (sweepWallAndGradients<D, Real, L><<<gridFor(n * L, b), b, 0, ctx->stream>>>(
-                                ctx->dRuns, ctx->runOf, order, start, rec, list, out, ctx->clock, y64, 0, 0, 1, n)));
+            WITH_LANES(ctx, (sweepWallAndGradients<D, Real, L><<<tileGrid(ctx), b, 0, ctx->stream>>>(
+                            ctx->dRuns, tilesOf(ctx), order, start, rec, list, out, ctx->clock, y64, 0, 0, 1, n)));
             break;
         case 2:   // correction matrices
             out.corrections = static_cast<Real*>(out0);
-            WITH_LANES(ctx, (sweepAccelerations<D, Real, L><<<gridFor(n * L, b), b, 0, ctx->stream>>>(
-                                ctx->dRuns, ctx->runOf, order, start, rec, list, out, ctx->clock, y64, 0, 0, 1, 0, n)));
+            WITH_LANES(ctx, (sweepAccelerations<D, Real, L><<<tileGrid(ctx), b, 0, ctx->stream>>>(
+                            ctx->dRuns, tilesOf(ctx), order, start, rec, list, out, ctx->clock, y64, 0, 0, 1, 0, n)));
             break;
         case 3: {  // accelerations, with the limiter's gradients first when the run has it
             const int limiter = run0.limiter;
             if (limiter)
-                WITH_LANES(ctx, (sweepWallAndGradients<D, Real, L><<<gridFor(n * L, b), b, 0, ctx->stream>>>(
-                                    ctx->dRuns, ctx->runOf, order, start, rec, list, out, ctx->clock, y64, 0, 0, 1, n)));
+                WITH_LANES(ctx, (sweepWallAndGradients<D, Real, L><<<tileGrid(ctx), b, 0, ctx->stream>>>(
+                            ctx->dRuns, tilesOf(ctx), order, start, rec, list, out, ctx->clock, y64, 0, 0, 1, n)));
             out.acceleration = static_cast<Real*>(out0);
-            WITH_LANES(ctx, (sweepAccelerations<D, Real, L><<<gridFor(n * L, b), b, 0, ctx->stream>>>(
-                                ctx->dRuns, ctx->runOf, order, start, rec, list, out, ctx->clock, y64, 0, 1, 0, limiter, n)));
+            WITH_LANES(ctx, (sweepAccelerations<D, Real, L><<<tileGrid(ctx), b, 0, ctx->stream>>>(
+                            ctx->dRuns, tilesOf(ctx), order, start, rec, list, out, ctx->clock, y64, 0, 1, 0, limiter, n)));
             break;
         }
         case 4: {  // continuity, with the correction matrices first when the run has them
             const int correction = run0.correction;
             if (correction)
-                WITH_LANES(ctx, (sweepAccelerations<D, Real, L><<<gridFor(n * L, b), b, 0, ctx->stream>>>(
-                                    ctx->dRuns, ctx->runOf, order, start, rec, list, out, ctx->clock, y64, 0, 0, 1, 0, n)));
+                WITH_LANES(ctx, (sweepAccelerations<D, Real, L><<<tileGrid(ctx), b, 0, ctx->stream>>>(
+                            ctx->dRuns, tilesOf(ctx), order, start, rec, list, out, ctx->clock, y64, 0, 0, 1, 0, n)));
             out.rate = static_cast<Real*>(out0);
             for (int a = 0; a < 3; ++a) out.pos[a] = a < D ? ctx->state.position + a * ctx->state.stride : nullptr;
-            WITH_LANES(ctx, (sweepContinuity<D, Real, L><<<gridFor(n * L, b), b, 0, ctx->stream>>>(
-                                ctx->dRuns, ctx->runOf, order, start, rec, list, out, ctx->clock, 0, correction,
+            WITH_LANES(ctx, (sweepContinuity<D, Real, L><<<tileGrid(ctx), b, 0, ctx->stream>>>(
+                            ctx->dRuns, tilesOf(ctx), order, start, rec, list, out, ctx->clock, 0, correction,
                                 run0.diffusion, n)));
             break;
         }
@@ -1593,8 +1646,8 @@ int neighborsImpl(sph_context* ctx, int32_t* counts, const int64_t* offsets, int
     gatherRecords<D, Real><<<gridFor(n, b), b, 0, ctx->stream>>>(ctx->dRuns, ctx->runOf, ctx->order, src, rec, ctx->clock, 0, n);
     const double* y64 = ctx->state.position + ctx->state.stride;
     if (ctx->list.capacity > 0)
-        WITH_LANES(ctx, (buildNeighbors<D, Real, L, false><<<gridFor(n * L, b), b, 0, ctx->stream>>>(
-                            ctx->dRuns, ctx->runOf, ctx->order, ctx->cellStart, rec, ctx->list, emptyOut<Real>(ctx),
+        WITH_LANES(ctx, (buildNeighbors<D, Real, L, false><<<tileGrid(ctx), b, 0, ctx->stream>>>(
+                            ctx->dRuns, tilesOf(ctx), ctx->order, ctx->cellStart, rec, ctx->list, emptyOut<Real>(ctx),
                             ctx->clock, y64, 0, n)));
     neighborExport<D, Real><<<gridFor(n, b), b, 0, ctx->stream>>>(ctx->dRuns, ctx->runOf, ctx->order, ctx->cellStart, rec,
                                                                    ctx->list, y64, counts, offsets, rows, n);
