@@ -16,7 +16,8 @@ import os
 from pysurf_amd.errors import EngineError
 
 here = os.path.dirname(os.path.abspath(__file__))
-libraryPath = os.path.join(here, 'libsph_hip.so')
+# SPH_HIP_LIBRARY names another build of the same library (a variant compiled with other launch constants)
+libraryPath = os.environ.get('SPH_HIP_LIBRARY') or os.path.join(here, 'libsph_hip.so')
 
 SPH_OK, SPH_ERR_VALUE, SPH_ERR_HIP = 0, 1, 2
 

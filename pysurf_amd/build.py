@@ -34,17 +34,18 @@ def hipcc() -> str:
     return found
 
 
-def build(verbose: bool = False) -> str:
-    '''Compile the HIP library; returns its path.'''
-    temporary = library + '.tmp'
-    command = [hipcc()] + flags + ['-o', temporary] + sources
+def build(verbose: bool = False, output: str | None = None, defines: tuple = ()) -> str:
+    '''Compile the HIP library (to `output`, with extra -D `defines`); returns its path.'''
+    target = output or library
+    temporary = target + '.tmp'
+    command = [hipcc()] + flags + ['-D' + define for define in defines] + ['-o', temporary] + sources
     result = subprocess.run(command, capture_output = True, text = True)
     if result.returncode != 0:
         raise RuntimeError('hipcc failed:\n' + result.stdout + result.stderr)
     if verbose and result.stderr:
         print(result.stderr)
-    os.replace(temporary, library)
-    return library
+    os.replace(temporary, target)
+    return target
 
 
 if __name__ == '__main__':

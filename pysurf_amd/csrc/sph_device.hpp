@@ -212,7 +212,17 @@ struct Self {
     Real h;
     Real reach;
     bool fluid;
+    int32_t cell;   // the particle's cell within its run's grid
 };
+
+// A run-local cell index as (x, y, z) cell coordinates, x fastest.
+__device__ __forceinline__ void cellCoordinates(const RunDev& run, int32_t cell, int& cx, int& cy, int& cz) {
+    const uint32_t c = (uint32_t)cell, nx = (uint32_t)run.nx, ny = (uint32_t)run.ny;
+    const uint32_t row = c / nx;
+    cx = (int)(c - row * nx);
+    cy = (int)(row % ny);
+    cz = (int)(row / ny);
+}
 
 // d = r_i - r_j for particle i against record j, with `shift` added along y for j's periodic
 // image: (p_i - p_j) + shift + (l_i - l_j) on float32 points and remainders, the plain float64
@@ -257,6 +267,12 @@ __device__ __forceinline__ double pairDistance(double dx, double dy, double dz, 
     r2 = fma(dy, dy, r2);
     if (dims == 3) r2 = fma(dz, dz, r2);
     return sqrt(r2);
+}
+
+// The distance a pair sum uses once its pair is known: the predicate's own, correctly rounded.
+template <typename Real>
+__device__ __forceinline__ Real sumDistance(Real dx, Real dy, Real dz, int dims) {
+    return pairDistance(dx, dy, dz, dims);
 }
 
 // a / b, correctly rounded in both precisions, as the reference's float32 kernels divide.
@@ -336,7 +352,9 @@ struct PairKernel {
 template <int D, typename Real, int Lanes, bool SkipCoincident, class F>
 __device__ __forceinline__ void forNeighbors(const RunDev& run, const int32_t* __restrict__ start,
                                              const Records<Real>& rec, int64_t s, const Self<D, Real>& me,
-                                             int cx, int cy, int cz, double y64, int lane, bool uniform, F&& f) {
+                                             double y64, int lane, bool uniform, F&& f) {
+    int cx, cy, cz;
+    cellCoordinates(run, me.cell, cx, cy, cz);
     const Real period = (Real)run.period;
     const Real twoHmax = Real(2) * (Real)run.hmax;
     int images = 1;
@@ -384,39 +402,56 @@ struct NeighborList {
 
 constexpr int32_t kImageBit = (int32_t)0x80000000u;
 
-// Calls f(k, dx, dy, dz, r, h) for the neighbors of sorted particle s that lane `lane` of
+// List entries each lane loads ahead in the sweeps, so their gathers overlap.
+#ifndef SPH_LIST_AHEAD
+#define SPH_LIST_AHEAD 2
+#endif
+constexpr int kAhead = SPH_LIST_AHEAD;
+
+// Calls f(k, dx, dy, dz, r, h), r from sumDistance, for the neighbors of sorted particle s that lane `lane` of
 // its group takes: neighbor m, in the order forNeighbors visits them, goes to lane m % Lanes. From
 // the list when the particle's neighbors fit it, from its cells otherwise; the same pairs reach the
 // same lanes in the same order either way.
 template <int D, typename Real, int Lanes, bool SkipCoincident, class F>
 __device__ __forceinline__ void forEachNeighbor(const RunDev& run, const int32_t* __restrict__ start,
                                                 const Records<Real>& rec, const NeighborList& list, int64_t s,
-                                                const Self<D, Real>& me, int cx, int cy, int cz, double y64, int lane,
-                                                bool uniform, F&& f) {
+                                                const Self<D, Real>& me, double y64, int lane, bool uniform, F&& f) {
     const int32_t count = list.capacity > 0 ? list.count[s] : -1;
     if (count >= 0 && count <= list.capacity) {
         const Real period = (Real)run.period;
         const Real shiftImage = me.p[1] < Real(0.5) * period ? period : -period;
-        for (int32_t m = lane; m < count; m += Lanes) {
-            const int32_t entry = list.index[(int64_t)m * list.stride + s];
+        const int32_t* column = list.index + s;
+        auto visit = [&](int32_t entry) {
             const int32_t k = entry & 0x7fffffff;
             const Real shift = entry < 0 ? shiftImage : Real(0);
             const Record<Real>& other = rec.r[k];
             Real dx, dy, dz;
             separation<D, Real>(me, other, shift, dx, dy, dz);
-            const Real r = pairDistance(dx, dy, dz, D);
-            if (SkipCoincident && r < Real(1.0e-12)) continue;
+            const Real r = sumDistance(dx, dy, dz, D);
+            if (SkipCoincident && r < Real(1.0e-12)) return;
             const Real h = uniform ? (Real)run.hmax : Real(0.5) * (me.h + fabs(signedLength<D, Real>(other)));
             f(k, dx, dy, dz, r, h);
+        };
+        // kAhead entries of a lane in flight at once; each lane still takes its entries in order
+        int32_t m = lane;
+        for (; m + (kAhead - 1) * Lanes < count; m += kAhead * Lanes) {
+            int32_t entry[kAhead];
+#pragma unroll
+            for (int u = 0; u < kAhead; ++u) entry[u] = column[(int64_t)(m + u * Lanes) * list.stride];
+#pragma unroll
+            for (int u = 0; u < kAhead; ++u) visit(entry[u]);
         }
+        for (; m < count; m += Lanes) visit(column[(int64_t)m * list.stride]);
         return;
     }
     int32_t m = 0;
-    forNeighbors<D, Real, 1, false>(run, start, rec, s, me, cx, cy, cz, y64, 0, uniform,
-                                    [&](int32_t k, Real dx, Real dy, Real dz, Real r, Real h) {
+    forNeighbors<D, Real, 1, false>(run, start, rec, s, me, y64, 0, uniform,
+                                    [&](int32_t k, Real dx, Real dy, Real dz, Real, Real h) {
                                         const bool mine = (m % Lanes) == lane;
                                         ++m;
-                                        if (!mine || (SkipCoincident && r < Real(1.0e-12))) return;
+                                        if (!mine) return;
+                                        const Real r = sumDistance(dx, dy, dz, D);
+                                        if (SkipCoincident && r < Real(1.0e-12)) return;
                                         f(k, dx, dy, dz, r, h);
                                     });
 }
@@ -427,6 +462,7 @@ __device__ __forceinline__ void forEachNeighbor(const RunDev& run, const int32_t
 struct Tiles {
     const int32_t* run;     // run of each workgroup
     const int64_t* first;   // first sorted particle of each workgroup
+    int64_t count;
 };
 
 // The workgroup's place in the launch with the blocks of each XCD made contiguous: an MI355X deals

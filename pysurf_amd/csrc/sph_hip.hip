@@ -321,8 +321,7 @@ __global__ void gatherRecords(const RunDev* __restrict__ runs, const int32_t* __
 // ---------------------------------------------------------------------------------------------
 
 template <int D, typename Real>
-__device__ __forceinline__ Self<D, Real> loadSelf(const RunDev& run, const Records<Real>& rec, int64_t s, int& cx,
-                                                  int& cy, int& cz) {
+__device__ __forceinline__ Self<D, Real> loadSelf(const RunDev& run, const Records<Real>& rec, int64_t s) {
     Self<D, Real> me;
     const Record<Real>& own = rec.r[s];
     const Quad<Real> a = own.q0;
@@ -350,10 +349,7 @@ __device__ __forceinline__ Self<D, Real> loadSelf(const RunDev& run, const Recor
     me.fluid = sh > Real(0);
     me.h = fabs(sh);
     me.reach = me.h + (Real)run.hmax;
-    const int64_t local = (int64_t)rec.key[s] - run.cellFirst;
-    cx = (int)(local % run.nx);
-    cy = (int)((local / run.nx) % run.ny);
-    cz = D == 3 ? (int)(local / ((int64_t)run.nx * run.ny)) : 0;
+    me.cell = (int32_t)((int64_t)rec.key[s] - run.cellFirst);
     return me;
 }
 
@@ -538,8 +534,9 @@ __global__ void __launch_bounds__(256, SPH_WAVES_PER_EU)
     bool fluidI = false;
     if (valid) {
         const RunDev& run = runs[r];
+        const Self<D, Real> me = loadSelf<D, Real>(run, rec, s);
         int cx, cy, cz;
-        const Self<D, Real> me = loadSelf<D, Real>(run, rec, s, cx, cy, cz);
+        cellCoordinates(run, me.cell, cx, cy, cz);
         const bool uniform = run.uniform != 0;
         PairKernel<Real, D> kernel;
         kernel.init(run.kernel, run.sigmaUnit, run.hmax, uniform);
@@ -608,15 +605,14 @@ __global__ void __launch_bounds__(256, SPH_WAVES_PER_EU)
     (void)n;
     if (valid) {
         const RunDev& run = runs[r];
-        int cx, cy, cz;
-        const Self<D, Real> me = loadSelf<D, Real>(run, rec, s, cx, cy, cz);
+        const Self<D, Real> me = loadSelf<D, Real>(run, rec, s);
         fluidI = me.fluid;
         sweep.begin(run, rec, s, sweep1Mode(run, fluidI, resident, doWall, doGradients));
         if (sweep.mode) {
             PairKernel<Real, D> kernel;
             kernel.init(run.kernel, run.sigmaUnit, run.hmax, run.uniform != 0);
             const double yy = run.period > 0.0 ? y64[order[s]] : 0.0;
-            forNeighbors<D, Real, Lanes, false>(run, start, rec, s, me, cx, cy, cz, yy, lane, kernel.uniform,
+            forNeighbors<D, Real, Lanes, false>(run, start, rec, s, me, yy, lane, kernel.uniform,
                                                 [&](int32_t k, Real dx, Real dy, Real dz, Real rr, Real h) {
                                                     sweep.add(rec, kernel, k, dx, dy, dz, rr, h);
                                                 });
@@ -651,8 +647,7 @@ __global__ void __launch_bounds__(256, SPH_WAVES_PER_EU)
         const RunDev& run = runs[r];
         correction = doCorrection && (!resident || run.correction);
         const bool limiter = resident ? run.limiter != 0 : hostLimiter != 0;
-        int cx, cy, cz;
-        const Self<D, Real> me = loadSelf<D, Real>(run, rec, s, cx, cy, cz);
+        const Self<D, Real> me = loadSelf<D, Real>(run, rec, s);
         fluidI = me.fluid;
         if (fluidI) {
             PairKernel<Real, D> kernel;
@@ -670,7 +665,7 @@ __global__ void __launch_bounds__(256, SPH_WAVES_PER_EU)
                     for (int b = 0; b < D; ++b) Gi[a][b] = rec.G[s * (D * D) + a * D + b];
             const Real alphaC0 = (Real)(run.alpha * run.c0), bf = (Real)run.boundaryFactor;
             forEachNeighbor<D, Real, Lanes, true>(
-                run, start, rec, list, s, me, cx, cy, cz, yy, lane, kernel.uniform,
+                run, start, rec, list, s, me, yy, lane, kernel.uniform,
                 [&](int32_t k, Real dx, Real dy, Real dz, Real rr, Real h) {
                     const Record<Real>& other = rec.r[k];
                     const Real F = kernel.slopeOverR(rr, h);
@@ -925,8 +920,7 @@ __global__ void __launch_bounds__(256, SPH_WAVES_PER_EU)
     (void)n;
     if (activeRun) {
         const RunDev& run = runs[r];
-        int cx, cy, cz;
-        const Self<D, Real> me = loadSelf<D, Real>(run, rec, s, cx, cy, cz);
+        const Self<D, Real> me = loadSelf<D, Real>(run, rec, s);
         boundaryI = !me.fluid;
         const int diffusion = resident ? run.diffusion : hostDiffusion;
         const bool correction = (resident ? run.correction != 0 : hostCorrection != 0) && !boundaryI;
@@ -948,7 +942,7 @@ __global__ void __launch_bounds__(256, SPH_WAVES_PER_EU)
         const Real invRho0 = (Real)(1.0 / run.rho0);
         const Real head = (Real)(run.rho0 * fabs(run.gravity[D - 1]) / (run.c0 * run.c0));
         forEachNeighbor<D, Real, Lanes, true>(
-            run, start, rec, list, s, me, cx, cy, cz, yy, lane, kernel.uniform,
+            run, start, rec, list, s, me, yy, lane, kernel.uniform,
             [&](int32_t k, Real dx, Real dy, Real dz, Real rr, Real h) {
                 const Record<Real>& other = rec.r[k];
                 const bool fluidJ = signedLength<D, Real>(other) > Real(0);
@@ -1038,13 +1032,12 @@ __global__ void __launch_bounds__(256, SPH_WAVES_PER_EU)
     const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const RunDev& run = runs[runOf[s]];
-    int cx, cy, cz;
-    const Self<D, Real> me = loadSelf<D, Real>(run, rec, s, cx, cy, cz);
+    const Self<D, Real> me = loadSelf<D, Real>(run, rec, s);
     const int32_t i = order[s];
     const double yy = run.period > 0.0 ? y64[i] : 0.0;
     int32_t count = 0;
     const int64_t base = rows ? offsets[i] : 0;
-    forEachNeighbor<D, Real, 1, false>(run, start, rec, list, s, me, cx, cy, cz, yy, 0, run.uniform != 0,
+    forEachNeighbor<D, Real, 1, false>(run, start, rec, list, s, me, yy, 0, run.uniform != 0,
                                        [&](int32_t k, Real, Real, Real, Real, Real) {
                                            if (rows) rows[base + count] = order[k];
                                            ++count;
@@ -1348,7 +1341,7 @@ int ensureTiles(sph_context* ctx) {
     return SPH_OK;
 }
 
-inline Tiles tilesOf(const sph_context* ctx) { return Tiles{ctx->tileRun, ctx->tileFirst}; }
+inline Tiles tilesOf(const sph_context* ctx) { return Tiles{ctx->tileRun, ctx->tileFirst, ctx->tileCount}; }
 inline unsigned tileGrid(const sph_context* ctx) { return (unsigned)ctx->tileCount; }
 
 template <typename Real>
