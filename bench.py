@@ -93,6 +93,7 @@ def parse():
     parser.add_argument('--no-tune', action = 'store_true', help = 'the default launch shape instead of the tuned one')
     parser.add_argument('--lanes', type = int, default = 0, help = 'lanes per particle of the batched sweep (0: tuned)')
     parser.add_argument('--block', type = int, default = 256, help = 'workgroup threads with --lanes')
+    parser.add_argument('--no-sort', action = 'store_true', help = 'keep the batched state in slot order')
     parser.add_argument('--profile-steps', type = int, default = 20, help = 'stream-launched steps timed per kernel')
     parser.add_argument('--plain-sample', type = int, default = 16, help = 'runs the plain float64 kernel steps alone')
     parser.add_argument('--plain-steps', type = int, default = 200)
@@ -114,8 +115,16 @@ def log(message: str) -> None:
     print(f'# {message}', file = sys.stderr, flush = True)
 
 
-def timedSteps(batch, steps: int, chunk: int) -> tuple[float, float]:
-    '''Wall seconds between device synchronizations, and event seconds on the engine's stream, for `steps` graph-replayed steps.'''
+def timedSteps(batch, steps: int, chunk: int, sort: bool = False) -> tuple[float, float]:
+
+    '''
+
+    Wall seconds between device synchronizations, and event seconds on the engine's stream, for
+    `steps` graph-replayed steps; with `sort`, the state is sorted into cell order after every chunk,
+    inside the timed region.
+
+    '''
+
     import torch
     stream = batch.engine.stream
     start, end = torch.cuda.Event(enable_timing = True), torch.cuda.Event(enable_timing = True)
@@ -126,6 +135,8 @@ def timedSteps(batch, steps: int, chunk: int) -> tuple[float, float]:
     while done < steps:
         take = min(chunk, steps - done)
         batch.advance(take, graph = True)
+        if sort:
+            batch.sortParticles()
         done += take
     end.record(stream)
     torch.cuda.synchronize()
@@ -406,7 +417,9 @@ def main() -> int:
         batch = BatchEngine(cases, 'float32', tune = not args.no_tune)
     log(f'launch shape: {batch.launch.get("lanes")} lanes, {batch.launch.get("block")} threads')
     warm(batch, args.warmup, args.chunk)
-    wall, events = timedSteps(batch, args.steps, args.chunk)
+    if not args.no_sort:
+        batch.sortParticles()
+    wall, events = timedSteps(batch, args.steps, args.chunk, sort = not args.no_sort)
     value = fluid * args.steps / wall
     msPerStep = 1000.0 * wall / args.steps
     clock = batch.clocks()
@@ -443,6 +456,7 @@ def main() -> int:
                    'runs': len(cases), 'particles': particles, 'fluid_particles': fluid,
                    'run_particles_min': int(sizes.min()), 'run_particles_max': int(sizes.max()),
                    'kernel': 'Wendland C2, h = 1.3 dx', 'graph_chunk': args.chunk,
+                   'state_sort': None if args.no_sort else f'into cell order after every {args.chunk} steps, timed',
                    'launch': launch,
                    'event_ms_per_step': 1000.0 * events / args.steps,
                    'simulated_seconds_min': float(clock['time'].min()),

@@ -81,9 +81,9 @@ typedef struct {
     double* density;       /* 1 row */
     void* pressure;        /* Real, 1 row */
     void* acceleration;    /* Real, dims rows */
-    const void* mass;      /* Real */
-    const void* length;    /* Real, each particle's smoothing length */
-    const uint8_t* fluid;  /* 1 for fluid, 0 for boundary */
+    void* mass;            /* Real; read by the steps, permuted by sph_sort_state */
+    void* length;          /* Real, each particle's smoothing length; likewise */
+    uint8_t* fluid;        /* 1 for fluid, 0 for boundary; likewise */
     int64_t stride;        /* elements between rows */
 } sph_state;
 
@@ -152,6 +152,13 @@ int sph_integrate(sph_context* ctx, double timeStep);
  * so the sums repeat bit for bit. Synchronizes. */
 int sph_global_sums(sph_context* ctx, double* sums);
 
+/* Permutes the bound state, every field, into the cell order of the current positions, each run
+ * within its own slots, carrying `identity` (one int32 per slot; may be null) along:
+ * WcsphSolver.sortParticles (sphSolver.py:2093-2126) on the device, with the boundary sorted too.
+ * Gathers through the sorted order then read neighboring slots. Asynchronous; must not be called
+ * during graph capture (it may allocate its copy buffer on first use). */
+int sph_sort_state(sph_context* ctx, int32_t* identity);
+
 int sph_synchronize(sph_context* ctx);
 
 /* Per-kernel timing with HIP events recorded on the context's stream around every launch of a
@@ -160,7 +167,8 @@ int sph_synchronize(sph_context* ctx);
  * synchronizes and copies, per kernel id, the accumulated milliseconds and launch counts:
  * 0 clock, 1 cell keys, 2 cell-count scan, 3 cell scatter, 4 cell order, 5 record gather,
  * 6 sweep 1 (wall pressure, G), 7 sweep 2 (accelerations, L), 8 kick, 9 sweep 3 (continuity,
- * density advance, drift, enforcement), 10 neighbor list. */
+ * density advance, drift, enforcement), 10 neighbor list, 11 enforcement alone, 12 the runs'
+ * maxima. */
 int sph_profile(sph_context* ctx, int32_t enable);
 int sph_kernel_times(sph_context* ctx, double* milliseconds, int64_t* launches, int32_t count);
 

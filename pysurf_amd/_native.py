@@ -80,6 +80,7 @@ signatures = {
     'sph_integrate': [ctypes.c_void_p, ctypes.c_double],
     'sph_synchronize': [ctypes.c_void_p],
     'sph_global_sums': [ctypes.c_void_p, ctypes.c_void_p],
+    'sph_sort_state': [ctypes.c_void_p, ctypes.c_void_p],
     'sph_set_launch': [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32],
     'sph_profile': [ctypes.c_void_p, ctypes.c_int32],
     'sph_kernel_times': [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32],
@@ -90,7 +91,7 @@ globalSumNames = ['mass', 'kinetic', 'potential', 'elastic', 'momentumX', 'momen
 
 # Kernel ids of sph_kernel_times
 kernelNames = ['clock', 'cellKeys', 'cellScan', 'cellScatter', 'cellOrder', 'gatherRecords', 'sweepWallAndGradients',
-               'sweepAccelerations', 'kick', 'sweepContinuity', 'buildNeighbors']
+               'sweepAccelerations', 'kick', 'sweepContinuity', 'buildNeighbors', 'enforce', 'maxima']
 
 _library = None
 

@@ -101,6 +101,7 @@ class BatchEngine:
         self.offsets.append(first)
         self.engine.configure(runs)
         self.engine.setLaunch(blockSize, lanes, listCapacity)
+        self.identity = None
         self.load()
         self.launch = {'lanes': lanes, 'block': blockSize, 'listCapacity': listCapacity}
         if tune:
@@ -116,8 +117,9 @@ class BatchEngine:
         return sum(case.fluidCount for case in self.cases)
 
     def load(self) -> None:
-        '''Copy every run's particle state into the engine.'''
+        '''Copy every run's particle state into the engine, in the cases' own order.'''
         engine = self.engine
+        self.identity = None
         parts = [case.particles for case in self.cases]
         engine.upload('position', np.vstack([p.positions for p in parts]))
         engine.upload('velocity', np.vstack([p.velocities for p in parts]))
@@ -147,6 +149,33 @@ class BatchEngine:
     def clocks(self) -> dict:
         return self.engine.clock(len(self.cases))
 
+    def sortParticles(self) -> None:
+
+        '''
+
+        Permute every run's particles into the cell order of their current positions, on the device
+        (WcsphSolver.sortParticles, sphSolver.py:2093-2126): the gathers of the next steps then
+        read neighboring slots. The engine remembers each slot's particle, so `state` still returns
+        particles in the cases' order. Asynchronous.
+
+        '''
+
+        import torch
+        if self.identity is None:
+            with torch.cuda.device(self.device):
+                self.identity = torch.arange(max(self.count, 1), dtype = torch.int32, device = self.device)
+                torch.cuda.current_stream(self.device).synchronize()
+        self.engine.sortState(self.identity)
+
+    def _original(self, values: np.ndarray, a: int, b: int) -> np.ndarray:
+        '''Slots a..b of a downloaded field, back in the case's particle order.'''
+        if self.identity is None:
+            return values[a:b]
+        identity = self.identity[a:b].cpu().numpy().astype(np.int64) - a
+        ordered = np.empty_like(values[a:b])
+        ordered[identity] = values[a:b]
+        return ordered
+
     def globalSums(self) -> np.ndarray:
         '''Every run's float64 sums over its fluid, shape (runs, 7), columns as _native.globalSumNames.'''
         return self.engine.globalSums(len(self.cases))
@@ -156,9 +185,9 @@ class BatchEngine:
         a, b = self.offsets[index], self.offsets[index + 1]
         engine, d = self.engine, self.dimensions
         particles = self.cases[index].particles.copy()
-        particles.positions = engine.download('position', d, b)[a:b]
-        particles.velocities = engine.download('velocity', d, b)[a:b]
-        particles.densities = engine.download('density', None, b)[a:b]
-        particles.pressures = engine.download('pressure', None, b)[a:b]
-        particles.accelerations = engine.download('acceleration', d, b)[a:b]
+        particles.positions = self._original(engine.download('position', d, b), a, b)
+        particles.velocities = self._original(engine.download('velocity', d, b), a, b)
+        particles.densities = self._original(engine.download('density', None, b), a, b)
+        particles.pressures = self._original(engine.download('pressure', None, b), a, b)
+        particles.accelerations = self._original(engine.download('acceleration', d, b), a, b)
         return particles

@@ -36,6 +36,8 @@ struct Clock {
     int64_t* maxSteps;
     unsigned long long* maxima;   // 2 per run: |v|max and |a|max as float64 bit patterns
     int32_t* active;
+    double* speed;                // per sorted slot: |v| of a fluid particle at the gather, else 0
+    double* acceleration;         // per sorted slot: |a| of a fluid particle, else 0
 };
 
 template <typename Real>

@@ -148,31 +148,56 @@ __device__ __forceinline__ double runTimeStep(const RunDev& run, const Clock& cl
     return fmin(dt, run.maxTimeStep);
 }
 
-// max(*target, bits), as an atomic only when bits exceed the value already there: once a run's
-// maximum is reached the waves that follow read it and leave, rather than queueing atomics on one
-// address. A stale read can only cost an atomic that changes nothing.
-__device__ __forceinline__ void raiseMax(unsigned long long* target, unsigned long long bits) {
-    if (bits == 0ull) return;
-    const unsigned long long seen = __hip_atomic_load(target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (bits > seen) atomicMax(target, bits);
+constexpr int kSums = 7;              // fluid mass, kinetic, potential, elastic energy, momentum x, y, z
+constexpr int64_t kSumChunk = 8192;   // slots per partial sum and per partial maximum
+
+// Each run's maxima from the per-slot magnitudes the step wrote (|v| at the gather, |a| in the
+// acceleration sweep; zero on boundary particles): a maximum over each chunk of a run's slots,
+// then over the run's chunks, written as the run's float64 maxima. No atomics, so no waves queue on
+// one address, and a maximum does not depend on the order it is taken in.
+__global__ void __launch_bounds__(256)
+    chunkMaxima(const int32_t* __restrict__ chunkRun, const int64_t* __restrict__ chunkFirst, const RunDev* __restrict__ runs,
+                const double* __restrict__ speed, const double* __restrict__ acceleration, Clock clock,
+                double* __restrict__ partial) {
+    __shared__ double top[2][256];
+    const int64_t b = blockIdx.x;
+    const int r = chunkRun[b];
+    if (!clock.active[r]) return;
+    const RunDev& run = runs[r];
+    const int64_t first = chunkFirst[b];
+    const int64_t end = min(first + kSumChunk, run.first + run.count);
+    double v = 0.0, a = 0.0;
+    for (int64_t i = first + threadIdx.x; i < end; i += blockDim.x) {
+        v = fmax(v, speed[i]);
+        a = fmax(a, acceleration[i]);
+    }
+    top[0][threadIdx.x] = v;
+    top[1][threadIdx.x] = a;
+    __syncthreads();
+    for (int width = blockDim.x / 2; width > 0; width >>= 1) {
+        if ((int)threadIdx.x < width) {
+            top[0][threadIdx.x] = fmax(top[0][threadIdx.x], top[0][threadIdx.x + width]);
+            top[1][threadIdx.x] = fmax(top[1][threadIdx.x], top[1][threadIdx.x + width]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partial[2 * b] = top[0][0];
+        partial[2 * b + 1] = top[1][0];
+    }
 }
 
-// The largest value of a wave's lanes of one run, folded into maxima[slot] with one atomic when
-// the wave holds one run, lane by lane otherwise. Every lane of the wave must call it.
-__device__ __forceinline__ void waveMaxInto(unsigned long long* maxima, int run, bool valid, double value) {
-    unsigned long long bits = valid ? (unsigned long long)__double_as_longlong(value) : 0ull;
-    const int first = __shfl(run, 0, 64);
-    const bool uniform = __all(run == first || !valid);
-    if (uniform) {
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            const unsigned long long other = __shfl_xor(bits, o, 64);
-            bits = other > bits ? other : bits;
-        }
-        if ((threadIdx.x & 63) == 0) raiseMax(maxima + 2 * first, bits);
-    } else if (valid) {
-        raiseMax(maxima + 2 * run, bits);
+__global__ void runMaxima(const int64_t* __restrict__ runChunk, const double* __restrict__ partial, Clock clock,
+                          int32_t runs) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= runs || !clock.active[r]) return;
+    double v = 0.0, a = 0.0;
+    for (int64_t b = runChunk[r]; b < runChunk[r + 1]; ++b) {
+        v = fmax(v, partial[2 * b]);
+        a = fmax(a, partial[2 * b + 1]);
     }
+    clock.maxima[2 * r] = (unsigned long long)__double_as_longlong(v);
+    clock.maxima[2 * r + 1] = (unsigned long long)__double_as_longlong(a);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -313,7 +338,7 @@ __global__ void gatherRecords(const RunDev* __restrict__ runs, const int32_t* __
         speed = sqrt(v2);
         valid = isFluid;
     }
-    if (resident) waveMaxInto(clock.maxima, r, valid, speed);
+    if (resident && inRange && clock.active[r]) clock.speed[s] = valid ? speed : 0.0;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -789,7 +814,7 @@ __global__ void __launch_bounds__(256, SPH_WAVES_PER_EU)
             }
         }
     }
-    if (resident && doAcceleration) waveMaxInto(clock.maxima + 1, r, activeRun && fluidI && lane == 0, magnitude);
+    if (resident && doAcceleration && activeRun && lane == 0) clock.acceleration[s] = fluidI ? magnitude : 0.0;
 }
 
 // Applies v += a dt on the fluid (sphGpu.py:1084-1109), refreshing the velocity records; the run's
@@ -1049,8 +1074,6 @@ __global__ void __launch_bounds__(256, SPH_WAVES_PER_EU)
 // Global float64 sums
 // ---------------------------------------------------------------------------------------------
 
-constexpr int kSums = 7;              // fluid mass, kinetic, potential, elastic energy, momentum x, y, z
-constexpr int64_t kSumChunk = 8192;   // slots per partial sum
 
 // The elastic energy per unit mass the Tait equation stores at density rho (sphEnergy.py:120-160,
 // sphGpu.py:271-282): e = (c0^2 / 7) x^2 (21 + 35x + 35x^2 + 21x^3 + 7x^4 + x^5) / (6 (1 + x)) with
@@ -1114,6 +1137,32 @@ __global__ void sumRuns(const int64_t* __restrict__ runChunk, const double* __re
     for (int e = 0; e < kSums; ++e) out[(int64_t)r * kSums + e] = acc[e];
 }
 
+// ---------------------------------------------------------------------------------------------
+// State order
+// ---------------------------------------------------------------------------------------------
+
+// Slot s of the bound state takes what slot order[s] held (read from the copy `from`), in every
+// field and in `identity`: each run's particles in the cell order of the last cell build, within the
+// run's own slots (WcsphSolver.sortParticles, sphSolver.py:2093-2126, on the device).
+template <typename Real>
+__global__ void permuteState(const int32_t* __restrict__ order, sph_state from, sph_state to,
+                             const int32_t* __restrict__ identityFrom, int32_t* identityTo, int32_t dims, int64_t n) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int64_t i = order[s], stride = to.stride;
+    for (int a = 0; a < dims; ++a) {
+        to.position[a * stride + s] = from.position[a * stride + i];
+        to.velocity[a * stride + s] = from.velocity[a * stride + i];
+        static_cast<Real*>(to.acceleration)[a * stride + s] = static_cast<const Real*>(from.acceleration)[a * stride + i];
+    }
+    to.density[s] = from.density[i];
+    static_cast<Real*>(to.pressure)[s] = static_cast<const Real*>(from.pressure)[i];
+    static_cast<Real*>(to.mass)[s] = static_cast<const Real*>(from.mass)[i];
+    static_cast<Real*>(to.length)[s] = static_cast<const Real*>(from.length)[i];
+    to.fluid[s] = from.fluid[i];
+    if (identityTo) identityTo[s] = identityFrom[i];
+}
+
 inline unsigned gridFor(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
 
 // Runs BODY with the compile-time constant L set to the context's lanes per particle.
@@ -1141,6 +1190,9 @@ struct sph_context {
     int lanes = 0;           // lanes per particle in the sweeps; 0 picks by dimension
     int listCapacity = -1;   // neighbor-list entries per particle; -1 picks by dimension, 0 walks the cells
     NeighborList list{};
+    // A copy of the bound state the state sort permutes from, allocated on first use
+    void* sortCopy = nullptr;
+    size_t sortBytes = 0;
     // The sweeps' tiles for the current workgroup size and lanes (tilePer particles each)
     int32_t* tileRun = nullptr;
     int64_t* tileFirst = nullptr;
@@ -1245,6 +1297,9 @@ void releaseScratch(sph_context* ctx) {
     ctx->tileFirst = nullptr;
     ctx->tileCount = 0;
     ctx->tilePer = 0;
+    if (ctx->sortCopy) (void)hipFree(ctx->sortCopy);
+    ctx->sortCopy = nullptr;
+    ctx->sortBytes = 0;
     for (void* p : ctx->allocations) (void)hipFree(p);
     ctx->allocations.clear();
     ctx->dRuns = nullptr;
@@ -1359,7 +1414,7 @@ int launchCheck(sph_context* ctx, const char* what) {
 
 // Kernel ids of the per-kernel timing: include/sph_hip.h lists them for sph_kernel_times.
 enum KernelId { kClock = 0, kCellKeys, kScan, kCellScatter, kCellOrder, kGather, kSweepWall, kSweepAcceleration,
-                kKick, kSweepContinuity, kNeighbors, kEnforce, kKernelKinds };
+                kKick, kSweepContinuity, kNeighbors, kEnforce, kMaxima, kKernelKinds };
 
 // Records a start event before a timed launch; returns its slot, or -1 while not profiling.
 int timeBegin(sph_context* ctx, int kernel) {
@@ -1513,6 +1568,14 @@ int stepOnce(sph_context* ctx, int phase, int useClock, double forcedDt) {
                               (sweepAccelerations<D, Real, L><<<tileGrid(ctx), b, 0, ctx->stream>>>(
                             ctx->dRuns, tilesOf(ctx), ctx->order, ctx->cellStart, rec, ctx->list, out, ctx->clock,
                                   y64, 1, 1, 1, 0, n))));
+        if (ctx->sumChunkCount > 0) {
+            const int slot = timeBegin(ctx, kMaxima);
+            chunkMaxima<<<(unsigned)ctx->sumChunkCount, 256, 0, ctx->stream>>>(
+                ctx->sumChunkRun, ctx->sumChunkFirst, ctx->dRuns, ctx->clock.speed, ctx->clock.acceleration, ctx->clock,
+                ctx->sumPartial);
+            runMaxima<<<gridFor(nruns, 64), 64, 0, ctx->stream>>>(ctx->sumRunChunk, ctx->sumPartial, ctx->clock, nruns);
+            timeEnd(ctx, slot);
+        }
     }
     if (phase & 2) {
         TIMED(ctx, kKick,
@@ -1778,6 +1841,8 @@ int sph_configure(sph_context* ctx, const sph_run* runs, int32_t count, int64_t 
     if (status == SPH_OK) status = allocate(ctx, &ctx->clock.maxSteps, count);
     if (status == SPH_OK) status = allocate(ctx, &ctx->clock.maxima, 2 * (size_t)count);
     if (status == SPH_OK) status = allocate(ctx, &ctx->clock.active, count);
+    if (status == SPH_OK) status = allocate(ctx, &ctx->clock.speed, cap);
+    if (status == SPH_OK) status = allocate(ctx, &ctx->clock.acceleration, cap);
     if (status == SPH_OK) status = allocate(ctx, &ctx->dVel, 3);
     std::vector<int32_t> chunkRun;
     std::vector<int64_t> chunkFirst, runChunk(1, 0);
@@ -1978,6 +2043,52 @@ int sph_rates(sph_context* ctx, double* maxima) {
         HIP_TRY(hipStreamSynchronize(ctx->stream));
     }
     return SPH_OK;
+}
+
+int sph_sort_state(sph_context* ctx, int32_t* identity) {
+    int status = requireBound(ctx);
+    if (status != SPH_OK) return status;
+    const int64_t n = ctx->total;
+    if (n == 0) return SPH_OK;
+    const int D = ctx->dims;
+    const size_t real = ctx->precision == 0 ? sizeof(float) : sizeof(double);
+    const size_t stride = (size_t)ctx->state.stride;
+    const size_t sizes[9] = {8 * D * stride, 8 * D * stride, 8 * stride, real * stride, real * D * stride, real * stride,
+                             real * stride, stride, identity ? 4 * stride : 0};
+    size_t offsets[9], total = 0;
+    for (int f = 0; f < 9; ++f) {
+        offsets[f] = total;
+        total += (sizes[f] + 255) / 256 * 256;
+    }
+    if (total > ctx->sortBytes) {
+        if (ctx->sortCopy) (void)hipFree(ctx->sortCopy);
+        ctx->sortCopy = nullptr;
+        ctx->sortBytes = 0;
+        HIP_TRY(hipMalloc(&ctx->sortCopy, total));
+        ctx->sortBytes = total;
+    }
+    char* base = static_cast<char*>(ctx->sortCopy);
+    sph_state from = ctx->state;
+    void* fields[9] = {ctx->state.position, ctx->state.velocity, ctx->state.density, ctx->state.pressure,
+                       ctx->state.acceleration, ctx->state.mass, ctx->state.length, ctx->state.fluid, identity};
+    for (int f = 0; f < 9; ++f)
+        if (sizes[f]) HIP_TRY(hipMemcpyAsync(base + offsets[f], fields[f], sizes[f], hipMemcpyDeviceToDevice, ctx->stream));
+    from.position = reinterpret_cast<double*>(base + offsets[0]);
+    from.velocity = reinterpret_cast<double*>(base + offsets[1]);
+    from.density = reinterpret_cast<double*>(base + offsets[2]);
+    from.pressure = base + offsets[3];
+    from.acceleration = base + offsets[4];
+    from.mass = base + offsets[5];
+    from.length = base + offsets[6];
+    from.fluid = reinterpret_cast<uint8_t*>(base + offsets[7]);
+    const int32_t* identityFrom = identity ? reinterpret_cast<const int32_t*>(base + offsets[8]) : nullptr;
+    status = D == 2 ? buildCells<2>(ctx) : buildCells<3>(ctx);
+    if (status != SPH_OK) return status;
+    if (ctx->precision == 0)
+        permuteState<float><<<gridFor(n, 256), 256, 0, ctx->stream>>>(ctx->order, from, ctx->state, identityFrom, identity, D, n);
+    else
+        permuteState<double><<<gridFor(n, 256), 256, 0, ctx->stream>>>(ctx->order, from, ctx->state, identityFrom, identity, D, n);
+    return launchCheck(ctx, "state sort");
 }
 
 int sph_global_sums(sph_context* ctx, double* sums) {

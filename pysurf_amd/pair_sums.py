@@ -221,6 +221,11 @@ class Engine:
     def advance(self, steps: int, graph: bool = True) -> None:
         self.check(self.library.sph_advance(self.handle, int(steps), 1 if graph else 0), 'sph_advance')
 
+    def sortState(self, identity: torch.Tensor | None = None) -> None:
+        '''Permute the state into cell order within each run, carrying `identity` (int32 per slot) along (sph_sort_state).'''
+        pointer = ctypes.c_void_p(identity.data_ptr()) if identity is not None else None
+        self.check(self.library.sph_sort_state(self.handle, pointer), 'sph_sort_state')
+
     def globalSums(self, runs: int) -> np.ndarray:
         '''Each run's float64 sums over its fluid (sph_global_sums), shape (runs, 7) in _native.globalSumNames order.'''
         sums = np.zeros((runs, len(_native.globalSumNames)))

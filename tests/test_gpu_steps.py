@@ -284,3 +284,62 @@ def testHeightFieldEnforcementMatchesTheOracle():
     assert np.count_nonzero(np.any(x != positions, axis = 1)) > 0.2 * count
     assert particles.positions == pytest.approx(x, abs = 1e-12)
     assert particles.velocities == pytest.approx(v, abs = 1e-12)
+
+
+def testSortingTheStateKeepsEveryParticle():
+    # The device sort permutes the slots, and the engine hands the particles back in their own order
+    from pysurf_amd.batch import BatchEngine
+    print('state sort', flush = True)
+    cases = [smallCase(seed = 12, ppd = 6), smallCase(seed = 13, ppd = 8)]
+    batch = BatchEngine(cases, 'float32')
+    batch.setClocks(maxSteps = 10)
+    batch.advance(10)
+    before = [batch.state(index) for index in range(2)]
+    batch.sortParticles()
+    for index in range(2):
+        after = batch.state(index)
+        for name in ('positions', 'velocities', 'densities'):
+            assert np.array_equal(getattr(before[index], name), getattr(after, name)), (index, name)
+    identity = batch.identity.cpu().numpy()
+    assert not np.array_equal(identity, np.arange(batch.count))
+    assert np.array_equal(np.sort(identity[:cases[0].count]), np.arange(cases[0].count))
+
+
+@pytest.mark.parametrize('precision, tolerances', [('float64', (1e-10, 1e-8, 1e-9)), ('float32', (1e-6, 1e-3, 1e-3))])
+def testSortedStepsFollowTheOracle(precision, tolerances):
+    # Sorting every 10 steps changes only the order of the sums' terms
+    from pysurf_amd.batch import BatchEngine
+    print(f'sorted steps {precision}', flush = True)
+    case = smallCase(seed = 14)
+    steps = 60
+    expected = oracleRun(case, steps)
+    batch = BatchEngine([case], precision)
+    batch.setClocks(maxSteps = steps)
+    for _ in range(steps // 10):
+        batch.advance(10)
+        batch.sortParticles()
+    state = batch.state(0)
+    positionTol, velocityTol, densityTol = tolerances
+    assert int(batch.clocks()['steps'][0]) == steps
+    assert state.positions == pytest.approx(expected.positions, abs = positionTol)
+    assert state.velocities == pytest.approx(expected.velocities, abs = velocityTol)
+    assert state.densities == pytest.approx(expected.dens, abs = densityTol)
+
+
+def testSortedRunsAloneAndInABatchAgreeBitForBit():
+    from pysurf_amd.batch import BatchEngine
+    cases = [smallCase(seed = 15, ppd = 6), smallCase(seed = 16, ppd = 8)]
+
+    def stepped(selection):
+        batch = BatchEngine(selection, 'float32')
+        batch.setClocks(maxSteps = 30)
+        for _ in range(3):
+            batch.advance(10)
+            batch.sortParticles()
+        return [batch.state(index) for index in range(len(selection))]
+
+    together = stepped(cases)
+    for index, case in enumerate(cases):
+        alone = stepped([case])[0]
+        for name in ('positions', 'velocities', 'densities'):
+            assert np.array_equal(getattr(alone, name), getattr(together[index], name)), (index, name)
