@@ -204,6 +204,23 @@ __global__ void runMaxima(const int64_t* __restrict__ runChunk, const double* __
 // Cell list
 // ---------------------------------------------------------------------------------------------
 
+// Runs of equal keys in consecutive lanes of a wave (a lane with `valid` false stands alone): the
+// lane's run's first lane, the run's length and the lane's rank in it. Slots sorted into cell order
+// put whole cells in one run, which then takes one atomic for the cell instead of one per slot.
+// Every lane of the wave must call it; workgroups are whole waves.
+__device__ __forceinline__ void equalRuns(int32_t key, bool valid, int& head, int& length, int& rank) {
+    const int lane = (int)(threadIdx.x & 63);
+    const int32_t previous = __shfl_up(key, 1, 64);
+    const int previousValid = __shfl_up(valid ? 1 : 0, 1, 64);
+    const bool starts = !valid || lane == 0 || !previousValid || previous != key;
+    const unsigned long long heads = __ballot(starts);
+    const unsigned long long upTo = 2ull << lane;   // wraps to 0 for lane 63, so upTo - 1 keeps every lane
+    head = 63 - __clzll(heads & (upTo - 1));
+    const unsigned long long after = heads & ~(upTo - 1);
+    length = (after ? __ffsll(after) - 1 : 64) - head;
+    rank = lane - head;
+}
+
 // Each slot's cell and the cells' counts. With `clockMode` the first slot of each run also opens the
 // run's step: 1 sets its active flag from its clock, 2 makes it active whatever the clock; both
 // clear its maxima.
@@ -212,28 +229,38 @@ __global__ void cellKeys(const RunDev* __restrict__ runs, const int32_t* __restr
                          const double* __restrict__ y, const double* __restrict__ z, int64_t n, int32_t* __restrict__ key,
                          int32_t* __restrict__ count, Clock clock, int32_t clockMode) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int r = runOf[i];
-    const RunDev& run = runs[r];
-    if (clockMode && i == run.first) {
-        clock.active[r] = clockMode == 2 || (clock.time[r] < clock.until[r] && clock.steps[r] < clock.maxSteps[r]);
-        clock.maxima[2 * r] = 0ull;
-        clock.maxima[2 * r + 1] = 0ull;
+    const bool valid = i < n;
+    int32_t k = 0;
+    if (valid) {
+        const int r = runOf[i];
+        const RunDev& run = runs[r];
+        if (clockMode && i == run.first) {
+            clock.active[r] = clockMode == 2 || (clock.time[r] < clock.until[r] && clock.steps[r] < clock.maxSteps[r]);
+            clock.maxima[2 * r] = 0ull;
+            clock.maxima[2 * r + 1] = 0ull;
+        }
+        const int cx = wrapIndex(cellCoordinate(x[i], run.cell), run.nx);
+        const int cy = wrapIndex(cellCoordinate(y[i], run.cell), run.ny);
+        const int cz = D == 3 ? wrapIndex(cellCoordinate(z[i], run.cell), run.nz) : 0;
+        k = (int32_t)(run.cellFirst + ((int64_t)cz * run.ny + cy) * run.nx + cx);
+        key[i] = k;
     }
-    const int cx = wrapIndex(cellCoordinate(x[i], run.cell), run.nx);
-    const int cy = wrapIndex(cellCoordinate(y[i], run.cell), run.ny);
-    const int cz = D == 3 ? wrapIndex(cellCoordinate(z[i], run.cell), run.nz) : 0;
-    const int32_t k = (int32_t)(run.cellFirst + ((int64_t)cz * run.ny + cy) * run.nx + cx);
-    key[i] = k;
-    atomicAdd(count + k, 1);
+    int head, length, rank;
+    equalRuns(k, valid, head, length, rank);
+    if (valid && rank == 0) atomicAdd(count + k, length);
 }
 
 __global__ void cellScatter(const int32_t* __restrict__ key, const int32_t* __restrict__ start, int32_t* __restrict__ fill,
                             int32_t* __restrict__ order, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int32_t k = key[i];
-    order[start[k] + atomicAdd(fill + k, 1)] = (int32_t)i;
+    const bool valid = i < n;
+    const int32_t k = valid ? key[i] : 0;
+    int head, length, rank;
+    equalRuns(k, valid, head, length, rank);
+    int32_t base = 0;
+    if (valid && rank == 0) base = atomicAdd(fill + k, length);
+    base = __shfl(base, head, 64);
+    if (valid) order[start[k] + base + rank] = (int32_t)i;
 }
 
 // Orders each cell's slots ascending, the order a stable counting sort gives, and clears the
