@@ -46,6 +46,14 @@ using namespace sph;
 #define SPH_WAVES_ACCELERATIONS 6
 #endif
 
+// The float32 2D sweeps fit those waves without spilling; the float64 and 3D sweeps need more
+// registers than that leaves, so they plan for one wave and take what they need.
+template <int D, typename Real>
+constexpr int sweepWaves() { return (D == 2 && sizeof(Real) == 4) ? SPH_WAVES_PER_EU : 1; }
+template <int D, typename Real>
+constexpr int accelerationWaves() { return (D == 2 && sizeof(Real) == 4) ? SPH_WAVES_ACCELERATIONS : 1; }
+#define SWEEP_BOUNDS(waves) __attribute__((amdgpu_flat_work_group_size(1, 256), amdgpu_waves_per_eu(waves)))
+
 namespace {
 
 constexpr int kScanThreads = 256;
@@ -574,7 +582,7 @@ __device__ __forceinline__ void sweep1Finish(const RunDev& run, const Sweep1<D, 
 // and compacts the hits with a ballot, so the list keeps the walk's order. With `WithSweep1` the
 // same walk accumulates sweep 1's sums, each lane over the candidates it tested.
 template <int D, typename Real, int Lanes, bool WithSweep1>
-__global__ void __launch_bounds__(256, SPH_WAVES_PER_EU)
+__global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
     buildNeighbors(const RunDev* __restrict__ runs, Tiles tiles, const int32_t* __restrict__ order, const int32_t* __restrict__ start, Records<Real> rec,
                    NeighborList list, SweepOut<Real> out, Clock clock, const double* __restrict__ y64,
                    int32_t resident, int64_t n) {
@@ -646,7 +654,7 @@ __global__ void __launch_bounds__(256, SPH_WAVES_PER_EU)
 
 // Sweep 1 on its own: wall pressure on boundary particles, G on fluid particles, over the cells.
 template <int D, typename Real, int Lanes>
-__global__ void __launch_bounds__(256, SPH_WAVES_PER_EU)
+__global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
     sweepWallAndGradients(const RunDev* __restrict__ runs, Tiles tiles, const int32_t* __restrict__ order, const int32_t* __restrict__ start, Records<Real> rec,
                           NeighborList list, SweepOut<Real> out, Clock clock, const double* __restrict__ y64,
                           int32_t resident, int32_t doWall, int32_t doGradients, int64_t n) {
@@ -684,7 +692,7 @@ __global__ void __launch_bounds__(256, SPH_WAVES_PER_EU)
 // Sweep 2: accelerations with the limited viscosity (sphGpu.py:303-414) and, when asked, the
 // correction matrix L (sphGpu.py:593-641) on fluid particles.
 template <int D, typename Real, int Lanes>
-__global__ void __launch_bounds__(256, SPH_WAVES_ACCELERATIONS)
+__global__ void SWEEP_BOUNDS((accelerationWaves<D, Real>()))
     sweepAccelerations(const RunDev* __restrict__ runs, Tiles tiles, const int32_t* __restrict__ order, const int32_t* __restrict__ start, Records<Real> rec,
                        NeighborList list, SweepOut<Real> out, Clock clock, const double* __restrict__ y64,
                        int32_t resident, int32_t doAcceleration, int32_t doCorrection, int32_t hostLimiter, int64_t n) {
@@ -962,7 +970,7 @@ __device__ __forceinline__ void enforceBoundary(const sph_boundary& b, double* x
 // density advance with its floor (:1111-1140), drift (:1142-1186) and boundary enforcement
 // follow on its float64 state.
 template <int D, typename Real, int Lanes>
-__global__ void __launch_bounds__(256, SPH_WAVES_PER_EU)
+__global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
     sweepContinuity(const RunDev* __restrict__ runs, Tiles tiles, const int32_t* __restrict__ order, const int32_t* __restrict__ start, Records<Real> rec,
                     NeighborList list, SweepOut<Real> out, Clock clock, int32_t resident, int32_t hostCorrection,
                     int32_t hostDiffusion, int64_t n) {
@@ -1081,7 +1089,7 @@ __global__ void enforceKernel(const RunDev* __restrict__ runs, const int32_t* __
 // Neighbor counts (rows null) or rows as slots, in walk order: from the list where the particle's
 // neighbors fit it, from its cells otherwise.
 template <int D, typename Real>
-__global__ void __launch_bounds__(256, SPH_WAVES_PER_EU)
+__global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
     neighborExport(const RunDev* __restrict__ runs, const int32_t* __restrict__ runOf, const int32_t* __restrict__ order,
                    const int32_t* __restrict__ start, Records<Real> rec, NeighborList list,
                    const double* __restrict__ y64, int32_t* counts, const int64_t* offsets, int32_t* rows, int64_t n) {
@@ -1120,7 +1128,7 @@ __device__ __forceinline__ double elasticEnergy(double rho, double rho0, double 
 // over a fixed stride, then a fixed tree. K = m |v|^2 / 2 and P = m g z with g the downward
 // gravity along the last axis (sphParticles.py:236-255); masses as the device stores them.
 template <int D, typename Real>
-__global__ void __launch_bounds__(256, SPH_WAVES_PER_EU)
+__global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
     sumChunks(const RunDev* __restrict__ runs, const int32_t* __restrict__ chunkRun,
               const int64_t* __restrict__ chunkFirst, sph_state state, double* __restrict__ partial) {
     __shared__ double tree[kSums][256];
