@@ -163,6 +163,10 @@ __device__ __forceinline__ int wrapIndex(int64_t a, int32_t n) {
     return (int)(r < 0 ? r + n : r);
 }
 
+// c wrapped onto [0, n) for -n <= c < 2n, as a neighbor cell one step from a cell in range is:
+// a select instead of a division.
+__device__ __forceinline__ int wrapNear(int c, int32_t n) { return c < 0 ? c + n : (c >= n ? c - n : c); }
+
 // The distinct cell offsets along an axis of n cells; fewer than three cells would repeat one.
 __device__ __forceinline__ int stencilOffsets(int32_t n, int* out) {
     if (n >= 3) { out[0] = -1; out[1] = 0; out[2] = 1; return 3; }
@@ -190,16 +194,16 @@ __device__ __forceinline__ void forStencil(const RunDev& run, const int32_t* __r
     oz[0] = 0;
     if (D == 3) kz = stencilOffsets(nz, oz);
     for (int a = 0; a < kz; ++a) {
-        const int64_t plane = D == 3 ? (int64_t)wrapIndex(cz + oz[a], nz) : 0;
+        const int64_t plane = D == 3 ? (int64_t)wrapNear(cz + oz[a], nz) : 0;
         for (int b = 0; b < ky; ++b) {
-            const int64_t row = run.cellFirst + (plane * ny + wrapIndex(cy + oy[b], ny)) * nx;
+            const int64_t row = run.cellFirst + (plane * ny + wrapNear(cy + oy[b], ny)) * nx;
             if (nx >= 3 && cx >= 1 && cx <= nx - 2) {
                 g(start[row + cx - 1], start[row + cx + 2]);
             } else {
                 int ox[3];
                 const int kx = stencilOffsets(nx, ox);
                 for (int c = 0; c < kx; ++c) {
-                    const int64_t cellIndex = row + wrapIndex(cx + ox[c], nx);
+                    const int64_t cellIndex = row + wrapNear(cx + ox[c], nx);
                     g(start[cellIndex], start[cellIndex + 1]);
                 }
             }
@@ -257,18 +261,42 @@ __device__ __forceinline__ void separation(const Self<D, Real>& me, const Record
     }
 }
 
-// sqrt(fma(dz, dz, fma(dy, dy, dx * dx))), the neighbor predicate's distance, correctly rounded.
-__device__ __forceinline__ float pairDistance(float dx, float dy, float dz, int dims) {
+// fma(dz, dz, fma(dy, dy, dx * dx)): the squared separation, every fma written out so that every
+// caller rounds it alike.
+__device__ __forceinline__ float squaredDistance(float dx, float dy, float dz, int dims) {
     float r2 = dx * dx;
     r2 = fmaf(dy, dy, r2);
     if (dims == 3) r2 = fmaf(dz, dz, r2);
-    return sqrtf(r2);
+    return r2;
 }
-__device__ __forceinline__ double pairDistance(double dx, double dy, double dz, int dims) {
+__device__ __forceinline__ double squaredDistance(double dx, double dy, double dz, int dims) {
     double r2 = dx * dx;
     r2 = fma(dy, dy, r2);
     if (dims == 3) r2 = fma(dz, dz, r2);
-    return sqrt(r2);
+    return r2;
+}
+
+// The neighbor predicate's distance: the square root of squaredDistance, correctly rounded.
+__device__ __forceinline__ float pairDistance(float dx, float dy, float dz, int dims) {
+    return sqrtf(squaredDistance(dx, dy, dz, dims));
+}
+__device__ __forceinline__ double pairDistance(double dx, double dy, double dz, int dims) {
+    return sqrt(squaredDistance(dx, dy, dz, dims));
+}
+
+// Whether pairDistance < t for a pair of squared distance r2, as the rounded square root decides
+// it. Outside a band of t^2 (1 -+ 2^-20) (2^-48 in double) the comparison of r2 with the band's
+// edges already decides it: the roundings of t^2, of the edges and of the square root together
+// shift the comparison by at most a quarter of 2^-20 (2^-48), so r2 below the band gives a root
+// below t and r2 above it a root above t. Inside the band, and for a NaN, the root itself is
+// compared.
+template <typename Real>
+__device__ __forceinline__ bool withinSupport(Real r2, Real t) {
+    constexpr Real band = sizeof(Real) == 4 ? Real(9.5367431640625e-07) : Real(3.552713678800501e-15);
+    const Real t2 = t * t;
+    if (r2 < t2 * (Real(1) - band)) return true;
+    if (r2 > t2 * (Real(1) + band)) return false;
+    return sqrt(r2) < t;
 }
 
 // The distance a pair sum uses once its pair is known: the predicate's own, correctly rounded.
@@ -344,17 +372,157 @@ struct PairKernel {
     }
 };
 
+// Where the neighbor walks read a neighbor's record: straight from the sorted records...
+template <typename Real>
+struct GlobalRecords {
+    const Record<Real>* r;
+    __device__ __forceinline__ Record<Real> operator()(int32_t k) const { return r[k]; }
+};
+
+// A sweep workgroup's dynamic LDS, which holds its NearCells copy, and a record there as an LDS
+// address: reads through it stay LDS reads rather than merging with a global read of the same
+// record into one generic read.
+extern __shared__ __align__(16) unsigned char sweepShared[];
+template <typename Real>
+using Vec4 = Real __attribute__((ext_vector_type(4)));
+template <typename Real>
+using LdsVec4 = __attribute__((address_space(3))) Vec4<Real>;
+
+template <typename Real>
+__device__ __forceinline__ void storeLds(LdsVec4<Real>* to, const Quad<Real>& q) {
+    *to = Vec4<Real>{q.x, q.y, q.z, q.w};
+}
+template <typename Real>
+__device__ __forceinline__ Quad<Real> loadLds(const LdsVec4<Real>* from) {
+    const Vec4<Real> v = *from;
+    return Quad<Real>{v.x, v.y, v.z, v.w};
+}
+// Quad q of the record at LDS index `index` of the workgroup's copy.
+template <typename Real>
+__device__ __forceinline__ Quad<Real> ldsQuad(int32_t index, int q) {
+    return loadLds((const LdsVec4<Real>*)sweepShared + 4 * index + q);
+}
+
+// ...or from a workgroup's copy in LDS of the records of the cells around its particles. A tile's
+// particles are consecutive sorted particles of one run, so they lie in the run's cells c0..c1
+// (cells numbered x fastest), and their neighbors lie in the cells c0 - 1 + o .. c1 + 1 + o for the
+// stencil's row offsets o (3 rows in 2D, 9 in 3D): contiguous ranges of sorted records, merged
+// where they overlap. The workgroup copies them in one pass with all its loads in flight, and a
+// walk then reads a neighbor from LDS when it lies in a copied range, from the records otherwise
+// (a periodic image, a wrapped cell, or a range past the copy's capacity).
+template <int D, typename Real>
+struct NearCells {
+    static constexpr int kRows = D == 2 ? 3 : 9;
+    int32_t lo[kRows], hi[kRows];   // sorted ranges [lo, hi), ascending and disjoint, some empty
+    int32_t at[kRows];              // LDS index of lo
+    int32_t held;                   // records copied, at most the capacity
+    const Record<Real>* r;
+
+    // The ranges around sorted particles s0..s1 of `run`, and the first `capacity` of their
+    // records to copy. Every value is the same across the workgroup.
+    __device__ __forceinline__ void plan(const RunDev& run, const int32_t* __restrict__ start, const int32_t* key,
+                                         int64_t s0, int64_t s1, int32_t capacity) {
+        const int64_t cells = (int64_t)run.nx * run.ny * run.nz;
+        const int64_t c0 = (int64_t)key[s0] - run.cellFirst, c1 = (int64_t)key[s1] - run.cellFirst;
+        const int64_t plane = (int64_t)run.nx * run.ny;
+        int64_t covered = -1;   // the last cell an earlier range holds
+        int32_t total = 0;
+#pragma unroll
+        for (int row = 0; row < kRows; ++row) {
+            const int64_t offset = D == 2 ? (row - 1) * (int64_t)run.nx : (row / 3 - 1) * plane + (row % 3 - 1) * (int64_t)run.nx;
+            int64_t low = c0 + offset - 1, high = c1 + offset + 1;
+            low = low <= covered ? covered + 1 : low;
+            low = low < 0 ? 0 : low;
+            high = high > cells - 1 ? cells - 1 : high;
+            if (low > high) {
+                lo[row] = hi[row] = 0;
+            } else {
+                lo[row] = start[run.cellFirst + low];
+                hi[row] = start[run.cellFirst + high + 1];
+                covered = high;
+            }
+            at[row] = total;
+            total += hi[row] - lo[row];
+        }
+        held = total < capacity ? total : capacity;
+    }
+
+    // The sorted position of the record at LDS index `index`.
+    __device__ __forceinline__ int32_t sourceOf(int32_t index) const {
+        int32_t from = lo[0] + index;
+#pragma unroll
+        for (int j = 1; j < kRows; ++j)
+            if (index >= at[j]) from = lo[j] + (index - at[j]);
+        return from;
+    }
+
+    // Copies the held records into the workgroup's LDS, two records per thread in flight; every
+    // thread of the workgroup calls it, and it ends at a barrier.
+    __device__ __forceinline__ void fill() const {
+        LdsVec4<Real>* shared = (LdsVec4<Real>*)sweepShared;
+        const int32_t step = (int32_t)blockDim.x;
+        for (int32_t first = (int32_t)threadIdx.x; first < held; first += 2 * step) {
+            const int32_t second = first + step;
+            const bool both = second < held;
+            const Record<Real> a = r[sourceOf(first)];
+            const Record<Real> b = r[sourceOf(both ? second : first)];
+            LdsVec4<Real>* to = shared + 4 * first;
+            storeLds(to, a.q0);
+            storeLds(to + 1, a.q1);
+            storeLds(to + 2, a.q2);
+            storeLds(to + 3, a.q3);
+            if (both) {
+                to = shared + 4 * second;
+                storeLds(to, b.q0);
+                storeLds(to + 1, b.q1);
+                storeLds(to + 2, b.q2);
+                storeLds(to + 3, b.q3);
+            }
+        }
+        __syncthreads();
+    }
+
+    // The LDS index of `begin` when all of [begin, end) is copied, else -1.
+    __device__ __forceinline__ int32_t rangeBase(int32_t begin, int32_t end) const {
+        int32_t base = -1;
+#pragma unroll
+        for (int i = 0; i < kRows; ++i)
+            if (begin >= lo[i] && end <= hi[i]) base = at[i] + (begin - lo[i]);
+        return base >= 0 && base + (end - begin) <= held ? base : -1;
+    }
+
+    // The record of sorted particle k, from LDS when copied there.
+    __device__ __forceinline__ Record<Real> operator()(int32_t k) const {
+        const LdsVec4<Real>* shared = (const LdsVec4<Real>*)sweepShared;
+        int32_t index = -1;
+#pragma unroll
+        for (int i = 0; i < kRows; ++i)
+            if (k >= lo[i] && k < hi[i]) index = at[i] + (k - lo[i]);
+        Record<Real> out;
+        if (index >= 0 && index < held) {
+            const LdsVec4<Real>* from = shared + 4 * index;
+            out.q0 = loadLds(from);
+            out.q1 = loadLds(from + 1);
+            out.q2 = loadLds(from + 2);
+            out.q3 = loadLds(from + 3);
+        } else {
+            out = r[k];
+        }
+        return out;
+    }
+};
+
 // Walks every neighbor j != i of sorted particle s with r < 2 h_ij (sphGpu.py:352-363), calling
-// f(k, dx, dy, dz, r, h) with k the neighbor's sorted position, d = r_i - r_j at the
-// nearest periodic image, r = |d| and h = h_ij = (h_i + h_j) / 2. Each cell range's candidates are
-// dealt to `Lanes` lanes in turn, lane `lane` taking the range's candidates lane, lane + Lanes, ...,
-// so a group of lanes shares one particle's walk. With `SkipCoincident` pairs closer than 1e-12 are
-// skipped too, as every sum but the wall pressure skips them. `y64` is the particle's float64 y,
-// read only for a search from its periodic image.
-template <int D, typename Real, int Lanes, bool SkipCoincident, class F>
-__device__ __forceinline__ void forNeighbors(const RunDev& run, const int32_t* __restrict__ start,
-                                             const Records<Real>& rec, int64_t s, const Self<D, Real>& me,
-                                             double y64, int lane, bool uniform, F&& f) {
+// f(k, other, dx, dy, dz, r, h) with k the neighbor's sorted position, other its record as `src`
+// reads it, d = r_i - r_j at the nearest periodic image, r = |d| and h = h_ij = (h_i + h_j) / 2.
+// Each cell range's candidates are dealt to `Lanes` lanes in turn, lane `lane` taking the range's
+// candidates lane, lane + Lanes, ..., so a group of lanes shares one particle's walk. With
+// `SkipCoincident` pairs closer than 1e-12 are skipped too, as every sum but the wall pressure skips
+// them. `y64` is the particle's float64 y, read only for a search from its periodic image.
+template <int D, typename Real, int Lanes, bool SkipCoincident, class Src, class F>
+__device__ __forceinline__ void forNeighbors(const RunDev& run, const int32_t* __restrict__ start, const Src& src,
+                                             int64_t s, const Self<D, Real>& me, double y64, int lane, bool uniform,
+                                             F&& f) {
     int cx, cy, cz;
     cellCoordinates(run, me.cell, cx, cy, cz);
     const Real period = (Real)run.period;
@@ -371,7 +539,7 @@ __device__ __forceinline__ void forNeighbors(const RunDev& run, const int32_t* _
         forStencil<D>(run, start, cx, oy, cz, [&](int32_t begin, int32_t end) {
             for (int32_t k = begin + lane; k < end; k += Lanes) {
                 if (k == s) continue;
-                const Record<Real>& other = rec.r[k];
+                const Record<Real> other = src(k);
                 Real dx, dy, dz;
                 separation<D, Real>(me, other, shift, dx, dy, dz);
                 const Real r = pairDistance(dx, dy, dz, D);
@@ -385,7 +553,7 @@ __device__ __forceinline__ void forNeighbors(const RunDev& run, const int32_t* _
                 }
                 if (!(r < support)) continue;
                 if (SkipCoincident && r < Real(1.0e-12)) continue;
-                f(k, dx, dy, dz, r, h);
+                f(k, other, dx, dy, dz, r, h);
             }
         });
     }
@@ -410,15 +578,14 @@ constexpr int32_t kImageBit = (int32_t)0x80000000u;
 #endif
 constexpr int kAhead = SPH_LIST_AHEAD;
 
-// Calls f(k, dx, dy, dz, r, h), r from sumDistance, for the neighbors of sorted particle s that lane `lane` of
-// its group takes: neighbor m, in the order forNeighbors visits them, goes to lane m % Lanes. From
-// the list when the particle's neighbors fit it, from its cells otherwise; the same pairs reach the
-// same lanes in the same order either way.
-template <int D, typename Real, int Lanes, bool SkipCoincident, class F>
-__device__ __forceinline__ void forEachNeighbor(const RunDev& run, const int32_t* __restrict__ start,
-                                                const Records<Real>& rec, const NeighborList& list, int64_t s,
-                                                const Self<D, Real>& me, double y64, int lane, bool uniform, F&& f) {
-    const int32_t count = list.capacity > 0 ? list.count[s] : -1;
+// Calls f(k, other, dx, dy, dz, r, h), r from sumDistance, for the neighbors of sorted particle s
+// that lane `lane` of its group takes: neighbor m, in the order forNeighbors visits them, goes to
+// lane m % Lanes. From the list when the particle's `count` neighbors fit it, from its cells
+// otherwise; the same pairs reach the same lanes in the same order either way.
+template <int D, typename Real, int Lanes, bool SkipCoincident, class Src, class F>
+__device__ __forceinline__ void forListedNeighbors(const RunDev& run, const int32_t* __restrict__ start, const Src& src,
+                                                   const NeighborList& list, int64_t s, int32_t count,
+                                                   const Self<D, Real>& me, double y64, int lane, bool uniform, F&& f) {
     if (count >= 0 && count <= list.capacity) {
         const Real period = (Real)run.period;
         const Real shiftImage = me.p[1] < Real(0.5) * period ? period : -period;
@@ -426,13 +593,13 @@ __device__ __forceinline__ void forEachNeighbor(const RunDev& run, const int32_t
         auto visit = [&](int32_t entry) {
             const int32_t k = entry & 0x7fffffff;
             const Real shift = entry < 0 ? shiftImage : Real(0);
-            const Record<Real>& other = rec.r[k];
+            const Record<Real> other = src(k);
             Real dx, dy, dz;
             separation<D, Real>(me, other, shift, dx, dy, dz);
             const Real r = sumDistance(dx, dy, dz, D);
             if (SkipCoincident && r < Real(1.0e-12)) return;
             const Real h = uniform ? (Real)run.hmax : Real(0.5) * (me.h + fabs(signedLength<D, Real>(other)));
-            f(k, dx, dy, dz, r, h);
+            f(k, other, dx, dy, dz, r, h);
         };
         // kAhead entries of a lane in flight at once; each lane still takes its entries in order
         int32_t m = lane;
@@ -447,15 +614,25 @@ __device__ __forceinline__ void forEachNeighbor(const RunDev& run, const int32_t
         return;
     }
     int32_t m = 0;
-    forNeighbors<D, Real, 1, false>(run, start, rec, s, me, y64, 0, uniform,
-                                    [&](int32_t k, Real dx, Real dy, Real dz, Real, Real h) {
+    forNeighbors<D, Real, 1, false>(run, start, src, s, me, y64, 0, uniform,
+                                    [&](int32_t k, const Record<Real>& other, Real dx, Real dy, Real dz, Real, Real h) {
                                         const bool mine = (m % Lanes) == lane;
                                         ++m;
                                         if (!mine) return;
                                         const Real r = sumDistance(dx, dy, dz, D);
                                         if (SkipCoincident && r < Real(1.0e-12)) return;
-                                        f(k, dx, dy, dz, r, h);
+                                        f(k, other, dx, dy, dz, r, h);
                                     });
+}
+
+// forListedNeighbors with the count the list holds for the particle; without a list every sum
+// walks the cells.
+template <int D, typename Real, int Lanes, bool SkipCoincident, class Src, class F>
+__device__ __forceinline__ void forEachNeighbor(const RunDev& run, const int32_t* __restrict__ start, const Src& src,
+                                                const NeighborList& list, int64_t s, const Self<D, Real>& me,
+                                                double y64, int lane, bool uniform, F&& f) {
+    const int32_t count = list.capacity > 0 ? list.count[s] : -1;
+    forListedNeighbors<D, Real, Lanes, SkipCoincident>(run, start, src, list, s, count, me, y64, lane, uniform, f);
 }
 
 // The pair sweeps' workgroups: each covers up to blockDim / Lanes consecutive sorted particles of

@@ -54,6 +54,11 @@ template <int D, typename Real>
 constexpr int accelerationWaves() { return (D == 2 && sizeof(Real) == 4) ? SPH_WAVES_ACCELERATIONS : 1; }
 #define SWEEP_BOUNDS(waves) __attribute__((amdgpu_flat_work_group_size(1, 256), amdgpu_waves_per_eu(waves)))
 
+// Whether the sweeps copy the records around their particles into LDS (NearCells).
+#ifndef SPH_NEAR_CELLS
+#define SPH_NEAR_CELLS 1
+#endif
+
 namespace {
 
 constexpr int kScanThreads = 256;
@@ -485,10 +490,11 @@ struct Sweep1 {
 #pragma unroll
         for (int a = 0; a < 3; ++a) g[a] = (Real)run.gravity[a];
     }
-    __device__ __forceinline__ void add(const Records<Real>& rec, const PairKernel<Real, D>& kernel, int32_t k,
-                                        Real dx, Real dy, Real dz, Real rr, Real h) {
+    // Adds neighbor `other`, of which it reads the signed length, and the pressure and density
+    // (wall) or the volume and velocity (gradients).
+    __device__ __forceinline__ void add(const Record<Real>& other, const PairKernel<Real, D>& kernel, Real dx, Real dy,
+                                        Real dz, Real rr, Real h) {
         if (mode == 0) return;
-        const Record<Real>& other = rec.r[k];
         if (!(signedLength<D, Real>(other) > Real(0))) return;
         if (mode == 1) {
             const Real w = kernel.value(rr, h);
@@ -585,14 +591,27 @@ template <int D, typename Real, int Lanes, bool WithSweep1>
 __global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
     buildNeighbors(const RunDev* __restrict__ runs, Tiles tiles, const int32_t* __restrict__ order, const int32_t* __restrict__ start, Records<Real> rec,
                    NeighborList list, SweepOut<Real> out, Clock clock, const double* __restrict__ y64,
-                   int32_t resident, int64_t n) {
+                   int32_t resident, int32_t cached, int64_t n) {
     const int64_t tile = xcdBlock();
     const int r = tiles.run[tile];
     const int64_t s = tiles.first[tile] + threadIdx.x / Lanes;
     const int lane = (int)(threadIdx.x % Lanes);
     const int groupBase = (int)(threadIdx.x & 63) & ~(Lanes - 1);
-    bool valid = s < runs[r].first + runs[r].count && (!resident || clock.active[r]);
+    const bool runSteps = !resident || clock.active[r];
+    bool valid = s < runs[r].first + runs[r].count && runSteps;
     (void)n;
+    NearCells<D, Real> cells;
+    cells.r = rec.r;
+    cells.held = 0;
+#pragma unroll
+    for (int i = 0; i < NearCells<D, Real>::kRows; ++i) cells.lo[i] = cells.hi[i] = cells.at[i] = 0;
+    if (runSteps && cached > 0) {
+        const RunDev& run = runs[r];
+        const int64_t first = tiles.first[tile];
+        const int64_t last = min(first + (int64_t)(blockDim.x / Lanes), run.first + run.count) - 1;
+        cells.plan(run, start, rec.key, first, last, cached);
+        cells.fill();
+    }
     Sweep1<D, Real> sweep;
     sweep.mode = 0;
     bool fluidI = false;
@@ -622,22 +641,53 @@ __global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
             }
             const int32_t tag = image ? kImageBit : 0;
             forStencil<D>(run, start, cx, oy, cz, [&](int32_t begin, int32_t end) {
+                const int32_t copied = cells.rangeBase(begin, end);
                 for (int32_t base = begin; base < end; base += Lanes) {
                     const int32_t k = base + lane;
                     bool hit = false;
                     Real dx = Real(0), dy = Real(0), dz = Real(0), rr = Real(0), h = Real(0);
+                    Record<Real> other;
                     if (k < end && k != s) {
-                        const Record<Real>& other = rec.r[k];
+                        // the fields the test and sweep 1 read, loaded together
+                        const bool wide = D == 3 || WithSweep1 || !uniform;
+                        if (copied >= 0) {
+                            const int32_t index = copied + (k - begin);
+                            other.q0 = ldsQuad<Real>(index, 0);
+                            if (wide) other.q1 = ldsQuad<Real>(index, 1);
+                            if (WithSweep1) {
+                                other.q2 = ldsQuad<Real>(index, 2);
+                                other.q3 = ldsQuad<Real>(index, 3);
+                            }
+                        } else {
+                            other.q0 = rec.r[k].q0;
+                            if (wide) other.q1 = rec.r[k].q1;
+                            if (WithSweep1) {
+                                if constexpr (D == 2) {
+                                    other.q2.y = rec.r[k].q2.y;
+                                    other.q3.x = rec.r[k].q3.x;
+                                } else {
+                                    other.q2 = rec.r[k].q2;
+                                    other.q3.y = rec.r[k].q3.y;
+                                    other.q3.w = rec.r[k].q3.w;
+                                }
+                            }
+                        }
                         separation<D, Real>(me, other, shift, dx, dy, dz);
-                        rr = pairDistance(dx, dy, dz, D);
-                        h = uniform ? (Real)run.hmax : Real(0.5) * (me.h + fabs(signedLength<D, Real>(other)));
-                        hit = rr < (uniform ? twoHmax : Real(2) * h);
+                        if (WithSweep1) {
+                            rr = pairDistance(dx, dy, dz, D);
+                            h = uniform ? (Real)run.hmax : Real(0.5) * (me.h + fabs(signedLength<D, Real>(other)));
+                            hit = rr < (uniform ? twoHmax : Real(2) * h);
+                        } else {
+                            // 2 h_ij = h_i + h_j exactly, as 2 (0.5 (h_i + h_j)) is
+                            const Real support = uniform ? twoHmax : me.h + fabs(signedLength<D, Real>(other));
+                            hit = withinSupport(squaredDistance(dx, dy, dz, D), support);
+                        }
                     }
                     const uint32_t mine = (uint32_t)(__ballot(hit) >> groupBase) & groupMask;
                     if (hit) {
                         const int32_t slot = count + __popc(mine & ((1u << lane) - 1u));
                         if (slot < list.capacity) list.index[(int64_t)slot * list.stride + s] = k | tag;
-                        if (WithSweep1) sweep.add(rec, kernel, k, dx, dy, dz, rr, h);
+                        if (WithSweep1) sweep.add(other, kernel, dx, dy, dz, rr, h);
                     }
                     count += __popc(mine);
                 }
@@ -676,10 +726,9 @@ __global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
             PairKernel<Real, D> kernel;
             kernel.init(run.kernel, run.sigmaUnit, run.hmax, run.uniform != 0);
             const double yy = run.period > 0.0 ? y64[order[s]] : 0.0;
-            forNeighbors<D, Real, Lanes, false>(run, start, rec, s, me, yy, lane, kernel.uniform,
-                                                [&](int32_t k, Real dx, Real dy, Real dz, Real rr, Real h) {
-                                                    sweep.add(rec, kernel, k, dx, dy, dz, rr, h);
-                                                });
+            forNeighbors<D, Real, Lanes, false>(run, start, GlobalRecords<Real>{rec.r}, s, me, yy, lane, kernel.uniform,
+                                                [&](int32_t, const Record<Real>& other, Real dx, Real dy, Real dz,
+                                                    Real rr, Real h) { sweep.add(other, kernel, dx, dy, dz, rr, h); });
         }
     }
     (void)list;
@@ -695,7 +744,8 @@ template <int D, typename Real, int Lanes>
 __global__ void SWEEP_BOUNDS((accelerationWaves<D, Real>()))
     sweepAccelerations(const RunDev* __restrict__ runs, Tiles tiles, const int32_t* __restrict__ order, const int32_t* __restrict__ start, Records<Real> rec,
                        NeighborList list, SweepOut<Real> out, Clock clock, const double* __restrict__ y64,
-                       int32_t resident, int32_t doAcceleration, int32_t doCorrection, int32_t hostLimiter, int64_t n) {
+                       int32_t resident, int32_t doAcceleration, int32_t doCorrection, int32_t hostLimiter,
+                       int32_t cached, int64_t n) {
     const int64_t tile = xcdBlock();
     const int r = tiles.run[tile];
     const int64_t s = tiles.first[tile] + threadIdx.x / Lanes;
@@ -705,8 +755,21 @@ __global__ void SWEEP_BOUNDS((accelerationWaves<D, Real>()))
 #pragma unroll
     for (int e = 0; e < K; ++e) acc[e] = Real(0);
     bool fluidI = false, correction = false;
-    const bool activeRun = s < runs[r].first + runs[r].count && (!resident || clock.active[r]);
+    const bool runSteps = !resident || clock.active[r];
+    const bool activeRun = s < runs[r].first + runs[r].count && runSteps;
     (void)n;
+    NearCells<D, Real> cells;
+    cells.r = rec.r;
+    cells.held = 0;
+#pragma unroll
+    for (int i = 0; i < NearCells<D, Real>::kRows; ++i) cells.lo[i] = cells.hi[i] = cells.at[i] = 0;
+    if (runSteps && cached > 0) {
+        const RunDev& run = runs[r];
+        const int64_t first = tiles.first[tile];
+        const int64_t last = min(first + (int64_t)(blockDim.x / Lanes), run.first + run.count) - 1;
+        cells.plan(run, start, rec.key, first, last, cached);
+        cells.fill();
+    }
     if (activeRun) {
         const RunDev& run = runs[r];
         correction = doCorrection && (!resident || run.correction);
@@ -729,9 +792,13 @@ __global__ void SWEEP_BOUNDS((accelerationWaves<D, Real>()))
                     for (int b = 0; b < D; ++b) Gi[a][b] = rec.G[s * (D * D) + a * D + b];
             const Real alphaC0 = (Real)(run.alpha * run.c0), bf = (Real)run.boundaryFactor;
             forEachNeighbor<D, Real, Lanes, true>(
-                run, start, rec, list, s, me, yy, lane, kernel.uniform,
-                [&](int32_t k, Real dx, Real dy, Real dz, Real rr, Real h) {
-                    const Record<Real>& other = rec.r[k];
+                run, start, cells, list, s, me, yy, lane, kernel.uniform,
+                [&](int32_t k, const Record<Real>& other, Real dx, Real dy, Real dz, Real rr, Real h) {
+                    // G_j with the record, before the sums need it
+                    Real Gj[D * D];
+                    if (limiter && doAcceleration)
+#pragma unroll
+                        for (int e = 0; e < D * D; ++e) Gj[e] = rec.G[(int64_t)k * (D * D) + e];
                     const Real F = kernel.slopeOverR(rr, h);
                     const Real d[3] = {dx, dy, dz};
                     if (correction) {
@@ -759,7 +826,6 @@ __global__ void SWEEP_BOUNDS((accelerationWaves<D, Real>()))
                     if (approach < Real(0) && limiter && fluidJ) {
                         Real rampI[3] = {}, rampJ[3] = {};
                         Real slopeI = Real(0), slopeJ = Real(0);
-                        const Real* Gj = rec.G + (int64_t)k * (D * D);
 #pragma unroll
                         for (int a = 0; a < D; ++a) {
                             Real ti = Real(0), tj = Real(0);
@@ -973,15 +1039,28 @@ template <int D, typename Real, int Lanes>
 __global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
     sweepContinuity(const RunDev* __restrict__ runs, Tiles tiles, const int32_t* __restrict__ order, const int32_t* __restrict__ start, Records<Real> rec,
                     NeighborList list, SweepOut<Real> out, Clock clock, int32_t resident, int32_t hostCorrection,
-                    int32_t hostDiffusion, int64_t n) {
+                    int32_t hostDiffusion, int32_t cached, int64_t n) {
     const int64_t tile = xcdBlock();
     const int r = tiles.run[tile];
     const int64_t s = tiles.first[tile] + threadIdx.x / Lanes;
     const int lane = (int)(threadIdx.x % Lanes);
     Real total = Real(0);
     bool boundaryI = false;
-    const bool activeRun = s < runs[r].first + runs[r].count && (!resident || clock.active[r]);
+    const bool runSteps = !resident || clock.active[r];
+    const bool activeRun = s < runs[r].first + runs[r].count && runSteps;
     (void)n;
+    NearCells<D, Real> cells;
+    cells.r = rec.r;
+    cells.held = 0;
+#pragma unroll
+    for (int i = 0; i < NearCells<D, Real>::kRows; ++i) cells.lo[i] = cells.hi[i] = cells.at[i] = 0;
+    if (runSteps && cached > 0) {
+        const RunDev& run = runs[r];
+        const int64_t first = tiles.first[tile];
+        const int64_t last = min(first + (int64_t)(blockDim.x / Lanes), run.first + run.count) - 1;
+        cells.plan(run, start, rec.key, first, last, cached);
+        cells.fill();
+    }
     if (activeRun) {
         const RunDev& run = runs[r];
         const Self<D, Real> me = loadSelf<D, Real>(run, rec, s);
@@ -1006,9 +1085,8 @@ __global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
         const Real invRho0 = (Real)(1.0 / run.rho0);
         const Real head = (Real)(run.rho0 * fabs(run.gravity[D - 1]) / (run.c0 * run.c0));
         forEachNeighbor<D, Real, Lanes, true>(
-            run, start, rec, list, s, me, yy, lane, kernel.uniform,
-            [&](int32_t k, Real dx, Real dy, Real dz, Real rr, Real h) {
-                const Record<Real>& other = rec.r[k];
+            run, start, cells, list, s, me, yy, lane, kernel.uniform,
+            [&](int32_t, const Record<Real>& other, Real dx, Real dy, Real dz, Real rr, Real h) {
                 const bool fluidJ = signedLength<D, Real>(other) > Real(0);
                 if (boundaryI && !fluidJ) return;
                 const Real F = kernel.slopeOverR(rr, h);
@@ -1101,8 +1179,8 @@ __global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
     const double yy = run.period > 0.0 ? y64[i] : 0.0;
     int32_t count = 0;
     const int64_t base = rows ? offsets[i] : 0;
-    forEachNeighbor<D, Real, 1, false>(run, start, rec, list, s, me, yy, 0, run.uniform != 0,
-                                       [&](int32_t k, Real, Real, Real, Real, Real) {
+    forEachNeighbor<D, Real, 1, false>(run, start, GlobalRecords<Real>{rec.r}, list, s, me, yy, 0, run.uniform != 0,
+                                       [&](int32_t k, const Record<Real>&, Real, Real, Real, Real, Real) {
                                            if (rows) rows[base + count] = order[k];
                                            ++count;
                                        });
@@ -1438,6 +1516,20 @@ int ensureTiles(sph_context* ctx) {
 inline Tiles tilesOf(const sph_context* ctx) { return Tiles{ctx->tileRun, ctx->tileFirst, ctx->tileCount}; }
 inline unsigned tileGrid(const sph_context* ctx) { return (unsigned)ctx->tileCount; }
 
+// Records a sweep workgroup copies into LDS: the cells around its particles hold about 3 records
+// per particle in 2D and 9 in 3D plus the ends of the ranges; the copy is capped at 32 KiB so
+// that LDS leaves the waves' occupancy alone.
+template <int D, typename Real>
+int32_t nearRecords(const sph_context* ctx) {
+    if (!SPH_NEAR_CELLS) return 0;
+    const int32_t per = (int32_t)ctx->tilePer;
+    const int32_t want = D == 2 ? 4 * per + 64 : 12 * per + 128;
+    const int32_t most = (int32_t)(32768 / sizeof(Record<Real>));
+    return want < most ? want : most;
+}
+template <int D, typename Real>
+size_t nearShared(const sph_context* ctx) { return (size_t)nearRecords<D, Real>(ctx) * sizeof(Record<Real>); }
+
 template <typename Real>
 Records<Real>& recordsOf(sph_context* ctx);
 template <>
@@ -1594,9 +1686,9 @@ int stepOnce(sph_context* ctx, int phase, int useClock, double forcedDt) {
                                                                              ctx->clock, 1, n)));
         if (ctx->list.capacity > 0) {
             WITH_LANES(ctx, TIMED(ctx, kNeighbors,
-                                  (buildNeighbors<D, Real, L, true><<<tileGrid(ctx), b, 0, ctx->stream>>>(
+                                  (buildNeighbors<D, Real, L, true><<<tileGrid(ctx), b, nearShared<D, Real>(ctx), ctx->stream>>>(
                             ctx->dRuns, tilesOf(ctx), ctx->order, ctx->cellStart, rec, ctx->list, out,
-                                      ctx->clock, y64, 1, n))));
+                                      ctx->clock, y64, 1, nearRecords<D, Real>(ctx), n))));
         } else {
             WITH_LANES(ctx, TIMED(ctx, kSweepWall,
                                   (sweepWallAndGradients<D, Real, L><<<tileGrid(ctx), b, 0, ctx->stream>>>(
@@ -1604,9 +1696,9 @@ int stepOnce(sph_context* ctx, int phase, int useClock, double forcedDt) {
                                       ctx->clock, y64, 1, 1, 1, n))));
         }
         WITH_LANES(ctx, TIMED(ctx, kSweepAcceleration,
-                              (sweepAccelerations<D, Real, L><<<tileGrid(ctx), b, 0, ctx->stream>>>(
+                              (sweepAccelerations<D, Real, L><<<tileGrid(ctx), b, nearShared<D, Real>(ctx), ctx->stream>>>(
                             ctx->dRuns, tilesOf(ctx), ctx->order, ctx->cellStart, rec, ctx->list, out, ctx->clock,
-                                  y64, 1, 1, 1, 0, n))));
+                                  y64, 1, 1, 1, 0, nearRecords<D, Real>(ctx), n))));
         if (ctx->sumChunkCount > 0) {
             const int slot = timeBegin(ctx, kMaxima);
             chunkMaxima<<<(unsigned)ctx->sumChunkCount, 256, 0, ctx->stream>>>(
@@ -1622,9 +1714,10 @@ int stepOnce(sph_context* ctx, int phase, int useClock, double forcedDt) {
                                                                     static_cast<const Real*>(ctx->state.acceleration),
                                                                     ctx->dVel, ctx->clock, forcedDt, st, n)));
         WITH_LANES(ctx, TIMED(ctx, kSweepContinuity,
-                              (sweepContinuity<D, Real, L><<<tileGrid(ctx), b, 0, ctx->stream>>>(
+                              (sweepContinuity<D, Real, L><<<tileGrid(ctx), b,
+                                                              nearShared<D, Real>(ctx), ctx->stream>>>(
                             ctx->dRuns, tilesOf(ctx), ctx->order, ctx->cellStart, rec, ctx->list, out, ctx->clock,
-                                  1, 0, 0, n))));
+                                  1, 0, 0, nearRecords<D, Real>(ctx), n))));
     }
     return launchCheck(ctx, "device step");
 }
@@ -1669,9 +1762,9 @@ int hostSweep(sph_context* ctx, int which, const double* velocity, const double*
     const int32_t* start = ctx->cellStart;
     const NeighborList list = ctx->list;
     if (list.capacity > 0)
-        WITH_LANES(ctx, (buildNeighbors<D, Real, L, false><<<tileGrid(ctx), b, 0, ctx->stream>>>(
+        WITH_LANES(ctx, (buildNeighbors<D, Real, L, false><<<tileGrid(ctx), b, nearShared<D, Real>(ctx), ctx->stream>>>(
                             ctx->dRuns, tilesOf(ctx), order, start, rec, list, emptyOut<Real>(ctx), ctx->clock, y64, 0,
-                            n)));
+                            nearRecords<D, Real>(ctx), n)));
     switch (which) {
         case 0:   // wall pressure
             out.pressure = static_cast<Real*>(out0);
@@ -1686,8 +1779,8 @@ int hostSweep(sph_context* ctx, int which, const double* velocity, const double*
             break;
         case 2:   // correction matrices
             out.corrections = static_cast<Real*>(out0);
-            WITH_LANES(ctx, (sweepAccelerations<D, Real, L><<<tileGrid(ctx), b, 0, ctx->stream>>>(
-                            ctx->dRuns, tilesOf(ctx), order, start, rec, list, out, ctx->clock, y64, 0, 0, 1, 0, n)));
+            WITH_LANES(ctx, (sweepAccelerations<D, Real, L><<<tileGrid(ctx), b, nearShared<D, Real>(ctx), ctx->stream>>>(
+                            ctx->dRuns, tilesOf(ctx), order, start, rec, list, out, ctx->clock, y64, 0, 0, 1, 0, nearRecords<D, Real>(ctx), n)));
             break;
         case 3: {  // accelerations, with the limiter's gradients first when the run has it
             const int limiter = run0.limiter;
@@ -1695,20 +1788,21 @@ int hostSweep(sph_context* ctx, int which, const double* velocity, const double*
                 WITH_LANES(ctx, (sweepWallAndGradients<D, Real, L><<<tileGrid(ctx), b, 0, ctx->stream>>>(
                             ctx->dRuns, tilesOf(ctx), order, start, rec, list, out, ctx->clock, y64, 0, 0, 1, n)));
             out.acceleration = static_cast<Real*>(out0);
-            WITH_LANES(ctx, (sweepAccelerations<D, Real, L><<<tileGrid(ctx), b, 0, ctx->stream>>>(
-                            ctx->dRuns, tilesOf(ctx), order, start, rec, list, out, ctx->clock, y64, 0, 1, 0, limiter, n)));
+            WITH_LANES(ctx, (sweepAccelerations<D, Real, L><<<tileGrid(ctx), b, nearShared<D, Real>(ctx), ctx->stream>>>(
+                            ctx->dRuns, tilesOf(ctx), order, start, rec, list, out, ctx->clock, y64, 0, 1, 0, limiter, nearRecords<D, Real>(ctx), n)));
             break;
         }
         case 4: {  // continuity, with the correction matrices first when the run has them
             const int correction = run0.correction;
             if (correction)
-                WITH_LANES(ctx, (sweepAccelerations<D, Real, L><<<tileGrid(ctx), b, 0, ctx->stream>>>(
-                            ctx->dRuns, tilesOf(ctx), order, start, rec, list, out, ctx->clock, y64, 0, 0, 1, 0, n)));
+                WITH_LANES(ctx, (sweepAccelerations<D, Real, L><<<tileGrid(ctx), b, nearShared<D, Real>(ctx), ctx->stream>>>(
+                            ctx->dRuns, tilesOf(ctx), order, start, rec, list, out, ctx->clock, y64, 0, 0, 1, 0, nearRecords<D, Real>(ctx), n)));
             out.rate = static_cast<Real*>(out0);
             for (int a = 0; a < 3; ++a) out.pos[a] = a < D ? ctx->state.position + a * ctx->state.stride : nullptr;
-            WITH_LANES(ctx, (sweepContinuity<D, Real, L><<<tileGrid(ctx), b, 0, ctx->stream>>>(
+            WITH_LANES(ctx, (sweepContinuity<D, Real, L><<<tileGrid(ctx), b,
+                                                          nearShared<D, Real>(ctx), ctx->stream>>>(
                             ctx->dRuns, tilesOf(ctx), order, start, rec, list, out, ctx->clock, 0, correction,
-                                run0.diffusion, n)));
+                                run0.diffusion, nearRecords<D, Real>(ctx), n)));
             break;
         }
         default:
@@ -1741,9 +1835,9 @@ int neighborsImpl(sph_context* ctx, int32_t* counts, const int64_t* offsets, int
     gatherRecords<D, Real><<<gridFor(n, b), b, 0, ctx->stream>>>(ctx->dRuns, ctx->runOf, ctx->order, src, rec, ctx->clock, 0, n);
     const double* y64 = ctx->state.position + ctx->state.stride;
     if (ctx->list.capacity > 0)
-        WITH_LANES(ctx, (buildNeighbors<D, Real, L, false><<<tileGrid(ctx), b, 0, ctx->stream>>>(
+        WITH_LANES(ctx, (buildNeighbors<D, Real, L, false><<<tileGrid(ctx), b, nearShared<D, Real>(ctx), ctx->stream>>>(
                             ctx->dRuns, tilesOf(ctx), ctx->order, ctx->cellStart, rec, ctx->list, emptyOut<Real>(ctx),
-                            ctx->clock, y64, 0, n)));
+                            ctx->clock, y64, 0, nearRecords<D, Real>(ctx), n)));
     neighborExport<D, Real><<<gridFor(n, b), b, 0, ctx->stream>>>(ctx->dRuns, ctx->runOf, ctx->order, ctx->cellStart, rec,
                                                                    ctx->list, y64, counts, offsets, rows, n);
     int status = launchCheck(ctx, "neighbor list");
