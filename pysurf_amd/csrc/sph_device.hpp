@@ -299,10 +299,21 @@ __device__ __forceinline__ bool withinSupport(Real r2, Real t) {
     return sqrt(r2) < t;
 }
 
-// The distance a pair sum uses once its pair is known: the predicate's own, correctly rounded.
+// The distance a pair sum uses once its pair is known, from its squared distance: in float32 the
+// hardware square root (within 1 ulp; the sums are float32 anyway), in float64 the correctly
+// rounded one. Every sum takes its pair's distance from here, so the list, the cell walk and the
+// list build's fused sweep see the same value; membership is decided apart, by withinSupport.
+__device__ __forceinline__ float rootOf(float r2) { return __builtin_amdgcn_sqrtf(r2); }
+__device__ __forceinline__ double rootOf(double r2) { return sqrt(r2); }
 template <typename Real>
 __device__ __forceinline__ Real sumDistance(Real dx, Real dy, Real dz, int dims) {
-    return pairDistance(dx, dy, dz, dims);
+    return rootOf(squaredDistance(dx, dy, dz, dims));
+}
+
+// Entry m's offset in a column-major list of `stride` rows: both are below 2^31, so one 32 x 32 ->
+// 64-bit multiply.
+__device__ __forceinline__ int64_t listOffset(int32_t m, int64_t stride) {
+    return (int64_t)((uint64_t)(uint32_t)m * (uint64_t)(uint32_t)stride);
 }
 
 // a / b, correctly rounded in both precisions, as the reference's float32 kernels divide.
@@ -397,10 +408,15 @@ __device__ __forceinline__ Quad<Real> loadLds(const LdsVec4<Real>* from) {
     const Vec4<Real> v = *from;
     return Quad<Real>{v.x, v.y, v.z, v.w};
 }
-// Quad q of the record at LDS index `index` of the workgroup's copy.
+// Quad q of the record at LDS index `index` of the workgroup's copy, and component c of it.
 template <typename Real>
 __device__ __forceinline__ Quad<Real> ldsQuad(int32_t index, int q) {
     return loadLds((const LdsVec4<Real>*)sweepShared + 4 * index + q);
+}
+template <typename Real>
+__device__ __forceinline__ Real ldsValue(int32_t index, int q, int c) {
+    using LdsReal = __attribute__((address_space(3))) const Real;
+    return ((LdsReal*)sweepShared)[16 * index + 4 * q + c];
 }
 
 // ...or from a workgroup's copy in LDS of the records of the cells around its particles. A tile's
@@ -542,7 +558,7 @@ __device__ __forceinline__ void forNeighbors(const RunDev& run, const int32_t* _
                 const Record<Real> other = src(k);
                 Real dx, dy, dz;
                 separation<D, Real>(me, other, shift, dx, dy, dz);
-                const Real r = pairDistance(dx, dy, dz, D);
+                const Real r2 = squaredDistance(dx, dy, dz, D);
                 Real h, support;
                 if (uniform) {
                     h = (Real)run.hmax;
@@ -551,7 +567,8 @@ __device__ __forceinline__ void forNeighbors(const RunDev& run, const int32_t* _
                     h = Real(0.5) * (me.h + fabs(signedLength<D, Real>(other)));
                     support = Real(2) * h;
                 }
-                if (!(r < support)) continue;
+                if (!withinSupport(r2, support)) continue;
+                const Real r = rootOf(r2);
                 if (SkipCoincident && r < Real(1.0e-12)) continue;
                 f(k, other, dx, dy, dz, r, h);
             }
@@ -606,11 +623,11 @@ __device__ __forceinline__ void forListedNeighbors(const RunDev& run, const int3
         for (; m + (kAhead - 1) * Lanes < count; m += kAhead * Lanes) {
             int32_t entry[kAhead];
 #pragma unroll
-            for (int u = 0; u < kAhead; ++u) entry[u] = column[(int64_t)(m + u * Lanes) * list.stride];
+            for (int u = 0; u < kAhead; ++u) entry[u] = column[listOffset(m + u * Lanes, list.stride)];
 #pragma unroll
             for (int u = 0; u < kAhead; ++u) visit(entry[u]);
         }
-        for (; m < count; m += Lanes) visit(column[(int64_t)m * list.stride]);
+        for (; m < count; m += Lanes) visit(column[listOffset(m, list.stride)]);
         return;
     }
     int32_t m = 0;

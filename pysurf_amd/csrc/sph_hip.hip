@@ -645,7 +645,7 @@ __global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
                 for (int32_t base = begin; base < end; base += Lanes) {
                     const int32_t k = base + lane;
                     bool hit = false;
-                    Real dx = Real(0), dy = Real(0), dz = Real(0), rr = Real(0), h = Real(0);
+                    Real dx, dy, dz, r2;
                     Record<Real> other;
                     if (k < end && k != s) {
                         // the fields the test and sweep 1 read, loaded together
@@ -655,8 +655,14 @@ __global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
                             other.q0 = ldsQuad<Real>(index, 0);
                             if (wide) other.q1 = ldsQuad<Real>(index, 1);
                             if (WithSweep1) {
-                                other.q2 = ldsQuad<Real>(index, 2);
-                                other.q3 = ldsQuad<Real>(index, 3);
+                                if constexpr (D == 2) {
+                                    other.q2.y = ldsValue<Real>(index, 2, 1);
+                                    other.q3.x = ldsValue<Real>(index, 3, 0);
+                                } else {
+                                    other.q2 = ldsQuad<Real>(index, 2);
+                                    other.q3.y = ldsValue<Real>(index, 3, 1);
+                                    other.q3.w = ldsValue<Real>(index, 3, 3);
+                                }
                             }
                         } else {
                             other.q0 = rec.r[k].q0;
@@ -673,21 +679,20 @@ __global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
                             }
                         }
                         separation<D, Real>(me, other, shift, dx, dy, dz);
-                        if (WithSweep1) {
-                            rr = pairDistance(dx, dy, dz, D);
-                            h = uniform ? (Real)run.hmax : Real(0.5) * (me.h + fabs(signedLength<D, Real>(other)));
-                            hit = rr < (uniform ? twoHmax : Real(2) * h);
-                        } else {
-                            // 2 h_ij = h_i + h_j exactly, as 2 (0.5 (h_i + h_j)) is
-                            const Real support = uniform ? twoHmax : me.h + fabs(signedLength<D, Real>(other));
-                            hit = withinSupport(squaredDistance(dx, dy, dz, D), support);
-                        }
+                        r2 = squaredDistance(dx, dy, dz, D);
+                        // 2 h_ij = h_i + h_j exactly, as 2 (0.5 (h_i + h_j)) is
+                        const Real support = uniform ? twoHmax : me.h + fabs(signedLength<D, Real>(other));
+                        hit = withinSupport(r2, support);
                     }
                     const uint32_t mine = (uint32_t)(__ballot(hit) >> groupBase) & groupMask;
                     if (hit) {
                         const int32_t slot = count + __popc(mine & ((1u << lane) - 1u));
-                        if (slot < list.capacity) list.index[(int64_t)slot * list.stride + s] = k | tag;
-                        if (WithSweep1) sweep.add(other, kernel, dx, dy, dz, rr, h);
+                        if (slot < list.capacity) list.index[listOffset(slot, list.stride) + s] = k | tag;
+                        if (WithSweep1) {
+                            const Real h = uniform ? (Real)run.hmax
+                                                   : Real(0.5) * (me.h + fabs(signedLength<D, Real>(other)));
+                            sweep.add(other, kernel, dx, dy, dz, rootOf(r2), h);
+                        }
                     }
                     count += __popc(mine);
                 }
