@@ -652,6 +652,18 @@ __device__ __forceinline__ void forEachNeighbor(const RunDev& run, const int32_t
     forListedNeighbors<D, Real, Lanes, SkipCoincident>(run, start, src, list, s, count, me, y64, lane, uniform, f);
 }
 
+// A workgroup's per-particle sums in LDS, after its NearCells copy of `cached` records: each group
+// of lanes leaves its particle's sums there, and after a barrier thread p of the workgroup finishes
+// particle p. Closing arithmetic that is long against the walk (the continuity sweep's float64
+// integration and boundary enforcement) then runs on the fewest waves, rather than on one lane of
+// every group while the group's other lanes wait.
+template <typename Real>
+using LdsReal = __attribute__((address_space(3))) Real;
+template <typename Real>
+__device__ __forceinline__ LdsReal<Real>* packedSums(int32_t cached) {
+    return (LdsReal<Real>*)(sweepShared + (size_t)cached * sizeof(Record<Real>));
+}
+
 // The pair sweeps' workgroups: each covers up to blockDim / Lanes consecutive sorted particles of
 // one run, so the run and everything read from its description are the same across the
 // workgroup and live in scalar registers.

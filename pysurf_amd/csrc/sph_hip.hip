@@ -609,8 +609,10 @@ __global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
         const RunDev& run = runs[r];
         const int64_t first = tiles.first[tile];
         const int64_t last = min(first + (int64_t)(blockDim.x / Lanes), run.first + run.count) - 1;
-        cells.plan(run, start, rec.key, first, last, cached);
-        cells.fill();
+        if (last >= first) {
+            cells.plan(run, start, rec.key, first, last, cached);
+            cells.fill();
+        }
     }
     Sweep1<D, Real> sweep;
     sweep.mode = 0;
@@ -772,8 +774,10 @@ __global__ void SWEEP_BOUNDS((accelerationWaves<D, Real>()))
         const RunDev& run = runs[r];
         const int64_t first = tiles.first[tile];
         const int64_t last = min(first + (int64_t)(blockDim.x / Lanes), run.first + run.count) - 1;
-        cells.plan(run, start, rec.key, first, last, cached);
-        cells.fill();
+        if (last >= first) {
+            cells.plan(run, start, rec.key, first, last, cached);
+            cells.fill();
+        }
     }
     if (activeRun) {
         const RunDev& run = runs[r];
@@ -1063,8 +1067,10 @@ __global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
         const RunDev& run = runs[r];
         const int64_t first = tiles.first[tile];
         const int64_t last = min(first + (int64_t)(blockDim.x / Lanes), run.first + run.count) - 1;
-        cells.plan(run, start, rec.key, first, last, cached);
-        cells.fill();
+        if (last >= first) {
+            cells.plan(run, start, rec.key, first, last, cached);
+            cells.fill();
+        }
     }
     if (activeRun) {
         const RunDev& run = runs[r];
@@ -1127,18 +1133,24 @@ __global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
             });
     }
     total = groupSum<Lanes>(total);
-    if (!activeRun || lane != 0) return;
+    LdsReal<Real>* packed = packedSums<Real>(cached);
+    if (lane == 0) packed[threadIdx.x / Lanes] = total;
+    __syncthreads();
+    const int64_t own = tiles.first[tile] + threadIdx.x;
+    if (threadIdx.x >= blockDim.x / Lanes || !(own < runs[r].first + runs[r].count && runSteps)) return;
+    total = packed[threadIdx.x];
     const RunDev& run = runs[r];
-    const int32_t i = order[s];
+    const int32_t i = order[own];
     if (!resident) {
         out.rate[i] = total;
         return;
     }
+    const bool boundaryOwn = !(signedLength<D, Real>(rec.r[own]) > Real(0));
     const double dt = clock.dt[r];
     double rho = out.density64[i] + (double)total * dt;
-    if (run.densityFloor > 0.0 && !boundaryI) rho = fmax(rho, run.densityFloor);
+    if (run.densityFloor > 0.0 && !boundaryOwn) rho = fmax(rho, run.densityFloor);
     out.density64[i] = rho;
-    if (boundaryI) return;
+    if (boundaryOwn) return;
     double x[3] = {0.0, 0.0, 0.0}, v[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int a = 0; a < D; ++a) {
@@ -1532,8 +1544,11 @@ int32_t nearRecords(const sph_context* ctx) {
     const int32_t most = (int32_t)(32768 / sizeof(Record<Real>));
     return want < most ? want : most;
 }
+// A sweep's dynamic LDS: the NearCells copy, then `sums` values per particle of the workgroup.
 template <int D, typename Real>
-size_t nearShared(const sph_context* ctx) { return (size_t)nearRecords<D, Real>(ctx) * sizeof(Record<Real>); }
+size_t sweepLds(const sph_context* ctx, int sums) {
+    return (size_t)nearRecords<D, Real>(ctx) * sizeof(Record<Real>) + (size_t)ctx->tilePer * sums * sizeof(Real);
+}
 
 template <typename Real>
 Records<Real>& recordsOf(sph_context* ctx);
@@ -1691,7 +1706,7 @@ int stepOnce(sph_context* ctx, int phase, int useClock, double forcedDt) {
                                                                              ctx->clock, 1, n)));
         if (ctx->list.capacity > 0) {
             WITH_LANES(ctx, TIMED(ctx, kNeighbors,
-                                  (buildNeighbors<D, Real, L, true><<<tileGrid(ctx), b, nearShared<D, Real>(ctx), ctx->stream>>>(
+                                  (buildNeighbors<D, Real, L, true><<<tileGrid(ctx), b, sweepLds<D, Real>(ctx, 0), ctx->stream>>>(
                             ctx->dRuns, tilesOf(ctx), ctx->order, ctx->cellStart, rec, ctx->list, out,
                                       ctx->clock, y64, 1, nearRecords<D, Real>(ctx), n))));
         } else {
@@ -1701,7 +1716,7 @@ int stepOnce(sph_context* ctx, int phase, int useClock, double forcedDt) {
                                       ctx->clock, y64, 1, 1, 1, n))));
         }
         WITH_LANES(ctx, TIMED(ctx, kSweepAcceleration,
-                              (sweepAccelerations<D, Real, L><<<tileGrid(ctx), b, nearShared<D, Real>(ctx), ctx->stream>>>(
+                              (sweepAccelerations<D, Real, L><<<tileGrid(ctx), b, sweepLds<D, Real>(ctx, 0), ctx->stream>>>(
                             ctx->dRuns, tilesOf(ctx), ctx->order, ctx->cellStart, rec, ctx->list, out, ctx->clock,
                                   y64, 1, 1, 1, 0, nearRecords<D, Real>(ctx), n))));
         if (ctx->sumChunkCount > 0) {
@@ -1720,7 +1735,7 @@ int stepOnce(sph_context* ctx, int phase, int useClock, double forcedDt) {
                                                                     ctx->dVel, ctx->clock, forcedDt, st, n)));
         WITH_LANES(ctx, TIMED(ctx, kSweepContinuity,
                               (sweepContinuity<D, Real, L><<<tileGrid(ctx), b,
-                                                              nearShared<D, Real>(ctx), ctx->stream>>>(
+                                                              sweepLds<D, Real>(ctx, 1), ctx->stream>>>(
                             ctx->dRuns, tilesOf(ctx), ctx->order, ctx->cellStart, rec, ctx->list, out, ctx->clock,
                                   1, 0, 0, nearRecords<D, Real>(ctx), n))));
     }
@@ -1767,7 +1782,7 @@ int hostSweep(sph_context* ctx, int which, const double* velocity, const double*
     const int32_t* start = ctx->cellStart;
     const NeighborList list = ctx->list;
     if (list.capacity > 0)
-        WITH_LANES(ctx, (buildNeighbors<D, Real, L, false><<<tileGrid(ctx), b, nearShared<D, Real>(ctx), ctx->stream>>>(
+        WITH_LANES(ctx, (buildNeighbors<D, Real, L, false><<<tileGrid(ctx), b, sweepLds<D, Real>(ctx, 0), ctx->stream>>>(
                             ctx->dRuns, tilesOf(ctx), order, start, rec, list, emptyOut<Real>(ctx), ctx->clock, y64, 0,
                             nearRecords<D, Real>(ctx), n)));
     switch (which) {
@@ -1784,7 +1799,7 @@ int hostSweep(sph_context* ctx, int which, const double* velocity, const double*
             break;
         case 2:   // correction matrices
             out.corrections = static_cast<Real*>(out0);
-            WITH_LANES(ctx, (sweepAccelerations<D, Real, L><<<tileGrid(ctx), b, nearShared<D, Real>(ctx), ctx->stream>>>(
+            WITH_LANES(ctx, (sweepAccelerations<D, Real, L><<<tileGrid(ctx), b, sweepLds<D, Real>(ctx, 0), ctx->stream>>>(
                             ctx->dRuns, tilesOf(ctx), order, start, rec, list, out, ctx->clock, y64, 0, 0, 1, 0, nearRecords<D, Real>(ctx), n)));
             break;
         case 3: {  // accelerations, with the limiter's gradients first when the run has it
@@ -1793,19 +1808,19 @@ int hostSweep(sph_context* ctx, int which, const double* velocity, const double*
                 WITH_LANES(ctx, (sweepWallAndGradients<D, Real, L><<<tileGrid(ctx), b, 0, ctx->stream>>>(
                             ctx->dRuns, tilesOf(ctx), order, start, rec, list, out, ctx->clock, y64, 0, 0, 1, n)));
             out.acceleration = static_cast<Real*>(out0);
-            WITH_LANES(ctx, (sweepAccelerations<D, Real, L><<<tileGrid(ctx), b, nearShared<D, Real>(ctx), ctx->stream>>>(
+            WITH_LANES(ctx, (sweepAccelerations<D, Real, L><<<tileGrid(ctx), b, sweepLds<D, Real>(ctx, 0), ctx->stream>>>(
                             ctx->dRuns, tilesOf(ctx), order, start, rec, list, out, ctx->clock, y64, 0, 1, 0, limiter, nearRecords<D, Real>(ctx), n)));
             break;
         }
         case 4: {  // continuity, with the correction matrices first when the run has them
             const int correction = run0.correction;
             if (correction)
-                WITH_LANES(ctx, (sweepAccelerations<D, Real, L><<<tileGrid(ctx), b, nearShared<D, Real>(ctx), ctx->stream>>>(
+                WITH_LANES(ctx, (sweepAccelerations<D, Real, L><<<tileGrid(ctx), b, sweepLds<D, Real>(ctx, 0), ctx->stream>>>(
                             ctx->dRuns, tilesOf(ctx), order, start, rec, list, out, ctx->clock, y64, 0, 0, 1, 0, nearRecords<D, Real>(ctx), n)));
             out.rate = static_cast<Real*>(out0);
             for (int a = 0; a < 3; ++a) out.pos[a] = a < D ? ctx->state.position + a * ctx->state.stride : nullptr;
             WITH_LANES(ctx, (sweepContinuity<D, Real, L><<<tileGrid(ctx), b,
-                                                          nearShared<D, Real>(ctx), ctx->stream>>>(
+                                                          sweepLds<D, Real>(ctx, 1), ctx->stream>>>(
                             ctx->dRuns, tilesOf(ctx), order, start, rec, list, out, ctx->clock, 0, correction,
                                 run0.diffusion, nearRecords<D, Real>(ctx), n)));
             break;
@@ -1840,7 +1855,7 @@ int neighborsImpl(sph_context* ctx, int32_t* counts, const int64_t* offsets, int
     gatherRecords<D, Real><<<gridFor(n, b), b, 0, ctx->stream>>>(ctx->dRuns, ctx->runOf, ctx->order, src, rec, ctx->clock, 0, n);
     const double* y64 = ctx->state.position + ctx->state.stride;
     if (ctx->list.capacity > 0)
-        WITH_LANES(ctx, (buildNeighbors<D, Real, L, false><<<tileGrid(ctx), b, nearShared<D, Real>(ctx), ctx->stream>>>(
+        WITH_LANES(ctx, (buildNeighbors<D, Real, L, false><<<tileGrid(ctx), b, sweepLds<D, Real>(ctx, 0), ctx->stream>>>(
                             ctx->dRuns, tilesOf(ctx), ctx->order, ctx->cellStart, rec, ctx->list, emptyOut<Real>(ctx),
                             ctx->clock, y64, 0, nearRecords<D, Real>(ctx), n)));
     neighborExport<D, Real><<<gridFor(n, b), b, 0, ctx->stream>>>(ctx->dRuns, ctx->runOf, ctx->order, ctx->cellStart, rec,
