@@ -316,8 +316,9 @@ __device__ __forceinline__ int64_t listOffset(int32_t m, int64_t stride) {
     return (int64_t)((uint64_t)(uint32_t)m * (uint64_t)(uint32_t)stride);
 }
 
-// a / b, correctly rounded in both precisions, as the reference's float32 kernels divide.
-__device__ __forceinline__ float quotient(float a, float b) { return a / b; }
+// a / b in the pair sums: in float32 a times the hardware reciprocal of b (within about 2 ulp of
+// the rounded quotient; the sums carry float32 rounding throughout), in float64 correctly rounded.
+__device__ __forceinline__ float quotient(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 __device__ __forceinline__ double quotient(double a, double b) { return a / b; }
 
 // The smoothing kernels (sphGpu.py:139-186) at one run's constants: W and dW/dr / r, with a fast
