@@ -77,9 +77,8 @@ def kernelBytes(dimensions: int, real: int, entries: float) -> dict:
         'cellOrder': 8,                          # the order read and written in place
         'gatherRecords': ids + 16 * d + 8 + 2 * r + 1 + 4 + record + 4 + r,
         'buildNeighbors': ids + record + 4 + 4 + 4 * k + d * d * r,
-        'sweepAccelerations': ids + record + 4 + d * d * r + 4 + 4 * k + d * r + d * d * r,
-        'kick': ids + 16 + 16 * d + d * r + 4 * r,
-        'sweepContinuity': ids + record + 4 + d * d * r + 4 + 4 * k + 16 + 32 * d,
+        'sweepAccelerations': ids + record + 4 + d * d * r + 4 + 4 * k + 2 * d * r + d * d * r,
+        'sweepContinuity': ids + record + 4 + d * d * r + 4 + 4 * k + 16 + 32 * d + 2 * d * r,
     }
 
 

@@ -156,6 +156,7 @@ struct Records {
     int32_t* key;    // the particle's cell, in the context's cell array
     Real* G;         // velocity gradient
     Real* L;         // kernel gradient correction
+    Real* A;         // acceleration, D values per particle (zero on the boundary), for the kick
 };
 
 __device__ __forceinline__ int wrapIndex(int64_t a, int32_t n) {
@@ -434,6 +435,21 @@ struct NearCells {
     int32_t at[kRows];              // LDS index of lo
     int32_t held;                   // records copied, at most the capacity
     const Record<Real>* r;
+    // With `kicked`, every record read takes the kick v + a dt on a fluid particle's velocity, as the
+    // continuity sums read the velocities after the kick: `A` is each sorted particle's
+    // acceleration and `dt` the step.
+    bool kicked;
+    const Real* A;
+    Real dt;
+
+    __device__ __forceinline__ void kick(Record<Real>& record, int32_t k) const {
+        if (!kicked || !(signedLength<D, Real>(record) > Real(0))) return;
+        Real v[3];
+        velocityOf<D, Real>(record, v);
+#pragma unroll
+        for (int a = 0; a < D; ++a) v[a] = v[a] + A[(int64_t)k * D + a] * dt;
+        setVelocity<D, Real>(record, v);
+    }
 
     // The ranges around sorted particles s0..s1 of `run`, and the first `capacity` of their
     // records to copy. Every value is the same across the workgroup.
@@ -481,8 +497,11 @@ struct NearCells {
         for (int32_t first = (int32_t)threadIdx.x; first < held; first += 2 * step) {
             const int32_t second = first + step;
             const bool both = second < held;
-            const Record<Real> a = r[sourceOf(first)];
-            const Record<Real> b = r[sourceOf(both ? second : first)];
+            const int32_t fromA = sourceOf(first), fromB = sourceOf(both ? second : first);
+            Record<Real> a = r[fromA];
+            Record<Real> b = r[fromB];
+            kick(a, fromA);
+            kick(b, fromB);
             LdsVec4<Real>* to = shared + 4 * first;
             storeLds(to, a.q0);
             storeLds(to + 1, a.q1);
@@ -524,6 +543,7 @@ struct NearCells {
             out.q3 = loadLds(from + 3);
         } else {
             out = r[k];
+            kick(out, k);
         }
         return out;
     }

@@ -603,6 +603,7 @@ __global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
     NearCells<D, Real> cells;
     cells.r = rec.r;
     cells.held = 0;
+    cells.kicked = false;
 #pragma unroll
     for (int i = 0; i < NearCells<D, Real>::kRows; ++i) cells.lo[i] = cells.hi[i] = cells.at[i] = 0;
     if (runSteps && cached > 0) {
@@ -768,6 +769,7 @@ __global__ void SWEEP_BOUNDS((accelerationWaves<D, Real>()))
     NearCells<D, Real> cells;
     cells.r = rec.r;
     cells.held = 0;
+    cells.kicked = false;
 #pragma unroll
     for (int i = 0; i < NearCells<D, Real>::kRows; ++i) cells.lo[i] = cells.hi[i] = cells.at[i] = 0;
     if (runSteps && cached > 0) {
@@ -884,7 +886,10 @@ __global__ void SWEEP_BOUNDS((accelerationWaves<D, Real>()))
         if (!fluidI) {
             if (doAcceleration)
 #pragma unroll
-                for (int a = 0; a < D; ++a) out.acceleration[a * out.stride + i] = Real(0);
+                for (int a = 0; a < D; ++a) {
+                    out.acceleration[a * out.stride + i] = Real(0);
+                    if (resident) rec.A[s * D + a] = Real(0);
+                }
             if (correction)
 #pragma unroll
                 for (int a = 0; a < D; ++a)
@@ -901,6 +906,7 @@ __global__ void SWEEP_BOUNDS((accelerationWaves<D, Real>()))
                 for (int a = 0; a < D; ++a) {
                     const Real v = acc[a] + (Real)run.gravity[a];
                     out.acceleration[a * out.stride + i] = v;
+                    if (resident) rec.A[s * D + a] = v;
                     a2 += v * v;
                 }
                 magnitude = (double)sqrt(a2);
@@ -929,36 +935,6 @@ __global__ void SWEEP_BOUNDS((accelerationWaves<D, Real>()))
         }
     }
     if (resident && doAcceleration && activeRun && lane == 0) clock.acceleration[s] = fluidI ? magnitude : 0.0;
-}
-
-// Applies v += a dt on the fluid (sphGpu.py:1084-1109), refreshing the velocity records; the run's
-// first lane advances its clock.
-template <int D, typename Real>
-__global__ void kick(const RunDev* __restrict__ runs, const int32_t* __restrict__ runOf, const int32_t* __restrict__ order,
-                     Records<Real> rec, const Real* __restrict__ acceleration, double* const* __restrict__ vel64p,
-                     Clock clock, double forcedDt, int64_t stride, int64_t n) {
-    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    const int r = runOf[s];
-    if (!clock.active[r]) return;
-    const RunDev& run = runs[r];
-    const double dt = forcedDt > 0.0 ? forcedDt : runTimeStep(run, clock, r);
-    if (s == run.first) {
-        clock.dt[r] = dt;
-        clock.time[r] += dt;
-        clock.steps[r] += 1;
-    }
-    if (!(signedLength<D, Real>(rec.r[s]) > Real(0))) return;
-    const int32_t i = order[s];
-    Real v[3] = {Real(0), Real(0), Real(0)};
-#pragma unroll
-    for (int a = 0; a < D; ++a) {
-        double* u = vel64p[a];
-        const double value = u[i] + (double)acceleration[a * stride + i] * dt;
-        u[i] = value;
-        v[a] = (Real)value;
-    }
-    setVelocity<D, Real>(rec.r[s], v);
 }
 
 // The bed's boundary enforcement on one fluid particle (sphGpu.py:1197-1229, 1246-1310,
@@ -1041,14 +1017,16 @@ __device__ __forceinline__ void enforceBoundary(const sph_boundary& b, double* x
     }
 }
 
-// Sweep 3: continuity with the delta-SPH term (sphGpu.py:416-545); resident, the particle's
-// density advance with its floor (:1111-1140), drift (:1142-1186) and boundary enforcement
-// follow on its float64 state.
+// Sweep 3: continuity with the delta-SPH term (sphGpu.py:416-545). Resident, it is also the
+// step's integration: the sums read every velocity after the kick v += a dt (:1084-1109, taken in
+// float32 on the records as they are read), and each particle's float64 kick, density advance
+// with its floor (:1111-1140), drift (:1142-1186) and boundary enforcement follow; the run's first
+// particle advances its clock.
 template <int D, typename Real, int Lanes>
 __global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
     sweepContinuity(const RunDev* __restrict__ runs, Tiles tiles, const int32_t* __restrict__ order, const int32_t* __restrict__ start, Records<Real> rec,
                     NeighborList list, SweepOut<Real> out, Clock clock, int32_t resident, int32_t hostCorrection,
-                    int32_t hostDiffusion, int32_t cached, int64_t n) {
+                    int32_t hostDiffusion, double forcedDt, int32_t cached, int64_t n) {
     const int64_t tile = xcdBlock();
     const int r = tiles.run[tile];
     const int64_t s = tiles.first[tile] + threadIdx.x / Lanes;
@@ -1061,6 +1039,11 @@ __global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
     NearCells<D, Real> cells;
     cells.r = rec.r;
     cells.held = 0;
+    // resident, the step's kick v += a dt (sphGpu.py:1084-1109) reaches every velocity the sums read
+    const double dt = resident && runSteps ? (forcedDt > 0.0 ? forcedDt : runTimeStep(runs[r], clock, r)) : 0.0;
+    cells.kicked = resident != 0;
+    cells.A = rec.A;
+    cells.dt = (Real)dt;
 #pragma unroll
     for (int i = 0; i < NearCells<D, Real>::kRows; ++i) cells.lo[i] = cells.hi[i] = cells.at[i] = 0;
     if (runSteps && cached > 0) {
@@ -1083,7 +1066,7 @@ __global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
         kernel.init(run.kernel, run.sigmaUnit, run.hmax, run.uniform != 0);
         const double yy = run.period > 0.0 ? out.pos[1][order[s]] : 0.0;
         Real vi[3];
-        velocityOf<D, Real>(rec.r[s], vi);
+        velocityOf<D, Real>(cells((int32_t)s), vi);
         Real Li[3][3] = {};
         if (correction) {
 #pragma unroll
@@ -1145,16 +1128,21 @@ __global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
         out.rate[i] = total;
         return;
     }
+    if (own == run.first) {
+        clock.dt[r] = dt;
+        clock.time[r] += dt;
+        clock.steps[r] += 1;
+    }
     const bool boundaryOwn = !(signedLength<D, Real>(rec.r[own]) > Real(0));
-    const double dt = clock.dt[r];
     double rho = out.density64[i] + (double)total * dt;
     if (run.densityFloor > 0.0 && !boundaryOwn) rho = fmax(rho, run.densityFloor);
     out.density64[i] = rho;
     if (boundaryOwn) return;
+    // the kick in float64, then the drift
     double x[3] = {0.0, 0.0, 0.0}, v[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int a = 0; a < D; ++a) {
-        v[a] = out.vel[a][i];
+        v[a] = out.vel[a][i] + (double)out.acceleration[a * out.stride + i] * dt;
         x[a] = out.pos[a][i] + v[a] * dt;
     }
     if (run.period > 0.0) x[1] = x[1] - run.period * floor(x[1] / run.period);
@@ -1452,6 +1440,7 @@ int allocateRecords(sph_context* ctx, Records<Real>& rec, int64_t cap) {
     if (status == SPH_OK) status = allocate(ctx, &rec.key, cap);
     if (status == SPH_OK) status = allocate(ctx, &rec.G, cap * D * D);
     if (status == SPH_OK) status = allocate(ctx, &rec.L, cap * D * D);
+    if (status == SPH_OK) status = allocate(ctx, &rec.A, cap * D);
     return status;
 }
 
@@ -1729,15 +1718,11 @@ int stepOnce(sph_context* ctx, int phase, int useClock, double forcedDt) {
         }
     }
     if (phase & 2) {
-        TIMED(ctx, kKick,
-              (kick<D, Real><<<gridFor(n, b), b, 0, ctx->stream>>>(ctx->dRuns, ctx->runOf, ctx->order, rec,
-                                                                    static_cast<const Real*>(ctx->state.acceleration),
-                                                                    ctx->dVel, ctx->clock, forcedDt, st, n)));
         WITH_LANES(ctx, TIMED(ctx, kSweepContinuity,
                               (sweepContinuity<D, Real, L><<<tileGrid(ctx), b,
                                                               sweepLds<D, Real>(ctx, 1), ctx->stream>>>(
                             ctx->dRuns, tilesOf(ctx), ctx->order, ctx->cellStart, rec, ctx->list, out, ctx->clock,
-                                  1, 0, 0, nearRecords<D, Real>(ctx), n))));
+                                  1, 0, 0, forcedDt, nearRecords<D, Real>(ctx), n))));
     }
     return launchCheck(ctx, "device step");
 }
@@ -1822,7 +1807,7 @@ int hostSweep(sph_context* ctx, int which, const double* velocity, const double*
             WITH_LANES(ctx, (sweepContinuity<D, Real, L><<<tileGrid(ctx), b,
                                                           sweepLds<D, Real>(ctx, 1), ctx->stream>>>(
                             ctx->dRuns, tilesOf(ctx), order, start, rec, list, out, ctx->clock, 0, correction,
-                                run0.diffusion, nearRecords<D, Real>(ctx), n)));
+                                run0.diffusion, 0.0, nearRecords<D, Real>(ctx), n)));
             break;
         }
         default:
