@@ -130,14 +130,20 @@ class BatchEngine:
         engine.upload('length', np.concatenate(lengths))
         engine.upload('fluid', np.concatenate([p.isFluid for p in parts]).astype(np.uint8))
         engine.synchronize()
+        self.until = [0.0] * len(self.cases)
+        self.maxSteps = [0] * len(self.cases)
         for r in range(len(self.cases)):
-            engine.setClock(r, 0.0, 0.0, 0, 0)
+            self.setClock(r, 0.0, 0.0, 0, 0)
+
+    def setClock(self, run: int, time: float, untilTime: float, steps: int, maxSteps: int) -> None:
+        '''Run `run` is at `time` after `steps` steps, and steps while time < `untilTime` and steps < `maxSteps`.'''
+        self.engine.setClock(run, time, untilTime, steps, maxSteps)
+        self.until[run], self.maxSteps[run] = float(untilTime), int(maxSteps)
 
     def setClocks(self, untilTime: float = np.inf, maxSteps: int = 2**62, times = None, steps = None) -> None:
         '''Every run steps until its time reaches `untilTime` or it has taken `maxSteps` steps.'''
         for r in range(len(self.cases)):
-            self.engine.setClock(r, 0.0 if times is None else times[r], untilTime, 0 if steps is None else steps[r],
-                                 maxSteps)
+            self.setClock(r, 0.0 if times is None else times[r], untilTime, 0 if steps is None else steps[r], maxSteps)
 
     def advance(self, steps: int, graph: bool = True) -> None:
         '''Enqueue `steps` steps of every active run (asynchronous).'''

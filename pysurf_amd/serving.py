@@ -151,7 +151,7 @@ class Fleet:
         self.batch = BatchEngine(cases, self.precision, self.device, blockSize = block, lanes = lanes,
                                  listCapacity = capacity)
         for index, member in enumerate(self.members):
-            self.batch.engine.setClock(index, member.time, member.nextTrack, member.steps, 2**62)
+            self.batch.setClock(index, member.time, member.nextTrack, member.steps, 2**62)
 
     @property
     def particles(self) -> int:
@@ -171,7 +171,7 @@ class Fleet:
         needed = 1
         for index, member in enumerate(self.members):
             until = member.time if member.done else member.nextTrack
-            self.batch.engine.setClock(index, member.time, until, member.steps, 2**62)
+            self.batch.setClock(index, member.time, until, member.steps, 2**62)
             dt = float(clock['dt'][index])
             if not member.done and dt > 0.0:
                 needed = max(needed, int(math.ceil((until - member.time) / dt)) + 1)
