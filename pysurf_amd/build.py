@@ -2,9 +2,10 @@
 
 Builds libsph_hip.so, the engine's HIP library, in-tree for gfx950 with hipcc.
 
-Packed float32 instructions are switched off (-packed-fp32-ops) and float32 division and square
-root stay correctly rounded, so the float32 neighbor predicate is the one the float32 restatement
-evaluates on the host.
+Packed float32 instructions are switched off (-packed-fp32-ops), and float32 division and square
+root are correctly rounded where the source writes them: the neighbor predicate's root inside its
+rounding band, so the float32 predicate is the one the float32 restatement evaluates on the host.
+The pair sums call the hardware root and reciprocal by name.
 
 '''
 

@@ -120,7 +120,7 @@ __device__ __forceinline__ void velocityOf(const Record<Real>& r, Real (&v)[3]) 
 }
 
 // Field writers of one record: the wall pressure's results on a boundary particle, and the
-// velocity the kick refreshes.
+// kicked velocity the continuity sweep reads.
 template <int D, typename Real>
 __device__ __forceinline__ void setWall(Record<Real>& r, Real pressure, Real rho) {
     const Real pr2 = pressure / (rho * rho);
