@@ -159,6 +159,43 @@ int sph_global_sums(sph_context* ctx, double* sums);
  * during graph capture (it may allocate its copy buffer on first use). */
 int sph_sort_state(sph_context* ctx, int32_t* identity);
 
+/* The breaking read-out's measures, taken on the device from the bound state. A run's tracked
+ * particles are its fluid particles (in 3D those with |y - stripCentre| < stripHalfWidth), read as
+ * (x, z). They replace the host arrays the reference's BreakingTracker.update reads every
+ * 0.01 sqrt(d/g) (waveBreaking.py:810-906; the cells face measure :559-655): no reference entry
+ * point computes them. Each call reads the runs with `columns` > 0 (`active` for the face cells)
+ * and synchronizes. */
+typedef struct {
+    double x0, width;                     /* columns [x0 + c width, x0 + (c + 1) width) */
+    int32_t columns;                      /* 0 skips the run */
+    int32_t offset;                       /* the run's first column in `tops` */
+    double stripCentre, stripHalfWidth;   /* 3D only */
+} sph_profile_spec;
+
+/* tops[offset + c]: the highest z of the run's tracked particles in column c, -inf where none
+ * (freeSurfaceProfile before its half-column lift, waveBreaking.py:129-162). */
+int sph_surface_profile(sph_context* ctx, const sph_profile_spec* specs, double* tops);
+
+typedef struct {
+    double windowStart, windowEnd, base, spacing;   /* the face window (waveBreaking.py:559-634) */
+    int32_t columns, bands;               /* the occupancy kept: bands x columns cells */
+    int32_t offset;                       /* the run's first 32-bit word of `occupancy` */
+    int32_t active;                       /* 0 skips the run */
+    double wetHeight;                     /* the waterline's reach above the bed (:636-655) */
+    const double* bedX;                   /* the tracker's bed, bedPoints device values each */
+    const double* bedZ;
+    int32_t bedPoints;
+    double stripCentre, stripHalfWidth;   /* 3D only */
+} sph_face_spec;
+
+/* For each active run: `occupancy`, bands rows of ceil(columns / 32) words from `offset`, bit c of
+ * row b set where frontFaceLean marks cell (band b, column c); overflow[r] = 1 where a tracked
+ * particle marks a band at or above `bands`; landward[r * 16 + 2 m + (0, 1)], the (x, z) of the
+ * 8 most landward tracked particles within wetHeight (widened by 1e-9 of a metre) of the bed's
+ * linear interpolation, by descending x, NaN past the last. */
+int sph_face_cells(sph_context* ctx, const sph_face_spec* specs, uint32_t* occupancy, int32_t* overflow,
+                   double* landward);
+
 int sph_synchronize(sph_context* ctx);
 
 /* Per-kernel timing with HIP events recorded on the context's stream around every launch of a

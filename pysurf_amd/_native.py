@@ -56,6 +56,21 @@ class State(ctypes.Structure):
                 ('length', ctypes.c_void_p), ('fluid', ctypes.c_void_p), ('stride', ctypes.c_int64)]
 
 
+class ProfileSpec(ctypes.Structure):
+    '''sph_profile_spec'''
+    _fields_ = [('x0', ctypes.c_double), ('width', ctypes.c_double), ('columns', ctypes.c_int32),
+                ('offset', ctypes.c_int32), ('stripCentre', ctypes.c_double), ('stripHalfWidth', ctypes.c_double)]
+
+
+class FaceSpec(ctypes.Structure):
+    '''sph_face_spec'''
+    _fields_ = [('windowStart', ctypes.c_double), ('windowEnd', ctypes.c_double), ('base', ctypes.c_double),
+                ('spacing', ctypes.c_double), ('columns', ctypes.c_int32), ('bands', ctypes.c_int32),
+                ('offset', ctypes.c_int32), ('active', ctypes.c_int32), ('wetHeight', ctypes.c_double),
+                ('bedX', ctypes.c_void_p), ('bedZ', ctypes.c_void_p), ('bedPoints', ctypes.c_int32),
+                ('stripCentre', ctypes.c_double), ('stripHalfWidth', ctypes.c_double)]
+
+
 # Every entry point include/sph_hip.h declares, with its argument types
 signatures = {
     'sph_create': [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p],
@@ -81,6 +96,8 @@ signatures = {
     'sph_synchronize': [ctypes.c_void_p],
     'sph_global_sums': [ctypes.c_void_p, ctypes.c_void_p],
     'sph_sort_state': [ctypes.c_void_p, ctypes.c_void_p],
+    'sph_surface_profile': [ctypes.c_void_p, ctypes.POINTER(ProfileSpec), ctypes.c_void_p],
+    'sph_face_cells': [ctypes.c_void_p, ctypes.POINTER(FaceSpec), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
     'sph_set_launch': [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32],
     'sph_profile': [ctypes.c_void_p, ctypes.c_int32],
     'sph_kernel_times': [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32],

@@ -81,23 +81,37 @@ def frontFaceLean(positions: np.ndarray, crestX: float, stillWaterLevel: float, 
 
     '''
 
+    occupied, windowStart = faceCells(positions, crestX, stillWaterLevel, spacing, reach, shoreLimit)
+    return leanOfCells(occupied, windowStart, spacing)
+
+
+def faceWindow(crestX: float, stillWaterLevel: float, spacing: float, reach: float,
+               shoreLimit: float = np.inf) -> tuple[float, float, float, int]:
+    '''The face window's start, end, band base and column count (waveBreaking.py:575-590).'''
     windowStart = crestX - 0.5 * spacing
     windowEnd = min(crestX + reach, shoreLimit)
     base = stillWaterLevel + 0.5 * spacing
+    nColumns = max(1, int(np.ceil((windowEnd - windowStart) / spacing)))
+    return windowStart, windowEnd, base, nColumns
+
+
+def faceCells(positions: np.ndarray, crestX: float, stillWaterLevel: float, spacing: float, reach: float,
+              shoreLimit: float = np.inf) -> tuple[np.ndarray, float]:
+    '''The marked cells of the face window, (bands, columns) upward from still water, and the window's start.'''
+    windowStart, windowEnd, base, nColumns = faceWindow(crestX, stillWaterLevel, spacing, reach, shoreLimit)
     x, z = positions[:, 0], positions[:, 1]
     window = (x >= windowStart - 0.5 * spacing) & (x < windowEnd + 0.5 * spacing) & (z > base - 0.5 * spacing)
     if not np.any(window):
-        return -np.inf
+        return np.zeros((0, nColumns), dtype = bool), windowStart
     firstColumn = np.floor((x[window] - 0.5 * spacing - windowStart) / spacing).astype(np.int64)
     firstBand = np.floor((z[window] - 0.5 * spacing - base) / spacing).astype(np.int64)
-    nColumns = max(1, int(np.ceil((windowEnd - windowStart) / spacing)))
     occupied = np.zeros((int(firstBand.max()) + 2, nColumns), dtype = bool)
     for columnStep in (0, 1):
         for bandStep in (0, 1):
             column, band = firstColumn + columnStep, firstBand + bandStep
             inside = (column >= 0) & (column < nColumns) & (band >= 0)
             occupied[band[inside], column[inside]] = True
-    return leanOfCells(occupied, windowStart, spacing)
+    return occupied, windowStart
 
 
 def leanOfCells(occupied: np.ndarray, windowStart: float, spacing: float) -> float:
@@ -124,6 +138,26 @@ def waterlineX(positions: np.ndarray, bathymetry, wetHeight: float, minimumParti
     if len(nearBed) < minimumParticles:
         return float('nan')
     return float(np.sort(nearBed[:, 0])[-minimumParticles])
+
+
+def waterlineFromLandward(landward: np.ndarray, bathymetry, wetHeight: float, complete: bool,
+                          minimumParticles: int = 3) -> float | None:
+
+    '''
+
+    waterlineX from the most landward candidates only: `landward` (m, 2) holds the particles of
+    largest x among a superset of those within `wetHeight` of the bed, by descending x, and
+    `complete` says whether it holds the whole superset. The exact test on them decides the answer
+    whenever `minimumParticles` of them pass it or the candidates are all there are; otherwise
+    returns None, and the full positions must decide.
+
+    '''
+
+    landward = landward[np.isfinite(landward[:, 0])]
+    nearBed = landward[landward[:, 1] - bathymetry.elevation(landward[:, 0]) < wetHeight]
+    if len(nearBed) >= minimumParticles:
+        return float(np.sort(nearBed[:, 0])[-minimumParticles])
+    return float('nan') if complete else None
 
 
 @dataclass
@@ -213,14 +247,28 @@ class BreakingTracker:
 
     def update(self, time: float, positions: np.ndarray) -> None:
         '''Read the crest, face, trough and waterline of the fluid `positions` (N, 2) at `time` (waveBreaking.py:810-906).'''
+        self.observeMeasures(time, *self.measures(positions))
+
+    def measures(self, positions: np.ndarray) -> tuple:
+        '''The profile's column centres and elevations, the face window's marked cells and start, and the waterline of `positions` (N, 2).'''
         bathymetry, s = self.bathymetry, self.spacing
-        swl = bathymetry.stillWaterLevel
         centers, elevation = freeSurfaceProfile(positions, bathymetry.startX, bathymetry.endX, s)
+        crestX = self.crestOf(centers, elevation)
+        occupied, windowStart = faceCells(positions, crestX, bathymetry.stillWaterLevel, s, self.faceReach)
+        return centers, elevation, occupied, windowStart, waterlineX(positions, bathymetry, 2.0 * s)
+
+    def crestOf(self, centers: np.ndarray, elevation: np.ndarray) -> float:
+        '''The crest's x: the highest column of the free-surface profile.'''
+        return float(centers[int(np.nanargmax(elevation))])
+
+    def observeMeasures(self, time: float, centers: np.ndarray, elevation: np.ndarray, occupied: np.ndarray,
+                        windowStart: float, tip: float) -> None:
+        '''Record one update from its measures: the profile, the face window's marked cells and the waterline.'''
+        swl = self.bathymetry.stillWaterLevel
         crestIndex = int(np.nanargmax(elevation))
         crestX, crestHeight = float(centers[crestIndex]), float(elevation[crestIndex] - swl)
-        lean = frontFaceLean(positions, crestX, swl, s, self.faceReach)
+        lean = leanOfCells(occupied, windowStart, self.spacing)
         troughHeight = troughAhead(centers, elevation, crestX, 2.0 * self.faceReach) - swl
-        tip = waterlineX(positions, bathymetry, 2.0 * s)
         self.observe(time, crestX, crestHeight, lean, troughHeight, tip)
 
 

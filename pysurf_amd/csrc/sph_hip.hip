@@ -28,6 +28,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <map>
 #include <string>
 #include <vector>
@@ -1193,6 +1194,169 @@ __global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
 }
 
 // ---------------------------------------------------------------------------------------------
+// The breaking read-out
+// ---------------------------------------------------------------------------------------------
+
+// A float64's bits as an unsigned key ordered as the values are; 0 lies below every key.
+__device__ __forceinline__ unsigned long long orderedKey(double v) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+
+// Whether slot i is one of its run's tracked particles (fluid; in 3D within halfWidth of the strip's
+// centre line), and its (x, z).
+template <int D>
+__device__ __forceinline__ bool trackedPoint(const sph_state& st, int64_t i, double centre, double halfWidth, double& x,
+                                             double& z) {
+    if (!st.fluid[i]) return false;
+    x = st.position[i];
+    z = st.position[(D - 1) * st.stride + i];
+    return D == 2 || fabs(st.position[st.stride + i] - centre) < halfWidth;
+}
+
+// Each column's highest tracked particle: floor((x - x0) / width) as freeSurfaceProfile takes it
+// (waveBreaking.py:129-162), the maximum by an atomic on ordered keys, so in any order the same.
+template <int D>
+__global__ void surfaceTops(const int32_t* __restrict__ runOf, sph_state st, const sph_profile_spec* __restrict__ specs,
+                            unsigned long long* __restrict__ tops, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const sph_profile_spec& spec = specs[runOf[i]];
+    if (spec.columns <= 0) return;
+    double x, z;
+    if (!trackedPoint<D>(st, i, spec.stripCentre, spec.stripHalfWidth, x, z)) return;
+    const double column = floor((x - spec.x0) / spec.width);
+    if (!(column >= 0.0 && column < (double)spec.columns)) return;
+    atomicMax(&tops[spec.offset + (int64_t)column], orderedKey(z));
+}
+
+// The face window's cells, one spacing square, as frontFaceLean marks them (waveBreaking.py:
+// 559-634): each tracked particle in the window marks the 2 x 2 cells its own cell overlaps.
+template <int D>
+__global__ void faceCells(const int32_t* __restrict__ runOf, sph_state st, const sph_face_spec* __restrict__ specs,
+                          uint32_t* __restrict__ occupancy, int32_t* __restrict__ overflow, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t r = runOf[i];
+    const sph_face_spec& spec = specs[r];
+    if (!spec.active) return;
+    double x, z;
+    if (!trackedPoint<D>(st, i, spec.stripCentre, spec.stripHalfWidth, x, z)) return;
+    const double s = spec.spacing;
+    if (!(x >= spec.windowStart - 0.5 * s && x < spec.windowEnd + 0.5 * s && z > spec.base - 0.5 * s)) return;
+    const double firstColumn = floor((x - 0.5 * s - spec.windowStart) / s);
+    const double firstBand = floor((z - 0.5 * s - spec.base) / s);
+    const int64_t words = (spec.columns + 31) / 32;
+    for (int columnStep = 0; columnStep < 2; ++columnStep)
+        for (int bandStep = 0; bandStep < 2; ++bandStep) {
+            const double column = firstColumn + columnStep, band = firstBand + bandStep;
+            if (!(column >= 0.0 && column < (double)spec.columns && band >= 0.0)) continue;
+            if (band >= (double)spec.bands) {
+                overflow[r] = 1;
+                continue;
+            }
+            const int64_t c = (int64_t)column;
+            atomicOr(&occupancy[spec.offset + (int64_t)band * words + c / 32], 1u << (c % 32));
+        }
+}
+
+// z of the bed's piecewise-linear profile at x, constant beyond its ends.
+__device__ __forceinline__ double bedAt(const sph_face_spec& spec, double x) {
+    const int32_t m = spec.bedPoints;
+    if (x <= spec.bedX[0]) return spec.bedZ[0];
+    if (x >= spec.bedX[m - 1]) return spec.bedZ[m - 1];
+    int32_t low = 0, high = m - 1;
+    while (high - low > 1) {
+        const int32_t middle = (low + high) / 2;
+        if (spec.bedX[middle] <= x) low = middle; else high = middle;
+    }
+    const double share = (x - spec.bedX[low]) / (spec.bedX[high] - spec.bedX[low]);
+    return spec.bedZ[low] + share * (spec.bedZ[high] - spec.bedZ[low]);
+}
+
+constexpr int kLandward = 8;
+
+// Inserts (x, z) into a list of kLandward points kept by descending x.
+__device__ __forceinline__ void keepLandward(double (&px)[kLandward], double (&pz)[kLandward], double x, double z) {
+    if (!(x > px[kLandward - 1])) return;
+#pragma unroll
+    for (int m = 0; m < kLandward; ++m) {
+        if (x > px[m]) {
+            const double tx = px[m], tz = pz[m];
+            px[m] = x;
+            pz[m] = z;
+            x = tx;
+            z = tz;
+        }
+    }
+}
+
+// The kLandward most landward tracked particles near the bed among `chunk` slots of a run, one
+// workgroup per (run, slice): each thread keeps its own, and the workgroup merges them pairwise into
+// the slice's list at partial[(r * slices + slice) * kLandward].
+template <int D>
+__global__ void __launch_bounds__(256)
+    landwardPoints(const RunDev* __restrict__ runs, sph_state st, const sph_face_spec* __restrict__ specs,
+                   int64_t chunk, double2* __restrict__ partial) {
+    __shared__ double sx[256][kLandward], sz[256][kLandward];
+    const int r = blockIdx.x;
+    const sph_face_spec& spec = specs[r];
+    const RunDev& run = runs[r];
+    double px[kLandward], pz[kLandward];
+#pragma unroll
+    for (int m = 0; m < kLandward; ++m) px[m] = pz[m] = -INFINITY;
+    if (spec.active && spec.bedPoints > 0) {
+        const int64_t begin = run.first + (int64_t)blockIdx.y * chunk;
+        const int64_t end = min(run.first + run.count, begin + chunk);
+        for (int64_t i = begin + threadIdx.x; i < end; i += blockDim.x) {
+            double x, z;
+            if (!trackedPoint<D>(st, i, spec.stripCentre, spec.stripHalfWidth, x, z)) continue;
+            if (z - bedAt(spec, x) < spec.wetHeight + 1e-9) keepLandward(px, pz, x, z);
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < kLandward; ++m) {
+        sx[threadIdx.x][m] = px[m];
+        sz[threadIdx.x][m] = pz[m];
+    }
+    __syncthreads();
+    for (unsigned half = blockDim.x / 2; half > 0; half /= 2) {
+        if (threadIdx.x < half) {
+#pragma unroll
+            for (int m = 0; m < kLandward; ++m) keepLandward(px, pz, sx[threadIdx.x + half][m], sz[threadIdx.x + half][m]);
+#pragma unroll
+            for (int m = 0; m < kLandward; ++m) {
+                sx[threadIdx.x][m] = px[m];
+                sz[threadIdx.x][m] = pz[m];
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int m = 0; m < kLandward; ++m)
+            partial[((int64_t)r * gridDim.y + blockIdx.y) * kLandward + m] = make_double2(px[m], pz[m]);
+}
+
+// Each run's slices merged into its kLandward points, (x, z) by descending x, NaN past the last.
+__global__ void mergeLandward(const double2* __restrict__ partial, int32_t slices, int32_t runs,
+                              double* __restrict__ landward) {
+    const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= runs) return;
+    double px[kLandward], pz[kLandward];
+#pragma unroll
+    for (int m = 0; m < kLandward; ++m) px[m] = pz[m] = -INFINITY;
+    for (int64_t k = 0; k < (int64_t)slices * kLandward; ++k) {
+        const double2 p = partial[(int64_t)r * slices * kLandward + k];
+        keepLandward(px, pz, p.x, p.y);
+    }
+    for (int m = 0; m < kLandward; ++m) {
+        const bool kept = px[m] > -INFINITY;
+        landward[(int64_t)r * 2 * kLandward + 2 * m] = kept ? px[m] : NAN;
+        landward[(int64_t)r * 2 * kLandward + 2 * m + 1] = kept ? pz[m] : NAN;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
 // Global float64 sums
 // ---------------------------------------------------------------------------------------------
 
@@ -1315,6 +1479,8 @@ struct sph_context {
     // A copy of the bound state the state sort permutes from, allocated on first use
     void* sortCopy = nullptr;
     size_t sortBytes = 0;
+    void* readScratch = nullptr;   // the breaking read-out's buffers
+    size_t readBytes = 0;
     // The sweeps' tiles for the current workgroup size and lanes (tilePer particles each)
     int32_t* tileRun = nullptr;
     int64_t* tileFirst = nullptr;
@@ -1422,6 +1588,9 @@ void releaseScratch(sph_context* ctx) {
     if (ctx->sortCopy) (void)hipFree(ctx->sortCopy);
     ctx->sortCopy = nullptr;
     ctx->sortBytes = 0;
+    if (ctx->readScratch) (void)hipFree(ctx->readScratch);
+    ctx->readScratch = nullptr;
+    ctx->readBytes = 0;
     for (void* p : ctx->allocations) (void)hipFree(p);
     ctx->allocations.clear();
     ctx->dRuns = nullptr;
@@ -2256,6 +2425,129 @@ int sph_global_sums(sph_context* ctx, double* sums) {
     status = launchCheck(ctx, "global sums");
     if (status != SPH_OK) return status;
     HIP_TRY(hipMemcpyAsync(sums, ctx->sumOut, (size_t)runs * kSums * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return SPH_OK;
+}
+
+namespace {
+
+// The read-out's device buffers, grown to `bytes` (allocates outside graph capture only).
+int readBuffers(sph_context* ctx, size_t bytes) {
+    if (bytes <= ctx->readBytes) return SPH_OK;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (ctx->readScratch) (void)hipFree(ctx->readScratch);
+    ctx->readScratch = nullptr;
+    ctx->readBytes = 0;
+    HIP_TRY(hipMalloc(&ctx->readScratch, bytes));
+    ctx->readBytes = bytes;
+    return SPH_OK;
+}
+
+size_t aligned(size_t bytes) { return (bytes + 255) & ~(size_t)255; }
+
+}  // namespace
+
+int sph_surface_profile(sph_context* ctx, const sph_profile_spec* specs, double* tops) {
+    int status = requireBound(ctx);
+    if (status != SPH_OK) return status;
+    if (!specs || !tops) return fail(ctx, SPH_ERR_VALUE, "The profile needs its specifications and an output.");
+    const int32_t runs = (int32_t)ctx->runs.size();
+    int64_t columns = 0;
+    for (int32_t r = 0; r < runs; ++r) {
+        if (specs[r].columns <= 0) continue;
+        if (specs[r].offset < 0 || !(specs[r].width > 0.0))
+            return fail(ctx, SPH_ERR_VALUE, "A profile needs a positive column width and a place in the output.");
+        columns = std::max<int64_t>(columns, (int64_t)specs[r].offset + specs[r].columns);
+    }
+    const size_t specBytes = aligned(runs * sizeof(sph_profile_spec));
+    status = readBuffers(ctx, specBytes + aligned(std::max<int64_t>(columns, 1) * sizeof(unsigned long long)));
+    if (status != SPH_OK) return status;
+    sph_profile_spec* dSpecs = static_cast<sph_profile_spec*>(ctx->readScratch);
+    unsigned long long* dTops = reinterpret_cast<unsigned long long*>(static_cast<char*>(ctx->readScratch) + specBytes);
+    HIP_TRY(hipMemcpyAsync(dSpecs, specs, runs * sizeof(sph_profile_spec), hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemsetAsync(dTops, 0, std::max<int64_t>(columns, 1) * sizeof(unsigned long long), ctx->stream));
+    const int64_t n = ctx->total;
+    if (n > 0) {
+        if (ctx->dims == 2)
+            surfaceTops<2><<<gridFor(n, 256), 256, 0, ctx->stream>>>(ctx->runOf, ctx->state, dSpecs, dTops, n);
+        else
+            surfaceTops<3><<<gridFor(n, 256), 256, 0, ctx->stream>>>(ctx->runOf, ctx->state, dSpecs, dTops, n);
+    }
+    status = launchCheck(ctx, "surface profile");
+    if (status != SPH_OK) return status;
+    std::vector<unsigned long long> keys(std::max<int64_t>(columns, 1));
+    HIP_TRY(hipMemcpyAsync(keys.data(), dTops, keys.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                           ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    for (int32_t r = 0; r < runs; ++r)
+        for (int32_t c = 0; c < specs[r].columns; ++c) {
+            const unsigned long long key = keys[specs[r].offset + c];
+            double value = -std::numeric_limits<double>::infinity();
+            if (key) {
+                const unsigned long long bits = (key >> 63) ? (key & 0x7fffffffffffffffull) : ~key;
+                std::memcpy(&value, &bits, sizeof value);
+            }
+            tops[specs[r].offset + c] = value;
+        }
+    return SPH_OK;
+}
+
+int sph_face_cells(sph_context* ctx, const sph_face_spec* specs, uint32_t* occupancy, int32_t* overflow,
+                   double* landward) {
+    int status = requireBound(ctx);
+    if (status != SPH_OK) return status;
+    if (!specs || !occupancy || !overflow || !landward) return fail(ctx, SPH_ERR_VALUE, "The face cells need their arrays.");
+    const int32_t runs = (int32_t)ctx->runs.size();
+    int64_t words = 0;
+    for (int32_t r = 0; r < runs; ++r) {
+        const sph_face_spec& spec = specs[r];
+        if (!spec.active) continue;
+        if (spec.columns <= 0 || spec.bands <= 0 || spec.offset < 0 || !(spec.spacing > 0.0) || spec.bedPoints < 0 ||
+            (spec.bedPoints > 0 && (!spec.bedX || !spec.bedZ)))
+            return fail(ctx, SPH_ERR_VALUE, "A face window needs columns, bands, a spacing, a place in the output and its bed.");
+        words = std::max<int64_t>(words, (int64_t)spec.offset + (int64_t)spec.bands * ((spec.columns + 31) / 32));
+    }
+    const size_t specBytes = aligned(runs * sizeof(sph_face_spec));
+    const size_t wordBytes = aligned(std::max<int64_t>(words, 1) * sizeof(uint32_t));
+    const size_t flagBytes = aligned(runs * sizeof(int32_t));
+    const size_t pointBytes = aligned((size_t)runs * 2 * kLandward * sizeof(double));
+    // Slices of at most 16384 slots per workgroup, at most 64 per run
+    int64_t longest = 0;
+    for (int32_t r = 0; r < runs; ++r) longest = std::max<int64_t>(longest, ctx->runs[r].count);
+    const int32_t slices = (int32_t)std::min<int64_t>(64, std::max<int64_t>(1, (longest + 16383) / 16384));
+    const int64_t chunk = std::max<int64_t>(1, (longest + slices - 1) / slices);
+    const size_t partialBytes = aligned((size_t)runs * slices * kLandward * sizeof(double2));
+    status = readBuffers(ctx, specBytes + wordBytes + flagBytes + pointBytes + partialBytes);
+    if (status != SPH_OK) return status;
+    char* base = static_cast<char*>(ctx->readScratch);
+    sph_face_spec* dSpecs = reinterpret_cast<sph_face_spec*>(base);
+    uint32_t* dWords = reinterpret_cast<uint32_t*>(base + specBytes);
+    int32_t* dFlags = reinterpret_cast<int32_t*>(base + specBytes + wordBytes);
+    double* dPoints = reinterpret_cast<double*>(base + specBytes + wordBytes + flagBytes);
+    double2* dPartial = reinterpret_cast<double2*>(base + specBytes + wordBytes + flagBytes + pointBytes);
+    HIP_TRY(hipMemcpyAsync(dSpecs, specs, runs * sizeof(sph_face_spec), hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemsetAsync(dWords, 0, std::max<int64_t>(words, 1) * sizeof(uint32_t), ctx->stream));
+    HIP_TRY(hipMemsetAsync(dFlags, 0, runs * sizeof(int32_t), ctx->stream));
+    const int64_t n = ctx->total;
+    if (n > 0) {
+        if (ctx->dims == 2)
+            faceCells<2><<<gridFor(n, 256), 256, 0, ctx->stream>>>(ctx->runOf, ctx->state, dSpecs, dWords, dFlags, n);
+        else
+            faceCells<3><<<gridFor(n, 256), 256, 0, ctx->stream>>>(ctx->runOf, ctx->state, dSpecs, dWords, dFlags, n);
+    }
+    const dim3 grid((unsigned)runs, (unsigned)slices);
+    if (ctx->dims == 2)
+        landwardPoints<2><<<grid, 256, 0, ctx->stream>>>(ctx->dRuns, ctx->state, dSpecs, chunk, dPartial);
+    else
+        landwardPoints<3><<<grid, 256, 0, ctx->stream>>>(ctx->dRuns, ctx->state, dSpecs, chunk, dPartial);
+    mergeLandward<<<(runs + 63) / 64, 64, 0, ctx->stream>>>(dPartial, slices, runs, dPoints);
+    status = launchCheck(ctx, "face cells");
+    if (status != SPH_OK) return status;
+    if (words > 0)
+        HIP_TRY(hipMemcpyAsync(occupancy, dWords, words * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(overflow, dFlags, runs * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(landward, dPoints, (size_t)runs * 2 * kLandward * sizeof(double), hipMemcpyDeviceToHost,
+                           ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     return SPH_OK;
 }

@@ -232,6 +232,45 @@ class Engine:
         self.check(self.library.sph_global_sums(self.handle, sums.ctypes.data_as(ctypes.c_void_p)), 'sph_global_sums')
         return sums
 
+    # -- Breaking read-out -- #
+    def _readSpecs(self, kind, specs: list):
+        if self._runs is None or len(specs) != len(self._runs):
+            raise ValueError(f'The read-out needs one specification per run ({len(specs)} given).')
+        return (kind * len(specs))(*specs)
+
+    def surfaceProfile(self, specs: list) -> np.ndarray:
+        '''
+
+        The highest tracked particle of each column (sph_surface_profile): `specs` holds one
+        _native.ProfileSpec per run; returns the tops, -inf where a column is empty. Synchronizes.
+
+        '''
+        array = self._readSpecs(_native.ProfileSpec, specs)
+        total = max([spec.offset + spec.columns for spec in specs if spec.columns > 0], default = 0)
+        tops = np.empty(max(total, 1))
+        self.check(self.library.sph_surface_profile(self.handle, array, tops.ctypes.data_as(ctypes.c_void_p)),
+                   'sph_surface_profile')
+        return tops[:total]
+
+    def faceCells(self, specs: list) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+        '''
+
+        The face windows' marked cells and the most landward particles near the bed
+        (sph_face_cells): `specs` holds one _native.FaceSpec per run. Returns the occupancy words,
+        each run's overflow flag and its landward points, (runs, 8, 2) by descending x, NaN past the
+        last. Synchronizes.
+
+        '''
+        array = self._readSpecs(_native.FaceSpec, specs)
+        words = max([spec.offset + spec.bands * -(-spec.columns // 32) for spec in specs if spec.active], default = 0)
+        occupancy = np.zeros(max(words, 1), dtype = np.uint32)
+        overflow = np.zeros(len(specs), dtype = np.int32)
+        landward = np.zeros((len(specs), 8, 2))
+        self.check(self.library.sph_face_cells(self.handle, array, occupancy.ctypes.data_as(ctypes.c_void_p),
+                                               overflow.ctypes.data_as(ctypes.c_void_p),
+                                               landward.ctypes.data_as(ctypes.c_void_p)), 'sph_face_cells')
+        return occupancy[:words], overflow, landward
+
     def setLaunch(self, blockSize: int = 256, lanes: int = 0, listCapacity: int = -1) -> None:
         '''
 

@@ -26,6 +26,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
+from pysurf_amd import readout
 from pysurf_amd.batch import BatchEngine
 from pysurf_amd.breaking import BreakingTracker, trackInterval
 
@@ -49,14 +50,17 @@ def trackerOf(case) -> BreakingTracker:
     return BreakingTracker(bed, case.spacing, faceReach = depth, searchFromX = toe)
 
 
+def stripOf(case) -> tuple[float, float]:
+    '''The centre line and half width of the strip a 3D run's tracker reads: the tank's centre, a quarter depth wide.'''
+    if case.dimensions == 2:
+        return 0.0, float('inf')
+    startY, endY = case.boundary.field.bounds[2:]
+    return 0.5 * (startY + endY), 0.125 * case.parameters.get('depth', 1.0)
+
+
 def trackedPositions(case, positions: np.ndarray, isFluid: np.ndarray) -> np.ndarray:
     '''The fluid positions a run's tracker reads: (x, z); in 3D the centre strip a quarter depth wide.'''
-    fluid = positions[isFluid]
-    if case.dimensions == 2:
-        return fluid
-    startY, endY = case.boundary.field.bounds[2:]
-    strip = np.abs(fluid[:, 1] - 0.5 * (startY + endY)) < 0.125 * case.parameters.get('depth', 1.0)
-    return fluid[strip][:, [0, 2]]
+    return readout.trackedPositions(positions, isFluid, case.dimensions, *stripOf(case))
 
 
 @dataclass
@@ -104,16 +108,23 @@ class Fleet:
         the mix
     chunk : int
         Steps per graph-replayed advance
+    measure : str
+        'device': the trackers' measures are taken on the device (pysurf_amd.readout) and only they
+        come back; 'host': every reading downloads the positions and the trackers measure them
 
     '''
 
     def __init__(self, dimensions: int, precision: str = 'float32', lanes: int = 2, block: int = 256,
-                 listCapacity: int = -1, chunk: int = 8, device = None) -> None:
+                 listCapacity: int = -1, chunk: int = 8, device = None, measure: str = 'device') -> None:
+        if measure not in ('device', 'host'):
+            raise ValueError(f"A fleet measures on 'device' or 'host', not {measure!r}.")
         self.dimensions = dimensions
         self.precision = precision
         self.shape = (block, lanes, listCapacity)
         self.chunk = chunk
         self.device = device
+        self.measure = measure
+        self.readout = readout.DeviceReadout()
         self.members: list[Member] = []
         self.batch: BatchEngine | None = None
         self.updates = 0
@@ -188,13 +199,17 @@ class Fleet:
 
         while not self.reached():
             self.batch.advance(self.chunk)
-        positions = self.batch.engine.download('position', self.dimensions)
-        reading = {'positions': positions, 'offsets': list(self.batch.offsets), 'members': []}
+        reading = {'offsets': list(self.batch.offsets), 'members': []}
         for index, member in enumerate(self.members):
             if member.done:
                 continue
             reading['members'].append((index, member, member.time, member.steps))
             member.nextTrack += member.interval
+        if self.measure == 'device':
+            entries = [(index, member.tracker, *stripOf(member.case)) for index, member, _, _ in reading['members']]
+            reading['measures'] = self.readout.measure(self.batch, entries)
+        else:
+            reading['positions'] = self.batch.engine.download('position', self.dimensions)
         return reading
 
     def reached(self) -> bool:
@@ -208,16 +223,21 @@ class Fleet:
         return ready
 
     def track(self, reading: dict, wall: float) -> list:
-        '''Each member's tracker reads its positions of `reading`; returns the members that finished at it.'''
-        positions, offsets = reading['positions'], reading['offsets']
+        '''Each member's tracker reads its measures (or positions) of `reading`; returns the members that finished at it.'''
+        offsets = reading['offsets']
         finished = []
         for index, member, time, steps in reading['members']:
             if member.done:
                 continue
-            a, b = offsets[index], offsets[index + 1]
-            tracked = trackedPositions(member.case, positions[a:b], member.case.particles.isFluid)
-            if len(tracked):
-                member.tracker.update(time, tracked)
+            if 'measures' in reading:
+                measures = reading['measures'][index]
+                if measures is not None:
+                    member.tracker.observeMeasures(time, *measures)
+            else:
+                a, b = offsets[index], offsets[index + 1]
+                tracked = trackedPositions(member.case, reading['positions'][a:b], member.case.particles.isFluid)
+                if len(tracked):
+                    member.tracker.update(time, tracked)
             self.updates += 1
             if member.tracker.breakingFinal or time >= member.endTime:
                 member.done = True
@@ -228,12 +248,13 @@ class Fleet:
 
 
 def runToBreaking(cases: list, precision: str = 'float32', endTime: float = 10.0, lanes: int = 2,
-                  listCapacity: int = -1, chunk: int = 8) -> list:
+                  listCapacity: int = -1, chunk: int = 8, measure: str = 'device') -> list:
     '''Step `cases` together to their breaking points; returns each run's tracker, in order.'''
     dimensions = {case.dimensions for case in cases}
     if len(dimensions) != 1:
         raise ValueError('One fleet steps runs of one dimension.')
-    fleet = Fleet(dimensions.pop(), precision, lanes = lanes, listCapacity = listCapacity, chunk = chunk)
+    fleet = Fleet(dimensions.pop(), precision, lanes = lanes, listCapacity = listCapacity, chunk = chunk,
+                  measure = measure)
     members = [Member(key = index, case = case, endTime = endTime) for index, case in enumerate(cases)]
     fleet.rebuild([], members)
     fleet.launch()
@@ -297,6 +318,7 @@ class Server:
     chunk: int = 8
     listCapacity2D: int = 64
     listCapacity3D: int = 768
+    measure: str = 'device'
     fleets: dict = field(default_factory = dict)
     served: list = field(default_factory = list)
     queueDepth: list = field(default_factory = list)
@@ -309,7 +331,7 @@ class Server:
         if dimensions not in self.fleets:
             capacity = self.listCapacity2D if dimensions == 2 else self.listCapacity3D
             self.fleets[dimensions] = Fleet(dimensions, self.precision, lanes = self.lanes, listCapacity = capacity,
-                                            chunk = self.chunk)
+                                            chunk = self.chunk, measure = self.measure)
         return self.fleets[dimensions]
 
     def serve(self, requests: list, wallLimit: float = float('inf')) -> list:

@@ -40,6 +40,10 @@ def testStructuresMatchTheHeaderLayout():
     assert ctypes.sizeof(_native.Run) == 392
     assert ctypes.sizeof(_native.State) == 72
     assert _native.Run.boundary.offset == 176
+    assert ctypes.sizeof(_native.ProfileSpec) == 40
+    assert ctypes.sizeof(_native.FaceSpec) == 96
+    assert _native.FaceSpec.bedPoints.offset == 72
+    assert _native.FaceSpec.stripCentre.offset == 80
 
 
 def testBadArgumentsFailWithoutADevice():
