@@ -178,7 +178,7 @@ int sph_surface_profile(sph_context* ctx, const sph_profile_spec* specs, double*
 
 typedef struct {
     double windowStart, windowEnd, base, spacing;   /* the face window (waveBreaking.py:559-634) */
-    int32_t columns, bands;               /* the occupancy kept: bands x columns cells */
+    int32_t columns, bands;               /* the occupancy kept: bands x columns cells (0: landward points only) */
     int32_t offset;                       /* the run's first 32-bit word of `occupancy` */
     int32_t active;                       /* 0 skips the run */
     double wetHeight;                     /* the waterline's reach above the bed (:636-655) */
@@ -195,6 +195,27 @@ typedef struct {
  * linear interpolation, by descending x, NaN past the last. */
 int sph_face_cells(sph_context* ctx, const sph_face_spec* specs, uint32_t* occupancy, int32_t* overflow,
                    double* landward);
+
+typedef struct {
+    double x0, y0;                        /* node (0, 0) */
+    double stepX, stepY;                  /* node (i, j) stands at (x0 + i stepX, y0 + j stepY) */
+    double binWidth;                      /* the bins particles are sorted into: half a spacing */
+    int32_t nx, ny;                       /* nodes; nx = 0 skips the run */
+    int32_t reach;                        /* ceil(2 h / binWidth) */
+    int32_t pad;
+    int64_t offset;                       /* the run's first node in `fraction`, ny rows of nx */
+    double h;                             /* the cubic spline's smoothing length */
+    double area;                          /* each particle's area where areaPerMass is 0 */
+    double areaPerMass;                   /* > 0: a particle's area is its mass times this */
+    double lowX, lowZ, highX, highZ;      /* the particles taken: lowX < x < highX and lowZ < z < highZ */
+    double stripCentre, stripHalfWidth;   /* 3D only */
+} sph_fraction_spec;
+
+/* fraction[offset + j nx + i]: the kernel-smoothed fluid fraction sum_p A_p W(|node - p|, h) of the
+ * run's tracked particles at node (i, j), with the 2D cubic spline (fluidFraction,
+ * waveBreaking.py:206-262). Each node sums the particles bin by bin, rows of bins outer, and in slot
+ * order within a bin, so the result does not depend on the run's neighbours or the launch. */
+int sph_fluid_fraction(sph_context* ctx, const sph_fraction_spec* specs, double* fraction);
 
 int sph_synchronize(sph_context* ctx);
 

@@ -71,6 +71,16 @@ class FaceSpec(ctypes.Structure):
                 ('stripCentre', ctypes.c_double), ('stripHalfWidth', ctypes.c_double)]
 
 
+class FractionSpec(ctypes.Structure):
+    '''sph_fraction_spec'''
+    _fields_ = [('x0', ctypes.c_double), ('y0', ctypes.c_double), ('stepX', ctypes.c_double), ('stepY', ctypes.c_double),
+                ('binWidth', ctypes.c_double), ('nx', ctypes.c_int32), ('ny', ctypes.c_int32), ('reach', ctypes.c_int32),
+                ('pad', ctypes.c_int32), ('offset', ctypes.c_int64), ('h', ctypes.c_double), ('area', ctypes.c_double),
+                ('areaPerMass', ctypes.c_double), ('lowX', ctypes.c_double), ('lowZ', ctypes.c_double),
+                ('highX', ctypes.c_double), ('highZ', ctypes.c_double), ('stripCentre', ctypes.c_double),
+                ('stripHalfWidth', ctypes.c_double)]
+
+
 # Every entry point include/sph_hip.h declares, with its argument types
 signatures = {
     'sph_create': [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p],
@@ -98,6 +108,7 @@ signatures = {
     'sph_sort_state': [ctypes.c_void_p, ctypes.c_void_p],
     'sph_surface_profile': [ctypes.c_void_p, ctypes.POINTER(ProfileSpec), ctypes.c_void_p],
     'sph_face_cells': [ctypes.c_void_p, ctypes.POINTER(FaceSpec), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
+    'sph_fluid_fraction': [ctypes.c_void_p, ctypes.POINTER(FractionSpec), ctypes.c_void_p],
     'sph_set_launch': [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32],
     'sph_profile': [ctypes.c_void_p, ctypes.c_int32],
     'sph_kernel_times': [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32],

@@ -1356,6 +1356,104 @@ __global__ void mergeLandward(const double2* __restrict__ partial, int32_t slice
     }
 }
 
+// The fluid fraction. Each run's tracked particles inside its window are sorted into bins of half a
+// spacing: a bin grid reaching `reach` + 1 bins past the nodes on every side, (nx + 2 reach + 2) x
+// (ny + 2 reach + 2), holds every particle that can reach a node.
+
+// Each slot's bin (binOffset[run] + local bin), or -1; the bins' counts.
+template <int D>
+__global__ void fractionKeys(const int32_t* __restrict__ runOf, sph_state st, const sph_fraction_spec* __restrict__ specs,
+                             const int64_t* __restrict__ binOffset, int32_t* __restrict__ key, int32_t* __restrict__ counts,
+                             int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t r = runOf[i];
+    const sph_fraction_spec& spec = specs[r];
+    int32_t k = -1;
+    double x, z;
+    if (spec.nx > 0 && trackedPoint<D>(st, i, spec.stripCentre, spec.stripHalfWidth, x, z) && x > spec.lowX &&
+        x < spec.highX && z > spec.lowZ && z < spec.highZ) {
+        const double bx = floor((x - spec.x0) / spec.binWidth) + spec.reach + 1;
+        const double bz = floor((z - spec.y0) / spec.binWidth) + spec.reach + 1;
+        const int32_t width = spec.nx + 2 * spec.reach + 2, height = spec.ny + 2 * spec.reach + 2;
+        if (bx >= 0.0 && bx < (double)width && bz >= 0.0 && bz < (double)height) {
+            k = (int32_t)(binOffset[r] + (int64_t)bz * width + (int64_t)bx);
+            atomicAdd(&counts[k], 1);
+        }
+    }
+    key[i] = k;
+}
+
+// Slots into their bins, in any order (fill counts down from each bin's count).
+__global__ void fractionScatter(const int32_t* __restrict__ key, const int32_t* __restrict__ start,
+                                int32_t* __restrict__ fill, int32_t* __restrict__ members, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t k = key[i];
+    if (k < 0) return;
+    const int32_t at = atomicSub(&fill[k], 1) - 1;
+    members[start[k] + at] = (int32_t)i;
+}
+
+// Each bin's slots in ascending order (a bin holds a particle or so).
+__global__ void fractionOrder(const int32_t* __restrict__ start, int32_t* __restrict__ members, int64_t bins) {
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= bins) return;
+    const int32_t first = start[b], last = start[b + 1];
+    for (int32_t k = first + 1; k < last; ++k) {
+        const int32_t v = members[k];
+        int32_t m = k - 1;
+        while (m >= first && members[m] > v) {
+            members[m + 1] = members[m];
+            --m;
+        }
+        members[m + 1] = v;
+    }
+}
+
+// phi at each node of one run: the bins from `reach` + 1 below to `reach` above the node's own,
+// rows outer; a bin's particles summed apart, then added.
+template <int D, typename Real>
+__global__ void __launch_bounds__(256)
+    fractionNodes(sph_state st, const sph_fraction_spec* __restrict__ specs, int32_t run, int64_t binOffset,
+                  const int32_t* __restrict__ start, const int32_t* __restrict__ members, double* __restrict__ fraction) {
+#pragma clang fp contract(off)
+    const sph_fraction_spec& spec = specs[run];
+    const int64_t node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (node >= (int64_t)spec.nx * spec.ny) return;
+    const int32_t i = (int32_t)(node % spec.nx), j = (int32_t)(node / spec.nx);
+    const double gx = spec.x0 + (double)i * spec.stepX, gz = spec.y0 + (double)j * spec.stepY;
+    const int32_t width = spec.nx + 2 * spec.reach + 2;
+    const double h = spec.h, sigma = 10.0 / (7.0 * M_PI * h * h);
+    const Real* mass = static_cast<const Real*>(st.mass);
+    double phi = 0.0;
+    for (int32_t oy = -spec.reach; oy <= spec.reach + 1; ++oy)
+        for (int32_t ox = -spec.reach; ox <= spec.reach + 1; ++ox) {
+            // Particles whose bin is (i - ox, j - oy), shifted into the bin grid
+            const int64_t b = binOffset + (int64_t)(j - oy + spec.reach + 1) * width + (i - ox + spec.reach + 1);
+            const int32_t first = start[b], last = start[b + 1];
+            if (first == last) continue;
+            double partial = 0.0;
+            for (int32_t k = first; k < last; ++k) {
+                const int32_t p = members[k];
+                const double dx = gx - st.position[p];
+                const double dz = gz - st.position[(D - 1) * st.stride + p];
+                const double q = hypot(dx, dz) / h;
+                double w = 0.0;
+                if (q < 1.0)
+                    w = sigma * (1.0 - 1.5 * (q * q) + 0.75 * (q * q * q));
+                else if (q < 2.0) {
+                    const double t = 2.0 - q;
+                    w = sigma * 0.25 * (t * t * t);
+                }
+                const double area = spec.areaPerMass > 0.0 ? (double)mass[p] * spec.areaPerMass : spec.area;
+                partial += area * w;
+            }
+            phi += partial;
+        }
+    fraction[spec.offset + node] = phi;
+}
+
 // ---------------------------------------------------------------------------------------------
 // Global float64 sums
 // ---------------------------------------------------------------------------------------------
@@ -2502,7 +2600,7 @@ int sph_face_cells(sph_context* ctx, const sph_face_spec* specs, uint32_t* occup
     for (int32_t r = 0; r < runs; ++r) {
         const sph_face_spec& spec = specs[r];
         if (!spec.active) continue;
-        if (spec.columns <= 0 || spec.bands <= 0 || spec.offset < 0 || !(spec.spacing > 0.0) || spec.bedPoints < 0 ||
+        if (spec.columns < 0 || spec.bands < 0 || spec.offset < 0 || !(spec.spacing > 0.0) || spec.bedPoints < 0 ||
             (spec.bedPoints > 0 && (!spec.bedX || !spec.bedZ)))
             return fail(ctx, SPH_ERR_VALUE, "A face window needs columns, bands, a spacing, a place in the output and its bed.");
         words = std::max<int64_t>(words, (int64_t)spec.offset + (int64_t)spec.bands * ((spec.columns + 31) / 32));
@@ -2549,6 +2647,95 @@ int sph_face_cells(sph_context* ctx, const sph_face_spec* specs, uint32_t* occup
     HIP_TRY(hipMemcpyAsync(landward, dPoints, (size_t)runs * 2 * kLandward * sizeof(double), hipMemcpyDeviceToHost,
                            ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return SPH_OK;
+}
+
+int sph_fluid_fraction(sph_context* ctx, const sph_fraction_spec* specs, double* fraction) {
+    int status = requireBound(ctx);
+    if (status != SPH_OK) return status;
+    if (!specs || !fraction) return fail(ctx, SPH_ERR_VALUE, "The fluid fraction needs its specifications and an output.");
+    const int32_t runs = (int32_t)ctx->runs.size();
+    std::vector<int64_t> binOffset(runs, 0);
+    int64_t bins = 0, nodes = 0;
+    for (int32_t r = 0; r < runs; ++r) {
+        const sph_fraction_spec& spec = specs[r];
+        binOffset[r] = bins;
+        if (spec.nx <= 0) continue;
+        if (spec.ny <= 0 || spec.reach < 0 || spec.reach > 64 || spec.offset < 0 || !(spec.binWidth > 0.0) ||
+            !(spec.h > 0.0) || (int64_t)spec.nx * spec.ny > (1ll << 28))
+            return fail(ctx, SPH_ERR_VALUE, "A fraction grid needs nodes, a bin width, a smoothing length and a reach of at most 64 bins.");
+        bins += (int64_t)(spec.nx + 2 * spec.reach + 2) * (spec.ny + 2 * spec.reach + 2);
+        nodes = std::max<int64_t>(nodes, spec.offset + (int64_t)spec.nx * spec.ny);
+    }
+    if (bins >= (1ll << 31) - 1) return fail(ctx, SPH_ERR_VALUE, "The fraction grids hold too many bins.");
+    if (nodes == 0) return SPH_OK;
+    const int64_t n = ctx->total;
+    const int64_t tiles = (bins + 1 + kScanTile - 1) / kScanTile;
+    const size_t specBytes = aligned(runs * sizeof(sph_fraction_spec));
+    const size_t offsetBytes = aligned(runs * sizeof(int64_t));
+    const size_t keyBytes = aligned(std::max<int64_t>(n, 1) * sizeof(int32_t));
+    const size_t binBytes = aligned((bins + 1) * sizeof(int32_t));
+    const size_t tileBytes = aligned(std::max<int64_t>(tiles, 1) * sizeof(int32_t));
+    const size_t nodeBytes = aligned(nodes * sizeof(double));
+    status = readBuffers(ctx, specBytes + offsetBytes + 2 * keyBytes + 2 * binBytes + tileBytes + nodeBytes);
+    if (status != SPH_OK) return status;
+    char* base = static_cast<char*>(ctx->readScratch);
+    sph_fraction_spec* dSpecs = reinterpret_cast<sph_fraction_spec*>(base);
+    int64_t* dOffsets = reinterpret_cast<int64_t*>(base + specBytes);
+    int32_t* dKey = reinterpret_cast<int32_t*>(base + specBytes + offsetBytes);
+    int32_t* dMembers = reinterpret_cast<int32_t*>(base + specBytes + offsetBytes + keyBytes);
+    int32_t* dCounts = reinterpret_cast<int32_t*>(base + specBytes + offsetBytes + 2 * keyBytes);
+    int32_t* dStart = reinterpret_cast<int32_t*>(base + specBytes + offsetBytes + 2 * keyBytes + binBytes);
+    int32_t* dTiles = reinterpret_cast<int32_t*>(base + specBytes + offsetBytes + 2 * keyBytes + 2 * binBytes);
+    double* dFraction = reinterpret_cast<double*>(base + specBytes + offsetBytes + 2 * keyBytes + 2 * binBytes + tileBytes);
+    HIP_TRY(hipMemcpyAsync(dSpecs, specs, runs * sizeof(sph_fraction_spec), hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(dOffsets, binOffset.data(), runs * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemsetAsync(dCounts, 0, (bins + 1) * sizeof(int32_t), ctx->stream));
+    if (n > 0) {
+        if (ctx->dims == 2)
+            fractionKeys<2><<<gridFor(n, 256), 256, 0, ctx->stream>>>(ctx->runOf, ctx->state, dSpecs, dOffsets, dKey,
+                                                                       dCounts, n);
+        else
+            fractionKeys<3><<<gridFor(n, 256), 256, 0, ctx->stream>>>(ctx->runOf, ctx->state, dSpecs, dOffsets, dKey,
+                                                                       dCounts, n);
+    }
+    scanTiles<<<(unsigned)tiles, kScanThreads, 0, ctx->stream>>>(dCounts, dStart, dTiles, bins);
+    scanPartials<<<1, kScanThreads, 0, ctx->stream>>>(dTiles, tiles, dStart, bins);
+    scanAdd<<<(unsigned)tiles, kScanThreads, 0, ctx->stream>>>(dStart, dTiles, bins);
+    if (n > 0) fractionScatter<<<gridFor(n, 256), 256, 0, ctx->stream>>>(dKey, dStart, dCounts, dMembers, n);
+    if (bins > 0) fractionOrder<<<gridFor(bins, 256), 256, 0, ctx->stream>>>(dStart, dMembers, bins);
+    for (int32_t r = 0; r < runs; ++r) {
+        const sph_fraction_spec& spec = specs[r];
+        if (spec.nx <= 0) continue;
+        const int64_t count = (int64_t)spec.nx * spec.ny;
+        const unsigned grid = gridFor(count, 256);
+        if (ctx->dims == 2) {
+            if (ctx->precision == 0)
+                fractionNodes<2, float><<<grid, 256, 0, ctx->stream>>>(ctx->state, dSpecs, r, binOffset[r], dStart,
+                                                                        dMembers, dFraction);
+            else
+                fractionNodes<2, double><<<grid, 256, 0, ctx->stream>>>(ctx->state, dSpecs, r, binOffset[r], dStart,
+                                                                         dMembers, dFraction);
+        } else {
+            if (ctx->precision == 0)
+                fractionNodes<3, float><<<grid, 256, 0, ctx->stream>>>(ctx->state, dSpecs, r, binOffset[r], dStart,
+                                                                        dMembers, dFraction);
+            else
+                fractionNodes<3, double><<<grid, 256, 0, ctx->stream>>>(ctx->state, dSpecs, r, binOffset[r], dStart,
+                                                                         dMembers, dFraction);
+        }
+    }
+    status = launchCheck(ctx, "fluid fraction");
+    if (status != SPH_OK) return status;
+    // Only the runs' own node ranges are defined; the host copies those
+    std::vector<double> host(nodes);
+    HIP_TRY(hipMemcpyAsync(host.data(), dFraction, nodes * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    for (int32_t r = 0; r < runs; ++r) {
+        const sph_fraction_spec& spec = specs[r];
+        if (spec.nx <= 0) continue;
+        std::memcpy(fraction + spec.offset, host.data() + spec.offset, (size_t)spec.nx * spec.ny * sizeof(double));
+    }
     return SPH_OK;
 }
 

@@ -271,6 +271,20 @@ class Engine:
                                                landward.ctypes.data_as(ctypes.c_void_p)), 'sph_face_cells')
         return occupancy[:words], overflow, landward
 
+    def fluidFraction(self, specs: list) -> np.ndarray:
+        '''
+
+        The runs' smoothed fluid fractions (sph_fluid_fraction): `specs` holds one
+        _native.FractionSpec per run; returns every run's nodes, each at its offset. Synchronizes.
+
+        '''
+        array = self._readSpecs(_native.FractionSpec, specs)
+        total = max([spec.offset + spec.nx * spec.ny for spec in specs if spec.nx > 0], default = 0)
+        fraction = np.zeros(max(total, 1))
+        self.check(self.library.sph_fluid_fraction(self.handle, array, fraction.ctypes.data_as(ctypes.c_void_p)),
+                   'sph_fluid_fraction')
+        return fraction[:total]
+
     def setLaunch(self, blockSize: int = 256, lanes: int = 0, listCapacity: int = -1) -> None:
         '''
 

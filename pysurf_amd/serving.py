@@ -5,10 +5,11 @@ them: Poisson arrivals, admission under an HBM budget, continuous batching.
 
 A fleet holds the runs of one dimension in one BatchEngine. It steps them one tracking interval
 at a time (0.01 sqrt(d / g) of each run's simulated time, BreakingWave.py:514, 881): each run's
-device clock stops at its next reading, the positions come back once for all runs, and each run's
-breaking tracker reads its own (a 3D run's tracker reads the strip a quarter depth wide along the
-tank's centre line, in x-z, over the bed's section there). A run is done when its breaking point is
-recorded or its time runs out.
+device clock stops at its next reading, the trackers' measures are taken on the device for all runs
+at once (pysurf_amd.readout), and each run's breaking tracker reads its own (a 3D run's tracker
+reads the smoothed surface of the strip a quarter depth wide along the tank's centre line, in x-z,
+over the bed's section there). A run is done when its breaking point is recorded or its time runs
+out.
 
 Membership changes between intervals: finished runs leave and admitted runs join, and the fleet
 rebuilds its engine from the runs' current states, each run's particles in the same slot order and
@@ -39,15 +40,22 @@ def bytesPerParticle(dimensions: int, listCapacity: int) -> int:
 
 
 def trackerOf(case) -> BreakingTracker:
-    '''The run's breaking tracker: face reach one depth, crests offshore of the bed's toe ignored (BreakingWave.py:763-765).'''
+
+    '''
+
+    The run's breaking tracker: face reach one depth, crests offshore of the bed's toe ignored; a 2D
+    run's reads the face in cells (BreakingWave.py:763-765), a 3D run's reads the smoothed surface
+    of its strip over the bed's section there (BreakingWave3D.py:693-694).
+
+    '''
+
     depth = case.parameters.get('depth', 1.0)
     toe = case.parameters.get('toeX', 0.0) * (depth if case.dimensions == 3 else 1.0)
     if case.dimensions == 2:
-        bed = case.boundary.bathymetry
-    else:
-        startY, endY = case.boundary.field.bounds[2:]
-        bed = case.boundary.field.section(0.5 * (startY + endY))
-    return BreakingTracker(bed, case.spacing, faceReach = depth, searchFromX = toe)
+        return BreakingTracker(case.boundary.bathymetry, case.spacing, faceReach = depth, searchFromX = toe)
+    startY, endY = case.boundary.field.bounds[2:]
+    bed = case.boundary.field.section(0.5 * (startY + endY))
+    return BreakingTracker(bed, case.spacing, faceReach = depth, searchFromX = toe, faceMeasure = 'contour')
 
 
 def stripOf(case) -> tuple[float, float]:
@@ -58,9 +66,24 @@ def stripOf(case) -> tuple[float, float]:
     return 0.5 * (startY + endY), 0.125 * case.parameters.get('depth', 1.0)
 
 
-def trackedPositions(case, positions: np.ndarray, isFluid: np.ndarray) -> np.ndarray:
-    '''The fluid positions a run's tracker reads: (x, z); in 3D the centre strip a quarter depth wide.'''
-    return readout.trackedPositions(positions, isFluid, case.dimensions, *stripOf(case))
+def areaPerMass(case) -> float:
+
+    '''
+
+    A particle's area in the smoothed surface per unit mass: in 3D its volume over the strip's width
+    (BreakingWave3D.py:476-480); in 2D its volume where particles have split, and 0 (the spacing
+    squared) where they have not (BreakingWave.py:908).
+
+    '''
+
+    if case.dimensions == 3:
+        return 1.0 / (case.referenceDensity * 2.0 * stripOf(case)[1])
+    return 1.0 / case.referenceDensity if case.particles.lengths is not None else 0.0
+
+
+def readingEntry(index: int, member) -> tuple:
+    '''What the read-out needs of a member (pysurf_amd.readout): its run, tracker, strip and area rule.'''
+    return (index, member.tracker, *stripOf(member.case), areaPerMass(member.case))
 
 
 @dataclass
@@ -110,7 +133,7 @@ class Fleet:
         Steps per graph-replayed advance
     measure : str
         'device': the trackers' measures are taken on the device (pysurf_amd.readout) and only they
-        come back; 'host': every reading downloads the positions and the trackers measure them
+        come back; 'host': every reading downloads the state and the trackers measure it
 
     '''
 
@@ -205,11 +228,11 @@ class Fleet:
                 continue
             reading['members'].append((index, member, member.time, member.steps))
             member.nextTrack += member.interval
+        entries = [readingEntry(index, member) for index, member, _, _ in reading['members']]
         if self.measure == 'device':
-            entries = [(index, member.tracker, *stripOf(member.case)) for index, member, _, _ in reading['members']]
             reading['measures'] = self.readout.measure(self.batch, entries)
         else:
-            reading['positions'] = self.batch.engine.download('position', self.dimensions)
+            reading['measures'] = readout.hostMeasures(self.batch, entries)
         return reading
 
     def reached(self) -> bool:
@@ -223,21 +246,14 @@ class Fleet:
         return ready
 
     def track(self, reading: dict, wall: float) -> list:
-        '''Each member's tracker reads its measures (or positions) of `reading`; returns the members that finished at it.'''
-        offsets = reading['offsets']
+        '''Each member's tracker reads its measures of `reading`; returns the members that finished at it.'''
         finished = []
         for index, member, time, steps in reading['members']:
             if member.done:
                 continue
-            if 'measures' in reading:
-                measures = reading['measures'][index]
-                if measures is not None:
-                    member.tracker.observeMeasures(time, *measures)
-            else:
-                a, b = offsets[index], offsets[index + 1]
-                tracked = trackedPositions(member.case, reading['positions'][a:b], member.case.particles.isFluid)
-                if len(tracked):
-                    member.tracker.update(time, tracked)
+            measures = reading['measures'][index]
+            if measures is not None:
+                member.tracker.observeMeasures(time, measures)
             self.updates += 1
             if member.tracker.breakingFinal or time >= member.endTime:
                 member.done = True

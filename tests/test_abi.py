@@ -44,6 +44,10 @@ def testStructuresMatchTheHeaderLayout():
     assert ctypes.sizeof(_native.FaceSpec) == 96
     assert _native.FaceSpec.bedPoints.offset == 72
     assert _native.FaceSpec.stripCentre.offset == 80
+    assert ctypes.sizeof(_native.FractionSpec) == 136
+    assert _native.FractionSpec.offset.offset == 56
+    assert _native.FractionSpec.h.offset == 64
+    assert _native.FractionSpec.stripCentre.offset == 120
 
 
 def testBadArgumentsFailWithoutADevice():

@@ -139,6 +139,6 @@ def testTheTrackerReadsTheDrawdownAheadOfABreakingCrest():
     assert point.waveHeight == pytest.approx(point.crestHeight + 0.1, abs = 1e-9)
 
 
-def testTheContourMeasureIsRefused():
+def testAnUnknownFaceMeasureIsRefused():
     with pytest.raises(ValueError):
-        BreakingTracker(Bathymetry([0.0, 1.0], [0.0, 0.0], 1.0), s, faceMeasure = 'contour')
+        BreakingTracker(Bathymetry([0.0, 1.0], [0.0, 0.0], 1.0), s, faceMeasure = 'bands')

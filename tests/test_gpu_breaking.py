@@ -81,7 +81,7 @@ def testARunServedInAMixIsTheRunServedAlone():
         mixed = served[k].tracker
         print(f'run {k}: {len(alone.times)} readings alone, {len(mixed.times)} in the mix', flush = True)
         assert served[k].steps > 0 and len(alone.times) > 10
-        for name in ('times', 'crestX', 'crestHeight', 'lean', 'waterline'):
+        for name in ('times', 'crestX', 'crestHeight', 'lean', 'faceSlope', 'waterline'):
             assert np.array_equal(np.asarray(getattr(alone, name)), np.asarray(getattr(mixed, name)), equal_nan = True), \
                 (k, name)
 
