@@ -20,6 +20,8 @@ Beside the headline the line carries:
   - the 3D run: 12.4 M particles over a turned Gaussian ridge with a split crest zone;
   - the serving stream: Poisson arrivals, HBM admission, continuous batching, p50/p99 time to
     the breaking point;
+  - Grilli's 1:35 case from the exact solitary wave at 64 particles per depth, stepped to the jet's
+    touchdown and read by the smoothed-surface tracker, beside the reference's recorded run;
   - the dominant kernel's roofline, from HIP events on the engine's stream around each launch of a
     stream-launched pass over the same batch right after the timed region;
   - the float64 CPU restatement (oracle/) timed on the host's cores on a bounded sample of the
@@ -103,6 +105,7 @@ def parse():
     parser.add_argument('--serving-requests', type = int, default = 24, help = 'requests of the serving stream')
     parser.add_argument('--serving-wall', type = float, default = 150.0, help = 'wall seconds the serving stream may take')
     parser.add_argument('--no-serving', action = 'store_true', help = 'skip the serving stream')
+    parser.add_argument('--no-gate', action = 'store_true', help = "skip Grilli's 1:35 case from the exact start")
     parser.add_argument('--cpu-seconds', type = float, default = 15.0)
     parser.add_argument('--no-cpu', action = 'store_true', help = 'skip the CPU baseline')
     parser.add_argument('--copy-calibration', type = int, default = 0, metavar = 'MIB',
@@ -316,6 +319,50 @@ def servingRun(args) -> dict:
             'particles_max': max(case.count for case in cases), 'particles_min': min(case.count for case in cases)}
 
 
+def gateRun(args) -> dict:
+
+    '''
+
+    Grilli's 1:35 case from the exact solitary wave at 64 particles per depth (128,947 fluid
+    particles, Wendland kernel at h = 3 spacings, bed-fitted columns), float32 pairs, read by the
+    smoothed-surface tracker every 0.01 sqrt(d/g) until the jet's touchdown fixes the breaking
+    point; reported beside the reference's own GPU run of the same case
+    (validation/gpuSph/results/grilliBreaking_64.json: 693 s on an RTX 2070 Super).
+
+    '''
+
+    from pysurf_amd.scenarios import exactSolitaryOnSlope
+    from pysurf_amd.serving import Fleet, Member
+    case = exactSolitaryOnSlope(64)
+    fleet = Fleet(2, 'float32', lanes = 8, listCapacity = 160, chunk = 16)
+    member = Member(key = 0, case = case, endTime = 12.0)
+    start = time.perf_counter()
+    fleet.rebuild([], [member])
+    fleet.launch()
+    readings = 0
+    while not member.done:
+        reading = fleet.collect()
+        if not member.done:
+            fleet.launch()
+        fleet.track(reading, time.perf_counter())
+        readings += 1
+        if readings % 1000 == 0:
+            log(f'gate: {readings} readings, t = {member.time:.2f} s')
+    wall = time.perf_counter() - start
+    tracker, point = member.tracker, member.tracker.breakingPoint
+    result = {'case': 'exact solitary wave H/d = 0.3 on a 1:35 beach, 64 ppd', 'fluid_particles': case.fluidCount,
+              'steps': member.steps, 'readings': readings, 'wall_s': wall, 'touchdown_s': tracker.touchdownTime,
+              'breaking_s': point.time if point else None, 'Hb_over_h0': point.crestHeight if point else None,
+              'hb_over_h0': point.depth if point else None,
+              'reference_record': {'breaking_s': 8.7945, 'Hb_over_h0': 0.4963, 'hb_over_h0': 0.3159,
+                                   'touchdown_s': 9.3740, 'wall_s': 693.1},
+              'grilli_1997': {'Hb_over_h0': 0.476, 'hb_over_h0': 0.345},
+              'fluid_updates_per_s': case.fluidCount * member.steps / wall}
+    log(f'gate: breaking at {result["breaking_s"]:.4f} s, Hb/h0 {result["Hb_over_h0"]:.4f}, hb/h0 '
+        f'{result["hb_over_h0"]:.4f}, touchdown {tracker.touchdownTime:.4f} s; {member.steps} steps in {wall:.1f} s')
+    return result
+
+
 def cpuModel() -> str:
     try:
         with open('/proc/cpuinfo') as f:
@@ -432,6 +479,7 @@ def main() -> int:
     single = None if args.no_single else singleRun(args)
     ridge = None if args.no_ridge else ridgeRun(args)
     serving = None if args.no_serving else servingRun(args)
+    gate = None if args.no_gate else gateRun(args)
     cpu = None if args.no_cpu else cpuBaseline(cases[:2], args.cpu_seconds)
 
     sizes = np.array([case.count for case in cases])
@@ -466,6 +514,7 @@ def main() -> int:
         'single_run': single,
         'ridge_3d': ridge,
         'serving': serving,
+        'grilli_gate': gate,
         'cpu_baseline': cpu,
     }
     print(json.dumps(line))

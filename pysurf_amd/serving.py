@@ -44,15 +44,19 @@ def trackerOf(case) -> BreakingTracker:
     '''
 
     The run's breaking tracker: face reach one depth, crests offshore of the bed's toe ignored; a 2D
-    run's reads the face in cells (BreakingWave.py:763-765), a 3D run's reads the smoothed surface
-    of its strip over the bed's section there (BreakingWave3D.py:693-694).
+    run's reads the face in cells, or the smoothed surface from the exact start
+    (BreakingWave.py:763-765), a 3D run's the smoothed surface of its strip over the bed's section
+    there (BreakingWave3D.py:693-694).
 
     '''
 
     depth = case.parameters.get('depth', 1.0)
     toe = case.parameters.get('toeX', 0.0) * (depth if case.dimensions == 3 else 1.0)
     if case.dimensions == 2:
-        return BreakingTracker(case.boundary.bathymetry, case.spacing, faceReach = depth, searchFromX = toe)
+        # The exact start reads the smoothed surface (BreakingWave.py:765)
+        return BreakingTracker(case.boundary.bathymetry, case.spacing, faceReach = depth, searchFromX = toe,
+                               faceMeasure = case.parameters.get('faceMeasure', 'cells'),
+                               faceArc = case.parameters.get('faceArc', 2.0))
     startY, endY = case.boundary.field.bounds[2:]
     bed = case.boundary.field.section(0.5 * (startY + endY))
     return BreakingTracker(bed, case.spacing, faceReach = depth, searchFromX = toe, faceMeasure = 'contour')

@@ -1,0 +1,59 @@
+'''
+
+Grilli's 1:35 case from the exact solitary wave (pysurf_amd.scenarios.exactSolitaryOnSlope, the
+reference's BreakingWave 'deltaSph' start: bed-fitted columns, Wendland kernel at h = 3 spacings),
+stepped on the GPU in float32 pairs at 64 particles per depth and read by the smoothed-surface
+tracker until the jet's touchdown fixes the breaking point. It must break where the reference's own
+GPU run recorded it (tests/golden/grilli_breaking_64.json, from tests/golden/make_grilli_record.py):
+the time, crest height and depths within 1% (the recorded run's own scatter under rounding-level
+changes is a few tenths of a percent on this measure), and within the reference's criteria against
+Grilli et al. (1997): Hb/h0 within 10%, hb/h0 within 15%.
+
+'''
+
+import json
+import os
+import time
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'golden', 'grilli_breaking_64.json')
+
+
+@pytest.mark.timeout(600)
+def testTheGrilliCaseBreaksWhereTheReferenceRecordedIt():
+    from pysurf_amd.scenarios import exactSolitaryOnSlope
+    from pysurf_amd.serving import Fleet, Member
+    golden = json.load(open(GOLDEN))
+    recorded, grilli = golden['recorded'], golden['grilli1997']
+    case = exactSolitaryOnSlope(64)
+    assert case.fluidCount == recorded['fluidParticles']
+    fleet = Fleet(2, 'float32', lanes = 8, listCapacity = 160, chunk = 16)
+    member = Member(key = 0, case = case, endTime = 12.0)
+    fleet.rebuild([], [member])
+    fleet.launch()
+    started, readings = time.perf_counter(), 0
+    while not member.done:
+        reading = fleet.collect()
+        if not member.done:
+            fleet.launch()
+        fleet.track(reading, time.perf_counter())
+        readings += 1
+        if readings % 500 == 0:
+            print(f'{readings} readings: t = {member.time:.3f} s, {member.steps} steps, '
+                  f'{time.perf_counter() - started:.1f} s', flush = True)
+    tracker = member.tracker
+    point = tracker.breakingPoint
+    assert tracker.breakingFinal and point is not None
+    print(f'breaking at {point.time:.4f} s: Hb/h0 {point.crestHeight:.4f}, hb/h0 {point.depth:.4f}, face depth '
+          f'{point.faceDepth:.4f}; touchdown {tracker.touchdownTime:.4f} s; {member.steps} steps in '
+          f'{time.perf_counter() - started:.1f} s', flush = True)
+    assert point.time == pytest.approx(recorded['breakingTime'], rel = 0.01)
+    assert tracker.touchdownTime == pytest.approx(recorded['touchdownTime'], rel = 0.01)
+    assert point.crestHeight == pytest.approx(recorded['breakingHeightOverDepth'], rel = 0.01)
+    assert point.depth == pytest.approx(recorded['breakingDepthOverDepth'], rel = 0.01)
+    assert point.faceDepth == pytest.approx(recorded['breakingFaceDepthOverDepth'], rel = 0.01)
+    assert abs(point.crestHeight / grilli['breakingHeightOverDepth'] - 1.0) <= 0.10
+    assert abs(point.depth / grilli['breakingDepthOverDepth'] - 1.0) <= 0.15
