@@ -1,9 +1,9 @@
 '''
 
 The exact solitary wave of fully nonlinear potential flow and the δ-SPH start laid from it, after
-pySurf's waveTheory.py:353-651 and waveMakers.py:328-442: the wave by Clamond and Dutykh's (2013)
-method, the flow under it from its conformal map, and the fluid particles, their velocities and
-pressures, and the walls' pressures a run starts with.
+pySurf's waveTheory.py:353-651 and waveMakers.py:328-442, 463-590: the wave by Clamond and Dutykh's
+(2013) method, the flow under it from its conformal map, and the fluid particles, their velocities
+and pressures, and the walls' pressures a run starts with, in 2D and over a 3D bed.
 
 The method (Clamond & Dutykh, "Fast accurate computation of the fully nonlinear solitary surface
 gravity waves", Computers & Fluids 84, 2013): a conformal map takes the water, in units of the
@@ -315,6 +315,77 @@ def exactSolitaryBoundaryPressure(bathymetry, positions: np.ndarray, height: flo
     return np.where(wet, np.maximum(pressure, 0.0) + g * (inner[:, 1] - positions[:, 1]), 0.0)
 
 
+def exactSolitaryWaveFluid3D(field, height: float, depth: float, crestX: float, spacing: float,
+                             floor: float | None = None, bedFitted: bool = False, jitter: float = 0.0,
+                             seed: int = 1) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+
+    '''
+
+    Fluid particles under the exact wave, its crest straight along y, over a 3D bed (a HeightField),
+    with their velocities and pressures over density (waveMakers.py:463-568): the flat-bed wave of
+    `depth` in every (x, z) plane, no spanwise velocity. Points inside the field's bounds from half
+    a spacing above the bed up to the surface, on the lattice counted from `floor` (the bed's lowest
+    point by default) or in bed-fitted columns; with `jitter`, moved by up to that many spacings
+    with `seed`, and the flow taken at each point's place.
+
+    '''
+
+    s, level = spacing, field.stillWaterLevel
+    floor = field.lowestElevation() if floor is None else floor
+    startX, endX, startY, endY = field.bounds
+    top = level + height + s
+    surfaceAt = lambda x: exactSurfaceElevation(x, height, depth, crestX)
+    if jitter > 0.0:
+        if bedFitted:
+            points = bedFittedPoints([startX, startY, floor], [endX, endY, top], s, field.elevation)
+        else:
+            points = latticePoints([startX, startY, floor], [endX, endY, top], s)
+            points = points[points[:, 2] >= field.elevation(points[:, 0], points[:, 1]) + (0.5 - 1e-9) * s]
+        points = points[points[:, 2] < level + surfaceAt(points[:, 0])]
+        points = jitterPoints(points, s, jitter, seed, [startX, startY], [endX, endY],
+                              lambda x, y: np.maximum(field.elevation(x, y), floor), lambda x, y: level + surfaceAt(x))
+        velocities, pressures, _ = exactSolitaryFlow(points[:, [0, 2]], height, depth, crestX, level)
+        flow = np.column_stack([velocities[:, 0], np.zeros(len(points)), velocities[:, 1]])
+        return points, flow, np.maximum(pressures, 0.0)
+    if bedFitted:
+        # The flow once per distinct (x, z): rows across a bed flat in y share theirs
+        points = bedFittedPoints([startX, startY, floor], [endX, endY, top], s, field.elevation)
+        planar, inverse = np.unique(points[:, [0, 2]], axis = 0, return_inverse = True)
+        velocities, pressures, surface = exactSolitaryFlow(planar, height, depth, crestX, level)
+        inverse = inverse.reshape(-1)
+        wet = points[:, 2] < level + surface[inverse]
+        index = inverse[wet]
+        flow = np.column_stack([velocities[index, 0], np.zeros(len(index)), velocities[index, 1]])
+        return points[wet], flow, np.maximum(pressures[index], 0.0)
+    # The flow on one (x, z) plane of the lattice, laid at every row across
+    plane = latticePoints([startX, floor], [endX, top], s)
+    velocities, pressures, surface = exactSolitaryFlow(plane, height, depth, crestX, level)
+    wet = plane[:, 1] < level + surface
+    plane, velocities, pressures = plane[wet], velocities[wet], np.maximum(pressures[wet], 0.0)
+    rows = np.arange(startY + 0.5 * s, endY, s)
+    x, z = np.tile(plane[:, 0], rows.size), np.tile(plane[:, 1], rows.size)
+    y = np.repeat(rows, len(plane))
+    keep = z >= field.elevation(x, y) + (0.5 - 1e-9) * s
+    index = np.tile(np.arange(len(plane)), rows.size)[keep]
+    flow = np.column_stack([velocities[index, 0], np.zeros(len(index)), velocities[index, 1]])
+    return np.column_stack([x[keep], y[keep], z[keep]]), flow, pressures[index]
+
+
+def exactSolitaryBoundaryPressure3D(field, positions: np.ndarray, height: float, depth: float, crestX: float,
+                                    spacing: float, g: float = gravity) -> np.ndarray:
+    '''A 3D tank's wall particles' starting pressure over density under the exact wave, by the 2D rule, in from the side walls too (waveMakers.py:570-590).'''
+    startX, endX, startY, endY = field.bounds
+    level = field.stillWaterLevel
+    positions = np.asarray(positions, dtype = float)
+    innerX = np.clip(positions[:, 0], startX + 0.5 * spacing, endX - 0.5 * spacing)
+    innerY = np.clip(positions[:, 1], startY + 0.5 * spacing, endY - 0.5 * spacing)
+    innerZ = np.maximum(positions[:, 2], field.elevation(innerX, innerY) + 0.5 * spacing)
+    _, pressure, surface = exactSolitaryFlow(np.column_stack([innerX, innerZ]), height, depth, crestX, level, g)
+    wet = innerZ < level + surface
+    return np.where(wet, np.maximum(pressure, 0.0) + g * (innerZ - positions[:, 2]), 0.0)
+
+
 __all__ = ['ExactSolitaryWave', 'babenkoSolitaryWave', 'exactSolitaryWave', 'exactSolitaryWaveSpeed',
            'exactSolitarySurface', 'exactSolitaryInterior', 'exactSolitaryFlow', 'exactSurfaceElevation',
-           'bedFittedPoints', 'exactSolitaryWaveFluid', 'exactSolitaryBoundaryPressure', 'latticePoints']
+           'bedFittedPoints', 'exactSolitaryWaveFluid', 'exactSolitaryBoundaryPressure', 'exactSolitaryWaveFluid3D',
+           'exactSolitaryBoundaryPressure3D']

@@ -176,7 +176,8 @@ def ridge3D(particlesPerDepth: int = 32, depth: float = 1.0, waveHeight: float =
             ridgeHeight: float = 0.5, ridgeWidth: float = 2.0, slope: float = 1.0 / 15.0, toeX: float = 6.0,
             length: float = 20.0, width: float = 10.0, obliquity: float = 25.0, zoneHalfWidth: float | None = 0.75,
             lengthRatio: float = 1.0, jitter: float = 1e-3, seed: int = 0, smoothingRatio: float = 1.3,
-            kernelType: str = 'wendlandC2', soundSpeedFactor: float = 10.0, name: str | None = None) -> Case:
+            kernelType: str = 'wendlandC2', soundSpeedFactor: float = 10.0, start: str = 'first-order',
+            name: str | None = None) -> Case:
 
     '''
 
@@ -185,7 +186,10 @@ def ridge3D(particlesPerDepth: int = 32, depth: float = 1.0, waveHeight: float =
     shoreward from `toeX` at `slope`, its axis turned `obliquity` degrees from the wave's direction)
     in a walled tank `length` x `width` depths. Fluid within `zoneHalfWidth` depths of the crest is
     split once into 8 daughters (keeping h with `lengthRatio` 1); None leaves it whole. Lengths are
-    in units of `depth`.
+    in units of `depth`. `start` 'first-order' lays the sech^2 wave with hydrostatic densities;
+    'exact' lays the exact wave in bed-fitted columns (jittered by `jitter` spacings), every
+    particle's density from its pressure and each wall particle's from the water just inside it
+    (BreakingWave3D.py:660-690).
 
     '''
 
@@ -195,8 +199,17 @@ def ridge3D(particlesPerDepth: int = 32, depth: float = 1.0, waveHeight: float =
     field = HeightField.gaussianRidge(depth = d, height = ridgeHeight, width = ridgeWidth, slope = slope, toeX = toeX,
                                       length = length, halfWidth = 0.5 * width, direction = obliquity)
     boundary = HeightFieldBoundary(field, s, nLayers = layers, wallTop = wallTop)
-    fluid, velocities = solitaryWaveFluid3D(field, waveHeight, crestX * d, s)
     startX, endX, startY, endY = field.bounds
+    if start == 'exact':
+        return exactRidge(field, boundary, d, s, waveHeight, crestX, layers, wallTop, zoneHalfWidth, lengthRatio, jitter,
+                          seed, smoothingRatio, kernelType, soundSpeedFactor, name,
+                          dict(particlesPerDepth = particlesPerDepth, waveHeight = waveHeight, crestX = crestX,
+                               ridgeHeight = ridgeHeight, ridgeWidth = ridgeWidth, slope = slope, toeX = toeX,
+                               length = length, width = width, obliquity = obliquity, zoneHalfWidth = zoneHalfWidth,
+                               lengthRatio = lengthRatio, seed = seed, depth = d, start = 'exact'))
+    if start != 'first-order':
+        raise ValueError(f"The start is 'first-order' or 'exact', not {start!r}.")
+    fluid, velocities = solitaryWaveFluid3D(field, waveHeight, crestX * d, s)
     if jitter > 0.0:
         surface = lambda x, y: d + solitaryWaveProfile(x, waveHeight, d, crestX * d)
         fluid = jitterPoints(fluid, s, jitter, seed, [startX, startY], [endX, endY], field.elevation, surface)
@@ -219,6 +232,43 @@ def ridge3D(particlesPerDepth: int = 32, depth: float = 1.0, waveHeight: float =
                                   ridgeHeight = ridgeHeight, ridgeWidth = ridgeWidth, slope = slope, toeX = toeX,
                                   length = length, width = width, obliquity = obliquity,
                                   zoneHalfWidth = zoneHalfWidth, lengthRatio = lengthRatio, seed = seed, depth = d))
+
+
+def exactRidge(field, boundary, d: float, s: float, waveHeight: float, crestX: float, layers: int, wallTop: float,
+               zoneHalfWidth, lengthRatio: float, jitter: float, seed: int, smoothingRatio: float, kernelType: str,
+               soundSpeedFactor: float, name, parameters: dict) -> Case:
+    '''ridge3D's exact start: the exact wave's flow and pressure in bed-fitted columns, the walls' pressures from the water inside them.'''
+    from pysurf_amd.exact_wave import exactSolitaryBoundaryPressure3D, exactSolitaryWaveFluid3D
+    from pysurf_amd.waves import solitaryWaveSpeed
+    rho = freshwaterDensity
+    fluid, velocities, pressures = exactSolitaryWaveFluid3D(field, waveHeight, d, crestX * d, s, bedFitted = True,
+                                                            jitter = jitter, seed = seed)
+    particles = ParticleSystem.assemble(fluid, boundary.positions(), s, rho, velocities)
+    fluidPressure = np.zeros(particles.nParticles)
+    fluidPressure[particles.isFluid] = pressures
+    h = smoothingRatio * s
+    if zoneHalfWidth is not None:
+        zone = np.abs(particles.positions[:, 0] - crestX * d) < zoneHalfWidth * d
+        count = particles.nParticles
+        splitParticles(particles, zone, rho, h, lengthRatio)
+        # A daughter carries its mother's pressure: the first takes the mother's slot, the others
+        # follow mother by mother
+        mothers = np.flatnonzero(zone & particles.isFluid[:count])
+        daughters = (particles.nParticles - count) // max(len(mothers), 1)
+        fluidPressure = np.concatenate([fluidPressure, np.repeat(fluidPressure[mothers], daughters)])
+    c0 = soundSpeedFactor * solitaryWaveSpeed(waveHeight, d)
+    walls = exactSolitaryBoundaryPressure3D(field, particles.positions[~particles.isFluid], waveHeight, d, crestX * d, s,
+                                            gravity)
+    particles.densities[particles.isFluid] = taitDensity(rho * fluidPressure[particles.isFluid], rho, c0)
+    particles.densities[~particles.isFluid] = taitDensity(rho * walls, rho, c0)
+    startX, endX, startY, endY = field.bounds
+    margin = (layers + 1.0) * s
+    return Case(name = name or f'exact-ridge3d-{parameters["particlesPerDepth"]}ppd', particles = particles,
+                boundary = boundary, dimensions = 3, spacing = s, smoothingLength = h,
+                domainMin = np.array([startX - margin, startY - margin, field.lowestElevation() - (layers + 2.0) * s]),
+                domainMax = np.array([endX + margin, endY + margin, wallTop]), kernelType = kernelType,
+                soundSpeed = c0, gravity = np.array([0.0, 0.0, -gravity]), densityFloor = 0.9 * rho,
+                parameters = parameters)
 
 
 def servingRidge(seed: int) -> Case:

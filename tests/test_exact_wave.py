@@ -173,3 +173,70 @@ def testTheGrilliStartHoldsTheReferencesParticles():
     fluid = case.particles.isFluid
     # Every fluid particle's density carries the wave's pressure, the crest's water at rest density or above
     assert case.particles.densities[fluid].min() >= case.referenceDensity
+
+
+def flatField():
+    from pysurf_amd.bathymetry import HeightField
+    return HeightField.fromFunction(lambda x, y: np.zeros_like(x), (0.0, 8.0, -0.25, 0.25), 0.125, stillWaterLevel = 1.0)
+
+
+@pytest.mark.parametrize('bedFitted', [False, True])
+def testOverAFlatBedThe3DWaveIsThe2DWaveLaidAcross(bedFitted):
+    from pysurf_amd.exact_wave import exactSolitaryWaveFluid3D
+    spacing = 1.0 / 16.0
+    positions, velocities, pressures = exactSolitaryWaveFluid3D(flatField(), 0.3, 1.0, 3.0, spacing, bedFitted = bedFitted)
+    plane, planeVelocities, planePressures = exactSolitaryWaveFluid(Bathymetry([0.0, 8.0], [0.0, 0.0], 1.0), 0.3, 3.0,
+                                                                    spacing)
+    rows = np.unique(positions[:, 1])
+    assert rows == pytest.approx(np.arange(-0.25 + 0.5 * spacing, 0.25, spacing))
+    order = lambda points: np.lexsort(points.T[::-1])
+    expected = order(plane)
+    for y in rows:
+        row = np.flatnonzero(positions[:, 1] == y)
+        mine = row[order(positions[row][:, [0, 2]])]
+        assert positions[mine][:, [0, 2]] == pytest.approx(plane[expected], abs = 1e-12)
+        assert velocities[mine][:, [0, 2]] == pytest.approx(planeVelocities[expected], abs = 1e-12)
+        assert np.all(velocities[mine][:, 1] == 0.0)
+        assert pressures[mine] == pytest.approx(planePressures[expected], abs = 1e-12)
+
+
+def testAJittered3DWaveCarriesTheFlowWhereItLies():
+    from pysurf_amd.exact_wave import exactSolitaryWaveFluid3D
+    spacing = 1.0 / 16.0
+    field = flatField()
+    laid, _, _ = exactSolitaryWaveFluid3D(field, 0.3, 1.0, 3.0, spacing, bedFitted = True)
+    positions, velocities, pressures = exactSolitaryWaveFluid3D(field, 0.3, 1.0, 3.0, spacing, bedFitted = True,
+                                                                jitter = 0.15, seed = 2)
+    flow, pressure, surface = exactSolitaryFlow(positions[:, [0, 2]], 0.3, 1.0, 3.0, 1.0)
+    assert len(positions) == len(laid)
+    assert np.all(positions[:, 2] >= 0.5 * spacing - 1e-12) and np.all(positions[:, 2] < 1.0 + surface)
+    assert np.all(np.abs(positions[:, 1]) <= 0.25)
+    assert velocities[:, [0, 2]] == pytest.approx(flow, abs = 1e-12) and np.all(velocities[:, 1] == 0.0)
+    assert pressures == pytest.approx(np.maximum(pressure, 0.0), abs = 1e-12)
+
+
+def testThe3DWallsFollowThe2DRuleInFromTheSides():
+    from pysurf_amd.exact_wave import exactSolitaryBoundaryPressure3D
+    spacing = 1.0 / 16.0
+    walls = np.array([[3.0, 0.6, 0.4], [3.0, -0.1, -1.5 * spacing], [-0.5 * spacing, 0.0, 0.5]])
+    pressure3D = exactSolitaryBoundaryPressure3D(flatField(), walls, 0.3, 1.0, 3.0, spacing)
+    pressure2D = exactSolitaryBoundaryPressure(Bathymetry([0.0, 8.0], [0.0, 0.0], 1.0), walls[:, [0, 2]], 0.3, 3.0,
+                                               spacing)
+    assert pressure3D == pytest.approx(pressure2D, rel = 1e-12)
+
+
+def testTheExactRidgeCarriesItsPressureIntoTheSplitZone():
+    from pysurf_amd.scenarios import ridge3D
+    case = ridge3D(particlesPerDepth = 8, length = 6.0, width = 1.5, toeX = 2.0, crestX = 2.5, ridgeWidth = 0.6,
+                   zoneHalfWidth = 0.4, lengthRatio = 0.5, waveHeight = 0.3, start = 'exact', jitter = 0.0)
+    whole = ridge3D(particlesPerDepth = 8, length = 6.0, width = 1.5, toeX = 2.0, crestX = 2.5, ridgeWidth = 0.6,
+                    zoneHalfWidth = None, waveHeight = 0.3, start = 'exact', jitter = 0.0)
+    p, q = case.particles, whole.particles
+    count = q.nParticles
+    mothers = np.flatnonzero(q.isFluid & (np.abs(q.positions[:, 0] - 2.5) < 0.4))
+    assert p.nParticles == count + 7 * len(mothers) and len(mothers) > 0
+    # The first daughter keeps its mother's slot and density; the other seven follow, mother by mother
+    assert np.array_equal(p.densities[:count], q.densities)
+    assert np.array_equal(p.densities[count:], np.repeat(q.densities[mothers], 7))
+    with pytest.raises(ValueError):
+        ridge3D(particlesPerDepth = 8, length = 6.0, width = 1.5, start = 'serre')

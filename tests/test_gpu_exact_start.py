@@ -57,3 +57,40 @@ def testTheGrilliCaseBreaksWhereTheReferenceRecordedIt():
     assert point.faceDepth == pytest.approx(recorded['breakingFaceDepthOverDepth'], rel = 0.01)
     assert abs(point.crestHeight / grilli['breakingHeightOverDepth'] - 1.0) <= 0.10
     assert abs(point.depth / grilli['breakingDepthOverDepth'] - 1.0) <= 0.15
+
+
+def testAnExactWaveKeepsItsFormAndSpeedIn3D():
+    # The exact wave of H/d = 0.3 over the flat part of a ridge tank, stepped 0.3 s on the GPU:
+    # along the tank's centre strip its highest column stays within half a spacing (the profile's
+    # resolution) and 3% of H, and the centroid of its upper half travels at the exact celerity
+    # within 3%
+    import numpy as np
+    from pysurf_amd.batch import BatchEngine
+    from pysurf_amd.exact_wave import exactSolitaryWaveSpeed
+    from pysurf_amd.readout import hostMeasures
+    from pysurf_amd.scenarios import ridge3D
+    from pysurf_amd.serving import Member, readingEntry
+    case = ridge3D(particlesPerDepth = 16, length = 8.0, width = 1.0, toeX = 5.5, crestX = 2.5, ridgeWidth = 0.6,
+                   zoneHalfWidth = None, waveHeight = 0.3, start = 'exact', jitter = 0.0)
+    batch = BatchEngine([case], 'float32', lanes = 4, listCapacity = 256)
+    batch.setClocks(untilTime = 0.3)
+    entry = readingEntry(0, Member(key = 0, case = case, endTime = 1.0))
+
+    def wave() -> tuple:
+        measures = hostMeasures(batch, [entry])[0]
+        excess = np.nan_to_num(measures['elevation'] - 1.0, nan = 0.0)
+        # The wave's upper half, symmetric about its crest, clear of the disturbance the offshore
+        # wall sends after it
+        upper = np.where(excess > 0.5 * excess.max(), excess, 0.0)
+        return float(excess.max()), float(np.sum(measures['centers'] * upper) / np.sum(upper))
+
+    _, startCentroid = wave()
+    while batch.clocks()['time'][0] < 0.3:
+        batch.advance(64)
+    elapsed = float(batch.clocks()['time'][0])
+    height, centroid = wave()
+    travelled, expected = centroid - startCentroid, exactSolitaryWaveSpeed(0.3, 1.0) * elapsed
+    print(f'after {elapsed:.3f} s: crest {height:.4f} high, centroid moved {travelled:.4f}, {expected:.4f} expected',
+          flush = True)
+    assert abs(height - 0.3) <= 0.5 * case.spacing + 0.03 * 0.3
+    assert travelled == pytest.approx(expected, rel = 0.03)
