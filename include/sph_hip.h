@@ -217,6 +217,31 @@ typedef struct {
  * order within a bin, so the result does not depend on the run's neighbours or the launch. */
 int sph_fluid_fraction(sph_context* ctx, const sph_fraction_spec* specs, double* fraction);
 
+/* Splitting fluid particles into 2^dims daughters on the device (splitParticles,
+ * sphRefinement.py:135-186): a square (cube) s/2 across around each mother, s = (m / rho0)^(1/dims);
+ * each daughter takes m / 2^dims, the mother's length times lengthRatio and her velocity, density,
+ * pressure and acceleration; the first takes the mother's slot, the others follow the run's own
+ * particles, mother by mother in slot order. Two calls, with the caller's new arrays between them:
+ *
+ * sph_split_count marks the particles of each run with `active` that are fluid, lowX <= x < highX,
+ * of mass >= minMass and, when `mask` (a device array of one byte per slot) is given, set there;
+ * chosen[r] receives run r's count. Synchronizes.
+ *
+ * sph_split_apply writes the split state into `into` (its stride at least the new total): run r
+ * starts at the sum of the runs before it, each grown by chosen (2^dims - 1); the runs must lie
+ * packed from slot 0, and the context's state stays bound and unchanged. The caller then
+ * configures the grown runs (sph_configure) and binds `into` (sph_bind), and sets the runs'
+ * clocks again. No other call may come between the two. */
+typedef struct {
+    int32_t active;
+    int32_t pad;
+    double lowX, highX;
+    double minMass;
+} sph_split_spec;
+
+int sph_split_count(sph_context* ctx, const sph_split_spec* specs, const uint8_t* mask, int64_t* chosen);
+int sph_split_apply(sph_context* ctx, const sph_state* into, double referenceDensity, double lengthRatio);
+
 int sph_synchronize(sph_context* ctx);
 
 /* Per-kernel timing with HIP events recorded on the context's stream around every launch of a

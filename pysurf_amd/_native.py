@@ -81,6 +81,12 @@ class FractionSpec(ctypes.Structure):
                 ('stripHalfWidth', ctypes.c_double)]
 
 
+class SplitSpec(ctypes.Structure):
+    '''sph_split_spec'''
+    _fields_ = [('active', ctypes.c_int32), ('pad', ctypes.c_int32), ('lowX', ctypes.c_double), ('highX', ctypes.c_double),
+                ('minMass', ctypes.c_double)]
+
+
 # Every entry point include/sph_hip.h declares, with its argument types
 signatures = {
     'sph_create': [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p],
@@ -109,6 +115,8 @@ signatures = {
     'sph_surface_profile': [ctypes.c_void_p, ctypes.POINTER(ProfileSpec), ctypes.c_void_p],
     'sph_face_cells': [ctypes.c_void_p, ctypes.POINTER(FaceSpec), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
     'sph_fluid_fraction': [ctypes.c_void_p, ctypes.POINTER(FractionSpec), ctypes.c_void_p],
+    'sph_split_count': [ctypes.c_void_p, ctypes.POINTER(SplitSpec), ctypes.c_void_p, ctypes.c_void_p],
+    'sph_split_apply': [ctypes.c_void_p, ctypes.POINTER(State), ctypes.c_double, ctypes.c_double],
     'sph_set_launch': [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32],
     'sph_profile': [ctypes.c_void_p, ctypes.c_int32],
     'sph_kernel_times': [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32],

@@ -99,6 +99,7 @@ class BatchEngine:
             self.offsets.append(first)
             first += case.count
         self.offsets.append(first)
+        self.fluidCounts = [case.fluidCount for case in self.cases]
         self.engine.configure(runs)
         self.engine.setLaunch(blockSize, lanes, listCapacity)
         self.identity = None
@@ -114,7 +115,7 @@ class BatchEngine:
 
     @property
     def fluidCount(self) -> int:
-        return sum(case.fluidCount for case in self.cases)
+        return sum(self.fluidCounts)
 
     def load(self) -> None:
         '''Copy every run's particle state into the engine, in the cases' own order.'''
@@ -182,6 +183,44 @@ class BatchEngine:
         ordered[identity] = values[a:b]
         return ordered
 
+    def split(self, zones: list, lengthRatio: float = 1.0, minMass = 0.0, mask = None) -> np.ndarray:
+
+        '''
+
+        Split, on the device, each run's fluid particles with x in its zone (`zones`: one
+        (lowX, highX) or None per run) and mass at least `minMass` (one value, or one per run; the
+        cases' unsplit mass times 0.75 takes only particles that have not split), into 2^dims
+        daughters whose smoothing length is `lengthRatio` their mother's (splitParticles,
+        sphRefinement.py:135-186). Each run grows in place: its own particles keep their slots
+        relative to its start, the other daughters follow them, mother by mother. The clocks carry
+        over. Returns how many particles of each run split. The batch must be in slot order (not
+        sorted).
+
+        '''
+
+        if self.identity is not None:
+            raise ValueError('Split a batch before sorting it: its slots must hold the cases\' own order.')
+        runs = len(self.cases)
+        if len(zones) != runs:
+            raise ValueError(f'The split needs one zone (or None) per run, {runs} of them.')
+        masses = np.broadcast_to(np.asarray(minMass, dtype = float), (runs,))
+        specs = []
+        for zone, least in zip(zones, masses):
+            spec = _native.SplitSpec()
+            if zone is not None:
+                spec.active, spec.lowX, spec.highX, spec.minMass = 1, float(zone[0]), float(zone[1]), float(least)
+            specs.append(spec)
+        clock = self.clocks()
+        chosen = self.engine.split(specs, self.cases[0].referenceDensity, lengthRatio, mask)
+        if chosen.any():
+            grown = chosen * (2**self.dimensions - 1)
+            counts = np.diff(self.offsets) + grown
+            self.offsets = [0] + np.cumsum(counts).astype(int).tolist()
+            self.fluidCounts = [count + int(extra) for count, extra in zip(self.fluidCounts, grown)]
+            for r in range(runs):
+                self.setClock(r, float(clock['time'][r]), self.until[r], int(clock['steps'][r]), self.maxSteps[r])
+        return chosen
+
     def globalSums(self) -> np.ndarray:
         '''Every run's float64 sums over its fluid, shape (runs, 7), columns as _native.globalSumNames.'''
         return self.engine.globalSums(len(self.cases))
@@ -196,4 +235,9 @@ class BatchEngine:
         particles.densities = self._original(engine.download('density', None, b), a, b)
         particles.pressures = self._original(engine.download('pressure', None, b), a, b)
         particles.accelerations = self._original(engine.download('acceleration', d, b), a, b)
+        if b - a != self.cases[index].count:
+            # The run has split on the device: its masses, lengths and kinds come from there too
+            particles.masses = self._original(engine.download('mass', None, b), a, b)
+            particles.lengths = self._original(engine.download('length', None, b), a, b)
+            particles.isFluid = self._original(engine.download('fluid', None, b), a, b).astype(bool)
         return particles

@@ -1547,6 +1547,75 @@ __global__ void permuteState(const int32_t* __restrict__ order, sph_state from, 
     if (identityTo) identityTo[s] = identityFrom[i];
 }
 
+// -- Splitting on the device -- //
+
+// 1 where slot i splits: fluid, in its run's zone, heavy enough, and in the caller's mask if any.
+__global__ void splitMarks(const int32_t* __restrict__ runOf, sph_state st, const sph_split_spec* __restrict__ specs,
+                           const uint8_t* __restrict__ mask, int32_t precision, int32_t* __restrict__ marks, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const sph_split_spec& spec = specs[runOf[i]];
+    const double x = st.position[i];
+    const double m = precision == 0 ? (double)static_cast<const float*>(st.mass)[i] : static_cast<const double*>(st.mass)[i];
+    marks[i] = spec.active && st.fluid[i] && x >= spec.lowX && x < spec.highX && m >= spec.minMass &&
+               (mask == nullptr || mask[i] != 0);
+}
+
+// Each run's rank of its first slot, and the total, from the exclusive scan of the marks.
+__global__ void runRanks(const RunDev* __restrict__ runs, const int32_t* __restrict__ ranks, int32_t count, int64_t n,
+                         int64_t* __restrict__ out) {
+    const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r > count) return;
+    out[r] = r < count ? ranks[runs[r].first] : ranks[n];
+}
+
+// Slot i of the old layout into the new: every particle to its run's new place, a mother's first
+// daughter in her slot and the others after the run's own particles (splitParticles,
+// sphRefinement.py:135-186; the corners ordered as itertools.product((-1/4, 1/4), repeat = dims)).
+template <typename Real>
+__global__ void splitScatter(const int32_t* __restrict__ runOf, const RunDev* __restrict__ runs,
+                             const int64_t* __restrict__ newFirst, const int64_t* __restrict__ firstRank,
+                             const int32_t* __restrict__ marks, const int32_t* __restrict__ ranks, sph_state from,
+                             sph_state to, int32_t dims, double referenceDensity, double lengthRatio, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t r = runOf[i];
+    const int64_t base = newFirst[r], local = i - runs[r].first;
+    const int64_t fs = from.stride, ts = to.stride;
+    const Real* massFrom = static_cast<const Real*>(from.mass);
+    const Real* lengthFrom = static_cast<const Real*>(from.length);
+    const Real* pressureFrom = static_cast<const Real*>(from.pressure);
+    const Real* accelerationFrom = static_cast<const Real*>(from.acceleration);
+    Real* massTo = static_cast<Real*>(to.mass);
+    Real* lengthTo = static_cast<Real*>(to.length);
+    Real* pressureTo = static_cast<Real*>(to.pressure);
+    Real* accelerationTo = static_cast<Real*>(to.acceleration);
+    const int daughters = 1 << dims;
+    const bool splits = marks[i] != 0;
+    const Real mass = massFrom[i];
+    double spacing = 0.0;
+    if (splits) {
+        const double volume = (double)mass / referenceDensity;
+        spacing = dims == 2 ? sqrt(volume) : cbrt(volume);
+    }
+    for (int k = 0; k < (splits ? daughters : 1); ++k) {
+        const int64_t d = k == 0 ? base + local
+                                 : base + runs[r].count + (int64_t)(ranks[i] - firstRank[r]) * (daughters - 1) + (k - 1);
+        for (int a = 0; a < dims; ++a) {
+            // Bit (dims - 1 - a) of k picks the corner along axis a: the last axis varies fastest
+            const double corner = ((k >> (dims - 1 - a)) & 1) ? 0.25 : -0.25;
+            to.position[a * ts + d] = from.position[a * fs + i] + (splits ? spacing * corner : 0.0);
+            to.velocity[a * ts + d] = from.velocity[a * fs + i];
+            accelerationTo[a * ts + d] = accelerationFrom[a * fs + i];
+        }
+        to.density[d] = from.density[i];
+        pressureTo[d] = pressureFrom[i];
+        massTo[d] = splits ? mass / (Real)daughters : mass;
+        lengthTo[d] = splits ? (Real)((double)lengthFrom[i] * lengthRatio) : lengthFrom[i];
+        to.fluid[d] = from.fluid[i];
+    }
+}
+
 inline unsigned gridFor(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
 
 // Runs BODY with the compile-time constant L set to the context's lanes per particle.
@@ -1579,6 +1648,10 @@ struct sph_context {
     size_t sortBytes = 0;
     void* readScratch = nullptr;   // the breaking read-out's buffers
     size_t readBytes = 0;
+    // The split's marks and ranks between sph_split_count and sph_split_apply
+    int32_t* splitMarks = nullptr;
+    int64_t splitSlots = 0;
+    std::vector<int64_t> splitChosen;
     // The sweeps' tiles for the current workgroup size and lanes (tilePer particles each)
     int32_t* tileRun = nullptr;
     int64_t* tileFirst = nullptr;
@@ -1689,6 +1762,10 @@ void releaseScratch(sph_context* ctx) {
     if (ctx->readScratch) (void)hipFree(ctx->readScratch);
     ctx->readScratch = nullptr;
     ctx->readBytes = 0;
+    if (ctx->splitMarks) (void)hipFree(ctx->splitMarks);
+    ctx->splitMarks = nullptr;
+    ctx->splitSlots = 0;
+    ctx->splitChosen.clear();
     for (void* p : ctx->allocations) (void)hipFree(p);
     ctx->allocations.clear();
     ctx->dRuns = nullptr;
@@ -2450,11 +2527,10 @@ int sph_rates(sph_context* ctx, double* maxima) {
     return SPH_OK;
 }
 
-int sph_sort_state(sph_context* ctx, int32_t* identity) {
-    int status = requireBound(ctx);
-    if (status != SPH_OK) return status;
-    const int64_t n = ctx->total;
-    if (n == 0) return SPH_OK;
+namespace {
+
+// A copy of the bound state (and `identity`) in the context's copy buffer, as a state to read from.
+int snapshotState(sph_context* ctx, int32_t* identity, sph_state& from, const int32_t*& identityFrom) {
     const int D = ctx->dims;
     const size_t real = ctx->precision == 0 ? sizeof(float) : sizeof(double);
     const size_t stride = (size_t)ctx->state.stride;
@@ -2473,7 +2549,7 @@ int sph_sort_state(sph_context* ctx, int32_t* identity) {
         ctx->sortBytes = total;
     }
     char* base = static_cast<char*>(ctx->sortCopy);
-    sph_state from = ctx->state;
+    from = ctx->state;
     void* fields[9] = {ctx->state.position, ctx->state.velocity, ctx->state.density, ctx->state.pressure,
                        ctx->state.acceleration, ctx->state.mass, ctx->state.length, ctx->state.fluid, identity};
     for (int f = 0; f < 9; ++f)
@@ -2486,7 +2562,22 @@ int sph_sort_state(sph_context* ctx, int32_t* identity) {
     from.mass = base + offsets[5];
     from.length = base + offsets[6];
     from.fluid = reinterpret_cast<uint8_t*>(base + offsets[7]);
-    const int32_t* identityFrom = identity ? reinterpret_cast<const int32_t*>(base + offsets[8]) : nullptr;
+    identityFrom = identity ? reinterpret_cast<const int32_t*>(base + offsets[8]) : nullptr;
+    return SPH_OK;
+}
+
+}  // namespace
+
+int sph_sort_state(sph_context* ctx, int32_t* identity) {
+    int status = requireBound(ctx);
+    if (status != SPH_OK) return status;
+    const int64_t n = ctx->total;
+    if (n == 0) return SPH_OK;
+    const int D = ctx->dims;
+    sph_state from{};
+    const int32_t* identityFrom = nullptr;
+    status = snapshotState(ctx, identity, from, identityFrom);
+    if (status != SPH_OK) return status;
     status = D == 2 ? buildCells<2>(ctx) : buildCells<3>(ctx);
     if (status != SPH_OK) return status;
     if (ctx->precision == 0)
@@ -2494,6 +2585,96 @@ int sph_sort_state(sph_context* ctx, int32_t* identity) {
     else
         permuteState<double><<<gridFor(n, 256), 256, 0, ctx->stream>>>(ctx->order, from, ctx->state, identityFrom, identity, D, n);
     return launchCheck(ctx, "state sort");
+}
+
+int sph_split_count(sph_context* ctx, const sph_split_spec* specs, const uint8_t* mask, int64_t* chosen) {
+    int status = requireBound(ctx);
+    if (status != SPH_OK) return status;
+    if (!specs || !chosen) return fail(ctx, SPH_ERR_VALUE, "The split needs its specifications and an output.");
+    const int32_t runs = (int32_t)ctx->runs.size();
+    for (int32_t r = 0; r < runs; ++r)
+        if (ctx->runs[r].first != (r == 0 ? 0 : ctx->runs[r - 1].first + ctx->runs[r - 1].count))
+            return fail(ctx, SPH_ERR_VALUE, "The split needs the runs packed from slot 0.");
+    const int64_t n = ctx->total;
+    // Marks and their exclusive scan (n + 1 entries), the scan's tiles, the runs' specs and ranks
+    const int64_t tiles = (n + 1 + kScanTile - 1) / kScanTile;
+    const int64_t slots = 2 * (n + 1) + tiles + 2 * (runs + 1) + (int64_t)(runs * sizeof(sph_split_spec) + 3) / 4 + 256;
+    if (slots > ctx->splitSlots) {
+        if (ctx->splitMarks) (void)hipFree(ctx->splitMarks);
+        ctx->splitMarks = nullptr;
+        ctx->splitSlots = 0;
+        HIP_TRY(hipMalloc(&ctx->splitMarks, slots * sizeof(int32_t)));
+        ctx->splitSlots = slots;
+    }
+    int32_t* marks = ctx->splitMarks;
+    int32_t* ranks = marks + (n + 1);
+    int32_t* tilePartial = ranks + (n + 1);
+    int64_t* runFirst = reinterpret_cast<int64_t*>(tilePartial + tiles + (tiles & 1));
+    sph_split_spec* dSpecs = reinterpret_cast<sph_split_spec*>(runFirst + (runs + 1));
+    HIP_TRY(hipMemcpyAsync(dSpecs, specs, runs * sizeof(sph_split_spec), hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemsetAsync(marks, 0, (n + 1) * sizeof(int32_t), ctx->stream));
+    if (n > 0)
+        splitMarks<<<gridFor(n, 256), 256, 0, ctx->stream>>>(ctx->runOf, ctx->state, dSpecs, mask, ctx->precision, marks, n);
+    scanTiles<<<(unsigned)tiles, kScanThreads, 0, ctx->stream>>>(marks, ranks, tilePartial, n);
+    scanPartials<<<1, kScanThreads, 0, ctx->stream>>>(tilePartial, tiles, ranks, n);
+    scanAdd<<<(unsigned)tiles, kScanThreads, 0, ctx->stream>>>(ranks, tilePartial, n);
+    runRanks<<<(runs + 64) / 64, 64, 0, ctx->stream>>>(ctx->dRuns, ranks, runs, n, runFirst);
+    status = launchCheck(ctx, "split marks");
+    if (status != SPH_OK) return status;
+    std::vector<int64_t> first(runs + 1);
+    HIP_TRY(hipMemcpyAsync(first.data(), runFirst, (runs + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    ctx->splitChosen.assign(runs, 0);
+    for (int32_t r = 0; r < runs; ++r) chosen[r] = ctx->splitChosen[r] = first[r + 1] - first[r];
+    return SPH_OK;
+}
+
+int sph_split_apply(sph_context* ctx, const sph_state* into, double referenceDensity, double lengthRatio) {
+    int status = requireBound(ctx);
+    if (status != SPH_OK) return status;
+    const int32_t runs = (int32_t)ctx->runs.size();
+    if ((int32_t)ctx->splitChosen.size() != runs || !ctx->splitMarks)
+        return fail(ctx, SPH_ERR_VALUE, "sph_split_count must come right before sph_split_apply.");
+    if (!into || !into->position || !into->velocity || !into->density || !into->pressure || !into->acceleration ||
+        !into->mass || !into->length || !into->fluid)
+        return fail(ctx, SPH_ERR_VALUE, "The split needs every array of the state it writes.");
+    if (!(referenceDensity > 0.0) || !(lengthRatio > 0.0))
+        return fail(ctx, SPH_ERR_VALUE, "The split needs a positive reference density and length ratio.");
+    const int daughters = 1 << ctx->dims;
+    std::vector<int64_t> newFirst(runs);
+    int64_t total = 0;
+    for (int32_t r = 0; r < runs; ++r) {
+        newFirst[r] = total;
+        total += ctx->runs[r].count + ctx->splitChosen[r] * (daughters - 1);
+    }
+    if (into->stride < total) return fail(ctx, SPH_ERR_VALUE, "The state the split writes is too small for the grown runs.");
+    const int64_t n = ctx->total;
+    int32_t* marks = ctx->splitMarks;
+    int32_t* ranks = marks + (n + 1);
+    const int64_t tiles = (n + 1 + kScanTile - 1) / kScanTile;
+    int64_t* runFirst = reinterpret_cast<int64_t*>(ranks + (n + 1) + tiles + (tiles & 1));
+    // The new runs' first slots after the ranks, in the specs' place
+    int64_t* dNewFirst = runFirst + (runs + 1);
+    HIP_TRY(hipMemcpyAsync(dNewFirst, newFirst.data(), runs * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream));
+    sph_state from{};
+    const int32_t* identityFrom = nullptr;
+    status = snapshotState(ctx, nullptr, from, identityFrom);
+    if (status != SPH_OK) return status;
+    if (n > 0) {
+        if (ctx->precision == 0)
+            splitScatter<float><<<gridFor(n, 256), 256, 0, ctx->stream>>>(ctx->runOf, ctx->dRuns, dNewFirst, runFirst, marks,
+                                                                          ranks, from, *into, ctx->dims, referenceDensity,
+                                                                          lengthRatio, n);
+        else
+            splitScatter<double><<<gridFor(n, 256), 256, 0, ctx->stream>>>(ctx->runOf, ctx->dRuns, dNewFirst, runFirst,
+                                                                           marks, ranks, from, *into, ctx->dims,
+                                                                           referenceDensity, lengthRatio, n);
+    }
+    status = launchCheck(ctx, "split");
+    if (status != SPH_OK) return status;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    ctx->splitChosen.clear();
+    return SPH_OK;
 }
 
 int sph_global_sums(sph_context* ctx, double* sums) {

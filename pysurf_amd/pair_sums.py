@@ -132,10 +132,21 @@ class Engine:
         if count <= self.capacity and self.tensors:
             return
         capacity = max(count, int(1.25 * self.capacity), 1)
-        d, dev, real = self.dimensions, self.device, self.real
         old, oldCount = self.tensors, self.count
+        fresh = self._allocate(capacity)
+        with torch.cuda.device(self.device), torch.cuda.stream(self.stream):
+            if old and oldCount:
+                for name, tensor in fresh.items():
+                    tensor[..., :oldCount].copy_(old[name][..., :oldCount])
+        self.tensors = fresh
+        self.capacity = capacity
+        self._layout = None
+
+    def _allocate(self, capacity: int) -> dict:
+        '''Zeroed state and host-step tensors of `capacity` slots.'''
+        d, dev, real = self.dimensions, self.device, self.real
         with torch.cuda.device(dev), torch.cuda.stream(self.stream):
-            fresh = {
+            return {
                 'position': torch.zeros((d, capacity), dtype = torch.float64, device = dev),
                 'velocity': torch.zeros((d, capacity), dtype = torch.float64, device = dev),
                 'density': torch.zeros(capacity, dtype = torch.float64, device = dev),
@@ -153,12 +164,53 @@ class Engine:
                 'scalar64': torch.zeros(capacity, dtype = torch.float64, device = dev),
                 'matrix': torch.zeros((d * d, capacity), dtype = real, device = dev),
             }
-            if old and oldCount:
-                for name, tensor in fresh.items():
-                    tensor[..., :oldCount].copy_(old[name][..., :oldCount])
-        self.tensors = fresh
-        self.capacity = capacity
-        self._layout = None
+
+    def stateOf(self, tensors: dict, stride: int) -> _native.State:
+        t = tensors
+        return _native.State(t['position'].data_ptr(), t['velocity'].data_ptr(), t['density'].data_ptr(),
+                             t['pressure'].data_ptr(), t['acceleration'].data_ptr(), t['mass'].data_ptr(),
+                             t['length'].data_ptr(), t['fluid'].data_ptr(), stride)
+
+    def split(self, specs: list, referenceDensity: float, lengthRatio: float = 1.0, mask = None) -> np.ndarray:
+
+        '''
+
+        Split the fluid particles `specs` (one _native.SplitSpec per run) and `mask` (a uint8 tensor
+        over the slots, or None) select into 2^dims daughters on the device (sph_split_count,
+        sph_split_apply; splitParticles, sphRefinement.py:135-186), growing each run in place of
+        the layout: run r's own particles first, then its mothers' other daughters. Returns how
+        many particles of each run split. The runs' clocks restart at zero; the caller sets them
+        again. Synchronizes.
+
+        '''
+
+        runs = len(self._runs)
+        array = self._readSpecs(_native.SplitSpec, specs)
+        chosen = np.zeros(runs, dtype = np.int64)
+        maskPointer = ctypes.c_void_p(mask.data_ptr()) if mask is not None else None
+        self.check(self.library.sph_split_count(self.handle, array, maskPointer, chosen.ctypes.data_as(ctypes.c_void_p)),
+                   'sph_split_count')
+        if not chosen.any():
+            return chosen
+        grown = chosen * (2**self.dimensions - 1)
+        total = self.count + int(grown.sum())
+        target = self.tensors if total <= self.capacity else self._allocate(max(total, int(1.25 * self.capacity)))
+        stride = self.capacity if target is self.tensors else target['position'].shape[1]
+        self.check(self.library.sph_split_apply(self.handle, ctypes.byref(self.stateOf(target, stride)),
+                                                float(referenceDensity), float(lengthRatio)), 'sph_split_apply')
+        fresh = []
+        first = 0
+        for r in range(runs):
+            run = _native.Run.from_buffer_copy(self._runs[r])
+            run.first, run.count = first, int(self._runs[r].count) + int(grown[r])
+            if chosen[r] and lengthRatio < 1.0:
+                run.hmin = min(run.hmin, run.hmin * lengthRatio)
+            first += run.count
+            fresh.append(run)
+        if target is not self.tensors:
+            self.tensors, self.capacity = target, stride
+        self.configure(fresh)
+        return chosen
 
     def configure(self, runs: list) -> None:
         '''Hand the runs (a list of _native.Run) to the library and bind the state tensors.'''
@@ -168,10 +220,7 @@ class Engine:
         array = (_native.Run * len(runs))(*runs)
         self._runs = array
         self.check(self.library.sph_configure(self.handle, array, len(runs), self.capacity), 'sph_configure')
-        t = self.tensors
-        state = _native.State(t['position'].data_ptr(), t['velocity'].data_ptr(), t['density'].data_ptr(),
-                              t['pressure'].data_ptr(), t['acceleration'].data_ptr(), t['mass'].data_ptr(),
-                              t['length'].data_ptr(), t['fluid'].data_ptr(), self.capacity)
+        state = self.stateOf(self.tensors, self.capacity)
         self.check(self.library.sph_bind(self.handle, ctypes.byref(state)), 'sph_bind')
 
     def upload(self, name: str, values, count: int | None = None) -> torch.Tensor:

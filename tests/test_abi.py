@@ -48,6 +48,8 @@ def testStructuresMatchTheHeaderLayout():
     assert _native.FractionSpec.offset.offset == 56
     assert _native.FractionSpec.h.offset == 64
     assert _native.FractionSpec.stripCentre.offset == 120
+    assert ctypes.sizeof(_native.SplitSpec) == 32
+    assert _native.SplitSpec.minMass.offset == 24
 
 
 def testBadArgumentsFailWithoutADevice():
