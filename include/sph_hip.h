@@ -242,6 +242,32 @@ typedef struct {
 int sph_split_count(sph_context* ctx, const sph_split_spec* specs, const uint8_t* mask, int64_t* chosen);
 int sph_split_apply(sph_context* ctx, const sph_state* into, double referenceDensity, double lengthRatio);
 
+/* Merging split fluid particles on the device (mergeParticles, sphRefinement.py:212-311), in the
+ * bound state. In each run with `active`, the fluid particles with lowX <= x < highX (and set in
+ * `mask`, one byte per slot, when given) that have split, level = rint(log2(referenceMass / m) /
+ * dims) > 0, merge 2^dims of one level at a time: level by level, a pass with the grid unshifted
+ * and one shifted by half a cell groups the level's free particles in cells the size of a particle
+ * one level up, (referenceMass / 2^(dims (level - 1)) / referenceDensity)^(1/dims) across; in each
+ * cell holding 2^dims or more, the 2^dims nearest its centre (ties by slot) form a group. A group
+ * becomes one particle in its first member's slot: mass-weighted position, velocity, acceleration
+ * and pressure, the total mass, the total mass over the members' volumes as density, and the
+ * members' mean length over lengthRatio. The others leave; the holes they leave at or before a run's
+ * last boundary slot take, in order, the run's last as many remaining particles after it, and the
+ * runs close up, packed from slot 0 (which they must be). groups[r] and removedEnergy[r] (the
+ * kinetic energy of the members' motion about their groups' centres of mass) receive each run's.
+ * The caller then configures the runs with counts less groups (2^dims - 1), binds the same state
+ * and sets the clocks again. Synchronizes. */
+typedef struct {
+    int32_t active;
+    int32_t pad;
+    double lowX, highX;
+    double referenceMass;      /* the mass of a fluid particle that has not split */
+    double referenceDensity;
+    double lengthRatio;        /* the ratio the daughters' lengths took at their split */
+} sph_merge_spec;
+
+int sph_merge(sph_context* ctx, const sph_merge_spec* specs, const uint8_t* mask, int64_t* groups, double* removedEnergy);
+
 int sph_synchronize(sph_context* ctx);
 
 /* Per-kernel timing with HIP events recorded on the context's stream around every launch of a

@@ -87,6 +87,12 @@ class SplitSpec(ctypes.Structure):
                 ('minMass', ctypes.c_double)]
 
 
+class MergeSpec(ctypes.Structure):
+    '''sph_merge_spec'''
+    _fields_ = [('active', ctypes.c_int32), ('pad', ctypes.c_int32), ('lowX', ctypes.c_double), ('highX', ctypes.c_double),
+                ('referenceMass', ctypes.c_double), ('referenceDensity', ctypes.c_double), ('lengthRatio', ctypes.c_double)]
+
+
 # Every entry point include/sph_hip.h declares, with its argument types
 signatures = {
     'sph_create': [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p],
@@ -117,6 +123,7 @@ signatures = {
     'sph_fluid_fraction': [ctypes.c_void_p, ctypes.POINTER(FractionSpec), ctypes.c_void_p],
     'sph_split_count': [ctypes.c_void_p, ctypes.POINTER(SplitSpec), ctypes.c_void_p, ctypes.c_void_p],
     'sph_split_apply': [ctypes.c_void_p, ctypes.POINTER(State), ctypes.c_double, ctypes.c_double],
+    'sph_merge': [ctypes.c_void_p, ctypes.POINTER(MergeSpec), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
     'sph_set_launch': [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32],
     'sph_profile': [ctypes.c_void_p, ctypes.c_int32],
     'sph_kernel_times': [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32],

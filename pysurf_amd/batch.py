@@ -221,6 +221,45 @@ class BatchEngine:
                 self.setClock(r, float(clock['time'][r]), self.until[r], int(clock['steps'][r]), self.maxSteps[r])
         return chosen
 
+    def merge(self, zones: list, referenceMass, lengthRatio: float = 1.0, mask = None):
+
+        '''
+
+        Merge, on the device, each run's split fluid particles with x in its zone (`zones`: one
+        (lowX, highX) or None per run), 2^dims of one level at a time into one particle of the level
+        above (mergeParticles, sphRefinement.py:212-311), `referenceMass` (one value, or one per run)
+        being an unsplit fluid particle's mass and `lengthRatio` the ratio the daughters' lengths
+        took. Each run closes up in place; the clocks carry over. Returns a MergeResult per run
+        (its `source` left empty). The batch must be in slot order (not sorted).
+
+        '''
+
+        from pysurf_amd.refinement import MergeResult
+        if self.identity is not None:
+            raise ValueError('Merge a batch before sorting it: its slots must hold the cases\' own order.')
+        runs = len(self.cases)
+        if len(zones) != runs:
+            raise ValueError(f'The merge needs one zone (or None) per run, {runs} of them.')
+        masses = np.broadcast_to(np.asarray(referenceMass, dtype = float), (runs,))
+        specs = []
+        for zone, mass, case in zip(zones, masses, self.cases):
+            spec = _native.MergeSpec()
+            if zone is not None:
+                spec.active, spec.lowX, spec.highX = 1, float(zone[0]), float(zone[1])
+                spec.referenceMass, spec.referenceDensity = float(mass), float(case.referenceDensity)
+                spec.lengthRatio = float(lengthRatio)
+            specs.append(spec)
+        clock = self.clocks()
+        groups, energy = self.engine.merge(specs, mask)
+        if groups.any():
+            shrunk = groups * (2**self.dimensions - 1)
+            counts = np.diff(self.offsets) - shrunk
+            self.offsets = [0] + np.cumsum(counts).astype(int).tolist()
+            self.fluidCounts = [count - int(less) for count, less in zip(self.fluidCounts, shrunk)]
+            for r in range(runs):
+                self.setClock(r, float(clock['time'][r]), self.until[r], int(clock['steps'][r]), self.maxSteps[r])
+        return [MergeResult(groups = int(g), removedEnergy = float(e)) for g, e in zip(groups, energy)]
+
     def globalSums(self) -> np.ndarray:
         '''Every run's float64 sums over its fluid, shape (runs, 7), columns as _native.globalSumNames.'''
         return self.engine.globalSums(len(self.cases))

@@ -1616,6 +1616,261 @@ __global__ void splitScatter(const int32_t* __restrict__ runOf, const RunDev* __
     }
 }
 
+// -- Merging on the device -- //
+
+constexpr int kMergeLevels = 8;
+
+// One pass of a run's merge: the level it groups, the cells' size and shift, and the cell grid's
+// lowest integer coordinates, extent and place in the pass's cell array.
+struct MergePass {
+    int32_t active, level;
+    double cell, offset;
+    long long low[3], extent[3];
+    long long cellBase;
+};
+
+__device__ __forceinline__ double massOf(const sph_state& st, int32_t precision, int64_t i) {
+    return precision == 0 ? (double)static_cast<const float*>(st.mass)[i] : static_cast<const double*>(st.mass)[i];
+}
+
+// Each slot's level if it may merge (0 if not), each run's count per level and its last boundary slot.
+__global__ void mergeLevels(const int32_t* __restrict__ runOf, sph_state st, const sph_merge_spec* __restrict__ specs,
+                            const uint8_t* __restrict__ mask, int32_t precision, int32_t dims, int32_t* __restrict__ levelOf,
+                            int32_t* __restrict__ histogram, long long* __restrict__ lastBoundary, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t r = runOf[i];
+    levelOf[i] = 0;
+    if (!st.fluid[i]) {
+        atomicMax(&lastBoundary[r], (long long)i);
+        return;
+    }
+    const sph_merge_spec& spec = specs[r];
+    const double x = st.position[i];
+    if (!spec.active || !(x >= spec.lowX && x < spec.highX) || (mask && !mask[i])) return;
+    const int level = (int)rint(log2(spec.referenceMass / massOf(st, precision, i)) / dims);
+    if (level <= 0 || level >= kMergeLevels) return;
+    levelOf[i] = level;
+    atomicAdd(&histogram[r * kMergeLevels + level], 1);
+}
+
+// The scaled coordinate of slot i along axis a in its run's pass grid.
+__device__ __forceinline__ double mergeScaled(const sph_state& st, const MergePass& pass, int a, int64_t i) {
+    return st.position[a * st.stride + i] / pass.cell + pass.offset;
+}
+
+// The lowest and highest integer cell coordinates of each run's free particles of the pass's level.
+__global__ void mergeBounds(const int32_t* __restrict__ runOf, sph_state st, const MergePass* __restrict__ passes,
+                            int32_t dims, const int32_t* __restrict__ levelOf, long long* __restrict__ low,
+                            long long* __restrict__ high, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t r = runOf[i];
+    const MergePass& pass = passes[r];
+    if (!pass.active || levelOf[i] != pass.level) return;
+    for (int a = 0; a < dims; ++a) {
+        const long long k = (long long)floor(mergeScaled(st, pass, a, i));
+        atomicMin(&low[r * 3 + a], k);
+        atomicMax(&high[r * 3 + a], k);
+    }
+}
+
+// Each free particle's cell (row-major over its run's grid, after the run's cells) and squared
+// distance from the cell's centre in cells, and the cells' counts.
+__global__ void mergeKeys(const int32_t* __restrict__ runOf, sph_state st, const MergePass* __restrict__ passes,
+                          int32_t dims, const int32_t* __restrict__ levelOf, int32_t* __restrict__ key,
+                          double* __restrict__ distance, int32_t* __restrict__ counts, int64_t n) {
+#pragma clang fp contract(off)
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t r = runOf[i];
+    const MergePass& pass = passes[r];
+    key[i] = -1;
+    if (!pass.active || levelOf[i] != pass.level) return;
+    long long flat = 0;
+    double sum = 0.0;
+    for (int a = 0; a < dims; ++a) {
+        const double scaled = mergeScaled(st, pass, a, i);
+        const double corner = floor(scaled);
+        flat = flat * pass.extent[a] + ((long long)corner - pass.low[a]);
+        const double t = scaled - corner - 0.5;
+        sum = sum + t * t;
+    }
+    const int32_t k = (int32_t)(pass.cellBase + flat);
+    key[i] = k;
+    distance[i] = sum;
+    atomicAdd(&counts[k], 1);
+}
+
+// Each cell's particles by distance from its centre, ties by slot.
+__global__ void mergeOrder(const int32_t* __restrict__ start, int32_t* __restrict__ members,
+                           const double* __restrict__ distance, int64_t cells) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= cells) return;
+    const int32_t first = start[c], last = start[c + 1];
+    for (int32_t k = first + 1; k < last; ++k) {
+        const int32_t v = members[k];
+        const double dv = distance[v];
+        int32_t m = k - 1;
+        while (m >= first && (distance[members[m]] > dv || (distance[members[m]] == dv && members[m] > v))) {
+            members[m + 1] = members[m];
+            --m;
+        }
+        members[m + 1] = v;
+    }
+}
+
+// A group of the `size` nearest in every cell holding that many: its members by slot, no longer free.
+__global__ void mergeTake(const int32_t* __restrict__ start, const int32_t* __restrict__ members, int32_t size,
+                          const int32_t* __restrict__ runOf, int32_t* __restrict__ groups, int32_t* __restrict__ groupCount,
+                          int32_t* __restrict__ levelOf, int32_t* __restrict__ taken, int64_t cells) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= cells) return;
+    const int32_t first = start[c];
+    if (start[c + 1] - first < size) return;
+    int32_t chosen[8];
+    for (int k = 0; k < size; ++k) {
+        int32_t v = members[first + k];
+        int m = k - 1;
+        while (m >= 0 && chosen[m] > v) {
+            chosen[m + 1] = chosen[m];
+            --m;
+        }
+        chosen[m + 1] = v;
+    }
+    const int32_t g = atomicAdd(groupCount, 1);
+    for (int k = 0; k < size; ++k) {
+        groups[(int64_t)g * size + k] = chosen[k];
+        levelOf[chosen[k]] = 0;
+    }
+    atomicAdd(&taken[runOf[chosen[0]]], size);
+}
+
+// Each group into its first member's slot; the others marked to leave; the energy the merge removes.
+template <typename Real>
+__global__ void mergeGroups(const int32_t* __restrict__ groups, int32_t count, int32_t size, const int32_t* __restrict__ runOf,
+                            const sph_merge_spec* __restrict__ specs, sph_state st, int32_t dims,
+                            int32_t* __restrict__ removed, double* __restrict__ energy) {
+    const int32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= count) return;
+    const int32_t* member = groups + (int64_t)g * size;
+    const int64_t stride = st.stride;
+    Real* mass = static_cast<Real*>(st.mass);
+    Real* length = static_cast<Real*>(st.length);
+    Real* pressure = static_cast<Real*>(st.pressure);
+    Real* acceleration = static_cast<Real*>(st.acceleration);
+    double total = 0.0;
+    for (int k = 0; k < size; ++k) total += (double)mass[member[k]];
+    double position[3] = {0.0, 0.0, 0.0}, velocity[3] = {0.0, 0.0, 0.0}, accel[3] = {0.0, 0.0, 0.0};
+    double p = 0.0, volume = 0.0, h = 0.0;
+    for (int k = 0; k < size; ++k) {
+        const int32_t j = member[k];
+        const double m = (double)mass[j], w = m / total;
+        for (int a = 0; a < dims; ++a) {
+            position[a] += w * st.position[a * stride + j];
+            velocity[a] += w * st.velocity[a * stride + j];
+            accel[a] += w * (double)acceleration[a * stride + j];
+        }
+        p += w * (double)pressure[j];
+        volume += m / st.density[j];
+        h += (double)length[j];
+    }
+    double removedEnergy = 0.0;
+    for (int k = 0; k < size; ++k) {
+        const int32_t j = member[k];
+        double speed2 = 0.0;
+        for (int a = 0; a < dims; ++a) {
+            const double du = st.velocity[a * stride + j] - velocity[a];
+            speed2 += du * du;
+        }
+        removedEnergy += (double)mass[j] * speed2;
+    }
+    const int32_t first = member[0];
+    for (int a = 0; a < dims; ++a) {
+        st.position[a * stride + first] = position[a];
+        st.velocity[a * stride + first] = velocity[a];
+        acceleration[a * stride + first] = (Real)accel[a];
+    }
+    pressure[first] = (Real)p;
+    st.density[first] = total / volume;
+    mass[first] = (Real)total;
+    length[first] = (Real)(h / size / specs[runOf[first]].lengthRatio);
+    atomicAdd(&energy[runOf[first]], 0.5 * removedEnergy);
+}
+
+// Every member of a group but its first leaves.
+__global__ void markRemoved(const int32_t* __restrict__ groups, int32_t count, int32_t size, int32_t* __restrict__ removed) {
+    const int32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= count) return;
+    for (int k = 1; k < size; ++k) removed[groups[(int64_t)g * size + k]] = 1;
+}
+
+// Which slots are holes at or before their run's last boundary slot, and which remain after it.
+__global__ void compactFlags(const int32_t* __restrict__ runOf, const long long* __restrict__ lastBoundary,
+                             const int32_t* __restrict__ removed, int32_t* __restrict__ hole, int32_t* __restrict__ tail,
+                             int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const long long last = lastBoundary[runOf[i]];
+    hole[i] = removed[i] && i <= last;
+    tail[i] = !removed[i] && i > last;
+}
+
+// The last as many remaining particles after a run's last boundary slot as it has holes: movers,
+// the k-th of them filling the run's k-th hole; and which slots keep a place.
+__global__ void compactMovers(const int32_t* __restrict__ runOf, const long long* __restrict__ lastBoundary,
+                              const int32_t* __restrict__ removed, const int32_t* __restrict__ tail,
+                              const int32_t* __restrict__ tailRank, const long long* __restrict__ tailBase,
+                              const long long* __restrict__ holeBase, int32_t* __restrict__ movers,
+                              int32_t* __restrict__ keep, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t r = runOf[i];
+    if (i <= lastBoundary[r]) {
+        keep[i] = 1;
+        return;
+    }
+    bool mover = false;
+    if (tail[i]) {
+        const long long holes = holeBase[r + 1] - holeBase[r], tails = tailBase[r + 1] - tailBase[r];
+        const long long k = (tailRank[i] - tailBase[r]) - (tails - holes);
+        if (k >= 0) {
+            movers[holeBase[r] + k] = (int32_t)i;
+            mover = true;
+        }
+    }
+    keep[i] = !removed[i] && !mover;
+}
+
+// Each kept place takes its own particle, or a hole its mover, at its packed new slot.
+template <typename Real>
+__global__ void compactPermute(const int32_t* __restrict__ keep, const int32_t* __restrict__ newIndex,
+                               const int32_t* __restrict__ hole, const int32_t* __restrict__ holeRank,
+                               const int32_t* __restrict__ movers, sph_state from, sph_state to, int32_t dims, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !keep[i]) return;
+    const int64_t src = hole[i] ? movers[holeRank[i]] : i, d = newIndex[i];
+    const int64_t fs = from.stride, ts = to.stride;
+    for (int a = 0; a < dims; ++a) {
+        to.position[a * ts + d] = from.position[a * fs + src];
+        to.velocity[a * ts + d] = from.velocity[a * fs + src];
+        static_cast<Real*>(to.acceleration)[a * ts + d] = static_cast<const Real*>(from.acceleration)[a * fs + src];
+    }
+    to.density[d] = from.density[src];
+    static_cast<Real*>(to.pressure)[d] = static_cast<const Real*>(from.pressure)[src];
+    static_cast<Real*>(to.mass)[d] = static_cast<const Real*>(from.mass)[src];
+    static_cast<Real*>(to.length)[d] = static_cast<const Real*>(from.length)[src];
+    to.fluid[d] = from.fluid[src];
+}
+
+// Run r's value of an exclusive scan at its first slot, and the total after the last.
+__global__ void runBases(const RunDev* __restrict__ runs, const int32_t* __restrict__ ranks, int32_t count, int64_t n,
+                         long long* __restrict__ out) {
+    const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r > count) return;
+    out[r] = r < count ? ranks[runs[r].first] : ranks[n];
+}
+
 inline unsigned gridFor(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
 
 // Runs BODY with the compile-time constant L set to the context's lanes per particle.
@@ -1648,6 +1903,11 @@ struct sph_context {
     size_t sortBytes = 0;
     void* readScratch = nullptr;   // the breaking read-out's buffers
     size_t readBytes = 0;
+    // The merge's per-slot and per-cell buffers
+    void* mergeScratch = nullptr;
+    size_t mergeBytes = 0;
+    void* mergeCells = nullptr;
+    size_t mergeCellBytes = 0;
     // The split's marks and ranks between sph_split_count and sph_split_apply
     int32_t* splitMarks = nullptr;
     int64_t splitSlots = 0;
@@ -1766,6 +2026,12 @@ void releaseScratch(sph_context* ctx) {
     ctx->splitMarks = nullptr;
     ctx->splitSlots = 0;
     ctx->splitChosen.clear();
+    if (ctx->mergeScratch) (void)hipFree(ctx->mergeScratch);
+    ctx->mergeScratch = nullptr;
+    ctx->mergeBytes = 0;
+    if (ctx->mergeCells) (void)hipFree(ctx->mergeCells);
+    ctx->mergeCells = nullptr;
+    ctx->mergeCellBytes = 0;
     for (void* p : ctx->allocations) (void)hipFree(p);
     ctx->allocations.clear();
     ctx->dRuns = nullptr;
@@ -2200,6 +2466,42 @@ int neighborsImpl(sph_context* ctx, int32_t* counts, const int64_t* offsets, int
 // -------------------------------------------------------------------------------------------------
 // C ABI
 // -------------------------------------------------------------------------------------------------
+
+namespace {
+
+// A device buffer of at least `bytes`, grown when short (its contents are not kept).
+int growBuffer(sph_context* ctx, void*& buffer, size_t& size, size_t bytes) {
+    if (bytes <= size) return SPH_OK;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (buffer) (void)hipFree(buffer);
+    buffer = nullptr;
+    size = 0;
+    HIP_TRY(hipMalloc(&buffer, bytes));
+    size = bytes;
+    return SPH_OK;
+}
+
+// Pieces of one buffer, each 256-byte aligned.
+struct Carver {
+    char* base;
+    size_t used = 0;
+    template <typename T>
+    T* take(int64_t count) {
+        T* p = reinterpret_cast<T*>(base + used);
+        used += ((size_t)std::max<int64_t>(count, 1) * sizeof(T) + 255) / 256 * 256;
+        return p;
+    }
+};
+
+// An exclusive scan of n int32 values into out (n + 1 entries), with `tiles` partial sums.
+void scanInto(sph_context* ctx, const int32_t* in, int32_t* out, int32_t* tiles, int64_t n) {
+    const int64_t count = (n + kScanTile - 1) / kScanTile + 1;
+    scanTiles<<<(unsigned)count, kScanThreads, 0, ctx->stream>>>(in, out, tiles, n);
+    scanPartials<<<1, kScanThreads, 0, ctx->stream>>>(tiles, count, out, n);
+    scanAdd<<<(unsigned)count, kScanThreads, 0, ctx->stream>>>(out, tiles, n);
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -2674,6 +2976,211 @@ int sph_split_apply(sph_context* ctx, const sph_state* into, double referenceDen
     if (status != SPH_OK) return status;
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     ctx->splitChosen.clear();
+    return SPH_OK;
+}
+
+
+int sph_merge(sph_context* ctx, const sph_merge_spec* specs, const uint8_t* mask, int64_t* groups, double* removedEnergy) {
+    int status = requireBound(ctx);
+    if (status != SPH_OK) return status;
+    if (!specs || !groups || !removedEnergy) return fail(ctx, SPH_ERR_VALUE, "The merge needs its specifications and outputs.");
+    const int32_t runs = (int32_t)ctx->runs.size();
+    for (int32_t r = 0; r < runs; ++r) {
+        if (ctx->runs[r].first != (r == 0 ? 0 : ctx->runs[r - 1].first + ctx->runs[r - 1].count))
+            return fail(ctx, SPH_ERR_VALUE, "The merge needs the runs packed from slot 0.");
+        if (specs[r].active && !(specs[r].referenceMass > 0.0 && specs[r].referenceDensity > 0.0 && specs[r].lengthRatio > 0.0))
+            return fail(ctx, SPH_ERR_VALUE, "A merge needs a positive reference mass, density and length ratio.");
+        groups[r] = 0;
+        removedEnergy[r] = 0.0;
+    }
+    const int64_t n = ctx->total;
+    if (n == 0) return SPH_OK;
+    const int D = ctx->dims, size = 1 << D;
+    const int64_t tileCount = (n + 1 + kScanTile - 1) / kScanTile + 1;
+    // Sizes first, then the pieces
+    Carver sizing{nullptr};
+    for (int k = 0; k < 12; ++k) sizing.take<int32_t>(n + 1);
+    sizing.take<double>(n);
+    sizing.take<int32_t>(tileCount);
+    sizing.take<sph_merge_spec>(runs);
+    sizing.take<MergePass>(runs);
+    sizing.take<int32_t>((int64_t)runs * kMergeLevels);
+    for (int k = 0; k < 3; ++k) sizing.take<long long>(runs);
+    sizing.take<long long>((int64_t)runs * 3);
+    sizing.take<long long>((int64_t)runs * 3);
+    sizing.take<int32_t>(runs);
+    sizing.take<int32_t>(1);
+    sizing.take<double>(runs);
+    sizing.take<long long>(runs + 1);
+    sizing.take<long long>(runs + 1);
+    status = growBuffer(ctx, ctx->mergeScratch, ctx->mergeBytes, sizing.used);
+    if (status != SPH_OK) return status;
+    Carver carve{static_cast<char*>(ctx->mergeScratch)};
+    int32_t* levelOf = carve.take<int32_t>(n + 1);
+    int32_t* key = carve.take<int32_t>(n + 1);
+    int32_t* removed = carve.take<int32_t>(n + 1);
+    int32_t* hole = carve.take<int32_t>(n + 1);
+    int32_t* tail = carve.take<int32_t>(n + 1);
+    int32_t* keep = carve.take<int32_t>(n + 1);
+    int32_t* holeRank = carve.take<int32_t>(n + 1);
+    int32_t* tailRank = carve.take<int32_t>(n + 1);
+    int32_t* newIndex = carve.take<int32_t>(n + 1);
+    int32_t* members = carve.take<int32_t>(n + 1);
+    int32_t* groupMembers = carve.take<int32_t>(n + 1);
+    int32_t* movers = carve.take<int32_t>(n + 1);
+    double* distance = carve.take<double>(n);
+    int32_t* tiles = carve.take<int32_t>(tileCount);
+    sph_merge_spec* dSpecs = carve.take<sph_merge_spec>(runs);
+    MergePass* dPasses = carve.take<MergePass>(runs);
+    int32_t* histogram = carve.take<int32_t>((int64_t)runs * kMergeLevels);
+    long long* lastBoundary = carve.take<long long>(runs);
+    carve.take<long long>(runs);
+    carve.take<long long>(runs);
+    long long* low = carve.take<long long>((int64_t)runs * 3);
+    long long* high = carve.take<long long>((int64_t)runs * 3);
+    int32_t* taken = carve.take<int32_t>(runs);
+    int32_t* groupCount = carve.take<int32_t>(1);
+    double* energy = carve.take<double>(runs);
+    long long* holeBase = carve.take<long long>(runs + 1);
+    long long* tailBase = carve.take<long long>(runs + 1);
+
+    HIP_TRY(hipMemcpyAsync(dSpecs, specs, runs * sizeof(sph_merge_spec), hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemsetAsync(histogram, 0, (size_t)runs * kMergeLevels * sizeof(int32_t), ctx->stream));
+    HIP_TRY(hipMemsetAsync(lastBoundary, 0xff, runs * sizeof(long long), ctx->stream));
+    HIP_TRY(hipMemsetAsync(removed, 0, (n + 1) * sizeof(int32_t), ctx->stream));
+    HIP_TRY(hipMemsetAsync(taken, 0, runs * sizeof(int32_t), ctx->stream));
+    HIP_TRY(hipMemsetAsync(groupCount, 0, sizeof(int32_t), ctx->stream));
+    HIP_TRY(hipMemsetAsync(energy, 0, runs * sizeof(double), ctx->stream));
+    mergeLevels<<<gridFor(n, 256), 256, 0, ctx->stream>>>(ctx->runOf, ctx->state, dSpecs, mask, ctx->precision, D, levelOf,
+                                                         histogram, lastBoundary, n);
+    status = launchCheck(ctx, "merge levels");
+    if (status != SPH_OK) return status;
+    std::vector<int32_t> freeCount((size_t)runs * kMergeLevels);
+    std::vector<long long> lastHost(runs);
+    HIP_TRY(hipMemcpyAsync(freeCount.data(), histogram, freeCount.size() * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(lastHost.data(), lastBoundary, runs * sizeof(long long), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+
+    std::vector<MergePass> passes(runs);
+    std::vector<long long> lowHost((size_t)runs * 3), highHost((size_t)runs * 3);
+    std::vector<int32_t> takenHost(runs);
+    std::vector<int64_t> membersTaken(runs, 0);
+    const double shifts[2] = {0.0, 0.5};
+    for (int level = 1; level < kMergeLevels; ++level)
+        for (double shift : shifts) {
+            bool any = false;
+            for (int32_t r = 0; r < runs; ++r) {
+                MergePass& pass = passes[r];
+                pass = MergePass{};
+                pass.level = level;
+                pass.offset = shift;
+                pass.active = specs[r].active && freeCount[(size_t)r * kMergeLevels + level] >= size;
+                if (!pass.active) continue;
+                any = true;
+                pass.cell = std::pow(specs[r].referenceMass / std::pow((double)size, level - 1) / specs[r].referenceDensity,
+                                     1.0 / D);
+                for (int a = 0; a < 3; ++a) {
+                    lowHost[r * 3 + a] = std::numeric_limits<long long>::max();
+                    highHost[r * 3 + a] = std::numeric_limits<long long>::min();
+                }
+            }
+            if (!any) continue;
+            HIP_TRY(hipMemcpyAsync(dPasses, passes.data(), runs * sizeof(MergePass), hipMemcpyHostToDevice, ctx->stream));
+            HIP_TRY(hipMemcpyAsync(low, lowHost.data(), lowHost.size() * sizeof(long long), hipMemcpyHostToDevice, ctx->stream));
+            HIP_TRY(hipMemcpyAsync(high, highHost.data(), highHost.size() * sizeof(long long), hipMemcpyHostToDevice, ctx->stream));
+            mergeBounds<<<gridFor(n, 256), 256, 0, ctx->stream>>>(ctx->runOf, ctx->state, dPasses, D, levelOf, low, high, n);
+            HIP_TRY(hipMemcpyAsync(lowHost.data(), low, lowHost.size() * sizeof(long long), hipMemcpyDeviceToHost, ctx->stream));
+            HIP_TRY(hipMemcpyAsync(highHost.data(), high, highHost.size() * sizeof(long long), hipMemcpyDeviceToHost, ctx->stream));
+            HIP_TRY(hipStreamSynchronize(ctx->stream));
+            long long cells = 0;
+            for (int32_t r = 0; r < runs; ++r) {
+                MergePass& pass = passes[r];
+                if (!pass.active) continue;
+                long long extent = 1;
+                for (int a = 0; a < D; ++a) {
+                    pass.low[a] = lowHost[r * 3 + a];
+                    pass.extent[a] = highHost[r * 3 + a] - lowHost[r * 3 + a] + 1;
+                    extent *= pass.extent[a];
+                    if (extent > (1ll << 30)) return fail(ctx, SPH_ERR_VALUE, "The merge's cells span too wide a grid.");
+                }
+                pass.cellBase = cells;
+                cells += extent;
+            }
+            if (cells >= (1ll << 30)) return fail(ctx, SPH_ERR_VALUE, "The merge's cells span too wide a grid.");
+            const int64_t cellTiles = (cells + 1 + kScanTile - 1) / kScanTile + 1;
+            status = growBuffer(ctx, ctx->mergeCells, ctx->mergeCellBytes,
+                                (size_t)(2 * (cells + 1) + cellTiles) * sizeof(int32_t) + 1024);
+            if (status != SPH_OK) return status;
+            Carver cellCarve{static_cast<char*>(ctx->mergeCells)};
+            int32_t* counts = cellCarve.take<int32_t>(cells + 1);
+            int32_t* start = cellCarve.take<int32_t>(cells + 1);
+            int32_t* cellTileSums = cellCarve.take<int32_t>(cellTiles);
+            HIP_TRY(hipMemcpyAsync(dPasses, passes.data(), runs * sizeof(MergePass), hipMemcpyHostToDevice, ctx->stream));
+            HIP_TRY(hipMemsetAsync(counts, 0, (cells + 1) * sizeof(int32_t), ctx->stream));
+            mergeKeys<<<gridFor(n, 256), 256, 0, ctx->stream>>>(ctx->runOf, ctx->state, dPasses, D, levelOf, key, distance,
+                                                               counts, n);
+            scanInto(ctx, counts, start, cellTileSums, cells);
+            fractionScatter<<<gridFor(n, 256), 256, 0, ctx->stream>>>(key, start, counts, members, n);
+            mergeOrder<<<gridFor(cells, 256), 256, 0, ctx->stream>>>(start, members, distance, cells);
+            mergeTake<<<gridFor(cells, 256), 256, 0, ctx->stream>>>(start, members, size, ctx->runOf, groupMembers,
+                                                                    groupCount, levelOf, taken, cells);
+            status = launchCheck(ctx, "merge pass");
+            if (status != SPH_OK) return status;
+            HIP_TRY(hipMemcpyAsync(takenHost.data(), taken, runs * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+            HIP_TRY(hipMemsetAsync(taken, 0, runs * sizeof(int32_t), ctx->stream));
+            HIP_TRY(hipStreamSynchronize(ctx->stream));
+            for (int32_t r = 0; r < runs; ++r) {
+                freeCount[(size_t)r * kMergeLevels + level] -= takenHost[r];
+                membersTaken[r] += takenHost[r];
+            }
+        }
+    int32_t groupTotal = 0;
+    HIP_TRY(hipMemcpyAsync(&groupTotal, groupCount, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (groupTotal == 0) return SPH_OK;
+
+    // Who leaves, and whether the holes among each run's boundary can be filled, before any change
+    markRemoved<<<gridFor(groupTotal, 256), 256, 0, ctx->stream>>>(groupMembers, groupTotal, size, removed);
+    compactFlags<<<gridFor(n, 256), 256, 0, ctx->stream>>>(ctx->runOf, lastBoundary, removed, hole, tail, n);
+    scanInto(ctx, hole, holeRank, tiles, n);
+    scanInto(ctx, tail, tailRank, tiles, n);
+    runBases<<<(runs + 64) / 64, 64, 0, ctx->stream>>>(ctx->dRuns, holeRank, runs, n, holeBase);
+    runBases<<<(runs + 64) / 64, 64, 0, ctx->stream>>>(ctx->dRuns, tailRank, runs, n, tailBase);
+    std::vector<long long> holesHost(runs + 1), tailsHost(runs + 1);
+    HIP_TRY(hipMemcpyAsync(holesHost.data(), holeBase, (runs + 1) * sizeof(long long), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(tailsHost.data(), tailBase, (runs + 1) * sizeof(long long), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    for (int32_t r = 0; r < runs; ++r)
+        if (tailsHost[r + 1] - tailsHost[r] < holesHost[r + 1] - holesHost[r])
+            return fail(ctx, SPH_ERR_VALUE, "Too few fluid particles follow the boundary to fill the places a merge leaves among it.");
+    if (ctx->precision == 0)
+        mergeGroups<float><<<gridFor(groupTotal, 256), 256, 0, ctx->stream>>>(groupMembers, groupTotal, size, ctx->runOf,
+                                                                              dSpecs, ctx->state, D, removed, energy);
+    else
+        mergeGroups<double><<<gridFor(groupTotal, 256), 256, 0, ctx->stream>>>(groupMembers, groupTotal, size, ctx->runOf,
+                                                                               dSpecs, ctx->state, D, removed, energy);
+    compactMovers<<<gridFor(n, 256), 256, 0, ctx->stream>>>(ctx->runOf, lastBoundary, removed, tail, tailRank, tailBase,
+                                                            holeBase, movers, keep, n);
+    scanInto(ctx, keep, newIndex, tiles, n);
+    sph_state from{};
+    const int32_t* identityFrom = nullptr;
+    status = snapshotState(ctx, nullptr, from, identityFrom);
+    if (status != SPH_OK) return status;
+    if (ctx->precision == 0)
+        compactPermute<float><<<gridFor(n, 256), 256, 0, ctx->stream>>>(keep, newIndex, hole, holeRank, movers, from,
+                                                                        ctx->state, D, n);
+    else
+        compactPermute<double><<<gridFor(n, 256), 256, 0, ctx->stream>>>(keep, newIndex, hole, holeRank, movers, from,
+                                                                         ctx->state, D, n);
+    status = launchCheck(ctx, "merge");
+    if (status != SPH_OK) return status;
+    std::vector<double> energyHost(runs);
+    HIP_TRY(hipMemcpyAsync(energyHost.data(), energy, runs * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    for (int32_t r = 0; r < runs; ++r) {
+        groups[r] = membersTaken[r] / size;
+        removedEnergy[r] = energyHost[r];
+    }
     return SPH_OK;
 }
 

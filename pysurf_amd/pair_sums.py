@@ -212,6 +212,36 @@ class Engine:
         self.configure(fresh)
         return chosen
 
+    def merge(self, specs: list, mask = None) -> tuple[np.ndarray, np.ndarray]:
+
+        '''
+
+        Merge the split fluid particles `specs` (one _native.MergeSpec per run) and `mask` (a uint8
+        tensor over the slots, or None) allow, 2^dims of one level at a time, on the device
+        (sph_merge; mergeParticles, sphRefinement.py:212-311), the runs closing up in place.
+        Returns each run's groups merged and kinetic energy removed. The runs' clocks restart at
+        zero; the caller sets them again. Synchronizes.
+
+        '''
+
+        runs = len(self._runs)
+        array = self._readSpecs(_native.MergeSpec, specs)
+        groups = np.zeros(runs, dtype = np.int64)
+        energy = np.zeros(runs)
+        maskPointer = ctypes.c_void_p(mask.data_ptr()) if mask is not None else None
+        self.check(self.library.sph_merge(self.handle, array, maskPointer, groups.ctypes.data_as(ctypes.c_void_p),
+                                          energy.ctypes.data_as(ctypes.c_void_p)), 'sph_merge')
+        if groups.any():
+            shrunk = groups * (2**self.dimensions - 1)
+            fresh, first = [], 0
+            for r in range(runs):
+                run = _native.Run.from_buffer_copy(self._runs[r])
+                run.first, run.count = first, int(self._runs[r].count) - int(shrunk[r])
+                first += run.count
+                fresh.append(run)
+            self.configure(fresh)
+        return groups, energy
+
     def configure(self, runs: list) -> None:
         '''Hand the runs (a list of _native.Run) to the library and bind the state tensors.'''
         count = sum(int(run.count) for run in runs)

@@ -50,6 +50,8 @@ def testStructuresMatchTheHeaderLayout():
     assert _native.FractionSpec.stripCentre.offset == 120
     assert ctypes.sizeof(_native.SplitSpec) == 32
     assert _native.SplitSpec.minMass.offset == 24
+    assert ctypes.sizeof(_native.MergeSpec) == 48
+    assert _native.MergeSpec.lengthRatio.offset == 40
 
 
 def testBadArgumentsFailWithoutADevice():
