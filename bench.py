@@ -106,6 +106,10 @@ def parse():
     parser.add_argument('--serving-wall', type = float, default = 150.0, help = 'wall seconds the serving stream may take')
     parser.add_argument('--no-serving', action = 'store_true', help = 'skip the serving stream')
     parser.add_argument('--no-gate', action = 'store_true', help = "skip Grilli's 1:35 case from the exact start")
+    parser.add_argument('--gate-only', action = 'store_true',
+                        help = "run only Grilli's case and print its record (for profiling the breaking read-out)")
+    parser.add_argument('--gate-end', type = float, default = 12.0,
+                        help = "simulated seconds Grilli's case may run before its touchdown")
     parser.add_argument('--cpu-seconds', type = float, default = 15.0)
     parser.add_argument('--no-cpu', action = 'store_true', help = 'skip the CPU baseline')
     parser.add_argument('--copy-calibration', type = int, default = 0, metavar = 'MIB',
@@ -335,7 +339,7 @@ def gateRun(args) -> dict:
     from pysurf_amd.serving import Fleet, Member
     case = exactSolitaryOnSlope(64)
     fleet = Fleet(2, 'float32', lanes = 8, listCapacity = 160, chunk = 16)
-    member = Member(key = 0, case = case, endTime = 12.0)
+    member = Member(key = 0, case = case, endTime = args.gate_end)
     start = time.perf_counter()
     fleet.rebuild([], [member])
     fleet.launch()
@@ -358,8 +362,11 @@ def gateRun(args) -> dict:
                                    'touchdown_s': 9.3740, 'wall_s': 693.1},
               'grilli_1997': {'Hb_over_h0': 0.476, 'hb_over_h0': 0.345},
               'fluid_updates_per_s': case.fluidCount * member.steps / wall}
-    log(f'gate: breaking at {result["breaking_s"]:.4f} s, Hb/h0 {result["Hb_over_h0"]:.4f}, hb/h0 '
-        f'{result["hb_over_h0"]:.4f}, touchdown {tracker.touchdownTime:.4f} s; {member.steps} steps in {wall:.1f} s')
+    if point is not None:
+        log(f'gate: breaking at {point.time:.4f} s, Hb/h0 {point.crestHeight:.4f}, hb/h0 {point.depth:.4f}, '
+            f'touchdown {tracker.touchdownTime:.4f} s; {member.steps} steps in {wall:.1f} s')
+    else:
+        log(f'gate: no breaking point by {member.time:.3f} s; {member.steps} steps in {wall:.1f} s')
     return result
 
 
@@ -448,6 +455,9 @@ def main() -> int:
         return 1
     if args.copy_calibration:
         return copyCalibration(args.copy_calibration)
+    if args.gate_only:
+        print(json.dumps(gateRun(args)))
+        return 0
 
     import numpy as np
 

@@ -53,6 +53,12 @@ template <int D, typename Real>
 constexpr int sweepWaves() { return (D == 2 && sizeof(Real) == 4) ? SPH_WAVES_PER_EU : 1; }
 template <int D, typename Real>
 constexpr int accelerationWaves() { return (D == 2 && sizeof(Real) == 4) ? SPH_WAVES_ACCELERATIONS : 1; }
+#ifndef SPH_NEAR_BYTES_3D
+#define SPH_NEAR_BYTES_3D 32768
+#endif
+#ifndef SPH_NEAR_PER_3D
+#define SPH_NEAR_PER_3D 12
+#endif
 #define SWEEP_BOUNDS(waves) __attribute__((amdgpu_flat_work_group_size(1, 256), amdgpu_waves_per_eu(waves)))
 
 // Whether the sweeps copy the records around their particles into LDS (NearCells).
@@ -281,24 +287,26 @@ __global__ void cellScatter(const int32_t* __restrict__ key, const int32_t* __re
     if (valid) order[start[k] + base + rank] = (int32_t)i;
 }
 
-// Orders each cell's slots ascending, the order a stable counting sort gives, and clears the
-// cell's counters for the next build.
-__global__ void cellOrder(const int32_t* __restrict__ start, int32_t* __restrict__ order, int32_t* __restrict__ count,
-                          int32_t* __restrict__ fill, int64_t cells) {
-    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= cells) return;
-    const int32_t begin = start[c], end = start[c + 1];
-    for (int32_t a = begin + 1; a < end; ++a) {
-        const int32_t v = order[a];
-        int32_t b = a - 1;
-        while (b >= begin && order[b] > v) {
-            order[b + 1] = order[b];
-            --b;
-        }
-        order[b + 1] = v;
+// Orders each cell's slots ascending, the order a stable counting sort gives: the slot at each
+// scattered place goes to its cell's start plus the number of the cell's slots below it, one
+// thread per place, so a cell of k slots costs k reads per slot rather than a serial sort. The
+// first `cells` threads also clear the cells' counters for the next build.
+__global__ void cellRank(const int32_t* __restrict__ start, const int32_t* __restrict__ key,
+                         const int32_t* __restrict__ scattered, int32_t* __restrict__ order, int32_t* __restrict__ count,
+                         int32_t* __restrict__ fill, int64_t n, int64_t cells) {
+    const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a < n) {
+        const int32_t v = scattered[a];
+        const int32_t c = key[v];
+        const int32_t begin = start[c], end = start[c + 1];
+        int32_t below = 0;
+        for (int32_t b = begin; b < end; ++b) below += scattered[b] < v;
+        order[begin + below] = v;
     }
-    count[c] = 0;
-    fill[c] = 0;
+    if (a < cells) {
+        count[a] = 0;
+        fill[a] = 0;
+    }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1926,6 +1934,7 @@ struct sph_context {
     int32_t* runOf = nullptr;
     int32_t* key = nullptr;
     int32_t* order = nullptr;
+    int32_t* scattered = nullptr;   // the slots as the counting sort places them, before ordering
     int32_t* cellCount = nullptr;
     int32_t* cellFill = nullptr;
     int32_t* cellStart = nullptr;
@@ -2139,8 +2148,8 @@ template <int D, typename Real>
 int32_t nearRecords(const sph_context* ctx) {
     if (!SPH_NEAR_CELLS) return 0;
     const int32_t per = (int32_t)ctx->tilePer;
-    const int32_t want = D == 2 ? 4 * per + 64 : 12 * per + 128;
-    const int32_t most = (int32_t)(32768 / sizeof(Record<Real>));
+    const int32_t want = D == 2 ? 4 * per + 64 : SPH_NEAR_PER_3D * per + 128;
+    const int32_t most = (int32_t)((D == 2 ? 32768 : SPH_NEAR_BYTES_3D) / sizeof(Record<Real>));
     return want < most ? want : most;
 }
 // A sweep's dynamic LDS: the NearCells copy, then `sums` values per particle of the workgroup.
@@ -2232,9 +2241,9 @@ int buildCells(sph_context* ctx, int clockMode = 0) {
     }
     if (n > 0)
         TIMED(ctx, kCellScatter, (cellScatter<<<gridFor(n, b), b, 0, ctx->stream>>>(ctx->key, ctx->cellStart,
-                                                                                    ctx->cellFill, ctx->order, n)));
-    TIMED(ctx, kCellOrder, (cellOrder<<<gridFor(m, b), b, 0, ctx->stream>>>(ctx->cellStart, ctx->order, ctx->cellCount,
-                                                                            ctx->cellFill, m)));
+                                                                                    ctx->cellFill, ctx->scattered, n)));
+    TIMED(ctx, kCellOrder, (cellRank<<<gridFor(std::max<int64_t>(n, m), b), b, 0, ctx->stream>>>(
+                               ctx->cellStart, ctx->key, ctx->scattered, ctx->order, ctx->cellCount, ctx->cellFill, n, m)));
     return launchCheck(ctx, "cell build");
 }
 
@@ -2614,6 +2623,7 @@ int sph_configure(sph_context* ctx, const sph_run* runs, int32_t count, int64_t 
     if (status == SPH_OK) status = allocate(ctx, &ctx->runOf, cap);
     if (status == SPH_OK) status = allocate(ctx, &ctx->key, cap);
     if (status == SPH_OK) status = allocate(ctx, &ctx->order, cap);
+    if (status == SPH_OK) status = allocate(ctx, &ctx->scattered, cap);
     if (status == SPH_OK) status = allocate(ctx, &ctx->cellCount, cells);
     if (status == SPH_OK) status = allocate(ctx, &ctx->cellFill, cells);
     if (status == SPH_OK) status = allocate(ctx, &ctx->cellStart, cells + 1);
