@@ -338,7 +338,7 @@ def gateRun(args) -> dict:
     from pysurf_amd.scenarios import exactSolitaryOnSlope
     from pysurf_amd.serving import Fleet, Member
     case = exactSolitaryOnSlope(64)
-    fleet = Fleet(2, 'float32', lanes = 8, listCapacity = 160, chunk = 16)
+    fleet = Fleet(2, 'float32', lanes = 4, listCapacity = 160, chunk = 8)
     member = Member(key = 0, case = case, endTime = args.gate_end)
     start = time.perf_counter()
     fleet.rebuild([], [member])

@@ -1345,23 +1345,43 @@ __global__ void __launch_bounds__(256)
             partial[((int64_t)r * gridDim.y + blockIdx.y) * kLandward + m] = make_double2(px[m], pz[m]);
 }
 
-// Each run's slices merged into its kLandward points, (x, z) by descending x, NaN past the last.
-__global__ void mergeLandward(const double2* __restrict__ partial, int32_t slices, int32_t runs,
-                              double* __restrict__ landward) {
-    const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= runs) return;
+// Each run's slices merged into its kLandward points, (x, z) by descending x, NaN past the last: one
+// 64-thread workgroup per run, each thread merging every 64th slice, then the threads pairwise.
+__global__ void __launch_bounds__(64)
+    mergeLandward(const double2* __restrict__ partial, int32_t slices, double* __restrict__ landward) {
+    __shared__ double sx[64][kLandward], sz[64][kLandward];
+    const int32_t r = blockIdx.x;
     double px[kLandward], pz[kLandward];
 #pragma unroll
     for (int m = 0; m < kLandward; ++m) px[m] = pz[m] = -INFINITY;
-    for (int64_t k = 0; k < (int64_t)slices * kLandward; ++k) {
+    for (int64_t k = threadIdx.x; k < (int64_t)slices * kLandward; k += blockDim.x) {
         const double2 p = partial[(int64_t)r * slices * kLandward + k];
         keepLandward(px, pz, p.x, p.y);
     }
+#pragma unroll
     for (int m = 0; m < kLandward; ++m) {
-        const bool kept = px[m] > -INFINITY;
-        landward[(int64_t)r * 2 * kLandward + 2 * m] = kept ? px[m] : NAN;
-        landward[(int64_t)r * 2 * kLandward + 2 * m + 1] = kept ? pz[m] : NAN;
+        sx[threadIdx.x][m] = px[m];
+        sz[threadIdx.x][m] = pz[m];
     }
+    __syncthreads();
+    for (unsigned half = blockDim.x / 2; half > 0; half /= 2) {
+        if (threadIdx.x < half) {
+#pragma unroll
+            for (int m = 0; m < kLandward; ++m) keepLandward(px, pz, sx[threadIdx.x + half][m], sz[threadIdx.x + half][m]);
+#pragma unroll
+            for (int m = 0; m < kLandward; ++m) {
+                sx[threadIdx.x][m] = px[m];
+                sz[threadIdx.x][m] = pz[m];
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int m = 0; m < kLandward; ++m) {
+            const bool kept = px[m] > -INFINITY;
+            landward[(int64_t)r * 2 * kLandward + 2 * m] = kept ? px[m] : NAN;
+            landward[(int64_t)r * 2 * kLandward + 2 * m + 1] = kept ? pz[m] : NAN;
+        }
 }
 
 // The fluid fraction. Each run's tracked particles inside its window are sorted into bins of half a
@@ -1419,27 +1439,47 @@ __global__ void fractionOrder(const int32_t* __restrict__ start, int32_t* __rest
     }
 }
 
+constexpr int kFractionTile = 16;
+
+// The LDS a fractionNodes workgroup takes: the bin starts of its tile's reach.
+inline size_t fractionLds(int32_t reach) {
+    return (size_t)(kFractionTile + 2 * reach + 1) * (kFractionTile + 2 * reach + 2) * sizeof(int32_t);
+}
+
 // phi at each node of one run: the bins from `reach` + 1 below to `reach` above the node's own,
-// rows outer; a bin's particles summed apart, then added.
+// rows outer; a bin's particles summed apart, then added. A workgroup takes a tile of 16 x 16 nodes
+// and first copies the starts of the bins they reach into LDS, so a node's 196 bin lookups stay on
+// the chip.
 template <int D, typename Real>
 __global__ void __launch_bounds__(256)
     fractionNodes(sph_state st, const sph_fraction_spec* __restrict__ specs, int32_t run, int64_t binOffset,
                   const int32_t* __restrict__ start, const int32_t* __restrict__ members, double* __restrict__ fraction) {
 #pragma clang fp contract(off)
+    extern __shared__ int32_t binStart[];
     const sph_fraction_spec& spec = specs[run];
-    const int64_t node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (node >= (int64_t)spec.nx * spec.ny) return;
-    const int32_t i = (int32_t)(node % spec.nx), j = (int32_t)(node / spec.nx);
+    const int32_t R = spec.reach;
+    const int32_t width = spec.nx + 2 * R + 2, height = spec.ny + 2 * R + 2;
+    const int32_t tilesX = (spec.nx + kFractionTile - 1) / kFractionTile;
+    const int32_t i0 = (int32_t)(blockIdx.x % tilesX) * kFractionTile, j0 = (int32_t)(blockIdx.x / tilesX) * kFractionTile;
+    const int32_t rows = kFractionTile + 2 * R + 1, cols = kFractionTile + 2 * R + 2;
+    // Bin (row, column) of the run's grid at local (r, c); a column one past a row's last bin is the
+    // next row's first, whose start ends the row
+    for (int32_t k = threadIdx.x; k < rows * cols; k += blockDim.x) {
+        const int32_t row = j0 + k / cols, column = i0 + k % cols;
+        binStart[k] = row < height && column <= width ? start[binOffset + (int64_t)row * width + column] : 0;
+    }
+    __syncthreads();
+    const int32_t i = i0 + (int32_t)(threadIdx.x % kFractionTile), j = j0 + (int32_t)(threadIdx.x / kFractionTile);
+    if (i >= spec.nx || j >= spec.ny) return;
     const double gx = spec.x0 + (double)i * spec.stepX, gz = spec.y0 + (double)j * spec.stepY;
-    const int32_t width = spec.nx + 2 * spec.reach + 2;
     const double h = spec.h, sigma = 10.0 / (7.0 * M_PI * h * h);
     const Real* mass = static_cast<const Real*>(st.mass);
     double phi = 0.0;
-    for (int32_t oy = -spec.reach; oy <= spec.reach + 1; ++oy)
-        for (int32_t ox = -spec.reach; ox <= spec.reach + 1; ++ox) {
-            // Particles whose bin is (i - ox, j - oy), shifted into the bin grid
-            const int64_t b = binOffset + (int64_t)(j - oy + spec.reach + 1) * width + (i - ox + spec.reach + 1);
-            const int32_t first = start[b], last = start[b + 1];
+    for (int32_t oy = -R; oy <= R + 1; ++oy)
+        for (int32_t ox = -R; ox <= R + 1; ++ox) {
+            // Particles whose bin is (i - ox, j - oy), shifted into the bin grid, then into the tile's
+            const int32_t local = (j - oy + R + 1 - j0) * cols + (i - ox + R + 1 - i0);
+            const int32_t first = binStart[local], last = binStart[local + 1];
             if (first == last) continue;
             double partial = 0.0;
             for (int32_t k = first; k < last; ++k) {
@@ -1459,7 +1499,7 @@ __global__ void __launch_bounds__(256)
             }
             phi += partial;
         }
-    fraction[spec.offset + node] = phi;
+    fraction[spec.offset + (int64_t)j * spec.nx + i] = phi;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -3307,10 +3347,10 @@ int sph_face_cells(sph_context* ctx, const sph_face_spec* specs, uint32_t* occup
     const size_t wordBytes = aligned(std::max<int64_t>(words, 1) * sizeof(uint32_t));
     const size_t flagBytes = aligned(runs * sizeof(int32_t));
     const size_t pointBytes = aligned((size_t)runs * 2 * kLandward * sizeof(double));
-    // Slices of at most 16384 slots per workgroup, at most 64 per run
+    // Slices of at most 2048 slots per workgroup, at most 512 per run
     int64_t longest = 0;
     for (int32_t r = 0; r < runs; ++r) longest = std::max<int64_t>(longest, ctx->runs[r].count);
-    const int32_t slices = (int32_t)std::min<int64_t>(64, std::max<int64_t>(1, (longest + 16383) / 16384));
+    const int32_t slices = (int32_t)std::min<int64_t>(512, std::max<int64_t>(1, (longest + 2047) / 2048));
     const int64_t chunk = std::max<int64_t>(1, (longest + slices - 1) / slices);
     const size_t partialBytes = aligned((size_t)runs * slices * kLandward * sizeof(double2));
     status = readBuffers(ctx, specBytes + wordBytes + flagBytes + pointBytes + partialBytes);
@@ -3336,7 +3376,7 @@ int sph_face_cells(sph_context* ctx, const sph_face_spec* specs, uint32_t* occup
         landwardPoints<2><<<grid, 256, 0, ctx->stream>>>(ctx->dRuns, ctx->state, dSpecs, chunk, dPartial);
     else
         landwardPoints<3><<<grid, 256, 0, ctx->stream>>>(ctx->dRuns, ctx->state, dSpecs, chunk, dPartial);
-    mergeLandward<<<(runs + 63) / 64, 64, 0, ctx->stream>>>(dPartial, slices, runs, dPoints);
+    mergeLandward<<<(unsigned)runs, 64, 0, ctx->stream>>>(dPartial, slices, dPoints);
     status = launchCheck(ctx, "face cells");
     if (status != SPH_OK) return status;
     if (words > 0)
@@ -3359,9 +3399,9 @@ int sph_fluid_fraction(sph_context* ctx, const sph_fraction_spec* specs, double*
         const sph_fraction_spec& spec = specs[r];
         binOffset[r] = bins;
         if (spec.nx <= 0) continue;
-        if (spec.ny <= 0 || spec.reach < 0 || spec.reach > 64 || spec.offset < 0 || !(spec.binWidth > 0.0) ||
+        if (spec.ny <= 0 || spec.reach < 0 || spec.reach > 16 || spec.offset < 0 || !(spec.binWidth > 0.0) ||
             !(spec.h > 0.0) || (int64_t)spec.nx * spec.ny > (1ll << 28))
-            return fail(ctx, SPH_ERR_VALUE, "A fraction grid needs nodes, a bin width, a smoothing length and a reach of at most 64 bins.");
+            return fail(ctx, SPH_ERR_VALUE, "A fraction grid needs nodes, a bin width, a smoothing length and a reach of at most 16 bins.");
         bins += (int64_t)(spec.nx + 2 * spec.reach + 2) * (spec.ny + 2 * spec.reach + 2);
         nodes = std::max<int64_t>(nodes, spec.offset + (int64_t)spec.nx * spec.ny);
     }
@@ -3405,22 +3445,23 @@ int sph_fluid_fraction(sph_context* ctx, const sph_fraction_spec* specs, double*
     for (int32_t r = 0; r < runs; ++r) {
         const sph_fraction_spec& spec = specs[r];
         if (spec.nx <= 0) continue;
-        const int64_t count = (int64_t)spec.nx * spec.ny;
-        const unsigned grid = gridFor(count, 256);
+        const unsigned grid = (unsigned)(((spec.nx + kFractionTile - 1) / kFractionTile) *
+                                         ((spec.ny + kFractionTile - 1) / kFractionTile));
+        const size_t lds = fractionLds(spec.reach);
         if (ctx->dims == 2) {
             if (ctx->precision == 0)
-                fractionNodes<2, float><<<grid, 256, 0, ctx->stream>>>(ctx->state, dSpecs, r, binOffset[r], dStart,
-                                                                        dMembers, dFraction);
+                fractionNodes<2, float><<<grid, 256, lds, ctx->stream>>>(ctx->state, dSpecs, r, binOffset[r], dStart,
+                                                                          dMembers, dFraction);
             else
-                fractionNodes<2, double><<<grid, 256, 0, ctx->stream>>>(ctx->state, dSpecs, r, binOffset[r], dStart,
-                                                                         dMembers, dFraction);
+                fractionNodes<2, double><<<grid, 256, lds, ctx->stream>>>(ctx->state, dSpecs, r, binOffset[r], dStart,
+                                                                           dMembers, dFraction);
         } else {
             if (ctx->precision == 0)
-                fractionNodes<3, float><<<grid, 256, 0, ctx->stream>>>(ctx->state, dSpecs, r, binOffset[r], dStart,
-                                                                        dMembers, dFraction);
+                fractionNodes<3, float><<<grid, 256, lds, ctx->stream>>>(ctx->state, dSpecs, r, binOffset[r], dStart,
+                                                                          dMembers, dFraction);
             else
-                fractionNodes<3, double><<<grid, 256, 0, ctx->stream>>>(ctx->state, dSpecs, r, binOffset[r], dStart,
-                                                                         dMembers, dFraction);
+                fractionNodes<3, double><<<grid, 256, lds, ctx->stream>>>(ctx->state, dSpecs, r, binOffset[r], dStart,
+                                                                           dMembers, dFraction);
         }
     }
     status = launchCheck(ctx, "fluid fraction");

@@ -30,7 +30,7 @@ def testTheGrilliCaseBreaksWhereTheReferenceRecordedIt():
     recorded, grilli = golden['recorded'], golden['grilli1997']
     case = exactSolitaryOnSlope(64)
     assert case.fluidCount == recorded['fluidParticles']
-    fleet = Fleet(2, 'float32', lanes = 8, listCapacity = 160, chunk = 16)
+    fleet = Fleet(2, 'float32', lanes = 4, listCapacity = 160, chunk = 8)
     member = Member(key = 0, case = case, endTime = 12.0)
     fleet.rebuild([], [member])
     fleet.launch()
