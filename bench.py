@@ -76,7 +76,7 @@ def kernelBytes(dimensions: int, real: int, entries: float) -> dict:
     return {
         'cellKeys': 8 * d + 4 + 4 + 4,           # position; run id; key; the cell count's atomic
         'cellScatter': 4 + 4 + 4 + 4,            # key; cell start; the fill atomic; order
-        'cellOrder': 8,                          # the order read and written in place
+        'cellRank': 4 + 4 + 4,                   # the scattered slot; its key; the ordered slot
         'gatherRecords': ids + 16 * d + 8 + 2 * r + 1 + 4 + record + 4 + r,
         'buildNeighbors': ids + record + 4 + 4 + 4 * k + d * d * r,
         'sweepAccelerations': ids + record + 4 + d * d * r + 4 + 4 * k + 2 * d * r + d * d * r,

@@ -274,7 +274,7 @@ int sph_synchronize(sph_context* ctx);
  * step (no counterpart in the reference, which times whole runs only). sph_profile(1) clears the
  * totals and starts recording (steps then run as stream launches, not graphs); sph_kernel_times
  * synchronizes and copies, per kernel id, the accumulated milliseconds and launch counts:
- * 0 clock, 1 cell keys, 2 cell-count scan, 3 cell scatter, 4 cell order, 5 record gather,
+ * 0 clock, 1 cell keys, 2 cell-count scan, 3 cell scatter, 4 cell order (the rank within cells), 5 record gather,
  * 6 sweep 1 (wall pressure, G), 7 sweep 2 (accelerations, L), 8 unused (the kick is part of
  * sweep 3), 9 sweep 3 (kick, continuity, density advance, drift, enforcement), 10 neighbor list
  * (with sweep 1), 11 enforcement alone, 12 the runs' maxima. */

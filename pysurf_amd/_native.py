@@ -133,7 +133,7 @@ signatures = {
 globalSumNames = ['mass', 'kinetic', 'potential', 'elastic', 'momentumX', 'momentumY', 'momentumZ']
 
 # Kernel ids of sph_kernel_times
-kernelNames = ['clock', 'cellKeys', 'cellScan', 'cellScatter', 'cellOrder', 'gatherRecords', 'sweepWallAndGradients',
+kernelNames = ['clock', 'cellKeys', 'cellScan', 'cellScatter', 'cellRank', 'gatherRecords', 'sweepWallAndGradients',
                'sweepAccelerations', 'kick', 'sweepContinuity', 'buildNeighbors', 'enforce', 'maxima']
 
 _library = None
