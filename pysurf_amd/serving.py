@@ -32,11 +32,12 @@ from pysurf_amd.batch import BatchEngine
 from pysurf_amd.breaking import BreakingTracker, trackInterval
 
 # HBM per particle the admission counts: state 81 B (f64 position, velocity, density; f32 pressure,
-# acceleration, mass, length; a flag), the sorted record and its key, G and L, and the neighbor list
+# acceleration, mass, length; a flag), the sorted record and its key, G and L, the neighbor list,
+# the run id, cell key, order and scattered order, and the resident accelerations
 def bytesPerParticle(dimensions: int, listCapacity: int) -> int:
     d = dimensions
     state = 8 * d * 2 + 8 + 4 + 4 * d + 4 + 4 + 1
-    return state + 64 + 4 + 8 * d * d + 4 * (listCapacity + 1) + 12
+    return state + 64 + 4 + 8 * d * d + 4 * (listCapacity + 1) + 16 + 4 * d
 
 
 def trackerOf(case) -> BreakingTracker:
@@ -334,7 +335,7 @@ class Server:
 
     budgetBytes: float = 200e9
     precision: str = 'float32'
-    lanes: int = 2
+    lanes: int = 4
     chunk: int = 8
     listCapacity2D: int = 64
     listCapacity3D: int = 768

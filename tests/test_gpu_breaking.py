@@ -76,7 +76,7 @@ def testARunServedInAMixIsTheRunServedAlone():
     served = {record.key: record for record in server.serve(requests)}
     assert sorted(served) == [0, 1, 2]
     for k, (case, end) in enumerate(zip(cases, ends)):
-        alone = runToBreaking([case], 'float32', endTime = end, lanes = 2,
+        alone = runToBreaking([case], 'float32', endTime = end, lanes = server.lanes,
                               listCapacity = 64 if case.dimensions == 2 else 160)[0]
         mixed = served[k].tracker
         print(f'run {k}: {len(alone.times)} readings alone, {len(mixed.times)} in the mix', flush = True)
