@@ -183,8 +183,10 @@ __device__ __forceinline__ int64_t cellCoordinate(double v, double cell) {
     return (int64_t)c;
 }
 
-// Calls g(begin, end) for the sorted ranges of the 3x3 (2D) or 3x3x3 (3D) cells around cell
-// (cx, cy, cz), x fastest, the three cells of an unwrapped row as one range.
+// Calls g(begin, end, middle, right) for the sorted ranges of the 3x3 (2D) or 3x3x3 (3D) cells
+// around cell (cx, cy, cz), x fastest, the three cells of an unwrapped row as one range: [begin,
+// middle) its left cell, [middle, right) its middle one and [right, end) its right one, each
+// ascending in x. A cell taken alone comes as g(begin, end, begin, end).
 template <int D, class G>
 __device__ __forceinline__ void forStencil(const RunDev& run, const int32_t* __restrict__ start, int cx, int cy, int cz,
                                            G&& g) {
@@ -199,17 +201,45 @@ __device__ __forceinline__ void forStencil(const RunDev& run, const int32_t* __r
         for (int b = 0; b < ky; ++b) {
             const int64_t row = run.cellFirst + (plane * ny + wrapNear(cy + oy[b], ny)) * nx;
             if (nx >= 3 && cx >= 1 && cx <= nx - 2) {
-                g(start[row + cx - 1], start[row + cx + 2]);
+                g(start[row + cx - 1], start[row + cx + 2], start[row + cx], start[row + cx + 1]);
             } else {
                 int ox[3];
                 const int kx = stencilOffsets(nx, ox);
                 for (int c = 0; c < kx; ++c) {
                     const int64_t cellIndex = row + wrapNear(cx + ox[c], nx);
-                    g(start[cellIndex], start[cellIndex + 1]);
+                    const int32_t begin = start[cellIndex], end = start[cellIndex + 1];
+                    g(begin, end, begin, end);
                 }
             }
         }
     }
+}
+
+// The candidates of a 3D row that can be neighbors: the left cell's from the first whose point x is
+// at least x_i - reach, the right cell's up to the last at most x_i + reach (each cell ascending in
+// x), the middle cell whole. `reach` is 2 h_max with a margin over the points' rounding, so every
+// pair with r < h_i + h_j stays.
+template <typename Real>
+__device__ __forceinline__ Real rowReach(Real twoHmax, Real px) {
+    return twoHmax * Real(1.0 + 1.0 / 1024.0) + fabs(px) * Real(1.0 / 2097152.0);
+}
+
+template <typename Real, class X>
+__device__ __forceinline__ int32_t firstAtLeast(int32_t lo, int32_t hi, Real bound, X&& x) {
+    while (lo < hi) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (x(mid) < bound) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+template <typename Real, class X>
+__device__ __forceinline__ int32_t firstAbove(int32_t lo, int32_t hi, Real bound, X&& x) {
+    while (lo < hi) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (x(mid) <= bound) lo = mid + 1; else hi = mid;
+    }
+    return lo;
 }
 
 template <int D, typename Real>
@@ -390,6 +420,7 @@ template <typename Real>
 struct GlobalRecords {
     const Record<Real>* r;
     __device__ __forceinline__ Record<Real> operator()(int32_t k) const { return r[k]; }
+    __device__ __forceinline__ Real pointX(int32_t k) const { return r[k].q0.x; }
 };
 
 // A sweep workgroup's dynamic LDS, which holds its NearCells copy, and a record there as an LDS
@@ -527,6 +558,15 @@ struct NearCells {
         return base >= 0 && base + (end - begin) <= held ? base : -1;
     }
 
+    // The point x of sorted particle k, from LDS when copied there.
+    __device__ __forceinline__ Real pointX(int32_t k) const {
+        int32_t index = -1;
+#pragma unroll
+        for (int i = 0; i < kRows; ++i)
+            if (k >= lo[i] && k < hi[i]) index = at[i] + (k - lo[i]);
+        return index >= 0 && index < held ? ldsValue<Real>(index, 0, 0) : r[k].q0.x;
+    }
+
     // The record of sorted particle k, from LDS when copied there.
     __device__ __forceinline__ Record<Real> operator()(int32_t k) const {
         const LdsVec4<Real>* shared = (const LdsVec4<Real>*)sweepShared;
@@ -573,8 +613,12 @@ __device__ __forceinline__ void forNeighbors(const RunDev& run, const int32_t* _
             shift = me.p[1] < Real(0.5) * period ? period : -period;
             oy = wrapIndex(cellCoordinate(y64 + (double)shift, run.cell), run.ny);
         }
-        forStencil<D>(run, start, cx, oy, cz, [&](int32_t begin, int32_t end) {
-            for (int32_t k = begin + lane; k < end; k += Lanes) {
+        const Real reach = rowReach(twoHmax, me.p[0]);
+        forStencil<D>(run, start, cx, oy, cz, [&](int32_t begin, int32_t end, int32_t middle, int32_t right) {
+            auto pointX = [&](int32_t k) { return src.pointX(k); };
+            const int32_t from = D == 3 ? firstAtLeast(begin, middle, me.p[0] - reach, pointX) : begin;
+            const int32_t to = D == 3 ? firstAbove(right, end, me.p[0] + reach, pointX) : end;
+            for (int32_t k = from + lane; k < to; k += Lanes) {
                 if (k == s) continue;
                 const Record<Real> other = src(k);
                 Real dx, dy, dz;

@@ -287,11 +287,13 @@ __global__ void cellScatter(const int32_t* __restrict__ key, const int32_t* __re
     if (valid) order[start[k] + base + rank] = (int32_t)i;
 }
 
-// Orders each cell's slots ascending, the order a stable counting sort gives: the slot at each
-// scattered place goes to its cell's start plus the number of the cell's slots below it, one
-// thread per place, so a cell of k slots costs k reads per slot rather than a serial sort. The
-// first `cells` threads also clear the cells' counters for the next build.
-__global__ void cellRank(const int32_t* __restrict__ start, const int32_t* __restrict__ key,
+// Orders each cell's slots by x, ties by slot, or by slot alone when `x` is null: the slot at each
+// scattered place goes to its cell's start plus the number of the cell's slots before it, one
+// thread per place, so a cell of k slots costs k reads per slot rather than a serial sort.
+// Ascending x lets a 3D walk bound its candidates along x by a binary search (firstAtLeast,
+// firstAbove); a 2D cell holds too few for that to pay, and keeps slot order. The first `cells`
+// threads also clear the cells' counters for the next build.
+__global__ void cellRank(const int32_t* __restrict__ start, const int32_t* __restrict__ key, const double* __restrict__ x,
                          const int32_t* __restrict__ scattered, int32_t* __restrict__ order, int32_t* __restrict__ count,
                          int32_t* __restrict__ fill, int64_t n, int64_t cells) {
     const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -300,7 +302,16 @@ __global__ void cellRank(const int32_t* __restrict__ start, const int32_t* __res
         const int32_t c = key[v];
         const int32_t begin = start[c], end = start[c + 1];
         int32_t below = 0;
-        for (int32_t b = begin; b < end; ++b) below += scattered[b] < v;
+        if (x) {
+            const double xv = x[v];
+            for (int32_t b = begin; b < end; ++b) {
+                const int32_t u = scattered[b];
+                const double xu = x[u];
+                below += xu < xv || (xu == xv && u < v);
+            }
+        } else {
+            for (int32_t b = begin; b < end; ++b) below += scattered[b] < v;
+        }
         order[begin + below] = v;
     }
     if (a < cells) {
@@ -652,14 +663,20 @@ __global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
                 oy = wrapIndex(cellCoordinate(yy + (double)shift, run.cell), run.ny);
             }
             const int32_t tag = image ? kImageBit : 0;
-            forStencil<D>(run, start, cx, oy, cz, [&](int32_t begin, int32_t end) {
+            const Real reach = rowReach(twoHmax, me.p[0]);
+            forStencil<D>(run, start, cx, oy, cz, [&](int32_t begin, int32_t end, int32_t middle, int32_t right) {
                 const int32_t copied = cells.rangeBase(begin, end);
-                for (int32_t base = begin; base < end; base += Lanes) {
+                auto pointX = [&](int32_t k) {
+                    return copied >= 0 ? ldsValue<Real>(copied + (k - begin), 0, 0) : rec.r[k].q0.x;
+                };
+                const int32_t from = D == 3 ? firstAtLeast(begin, middle, me.p[0] - reach, pointX) : begin;
+                const int32_t to = D == 3 ? firstAbove(right, end, me.p[0] + reach, pointX) : end;
+                for (int32_t base = from; base < to; base += Lanes) {
                     const int32_t k = base + lane;
                     bool hit = false;
                     Real dx, dy, dz, r2;
                     Record<Real> other;
-                    if (k < end && k != s) {
+                    if (k < to && k != s) {
                         // the fields the test and sweep 1 read, loaded together
                         const bool wide = D == 3 || WithSweep1 || !uniform;
                         if (copied >= 0) {
@@ -2283,7 +2300,8 @@ int buildCells(sph_context* ctx, int clockMode = 0) {
         TIMED(ctx, kCellScatter, (cellScatter<<<gridFor(n, b), b, 0, ctx->stream>>>(ctx->key, ctx->cellStart,
                                                                                     ctx->cellFill, ctx->scattered, n)));
     TIMED(ctx, kCellOrder, (cellRank<<<gridFor(std::max<int64_t>(n, m), b), b, 0, ctx->stream>>>(
-                               ctx->cellStart, ctx->key, ctx->scattered, ctx->order, ctx->cellCount, ctx->cellFill, n, m)));
+                               ctx->cellStart, ctx->key, D == 3 ? P : nullptr, ctx->scattered, ctx->order, ctx->cellCount,
+                               ctx->cellFill, n, m)));
     return launchCheck(ctx, "cell build");
 }
 
