@@ -100,3 +100,37 @@ def testTheSingle64ppdRunBreaksInFloat32WhereItBreaksInFloat64():
     print(f'means: float64 {double}, float32 {single}', flush = True)
     assert single[0] == pytest.approx(double[0], rel = 0.01)
     assert single[1] == pytest.approx(double[1], rel = 0.01)
+
+
+CREST = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'golden', 'crest_c1_16ppd.json')
+
+
+@pytest.mark.timeout(300)
+def testTheCrestTravelsAsInTheOracle():
+    # The crest's speed, fitted to its track over 0.5-3 s (the flat bed, the toe at 1.1 s and the
+    # slope), and its mean height there, in float32 pairs and the plain float64 kernel, against the
+    # float64 restatement's track (tests/golden/crest_c1_16ppd.json, tests/golden/make_crest.py):
+    # speed within 1% over the whole window and 2% over its last second, height within 2%
+    from pysurf_amd.scenarios import solitaryOnSlope
+    from pysurf_amd.serving import runToBreaking
+    expected = json.load(open(CREST))
+
+    def measures(times, crestX, crestHeight) -> list:
+        t, x, h = np.asarray(times), np.asarray(crestX), np.asarray(crestHeight)
+        out = []
+        for low, high in ((0.5, 3.0), (2.0, 3.0)):
+            window = (t >= low) & (t <= high)
+            out.append(float(np.polyfit(t[window], x[window], 1)[0]))
+        window = (t >= 0.5) & (t <= 3.0)
+        out.append(float(h[window].mean()))
+        return out
+
+    oracleSpeed, oracleLate, oracleHeight = measures(expected['time'], expected['crestX'], expected['crestHeight'])
+    for precision in ('float64', 'float32'):
+        tracker = runToBreaking([solitaryOnSlope(particlesPerDepth = 16)], precision, endTime = 3.0)[0]
+        speed, late, height = measures(tracker.times, tracker.crestX, tracker.crestHeight)
+        print(f'{precision}: crest speed {speed:.4f} (oracle {oracleSpeed:.4f}), last second {late:.4f} '
+              f'({oracleLate:.4f}), mean height {height:.4f} ({oracleHeight:.4f})', flush = True)
+        assert speed == pytest.approx(oracleSpeed, rel = 0.01)
+        assert late == pytest.approx(oracleLate, rel = 0.02)
+        assert height == pytest.approx(oracleHeight, rel = 0.02)
