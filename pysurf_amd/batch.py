@@ -194,7 +194,8 @@ class BatchEngine:
         sphRefinement.py:135-186). Each run grows in place: its own particles keep their slots
         relative to its start, the other daughters follow them, mother by mother. The clocks carry
         over. Returns how many particles of each run split. The batch must be in slot order (not
-        sorted).
+        sorted). `load` afterwards restores the cases' own particles, and a checkpoint of the grown
+        runs restores only into a batch of the same grown runs.
 
         '''
 

@@ -91,8 +91,9 @@ class DeviceReadout:
 
     '''
 
-    Reads trackers' measures from a BatchEngine's state. `beds` keeps each tracker bed's polyline
-    on the device, uploaded once per bed; `fallbacks` counts the runs measured on the host.
+    Reads trackers' measures from a BatchEngine's state. `beds` keeps the polyline of each bed the
+    last reading's trackers read on the device, uploaded once while it stays in use; `fallbacks`
+    counts the runs measured on the host.
 
     An entry is (run, tracker, stripCentre, stripHalfWidth, areaPerMass): the run's index in the
     batch, its BreakingTracker, the strip a 3D run's tracker reads, and a particle's area in the
@@ -198,6 +199,9 @@ class DeviceReadout:
         if host:
             self.fallbacks += len(host)
             measures.update(hostMeasures(batch, [entry for entry in entries if entry[0] in host]))
+        # Only the beds of the trackers still read stay on the device
+        reading = {id(entry[1].bathymetry) for entry in entries}
+        self.beds = {key: bed for key, bed in self.beds.items() if key in reading}
         return measures
 
 
