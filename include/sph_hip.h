@@ -114,6 +114,13 @@ int sph_pair_accelerations(sph_context* ctx, const double* velocity, const doubl
                            const void* pressure, void* out);                      /* sphGpu.py:2044 */
 int sph_continuity_rate(sph_context* ctx, const double* velocity, const double* density,
                         void* out);                                               /* sphGpu.py:2061 */
+/* The two options the delta-SPH scheme leaves off, as host-step sums: the Shepard-filtered density
+ * of every slot, sum_j m_j W_ij / sum_j (m_j / rho_j) W_ij with the particle itself included, and
+ * the XSPH transport velocity (D rows of `stride`), the fluid's blended toward its fluid neighbors'
+ * by `epsilon` in [0, 1], the boundary's unchanged. */
+int sph_shepard_density(sph_context* ctx, const double* density, void* out);     /* sphGpu.py:2104 */
+int sph_xsph_velocity(sph_context* ctx, const double* velocity, const double* density, double epsilon,
+                      void* out);                                                 /* sphGpu.py:2080 */
 
 /* Each slot's neighbor count, then (offsets from the caller's prefix sum of the counts) its
  * neighbors' slots in visit order; the predicate is the sums' own. Synchronizes. */

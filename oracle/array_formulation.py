@@ -210,6 +210,32 @@ def extrapolatedWallPressure(densities, pressures, i, j, separation, values, isF
     return result
 
 
+def shepardDensity(masses, densities, i, j, values, selfValues):
+    '''sum_j m_j W_ij / sum_j (m_j / rho_j) W_ij, each particle's own W(0, h_i) included (sphSolver.py:1023-1043).'''
+    count = len(densities)
+    masses = np.broadcast_to(np.asarray(masses, dtype = float), (count,))
+    volumes = masses / densities
+    numerator = masses * selfValues
+    denominator = volumes * selfValues
+    scatterAdd(numerator, i, masses[j] * values)
+    scatterAdd(numerator, j, masses[i] * values)
+    scatterAdd(denominator, i, volumes[j] * values)
+    scatterAdd(denominator, j, volumes[i] * values)
+    return numerator / denominator
+
+
+def xsphVelocity(velocities, masses, densities, i, j, values, epsilon):
+    '''v_i + epsilon sum_j (m_j / rho_bar_ij) (v_j - v_i) W_ij over the pairs given (sphSolver.py:1358-1377).'''
+    count = len(densities)
+    masses = np.broadcast_to(np.asarray(masses, dtype = float), (count,))
+    weight = values / (0.5 * (densities[i] + densities[j]))
+    difference = velocities[j] - velocities[i]
+    correction = np.zeros_like(velocities)
+    scatterAdd(correction, i, (masses[j] * weight)[:, None] * difference)
+    scatterAdd(correction, j, -(masses[i] * weight)[:, None] * difference)
+    return velocities + epsilon * correction
+
+
 def artificialViscosity(relative, separation, distance, meanDensity, h, soundSpeed, alpha, reconstructed = None):
     '''Monaghan's Pi_ij, acting on approaching pairs (sphSolver.py:1122-1149).'''
     approach = np.sum(relative * separation, axis = 1)

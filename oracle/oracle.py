@@ -237,6 +237,24 @@ def continuityRate(physics: Physics, positions, velocities, densities, masses, i
     return out
 
 
+def shepardDensity(physics: Physics, positions, densities, masses, isFluid, lengths = None):
+    '''The Shepard-filtered density of every particle (sphGpu.py:768-802).'''
+    count, pos, vel, dens, mass, lens, fluid = _state(physics, positions, None, densities, masses, lengths, isFluid)
+    out = np.zeros(count)
+    library().oracle_shepard(ctypes.byref(physics.params()), count, _ptr(pos), _ptr(dens), _ptr(mass), _ptr(lens),
+                             _ptr(out))
+    return out
+
+
+def xsphVelocity(physics: Physics, positions, velocities, densities, masses, isFluid, epsilon: float, lengths = None):
+    '''The XSPH transport velocity of the fluid, the boundary's velocity unchanged (sphGpu.py:725-766).'''
+    count, pos, vel, dens, mass, lens, fluid = _state(physics, positions, velocities, densities, masses, lengths, isFluid)
+    out = np.zeros((count, 3))
+    library().oracle_xsph(ctypes.byref(physics.params()), count, _ptr(pos), _ptr(vel), _ptr(dens), _ptr(mass),
+                          _ptr(lens), _ptr(fluid), ctypes.c_double(epsilon), _ptr(out))
+    return out[:, :physics.dims].copy()
+
+
 def cflTimeStep(h, c0, vmax, amax, cfl, maxdt) -> float:
     return library().oracle_cfl_time_step(ctypes.c_double(h), ctypes.c_double(c0), ctypes.c_double(vmax),
                                           ctypes.c_double(amax), ctypes.c_double(cfl), ctypes.c_double(maxdt))

@@ -692,6 +692,48 @@ void oracle_continuity(const OracleParams* p, int n, const double* pos, const do
     for (int i = 0; i < n; ++i) outRate[i] = continuityAt(*p, cells, s, deviation.data(), l.data(), i);
 }
 
+// The Shepard-filtered density of every particle: sum_j m_j W_ij / sum_j (m_j / rho_j) W_ij over
+// every neighbor and the particle itself at W(0, h_i), coincident pairs included (sphGpu.py:768-802,
+// sphSolver.py:1023-1043).
+void oracle_shepard(const OracleParams* p, int n, const double* pos, const double* dens, const double* mass,
+                    const double* len, double* out) {
+    CellList cells;
+    cells.build(*p, n, pos);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < n; ++i) {
+        const double own = sigmaAt(*p, len[i]);
+        double numerator = mass[i] * own, denominator = mass[i] / dens[i] * own;
+        forNeighbors(*p, cells, pos, len, i, false, [&](const Pair& q) {
+            const double value = kernelValue(p->kernel, q.r, q.h, sigmaAt(*p, q.h));
+            numerator += mass[q.j] * value;
+            denominator += mass[q.j] / dens[q.j] * value;
+        });
+        out[i] = numerator / denominator;
+    }
+}
+
+// The XSPH transport velocity, (N, 3) rows: v_i + epsilon sum_j m_j W_ij / ((rho_i + rho_j) / 2)
+// (v_j - v_i) over the fluid neighbors of a fluid particle, coincident pairs included; a boundary
+// particle keeps its velocity (sphGpu.py:725-766, sphSolver.py:1358-1377).
+void oracle_xsph(const OracleParams* p, int n, const double* pos, const double* vel, const double* dens,
+                 const double* mass, const double* len, const uint8_t* fluid, double epsilon, double* out3) {
+    CellList cells;
+    cells.build(*p, n, pos);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < n; ++i) {
+        const V3 vi = vec(vel, i);
+        if (!fluid[i]) { store(out3, i, vi); continue; }
+        V3 correction = {0.0, 0.0, 0.0};
+        forNeighbors(*p, cells, pos, len, i, false, [&](const Pair& q) {
+            if (!fluid[q.j]) return;
+            const double value = kernelValue(p->kernel, q.r, q.h, sigmaAt(*p, q.h));
+            const double weight = mass[q.j] * value / (0.5 * (dens[i] + dens[q.j]));
+            correction = correction + weight * (vec(vel, q.j) - vi);
+        });
+        store(out3, i, vi + epsilon * correction);
+    }
+}
+
 void oracle_enforce(const OracleParams* p, const OracleBoundary* b, int n, double* pos, double* vel,
                     const uint8_t* fluid) {
     for (int i = 0; i < n; ++i)

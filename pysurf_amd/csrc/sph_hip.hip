@@ -772,6 +772,66 @@ __global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
     sweep1Finish<D, Real>(runs[r], sweep, rec, out, s, order[s], fluidI, resident, doWall, doGradients);
 }
 
+// The options off under delta-SPH, host steps only. `which` 0: the Shepard-filtered density,
+// sum_j m_j W_ij / sum_j (m_j / rho_j) W_ij over every neighbor and the particle itself at W(0, h_i)
+// (sphGpu.py:768-802); 1: the XSPH transport velocity, v_i + epsilon sum_j m_j W_ij / ((rho_i +
+// rho_j) / 2) (v_j - v_i) over the fluid neighbors of a fluid particle, the boundary's velocity
+// unchanged (sphGpu.py:725-766). Neither skips coincident pairs. One result per slot: `out[i]`, or
+// `out[a * stride + i]` for the velocity's axis a.
+template <int D, typename Real, int Lanes>
+__global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
+    sweepFilters(const RunDev* __restrict__ runs, Tiles tiles, const int32_t* __restrict__ order,
+                 const int32_t* __restrict__ start, Records<Real> rec, NeighborList list,
+                 const double* __restrict__ y64, int32_t which, Real epsilon, Real* __restrict__ out, int64_t stride) {
+    const int64_t tile = xcdBlock();
+    const int r = tiles.run[tile];
+    const int64_t s = tiles.first[tile] + threadIdx.x / Lanes;
+    const int lane = (int)(threadIdx.x % Lanes);
+    const bool valid = s < runs[r].first + runs[r].count;
+    Real acc[3] = {Real(0), Real(0), Real(0)};
+    Real vi[3] = {Real(0), Real(0), Real(0)};
+    Real own = Real(0);
+    if (valid) {
+        const RunDev& run = runs[r];
+        const Self<D, Real> me = loadSelf<D, Real>(run, rec, s);
+        const Real rhoI = densityOf<D, Real>(rec.r[s]);
+        velocityOf<D, Real>(rec.r[s], vi);
+        if (which == 0 || me.fluid) {
+            PairKernel<Real, D> kernel;
+            kernel.init(run.kernel, run.sigmaUnit, run.hmax, run.uniform != 0);
+            own = kernel.sigmaAt(me.h);
+            const double yy = run.period > 0.0 ? y64[order[s]] : 0.0;
+            forEachNeighbor<D, Real, Lanes, false>(
+                run, start, GlobalRecords<Real>{rec.r}, list, s, me, yy, lane, kernel.uniform,
+                [&](int32_t, const Record<Real>& other, Real, Real, Real, Real rr, Real h) {
+                    const Real w = kernel.value(rr, h);
+                    if (which == 0) {
+                        acc[0] += massOf<D, Real>(other) * w;
+                        acc[1] += volumeOf<D, Real>(other) * w;
+                        return;
+                    }
+                    if (!(signedLength<D, Real>(other) > Real(0))) return;
+                    const Real weight = massOf<D, Real>(other) * w / (Real(0.5) * (rhoI + densityOf<D, Real>(other)));
+                    Real vj[3];
+                    velocityOf<D, Real>(other, vj);
+#pragma unroll
+                    for (int a = 0; a < D; ++a) acc[a] += weight * (vj[a] - vi[a]);
+                });
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < 3; ++e) acc[e] = groupSum<Lanes>(acc[e]);
+    if (!valid || lane != 0) return;
+    const int32_t i = order[s];
+    if (which == 0) {
+        const Record<Real>& self = rec.r[s];
+        out[i] = (massOf<D, Real>(self) * own + acc[0]) / (volumeOf<D, Real>(self) * own + acc[1]);
+        return;
+    }
+#pragma unroll
+    for (int a = 0; a < D; ++a) out[a * stride + i] = vi[a] + epsilon * acc[a];
+}
+
 // Sweep 2: accelerations with the limited viscosity (sphGpu.py:303-414) and, when asked, the
 // correction matrix L (sphGpu.py:593-641) on fluid particles.
 template <int D, typename Real, int Lanes>
@@ -2427,7 +2487,7 @@ int dispatchPhase(sph_context* ctx, int phase, double forcedDt) {
 // Host-step sums: gather records from caller arrays, then one sweep.
 template <int D, typename Real>
 int hostSweep(sph_context* ctx, int which, const double* velocity, const double* density, const void* pressure,
-              void* out0, double* out1) {
+              void* out0, double* out1, double scalar) {
     const int64_t n = ctx->total;
     if (n == 0) return SPH_OK;
     const int b = ctx->block;
@@ -2487,6 +2547,12 @@ int hostSweep(sph_context* ctx, int which, const double* velocity, const double*
                                 run0.diffusion, 0.0, nearRecords<D, Real>(ctx), n)));
             break;
         }
+        case 5:   // the Shepard-filtered density
+        case 6:   // the XSPH transport velocity, epsilon in `scalar`
+            WITH_LANES(ctx, (sweepFilters<D, Real, L><<<tileGrid(ctx), b, 0, ctx->stream>>>(
+                            ctx->dRuns, tilesOf(ctx), order, start, rec, list, y64, which - 5, (Real)scalar,
+                                static_cast<Real*>(out0), ctx->state.stride)));
+            break;
         default:
             return fail(ctx, SPH_ERR_VALUE, "Unknown sum.");
     }
@@ -2494,15 +2560,15 @@ int hostSweep(sph_context* ctx, int which, const double* velocity, const double*
 }
 
 int dispatchHostSweep(sph_context* ctx, int which, const double* velocity, const double* density, const void* pressure,
-                      void* out0, double* out1) {
+                      void* out0, double* out1, double scalar = 0.0) {
     int status = requireBound(ctx);
     if (status != SPH_OK) return status;
     if (!density) return fail(ctx, SPH_ERR_VALUE, "The sums need the densities.");
     if (ctx->dims == 2)
-        return ctx->precision == 0 ? hostSweep<2, float>(ctx, which, velocity, density, pressure, out0, out1)
-                                   : hostSweep<2, double>(ctx, which, velocity, density, pressure, out0, out1);
-    return ctx->precision == 0 ? hostSweep<3, float>(ctx, which, velocity, density, pressure, out0, out1)
-                               : hostSweep<3, double>(ctx, which, velocity, density, pressure, out0, out1);
+        return ctx->precision == 0 ? hostSweep<2, float>(ctx, which, velocity, density, pressure, out0, out1, scalar)
+                                   : hostSweep<2, double>(ctx, which, velocity, density, pressure, out0, out1, scalar);
+    return ctx->precision == 0 ? hostSweep<3, float>(ctx, which, velocity, density, pressure, out0, out1, scalar)
+                               : hostSweep<3, double>(ctx, which, velocity, density, pressure, out0, out1, scalar);
 }
 
 template <int D, typename Real>
@@ -2794,6 +2860,17 @@ int sph_pair_accelerations(sph_context* ctx, const double* velocity, const doubl
 int sph_continuity_rate(sph_context* ctx, const double* velocity, const double* density, void* out) {
     if (!velocity || !out) return fail(ctx, SPH_ERR_VALUE, "The continuity rate needs velocities and an output.");
     return dispatchHostSweep(ctx, 4, velocity, density, nullptr, out, nullptr);
+}
+
+int sph_shepard_density(sph_context* ctx, const double* density, void* out) {
+    if (!out) return fail(ctx, SPH_ERR_VALUE, "The Shepard filter needs an output.");
+    return dispatchHostSweep(ctx, 5, nullptr, density, nullptr, out, nullptr);
+}
+
+int sph_xsph_velocity(sph_context* ctx, const double* velocity, const double* density, double epsilon, void* out) {
+    if (!velocity || !out) return fail(ctx, SPH_ERR_VALUE, "XSPH needs velocities and an output.");
+    if (!(epsilon >= 0.0 && epsilon <= 1.0)) return fail(ctx, SPH_ERR_VALUE, "XSPH's epsilon lies between 0 and 1.");
+    return dispatchHostSweep(ctx, 6, velocity, density, nullptr, out, nullptr, epsilon);
 }
 
 int sph_neighbor_counts(sph_context* ctx, int32_t* counts) {

@@ -5,8 +5,9 @@ method surface over the HIP engine (include/sph_hip.h), with device state held i
 tensors and every sum, step and enforcement run by hand-written HIP kernels.
 
     host steps      `update`, then `pairAccelerations`, `continuityRate`, `wallPressure`,
-                    `velocityGradients`, `correctionMatrices`: host float64 arrays in, host
-                    float64 arrays out, shaped as the reference returns them
+                    `velocityGradients`, `correctionMatrices`, `xsphVelocity`, `shepardDensity`:
+                    host float64 arrays in, host float64 arrays out, shaped as the reference
+                    returns them
     device steps    `setPhysics`, `push`, `setBoundary`, then `residentRates` and
                     `residentIntegrate(dt)` each step, or `advance`, which keeps the time step on
                     the device and replays the steps as a HIP graph; `pull` brings the state back
@@ -15,9 +16,11 @@ Pair arithmetic is float32 over float64 state by default, each float64 position 
 float32 point and remainder as the reference does (sphGpu.py:28-38); `pairPrecision = 'float64'`
 runs the plain float64 kernels of the same source.
 
-The options the delta-SPH scheme leaves off (XSPH, the Shepard filter, the renormalized 'antuono'
-diffusion, particle shifting, the delta-LES closure and the energy budget's ledger) are not
-built: their setters accept them switched off and raise ValueError when asked to switch one on.
+Of the options the delta-SPH scheme leaves off, XSPH and the Shepard filter are built as host-step
+sums; the device step runs without them, and setPhysics and residentRates raise ValueError when
+asked for either. The renormalized 'antuono' diffusion, particle shifting, the delta-LES closure
+and the energy budget's ledger are not built: their setters accept them switched off and raise
+ValueError when asked to switch one on.
 
 '''
 
@@ -677,6 +680,26 @@ class PairSums:
                    'sph_continuity_rate')
         return engine.download('scalar')
 
+    def xsphVelocity(self, velocities: np.ndarray, densities: np.ndarray, epsilon: float) -> np.ndarray:
+        '''The XSPH transport velocity of the fluid; boundary rows unchanged (sphGpu.py:2080-2088).'''
+        self._sync()
+        engine = self.engine
+        engine.upload('velocityIn', velocities)
+        engine.upload('densityIn', densities)
+        self.check(engine.library.sph_xsph_velocity(engine.handle, engine.pointer('velocityIn'),
+                                                    engine.pointer('densityIn'), float(epsilon),
+                                                    engine.pointer('vector')), 'sph_xsph_velocity')
+        return engine.download('vector')
+
+    def shepardDensity(self, densities: np.ndarray) -> np.ndarray:
+        '''The Shepard-filtered density of every particle (sphGpu.py:2104-2112).'''
+        self._sync()
+        engine = self.engine
+        engine.upload('densityIn', densities)
+        self.check(engine.library.sph_shepard_density(engine.handle, engine.pointer('densityIn'),
+                                                      engine.pointer('scalar')), 'sph_shepard_density')
+        return engine.download('scalar')
+
     def _diffusionGravity(self) -> np.ndarray:
         '''Gravity along the last axis, from setDiffusion's g / c0^2.'''
         g = np.zeros(self.dimensions)
@@ -764,9 +787,10 @@ class PairSums:
     def setPhysics(self, referenceDensity: float, soundSpeed: float, alpha: float, boundaryViscosityFactor: float,
                    gravity: np.ndarray, xsphEpsilon: float, densityFloor: float, cflNumber: float = 0.25,
                    maxTimeStep: float = 1.0) -> None:
-        '''The constants of the device step (sphGpu.py:2173-2205); XSPH is off under delta-SPH and not built.'''
+        '''The constants of the device step (sphGpu.py:2173-2205); XSPH is off under delta-SPH and left out of it.'''
         if xsphEpsilon > 0.0:
-            raise ValueError('XSPH is not built; the delta-SPH scheme moves the fluid with its velocity.')
+            raise ValueError('The device step does not blend in XSPH; the delta-SPH scheme moves the fluid with its '
+                             'velocity. xsphVelocity gives the blend as a host step.')
         self.physics = {'referenceDensity': float(referenceDensity), 'soundSpeed': float(soundSpeed),
                         'stiffness': float(referenceDensity * soundSpeed**2 / taitExponent), 'alpha': float(alpha),
                         'boundaryViscosityFactor': float(boundaryViscosityFactor),
@@ -818,7 +842,8 @@ class PairSums:
     def residentRates(self, filterDensity: bool = False) -> tuple[float, float]:
         '''The first half of a device step; returns the largest fluid speed and acceleration (sphGpu.py:2237-2273).'''
         if filterDensity:
-            raise ValueError('The Shepard filter is not built; the delta-SPH scheme runs without it.')
+            raise ValueError('The device step does not filter the density; the delta-SPH scheme runs without it. '
+                             'shepardDensity gives the filtered density as a host step.')
         self._sync()
         maxima = np.zeros(2)
         self.check(self.engine.library.sph_rates(self.engine.handle, maxima.ctypes.data_as(ctypes.c_void_p)),

@@ -168,9 +168,11 @@ class WcsphSolver:
         if config.turbulenceClosure not in ('none', 'deltaLes'):
             raise ValueError(f"Turbulence closure '{config.turbulenceClosure}' is not 'none' or 'deltaLes'.")
         if config.densityFilterInterval and config.densityDiffusion == 'none':
-            raise ValueError('The Shepard filter is not built; set densityFilterInterval to 0 with a diffusive term.')
+            raise ValueError('The device step does not run the Shepard filter (PairSums.shepardDensity gives it as a '
+                             'host step); set densityFilterInterval to 0 with a diffusive term.')
         if config.xsphEpsilon > 0.0:
-            raise ValueError('XSPH is not built; the delta-SPH scheme sets xsphEpsilon to 0.')
+            raise ValueError('The device step does not move the fluid with XSPH (PairSums.xsphVelocity gives it as a '
+                             'host step); the delta-SPH scheme sets xsphEpsilon to 0.')
         self._config = config
         self._boundary = boundary
         self._particles: ParticleSystem | None = None

@@ -116,6 +116,60 @@ def testWallPressureMatchesTheOracle(precision, dimensions, periodic, kernelType
 
 
 @pytest.mark.parametrize('precision', precisions)
+@pytest.mark.parametrize('dimensions, periodic', cases)
+@pytest.mark.parametrize('kernelType', kernelTypes)
+def testShepardMatchesTheOracle(precision, dimensions, periodic, kernelType):
+    # tests/testSphGpu.py:377-386: two float32 sums over up to 270 neighbors, relative 2e-6
+    print(f'shepard {precision} {dimensions}D periodic={periodic} {kernelType}', flush = True)
+    case = cloud(dimensions, periodic, kernelType = kernelType)
+    expected = oracle.shepardDensity(physicsFor(case), case['positions'], case['densities'], case['masses'],
+                                     case['isFluid'])
+    result = pairSums(case, precision).shepardDensity(case['densities'])
+    assert result == pytest.approx(expected, rel = 2e-6 if precision == 'float32' else 1e-12)
+
+
+@pytest.mark.parametrize('precision', precisions)
+@pytest.mark.parametrize('dimensions, periodic', cases)
+@pytest.mark.parametrize('kernelType', kernelTypes)
+def testXsphMatchesTheOracle(precision, dimensions, periodic, kernelType):
+    # tests/testSphGpu.py:362-375: the fluid to 1e-5 of the largest value, the boundary unchanged
+    print(f'xsph {precision} {dimensions}D periodic={periodic} {kernelType}', flush = True)
+    case = cloud(dimensions, periodic, kernelType = kernelType)
+    fluid = case['isFluid']
+    expected = oracle.xsphVelocity(physicsFor(case), case['positions'], case['velocities'], case['densities'],
+                                   case['masses'], fluid, 0.5)
+    result = pairSums(case, precision).xsphVelocity(case['velocities'], case['densities'], 0.5)
+    assert result[fluid] == pytest.approx(expected[fluid], abs = tolerance[precision] * np.abs(expected).max())
+    rounded = case['velocities'][~fluid].astype(np.float32) if precision == 'float32' else case['velocities'][~fluid]
+    assert np.array_equal(result[~fluid], rounded)
+
+
+@pytest.mark.parametrize('precision', precisions)
+def testShepardAndXsphWithMixedLengthsMatchTheOracle(precision):
+    # tests/testSphRefinement.py:307-326: h_ij = (h_i + h_j) / 2, each particle's own W(0, h_i)
+    case = cloud(3, False)
+    rng = np.random.default_rng(5)
+    lengths = case['h'] * np.where(rng.uniform(size = len(case['positions'])) < 0.3, 0.5, 1.0)
+    physics = physicsFor(case, hmin = float(lengths.min()))
+    fluid = case['isFluid']
+    sums = pairSums(case, precision, lengths)
+    shepard = oracle.shepardDensity(physics, case['positions'], case['densities'], case['masses'], fluid, lengths)
+    assert sums.shepardDensity(case['densities']) == pytest.approx(shepard,
+                                                                   rel = 2e-6 if precision == 'float32' else 1e-12)
+    xsph = oracle.xsphVelocity(physics, case['positions'], case['velocities'], case['densities'], case['masses'], fluid,
+                               0.5, lengths)
+    result = sums.xsphVelocity(case['velocities'], case['densities'], 0.5)
+    assert result[fluid] == pytest.approx(xsph[fluid], abs = tolerance[precision] * np.abs(xsph).max())
+
+
+def testXsphRefusesAnEpsilonOutsideZeroToOne():
+    case = cloud(2, False)
+    sums = pairSums(case, 'float32')
+    with pytest.raises(ValueError, match = 'epsilon'):
+        sums.xsphVelocity(case['velocities'], case['densities'], 1.5)
+
+
+@pytest.mark.parametrize('precision', precisions)
 def testDiffusionDifferencesKeepTheirDigits(precision):
     # tests/testSphGpu.py:284-297: a density gradient of 1e-4 kg/m^3 a spacing survives float32
     case = cloud(2, False)

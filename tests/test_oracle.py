@@ -202,6 +202,59 @@ def testWallPressureMatchesTheArrayFormulation(dimensions, periodic, kernelType)
     assert density[boundary] == pytest.approx(af.taitDensity(expected[boundary], rho0, c0), rel = 1e-14)
 
 
+@pytest.mark.parametrize('dimensions, periodic', cases)
+@pytest.mark.parametrize('kernelType', kernelTypes)
+def testShepardMatchesTheArrayFormulation(dimensions, periodic, kernelType):
+    # tests/testSphGpu.py:377-386: every pair and the particle itself
+    case = cloud(dimensions, periodic, kernelType = kernelType)
+    own = af.kernelValues(kernelType, np.zeros(1), case['h'], dimensions)[0]
+    expected = af.shepardDensity(case['masses'], case['densities'], case['i'], case['j'], case['values'], own)
+    result = oracle.shepardDensity(physicsFor(case), case['positions'], case['densities'], case['masses'],
+                                   case['isFluid'])
+    assert result == pytest.approx(expected, rel = 1e-13)
+
+
+@pytest.mark.parametrize('dimensions, periodic', cases)
+@pytest.mark.parametrize('kernelType', kernelTypes)
+def testXsphMatchesTheArrayFormulation(dimensions, periodic, kernelType):
+    # tests/testSphGpu.py:362-375: fluid-fluid pairs only; the boundary keeps its velocity
+    case = cloud(dimensions, periodic, kernelType = kernelType)
+    i, j, fluid = case['i'], case['j'], case['isFluid']
+    pairs = fluid[i] & fluid[j]
+    expected = af.xsphVelocity(case['velocities'], case['masses'], case['densities'], i[pairs], j[pairs],
+                               case['values'][pairs], 0.5)
+    result = oracle.xsphVelocity(physicsFor(case), case['positions'], case['velocities'], case['densities'],
+                                 case['masses'], fluid, 0.5)
+    assert result[fluid] == pytest.approx(expected[fluid], abs = 1e-13 * np.abs(expected).max())
+    assert np.array_equal(result[~fluid], case['velocities'][~fluid])
+
+
+def testShepardAndXsphWithMixedLengthsMatchTheArrayFormulation():
+    # tests/testSphRefinement.py:307-326: h_ij = (h_i + h_j) / 2 and each particle's own W(0, h_i)
+    case = cloud(2, False)
+    rng = np.random.default_rng(5)
+    lengths = case['h'] * np.where(rng.uniform(size = len(case['positions'])) < 0.3, 0.5, 1.0)
+    pairs = af.bruteForcePairs(case['positions'], lengths)
+    i = np.array([a for a, b in sorted(pairs) if a < b])
+    j = np.array([b for a, b in sorted(pairs) if a < b])
+    pairLengths = 0.5 * (lengths[i] + lengths[j])
+    values = af.kernelValues('wendlandC2', np.linalg.norm(case['positions'][i] - case['positions'][j], axis = 1),
+                             pairLengths, 2)
+    own = af.kernelValues('wendlandC2', np.zeros(len(lengths)), lengths, 2)
+    physics = physicsFor(case, hmin = float(lengths.min()))
+    shepard = oracle.shepardDensity(physics, case['positions'], case['densities'], case['masses'], case['isFluid'],
+                                    lengths)
+    assert shepard == pytest.approx(af.shepardDensity(case['masses'], case['densities'], i, j, values, own),
+                                    rel = 1e-13)
+    fluid = case['isFluid']
+    keep = fluid[i] & fluid[j]
+    expected = af.xsphVelocity(case['velocities'], case['masses'], case['densities'], i[keep], j[keep], values[keep],
+                               0.5)
+    xsph = oracle.xsphVelocity(physics, case['positions'], case['velocities'], case['densities'], case['masses'], fluid,
+                               0.5, lengths)
+    assert xsph[fluid] == pytest.approx(expected[fluid], abs = 1e-13 * np.abs(expected).max())
+
+
 # -- Physics the formulas must satisfy -- #
 def testPairForcesConserveMomentum():
     # tests/testSphSolver.py:126-145: a gravity-free all-fluid cloud keeps sum m a = 0 to rounding
@@ -235,6 +288,61 @@ def testRigidTranslationLeavesTheDensity():
     rate = oracle.continuityRate(physicsFor(case), case['positions'], v, case['densities'], case['masses'],
                                  case['isFluid'])
     assert np.abs(rate).max() < 1e-9
+
+
+@pytest.mark.parametrize('dimensions', [2, 3])
+def testShepardReturnsAUniformDensityUnchanged(dimensions):
+    # sphSolver.py:1028-1033 and tests/testSphSolver.py (the filter against a uniform density): the
+    # denominator is the interpolant of one, so a uniform density comes back at the free surface too
+    case = cloud(dimensions, False)
+    densities = np.full(len(case['positions']), 1012.5)
+    result = oracle.shepardDensity(physicsFor(case), case['positions'], densities, case['masses'], case['isFluid'])
+    assert result == pytest.approx(densities, rel = 1e-14)
+    # A lone particle reads its own density; a dropped self term would leave 0 / 0
+    lone = oracle.shepardDensity(physicsFor(case), case['positions'][:1], densities[:1], case['masses'][:1],
+                                 case['isFluid'][:1])
+    assert lone == pytest.approx(densities[:1], rel = 1e-15)
+
+
+def testShepardOfTwoParticlesIsTheirWeightedMean():
+    # Two particles r apart: rho_0 = (m0 W0 + m1 W) / (m0 W0 / rho0 + m1 W / rho1), by hand
+    case = cloud(2, False)
+    positions = np.array([[0.0, 0.0], [0.05, 0.02]]) + 25.0
+    masses, densities = np.array([1.6, 1.7]), np.array([990.0, 1010.0])
+    h = case['h']
+    r = math.hypot(0.05, 0.02)
+    w0 = af.kernelValues('wendlandC2', np.zeros(1), h, 2)[0]
+    w = af.kernelValues('wendlandC2', np.array([r]), h, 2)[0]
+    assert w > 0.0
+    result = oracle.shepardDensity(physicsFor(case), positions, densities, masses, np.ones(2, dtype = bool))
+    expected0 = (masses[0] * w0 + masses[1] * w) / (masses[0] / densities[0] * w0 + masses[1] / densities[1] * w)
+    expected1 = (masses[1] * w0 + masses[0] * w) / (masses[1] / densities[1] * w0 + masses[0] / densities[0] * w)
+    assert result == pytest.approx([expected0, expected1], rel = 1e-14)
+
+
+def testXsphLeavesARigidTranslationAndPullsTowardTheNeighbors():
+    # sphSolver.py:1363-1370: no correction for a uniform velocity; for two fluid particles the
+    # correction is epsilon m_j W / rho_bar (v_j - v_i), each pulled toward the other
+    case = cloud(2, False)
+    count = len(case['positions'])
+    uniform = np.tile([0.7, -0.2], (count, 1))
+    result = oracle.xsphVelocity(physicsFor(case), case['positions'], uniform, case['densities'], case['masses'],
+                                 case['isFluid'], 0.5)
+    assert result == pytest.approx(uniform, abs = 1e-15)
+    positions = np.array([[0.0, 0.0], [0.05, 0.02]]) + 25.0
+    masses, densities = np.array([1.6, 1.7]), np.array([990.0, 1010.0])
+    velocities = np.array([[1.0, 0.0], [0.0, 2.0]])
+    w = af.kernelValues('wendlandC2', np.array([math.hypot(0.05, 0.02)]), case['h'], 2)[0]
+    result = oracle.xsphVelocity(physicsFor(case), positions, velocities, densities, masses, np.ones(2, dtype = bool),
+                                 0.25)
+    rhoBar = 0.5 * (densities[0] + densities[1])
+    expected = velocities + 0.25 * w / rhoBar * np.array([masses[1] * (velocities[1] - velocities[0]),
+                                                          masses[0] * (velocities[0] - velocities[1])])
+    assert result == pytest.approx(expected, rel = 1e-14)
+    # A boundary neighbor pulls nothing; the boundary particle keeps its own velocity
+    wall = oracle.xsphVelocity(physicsFor(case), positions, velocities, densities, masses, np.array([True, False]),
+                               0.25)
+    assert np.array_equal(wall, velocities)
 
 
 def _lattice(dimensions, spacing, count):
