@@ -66,7 +66,7 @@ typedef struct {
     double period;         /* periodic y length [m], 0 for none (3D only) */
     double rho0, c0, alpha, boundaryFactor;
     double gravity[3];
-    int32_t diffusion;     /* 0 none, 1 molteni, 2 hydrostatic */
+    int32_t diffusion;     /* 0 none, 1 molteni, 2 hydrostatic, 3 antuono (host-step sums only) */
     double delta;
     int32_t extrapolatedWalls, negativePressure, correction, limiter;
     double densityFloor;   /* smallest fluid density [kg/m^3], 0 for none */
@@ -119,6 +119,10 @@ int sph_continuity_rate(sph_context* ctx, const double* velocity, const double* 
  * the XSPH transport velocity (D rows of `stride`), the fluid's blended toward its fluid neighbors'
  * by `epsilon` in [0, 1], the boundary's unchanged. */
 int sph_shepard_density(sph_context* ctx, const double* density, void* out);     /* sphGpu.py:2104 */
+/* The renormalized density gradient of every slot (D rows of `stride`), L_i sum_j V_j (rho_j -
+ * rho_i) grad_i W_ij, zero on the boundary: the 'antuono' diffusion's (sph_run.diffusion 3), which
+ * sph_continuity_rate subtracts; the device steps refuse that mode. */
+int sph_density_gradients(sph_context* ctx, const double* density, void* out);   /* sphGpu.py:1868 */
 int sph_xsph_velocity(sph_context* ctx, const double* velocity, const double* density, double epsilon,
                       void* out);                                                 /* sphGpu.py:2080 */
 

@@ -162,6 +162,19 @@ def correctionMatrices(masses, densities, i, j, separation, gradients, isFluid):
     return corrections
 
 
+def renormalizedDensityGradient(masses, densities, i, j, gradients, corrections, isFluid):
+    '''L_i sum_j V_j (rho_j - rho_i) grad_i W_ij over every neighbor, zero on the boundary (sphSolver.py:691-716).'''
+    count = len(densities)
+    volumes = np.broadcast_to(np.asarray(masses, dtype = float), (count,)) / densities
+    difference = densities[j] - densities[i]
+    result = np.zeros((count, gradients.shape[1]))
+    # grad_j W_ji = -grad_i W_ij and rho_i - rho_j = -difference: both members take the same sign
+    scatterAdd(result, i, (volumes[j] * difference)[:, None] * np.einsum('pab,pb->pa', corrections[i], gradients))
+    scatterAdd(result, j, (volumes[i] * difference)[:, None] * np.einsum('pab,pb->pa', corrections[j], gradients))
+    result[~np.asarray(isFluid, dtype = bool)] = 0.0
+    return result
+
+
 def hydrostaticDensityDifference(vertical, meanDensity, referenceDensity, soundSpeed, gravity):
     '''rho0 (g / c0^2) (z_i - z_j) (rho_bar / rho0)^(2 - gamma) (sphSolver.py:718-737).'''
     ratio = np.asarray(meanDensity) / referenceDensity
@@ -169,8 +182,14 @@ def hydrostaticDensityDifference(vertical, meanDensity, referenceDensity, soundS
 
 
 def densityDiffusionRate(masses, densities, i, j, separation, distance, gradients, count, h, soundSpeed, delta,
-                         referenceDensity, gravity = None, active = None):
-    '''The delta-SPH diffusive term, Molteni's or with the hydrostatic part removed (sphSolver.py:739-822).'''
+                         referenceDensity, gravity = None, active = None, densityGradients = None):
+    '''
+
+    The delta-SPH diffusive term, Molteni's, with the hydrostatic part removed, or with Antuono's
+    1/2 (grad rho_i + grad rho_j) . (r_j - r_i) removed given the renormalized `densityGradients`
+    (sphSolver.py:737-822).
+
+    '''
     if active is not None:
         i, j = i[active], j[active]
         separation, distance, gradients = separation[active], distance[active], gradients[active]
@@ -180,6 +199,8 @@ def densityDiffusionRate(masses, densities, i, j, separation, distance, gradient
     if gravity is not None:
         difference = difference - hydrostaticDensityDifference(separation[:, -1], 0.5 * (densities[i] + densities[j]),
                                                                referenceDensity, soundSpeed, gravity)
+    elif densityGradients is not None:
+        difference = difference + 0.5 * np.sum((densityGradients[i] + densityGradients[j]) * separation, axis = 1)
     outward = -np.sum(separation * gradients, axis = 1)
     weight = 2.0 * delta * h * soundSpeed * difference * outward / (distance**2 + viscositySoftening * h**2)
     volumes = masses / densities

@@ -129,7 +129,7 @@ class Physics:
         else:
             g[self.dims - 1] = given[-1]
         p.gravity[:] = list(g)
-        p.diffusion = {'none': 0, 'molteni': 1, 'hydrostatic': 2}[self.diffusion]
+        p.diffusion = {'none': 0, 'molteni': 1, 'hydrostatic': 2, 'antuono': 3}[self.diffusion]
         p.delta = self.delta
         p.extrapolatedWalls = int(self.extrapolatedWalls)
         p.negativePressure = int(self.negativePressure)
@@ -228,13 +228,27 @@ def accelerations(physics: Physics, positions, velocities, densities, pressures,
 
 
 def continuityRate(physics: Physics, positions, velocities, densities, masses, isFluid, lengths = None,
-                   corrections = None):
+                   corrections = None, densityGradients = None):
+    '''d rho / dt; 'antuono' diffusion takes the renormalized `densityGradients` (sphGpu.py:416-546).'''
     count, pos, vel, dens, mass, lens, fluid = _state(physics, positions, velocities, densities, masses, lengths, isFluid)
     l = _padMatrices(corrections, count, 1.0) if corrections is not None else None
+    g = _padded(densityGradients, count) if densityGradients is not None else None
+    if physics.diffusion == 'antuono' and g is None:
+        raise ValueError("The 'antuono' diffusion needs the density gradients.")
     out = np.zeros(count)
     library().oracle_continuity(ctypes.byref(physics.params()), count, _ptr(pos), _ptr(vel), _ptr(dens), _ptr(mass),
-                                _ptr(lens), _ptr(fluid), _ptr(l), _ptr(out))
+                                _ptr(lens), _ptr(fluid), _ptr(l), _ptr(g), _ptr(out))
     return out
+
+
+def densityGradients(physics: Physics, positions, densities, masses, isFluid, corrections, lengths = None):
+    '''Each particle's renormalized density gradient with the given L, (N, dim) (sphGpu.py:548-591).'''
+    count, pos, vel, dens, mass, lens, fluid = _state(physics, positions, None, densities, masses, lengths, isFluid)
+    l = _padMatrices(corrections, count, 1.0)
+    out = np.zeros((count, 3))
+    library().oracle_density_gradients(ctypes.byref(physics.params()), count, _ptr(pos), _ptr(dens), _ptr(mass),
+                                       _ptr(lens), _ptr(fluid), _ptr(l), _ptr(out))
+    return out[:, :physics.dims].copy()
 
 
 def shepardDensity(physics: Physics, positions, densities, masses, isFluid, lengths = None):
@@ -319,6 +333,8 @@ class OracleRun:
 
     def __init__(self, physics: Physics, boundary: dict | None, positions, velocities, densities, masses, isFluid,
                  lengths = None, time: float = 0.0) -> None:
+        if physics.diffusion == 'antuono':
+            raise ValueError("The restated step carries no 'antuono' diffusion; its host-step sums do.")
         self.physics = physics
         count, self.pos, self.vel, self.dens, self.mass, self.lens, self.fluid = _state(
             physics, positions, velocities, densities, masses, lengths, isFluid)

@@ -396,7 +396,7 @@ V3 accelerationAt(const OracleParams& p, const CellList& cells, const Fields& s,
 // d rho_i / dt with the delta-SPH diffusive term (sphGpu.py:416-545; sphSolver.py:623-654, 718-822).
 // `deviation` holds rho - rho0.
 double continuityAt(const OracleParams& p, const CellList& cells, const Fields& s, const double* deviation,
-                    const M3* corrections, int i) {
+                    const M3* corrections, int i, const V3* gradients = nullptr) {
     const V3 vi = vec(s.vel, i);
     const bool boundaryI = !s.fluid[i];
     const bool diffuses = p.diffusion != 0 && !boundaryI;
@@ -419,12 +419,29 @@ double continuityAt(const OracleParams& p, const CellList& cells, const Fields& 
                 const double ratio = 1.0 + 0.5 * (deviation[i] + deviation[j]) / p.rho0;
                 const double sq = ratio * ratio;
                 difference -= p.rho0 * gOverC2 * vert / (sq * sq * ratio);
+            } else if (p.diffusion == 3 && gradients) {
+                // less 1/2 (grad rho_i + grad rho_j) . (r_j - r_i), with d = r_i - r_j (sphGpu.py:521-524)
+                difference += 0.5 * dot(gradients[i] + gradients[j], q.d);
             }
             total += factor * q.h * s.mass[j] / s.dens[j] * difference * (-slope * q.r) /
                      (q.r * q.r + kSoftening * q.h * q.h);
         }
     });
     return total;
+}
+
+// The renormalized density gradient of fluid particle i, L_i sum_j V_j (rho_j - rho_i) grad_i W_ij
+// over every neighbor, boundary particles included, from the deviations rho - rho0; zero on the
+// boundary (sphGpu.py:548-591, sphSolver.py:691-716).
+V3 densityGradientAt(const OracleParams& p, const CellList& cells, const Fields& s, const double* deviation,
+                     const M3& li, int i) {
+    if (!s.fluid[i]) return {0.0, 0.0, 0.0};
+    V3 total = {0.0, 0.0, 0.0};
+    forNeighbors(p, cells, s.pos, s.len, i, true, [&](const Pair& q) {
+        const double slope = kernelSlope(p.kernel, q.r, q.h, sigmaAt(p, q.h));
+        total = total + (s.mass[q.j] / s.dens[q.j] * (deviation[q.j] - deviation[i]) * slope / q.r) * q.d;
+    });
+    return mul(li, total);
 }
 
 // The bed segment k with bedX[k] <= x < bedX[k + 1], clipped (sphGpu.py:1231-1244).
@@ -677,7 +694,7 @@ void oracle_accelerations(const OracleParams* p, int n, const double* pos, const
 // `corrections9` may be null when the correction is off.
 void oracle_continuity(const OracleParams* p, int n, const double* pos, const double* vel, const double* dens,
                        const double* mass, const double* len, const uint8_t* fluid, const double* corrections9,
-                       double* outRate) {
+                       const double* gradients3, double* outRate) {
     CellList cells;
     cells.build(*p, n, pos);
     const Fields s{n, pos, vel, dens, nullptr, mass, len, fluid};
@@ -688,8 +705,31 @@ void oracle_continuity(const OracleParams* p, int n, const double* pos, const do
     }
     std::vector<double> deviation(n);
     for (int i = 0; i < n; ++i) deviation[i] = dens[i] - p->rho0;
+    std::vector<V3> g;
+    if (p->diffusion == 3 && gradients3) {
+        g.resize(n);
+        for (int i = 0; i < n; ++i) g[i] = vec(gradients3, i);
+    }
 #pragma omp parallel for schedule(dynamic, 64)
-    for (int i = 0; i < n; ++i) outRate[i] = continuityAt(*p, cells, s, deviation.data(), l.data(), i);
+    for (int i = 0; i < n; ++i)
+        outRate[i] = continuityAt(*p, cells, s, deviation.data(), l.data(), i, g.empty() ? nullptr : g.data());
+}
+
+// Each particle's renormalized density gradient, (N, 3) rows, with the L of `corrections9` (the
+// caller's, from oracle_correction).
+void oracle_density_gradients(const OracleParams* p, int n, const double* pos, const double* dens, const double* mass,
+                              const double* len, const uint8_t* fluid, const double* corrections9, double* out3) {
+    CellList cells;
+    cells.build(*p, n, pos);
+    const Fields s{n, pos, nullptr, dens, nullptr, mass, len, fluid};
+    std::vector<double> deviation(n);
+    for (int i = 0; i < n; ++i) deviation[i] = dens[i] - p->rho0;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < n; ++i) {
+        M3 li;
+        std::memcpy(li.m, corrections9 + 9 * (size_t)i, sizeof li.m);
+        store(out3, i, densityGradientAt(*p, cells, s, deviation.data(), li, i));
+    }
 }
 
 // The Shepard-filtered density of every particle: sum_j m_j W_ij / sum_j (m_j / rho_j) W_ij over

@@ -107,6 +107,7 @@ signatures = {
     'sph_pair_accelerations': [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
     'sph_continuity_rate': [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
     'sph_shepard_density': [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
+    'sph_density_gradients': [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
     'sph_xsph_velocity': [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p],
     'sph_neighbor_counts': [ctypes.c_void_p, ctypes.c_void_p],
     'sph_neighbor_rows': [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],

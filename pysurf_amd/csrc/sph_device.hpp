@@ -154,7 +154,7 @@ template <typename Real>
 struct Records {
     Record<Real>* r;
     int32_t* key;    // the particle's cell, in the context's cell array
-    Real* G;         // velocity gradient
+    Real* G;         // velocity gradient; in a host step of the renormalized diffusion, the density gradient (D values)
     Real* L;         // kernel gradient correction
     Real* A;         // acceleration, D values per particle (zero on the boundary), for the kick
 };

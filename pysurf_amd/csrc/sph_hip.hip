@@ -776,8 +776,11 @@ __global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
 // sum_j m_j W_ij / sum_j (m_j / rho_j) W_ij over every neighbor and the particle itself at W(0, h_i)
 // (sphGpu.py:768-802); 1: the XSPH transport velocity, v_i + epsilon sum_j m_j W_ij / ((rho_i +
 // rho_j) / 2) (v_j - v_i) over the fluid neighbors of a fluid particle, the boundary's velocity
-// unchanged (sphGpu.py:725-766). Neither skips coincident pairs. One result per slot: `out[i]`, or
-// `out[a * stride + i]` for the velocity's axis a.
+// unchanged (sphGpu.py:725-766); neither skips coincident pairs. 2: the renormalized density
+// gradient, L_i sum_j V_j (rho_j - rho_i) grad_i W_ij over every neighbor but the coincident, from
+// the deviations rho - rho0 and the L in `rec.L`, zero on the boundary (sphGpu.py:548-591), also
+// kept per sorted slot in `rec.G` (D values each) for the continuity sum. One result per slot:
+// `out[i]`, or `out[a * stride + i]` for axis a; `out` may be null for the gradient.
 template <int D, typename Real, int Lanes>
 __global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
     sweepFilters(const RunDev* __restrict__ runs, Tiles tiles, const int32_t* __restrict__ order,
@@ -801,9 +804,19 @@ __global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
             kernel.init(run.kernel, run.sigmaUnit, run.hmax, run.uniform != 0);
             own = kernel.sigmaAt(me.h);
             const double yy = run.period > 0.0 ? y64[order[s]] : 0.0;
+            const Real devI = deviationOf<D, Real>(rec.r[s]);
             forEachNeighbor<D, Real, Lanes, false>(
                 run, start, GlobalRecords<Real>{rec.r}, list, s, me, yy, lane, kernel.uniform,
-                [&](int32_t, const Record<Real>& other, Real, Real, Real, Real rr, Real h) {
+                [&](int32_t, const Record<Real>& other, Real dx, Real dy, Real dz, Real rr, Real h) {
+                    if (which == 2) {
+                        if (rr < Real(1.0e-12)) return;
+                        const Real w = volumeOf<D, Real>(other) * (deviationOf<D, Real>(other) - devI) *
+                                       kernel.slopeOverR(rr, h);
+                        const Real d[3] = {dx, dy, dz};
+#pragma unroll
+                        for (int a = 0; a < D; ++a) acc[a] += w * d[a];
+                        return;
+                    }
                     const Real w = kernel.value(rr, h);
                     if (which == 0) {
                         acc[0] += massOf<D, Real>(other) * w;
@@ -826,6 +839,21 @@ __global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
     if (which == 0) {
         const Record<Real>& self = rec.r[s];
         out[i] = (massOf<D, Real>(self) * own + acc[0]) / (volumeOf<D, Real>(self) * own + acc[1]);
+        return;
+    }
+    if (which == 2) {
+        Real g[3] = {Real(0), Real(0), Real(0)};
+        if (signedLength<D, Real>(rec.r[s]) > Real(0)) {
+#pragma unroll
+            for (int a = 0; a < D; ++a)
+#pragma unroll
+                for (int b = 0; b < D; ++b) g[a] += rec.L[s * (D * D) + a * D + b] * acc[b];
+        }
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            rec.G[s * D + a] = g[a];
+            if (out) out[a * stride + i] = g[a];
+        }
         return;
     }
 #pragma unroll
@@ -1108,7 +1136,7 @@ __device__ __forceinline__ void enforceBoundary(const sph_boundary& b, double* x
 // float32 on the records as they are read), and each particle's float64 kick, density advance
 // with its floor (:1111-1140), drift (:1142-1186) and boundary enforcement follow; the run's first
 // particle advances its clock.
-template <int D, typename Real, int Lanes>
+template <int D, typename Real, int Lanes, bool Renormalized = false>
 __global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
     sweepContinuity(const RunDev* __restrict__ runs, Tiles tiles, const int32_t* __restrict__ order, const int32_t* __restrict__ start, Records<Real> rec,
                     NeighborList list, SweepOut<Real> out, Clock clock, int32_t resident, int32_t hostCorrection,
@@ -1161,12 +1189,17 @@ __global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
                 for (int b = 0; b < D; ++b) Li[a][b] = rec.L[s * (D * D) + a * D + b];
         }
         const Real devI = deviationOf<D, Real>(rec.r[s]);
+        Real gradI[3] = {Real(0), Real(0), Real(0)};
+        if constexpr (Renormalized) {
+#pragma unroll
+            for (int a = 0; a < D; ++a) gradI[a] = rec.G[s * D + a];
+        }
         const Real factor = (Real)(2.0 * run.delta * run.c0);
         const Real invRho0 = (Real)(1.0 / run.rho0);
         const Real head = (Real)(run.rho0 * fabs(run.gravity[D - 1]) / (run.c0 * run.c0));
         forEachNeighbor<D, Real, Lanes, true>(
             run, start, cells, list, s, me, yy, lane, kernel.uniform,
-            [&](int32_t, const Record<Real>& other, Real dx, Real dy, Real dz, Real rr, Real h) {
+            [&](int32_t k, const Record<Real>& other, Real dx, Real dy, Real dz, Real rr, Real h) {
                 const bool fluidJ = signedLength<D, Real>(other) > Real(0);
                 if (boundaryI && !fluidJ) return;
                 const Real F = kernel.slopeOverR(rr, h);
@@ -1194,6 +1227,13 @@ __global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
                         const Real ratio = Real(1) + Real(0.5) * (devI + devJ) * invRho0;
                         const Real sq = ratio * ratio;
                         difference -= quotient(head * d[D - 1], sq * sq * ratio);
+                    }
+                    if constexpr (Renormalized) {
+                        // less 1/2 (grad rho_i + grad rho_j) . (r_j - r_i), d = r_i - r_j (sphGpu.py:521-524)
+                        Real projection = Real(0);
+#pragma unroll
+                        for (int a = 0; a < D; ++a) projection += (gradI[a] + rec.G[(int64_t)k * D + a]) * d[a];
+                        difference += Real(0.5) * projection;
                     }
                     const Real soft = kernel.uniform ? kernel.soft : Real(0.01) * h * h;
                     const Real r2 = rr * rr;
@@ -2195,8 +2235,9 @@ int checkRuns(sph_context* ctx, const sph_run* runs, int32_t count, int64_t capa
         if (run.period > 0.0 && ctx->dims != 3) return fail(ctx, SPH_ERR_VALUE, "A periodic axis needs three dimensions: it is y.");
         if (run.period > 0.0 && run.period <= 4.0 * run.hmax)
             return fail(ctx, SPH_ERR_VALUE, "The periodic width must exceed twice the kernel support.");
-        if (run.diffusion < 0 || run.diffusion > 2)
-            return fail(ctx, SPH_ERR_VALUE, "Density diffusion is not 0 (none), 1 (molteni) or 2 (hydrostatic).");
+        if (run.diffusion < 0 || run.diffusion > 3)
+            return fail(ctx, SPH_ERR_VALUE,
+                        "Density diffusion is not 0 (none), 1 (molteni), 2 (hydrostatic) or 3 (antuono).");
         const sph_boundary& b = run.boundary;
         if (b.kind < 0 || b.kind > 3) return fail(ctx, SPH_ERR_VALUE, "Unknown boundary kind.");
         if (b.kind == 2 && (b.nBed < 2 || !b.bedX || !b.bedZ)) return fail(ctx, SPH_ERR_VALUE, "A bed needs at least two vertices.");
@@ -2400,6 +2441,16 @@ int requireBound(sph_context* ctx) {
 // The two halves of a device-resident step of every active run. `phase` 1 is the rates
 // (residentRates), 2 the integration (residentIntegrate) with `forcedDt` when positive and the
 // run's own time step otherwise, 3 both. `useClock` 0 steps every run whatever its clock.
+// The device step runs the delta-SPH scheme's options only: a run with the renormalized diffusion
+// is refused before anything is launched or captured.
+int refuseHostOnly(sph_context* ctx) {
+    for (const RunDev& run : ctx->runsHost)
+        if (run.diffusion == 3)
+            return fail(ctx, SPH_ERR_VALUE,
+                        "The device step does not carry the renormalized 'antuono' diffusion; its host-step sums do.");
+    return SPH_OK;
+}
+
 template <int D, typename Real>
 int stepOnce(sph_context* ctx, int phase, int useClock, double forcedDt) {
     const int64_t n = ctx->total;
@@ -2479,6 +2530,8 @@ int dispatchAdvance(sph_context* ctx, int32_t steps) {
 }
 
 int dispatchPhase(sph_context* ctx, int phase, double forcedDt) {
+    const int refused = refuseHostOnly(ctx);
+    if (refused != SPH_OK) return refused;
     if (ctx->dims == 2)
         return ctx->precision == 0 ? stepOnce<2, float>(ctx, phase, 0, forcedDt) : stepOnce<2, double>(ctx, phase, 0, forcedDt);
     return ctx->precision == 0 ? stepOnce<3, float>(ctx, phase, 0, forcedDt) : stepOnce<3, double>(ctx, phase, 0, forcedDt);
@@ -2534,19 +2587,39 @@ int hostSweep(sph_context* ctx, int which, const double* velocity, const double*
                             ctx->dRuns, tilesOf(ctx), order, start, rec, list, out, ctx->clock, y64, 0, 1, 0, limiter, nearRecords<D, Real>(ctx), n)));
             break;
         }
-        case 4: {  // continuity, with the correction matrices first when the run has them
+        case 4: {  // continuity, with the correction matrices first when the run has them or the
+                   // renormalized diffusion, then that diffusion's density gradients
             const int correction = run0.correction;
-            if (correction)
+            const bool renormalized = run0.diffusion == 3;
+            if (correction || renormalized)
                 WITH_LANES(ctx, (sweepAccelerations<D, Real, L><<<tileGrid(ctx), b, sweepLds<D, Real>(ctx, 0), ctx->stream>>>(
                             ctx->dRuns, tilesOf(ctx), order, start, rec, list, out, ctx->clock, y64, 0, 0, 1, 0, nearRecords<D, Real>(ctx), n)));
+            if (renormalized)
+                WITH_LANES(ctx, (sweepFilters<D, Real, L><<<tileGrid(ctx), b, 0, ctx->stream>>>(
+                            ctx->dRuns, tilesOf(ctx), order, start, rec, list, y64, 2, Real(0), nullptr,
+                                ctx->state.stride)));
             out.rate = static_cast<Real*>(out0);
             for (int a = 0; a < 3; ++a) out.pos[a] = a < D ? ctx->state.position + a * ctx->state.stride : nullptr;
-            WITH_LANES(ctx, (sweepContinuity<D, Real, L><<<tileGrid(ctx), b,
-                                                          sweepLds<D, Real>(ctx, 1), ctx->stream>>>(
+            if (renormalized) {
+                WITH_LANES(ctx, (sweepContinuity<D, Real, L, true><<<tileGrid(ctx), b,
+                                                                    sweepLds<D, Real>(ctx, 1), ctx->stream>>>(
                             ctx->dRuns, tilesOf(ctx), order, start, rec, list, out, ctx->clock, 0, correction,
                                 run0.diffusion, 0.0, nearRecords<D, Real>(ctx), n)));
+            } else {
+                WITH_LANES(ctx, (sweepContinuity<D, Real, L><<<tileGrid(ctx), b,
+                                                              sweepLds<D, Real>(ctx, 1), ctx->stream>>>(
+                            ctx->dRuns, tilesOf(ctx), order, start, rec, list, out, ctx->clock, 0, correction,
+                                run0.diffusion, 0.0, nearRecords<D, Real>(ctx), n)));
+            }
             break;
         }
+        case 7:   // the renormalized density gradients, after the correction matrices
+            WITH_LANES(ctx, (sweepAccelerations<D, Real, L><<<tileGrid(ctx), b, sweepLds<D, Real>(ctx, 0), ctx->stream>>>(
+                            ctx->dRuns, tilesOf(ctx), order, start, rec, list, out, ctx->clock, y64, 0, 0, 1, 0, nearRecords<D, Real>(ctx), n)));
+            WITH_LANES(ctx, (sweepFilters<D, Real, L><<<tileGrid(ctx), b, 0, ctx->stream>>>(
+                            ctx->dRuns, tilesOf(ctx), order, start, rec, list, y64, 2, Real(0),
+                                static_cast<Real*>(out0), ctx->state.stride)));
+            break;
         case 5:   // the Shepard-filtered density
         case 6:   // the XSPH transport velocity, epsilon in `scalar`
             WITH_LANES(ctx, (sweepFilters<D, Real, L><<<tileGrid(ctx), b, 0, ctx->stream>>>(
@@ -2862,6 +2935,11 @@ int sph_continuity_rate(sph_context* ctx, const double* velocity, const double* 
     return dispatchHostSweep(ctx, 4, velocity, density, nullptr, out, nullptr);
 }
 
+int sph_density_gradients(sph_context* ctx, const double* density, void* out) {
+    if (!out) return fail(ctx, SPH_ERR_VALUE, "The density gradients need an output.");
+    return dispatchHostSweep(ctx, 7, nullptr, density, nullptr, out, nullptr);
+}
+
 int sph_shepard_density(sph_context* ctx, const double* density, void* out) {
     if (!out) return fail(ctx, SPH_ERR_VALUE, "The Shepard filter needs an output.");
     return dispatchHostSweep(ctx, 5, nullptr, density, nullptr, out, nullptr);
@@ -2939,6 +3017,8 @@ int sph_advance(sph_context* ctx, int32_t steps, int32_t useGraph) {
     if (status != SPH_OK) return status;
     if (steps < 0) return fail(ctx, SPH_ERR_VALUE, "The step count cannot be negative.");
     if (steps == 0) return SPH_OK;
+    status = refuseHostOnly(ctx);
+    if (status != SPH_OK) return status;
     if (!useGraph || ctx->profiling) return dispatchAdvance(ctx, steps);
     if (ctx->ownsStream == false && ctx->stream == nullptr)
         return fail(ctx, SPH_ERR_VALUE, "A graph needs a stream other than the null stream.");

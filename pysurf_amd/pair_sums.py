@@ -5,7 +5,8 @@ method surface over the HIP engine (include/sph_hip.h), with device state held i
 tensors and every sum, step and enforcement run by hand-written HIP kernels.
 
     host steps      `update`, then `pairAccelerations`, `continuityRate`, `wallPressure`,
-                    `velocityGradients`, `correctionMatrices`, `xsphVelocity`, `shepardDensity`:
+                    `velocityGradients`, `correctionMatrices`, `densityGradients`, `xsphVelocity`,
+                    `shepardDensity`:
                     host float64 arrays in, host float64 arrays out, shaped as the reference
                     returns them
     device steps    `setPhysics`, `push`, `setBoundary`, then `residentRates` and
@@ -16,11 +17,12 @@ Pair arithmetic is float32 over float64 state by default, each float64 position 
 float32 point and remainder as the reference does (sphGpu.py:28-38); `pairPrecision = 'float64'`
 runs the plain float64 kernels of the same source.
 
-Of the options the delta-SPH scheme leaves off, XSPH and the Shepard filter are built as host-step
-sums; the device step runs without them, and setPhysics and residentRates raise ValueError when
-asked for either. The renormalized 'antuono' diffusion, particle shifting, the delta-LES closure
-and the energy budget's ledger are not built: their setters accept them switched off and raise
-ValueError when asked to switch one on.
+Of the options the delta-SPH scheme leaves off, XSPH, the Shepard filter and the renormalized
+'antuono' diffusion (`densityGradients`, and `continuityRate` under setDiffusion('antuono')) are
+built as host-step sums; the device step runs without them, and setPhysics, residentRates and the
+device steps raise ValueError when asked for one. Particle shifting, the delta-LES closure and the
+energy budget's ledger are not built: their setters accept them switched off and raise ValueError
+when asked to switch one on.
 
 '''
 
@@ -36,7 +38,7 @@ from pysurf_amd import _native
 from pysurf_amd.errors import EngineError
 
 taitExponent = 7.0
-diffusionModes = {'none': 0, 'molteni': 1, 'hydrostatic': 2}
+diffusionModes = {'none': 0, 'molteni': 1, 'hydrostatic': 2, 'antuono': 3}
 maxGridCells = 2**25
 
 
@@ -545,10 +547,8 @@ class PairSums:
 
     def setDiffusion(self, mode: str, delta: float, soundSpeed: float, referenceDensity: float, gravity: float) -> None:
         '''The delta-SPH term the continuity sum carries (sphGpu.py:1879-1904).'''
-        if mode == 'antuono':
-            raise ValueError("The renormalized 'antuono' diffusion is not built; use 'molteni' or 'hydrostatic'.")
         if mode not in diffusionModes:
-            raise ValueError(f"Density diffusion '{mode}' is not 'none', 'molteni' or 'hydrostatic'.")
+            raise ValueError(f"Density diffusion '{mode}' is not 'none', 'molteni', 'hydrostatic' or 'antuono'.")
         self.diffusion = {'mode': diffusionModes[mode], 'factor': 2.0 * delta * soundSpeed, 'delta': float(delta),
                           'referenceDensity': float(referenceDensity), 'gravityOverSound': gravity / soundSpeed**2,
                           'soundSpeed': float(soundSpeed)}
@@ -680,6 +680,17 @@ class PairSums:
                    'sph_continuity_rate')
         return engine.download('scalar')
 
+    def densityGradients(self, densities: np.ndarray) -> np.ndarray:
+        '''Each particle's renormalized density gradient, shape (N, dim) (sphGpu.py:1868-1880).'''
+        if not self.renormalizedDiffusion:
+            raise ValueError("The diffusive term is not 'antuono'; see setDiffusion.")
+        self._sync()
+        engine = self.engine
+        engine.upload('densityIn', densities)
+        self.check(engine.library.sph_density_gradients(engine.handle, engine.pointer('densityIn'),
+                                                        engine.pointer('vector')), 'sph_density_gradients')
+        return engine.download('vector')
+
     def xsphVelocity(self, velocities: np.ndarray, densities: np.ndarray, epsilon: float) -> np.ndarray:
         '''The XSPH transport velocity of the fluid; boundary rows unchanged (sphGpu.py:2080-2088).'''
         self._sync()
@@ -735,10 +746,15 @@ class PairSums:
         dim = self.dimensions
         return engine.download('matrix').reshape(self.count, dim, dim)
 
+    @property
+    def renormalizedDiffusion(self) -> bool:
+        '''Whether the diffusive term subtracts the renormalized density gradients (sphGpu.py:1654-1657).'''
+        return self.diffusion['mode'] == diffusionModes['antuono']
+
     def correctionMatrices(self, densities: np.ndarray) -> np.ndarray:
         '''L for every particle, shape (N, dim, dim) (sphGpu.py:1855-1866).'''
-        if not self.correctContinuity:
-            raise ValueError('The continuity correction is not on; see setContinuityCorrection.')
+        if not (self.correctContinuity or self.renormalizedDiffusion):
+            raise ValueError('Neither the continuity correction nor the renormalized diffusion is on.')
         self._sync()
         engine = self.engine
         engine.upload('densityIn', densities)

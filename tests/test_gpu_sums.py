@@ -102,6 +102,48 @@ def testContinuityMatchesTheOracle(precision, dimensions, periodic, kernelType, 
 @pytest.mark.parametrize('precision', precisions)
 @pytest.mark.parametrize('dimensions, periodic', cases)
 @pytest.mark.parametrize('kernelType', kernelTypes)
+@pytest.mark.parametrize('corrected', [False, True])
+def testRenormalizedDiffusionMatchesTheOracle(precision, dimensions, periodic, kernelType, corrected):
+    # tests/testSphGpu.py:184-205: with the correction off the device builds L for the gradients
+    # alone; the gradients to 2e-4 of the largest in float32, the rate to 1e-5
+    print(f'antuono {precision} {dimensions}D periodic={periodic} {kernelType} L={corrected}', flush = True)
+    case = cloud(dimensions, periodic, kernelType = kernelType)
+    fluid = case['isFluid']
+    physics = physicsFor(case, correction = corrected, diffusion = 'antuono', delta = 0.1)
+    L = oracle.correctionMatrices(physics, case['positions'], case['densities'], case['masses'], fluid)
+    gradients = oracle.densityGradients(physics, case['positions'], case['densities'], case['masses'], fluid, L)
+    expected = oracle.continuityRate(physics, case['positions'], case['velocities'], case['densities'], case['masses'],
+                                     fluid, corrections = L if corrected else None, densityGradients = gradients)
+    sums = pairSums(case, precision)
+    sums.setDiffusion('antuono', 0.1, c0, rho0, 9.81)
+    sums.setContinuityCorrection(corrected)
+    result = sums.continuityRate(case['velocities'], case['densities'])
+    deviceGradients = sums.densityGradients(case['densities'])
+    gradientTolerance = 2e-4 if precision == 'float32' else 1e-10
+    assert deviceGradients == pytest.approx(gradients, abs = gradientTolerance * np.abs(gradients).max())
+    assert np.all(deviceGradients[~fluid] == 0.0)
+    assert result == pytest.approx(expected, abs = tolerance[precision] * np.abs(expected).max())
+    # The correction matrices serve the gradients with the continuity correction off
+    assert sums.correctionMatrices(case['densities']) == pytest.approx(L, abs = 2 * tolerance[precision])
+
+
+def testTheDeviceStepRefusesTheRenormalizedDiffusion():
+    case = cloud(2, False)
+    sums = pairSums(case, 'float32')
+    sums.setDiffusion('antuono', 0.1, c0, rho0, 9.81)
+    sums.setPhysics(rho0, c0, alpha, 0.0, gravityOf(2), 0.0, 0.0)
+    # Refused before anything is launched, on the phases and on the graph-captured steps alike
+    with pytest.raises(ValueError, match = 'antuono'):
+        sums.residentRates()
+    with pytest.raises(ValueError, match = 'antuono'):
+        sums.residentIntegrate(1e-5)
+    with pytest.raises(ValueError, match = 'antuono'):
+        sums.engine.advance(2, graph = True)
+
+
+@pytest.mark.parametrize('precision', precisions)
+@pytest.mark.parametrize('dimensions, periodic', cases)
+@pytest.mark.parametrize('kernelType', kernelTypes)
 def testWallPressureMatchesTheOracle(precision, dimensions, periodic, kernelType):
     print(f'wall pressure {precision} {dimensions}D periodic={periodic} {kernelType}', flush = True)
     case = cloud(dimensions, periodic, kernelType = kernelType)

@@ -189,6 +189,35 @@ def testContinuityMatchesTheArrayFormulation(dimensions, periodic, kernelType, c
 
 @pytest.mark.parametrize('dimensions, periodic', cases)
 @pytest.mark.parametrize('kernelType', kernelTypes)
+@pytest.mark.parametrize('corrected', [False, True])
+def testRenormalizedDiffusionMatchesTheArrayFormulation(dimensions, periodic, kernelType, corrected):
+    # tests/testSphGpu.py (the 'antuono' continuity): gradients over every neighbor with L_i, the
+    # subtraction over fluid pairs
+    case = cloud(dimensions, periodic, kernelType = kernelType)
+    i, j, fluid = case['i'], case['j'], case['isFluid']
+    L = af.correctionMatrices(case['masses'], case['densities'], i, j, case['separation'], case['gradients'], fluid)
+    gradients = af.renormalizedDensityGradient(case['masses'], case['densities'], i, j, case['gradients'], L, fluid)
+    physics = physicsFor(case, correction = corrected, diffusion = 'antuono', delta = 0.1)
+    result = oracle.densityGradients(physics, case['positions'], case['densities'], case['masses'], fluid, L)
+    assert result == pytest.approx(gradients, abs = 1e-12 * np.abs(gradients).max())
+    assert np.all(result[~fluid] == 0.0) and np.abs(result[fluid]).min() < np.abs(result[fluid]).max()
+    expected = af.continuityRate(case['masses'], case['velocities'], i, j, case['gradients'], len(fluid),
+                                 fluid[i] | fluid[j], L if corrected else None)
+    expected += af.densityDiffusionRate(case['masses'], case['densities'], i, j, case['separation'], case['distance'],
+                                        case['gradients'], len(fluid), case['h'], c0, 0.1, rho0, None,
+                                        fluid[i] & fluid[j], densityGradients = gradients)
+    rate = oracle.continuityRate(physics, case['positions'], case['velocities'], case['densities'], case['masses'],
+                                 fluid, corrections = L if corrected else None, densityGradients = result)
+    assert rate == pytest.approx(expected, abs = 1e-12 * np.abs(expected).max())
+    # The subtraction is there: Molteni's term differs by more than rounding
+    molteni = oracle.continuityRate(physicsFor(case, correction = corrected, diffusion = 'molteni', delta = 0.1),
+                                    case['positions'], case['velocities'], case['densities'], case['masses'], fluid,
+                                    corrections = L if corrected else None)
+    assert np.abs(molteni - rate).max() > 1e-6 * np.abs(rate).max()
+
+
+@pytest.mark.parametrize('dimensions, periodic', cases)
+@pytest.mark.parametrize('kernelType', kernelTypes)
 def testWallPressureMatchesTheArrayFormulation(dimensions, periodic, kernelType):
     # tests/testSphGpu.py:272-282
     case = cloud(dimensions, periodic, kernelType = kernelType)
@@ -408,6 +437,31 @@ def testDiffusionLeavesAUniformDensityAlone():
     rate = oracle.continuityRate(physics, positions, np.zeros((n, 2)), np.full(n, 1003.0), np.full(n, rho0 * s * s),
                                  np.ones(n, dtype = bool))
     assert np.abs(rate).max() < 1e-9
+
+
+def testRenormalizedGradientsAreExactAndTheirDiffusionVanishesOnALinearDensity():
+    # sphSolver.py:700-703, 761-765: L makes the gradient exact for a linear density, a free surface
+    # included, and the renormalized term leaves any linear field alone; Molteni's does not
+    s = 0.05
+    positions = _lattice(2, s, 14) + 3.0
+    positions += np.random.default_rng(2).uniform(-0.1, 0.1, positions.shape) * s
+    n = len(positions)
+    slope = np.array([3.0, -7.0])
+    densities = 1000.0 + (positions - positions.mean(axis = 0)) @ slope
+    masses = np.full(n, 1000.0 * s * s)
+    fluid = np.ones(n, dtype = bool)
+    physics = oracle.Physics(dims = 2, h = 1.3 * s, domainMin = positions.min(axis = 0),
+                             domainMax = positions.max(axis = 0), kernelType = 'cubicSpline', c0 = c0,
+                             diffusion = 'antuono', delta = 0.1, correction = True)
+    L = oracle.correctionMatrices(physics, positions, densities, masses, fluid)
+    gradients = oracle.densityGradients(physics, positions, densities, masses, fluid, L)
+    assert gradients == pytest.approx(np.tile(slope, (n, 1)), rel = 1e-9)
+    still = np.zeros((n, 2))
+    renormalized = oracle.continuityRate(physics, positions, still, densities, masses, fluid, corrections = L,
+                                         densityGradients = gradients)
+    physics.diffusion = 'molteni'
+    molteni = oracle.continuityRate(physics, positions, still, densities, masses, fluid, corrections = L)
+    assert np.abs(renormalized).max() < 1e-9 * np.abs(molteni).max()
 
 
 def _waterOnABed(s = 0.01, width = 0.3, depth = 0.1, g = 9.80665):
