@@ -123,6 +123,16 @@ int sph_shepard_density(sph_context* ctx, const double* density, void* out);    
  * rho_i) grad_i W_ij, zero on the boundary: the 'antuono' diffusion's (sph_run.diffusion 3), which
  * sph_continuity_rate subtracts; the device steps refuse that mode. */
 int sph_density_gradients(sph_context* ctx, const double* density, void* out);   /* sphGpu.py:1868 */
+/* The delta-LES-SPH closure of the host-step sums (sphGpu.py:1754-1778): while on,
+ * sph_pair_accelerations builds G and each slot's eddy alpha and delta first and adds the closure's
+ * viscosity on every fluid pair, and sph_continuity_rate's diffusion takes each fluid pair's delta
+ * from the deltas the last such sum built (zero before any). The device steps refuse it. */
+int sph_set_closure(sph_context* ctx, int32_t enabled, double soundSpeed, double referenceDensity,
+                    double kinematicViscosity);                                   /* sphGpu.py:1754 */
+/* Each slot's eddy alpha and delta at the bound positions, into two device arrays of the pair
+ * precision: PairSums.closureCoefficients (sphGpu.py:1841). */
+int sph_closure_coefficients(sph_context* ctx, const double* velocity, const double* density, void* alphas,
+                             void* deltas);                                      /* sphGpu.py:1841 */
 int sph_xsph_velocity(sph_context* ctx, const double* velocity, const double* density, double epsilon,
                       void* out);                                                 /* sphGpu.py:2080 */
 

@@ -195,6 +195,8 @@ def densityDiffusionRate(masses, densities, i, j, separation, distance, gradient
         separation, distance, gradients = separation[active], distance[active], gradients[active]
         if np.ndim(h):
             h = h[active]
+        if np.ndim(delta):
+            delta = delta[active]
     difference = densities[j] - densities[i]
     if gravity is not None:
         difference = difference - hydrostaticDensityDifference(separation[:, -1], 0.5 * (densities[i] + densities[j]),
@@ -255,6 +257,40 @@ def xsphVelocity(velocities, masses, densities, i, j, values, epsilon):
     scatterAdd(correction, i, (masses[j] * weight)[:, None] * difference)
     scatterAdd(correction, j, -(masses[i] * weight)[:, None] * difference)
     return velocities + epsilon * correction
+
+
+closureSmagorinsky = 0.12    # sphSolver.py:279
+closureDiffusion = 1.5       # sphSolver.py:280
+closureFilterRatio = 2.0     # sphSolver.py:281
+closureCap = 0.2             # sphSolver.py:282
+
+
+def harmonicMean(a, b):
+    '''2 a b / (a + b), zero where both are zero.'''
+    both = a + b
+    return np.where(both > 0.0, 2.0 * a * b / np.where(both > 0.0, both, 1.0), 0.0)
+
+
+def closureCoefficients(gradients, isFluid, h, soundSpeed):
+    '''The delta-LES-SPH alpha_i = K (C_S l)^2 ||D|| / (c0 h), delta_i = (C_delta l)^2 ||D|| / (c0 h), capped (sphSolver.py:1240-1281).'''
+    dimensions = gradients.shape[1]
+    strain = 0.5 * (gradients + np.swapaxes(gradients, 1, 2))
+    norm = np.sqrt(2.0 * np.sum(strain * strain, axis = (1, 2)))
+    length = closureFilterRatio * np.broadcast_to(np.asarray(h, dtype = float), norm.shape)
+    scale = length**2 * norm / (soundSpeed * np.asarray(h, dtype = float))
+    alphas = np.minimum(2.0 * (dimensions + 2) * closureSmagorinsky**2 * scale, closureCap)
+    deltas = np.minimum(closureDiffusion**2 * scale, closureCap)
+    fluid = np.asarray(isFluid, dtype = bool)
+    return np.where(fluid, alphas, 0.0), np.where(fluid, deltas, 0.0)
+
+
+def closureViscosity(relative, separation, distance, densitiesI, densitiesJ, h, soundSpeed, referenceDensity, alphasI,
+                     alphasJ, kinematicViscosity):
+    '''Pi_ij = -alpha_ij c0 h rho0 (v_ij . r_ij) / (r^2 rho_i rho_j), alpha_ij = K nu / (c0 h) + mean (sphSolver.py:1283-1330).'''
+    dimensions = separation.shape[1]
+    pairAlpha = 2.0 * (dimensions + 2) * kinematicViscosity / (soundSpeed * h) + harmonicMean(alphasI, alphasJ)
+    approach = np.sum(relative * separation, axis = 1)
+    return -pairAlpha * soundSpeed * h * referenceDensity * approach / (distance**2 * densitiesI * densitiesJ)
 
 
 def artificialViscosity(relative, separation, distance, meanDensity, h, soundSpeed, alpha, reconstructed = None):

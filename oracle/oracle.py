@@ -47,6 +47,27 @@ class OracleBoundary(ctypes.Structure):
                 ('highY', ctypes.c_double), ('lowX', ctypes.c_double), ('highX', ctypes.c_double)]
 
 
+class OracleClosure(ctypes.Structure):
+    _fields_ = [('alphas', _dp), ('deltas', _dp), ('soundSpeed', ctypes.c_double),
+                ('referenceDensity', ctypes.c_double), ('waterTerm', ctypes.c_double)]
+
+
+def closureStruct(physics, closure: dict | None):
+    '''The OracleClosure of {'alphas', 'deltas', 'soundSpeed', 'referenceDensity', 'kinematicViscosity'}, and
+    the arrays it points at; (None, None) without a closure.'''
+    if closure is None:
+        return None, None
+    alphas = np.ascontiguousarray(closure['alphas'], dtype = float)
+    deltas = np.ascontiguousarray(closure['deltas'], dtype = float)
+    c = OracleClosure()
+    c.alphas = alphas.ctypes.data_as(_dp)
+    c.deltas = deltas.ctypes.data_as(_dp)
+    c.soundSpeed = closure['soundSpeed']
+    c.referenceDensity = closure['referenceDensity']
+    c.waterTerm = 2.0 * (physics.dims + 2) * closure['kinematicViscosity'] / closure['soundSpeed']
+    return c, (alphas, deltas)
+
+
 def build() -> str:
     '''Compile the oracle library with its Makefile; returns its path.'''
     subprocess.run(['make', '-s', '-C', _here], check = True)
@@ -217,27 +238,51 @@ def _padMatrices(matrices, count: int, fill: float) -> np.ndarray:
 
 
 def accelerations(physics: Physics, positions, velocities, densities, pressures, masses, isFluid, lengths = None,
-                  gradients = None):
+                  gradients = None, closure = None):
+    '''Accelerations of the fluid (sphGpu.py:303-414); `closure` adds the delta-LES-SPH viscosity.'''
     count, pos, vel, dens, mass, lens, fluid = _state(physics, positions, velocities, densities, masses, lengths, isFluid)
     press = np.ascontiguousarray(np.asarray(pressures, dtype = float))
     g = _padMatrices(gradients, count, 0.0) if gradients is not None else None
+    c, keep = closureStruct(physics, closure)
     out = np.zeros((count, 3))
     library().oracle_accelerations(ctypes.byref(physics.params()), count, _ptr(pos), _ptr(vel), _ptr(dens), _ptr(press),
-                                   _ptr(mass), _ptr(lens), _ptr(fluid), _ptr(g), _ptr(out))
+                                   _ptr(mass), _ptr(lens), _ptr(fluid), _ptr(g),
+                                   ctypes.byref(c) if c is not None else None, _ptr(out))
+    del keep
     return out[:, :physics.dims].copy()
 
 
+def closureCoefficients(physics: Physics, gradients, isFluid, soundSpeed: float, lengths = None):
+    '''Each particle's delta-LES-SPH alpha and delta from its G, (N, dim, dim) (sphGpu.py:699-723).'''
+    count = len(gradients)
+    g = _padMatrices(np.asarray(gradients, dtype = float), count, 0.0)
+    lens = _lengths(physics, count, lengths)
+    fluid = np.ascontiguousarray(np.asarray(isFluid, dtype = bool).view(np.uint8))
+    alphas, deltas = np.zeros(count), np.zeros(count)
+    library().oracle_closure_coefficients(ctypes.byref(physics.params()), count, _ptr(g), _ptr(lens), _ptr(fluid),
+                                          ctypes.c_double(soundSpeed), _ptr(alphas), _ptr(deltas))
+    return alphas, deltas
+
+
 def continuityRate(physics: Physics, positions, velocities, densities, masses, isFluid, lengths = None,
-                   corrections = None, densityGradients = None):
-    '''d rho / dt; 'antuono' diffusion takes the renormalized `densityGradients` (sphGpu.py:416-546).'''
+                   corrections = None, densityGradients = None, closure = None):
+    '''
+
+    d rho / dt (sphGpu.py:416-546); 'antuono' diffusion takes the renormalized `densityGradients`,
+    and `closure` each fluid pair's delta from the harmonic mean of the two particles'.
+
+    '''
     count, pos, vel, dens, mass, lens, fluid = _state(physics, positions, velocities, densities, masses, lengths, isFluid)
     l = _padMatrices(corrections, count, 1.0) if corrections is not None else None
     g = _padded(densityGradients, count) if densityGradients is not None else None
     if physics.diffusion == 'antuono' and g is None:
         raise ValueError("The 'antuono' diffusion needs the density gradients.")
+    c, keep = closureStruct(physics, closure)
     out = np.zeros(count)
     library().oracle_continuity(ctypes.byref(physics.params()), count, _ptr(pos), _ptr(vel), _ptr(dens), _ptr(mass),
-                                _ptr(lens), _ptr(fluid), _ptr(l), _ptr(g), _ptr(out))
+                                _ptr(lens), _ptr(fluid), _ptr(l), _ptr(g), ctypes.byref(c) if c is not None else None,
+                                _ptr(out))
+    del keep
     return out
 
 

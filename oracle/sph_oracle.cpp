@@ -71,6 +71,14 @@ typedef struct {
     double lowY, highY, lowX, highX;
 } OracleBoundary;
 
+// The delta-LES-SPH closure a sum carries (sphGpu.py:1754-1778): each particle's alpha and delta
+// (oracle_closure_coefficients), the closure's c0 and rho0, and K nu / c0 with K = 2 (D + 2).
+typedef struct {
+    const double* alphas;
+    const double* deltas;
+    double soundSpeed, referenceDensity, waterTerm;
+} OracleClosure;
+
 }  // extern "C"
 
 namespace {
@@ -353,7 +361,8 @@ M3 correctionAt(const OracleParams& p, const CellList& cells, const Fields& s, i
 
 // a_i = -sum_j m_j (p_i / rho_i^2 + p_j / rho_j^2 + Pi_ij) grad W_ij + g (sphGpu.py:303-414;
 // sphSolver.py:1122-1149, 1192-1230, 1328-1356).
-V3 accelerationAt(const OracleParams& p, const CellList& cells, const Fields& s, const M3* gradients, int i) {
+V3 accelerationAt(const OracleParams& p, const CellList& cells, const Fields& s, const M3* gradients, int i,
+                  const OracleClosure* closure = nullptr) {
     if (!s.fluid[i]) return {0.0, 0.0, 0.0};
     const V3 vi = vec(s.vel, i);
     const double rhoI = s.dens[i];
@@ -386,6 +395,15 @@ V3 accelerationAt(const OracleParams& p, const CellList& cells, const Fields& s,
             viscosity = -p.alpha * p.c0 * mu / (0.5 * (rhoI + rhoJ));
             if (!s.fluid[j]) viscosity *= p.boundaryFactor;
         }
+        if (closure && s.fluid[j]) {
+            // alpha_ij = K nu / (c0 h_ij) + the harmonic mean of the eddy alphas, on every fluid pair,
+            // with the unlimited v_ij . r_ij (sphGpu.py:388-393, sphSolver.py:1283-1330)
+            const double alphaI = closure->alphas[i], alphaJ = closure->alphas[j];
+            const double both = alphaI + alphaJ;
+            const double mean = both > 0.0 ? 2.0 * alphaI * alphaJ / both : 0.0;
+            viscosity += -(closure->waterTerm / q.h + mean) * closure->soundSpeed * q.h * closure->referenceDensity *
+                         dot(dv, q.d) / (q.r * q.r * rhoI * rhoJ);
+        }
         const double coefficient = -(own + s.press[j] / (rhoJ * rhoJ) + viscosity) * s.mass[j];
         const double slope = kernelSlope(p.kernel, q.r, q.h, sigmaAt(p, q.h));
         total = total + (coefficient * slope / q.r) * q.d;
@@ -396,7 +414,8 @@ V3 accelerationAt(const OracleParams& p, const CellList& cells, const Fields& s,
 // d rho_i / dt with the delta-SPH diffusive term (sphGpu.py:416-545; sphSolver.py:623-654, 718-822).
 // `deviation` holds rho - rho0.
 double continuityAt(const OracleParams& p, const CellList& cells, const Fields& s, const double* deviation,
-                    const M3* corrections, int i, const V3* gradients = nullptr) {
+                    const M3* corrections, int i, const V3* gradients = nullptr,
+                    const OracleClosure* closure = nullptr) {
     const V3 vi = vec(s.vel, i);
     const bool boundaryI = !s.fluid[i];
     const bool diffuses = p.diffusion != 0 && !boundaryI;
@@ -423,7 +442,14 @@ double continuityAt(const OracleParams& p, const CellList& cells, const Fields& 
                 // less 1/2 (grad rho_i + grad rho_j) . (r_j - r_i), with d = r_i - r_j (sphGpu.py:521-524)
                 difference += 0.5 * dot(gradients[i] + gradients[j], q.d);
             }
-            total += factor * q.h * s.mass[j] / s.dens[j] * difference * (-slope * q.r) /
+            double pairFactor = factor;
+            if (closure) {
+                // 2 c0 times the harmonic mean of the two deltas (sphGpu.py:526-531)
+                const double deltaI = closure->deltas[i], deltaJ = closure->deltas[j];
+                const double both = deltaI + deltaJ;
+                pairFactor = both > 0.0 ? 2.0 * closure->soundSpeed * 2.0 * deltaI * deltaJ / both : 0.0;
+            }
+            total += pairFactor * q.h * s.mass[j] / s.dens[j] * difference * (-slope * q.r) /
                      (q.r * q.r + kSoftening * q.h * q.h);
         }
     });
@@ -678,7 +704,7 @@ void oracle_correction(const OracleParams* p, int n, const double* pos, const do
 // `gradients9` may be null when the limiter is off.
 void oracle_accelerations(const OracleParams* p, int n, const double* pos, const double* vel, const double* dens,
                           const double* press, const double* mass, const double* len, const uint8_t* fluid,
-                          const double* gradients9, double* outAcc) {
+                          const double* gradients9, const OracleClosure* closure, double* outAcc) {
     CellList cells;
     cells.build(*p, n, pos);
     const Fields s{n, pos, vel, dens, press, mass, len, fluid};
@@ -688,13 +714,35 @@ void oracle_accelerations(const OracleParams* p, int n, const double* pos, const
         for (int i = 0; i < n; ++i) std::memcpy(g[i].m, gradients9 + 9 * (size_t)i, sizeof g[i].m);
     }
 #pragma omp parallel for schedule(dynamic, 64)
-    for (int i = 0; i < n; ++i) store(outAcc, i, accelerationAt(*p, cells, s, g.data(), i));
+    for (int i = 0; i < n; ++i) store(outAcc, i, accelerationAt(*p, cells, s, g.data(), i, closure));
+}
+
+// The closure's coefficients from each particle's G (9 per particle, row-major): alpha_i =
+// 2 (D + 2) (C_S l / h)^2 h ||D|| / c0 and delta_i = (C_delta l / h)^2 h ||D|| / c0, C_S = 0.12,
+// C_delta = 1.5, l / h = 2, ||D|| = sqrt(2 D:D) of G's symmetric part, each at most 0.2; zero on the
+// boundary (sphGpu.py:699-723, sphSolver.py:276-282).
+void oracle_closure_coefficients(const OracleParams* p, int n, const double* gradients9, const double* len,
+                                 const uint8_t* fluid, double soundSpeed, double* alphas, double* deltas) {
+    const double smagorinsky = (0.12 * 2.0) * (0.12 * 2.0), diffusion = (1.5 * 2.0) * (1.5 * 2.0), cap = 0.2;
+    for (int i = 0; i < n; ++i) {
+        if (!fluid[i]) { alphas[i] = deltas[i] = 0.0; continue; }
+        const double* g = gradients9 + 9 * (size_t)i;
+        double dd = 0.0;
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) {
+                const double strain = 0.5 * (g[3 * a + b] + g[3 * b + a]);
+                dd += strain * strain;
+            }
+        const double scale = len[i] * std::sqrt(2.0 * dd) / soundSpeed;
+        alphas[i] = std::min(2.0 * (p->dims + 2) * smagorinsky * scale, cap);
+        deltas[i] = std::min(diffusion * scale, cap);
+    }
 }
 
 // `corrections9` may be null when the correction is off.
 void oracle_continuity(const OracleParams* p, int n, const double* pos, const double* vel, const double* dens,
                        const double* mass, const double* len, const uint8_t* fluid, const double* corrections9,
-                       const double* gradients3, double* outRate) {
+                       const double* gradients3, const OracleClosure* closure, double* outRate) {
     CellList cells;
     cells.build(*p, n, pos);
     const Fields s{n, pos, vel, dens, nullptr, mass, len, fluid};
@@ -712,7 +760,7 @@ void oracle_continuity(const OracleParams* p, int n, const double* pos, const do
     }
 #pragma omp parallel for schedule(dynamic, 64)
     for (int i = 0; i < n; ++i)
-        outRate[i] = continuityAt(*p, cells, s, deviation.data(), l.data(), i, g.empty() ? nullptr : g.data());
+        outRate[i] = continuityAt(*p, cells, s, deviation.data(), l.data(), i, g.empty() ? nullptr : g.data(), closure);
 }
 
 // Each particle's renormalized density gradient, (N, 3) rows, with the L of `corrections9` (the

@@ -488,6 +488,10 @@ struct SweepOut {
     double* vel[3];
     double* density64;
     int64_t stride;
+    // host steps with the delta-LES-SPH closure: alpha_i at eddy[i] and delta_i at eddy[stride + i],
+    // in slot order, the closure's c0 and rho0, and K nu / c0 (null eddy: no closure)
+    const Real* eddy;
+    Real eddyC0, eddyRho0, eddyWater;
 };
 
 // Sweep 1's sums: the extrapolated wall pressure of a boundary particle (sphGpu.py:804-854, no
@@ -860,9 +864,39 @@ __global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
     for (int a = 0; a < D; ++a) out[a * stride + i] = vi[a] + epsilon * acc[a];
 }
 
+// The delta-LES-SPH closure's coefficients of each sorted slot from its G in `rec.G` (sphGpu.py:
+// 699-723): alpha_i = 2 (D + 2) (C_S l / h)^2 h ||D|| / c0 and delta_i = (C_delta l / h)^2 h ||D|| / c0
+// with C_S = 0.12, C_delta = 1.5, l / h = 2 and ||D|| = sqrt(2 D:D) of G's symmetric part, each at
+// most 0.2, zero on the boundary; alpha at eddy[i], delta at eddy[stride + i].
+template <int D, typename Real>
+__global__ void closureCoefficients(const int32_t* __restrict__ order, Records<Real> rec, Real* __restrict__ eddy,
+                                    int64_t stride, Real soundSpeed, int64_t n) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const Real sh = signedLength<D, Real>(rec.r[s]);
+    Real alpha = Real(0), delta = Real(0);
+    if (sh > Real(0)) {
+        Real dd = Real(0);
+#pragma unroll
+        for (int a = 0; a < D; ++a)
+#pragma unroll
+            for (int b = 0; b < D; ++b) {
+                const Real strain = Real(0.5) * (rec.G[s * (D * D) + a * D + b] + rec.G[s * (D * D) + b * D + a]);
+                dd += strain * strain;
+            }
+        const Real scale = sh * sqrt(Real(2) * dd) / soundSpeed;
+        const Real smagorinsky = (Real)((0.12 * 2.0) * (0.12 * 2.0)), diffusion = (Real)((1.5 * 2.0) * (1.5 * 2.0));
+        alpha = fmin(Real(2 * (D + 2)) * smagorinsky * scale, Real(0.2));
+        delta = fmin(diffusion * scale, Real(0.2));
+    }
+    const int32_t i = order[s];
+    eddy[i] = alpha;
+    eddy[stride + i] = delta;
+}
+
 // Sweep 2: accelerations with the limited viscosity (sphGpu.py:303-414) and, when asked, the
 // correction matrix L (sphGpu.py:593-641) on fluid particles.
-template <int D, typename Real, int Lanes>
+template <int D, typename Real, int Lanes, bool HostOptions = false>
 __global__ void SWEEP_BOUNDS((accelerationWaves<D, Real>()))
     sweepAccelerations(const RunDev* __restrict__ runs, Tiles tiles, const int32_t* __restrict__ order, const int32_t* __restrict__ start, Records<Real> rec,
                        NeighborList list, SweepOut<Real> out, Clock clock, const double* __restrict__ y64,
@@ -916,6 +950,10 @@ __global__ void SWEEP_BOUNDS((accelerationWaves<D, Real>()))
 #pragma unroll
                     for (int b = 0; b < D; ++b) Gi[a][b] = rec.G[s * (D * D) + a * D + b];
             const Real alphaC0 = (Real)(run.alpha * run.c0), bf = (Real)run.boundaryFactor;
+            Real eddyI = Real(0);
+            if constexpr (HostOptions) {
+                if (out.eddy) eddyI = out.eddy[order[s]];
+            }
             forEachNeighbor<D, Real, Lanes, true>(
                 run, start, cells, list, s, me, yy, lane, kernel.uniform,
                 [&](int32_t k, const Record<Real>& other, Real dx, Real dy, Real dz, Real rr, Real h) {
@@ -984,6 +1022,20 @@ __global__ void SWEEP_BOUNDS((accelerationWaves<D, Real>()))
                         const Real mu = quotient(h * approach, rr * rr + soft);
                         viscosity = quotient(-alphaC0 * mu, Real(0.5) * (rhoI + rhoJ));
                         if (!fluidJ) viscosity *= bf;
+                    }
+                    if constexpr (HostOptions) {
+                        if (out.eddy && fluidJ) {
+                            // the closure on every fluid pair: alpha_ij = K nu / (c0 h_ij) + the harmonic
+                            // mean of the eddy alphas, with the unlimited v_ij . r_ij (sphGpu.py:388-393)
+                            const Real eddyJ = out.eddy[order[k]];
+                            const Real both = eddyI + eddyJ;
+                            const Real mean = both > Real(0) ? Real(2) * eddyI * eddyJ / both : Real(0);
+                            Real plain = Real(0);
+#pragma unroll
+                            for (int a = 0; a < D; ++a) plain += dv[a] * d[a];
+                            viscosity += -(out.eddyWater / h + mean) * out.eddyC0 * h * out.eddyRho0 * plain /
+                                         (rr * rr * rhoI * rhoJ);
+                        }
                     }
                     const Real coefficient = -(own + pressureRatioOf<D, Real>(other) + viscosity) * massOf<D, Real>(other) * F;
 #pragma unroll
@@ -1136,7 +1188,7 @@ __device__ __forceinline__ void enforceBoundary(const sph_boundary& b, double* x
 // float32 on the records as they are read), and each particle's float64 kick, density advance
 // with its floor (:1111-1140), drift (:1142-1186) and boundary enforcement follow; the run's first
 // particle advances its clock.
-template <int D, typename Real, int Lanes, bool Renormalized = false>
+template <int D, typename Real, int Lanes, bool HostOptions = false>
 __global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
     sweepContinuity(const RunDev* __restrict__ runs, Tiles tiles, const int32_t* __restrict__ order, const int32_t* __restrict__ start, Records<Real> rec,
                     NeighborList list, SweepOut<Real> out, Clock clock, int32_t resident, int32_t hostCorrection,
@@ -1190,9 +1242,12 @@ __global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
         }
         const Real devI = deviationOf<D, Real>(rec.r[s]);
         Real gradI[3] = {Real(0), Real(0), Real(0)};
-        if constexpr (Renormalized) {
+        Real eddyI = Real(0);
+        if constexpr (HostOptions) {
+            if (diffusion == 3)
 #pragma unroll
-            for (int a = 0; a < D; ++a) gradI[a] = rec.G[s * D + a];
+                for (int a = 0; a < D; ++a) gradI[a] = rec.G[s * D + a];
+            if (out.eddy) eddyI = out.eddy[out.stride + order[s]];
         }
         const Real factor = (Real)(2.0 * run.delta * run.c0);
         const Real invRho0 = (Real)(1.0 / run.rho0);
@@ -1228,16 +1283,25 @@ __global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
                         const Real sq = ratio * ratio;
                         difference -= quotient(head * d[D - 1], sq * sq * ratio);
                     }
-                    if constexpr (Renormalized) {
-                        // less 1/2 (grad rho_i + grad rho_j) . (r_j - r_i), d = r_i - r_j (sphGpu.py:521-524)
-                        Real projection = Real(0);
+                    Real pairFactor = factor;
+                    if constexpr (HostOptions) {
+                        if (diffusion == 3) {
+                            // less 1/2 (grad rho_i + grad rho_j) . (r_j - r_i), d = r_i - r_j (sphGpu.py:521-524)
+                            Real projection = Real(0);
 #pragma unroll
-                        for (int a = 0; a < D; ++a) projection += (gradI[a] + rec.G[(int64_t)k * D + a]) * d[a];
-                        difference += Real(0.5) * projection;
+                            for (int a = 0; a < D; ++a) projection += (gradI[a] + rec.G[(int64_t)k * D + a]) * d[a];
+                            difference += Real(0.5) * projection;
+                        }
+                        if (out.eddy) {
+                            // 2 c0 times the harmonic mean of the two deltas (sphGpu.py:526-531)
+                            const Real eddyJ = out.eddy[out.stride + order[k]];
+                            const Real both = eddyI + eddyJ;
+                            pairFactor = both > Real(0) ? Real(2) * out.eddyC0 * Real(2) * eddyI * eddyJ / both : Real(0);
+                        }
                     }
                     const Real soft = kernel.uniform ? kernel.soft : Real(0.01) * h * h;
                     const Real r2 = rr * rr;
-                    total += factor * h * volumeOf<D, Real>(other) * difference * quotient(-F * r2, r2 + soft);
+                    total += pairFactor * h * volumeOf<D, Real>(other) * difference * quotient(-F * r2, r2 + soft);
                 }
             });
     }
@@ -2071,6 +2135,13 @@ struct sph_context {
     // The merge's per-slot and per-cell buffers
     void* mergeScratch = nullptr;
     size_t mergeBytes = 0;
+    // the delta-LES-SPH closure of the host steps: on, c0, rho0, K nu / c0, and the eddy alphas and
+    // deltas of the last sum that built them, zero until one has
+    int32_t closureOn = 0;
+    double closureC0 = 1.0, closureRho0 = 1000.0, closureWater = 0.0;
+    void* eddy = nullptr;
+    size_t eddyBytes = 0;
+    int64_t eddyStride = 0;
     void* mergeCells = nullptr;
     size_t mergeCellBytes = 0;
     // The split's marks and ranks between sph_split_count and sph_split_apply
@@ -2195,6 +2266,10 @@ void releaseScratch(sph_context* ctx) {
     if (ctx->mergeScratch) (void)hipFree(ctx->mergeScratch);
     ctx->mergeScratch = nullptr;
     ctx->mergeBytes = 0;
+    if (ctx->eddy) (void)hipFree(ctx->eddy);
+    ctx->eddy = nullptr;
+    ctx->eddyBytes = 0;
+    ctx->eddyStride = 0;
     if (ctx->mergeCells) (void)hipFree(ctx->mergeCells);
     ctx->mergeCells = nullptr;
     ctx->mergeCellBytes = 0;
@@ -2444,6 +2519,8 @@ int requireBound(sph_context* ctx) {
 // The device step runs the delta-SPH scheme's options only: a run with the renormalized diffusion
 // is refused before anything is launched or captured.
 int refuseHostOnly(sph_context* ctx) {
+    if (ctx->closureOn)
+        return fail(ctx, SPH_ERR_VALUE, "The device step does not carry the delta-LES-SPH closure; its host-step sums do.");
     for (const RunDev& run : ctx->runsHost)
         if (run.diffusion == 3)
             return fail(ctx, SPH_ERR_VALUE,
@@ -2537,6 +2614,24 @@ int dispatchPhase(sph_context* ctx, int phase, double forcedDt) {
     return ctx->precision == 0 ? stepOnce<3, float>(ctx, phase, 0, forcedDt) : stepOnce<3, double>(ctx, phase, 0, forcedDt);
 }
 
+// The closure's eddy alphas and deltas, two rows of the state's stride, zeroed when first made or
+// when the stride changes.
+template <typename Real>
+int ensureEddy(sph_context* ctx) {
+    const int64_t stride = ctx->state.stride;
+    const size_t bytes = 2 * (size_t)stride * sizeof(Real);
+    if (ctx->eddy && ctx->eddyStride == stride && bytes <= ctx->eddyBytes) return SPH_OK;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (ctx->eddy) (void)hipFree(ctx->eddy);
+    ctx->eddy = nullptr;
+    ctx->eddyBytes = 0;
+    HIP_TRY(hipMalloc(&ctx->eddy, bytes));
+    ctx->eddyBytes = bytes;
+    ctx->eddyStride = stride;
+    HIP_TRY(hipMemsetAsync(ctx->eddy, 0, bytes, ctx->stream));
+    return SPH_OK;
+}
+
 // Host-step sums: gather records from caller arrays, then one sweep.
 template <int D, typename Real>
 int hostSweep(sph_context* ctx, int which, const double* velocity, const double* density, const void* pressure,
@@ -2551,6 +2646,15 @@ int hostSweep(sph_context* ctx, int which, const double* velocity, const double*
     Sources<Real> src = sourcesOf<Real>(ctx, vel, density, static_cast<const Real*>(pressure), nullptr);
     gatherRecords<D, Real><<<gridFor(n, b), b, 0, ctx->stream>>>(ctx->dRuns, ctx->runOf, ctx->order, src, rec, ctx->clock, 0, n);
     SweepOut<Real> out = emptyOut<Real>(ctx);
+    const bool closure = ctx->closureOn != 0;
+    if (closure) {
+        const int status = ensureEddy<Real>(ctx);
+        if (status != SPH_OK) return status;
+        out.eddy = static_cast<Real*>(ctx->eddy);
+        out.eddyC0 = (Real)ctx->closureC0;
+        out.eddyRho0 = (Real)ctx->closureRho0;
+        out.eddyWater = (Real)ctx->closureWater;
+    }
     const double* y64 = ctx->state.position + ctx->state.stride;
     const RunDev& run0 = ctx->runsHost[0];
     const int32_t* order = ctx->order;
@@ -2577,18 +2681,31 @@ int hostSweep(sph_context* ctx, int which, const double* velocity, const double*
             WITH_LANES(ctx, (sweepAccelerations<D, Real, L><<<tileGrid(ctx), b, sweepLds<D, Real>(ctx, 0), ctx->stream>>>(
                             ctx->dRuns, tilesOf(ctx), order, start, rec, list, out, ctx->clock, y64, 0, 0, 1, 0, nearRecords<D, Real>(ctx), n)));
             break;
-        case 3: {  // accelerations, with the limiter's gradients first when the run has it
+        case 3:    // accelerations, with G first when the limiter or the closure needs it
+        case 8: {  // the closure's coefficients alone
             const int limiter = run0.limiter;
-            if (limiter)
+            if (which == 8 && !closure) return fail(ctx, SPH_ERR_VALUE, "The closure is off; see sph_set_closure.");
+            if (limiter || closure)
                 WITH_LANES(ctx, (sweepWallAndGradients<D, Real, L><<<tileGrid(ctx), b, 0, ctx->stream>>>(
                             ctx->dRuns, tilesOf(ctx), order, start, rec, list, out, ctx->clock, y64, 0, 0, 1, n)));
+            if (closure)
+                closureCoefficients<D, Real><<<gridFor(n, b), b, 0, ctx->stream>>>(
+                    order, rec, static_cast<Real*>(ctx->eddy), ctx->state.stride, (Real)ctx->closureC0, n);
+            if (which == 8) break;
             out.acceleration = static_cast<Real*>(out0);
-            WITH_LANES(ctx, (sweepAccelerations<D, Real, L><<<tileGrid(ctx), b, sweepLds<D, Real>(ctx, 0), ctx->stream>>>(
+            if (closure) {
+                WITH_LANES(ctx, (sweepAccelerations<D, Real, L, true><<<tileGrid(ctx), b, sweepLds<D, Real>(ctx, 0),
+                                                                        ctx->stream>>>(
                             ctx->dRuns, tilesOf(ctx), order, start, rec, list, out, ctx->clock, y64, 0, 1, 0, limiter, nearRecords<D, Real>(ctx), n)));
+            } else {
+                WITH_LANES(ctx, (sweepAccelerations<D, Real, L><<<tileGrid(ctx), b, sweepLds<D, Real>(ctx, 0), ctx->stream>>>(
+                            ctx->dRuns, tilesOf(ctx), order, start, rec, list, out, ctx->clock, y64, 0, 1, 0, limiter, nearRecords<D, Real>(ctx), n)));
+            }
             break;
         }
         case 4: {  // continuity, with the correction matrices first when the run has them or the
-                   // renormalized diffusion, then that diffusion's density gradients
+                   // renormalized diffusion, then that diffusion's density gradients; with the
+                   // closure, the deltas of the last sum that built them
             const int correction = run0.correction;
             const bool renormalized = run0.diffusion == 3;
             if (correction || renormalized)
@@ -2600,7 +2717,7 @@ int hostSweep(sph_context* ctx, int which, const double* velocity, const double*
                                 ctx->state.stride)));
             out.rate = static_cast<Real*>(out0);
             for (int a = 0; a < 3; ++a) out.pos[a] = a < D ? ctx->state.position + a * ctx->state.stride : nullptr;
-            if (renormalized) {
+            if (renormalized || closure) {
                 WITH_LANES(ctx, (sweepContinuity<D, Real, L, true><<<tileGrid(ctx), b,
                                                                     sweepLds<D, Real>(ctx, 1), ctx->stream>>>(
                             ctx->dRuns, tilesOf(ctx), order, start, rec, list, out, ctx->clock, 0, correction,
@@ -2933,6 +3050,33 @@ int sph_pair_accelerations(sph_context* ctx, const double* velocity, const doubl
 int sph_continuity_rate(sph_context* ctx, const double* velocity, const double* density, void* out) {
     if (!velocity || !out) return fail(ctx, SPH_ERR_VALUE, "The continuity rate needs velocities and an output.");
     return dispatchHostSweep(ctx, 4, velocity, density, nullptr, out, nullptr);
+}
+
+int sph_set_closure(sph_context* ctx, int32_t enabled, double soundSpeed, double referenceDensity,
+                    double kinematicViscosity) {
+    if (!ctx) return SPH_ERR_VALUE;
+    if (enabled && !(soundSpeed > 0.0 && referenceDensity > 0.0 && kinematicViscosity >= 0.0))
+        return fail(ctx, SPH_ERR_VALUE, "The closure needs a positive c0 and rho0 and a viscosity of at least zero.");
+    ctx->closureOn = enabled ? 1 : 0;
+    if (enabled) {
+        ctx->closureC0 = soundSpeed;
+        ctx->closureRho0 = referenceDensity;
+        ctx->closureWater = 2.0 * (ctx->dims + 2) * kinematicViscosity / soundSpeed;
+    }
+    return SPH_OK;
+}
+
+int sph_closure_coefficients(sph_context* ctx, const double* velocity, const double* density, void* alphas,
+                             void* deltas) {
+    if (!velocity || !alphas || !deltas) return fail(ctx, SPH_ERR_VALUE, "The closure needs velocities and two outputs.");
+    int status = dispatchHostSweep(ctx, 8, velocity, density, nullptr, nullptr, nullptr);
+    if (status != SPH_OK || ctx->total == 0) return status;
+    const size_t real = ctx->precision == 0 ? sizeof(float) : sizeof(double);
+    const size_t bytes = (size_t)ctx->total * real;
+    const char* eddy = static_cast<const char*>(ctx->eddy);
+    HIP_TRY(hipMemcpyAsync(alphas, eddy, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(deltas, eddy + (size_t)ctx->state.stride * real, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    return SPH_OK;
 }
 
 int sph_density_gradients(sph_context* ctx, const double* density, void* out) {

@@ -6,7 +6,7 @@ tensors and every sum, step and enforcement run by hand-written HIP kernels.
 
     host steps      `update`, then `pairAccelerations`, `continuityRate`, `wallPressure`,
                     `velocityGradients`, `correctionMatrices`, `densityGradients`, `xsphVelocity`,
-                    `shepardDensity`:
+                    `shepardDensity`, `closureCoefficients`:
                     host float64 arrays in, host float64 arrays out, shaped as the reference
                     returns them
     device steps    `setPhysics`, `push`, `setBoundary`, then `residentRates` and
@@ -17,12 +17,13 @@ Pair arithmetic is float32 over float64 state by default, each float64 position 
 float32 point and remainder as the reference does (sphGpu.py:28-38); `pairPrecision = 'float64'`
 runs the plain float64 kernels of the same source.
 
-Of the options the delta-SPH scheme leaves off, XSPH, the Shepard filter and the renormalized
-'antuono' diffusion (`densityGradients`, and `continuityRate` under setDiffusion('antuono')) are
-built as host-step sums; the device step runs without them, and setPhysics, residentRates and the
-device steps raise ValueError when asked for one. Particle shifting, the delta-LES closure and the
-energy budget's ledger are not built: their setters accept them switched off and raise ValueError
-when asked to switch one on.
+Of the options the delta-SPH scheme leaves off, XSPH, the Shepard filter, the renormalized
+'antuono' diffusion (`densityGradients`, and `continuityRate` under setDiffusion('antuono')) and
+the delta-LES-SPH closure (`setClosure`, `closureCoefficients`, and the closure's terms in
+`pairAccelerations` and `continuityRate`) are built as host-step sums; the device step runs without
+them, and setPhysics, residentRates and the device steps raise ValueError when asked for one.
+Particle shifting and the energy budget's ledger are not built: their setters accept them switched
+off and raise ValueError when asked to switch one on.
 
 '''
 
@@ -503,6 +504,7 @@ class PairSums:
         self.negativePressure = False
         self.correctContinuity = False
         self.limitViscosity = False
+        self.closure = {'enabled': False, 'soundSpeed': 1.0, 'referenceDensity': 1000.0, 'kinematicViscosity': 0.0}
         self.trackEnergy = False
         self.engine = Engine(dimensions, pairPrecision, device)
         self.device = self.engine.device
@@ -582,9 +584,37 @@ class PairSums:
 
     def setClosure(self, enabled: bool, soundSpeed: float = 1.0, referenceDensity: float = 1000.0,
                    kinematicViscosity: float = 1e-6) -> None:
-        '''The delta-LES-SPH closure (sphGpu.py:1742-1775), off under the delta-SPH scheme and not built.'''
-        if enabled:
-            raise ValueError('The delta-LES-SPH closure is not built; the delta-SPH scheme runs without it.')
+
+        '''
+
+        Whether the host-step accelerations and diffusive term carry the delta-LES-SPH closure
+        (sphGpu.py:1754-1778): each fluid pair's viscosity adds K nu / (c0 h) plus the harmonic mean
+        of the eddy alphas, and the diffusive term takes the harmonic mean of the eddy deltas. The
+        device steps refuse it; the delta-SPH scheme runs without it.
+
+        '''
+
+        self.closure = {'enabled': bool(enabled), 'soundSpeed': float(soundSpeed),
+                        'referenceDensity': float(referenceDensity), 'kinematicViscosity': float(kinematicViscosity)}
+        self.check(self.engine.library.sph_set_closure(self.engine.handle, 1 if enabled else 0, float(soundSpeed),
+                                                       float(referenceDensity), float(kinematicViscosity)),
+                   'sph_set_closure')
+
+    def closureCoefficients(self, velocities: np.ndarray, densities: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+        '''Each particle's eddy alpha and delta at the current positions (sphGpu.py:1841-1853).'''
+        if not self.closure['enabled']:
+            raise ValueError('The closure is off; see setClosure.')
+        self._sync()
+        engine = self.engine
+        engine.upload('velocityIn', velocities)
+        engine.upload('densityIn', densities)
+        deltas = torch.empty(max(self.count, 1), dtype = engine.tensors['scalar'].dtype, device = self.device)
+        self.check(engine.library.sph_closure_coefficients(engine.handle, engine.pointer('velocityIn'),
+                                                           engine.pointer('densityIn'), engine.pointer('scalar'),
+                                                           ctypes.c_void_p(deltas.data_ptr())),
+                   'sph_closure_coefficients')
+        engine.synchronize()
+        return engine.download('scalar'), deltas[:self.count].double().cpu().numpy()
 
     def setEnergyBudget(self, enabled: bool) -> None:
         '''The per-particle energy ledger (sphGpu.py:1777-1789), not built: `globalSums` gives the float64 totals.'''
@@ -734,8 +764,8 @@ class PairSums:
 
     def velocityGradients(self, velocities: np.ndarray, densities: np.ndarray) -> np.ndarray:
         '''G for every particle, shape (N, dim, dim) (sphGpu.py:1827-1839).'''
-        if not self.limitViscosity:
-            raise ValueError('The viscosity limiter is not on; see setViscosityLimiter.')
+        if not (self.limitViscosity or self.closure['enabled']):
+            raise ValueError('Neither the viscosity limiter nor the closure is on; see setViscosityLimiter, setClosure.')
         self._sync()
         engine = self.engine
         engine.upload('velocityIn', velocities)

@@ -173,6 +173,9 @@ class WcsphSolver:
         if config.xsphEpsilon > 0.0:
             raise ValueError('The device step does not move the fluid with XSPH (PairSums.xsphVelocity gives it as a '
                              'host step); the delta-SPH scheme sets xsphEpsilon to 0.')
+        if config.turbulenceClosure == 'deltaLes':
+            raise ValueError("The device step does not carry the delta-LES-SPH closure (PairSums.setClosure gives it "
+                             "to the host-step sums); the delta-SPH scheme sets turbulenceClosure to 'none'.")
         if config.densityDiffusion == 'antuono':
             raise ValueError("The device step does not carry the renormalized 'antuono' diffusion (PairSums."
                              "continuityRate gives it as a host step); the delta-SPH scheme uses 'hydrostatic'.")

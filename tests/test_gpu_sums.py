@@ -127,6 +127,50 @@ def testRenormalizedDiffusionMatchesTheOracle(precision, dimensions, periodic, k
     assert sums.correctionMatrices(case['densities']) == pytest.approx(L, abs = 2 * tolerance[precision])
 
 
+@pytest.mark.parametrize('precision', precisions)
+@pytest.mark.parametrize('dimensions, periodic', cases)
+@pytest.mark.parametrize('mode', ['hydrostatic', 'antuono'])
+def testClosureMatchesTheOracle(precision, dimensions, periodic, mode):
+    # tests/testSphGpu.py:216-290: the coefficients from G of a strained flow, then the accelerations
+    # with the closure's viscosity and the continuity with each pair's harmonic-mean delta
+    print(f'closure {precision} {dimensions}D periodic={periodic} {mode}', flush = True)
+    case = cloud(dimensions, periodic)
+    fluid = case['isFluid']
+    velocities = strainedFlow(case)
+    nu = 1.0034e-6
+    physics = physicsFor(case, boundaryFactor = 0.3, diffusion = mode, delta = 0.1)
+    G = oracle.velocityGradients(physics, case['positions'], velocities, case['densities'], case['masses'], fluid)
+    alphas, deltas = oracle.closureCoefficients(physics, G, fluid, c0)
+    closure = dict(alphas = alphas, deltas = deltas, soundSpeed = c0, referenceDensity = rho0, kinematicViscosity = nu)
+    expected = oracle.accelerations(physics, case['positions'], velocities, case['densities'], case['pressures'],
+                                    case['masses'], fluid, closure = closure)
+    densityGradients = None
+    if mode == 'antuono':
+        L = oracle.correctionMatrices(physics, case['positions'], case['densities'], case['masses'], fluid)
+        densityGradients = oracle.densityGradients(physics, case['positions'], case['densities'], case['masses'], fluid,
+                                                   L)
+    expectedRate = oracle.continuityRate(physics, case['positions'], velocities, case['densities'], case['masses'],
+                                         fluid, densityGradients = densityGradients, closure = closure)
+    sums = pairSums(case, precision)
+    sums.setDiffusion(mode, 0.1, c0, rho0, 9.81)
+    sums.setClosure(True, c0, rho0, nu)
+    deviceAlphas, deviceDeltas = sums.closureCoefficients(velocities, case['densities'])
+    scale = tolerance[precision]
+    assert deviceAlphas == pytest.approx(alphas, abs = 20 * scale * 0.2)
+    assert deviceDeltas == pytest.approx(deltas, abs = 20 * scale * 0.2)
+    cap = float(np.float32(0.2)) if precision == 'float32' else 0.2
+    assert np.all(deviceAlphas[~fluid] == 0.0) and np.any(deviceDeltas[fluid] == cap)
+    result = sums.pairAccelerations(velocities, case['densities'], case['pressures'], c0, alpha, 0.3,
+                                    gravityOf(dimensions))
+    assert result == pytest.approx(expected, abs = scale * np.abs(expected).max())
+    rate = sums.continuityRate(velocities, case['densities'])
+    assert rate == pytest.approx(expectedRate, abs = scale * np.abs(expectedRate).max())
+    # The device step refuses the closure before it launches anything
+    sums.setPhysics(rho0, c0, alpha, 0.3, gravityOf(dimensions), 0.0, 0.0)
+    with pytest.raises(ValueError, match = 'closure'):
+        sums.residentRates()
+
+
 def testTheDeviceStepRefusesTheRenormalizedDiffusion():
     case = cloud(2, False)
     sums = pairSums(case, 'float32')

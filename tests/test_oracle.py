@@ -284,6 +284,49 @@ def testShepardAndXsphWithMixedLengthsMatchTheArrayFormulation():
     assert xsph[fluid] == pytest.approx(expected[fluid], abs = 1e-13 * np.abs(expected).max())
 
 
+@pytest.mark.parametrize('dimensions, periodic', cases)
+def testClosureMatchesTheArrayFormulation(dimensions, periodic):
+    # tests/testSphGpu.py:216-290: the coefficients from G of a strained flow, the viscosity on
+    # every fluid pair with K nu / (c0 h) plus the harmonic mean, the harmonic-mean delta in the
+    # hydrostatic diffusion
+    case = cloud(dimensions, periodic)
+    i, j, fluid = case['i'], case['j'], case['isFluid']
+    velocities = strainedFlow(case)
+    relative = velocities[i] - velocities[j]
+    G = af.velocityGradients(case['masses'], case['densities'], velocities, i, j, case['separation'],
+                             case['gradients'], fluid)
+    alphas, deltas = af.closureCoefficients(G, fluid, case['h'], c0)
+    physics = physicsFor(case, boundaryFactor = 0.3)
+    result = oracle.closureCoefficients(physics, G, fluid, c0)
+    assert result[0] == pytest.approx(alphas, rel = 1e-14, abs = 1e-300)
+    assert result[1] == pytest.approx(deltas, rel = 1e-14, abs = 1e-300)
+    assert np.any(deltas[fluid] == 0.2) and np.any((alphas[fluid] > 0.0) & (alphas[fluid] < 0.2))
+    nu = 1.0e-6
+    densities = case['densities']
+    viscosity = af.artificialViscosity(relative, case['separation'], case['distance'], 0.5 * (densities[i] + densities[j]),
+                                       case['h'], c0, alpha)
+    viscosity[~(fluid[i] & fluid[j])] *= 0.3
+    pairs = fluid[i] & fluid[j]
+    viscosity[pairs] += af.closureViscosity(relative[pairs], case['separation'][pairs], case['distance'][pairs],
+                                            densities[i][pairs], densities[j][pairs], case['h'], c0, rho0,
+                                            alphas[i][pairs], alphas[j][pairs], nu)
+    expected = af.pairAccelerations(case['masses'], densities, case['pressures'], i, j, case['gradients'], viscosity,
+                                    len(fluid))
+    expected[fluid] += _gravity(dimensions)
+    expected[~fluid] = 0.0
+    closure = dict(alphas = alphas, deltas = deltas, soundSpeed = c0, referenceDensity = rho0, kinematicViscosity = nu)
+    accelerations = oracle.accelerations(physics, case['positions'], velocities, densities, case['pressures'],
+                                         case['masses'], fluid, closure = closure)
+    assert accelerations == pytest.approx(expected, abs = 1e-12 * np.abs(expected).max())
+    rate = af.continuityRate(case['masses'], velocities, i, j, case['gradients'], len(fluid), fluid[i] | fluid[j])
+    rate += af.densityDiffusionRate(case['masses'], densities, i, j, case['separation'], case['distance'],
+                                    case['gradients'], len(fluid), case['h'], c0, af.harmonicMean(deltas[i], deltas[j]),
+                                    rho0, 9.81, pairs)
+    result = oracle.continuityRate(physicsFor(case, diffusion = 'hydrostatic', delta = 0.1), case['positions'],
+                                   velocities, densities, case['masses'], fluid, closure = closure)
+    assert result == pytest.approx(rate, abs = 1e-12 * np.abs(rate).max())
+
+
 # -- Physics the formulas must satisfy -- #
 def testPairForcesConserveMomentum():
     # tests/testSphSolver.py:126-145: a gravity-free all-fluid cloud keeps sum m a = 0 to rounding
@@ -462,6 +505,40 @@ def testRenormalizedGradientsAreExactAndTheirDiffusionVanishesOnALinearDensity()
     physics.diffusion = 'molteni'
     molteni = oracle.continuityRate(physics, positions, still, densities, masses, fluid, corrections = L)
     assert np.abs(renormalized).max() < 1e-9 * np.abs(molteni).max()
+
+
+def testClosureCoefficientsOfAShearAndARotation():
+    # sphSolver.py:1240-1281 by hand: a shear G = [[0, s], [0, 0]] has ||D|| = |s|; a rotation none
+    case = cloud(2, False)
+    physics = physicsFor(case)
+    h, shear = case['h'], 3.0
+    G = np.array([[[0.0, shear], [0.0, 0.0]], [[0.0, -shear], [shear, 0.0]], [[0.0, 400.0], [0.0, 0.0]]])
+    alphas, deltas = oracle.closureCoefficients(physics, G, np.ones(3, dtype = bool), c0)
+    scale = h * shear / c0
+    assert alphas[:2] == pytest.approx([2.0 * 4 * 0.24**2 * scale, 0.0], rel = 1e-14, abs = 1e-300)
+    assert deltas[:2] == pytest.approx([9.0 * scale, 0.0], rel = 1e-14, abs = 1e-300)
+    assert alphas[2] == deltas[2] == 0.2
+    assert oracle.closureCoefficients(physics, G, np.zeros(3, dtype = bool), c0)[0].max() == 0.0
+
+
+def testClosureViscosityLeavesARotationAloneAndConservesMomentum():
+    # sphSolver.py:1296-1300: along r_ij, so a rigid rotation feels none of it and sum m a = 0
+    case = cloud(3, False)
+    fluid = np.ones(len(case['positions']), dtype = bool)
+    physics = physicsFor(case, gravity = np.zeros(3), alpha = 0.0)
+    zero = np.zeros(len(fluid))
+    rng = np.random.default_rng(4)
+    closure = dict(alphas = rng.uniform(0.0, 0.2, len(fluid)), deltas = zero, soundSpeed = c0, referenceDensity = rho0,
+                   kinematicViscosity = 1.0e-3)
+    centre = case['positions'].mean(axis = 0)
+    rotation = np.cross([0.3, -0.5, 0.8], case['positions'] - centre)
+    a = oracle.accelerations(physics, case['positions'], rotation, case['densities'], zero, case['masses'], fluid,
+                             closure = closure)
+    stirred = oracle.accelerations(physics, case['positions'], case['velocities'], case['densities'], zero,
+                                   case['masses'], fluid, closure = closure)
+    assert np.abs(a).max() < 1e-12 * np.abs(stirred).max()
+    momentum = np.sum(case['masses'][:, None] * stirred, axis = 0)
+    assert np.all(np.abs(momentum) < 1e-12 * np.sum(case['masses'][:, None] * np.abs(stirred), axis = 0))
 
 
 def _waterOnABed(s = 0.01, width = 0.3, depth = 0.1, g = 9.80665):
