@@ -129,6 +129,14 @@ int sph_density_gradients(sph_context* ctx, const double* density, void* out);  
  * from the deltas the last such sum built (zero before any). The device steps refuse it. */
 int sph_set_closure(sph_context* ctx, int32_t enabled, double soundSpeed, double referenceDensity,
                     double kinematicViscosity);                                   /* sphGpu.py:1754 */
+/* Particle shifting as a host step (sphGpu.py:1966-1979, runs of one smoothing length): each slot's
+ * shift (D rows of `stride`), zero on the boundary and the free surface, the density change that
+ * goes with it and the velocity change (D rows), all in the pair precision. `size` is the shift's
+ * size, CFL Ma (2h)^2 or, with `local`, A h dt to be scaled by each |v_i|; `spacingKernel` is W at
+ * the particle spacing. The device steps do not shift. */
+int sph_shifting_displacement(sph_context* ctx, const double* velocity, const double* density, double size,
+                              int32_t local, double spacingKernel, void* shifts, void* densityChanges,
+                              void* velocityChanges);                             /* sphGpu.py:1966 */
 /* Each slot's eddy alpha and delta at the bound positions, into two device arrays of the pair
  * precision: PairSums.closureCoefficients (sphGpu.py:1841). */
 int sph_closure_coefficients(sph_context* ctx, const double* velocity, const double* density, void* alphas,

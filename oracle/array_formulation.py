@@ -265,6 +265,73 @@ closureFilterRatio = 2.0     # sphSolver.py:281
 closureCap = 0.2             # sphSolver.py:282
 
 
+antiClumpingWeight = 0.2                                      # sphSolver.py:258
+freeSurfaceDivergence = {2: 1.5, 3: 2.4}                      # sphSolver.py:265
+freeSurfaceLopsidedness = {'cubicSpline': 0.3, 'wendlandC2': 0.4}  # sphSolver.py:266
+surfaceGradient = 0.03                                        # sphSolver.py:271
+
+
+def shiftingFields(masses, densities, i, j, separation, values, gradients, count, spacingKernel):
+    '''grad C with the anti-clumping weight, grad C0, div r and grad rho over the pairs (sphSolver.py:824-887).'''
+    volumes = np.broadcast_to(np.asarray(masses, dtype = float), (count,)) / densities
+    weight = 1.0 + antiClumpingWeight * (values / spacingKernel)**4
+    dimensions = separation.shape[1]
+    gradient, plain = np.zeros((count, dimensions)), np.zeros((count, dimensions))
+    scatterAdd(gradient, i, (volumes[j] * weight)[:, None] * gradients)
+    scatterAdd(gradient, j, -(volumes[i] * weight)[:, None] * gradients)
+    scatterAdd(plain, i, volumes[j][:, None] * gradients)
+    scatterAdd(plain, j, -volumes[i][:, None] * gradients)
+    term = -np.sum(separation * gradients, axis = 1)
+    divergence = np.zeros(count)
+    scatterAdd(divergence, i, volumes[j] * term)
+    scatterAdd(divergence, j, volumes[i] * term)
+    difference = (densities[j] - densities[i])[:, None] * gradients
+    densityGradient = np.zeros((count, dimensions))
+    scatterAdd(densityGradient, i, volumes[j][:, None] * difference)
+    scatterAdd(densityGradient, j, volumes[i][:, None] * difference)
+    return gradient, plain, divergence, densityGradient
+
+
+def freeSurface(plain, divergence, isFluid, h, kernelType):
+    '''div r below 1.5 (2D) or 2.4 (3D), or h |grad C0| above the kernel's lopsidedness (sphSolver.py:889-902).'''
+    lopsided = h * np.linalg.norm(plain, axis = 1) > freeSurfaceLopsidedness[kernelType]
+    return np.asarray(isFluid, dtype = bool) & ((divergence < freeSurfaceDivergence[plain.shape[1]]) | lopsided)
+
+
+def shiftingDisplacement(gradient, plain, divergence, densityGradient, i, j, values, isFluid, size, h, kernelType):
+    '''The shift, zero on the surface, projected beside it, and its density change (sphSolver.py:904-959).'''
+    fluid = np.asarray(isFluid, dtype = bool)
+    surface = freeSurface(plain, divergence, fluid, h, kernelType)
+    length = np.linalg.norm(plain, axis = 1)
+    surfaceNormal = np.zeros_like(plain)
+    surfaceNormal[surface] = -plain[surface] / length[surface, None]
+    normal = np.zeros_like(plain)
+    scatterAdd(normal, i, values[:, None] * surfaceNormal[j])
+    scatterAdd(normal, j, values[:, None] * surfaceNormal[i])
+    vicinity = np.zeros(len(fluid), dtype = bool)
+    vicinity[i[surface[j]]] = True
+    vicinity[j[surface[i]]] = True
+    weight = np.linalg.norm(normal, axis = 1)
+    normal = np.divide(normal, weight[:, None], out = np.zeros_like(normal), where = weight[:, None] > 0.0)
+    shift = -np.reshape(size, (-1, 1)) * gradient
+    shift[vicinity] -= np.sum(shift[vicinity] * normal[vicinity], axis = 1)[:, None] * normal[vicinity]
+    strong = vicinity & (h * np.linalg.norm(gradient, axis = 1) > surfaceGradient)
+    shift[surface | strong | ~fluid] = 0.0
+    return shift, np.sum(shift * densityGradient, axis = 1)
+
+
+def shiftedVelocityChange(masses, densities, velocities, i, j, gradients, shift, active):
+    '''sum_j V_j (v_j - v_i) (grad W_ij . delta r_i) over the active pairs (sphSolver.py:961-1010).'''
+    count = len(densities)
+    volumes = np.broadcast_to(np.asarray(masses, dtype = float), (count,)) / densities
+    i, j, gradients = i[active], j[active], gradients[active]
+    difference = velocities[j] - velocities[i]
+    change = np.zeros_like(velocities)
+    scatterAdd(change, i, (volumes[j] * np.sum(gradients * shift[i], axis = 1))[:, None] * difference)
+    scatterAdd(change, j, (volumes[i] * np.sum(gradients * shift[j], axis = 1))[:, None] * difference)
+    return change
+
+
 def harmonicMean(a, b):
     '''2 a b / (a + b), zero where both are zero.'''
     both = a + b

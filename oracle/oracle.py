@@ -252,6 +252,20 @@ def accelerations(physics: Physics, positions, velocities, densities, pressures,
     return out[:, :physics.dims].copy()
 
 
+def shiftingDisplacement(physics: Physics, positions, velocities, densities, masses, isFluid, size: float,
+                         local: bool, spacing: float):
+    '''Each particle's shift, density change and velocity change, on one h (sphGpu.py:856-1004).'''
+    from oracle.array_formulation import kernelValues
+    count, pos, vel, dens, mass, lens, fluid = _state(physics, positions, velocities, densities, masses, None, isFluid)
+    spacingKernel = float(kernelValues(physics.kernelType, np.array([spacing]), physics.h, physics.dims)[0])
+    shift, change, velocity = np.zeros((count, 3)), np.zeros(count), np.zeros((count, 3))
+    library().oracle_shifting(ctypes.byref(physics.params()), count, _ptr(pos), _ptr(vel), _ptr(dens), _ptr(mass),
+                              _ptr(fluid), ctypes.c_double(size), ctypes.c_int32(1 if local else 0),
+                              ctypes.c_double(spacingKernel), _ptr(shift), _ptr(change), _ptr(velocity))
+    dim = physics.dims
+    return shift[:, :dim].copy(), change, velocity[:, :dim].copy()
+
+
 def closureCoefficients(physics: Physics, gradients, isFluid, soundSpeed: float, lengths = None):
     '''Each particle's delta-LES-SPH alpha and delta from its G, (N, dim, dim) (sphGpu.py:699-723).'''
     count = len(gradients)

@@ -717,6 +717,84 @@ void oracle_accelerations(const OracleParams* p, int n, const double* pos, const
     for (int i = 0; i < n; ++i) store(outAcc, i, accelerationAt(*p, cells, s, g.data(), i, closure));
 }
 
+// Particle shifting, after Lind et al. (sphGpu.py:856-1004, sphSolver.py:824-1010), on one
+// smoothing length h = hmax. Pass 1, on the fluid over every neighbor but the coincident: grad C
+// with the anti-clumping weight 1 + 0.2 (W / W(dx))^4, grad C0 without it, div r = -sum V_j r W'(r)
+// (set to -1 where the support is lopsided, h |grad C0| above 0.3 for the cubic spline and 0.4 for
+// the Wendland C2) and grad rho; a boundary particle's div r is 1e6. Pass 2, on a fluid particle
+// whose div r is at least 1.5 (2D) or 2.4 (3D): the surface neighbors' normals -grad C0_j / |grad
+// C0_j| weighted by W; the shift -size grad C_i (size times |v_i| when `local`), zero with a surface
+// neighbor and h |grad C_i| above 0.03, else projected off the normal; the density change
+// shift . grad rho_i and the velocity change sum_j V_j (v_j - v_i) (grad W_ij . shift) over fluid
+// neighbors but the coincident.
+void oracle_shifting(const OracleParams* p, int n, const double* pos, const double* vel, const double* dens,
+                     const double* mass, const uint8_t* fluid, double size, int32_t local, double spacingKernel,
+                     double* outShift3, double* outDensity, double* outVelocity3) {
+    CellList cells;
+    cells.build(*p, n, pos);
+    const double h = p->hmax, sigma = sigmaAt(*p, h);
+    std::vector<double> len(n, h);
+    const double lopsided = p->kernel == 1 ? 0.4 : 0.3, threshold = p->dims == 2 ? 1.5 : 2.4;
+    std::vector<V3> gradient(n), plain(n), densityGradient(n);
+    std::vector<double> divergence(n);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < n; ++i) {
+        gradient[i] = plain[i] = densityGradient[i] = {0.0, 0.0, 0.0};
+        if (!fluid[i]) { divergence[i] = 1.0e6; continue; }
+        double total = 0.0;
+        forNeighbors(*p, cells, pos, len.data(), i, true, [&](const Pair& q) {
+            const double volume = mass[q.j] / dens[q.j];
+            const double slope = kernelSlope(p->kernel, q.r, h, sigma);
+            const double ratio = kernelValue(p->kernel, q.r, h, sigma) / spacingKernel;
+            const double square = ratio * ratio;
+            const V3 weighted = (volume * slope / q.r) * q.d;
+            plain[i] = plain[i] + weighted;
+            gradient[i] = gradient[i] + (1.0 + 0.2 * square * square) * weighted;
+            densityGradient[i] = densityGradient[i] + ((dens[q.j] - p->rho0) - (dens[i] - p->rho0)) * weighted;
+            total -= volume * slope * q.r;
+        });
+        divergence[i] = h * std::sqrt(dot(plain[i], plain[i])) > lopsided ? -1.0 : total;
+    }
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < n; ++i) {
+        const V3 zero = {0.0, 0.0, 0.0};
+        store(outShift3, i, zero);
+        store(outVelocity3, i, zero);
+        outDensity[i] = 0.0;
+        if (!fluid[i] || divergence[i] < threshold) continue;
+        bool vicinity = false;
+        V3 normal = zero;
+        forNeighbors(*p, cells, pos, len.data(), i, false, [&](const Pair& q) {
+            if (divergence[q.j] >= threshold) return;
+            vicinity = true;
+            const double length = std::sqrt(dot(plain[q.j], plain[q.j]));
+            if (length > 0.0) normal = normal - (kernelValue(p->kernel, q.r, h, sigma) / length) * plain[q.j];
+        });
+        const V3 vi = vec(vel, i);
+        const double scale = local ? size * std::sqrt(dot(vi, vi)) : size;
+        V3 shift = -scale * gradient[i];
+        if (vicinity) {
+            if (h * std::sqrt(dot(gradient[i], gradient[i])) > 0.03) shift = zero;
+            const double weight = std::sqrt(dot(normal, normal));
+            if (weight > 0.0) {
+                normal = (1.0 / weight) * normal;
+                shift = shift - dot(shift, normal) * normal;
+            }
+        }
+        store(outShift3, i, shift);
+        outDensity[i] = dot(shift, densityGradient[i]);
+        if (dot(shift, shift) > 0.0) {
+            V3 change = zero;
+            forNeighbors(*p, cells, pos, len.data(), i, true, [&](const Pair& q) {
+                if (!fluid[q.j]) return;
+                const double slope = kernelSlope(p->kernel, q.r, h, sigma);
+                change = change + (mass[q.j] / dens[q.j] * slope / q.r * dot(q.d, shift)) * (vec(vel, q.j) - vi);
+            });
+            store(outVelocity3, i, change);
+        }
+    }
+}
+
 // The closure's coefficients from each particle's G (9 per particle, row-major): alpha_i =
 // 2 (D + 2) (C_S l / h)^2 h ||D|| / c0 and delta_i = (C_delta l / h)^2 h ||D|| / c0, C_S = 0.12,
 // C_delta = 1.5, l / h = 2, ||D|| = sqrt(2 D:D) of G's symmetric part, each at most 0.2; zero on the

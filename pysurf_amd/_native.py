@@ -109,6 +109,8 @@ signatures = {
     'sph_shepard_density': [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
     'sph_density_gradients': [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
     'sph_set_closure': [ctypes.c_void_p, ctypes.c_int32, ctypes.c_double, ctypes.c_double, ctypes.c_double],
+    'sph_shifting_displacement': [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_int32,
+                                  ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
     'sph_closure_coefficients': [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
     'sph_xsph_velocity': [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p],
     'sph_neighbor_counts': [ctypes.c_void_p, ctypes.c_void_p],

@@ -864,6 +864,188 @@ __global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
     for (int a = 0; a < D; ++a) out[a * stride + i] = vi[a] + epsilon * acc[a];
 }
 
+// Particle shifting after Lind et al., host steps only, on one smoothing length h = hmax
+// (sphGpu.py:856-1004). Per sorted slot s, `fields` holds 3 D + 1 values from s (3 D + 1): grad C
+// (D), grad C0 (D), div r, grad rho (D).
+//   pass 0, on the fluid over every neighbor but the coincident: grad C with the anti-clumping
+//     weight 1 + 0.2 (W / W(dx))^4, grad C0 without it, div r = -sum V_j r W'(r), set to -1 where
+//     h |grad C0| exceeds 0.3 (cubic spline) or 0.4 (Wendland C2), and grad rho from the
+//     deviations; a boundary particle's div r is 1e6, so it never reads as the surface;
+//   pass 1, on a fluid particle with div r at least 1.5 (2D) or 2.4 (3D): the surface neighbors'
+//     -grad C0_j / |grad C0_j| weighted by W; the shift -size grad C_i (size |v_i| when `local`),
+//     zero with a surface neighbor and h |grad C_i| above 0.03, else projected off the normal; the
+//     density change shift . grad rho_i; the velocity change sum_j V_j (v_j - v_i) (grad W_ij .
+//     shift) over fluid neighbors but the coincident. Shift and velocity change at
+//     out[a * stride + i], the density change at density[i]; zero elsewhere.
+template <int D>
+__host__ __device__ constexpr int shiftWidth() { return 3 * D + 1; }
+
+template <int D, typename Real, int Lanes>
+__global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
+    sweepShifting(const RunDev* __restrict__ runs, Tiles tiles, const int32_t* __restrict__ order,
+                  const int32_t* __restrict__ start, Records<Real> rec, NeighborList list,
+                  const double* __restrict__ y64, int32_t pass, Real size, int32_t local, Real spacingKernel,
+                  Real* __restrict__ fields, Real* __restrict__ shiftOut, Real* __restrict__ densityOut,
+                  Real* __restrict__ velocityOut, int64_t stride) {
+    constexpr int W = shiftWidth<D>();
+    const int64_t tile = xcdBlock();
+    const int r = tiles.run[tile];
+    const int64_t s = tiles.first[tile] + threadIdx.x / Lanes;
+    const int lane = (int)(threadIdx.x % Lanes);
+    const bool valid = s < runs[r].first + runs[r].count;
+    const RunDev& run = runs[r];
+    const Real h = (Real)run.hmax;
+    const Real threshold = D == 2 ? Real(1.5) : Real(2.4);
+    PairKernel<Real, D> kernel;
+    kernel.init(run.kernel, run.sigmaUnit, run.hmax, true);
+    bool fluidI = false;
+    Self<D, Real> me{};
+    double yy = 0.0;
+    if (valid) {
+        me = loadSelf<D, Real>(run, rec, s);
+        fluidI = me.fluid;
+        yy = run.period > 0.0 ? y64[order[s]] : 0.0;
+    }
+    if (pass == 0) {
+        Real acc[W];
+#pragma unroll
+        for (int e = 0; e < W; ++e) acc[e] = Real(0);
+        if (valid && fluidI) {
+            const Real devI = deviationOf<D, Real>(rec.r[s]);
+            forEachNeighbor<D, Real, Lanes, true>(
+                run, start, GlobalRecords<Real>{rec.r}, list, s, me, yy, lane, true,
+                [&](int32_t, const Record<Real>& other, Real dx, Real dy, Real dz, Real rr, Real) {
+                    const Real volume = volumeOf<D, Real>(other);
+                    const Real F = kernel.slopeOverR(rr, h);
+                    const Real ratio = kernel.value(rr, h) / spacingKernel;
+                    const Real square = ratio * ratio;
+                    const Real anti = Real(1) + Real(0.2) * square * square;
+                    const Real change = deviationOf<D, Real>(other) - devI;
+                    const Real d[3] = {dx, dy, dz};
+#pragma unroll
+                    for (int a = 0; a < D; ++a) {
+                        const Real weighted = volume * F * d[a];
+                        acc[a] += anti * weighted;
+                        acc[D + a] += weighted;
+                        acc[2 * D + 1 + a] += change * weighted;
+                    }
+                    acc[2 * D] -= volume * F * rr * rr;
+                });
+        }
+#pragma unroll
+        for (int e = 0; e < W; ++e) acc[e] = groupSum<Lanes>(acc[e]);
+        if (!valid || lane != 0) return;
+        Real* own = fields + s * W;
+        if (!fluidI) {
+#pragma unroll
+            for (int e = 0; e < W; ++e) own[e] = Real(0);
+            own[2 * D] = Real(1.0e6);
+            return;
+        }
+        Real plain2 = Real(0);
+#pragma unroll
+        for (int a = 0; a < D; ++a) plain2 += acc[D + a] * acc[D + a];
+        const Real lopsided = run.kernel == 1 ? Real(0.4) : Real(0.3);
+        if (h * sqrt(plain2) > lopsided) acc[2 * D] = Real(-1);
+#pragma unroll
+        for (int e = 0; e < W; ++e) own[e] = acc[e];
+        return;
+    }
+    const Real* own = fields + (valid ? s : 0) * W;
+    const bool active = valid && fluidI && own[2 * D] >= threshold;
+    Real normal[3] = {Real(0), Real(0), Real(0)};
+    Real vicinity = Real(0);
+    if (active) {
+        forEachNeighbor<D, Real, Lanes, false>(
+            run, start, GlobalRecords<Real>{rec.r}, list, s, me, yy, lane, true,
+            [&](int32_t k, const Record<Real>&, Real, Real, Real, Real rr, Real) {
+                const Real* other = fields + (int64_t)k * W;
+                if (other[2 * D] >= threshold) return;
+                vicinity += Real(1);
+                Real length2 = Real(0);
+#pragma unroll
+                for (int a = 0; a < D; ++a) length2 += other[D + a] * other[D + a];
+                if (length2 > Real(0)) {
+                    const Real w = kernel.value(rr, h) / sqrt(length2);
+#pragma unroll
+                    for (int a = 0; a < D; ++a) normal[a] -= w * other[D + a];
+                }
+            });
+    }
+    vicinity = groupSum<Lanes>(vicinity);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) normal[a] = groupSum<Lanes>(normal[a]);
+    Real shift[3] = {Real(0), Real(0), Real(0)};
+    Real vi[3] = {Real(0), Real(0), Real(0)};
+    if (active) {
+        velocityOf<D, Real>(rec.r[s], vi);
+        Real scale = size;
+        if (local) {
+            Real v2 = Real(0);
+#pragma unroll
+            for (int a = 0; a < D; ++a) v2 += vi[a] * vi[a];
+            scale = size * sqrt(v2);
+        }
+        Real grad2 = Real(0);
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            shift[a] = -scale * own[a];
+            grad2 += own[a] * own[a];
+        }
+        if (vicinity > Real(0)) {
+            if (h * sqrt(grad2) > Real(0.03))
+#pragma unroll
+                for (int a = 0; a < D; ++a) shift[a] = Real(0);
+            Real weight2 = Real(0);
+#pragma unroll
+            for (int a = 0; a < D; ++a) weight2 += normal[a] * normal[a];
+            if (weight2 > Real(0)) {
+                const Real weight = sqrt(weight2);
+                Real along = Real(0);
+#pragma unroll
+                for (int a = 0; a < D; ++a) {
+                    normal[a] = normal[a] / weight;
+                    along += shift[a] * normal[a];
+                }
+#pragma unroll
+                for (int a = 0; a < D; ++a) shift[a] -= along * normal[a];
+            }
+        }
+    }
+    Real shift2 = Real(0);
+#pragma unroll
+    for (int a = 0; a < D; ++a) shift2 += shift[a] * shift[a];
+    Real change[3] = {Real(0), Real(0), Real(0)};
+    if (active && shift2 > Real(0)) {
+        forEachNeighbor<D, Real, Lanes, true>(
+            run, start, GlobalRecords<Real>{rec.r}, list, s, me, yy, lane, true,
+            [&](int32_t, const Record<Real>& other, Real dx, Real dy, Real dz, Real rr, Real) {
+                if (!(signedLength<D, Real>(other) > Real(0))) return;
+                const Real d[3] = {dx, dy, dz};
+                Real along = Real(0);
+#pragma unroll
+                for (int a = 0; a < D; ++a) along += d[a] * shift[a];
+                const Real w = volumeOf<D, Real>(other) * kernel.slopeOverR(rr, h) * along;
+                Real vj[3];
+                velocityOf<D, Real>(other, vj);
+#pragma unroll
+                for (int a = 0; a < D; ++a) change[a] += w * (vj[a] - vi[a]);
+            });
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) change[a] = groupSum<Lanes>(change[a]);
+    if (!valid || lane != 0) return;
+    const int32_t i = order[s];
+    Real density = Real(0);
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        density += shift[a] * own[2 * D + 1 + a];
+        shiftOut[a * stride + i] = shift[a];
+        velocityOut[a * stride + i] = change[a];
+    }
+    densityOut[i] = density;
+}
+
 // The delta-LES-SPH closure's coefficients of each sorted slot from its G in `rec.G` (sphGpu.py:
 // 699-723): alpha_i = 2 (D + 2) (C_S l / h)^2 h ||D|| / c0 and delta_i = (C_delta l / h)^2 h ||D|| / c0
 // with C_S = 0.12, C_delta = 1.5, l / h = 2 and ||D|| = sqrt(2 D:D) of G's symmetric part, each at
@@ -2142,6 +2324,9 @@ struct sph_context {
     void* eddy = nullptr;
     size_t eddyBytes = 0;
     int64_t eddyStride = 0;
+    // particle shifting's per-slot fields of the host step, 3 D + 1 values per sorted slot
+    void* shiftFields = nullptr;
+    size_t shiftBytes = 0;
     void* mergeCells = nullptr;
     size_t mergeCellBytes = 0;
     // The split's marks and ranks between sph_split_count and sph_split_apply
@@ -2270,6 +2455,9 @@ void releaseScratch(sph_context* ctx) {
     ctx->eddy = nullptr;
     ctx->eddyBytes = 0;
     ctx->eddyStride = 0;
+    if (ctx->shiftFields) (void)hipFree(ctx->shiftFields);
+    ctx->shiftFields = nullptr;
+    ctx->shiftBytes = 0;
     if (ctx->mergeCells) (void)hipFree(ctx->mergeCells);
     ctx->mergeCells = nullptr;
     ctx->mergeCellBytes = 0;
@@ -2749,6 +2937,46 @@ int hostSweep(sph_context* ctx, int which, const double* velocity, const double*
     return launchCheck(ctx, "host-step sum");
 }
 
+// Particle shifting as a host step: the records from the caller's velocities and densities, the
+// list, then sweepShifting's two passes.
+template <int D, typename Real>
+int shiftSweep(sph_context* ctx, const double* velocity, const double* density, double size, int32_t local,
+               double spacingKernel, void* shifts, void* densityChanges, void* velocityChanges) {
+    const int64_t n = ctx->total;
+    if (n == 0) return SPH_OK;
+    for (const RunDev& run : ctx->runsHost)
+        if (!run.uniform) return fail(ctx, SPH_ERR_VALUE, "Particle shifting needs one smoothing length for every particle.");
+    const size_t bytes = (size_t)ctx->state.stride * shiftWidth<D>() * sizeof(Real);
+    if (bytes > ctx->shiftBytes) {
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        if (ctx->shiftFields) (void)hipFree(ctx->shiftFields);
+        ctx->shiftFields = nullptr;
+        ctx->shiftBytes = 0;
+        HIP_TRY(hipMalloc(&ctx->shiftFields, bytes));
+        ctx->shiftBytes = bytes;
+    }
+    const int b = ctx->block;
+    const int32_t nruns = (int32_t)ctx->runs.size();
+    Records<Real>& rec = recordsOf<Real>(ctx);
+    clockBegin<<<gridFor(nruns, 64), 64, 0, ctx->stream>>>(ctx->clock, nruns, 0);
+    Sources<Real> src = sourcesOf<Real>(ctx, velocity, density, nullptr, nullptr);
+    gatherRecords<D, Real><<<gridFor(n, b), b, 0, ctx->stream>>>(ctx->dRuns, ctx->runOf, ctx->order, src, rec, ctx->clock, 0, n);
+    const double* y64 = ctx->state.position + ctx->state.stride;
+    const NeighborList list = ctx->list;
+    if (list.capacity > 0)
+        WITH_LANES(ctx, (buildNeighbors<D, Real, L, false><<<tileGrid(ctx), b, sweepLds<D, Real>(ctx, 0), ctx->stream>>>(
+                            ctx->dRuns, tilesOf(ctx), ctx->order, ctx->cellStart, rec, list, emptyOut<Real>(ctx),
+                                ctx->clock, y64, 0, nearRecords<D, Real>(ctx), n)));
+    Real* fields = static_cast<Real*>(ctx->shiftFields);
+    for (int32_t pass = 0; pass < 2; ++pass)
+        WITH_LANES(ctx, (sweepShifting<D, Real, L><<<tileGrid(ctx), b, 0, ctx->stream>>>(
+                            ctx->dRuns, tilesOf(ctx), ctx->order, ctx->cellStart, rec, list, y64, pass, (Real)size,
+                                local, (Real)spacingKernel, fields, static_cast<Real*>(shifts),
+                                static_cast<Real*>(densityChanges), static_cast<Real*>(velocityChanges),
+                                ctx->state.stride)));
+    return launchCheck(ctx, "particle shifting");
+}
+
 int dispatchHostSweep(sph_context* ctx, int which, const double* velocity, const double* density, const void* pressure,
                       void* out0, double* out1, double scalar = 0.0) {
     int status = requireBound(ctx);
@@ -3050,6 +3278,24 @@ int sph_pair_accelerations(sph_context* ctx, const double* velocity, const doubl
 int sph_continuity_rate(sph_context* ctx, const double* velocity, const double* density, void* out) {
     if (!velocity || !out) return fail(ctx, SPH_ERR_VALUE, "The continuity rate needs velocities and an output.");
     return dispatchHostSweep(ctx, 4, velocity, density, nullptr, out, nullptr);
+}
+
+int sph_shifting_displacement(sph_context* ctx, const double* velocity, const double* density, double size,
+                              int32_t local, double spacingKernel, void* shifts, void* densityChanges,
+                              void* velocityChanges) {
+    int status = requireBound(ctx);
+    if (status != SPH_OK) return status;
+    if (!velocity || !density || !shifts || !densityChanges || !velocityChanges)
+        return fail(ctx, SPH_ERR_VALUE, "Shifting needs velocities, densities and three outputs.");
+    if (!(size >= 0.0) || !(spacingKernel > 0.0))
+        return fail(ctx, SPH_ERR_VALUE, "The shift's size must be at least zero and W(dx) positive.");
+    if (ctx->dims == 2)
+        return ctx->precision == 0
+                   ? shiftSweep<2, float>(ctx, velocity, density, size, local, spacingKernel, shifts, densityChanges, velocityChanges)
+                   : shiftSweep<2, double>(ctx, velocity, density, size, local, spacingKernel, shifts, densityChanges, velocityChanges);
+    return ctx->precision == 0
+               ? shiftSweep<3, float>(ctx, velocity, density, size, local, spacingKernel, shifts, densityChanges, velocityChanges)
+               : shiftSweep<3, double>(ctx, velocity, density, size, local, spacingKernel, shifts, densityChanges, velocityChanges);
 }
 
 int sph_set_closure(sph_context* ctx, int32_t enabled, double soundSpeed, double referenceDensity,

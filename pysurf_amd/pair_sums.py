@@ -505,6 +505,7 @@ class PairSums:
         self.correctContinuity = False
         self.limitViscosity = False
         self.closure = {'enabled': False, 'soundSpeed': 1.0, 'referenceDensity': 1000.0, 'kinematicViscosity': 0.0}
+        self.shifting = {'enabled': False, 'mach': 0.0, 'spacingKernel': 0.0, 'soundSpeed': 1.0, 'local': False}
         self.trackEnergy = False
         self.engine = Engine(dimensions, pairPrecision, device)
         self.device = self.engine.device
@@ -578,9 +579,60 @@ class PairSums:
 
     def setShifting(self, enabled: bool, mach: float = 0.1, spacing: float = 0.0, soundSpeed: float = 1.0,
                     local: bool = False) -> None:
-        '''Particle shifting (sphGpu.py:1922-1955), off under the delta-SPH scheme and not built.'''
+
+        '''
+
+        Particle shifting (sphGpu.py:1924-1955) for `shiftingDisplacement`: Ma in the shift's size
+        CFL Ma (2h)^2, the spacing dx for the anti-clumping weight's W(dx), c0 for the step's Courant
+        number, or with `local` each particle's own size A h |v_i| dt. The device steps refuse it;
+        the delta-SPH scheme runs without it.
+
+        '''
+
+        if enabled and not self.uniformLengths:
+            raise ValueError('Particle shifting needs one smoothing length for every particle.')
+        if enabled and not spacing > 0.0:
+            raise ValueError('Particle shifting needs the particle spacing.')
+        spacingKernel = 0.0
         if enabled:
-            raise ValueError('Particle shifting is not built; the delta-SPH scheme runs without it.')
+            q = spacing / self.smoothingLength
+            if self.kernelType == 'wendlandC2':
+                spacingKernel = self.sigma * max(1.0 - 0.5 * q, 0.0)**4 * (2.0 * q + 1.0)
+            else:
+                spacingKernel = self.sigma * ((1.0 - 1.5 * q**2 + 0.75 * q**3) if q < 1.0
+                                              else 0.25 * max(2.0 - q, 0.0)**3)
+        self.shifting = {'enabled': bool(enabled), 'mach': float(mach), 'spacingKernel': spacingKernel,
+                         'soundSpeed': float(soundSpeed), 'local': bool(local)}
+
+    def shiftingDisplacement(self, densities: np.ndarray, timeStep: float,
+                             velocities: np.ndarray) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+
+        '''
+
+        This step's shift of every particle, zero on the boundary and the free surface [m], and the
+        density [kg/m^3] and velocity [m/s] changes that go with it (sphGpu.py:1966-1979).
+
+        '''
+
+        shifting, h = self.shifting, self.smoothingLength
+        if not shifting['enabled']:
+            raise ValueError('Particle shifting is off; see setShifting.')
+        if shifting['local']:
+            size = 2.0 * h * timeStep
+        else:
+            size = (shifting['soundSpeed'] * timeStep / h) * shifting['mach'] * (2.0 * h)**2
+        self._sync()
+        engine = self.engine
+        engine.upload('velocityIn', velocities)
+        engine.upload('densityIn', densities)
+        # shifts into 'vector', density changes into 'scalar', velocity changes into the first rows of 'matrix'
+        self.check(engine.library.sph_shifting_displacement(
+            engine.handle, engine.pointer('velocityIn'), engine.pointer('densityIn'), float(size),
+            1 if shifting['local'] else 0, float(shifting['spacingKernel']), engine.pointer('vector'),
+            engine.pointer('scalar'), engine.pointer('matrix')), 'sph_shifting_displacement')
+        dim = self.dimensions
+        velocity = engine.download('matrix')[:, :dim]
+        return engine.download('vector'), engine.download('scalar'), velocity
 
     def setClosure(self, enabled: bool, soundSpeed: float = 1.0, referenceDensity: float = 1000.0,
                    kinematicViscosity: float = 1e-6) -> None:
@@ -890,14 +942,21 @@ class PairSums:
         if filterDensity:
             raise ValueError('The device step does not filter the density; the delta-SPH scheme runs without it. '
                              'shepardDensity gives the filtered density as a host step.')
+        self._refuseShifting()
         self._sync()
         maxima = np.zeros(2)
         self.check(self.engine.library.sph_rates(self.engine.handle, maxima.ctypes.data_as(ctypes.c_void_p)),
                    'sph_rates')
         return float(maxima[0]), float(maxima[1])
 
+    def _refuseShifting(self) -> None:
+        if self.shifting['enabled']:
+            raise ValueError('The device step does not shift the particles; shiftingDisplacement gives the shift '
+                             'as a host step.')
+
     def residentIntegrate(self, timeStep: float) -> None:
         '''The second half of a device step with the host's time step (sphGpu.py:2275-2315).'''
+        self._refuseShifting()
         self.check(self.engine.library.sph_integrate(self.engine.handle, float(timeStep)), 'sph_integrate')
 
     def enforceBoundary(self) -> None:
@@ -917,6 +976,7 @@ class PairSums:
 
         if self.count == 0:
             return {'time': float(time), 'steps': int(steps), 'dt': 0.0}
+        self._refuseShifting()
         self._sync()
         engine = self.engine
         engine.setClock(0, time, untilTime, steps, maxSteps)

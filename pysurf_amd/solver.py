@@ -173,6 +173,9 @@ class WcsphSolver:
         if config.xsphEpsilon > 0.0:
             raise ValueError('The device step does not move the fluid with XSPH (PairSums.xsphVelocity gives it as a '
                              'host step); the delta-SPH scheme sets xsphEpsilon to 0.')
+        if getattr(config, 'particleShifting', False):
+            raise ValueError('The device step does not shift the particles (PairSums.shiftingDisplacement gives the '
+                             'shift as a host step); the delta-SPH scheme sets particleShifting to False.')
         if config.turbulenceClosure == 'deltaLes':
             raise ValueError("The device step does not carry the delta-LES-SPH closure (PairSums.setClosure gives it "
                              "to the host-step sums); the delta-SPH scheme sets turbulenceClosure to 'none'.")

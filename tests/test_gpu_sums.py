@@ -12,7 +12,7 @@ predicate.
 import numpy as np
 import pytest
 
-from clouds import alpha, c0, cloud, physicsFor, rho0, strainedFlow
+from clouds import alpha, c0, cloud, expectedShifts, physicsFor, rho0, shiftingBlock, strainedFlow
 from oracle import oracle
 
 pytestmark = pytest.mark.gpu
@@ -168,6 +168,38 @@ def testClosureMatchesTheOracle(precision, dimensions, periodic, mode):
     # The device step refuses the closure before it launches anything
     sums.setPhysics(rho0, c0, alpha, 0.3, gravityOf(dimensions), 0.0, 0.0)
     with pytest.raises(ValueError, match = 'closure'):
+        sums.residentRates()
+
+
+@pytest.mark.parametrize('precision', precisions)
+@pytest.mark.parametrize('dimensions', [2, 3])
+@pytest.mark.parametrize('kernelType', kernelTypes)
+@pytest.mark.parametrize('local', [False, True])
+def testShiftsMatchTheOracle(precision, dimensions, kernelType, local):
+    # tests/testSphGpu.py:318-359: the shifts to 1e-4 of the largest, the density changes to 1e-3 and
+    # the velocity changes to 1e-4 in float32 (the reference's own bounds); 1e-10 in float64
+    print(f'shifting {precision} {dimensions}D {kernelType} local={local}', flush = True)
+    block = shiftingBlock(dimensions, kernelType)
+    fluid = block['isFluid']
+    size = 2.0 * block['h'] * block['timeStep'] if local else block['size']
+    physics = physicsFor(block)
+    shift, change, velocity = oracle.shiftingDisplacement(physics, block['positions'], block['velocities'],
+                                                          block['densities'], block['masses'], fluid, size, local,
+                                                          block['spacing'])
+    surface = expectedShifts(block)[3]
+    assert (fluid & ~surface & (np.abs(shift).sum(axis = 1) > 0)).sum() > 50
+    sums = pairSums(block, precision)
+    sums.setDiffusion('none', 0.1, c0, rho0, 9.81)
+    sums.setShifting(True, block['mach'], block['spacing'], c0, local)
+    result, densityChange, velocityChange = sums.shiftingDisplacement(block['densities'], block['timeStep'],
+                                                                      block['velocities'])
+    loose = precision == 'float32'
+    assert result == pytest.approx(shift, abs = (1e-4 if loose else 1e-10) * np.abs(shift).max())
+    assert densityChange == pytest.approx(change, abs = (1e-3 if loose else 1e-10) * np.abs(change).max())
+    assert velocityChange == pytest.approx(velocity, abs = (1e-4 if loose else 1e-10) * np.abs(velocity).max())
+    assert np.all(result[surface | ~fluid] == 0.0)
+    sums.setPhysics(rho0, c0, alpha, 0.0, gravityOf(dimensions), 0.0, 0.0)
+    with pytest.raises(ValueError, match = 'shift'):
         sums.residentRates()
 
 

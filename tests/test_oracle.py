@@ -13,7 +13,7 @@ import math
 import numpy as np
 import pytest
 
-from clouds import alpha, c0, cloud, physicsFor, rho0, strainedFlow
+from clouds import alpha, c0, cloud, expectedShifts, physicsFor, rho0, shiftingBlock, strainedFlow
 from oracle import array_formulation as af
 from oracle import oracle
 
@@ -325,6 +325,32 @@ def testClosureMatchesTheArrayFormulation(dimensions, periodic):
     result = oracle.continuityRate(physicsFor(case, diffusion = 'hydrostatic', delta = 0.1), case['positions'],
                                    velocities, densities, case['masses'], fluid, closure = closure)
     assert result == pytest.approx(rate, abs = 1e-12 * np.abs(rate).max())
+
+
+@pytest.mark.parametrize('dimensions', [2, 3])
+@pytest.mark.parametrize('kernelType', kernelTypes)
+def testShiftsMatchTheArrayFormulation(dimensions, kernelType):
+    # tests/testSphGpu.py:318-359: interior, surface and next-to-surface particles
+    block = shiftingBlock(dimensions, kernelType)
+    fluid = block['isFluid']
+    shift, change, velocity, surface = expectedShifts(block)
+    assert surface.sum() > 10 and (fluid & ~surface & (np.abs(shift).sum(axis = 1) > 0)).sum() > 50
+    physics = physicsFor(block)
+    result = oracle.shiftingDisplacement(physics, block['positions'], block['velocities'], block['densities'],
+                                         block['masses'], fluid, block['size'], False, block['spacing'])
+    assert result[0] == pytest.approx(shift, abs = 1e-12 * np.abs(shift).max())
+    assert result[1] == pytest.approx(change, abs = 1e-12 * np.abs(change).max())
+    assert result[2] == pytest.approx(velocity, abs = 1e-12 * np.abs(velocity).max())
+    assert np.all(result[0][surface | ~fluid] == 0.0)
+    # A uniform velocity has nothing to carry; a local size scales each shift by |v_i|
+    uniform = np.tile(np.linspace(0.3, -0.2, dimensions), (len(fluid), 1))
+    still = oracle.shiftingDisplacement(physics, block['positions'], uniform, block['densities'], block['masses'], fluid,
+                                        block['size'], False, block['spacing'])
+    assert np.abs(still[2]).max() < 1e-12 * np.abs(velocity).max()
+    local = oracle.shiftingDisplacement(physics, block['positions'], uniform, block['densities'], block['masses'], fluid,
+                                        block['size'], True, block['spacing'])
+    scaled = still[0] * np.linalg.norm(uniform[0])
+    assert local[0] == pytest.approx(scaled, abs = 1e-12 * np.abs(scaled).max())
 
 
 # -- Physics the formulas must satisfy -- #
