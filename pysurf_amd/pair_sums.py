@@ -6,7 +6,7 @@ tensors and every sum, step and enforcement run by hand-written HIP kernels.
 
     host steps      `update`, then `pairAccelerations`, `continuityRate`, `wallPressure`,
                     `velocityGradients`, `correctionMatrices`, `densityGradients`, `xsphVelocity`,
-                    `shepardDensity`, `closureCoefficients`:
+                    `shepardDensity`, `closureCoefficients`, `shiftingDisplacement`:
                     host float64 arrays in, host float64 arrays out, shaped as the reference
                     returns them
     device steps    `setPhysics`, `push`, `setBoundary`, then `residentRates` and
@@ -17,13 +17,13 @@ Pair arithmetic is float32 over float64 state by default, each float64 position 
 float32 point and remainder as the reference does (sphGpu.py:28-38); `pairPrecision = 'float64'`
 runs the plain float64 kernels of the same source.
 
-Of the options the delta-SPH scheme leaves off, XSPH, the Shepard filter, the renormalized
-'antuono' diffusion (`densityGradients`, and `continuityRate` under setDiffusion('antuono')) and
-the delta-LES-SPH closure (`setClosure`, `closureCoefficients`, and the closure's terms in
-`pairAccelerations` and `continuityRate`) are built as host-step sums; the device step runs without
-them, and setPhysics, residentRates and the device steps raise ValueError when asked for one.
-Particle shifting and the energy budget's ledger are not built: their setters accept them switched
-off and raise ValueError when asked to switch one on.
+The options the delta-SPH scheme leaves off are built as host-step sums: XSPH, the Shepard filter,
+the renormalized 'antuono' diffusion (`densityGradients`, and `continuityRate` under
+setDiffusion('antuono')), the delta-LES-SPH closure (`setClosure`, `closureCoefficients`, and the
+closure's terms in `pairAccelerations` and `continuityRate`) and particle shifting (`setShifting`,
+`shiftingDisplacement`). The device step runs without them, and setPhysics, residentRates and the
+device steps raise ValueError when asked for one. The energy budget's per-particle ledger is not
+built: its setter accepts it switched off and raises ValueError when asked to switch it on.
 
 '''
 
