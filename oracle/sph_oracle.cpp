@@ -16,6 +16,10 @@
 //   integrate  kick (sphGpu.py:1084-1109), correction matrices (sphGpu.py:593-641), continuity with
 //              delta-SPH diffusion (sphGpu.py:430-545), density advance with floor (:1111-1140),
 //              drift (:1142-1186), boundary enforcement (:1197-1229, :1246-1310, :1332-1419)
+//   options    the host-step sums the delta-SPH scheme leaves off: the Shepard density
+//              (sphGpu.py:768-802), the XSPH velocity (:725-766), the renormalized density gradient
+//              and its diffusion (:548-591, :521-524), the delta-LES-SPH closure's coefficients,
+//              viscosity and diffusion (:699-723, :388-393, :526-531), particle shifting (:856-1004)
 //
 // Parity status of each function is pinned in tests/test_oracle_*.py (invariants, closed forms,
 // brute force, and the numpy pair-list restatement in oracle/array_formulation.py).
