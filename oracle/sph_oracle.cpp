@@ -21,7 +21,7 @@
 //              and its diffusion (:548-591, :521-524), the delta-LES-SPH closure's coefficients,
 //              viscosity and diffusion (:699-723, :388-393, :526-531), particle shifting (:856-1004)
 //
-// Parity status of each function is pinned in tests/test_oracle_*.py (invariants, closed forms,
+// Parity status of each function is pinned in tests/test_oracle.py (invariants, closed forms,
 // brute force, and the numpy pair-list restatement in oracle/array_formulation.py).
 
 #include <algorithm>
