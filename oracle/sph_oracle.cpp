@@ -46,7 +46,7 @@ typedef struct {
     double period;           // periodic y length [m], 0 for none
     double rho0, c0, alpha, boundaryFactor;
     double gravity[3];
-    int32_t diffusion;       // 0 none, 1 molteni, 2 hydrostatic
+    int32_t diffusion;       // 0 none, 1 molteni, 2 hydrostatic, 3 antuono (host-step sums)
     double delta;
     int32_t extrapolatedWalls, negativePressure, correction, limiter;
     double densityFloor;     // smallest fluid density [kg/m^3], 0 for none
