@@ -299,6 +299,21 @@ int sph_merge(sph_context* ctx, const sph_merge_spec* specs, const uint8_t* mask
 
 int sph_synchronize(sph_context* ctx);
 
+/* sph_global_sums without a context, for callers that cannot allocate device memory: code under
+ * HIP graph capture, serving loops and harnesses that forbid hipMalloc. No counterpart in the
+ * reference (PairSums allocates through Warp). The same 7 float64 sums per run, in the same order
+ * and bit for bit the same, over `state` (device pointers, slot order) for the `count` runs of
+ * `runs` (only first, count, rho0, c0, gravity and negativePressure are read), on `stream`
+ * (a hipStream_t; null for the default stream); `sums` is a host array of 7 * count doubles.
+ * It allocates nothing: its device buffers live in one file-scope array of about 1.1 MB per GPU,
+ * held until the process ends. At most 1024 runs and 16384 chunks of 8192 slots (each run's slots
+ * rounded up to whole chunks); larger calls return SPH_ERR_VALUE. Calls on one GPU share that
+ * array, so calls that overlap on one GPU race. Synchronizes the stream. Errors are read with
+ * sph_static_error, thread by thread. */
+int sph_state_sums(int32_t dims, int32_t precision, const sph_run* runs, int32_t count, const sph_state* state,
+                   double* sums, void* stream);
+const char* sph_static_error(void);
+
 /* Per-kernel timing with HIP events recorded on the context's stream around every launch of a
  * step (no counterpart in the reference, which times whole runs only). sph_profile(1) clears the
  * totals and starts recording (steps then run as stream launches, not graphs); sph_kernel_times

@@ -134,6 +134,9 @@ signatures = {
     'sph_set_launch': [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32],
     'sph_profile': [ctypes.c_void_p, ctypes.c_int32],
     'sph_kernel_times': [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32],
+    'sph_state_sums': [ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(Run), ctypes.c_int32, ctypes.POINTER(State),
+                       ctypes.c_void_p, ctypes.c_void_p],
+    'sph_static_error': [],
 }
 
 # The columns of sph_global_sums, per run
@@ -159,7 +162,7 @@ def load(path: str | None = None):
     for name, arguments in signatures.items():
         function = getattr(library, name)
         function.argtypes = arguments
-        function.restype = ctypes.c_char_p if name == 'sph_last_error' else ctypes.c_int
+        function.restype = ctypes.c_char_p if name in ('sph_last_error', 'sph_static_error') else ctypes.c_int
     if path == libraryPath:
         _library = library
     return library

@@ -34,6 +34,7 @@
 #include <vector>
 
 #include "sph_device.hpp"
+#include "sph_sums.hpp"
 
 using namespace sph;
 
@@ -172,8 +173,6 @@ __device__ __forceinline__ double runTimeStep(const RunDev& run, const Clock& cl
     return fmin(dt, run.maxTimeStep);
 }
 
-constexpr int kSums = 7;              // fluid mass, kinetic, potential, elastic energy, momentum x, y, z
-constexpr int64_t kSumChunk = 8192;   // slots per partial sum and per partial maximum
 
 // Each run's maxima from the per-slot magnitudes the step wrote (|v| at the gather, |a| in the
 // acceleration sweep; zero on boundary particles): a maximum over each chunk of a run's slots,
@@ -1870,55 +1869,18 @@ __global__ void __launch_bounds__(256)
 // ---------------------------------------------------------------------------------------------
 
 
-// The elastic energy per unit mass the Tait equation stores at density rho (sphEnergy.py:120-160,
-// sphGpu.py:271-282): e = (c0^2 / 7) x^2 (21 + 35x + 35x^2 + 21x^3 + 7x^4 + x^5) / (6 (1 + x)) with
-// x = rho / rho0 - 1, zero below rho0 unless the fluid carries tension.
-__device__ __forceinline__ double elasticEnergy(double rho, double rho0, double scale, int tension) {
-    const double x = rho / rho0 - 1.0;
-    const double polynomial = 21.0 + x * (35.0 + x * (35.0 + x * (21.0 + x * (7.0 + x))));
-    const double energy = scale * x * x * polynomial / (6.0 * (1.0 + x));
-    return (!tension && x < 0.0) ? 0.0 : energy;
-}
-
-// One partial of every sum over chunk b's slots, fluid only, summed in a fixed order: each thread
-// over a fixed stride, then a fixed tree. K = m |v|^2 / 2 and P = m g z with g the downward
-// gravity along the last axis (sphParticles.py:236-255); masses as the device stores them.
+// One partial of every sum over chunk b's slots (chunkSums, sph_sums.hpp).
 template <int D, typename Real>
-__global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
+__global__ void __launch_bounds__(kSumThreads)
     sumChunks(const RunDev* __restrict__ runs, const int32_t* __restrict__ chunkRun,
               const int64_t* __restrict__ chunkFirst, sph_state state, double* __restrict__ partial) {
-    __shared__ double tree[kSums][256];
+    __shared__ double tree[kSums][kSumThreads];
     const int64_t b = blockIdx.x;
     const RunDev& run = runs[chunkRun[b]];
     const int64_t first = chunkFirst[b];
     const int64_t end = min(first + kSumChunk, run.first + run.count);
-    const double scale = run.c0 * run.c0 / 7.0;
-    const double g = -run.gravity[D - 1];
-    const Real* mass = static_cast<const Real*>(state.mass);
-    double acc[kSums] = {};
-    for (int64_t i = first + threadIdx.x; i < end; i += blockDim.x) {
-        if (!state.fluid[i]) continue;
-        const double m = (double)mass[i];
-        double v2 = 0.0;
-        for (int a = 0; a < D; ++a) {
-            const double v = state.velocity[a * state.stride + i];
-            v2 += v * v;
-            acc[4 + a] += m * v;
-        }
-        acc[0] += m;
-        acc[1] += 0.5 * m * v2;
-        acc[2] += m * g * state.position[(D - 1) * state.stride + i];
-        acc[3] += m * elasticEnergy(state.density[i], run.rho0, scale, run.negativePressure);
-    }
-    for (int e = 0; e < kSums; ++e) tree[e][threadIdx.x] = acc[e];
-    __syncthreads();
-    for (int width = blockDim.x / 2; width > 0; width >>= 1) {
-        if ((int)threadIdx.x < width)
-            for (int e = 0; e < kSums; ++e) tree[e][threadIdx.x] += tree[e][threadIdx.x + width];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0)
-        for (int e = 0; e < kSums; ++e) partial[b * kSums + e] = tree[e][0];
+    const SumPhysics physics{run.rho0, run.c0, -run.gravity[D - 1], run.negativePressure};
+    chunkSums<D, Real>(physics, first, end, state, tree, partial + b * kSums);
 }
 
 // Each run's sums from its chunks' partials, in chunk order.
@@ -1926,10 +1888,7 @@ __global__ void sumRuns(const int64_t* __restrict__ runChunk, const double* __re
                         int32_t runs) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= runs) return;
-    double acc[kSums] = {};
-    for (int64_t b = runChunk[r]; b < runChunk[r + 1]; ++b)
-        for (int e = 0; e < kSums; ++e) acc[e] += partial[b * kSums + e];
-    for (int e = 0; e < kSums; ++e) out[(int64_t)r * kSums + e] = acc[e];
+    addChunks(partial, runChunk[r], runChunk[r + 1], out + (int64_t)r * kSums);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -3808,17 +3767,17 @@ int sph_global_sums(sph_context* ctx, double* sums) {
         const unsigned chunks = (unsigned)ctx->sumChunkCount;
         if (ctx->dims == 2) {
             if (ctx->precision == 0)
-                sumChunks<2, float><<<chunks, 256, 0, ctx->stream>>>(ctx->dRuns, ctx->sumChunkRun, ctx->sumChunkFirst,
+                sumChunks<2, float><<<chunks, kSumThreads, 0, ctx->stream>>>(ctx->dRuns, ctx->sumChunkRun, ctx->sumChunkFirst,
                                                                        ctx->state, ctx->sumPartial);
             else
-                sumChunks<2, double><<<chunks, 256, 0, ctx->stream>>>(ctx->dRuns, ctx->sumChunkRun, ctx->sumChunkFirst,
+                sumChunks<2, double><<<chunks, kSumThreads, 0, ctx->stream>>>(ctx->dRuns, ctx->sumChunkRun, ctx->sumChunkFirst,
                                                                         ctx->state, ctx->sumPartial);
         } else {
             if (ctx->precision == 0)
-                sumChunks<3, float><<<chunks, 256, 0, ctx->stream>>>(ctx->dRuns, ctx->sumChunkRun, ctx->sumChunkFirst,
+                sumChunks<3, float><<<chunks, kSumThreads, 0, ctx->stream>>>(ctx->dRuns, ctx->sumChunkRun, ctx->sumChunkFirst,
                                                                        ctx->state, ctx->sumPartial);
             else
-                sumChunks<3, double><<<chunks, 256, 0, ctx->stream>>>(ctx->dRuns, ctx->sumChunkRun, ctx->sumChunkFirst,
+                sumChunks<3, double><<<chunks, kSumThreads, 0, ctx->stream>>>(ctx->dRuns, ctx->sumChunkRun, ctx->sumChunkFirst,
                                                                         ctx->state, ctx->sumPartial);
         }
     }
