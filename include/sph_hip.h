@@ -104,7 +104,9 @@ int sph_bind(sph_context* ctx, const sph_state* state);
 int sph_build_cells(sph_context* ctx);
 
 /* Host-step sums over the cells of the last sph_build_cells, each reading caller arrays in slot
- * order and writing one result per slot. */
+ * order and writing one result per slot. Every run of the context must share its viscosity limiter,
+ * continuity correction and density diffusion (the reference's PairSums holds one run); otherwise
+ * they return SPH_ERR_VALUE. */
 int sph_wall_pressure(sph_context* ctx, const double* density, const void* pressure, void* pressureOut,
                       double* densityOut);                                        /* sphGpu.py:2090 */
 int sph_velocity_gradients(sph_context* ctx, const double* velocity, const double* density,

@@ -131,6 +131,8 @@ class BatchEngine:
         engine.upload('length', np.concatenate(lengths))
         engine.upload('fluid', np.concatenate([p.isFluid for p in parts]).astype(np.uint8))
         engine.synchronize()
+        # runs whose masses, lengths and kinds the device changed (a split or a merge)
+        self.refined = [False] * len(self.cases)
         self.until = [0.0] * len(self.cases)
         self.maxSteps = [0] * len(self.cases)
         for r in range(len(self.cases)):
@@ -213,6 +215,7 @@ class BatchEngine:
             specs.append(spec)
         clock = self.clocks()
         chosen = self.engine.split(specs, self.cases[0].referenceDensity, lengthRatio, mask)
+        self.refined = [refined or bool(count) for refined, count in zip(self.refined, chosen)]
         if chosen.any():
             grown = chosen * (2**self.dimensions - 1)
             counts = np.diff(self.offsets) + grown
@@ -252,6 +255,7 @@ class BatchEngine:
             specs.append(spec)
         clock = self.clocks()
         groups, energy = self.engine.merge(specs, mask)
+        self.refined = [refined or bool(count) for refined, count in zip(self.refined, groups)]
         if groups.any():
             shrunk = groups * (2**self.dimensions - 1)
             counts = np.diff(self.offsets) - shrunk
@@ -275,8 +279,9 @@ class BatchEngine:
         particles.densities = self._original(engine.download('density', None, b), a, b)
         particles.pressures = self._original(engine.download('pressure', None, b), a, b)
         particles.accelerations = self._original(engine.download('acceleration', d, b), a, b)
-        if b - a != self.cases[index].count:
-            # The run has split on the device: its masses, lengths and kinds come from there too
+        if self.refined[index] or b - a != self.cases[index].count:
+            # The run has split or merged on the device (a split then a merge may restore its count):
+            # its masses, lengths and kinds come from there too
             particles.masses = self._original(engine.download('mass', None, b), a, b)
             particles.lengths = self._original(engine.download('length', None, b), a, b)
             particles.isFluid = self._original(engine.download('fluid', None, b), a, b).astype(bool)

@@ -16,19 +16,24 @@ The file, little-endian throughout:
     arrays     each one raw, in C order, starting at a multiple of 16 bytes
 
 The arrays are the engine's state in slot order as the device holds it (after a device sort, too),
-with the slot-to-particle identity when the batch has been sorted.
+with the slot-to-particle identity when the batch has been sorted. The header also keeps a digest
+of each run's physics and boundary, and a restore into runs of other physics (sound speed,
+smoothing length, diffusion, gravity, ...) or over another bed or height field is refused, as the
+reference refuses a checkpoint whose settings or bed hash differ (BreakingWave3D.py:578-597,
+842-849).
 
 '''
 
 from __future__ import annotations
 
+import hashlib
 import json
 import os
 
 import numpy as np
 
 MAGIC = b'SPHCKPT\x00'
-FORMAT_VERSION = 1
+FORMAT_VERSION = 2
 ALIGN = 16
 DTYPES = {'float64': np.float64, 'float32': np.float32, 'int32': np.int32, 'int64': np.int64, 'uint8': np.uint8}
 
@@ -81,6 +86,30 @@ def readCheckpoint(path: str) -> tuple[dict, dict]:
 STATE = ('position', 'velocity', 'density', 'pressure', 'acceleration', 'mass', 'length', 'fluid')
 
 
+def runDigest(case) -> str:
+
+    '''
+
+    SHA-256 of what fixes a run's step apart from its particles: the run's sph_run as the engine is
+    given it (kernel, grid, lengths, physics, options, boundary), with its first slot and the
+    boundary tables' addresses blanked, followed by the values of those tables (the bed polyline,
+    or the height field).
+
+    '''
+
+    import torch
+
+    from pysurf_amd.batch import runOf
+    from pysurf_amd.pair_sums import boundaryOf
+    boundary, tables = boundaryOf(case.boundary, case.dimensions, torch.device('cpu'))
+    run = runOf(case, 0, boundary)
+    run.boundary.bedX = run.boundary.bedZ = run.boundary.table = None
+    digest = hashlib.sha256(bytes(run))
+    for table in tables:
+        digest.update(table.numpy().tobytes())
+    return digest.hexdigest()
+
+
 def save(batch, path: str, extra: dict | None = None) -> None:
 
     '''
@@ -104,6 +133,7 @@ def save(batch, path: str, extra: dict | None = None) -> None:
     header = {
         'dimensions': batch.dimensions, 'precision': batch.precision, 'count': count,
         'runs': [{'first': batch.offsets[r], 'count': batch.offsets[r + 1] - batch.offsets[r]} for r in range(runs)],
+        'digests': [runDigest(case) for case in batch.cases],
         'launch': {key: int(batch.launch[key]) for key in ('lanes', 'block', 'listCapacity')},
         'clock': {'time': [float(t).hex() for t in clock['time']], 'steps': [int(s) for s in clock['steps']],
                   'until': [float(u).hex() for u in batch.until], 'maxSteps': [int(m) for m in batch.maxSteps]},
@@ -118,7 +148,8 @@ def restore(batch, path: str) -> dict:
 
     Load the checkpoint at `path` into `batch`, a BatchEngine built on the same runs as the one
     saved, and return the caller's `extra`. Raises ValueError when the batch does not match the
-    checkpoint's runs, dimensions or precision.
+    checkpoint's runs, dimensions or precision, or when a run's physics or boundary differ from
+    the saved run's.
 
     '''
 
@@ -127,6 +158,11 @@ def restore(batch, path: str) -> dict:
     runs = [{'first': batch.offsets[r], 'count': batch.offsets[r + 1] - batch.offsets[r]} for r in range(len(batch.cases))]
     if header['dimensions'] != batch.dimensions or header['precision'] != batch.precision or header['runs'] != runs:
         raise ValueError('The checkpoint was written by a batch of other runs, dimensions or precision.')
+    digests = [runDigest(case) for case in batch.cases]
+    differ = [r for r, (saved, now) in enumerate(zip(header['digests'], digests)) if saved != now]
+    if differ:
+        raise ValueError(f'The checkpoint was written for other physics or another bed: runs {differ} differ from '
+                         'the saved runs (settings or bed hash).')
     launch = header['launch']
     batch.engine.setLaunch(launch['block'], launch['lanes'], launch['listCapacity'])
     batch.launch = dict(launch)

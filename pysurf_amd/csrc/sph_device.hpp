@@ -322,12 +322,19 @@ __device__ __forceinline__ double pairDistance(double dx, double dy, double dz, 
 // below t and r2 above it a root above t. Inside the band, and for a NaN, the root itself is
 // compared.
 template <typename Real>
+__device__ __forceinline__ bool rootBelow(Real r2, Real t) {
+    // an empty volatile asm keeps the rare rounded root in a branch of its own: the compiler would
+    // otherwise compute it for every pair and select
+    __asm__ volatile("" : "+v"(r2));
+    return sqrt(r2) < t;
+}
+template <typename Real>
 __device__ __forceinline__ bool withinSupport(Real r2, Real t) {
     constexpr Real band = sizeof(Real) == 4 ? Real(9.5367431640625e-07) : Real(3.552713678800501e-15);
     const Real t2 = t * t;
     if (r2 < t2 * (Real(1) - band)) return true;
     if (r2 > t2 * (Real(1) + band)) return false;
-    return sqrt(r2) < t;
+    return rootBelow(r2, t);
 }
 
 // The distance a pair sum uses once its pair is known, from its squared distance: in float32 the
@@ -441,15 +448,18 @@ __device__ __forceinline__ Quad<Real> loadLds(const LdsVec4<Real>* from) {
     const Vec4<Real> v = *from;
     return Quad<Real>{v.x, v.y, v.z, v.w};
 }
-// Quad q of the record at LDS index `index` of the workgroup's copy, and component c of it.
+// The workgroup's copy in LDS is a structure of arrays: quad q of the record at LDS index `index`
+// sits at quad position q * stride + index, `stride` being the copy's capacity in records. A wave
+// reading one field of many records then reads consecutive 16-byte slots, spread over the banks,
+// where 64-byte records would map that field of every record onto a quarter of them.
 template <typename Real>
-__device__ __forceinline__ Quad<Real> ldsQuad(int32_t index, int q) {
-    return loadLds((const LdsVec4<Real>*)sweepShared + 4 * index + q);
+__device__ __forceinline__ Quad<Real> ldsQuad(int32_t index, int q, int32_t stride) {
+    return loadLds((const LdsVec4<Real>*)sweepShared + q * stride + index);
 }
 template <typename Real>
-__device__ __forceinline__ Real ldsValue(int32_t index, int q, int c) {
+__device__ __forceinline__ Real ldsValue(int32_t index, int q, int c, int32_t stride) {
     using LdsReal = __attribute__((address_space(3))) const Real;
-    return ((LdsReal*)sweepShared)[16 * index + 4 * q + c];
+    return ((LdsReal*)sweepShared)[4 * (q * stride + index) + c];
 }
 
 // ...or from a workgroup's copy in LDS of the records of the cells around its particles. A tile's
@@ -465,6 +475,7 @@ struct NearCells {
     int32_t lo[kRows], hi[kRows];   // sorted ranges [lo, hi), ascending and disjoint, some empty
     int32_t at[kRows];              // LDS index of lo
     int32_t held;                   // records copied, at most the capacity
+    int32_t stride;                 // the capacity: quads per field array of the copy
     const Record<Real>* r;
     // With `kicked`, every record read takes the kick v + a dt on a fluid particle's velocity, as the
     // continuity sums read the velocities after the kick: `A` is each sorted particle's
@@ -509,6 +520,7 @@ struct NearCells {
             total += hi[row] - lo[row];
         }
         held = total < capacity ? total : capacity;
+        stride = capacity;
     }
 
     // The sorted position of the record at LDS index `index`.
@@ -533,17 +545,17 @@ struct NearCells {
             Record<Real> b = r[fromB];
             kick(a, fromA);
             kick(b, fromB);
-            LdsVec4<Real>* to = shared + 4 * first;
+            LdsVec4<Real>* to = shared + first;
             storeLds(to, a.q0);
-            storeLds(to + 1, a.q1);
-            storeLds(to + 2, a.q2);
-            storeLds(to + 3, a.q3);
+            storeLds(to + stride, a.q1);
+            storeLds(to + 2 * stride, a.q2);
+            storeLds(to + 3 * stride, a.q3);
             if (both) {
-                to = shared + 4 * second;
+                to = shared + second;
                 storeLds(to, b.q0);
-                storeLds(to + 1, b.q1);
-                storeLds(to + 2, b.q2);
-                storeLds(to + 3, b.q3);
+                storeLds(to + stride, b.q1);
+                storeLds(to + 2 * stride, b.q2);
+                storeLds(to + 3 * stride, b.q3);
             }
         }
         __syncthreads();
@@ -564,7 +576,7 @@ struct NearCells {
 #pragma unroll
         for (int i = 0; i < kRows; ++i)
             if (k >= lo[i] && k < hi[i]) index = at[i] + (k - lo[i]);
-        return index >= 0 && index < held ? ldsValue<Real>(index, 0, 0) : r[k].q0.x;
+        return index >= 0 && index < held ? ldsValue<Real>(index, 0, 0, stride) : r[k].q0.x;
     }
 
     // The record of sorted particle k, from LDS when copied there.
@@ -576,11 +588,11 @@ struct NearCells {
             if (k >= lo[i] && k < hi[i]) index = at[i] + (k - lo[i]);
         Record<Real> out;
         if (index >= 0 && index < held) {
-            const LdsVec4<Real>* from = shared + 4 * index;
+            const LdsVec4<Real>* from = shared + index;
             out.q0 = loadLds(from);
-            out.q1 = loadLds(from + 1);
-            out.q2 = loadLds(from + 2);
-            out.q3 = loadLds(from + 3);
+            out.q1 = loadLds(from + stride);
+            out.q2 = loadLds(from + 2 * stride);
+            out.q3 = loadLds(from + 3 * stride);
         } else {
             out = r[k];
             kick(out, k);

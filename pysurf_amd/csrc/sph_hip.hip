@@ -626,6 +626,7 @@ __global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
     NearCells<D, Real> cells;
     cells.r = rec.r;
     cells.held = 0;
+    cells.stride = cached;
     cells.kicked = false;
 #pragma unroll
     for (int i = 0; i < NearCells<D, Real>::kRows; ++i) cells.lo[i] = cells.hi[i] = cells.at[i] = 0;
@@ -670,7 +671,7 @@ __global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
             forStencil<D>(run, start, cx, oy, cz, [&](int32_t begin, int32_t end, int32_t middle, int32_t right) {
                 const int32_t copied = cells.rangeBase(begin, end);
                 auto pointX = [&](int32_t k) {
-                    return copied >= 0 ? ldsValue<Real>(copied + (k - begin), 0, 0) : rec.r[k].q0.x;
+                    return copied >= 0 ? ldsValue<Real>(copied + (k - begin), 0, 0, cached) : rec.r[k].q0.x;
                 };
                 const int32_t from = D == 3 ? firstAtLeast(begin, middle, me.p[0] - reach, pointX) : begin;
                 const int32_t to = D == 3 ? firstAbove(right, end, me.p[0] + reach, pointX) : end;
@@ -684,16 +685,16 @@ __global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
                         const bool wide = D == 3 || WithSweep1 || !uniform;
                         if (copied >= 0) {
                             const int32_t index = copied + (k - begin);
-                            other.q0 = ldsQuad<Real>(index, 0);
-                            if (wide) other.q1 = ldsQuad<Real>(index, 1);
+                            other.q0 = ldsQuad<Real>(index, 0, cached);
+                            if (wide) other.q1 = ldsQuad<Real>(index, 1, cached);
                             if (WithSweep1) {
                                 if constexpr (D == 2) {
-                                    other.q2.y = ldsValue<Real>(index, 2, 1);
-                                    other.q3.x = ldsValue<Real>(index, 3, 0);
+                                    other.q2.y = ldsValue<Real>(index, 2, 1, cached);
+                                    other.q3.x = ldsValue<Real>(index, 3, 0, cached);
                                 } else {
-                                    other.q2 = ldsQuad<Real>(index, 2);
-                                    other.q3.y = ldsValue<Real>(index, 3, 1);
-                                    other.q3.w = ldsValue<Real>(index, 3, 3);
+                                    other.q2 = ldsQuad<Real>(index, 2, cached);
+                                    other.q3.y = ldsValue<Real>(index, 3, 1, cached);
+                                    other.q3.w = ldsValue<Real>(index, 3, 3, cached);
                                 }
                             }
                         } else {
@@ -1098,6 +1099,7 @@ __global__ void SWEEP_BOUNDS((accelerationWaves<D, Real>()))
     NearCells<D, Real> cells;
     cells.r = rec.r;
     cells.held = 0;
+    cells.stride = cached;
     cells.kicked = false;
 #pragma unroll
     for (int i = 0; i < NearCells<D, Real>::kRows; ++i) cells.lo[i] = cells.hi[i] = cells.at[i] = 0;
@@ -1386,6 +1388,7 @@ __global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
     NearCells<D, Real> cells;
     cells.r = rec.r;
     cells.held = 0;
+    cells.stride = cached;
     // resident, the step's kick v += a dt (sphGpu.py:1084-1109) reaches every velocity the sums read
     const double dt = resident && runSteps ? (forcedDt > 0.0 ? forcedDt : runTimeStep(runs[r], clock, r)) : 0.0;
     cells.kicked = resident != 0;
@@ -2785,6 +2788,13 @@ int hostSweep(sph_context* ctx, int which, const double* velocity, const double*
               void* out0, double* out1, double scalar) {
     const int64_t n = ctx->total;
     if (n == 0) return SPH_OK;
+    // the host steps pick their launches by the first run's options, which every run must then share
+    const RunDev& run0 = ctx->runsHost[0];
+    for (const RunDev& run : ctx->runsHost)
+        if (run.limiter != run0.limiter || run.correction != run0.correction || run.diffusion != run0.diffusion)
+            return fail(ctx, SPH_ERR_VALUE,
+                        "The host-step sums need every run of the context to share its viscosity limiter, continuity "
+                        "correction and density diffusion.");
     const int b = ctx->block;
     const int32_t nruns = (int32_t)ctx->runs.size();
     Records<Real>& rec = recordsOf<Real>(ctx);
@@ -2803,7 +2813,6 @@ int hostSweep(sph_context* ctx, int which, const double* velocity, const double*
         out.eddyWater = (Real)ctx->closureWater;
     }
     const double* y64 = ctx->state.position + ctx->state.stride;
-    const RunDev& run0 = ctx->runsHost[0];
     const int32_t* order = ctx->order;
     const int32_t* start = ctx->cellStart;
     const NeighborList list = ctx->list;
@@ -2924,7 +2933,7 @@ int shiftSweep(sph_context* ctx, const double* velocity, const double* density, 
     const NeighborList list = ctx->list;
     if (list.capacity > 0)
         WITH_LANES(ctx, (buildNeighbors<D, Real, L, false><<<tileGrid(ctx), b, sweepLds<D, Real>(ctx, 0), ctx->stream>>>(
-                            ctx->dRuns, tilesOf(ctx), ctx->order, ctx->cellStart, rec, list, emptyOut<Real>(ctx),
+                            ctx->dRuns, tilesOf(ctx), ctx->order, ctx->cellStart, rec, ctx->list, emptyOut<Real>(ctx),
                                 ctx->clock, y64, 0, nearRecords<D, Real>(ctx), n)));
     Real* fields = static_cast<Real*>(ctx->shiftFields);
     for (int32_t pass = 0; pass < 2; ++pass)

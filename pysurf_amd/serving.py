@@ -315,6 +315,9 @@ class Served:
     steps: int
     breaking: object
     tracker: object = None
+    # why the request ended without its run finishing: 'budget' (its state can never fit the HBM
+    # budget, refused on arrival), or the run's device failure ('non-finite', 'time step collapsed')
+    refused: str = ''
 
     @property
     def latency(self) -> float:
@@ -355,6 +358,12 @@ class Server:
                                             chunk = self.chunk, measure = self.measure)
         return self.fleets[dimensions]
 
+    def refuse(self, request, now: float, reason: str) -> None:
+        '''Record `request` as ended at `now` without a run, for `reason`.'''
+        self.served.append(Served(key = request.key, dimensions = request.case.dimensions, particles = request.case.count,
+                                  fluid = request.case.fluidCount, arrival = request.arrival, admitted = float('nan'),
+                                  finished = now, simulated = 0.0, steps = 0, breaking = None, refused = reason))
+
     def serve(self, requests: list, wallLimit: float = float('inf')) -> list:
 
         '''
@@ -362,6 +371,11 @@ class Server:
         Serve `requests` (sorted by arrival) until all finish or `wallLimit` seconds pass; returns
         the Served records. Each round launches every fleet's next interval, then reads the last
         interval's positions on the host while the device steps, so the trackers overlap the GPU.
+
+        Admission takes the queue in arrival order and admits every request whose state fits the
+        budget left, so a large request waiting for room does not hold back smaller ones behind it
+        (it keeps its place and is admitted as soon as room frees up). A request whose state alone
+        exceeds the budget can never be admitted: it is refused on arrival.
 
         '''
 
@@ -376,15 +390,24 @@ class Server:
         while pending and clock() < wallLimit:
             now = clock()
             while waiting and waiting[0].arrival <= now:
-                queued.append(waiting.pop(0))
+                request = waiting.pop(0)
+                if self.footprint(request.case) > self.budgetBytes:
+                    self.refuse(request, now, 'budget')
+                    pending -= 1
+                else:
+                    queued.append(request)
             # Membership changes while every fleet is idle: finished runs leave, admitted runs join
             joining: dict = {}
-            while queued and inUse + self.footprint(queued[0].case) <= self.budgetBytes:
-                request = queued.pop(0)
-                inUse += self.footprint(request.case)
-                admittedAt[request.key] = now
-                joining.setdefault(request.case.dimensions, []).append(
-                    Member(key = request, case = request.case, endTime = request.endTime))
+            still: list = []
+            for request in queued:
+                if inUse + self.footprint(request.case) <= self.budgetBytes:
+                    inUse += self.footprint(request.case)
+                    admittedAt[request.key] = now
+                    joining.setdefault(request.case.dimensions, []).append(
+                        Member(key = request, case = request.case, endTime = request.endTime))
+                else:
+                    still.append(request)
+            queued = still
             self.queueDepth.append((now, len(queued)))
             for dimensions in set(joining) | set(self.fleets):
                 fleet = self.fleet(dimensions)
@@ -394,8 +417,14 @@ class Server:
             active = [fleet for fleet in self.fleets.values() if fleet.batch is not None
                       and not all(member.done for member in fleet.members)]
             if not active and not readings:
+                # nothing on the device: wait for the next arrival (queued requests fit once room
+                # frees, and with nothing running all the room is free, so none is left queued)
                 if waiting:
                     timeModule.sleep(max(0.0, min(waiting[0].arrival - clock(), 0.05)))
+                elif not queued:
+                    break
+                else:
+                    timeModule.sleep(0.001)
                 continue
             for fleet in active:
                 fleet.launch()
@@ -426,7 +455,8 @@ def poissonRequests(cases: list, rate: float, seed: int = 0, endTime: float = 10
 
 
 def latencyPercentiles(served: list) -> dict:
-    latencies = np.array([record.latency for record in served])
+    '''p50 and p99 of the served runs' latencies (requests refused without a run left out).'''
+    latencies = np.array([record.latency for record in served if not record.refused])
     if latencies.size == 0:
         return {'p50': float('nan'), 'p99': float('nan')}
     return {'p50': float(np.percentile(latencies, 50)), 'p99': float(np.percentile(latencies, 99))}

@@ -59,3 +59,19 @@ def testOtherFilesAreRefused(tmp_path):
 def testUnstoredDtypesAreRefused(tmp_path):
     with pytest.raises(ValueError, match = 'dtype'):
         checkpoint.writeCheckpoint(str(tmp_path / 'x'), {}, {'c': np.zeros(2, dtype = np.complex128)})
+
+
+def testRunDigestsTellPhysicsAndBedsApart():
+    # The digest a checkpoint keeps of each run changes with the bed (the same particle count over a
+    # steeper slope) and with the physics (another sound speed), and repeats for the same run
+    import dataclasses
+
+    from pysurf_amd.scenarios import solitaryOnSlope
+    case = solitaryOnSlope(particlesPerDepth = 6, seed = 1)
+    again = solitaryOnSlope(particlesPerDepth = 6, seed = 1)
+    assert checkpoint.runDigest(case) == checkpoint.runDigest(again)
+    otherBed = dataclasses.replace(case, boundary = solitaryOnSlope(particlesPerDepth = 6, seed = 1, slope = 1.0 / 12.0).boundary)
+    assert otherBed.count == case.count
+    assert checkpoint.runDigest(otherBed) != checkpoint.runDigest(case)
+    louder = dataclasses.replace(case, soundSpeed = case.soundSpeed * 1.01)
+    assert checkpoint.runDigest(louder) != checkpoint.runDigest(case)

@@ -54,3 +54,25 @@ def testACheckpointOfOtherRunsIsRefused(tmp_path):
         restore(BatchEngine(cases()[:1], 'float32'), path)
     with pytest.raises(ValueError, match = 'other runs'):
         restore(BatchEngine(cases(), 'float64'), path)
+
+
+def testACheckpointForAnotherBedOrOtherPhysicsIsRefused(tmp_path):
+    # The same particle counts over another bed, or with another sound speed: the run digests differ
+    import dataclasses
+
+    from pysurf_amd.batch import BatchEngine
+    from pysurf_amd.checkpoint import restore, save
+    from pysurf_amd.scenarios import solitaryOnSlope
+    path = str(tmp_path / 'batch.ckpt')
+    runs = cases()
+    save(BatchEngine(runs, 'float32'), path)
+    steeper = solitaryOnSlope(particlesPerDepth = 8, seed = 3, slope = 1.0 / 12.0)
+    otherBed = [runs[0], dataclasses.replace(runs[1], boundary = steeper.boundary)]
+    assert otherBed[1].count == runs[1].count
+    with pytest.raises(ValueError, match = 'another bed'):
+        restore(BatchEngine(otherBed, 'float32'), path)
+    louder = [runs[0], dataclasses.replace(runs[1], soundSpeed = 1.05 * runs[1].soundSpeed)]
+    with pytest.raises(ValueError, match = 'other physics'):
+        restore(BatchEngine(louder, 'float32'), path)
+    # the batch it was written from still restores
+    restore(BatchEngine(cases(), 'float32'), path)

@@ -66,3 +66,23 @@ def testTheDeviceMergeIsTheHostMerge(dimensions):
     batch.advance(10)
     assert int(batch.clocks()['steps'][0]) == steps + 10
     assert np.all(np.isfinite(batch.state(0).positions))
+
+
+def testASplitThenMergedRunReportsTheDevicesMasses():
+    # A split on the device followed by a merge of every daughter restores the run's count, but its
+    # slots now hold merged particles in another order: state() must read masses, lengths and kinds
+    # from the device, not the case's
+    from pysurf_amd.batch import BatchEngine
+    from pysurf_amd.scenarios import solitaryOnSlope
+    case = solitaryOnSlope(particlesPerDepth = 8, seed = 4)
+    unsplit = float(case.particles.masses[case.particles.isFluid][0])
+    batch = BatchEngine([case], 'float32', lanes = 2, listCapacity = 64)
+    chosen = batch.split([(3.0, 5.0)], lengthRatio = 1.0)
+    assert int(chosen[0]) > 0
+    result = batch.merge([(3.0, 5.0)], unsplit, lengthRatio = 1.0)
+    assert result[0].groups == int(chosen[0])
+    assert batch.offsets[1] == case.count
+    state = batch.state(0)
+    device = batch.engine.download('mass', None, case.count)
+    assert np.array_equal(state.masses, device)
+    assert state.masses.sum() == pytest.approx(case.particles.masses.sum(), rel = 1e-6)

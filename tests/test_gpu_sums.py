@@ -337,3 +337,28 @@ def testTheEngineFailsLoudlyWithoutADevicePath():
     from pysurf_amd.pair_sums import PairSums
     with pytest.raises(ValueError):
         PairSums(2, 0.05, np.zeros(2), np.ones(2), device = 'cpu')
+
+
+def testHostStepSumsRefuseRunsWithOtherOptions():
+    # The host-step sums launch by the first run's options, so a context whose runs differ in the
+    # limiter, the correction or the diffusion is refused rather than summed with run 0's options
+    import dataclasses
+
+    from pysurf_amd import _native
+    from pysurf_amd.batch import BatchEngine
+    from pysurf_amd.scenarios import solitaryOnSlope
+    first = solitaryOnSlope(particlesPerDepth = 6, seed = 1)
+    for change in ({'limiter': not first.limiter}, {'correction': not first.correction}, {'diffusion': 'molteni'}):
+        batch = BatchEngine([first, dataclasses.replace(solitaryOnSlope(particlesPerDepth = 6, seed = 2), **change)],
+                            'float32')
+        engine = batch.engine
+        status = engine.library.sph_velocity_gradients(engine.handle, engine.pointer('velocity'),
+                                                       engine.pointer('density'), engine.pointer('matrix'))
+        assert status == _native.SPH_ERR_VALUE, change
+        assert b'share' in engine.library.sph_last_error(engine.handle)
+    # runs that agree are summed, over the cells of the bound positions
+    batch = BatchEngine([first, solitaryOnSlope(particlesPerDepth = 6, seed = 2)], 'float32')
+    engine = batch.engine
+    assert engine.library.sph_build_cells(engine.handle) == _native.SPH_OK
+    assert engine.library.sph_velocity_gradients(engine.handle, engine.pointer('velocity'), engine.pointer('density'),
+                                                 engine.pointer('matrix')) == _native.SPH_OK
