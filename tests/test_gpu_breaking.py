@@ -1,7 +1,7 @@
 '''
 
-Breaking on the device against the float64 restatement, and the request loop's promise that a run
-served in a mix is the run served alone.
+Breaking on the device against the float64 restatement, at 16 and 64 particles per depth, and the
+request loop's promise that a run served in a mix is the run served alone.
 
 The 16-particles-per-depth solitary wave on a 1:15 beach is stepped on the GPU to its breaking point
 with float32 pairs and with the plain float64 kernel, read by the same tracker every 0.01 sqrt(d/g)
@@ -86,18 +86,32 @@ def testARunServedInAMixIsTheRunServedAlone():
                 (k, name)
 
 
+GOLDEN64 = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'golden', 'breaking_c2_64ppd.json')
+
+
 @pytest.mark.timeout(600)
-def testTheSingle64ppdRunBreaksInFloat32WhereItBreaksInFloat64():
+def testTheSingle64ppdRunBreaksWhereTheOracleBreaks():
     # The single-run configuration (64 particles per depth, H/d and slope drawn from seed 0): the
-    # float32-pair engine against the plain float64 kernel, which follows the oracle step for step,
-    # over the same perturbed ensemble of 8 starts: means within 1% in time and place
+    # float32-pair engine and the plain float64 kernel each step the perturbed ensemble of 8 starts to
+    # its breaking point, and each precision's mean breaking time and crest position must lie within
+    # 1% of the float64 oracle's mean over the same members (tests/golden/breaking_c2_64ppd.json,
+    # tests/golden/make_breaking_64.py), and within 1% of each other
     from pysurf_amd.scenarios import singleRun
+    expected = json.load(open(GOLDEN64))
+    members = expected['members']
+    assert len(members) >= 3
+    oracleMean = np.array([np.mean([m['time'] for m in members]), np.mean([m['crestX'] for m in members])])
     case = singleRun(seed = 0, particlesPerDepth = 64)
     samples = {precision: ensembleBreaking(case, 8, precision) for precision in ('float64', 'float32')}
     for precision, values in samples.items():
         print(f'{precision}: times {values[:, 0].round(4).tolist()}, crest x {values[:, 1].round(3).tolist()}', flush = True)
+    print(f'oracle ({len(members)} members): times {[round(m["time"], 4) for m in members]}, '
+          f'crest x {[round(m["crestX"], 3) for m in members]}', flush = True)
     double, single = samples['float64'].mean(axis = 0), samples['float32'].mean(axis = 0)
-    print(f'means: float64 {double}, float32 {single}', flush = True)
+    print(f'means: float64 {double}, float32 {single}, oracle {oracleMean}', flush = True)
+    for mean in (double, single):
+        assert mean[0] == pytest.approx(oracleMean[0], rel = 0.01)
+        assert mean[1] == pytest.approx(oracleMean[1], rel = 0.01)
     assert single[0] == pytest.approx(double[0], rel = 0.01)
     assert single[1] == pytest.approx(double[1], rel = 0.01)
 
