@@ -381,52 +381,57 @@ def cpuModel() -> str:
     return 'unknown'
 
 
-def cpuBaseline(cases, seconds: float) -> dict:
+def oracleRunOf(case):
+    '''The float64 restatement (oracle/sph_oracle.cpp) of `case` from its start.'''
+    from oracle import oracle
+    p = case.particles
+    physics = oracle.Physics(dims = case.dimensions, h = case.smoothingLength, domainMin = case.domainMin,
+                             domainMax = case.domainMax, kernelType = case.kernelType, rho0 = case.referenceDensity,
+                             c0 = case.soundSpeed, alpha = case.alpha, boundaryFactor = case.boundaryViscosityFactor,
+                             gravity = case.gravity, diffusion = case.diffusion, delta = case.delta,
+                             extrapolatedWalls = case.extrapolatedWalls, correction = case.correction,
+                             limiter = case.limiter, densityFloor = case.densityFloor, cfl = case.cfl,
+                             maxTimeStep = case.maxTimeStep)
+    return oracle.OracleRun(physics, case.boundary.description(), p.positions, p.velocities, p.densities, p.masses,
+                            p.isFluid)
+
+
+def cpuBaseline(seconds: float) -> dict:
 
     '''
 
-    The float64 CPU restatement (oracle/sph_oracle.cpp) on the host's cores: the sample's runs
-    stepped in turn after a warm-up step each, three repeats of about `seconds` / 3, the median.
+    The float64 CPU restatement (oracle/sph_oracle.cpp) on the host's cores, on the reference's
+    CPU-runnable case and a subset of the single-run case: the 16 particles-per-depth solitary wave
+    (H/d 0.45, 1:15 beach) stepped its 3000 steps from the start, then the 64 particles-per-depth
+    run (H/d and slope from seed 0) for as many steps as fill about a sixth of `seconds`. Three
+    repeats after a warm-up, each from the runs' starts; the median of their fluid updates per second.
 
     '''
-
-    import numpy as np
 
     from oracle import oracle
+    from pysurf_amd.scenarios import singleRun, solitaryOnSlope
     threads = min(16, os.cpu_count() or 1)
     oracle.threads(threads)
-    runs = []
-    for case in cases:
-        p = case.particles
-        physics = oracle.Physics(dims = case.dimensions, h = case.smoothingLength, domainMin = case.domainMin,
-                                 domainMax = case.domainMax, kernelType = case.kernelType,
-                                 rho0 = case.referenceDensity, c0 = case.soundSpeed, alpha = case.alpha,
-                                 boundaryFactor = case.boundaryViscosityFactor, gravity = case.gravity,
-                                 diffusion = case.diffusion, delta = case.delta,
-                                 extrapolatedWalls = case.extrapolatedWalls, correction = case.correction,
-                                 limiter = case.limiter, densityFloor = case.densityFloor)
-        run = oracle.OracleRun(physics, case.boundary.description(), p.positions, p.velocities, p.densities,
-                               p.masses, p.isFluid)
-        run.advance(np.inf, 1)
-        runs.append((run, case.fluidCount))
+    small, single = solitaryOnSlope(particlesPerDepth = 16), singleRun(seed = 0, particlesPerDepth = 64)
+    warm = oracleRunOf(single)
     start = time.perf_counter()
-    for run, _ in runs:
-        run.advance(np.inf, 1)
-    perRound = max(time.perf_counter() - start, 1e-6)
-    rounds = max(1, int(seconds / 3.0 / perRound))
+    warm.advance(float('inf'), 2)
+    perStep = max((time.perf_counter() - start) / 2.0, 1e-6)
+    subset = max(5, int(seconds / 6.0 / perStep))
     rates = []
     for _ in range(3):
+        runs = [(oracleRunOf(small), 3000, small.fluidCount), (oracleRunOf(single), subset, single.fluidCount)]
         start = time.perf_counter()
-        for run, _ in runs:
-            run.advance(np.inf, rounds)
+        for run, steps, _ in runs:
+            assert run.advance(float('inf'), steps) == steps
         elapsed = time.perf_counter() - start
-        rates.append(sum(fluid for _, fluid in runs) * rounds / elapsed)
+        rates.append(sum(steps * fluid for _, steps, fluid in runs) / elapsed)
     value = statistics.median(rates)
-    sizes = ', '.join(f'{case.name} ({case.count} particles)' for case in cases)
     log(f'cpu: {value:.4e} fluid updates/s')
     return {'value': value, 'unit': 'particle-updates/s', 'cores': threads, 'kind': 'port',
-            'sample': f'the float64 C++ OpenMP restatement (oracle/sph_oracle.cpp) stepping {sizes} '
-                      f'{rounds} steps each, median of 3 repeats',
+            'sample': f'the float64 C++ OpenMP restatement (oracle/sph_oracle.cpp) stepping {small.name} '
+                      f'({small.count} particles) its 3000 steps and {single.name} ({single.count} particles) '
+                      f'{subset} steps, each from its start, median of 3 repeats',
             'repeats': rates, 'nproc': os.cpu_count(), 'cpu_model': cpuModel()}
 
 
@@ -490,7 +495,7 @@ def main() -> int:
     ridge = None if args.no_ridge else ridgeRun(args)
     serving = None if args.no_serving else servingRun(args)
     gate = None if args.no_gate else gateRun(args)
-    cpu = None if args.no_cpu else cpuBaseline(cases[:2], args.cpu_seconds)
+    cpu = None if args.no_cpu else cpuBaseline(args.cpu_seconds)
 
     sizes = np.array([case.count for case in cases])
     line = {

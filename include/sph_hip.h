@@ -155,11 +155,19 @@ int sph_neighbor_rows(sph_context* ctx, const int64_t* offsets, int32_t* rows);
 int sph_enforce(sph_context* ctx);
 
 /* Each run's clock: WcsphSolver.advance's loop condition (sphSolver.py:1983). A run steps while
- * time < untilTime and steps < maxSteps. */
+ * time < untilTime and steps < maxSteps and it has not failed (sph_run_failures); setting the clock
+ * clears a failure. */
 int sph_set_clock(sph_context* ctx, int32_t run, double time, double untilTime, int64_t steps, int64_t maxSteps);
 /* Copies every run's time, last step size, steps taken and maxima (vmax, amax) to host arrays
  * (any may be null). Synchronizes. */
 int sph_get_clock(sph_context* ctx, double* time, double* lastDt, int64_t* steps, double* maxima);
+/* Each run's failure, one int32 per run into a host array: 0 none, 1 its state stopped being finite
+ * (a speed, acceleration or position), 2 its time step fell below a hundredth of the acoustic step,
+ * cfl hmin / (100 c0). The reference's run loops raise NumericalInstabilityError for both
+ * (BreakingWave.py:892-919, BreakingWave3D.py:748-769); here a failed run's clock stops on the device
+ * at the step that found it, the other runs of the context step on, and sph_set_clock clears it.
+ * Synchronizes. */
+int sph_run_failures(sph_context* ctx, int32_t* failures);
 
 /* `steps` device-resident steps of every active run: PairSums.residentRates + residentIntegrate
  * (sphGpu.py:2237-2315) with the time step reduced on the device instead of cflTimeStep on the

@@ -119,6 +119,7 @@ signatures = {
     'sph_set_clock': [ctypes.c_void_p, ctypes.c_int32, ctypes.c_double, ctypes.c_double, ctypes.c_int64,
                       ctypes.c_int64],
     'sph_get_clock': [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
+    'sph_run_failures': [ctypes.c_void_p, ctypes.c_void_p],
     'sph_advance': [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32],
     'sph_rates': [ctypes.c_void_p, ctypes.c_void_p],
     'sph_integrate': [ctypes.c_void_p, ctypes.c_double],
@@ -141,6 +142,9 @@ signatures = {
 
 # The columns of sph_global_sums, per run
 globalSumNames = ['mass', 'kinetic', 'potential', 'elastic', 'momentumX', 'momentumY', 'momentumZ']
+
+# The failure codes of sph_run_failures, and what each means
+failureNames = {0: '', 1: 'non-finite', 2: 'time step collapsed'}
 
 # Kernel ids of sph_kernel_times
 kernelNames = ['clock', 'cellKeys', 'cellScan', 'cellScatter', 'cellRank', 'gatherRecords', 'sweepWallAndGradients',

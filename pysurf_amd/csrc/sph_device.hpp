@@ -36,6 +36,10 @@ struct Clock {
     int64_t* maxSteps;
     unsigned long long* maxima;   // 2 per run: |v|max and |a|max as float64 bit patterns
     int32_t* active;
+    // why a run stopped by itself: 0 none, 1 its state stopped being finite, 2 its time step collapsed
+    // (set by the step's maxima; a failed run takes no further device-clocked step until its clock
+    // is set again)
+    int32_t* failed;
     double* speed;                // per sorted slot: |v| of a fluid particle at the gather, else 0
     double* acceleration;         // per sorted slot: |a| of a fluid particle, else 0
 };

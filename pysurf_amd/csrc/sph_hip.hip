@@ -159,7 +159,8 @@ __global__ void __launch_bounds__(kScanThreads) scanAdd(int32_t* __restrict__ ou
 __global__ void clockBegin(Clock clock, int32_t runs, int32_t useClock) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= runs) return;
-    clock.active[r] = useClock ? (clock.time[r] < clock.until[r] && clock.steps[r] < clock.maxSteps[r]) : 1;
+    clock.active[r] =
+        useClock ? (clock.time[r] < clock.until[r] && clock.steps[r] < clock.maxSteps[r] && !clock.failed[r]) : 1;
     clock.maxima[2 * r] = 0ull;
     clock.maxima[2 * r + 1] = 0ull;
 }
@@ -210,8 +211,14 @@ __global__ void __launch_bounds__(256)
     }
 }
 
-__global__ void runMaxima(const int64_t* __restrict__ runChunk, const double* __restrict__ partial, Clock clock,
-                          int32_t runs) {
+// Each run's maxima; then, as the reference's run loops stop a run (BreakingWave.py:892-919,
+// BreakingWave3D.py:748-769), its failure: a maximum that is not finite (a non-finite speed,
+// acceleration or position reads as infinite, since fmax passes NaN by), or a time step below a
+// hundredth of the acoustic one, cfl hmin / (100 c0) (the reference's step limit: a hundred times
+// the steps an interval takes at the acoustic step). With `useClock` a failed run stops at once: the
+// step's integration and every later device-clocked step leave it alone.
+__global__ void runMaxima(const RunDev* __restrict__ runsDev, const int64_t* __restrict__ runChunk,
+                          const double* __restrict__ partial, Clock clock, int32_t runs, int32_t useClock) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= runs || !clock.active[r]) return;
     double v = 0.0, a = 0.0;
@@ -221,6 +228,12 @@ __global__ void runMaxima(const int64_t* __restrict__ runChunk, const double* __
     }
     clock.maxima[2 * r] = (unsigned long long)__double_as_longlong(v);
     clock.maxima[2 * r + 1] = (unsigned long long)__double_as_longlong(a);
+    const RunDev& run = runsDev[r];
+    int32_t failure = 0;
+    if (!(v < INFINITY) || !(a < INFINITY)) failure = 1;
+    else if (runTimeStep(run, clock, r) < run.cfl * run.hmin / (100.0 * run.c0)) failure = 2;
+    if (failure && !clock.failed[r]) clock.failed[r] = failure;
+    if (failure && useClock) clock.active[r] = 0;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -258,7 +271,8 @@ __global__ void cellKeys(const RunDev* __restrict__ runs, const int32_t* __restr
         const int r = runOf[i];
         const RunDev& run = runs[r];
         if (clockMode && i == run.first) {
-            clock.active[r] = clockMode == 2 || (clock.time[r] < clock.until[r] && clock.steps[r] < clock.maxSteps[r]);
+            clock.active[r] = clockMode == 2 ||
+                              (clock.time[r] < clock.until[r] && clock.steps[r] < clock.maxSteps[r] && !clock.failed[r]);
             clock.maxima[2 * r] = 0ull;
             clock.maxima[2 * r + 1] = 0ull;
         }
@@ -398,7 +412,10 @@ __global__ void gatherRecords(const RunDev* __restrict__ runs, const int32_t* __
         f.press = pressure;
         rec.r[s] = pack<D, Real>(f);
         rec.key[s] = src.key[i];
-        speed = sqrt(v2);
+        bool finite = v2 < INFINITY;
+#pragma unroll
+        for (int a = 0; a < D; ++a) finite = finite && fabs(src.pos[a][i]) < INFINITY;
+        speed = finite ? sqrt(v2) : INFINITY;
         valid = isFluid;
     }
     if (resident && inRange && clock.active[r]) clock.speed[s] = valid ? speed : 0.0;
@@ -1258,7 +1275,7 @@ __global__ void SWEEP_BOUNDS((accelerationWaves<D, Real>()))
                     if (resident) rec.A[s * D + a] = v;
                     a2 += v * v;
                 }
-                magnitude = (double)sqrt(a2);
+                magnitude = a2 < Real(INFINITY) ? (double)sqrt(a2) : INFINITY;
             }
             if (correction) {
                 Real M[3][3] = {}, Lm[3][3] = {};
@@ -2728,7 +2745,8 @@ int stepOnce(sph_context* ctx, int phase, int useClock, double forcedDt) {
             chunkMaxima<<<(unsigned)ctx->sumChunkCount, 256, 0, ctx->stream>>>(
                 ctx->sumChunkRun, ctx->sumChunkFirst, ctx->dRuns, ctx->clock.speed, ctx->clock.acceleration, ctx->clock,
                 ctx->sumPartial);
-            runMaxima<<<gridFor(nruns, 64), 64, 0, ctx->stream>>>(ctx->sumRunChunk, ctx->sumPartial, ctx->clock, nruns);
+            runMaxima<<<gridFor(nruns, 64), 64, 0, ctx->stream>>>(ctx->dRuns, ctx->sumRunChunk, ctx->sumPartial,
+                                                                  ctx->clock, nruns, useClock);
             timeEnd(ctx, slot);
         }
     }
@@ -3145,6 +3163,7 @@ int sph_configure(sph_context* ctx, const sph_run* runs, int32_t count, int64_t 
     if (status == SPH_OK) status = allocate(ctx, &ctx->clock.maxSteps, count);
     if (status == SPH_OK) status = allocate(ctx, &ctx->clock.maxima, 2 * (size_t)count);
     if (status == SPH_OK) status = allocate(ctx, &ctx->clock.active, count);
+    if (status == SPH_OK) status = allocate(ctx, &ctx->clock.failed, count);
     if (status == SPH_OK) status = allocate(ctx, &ctx->clock.speed, cap);
     if (status == SPH_OK) status = allocate(ctx, &ctx->clock.acceleration, cap);
     if (status == SPH_OK) status = allocate(ctx, &ctx->dVel, 3);
@@ -3193,6 +3212,7 @@ int sph_configure(sph_context* ctx, const sph_run* runs, int32_t count, int64_t 
     HIP_TRY(hipMemcpyAsync(ctx->clock.steps, zeroSteps.data(), count * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(hipMemcpyAsync(ctx->clock.maxSteps, zeroSteps.data(), count * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(hipMemsetAsync(ctx->clock.maxima, 0, 2 * count * sizeof(unsigned long long), ctx->stream));
+    HIP_TRY(hipMemsetAsync(ctx->clock.failed, 0, count * sizeof(int32_t), ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     return SPH_OK;
 }
@@ -3354,6 +3374,18 @@ int sph_set_clock(sph_context* ctx, int32_t run, double time, double untilTime, 
     HIP_TRY(hipMemcpyAsync(ctx->clock.until + run, &untilTime, sizeof untilTime, hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(hipMemcpyAsync(ctx->clock.steps + run, &steps, sizeof steps, hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(hipMemcpyAsync(ctx->clock.maxSteps + run, &maxSteps, sizeof maxSteps, hipMemcpyHostToDevice, ctx->stream));
+    const int32_t cleared = 0;
+    HIP_TRY(hipMemcpyAsync(ctx->clock.failed + run, &cleared, sizeof cleared, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return SPH_OK;
+}
+
+int sph_run_failures(sph_context* ctx, int32_t* failures) {
+    if (!ctx) return SPH_ERR_VALUE;
+    if (!failures) return fail(ctx, SPH_ERR_VALUE, "No failures array.");
+    const size_t count = ctx->runs.size();
+    if (count == 0) return SPH_OK;
+    HIP_TRY(hipMemcpyAsync(failures, ctx->clock.failed, count * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     return SPH_OK;
 }

@@ -301,7 +301,9 @@ class Engine:
                                               dt.ctypes.data_as(ctypes.c_void_p),
                                               steps.ctypes.data_as(ctypes.c_void_p),
                                               maxima.ctypes.data_as(ctypes.c_void_p)), 'sph_get_clock')
-        return {'time': time, 'dt': dt, 'steps': steps, 'maxima': maxima.reshape(runs, 2)}
+        failed = np.zeros(runs, dtype = np.int32)
+        self.check(self.library.sph_run_failures(self.handle, failed.ctypes.data_as(ctypes.c_void_p)), 'sph_run_failures')
+        return {'time': time, 'dt': dt, 'steps': steps, 'maxima': maxima.reshape(runs, 2), 'failed': failed}
 
     def advance(self, steps: int, graph: bool = True) -> None:
         self.check(self.library.sph_advance(self.handle, int(steps), 1 if graph else 0), 'sph_advance')

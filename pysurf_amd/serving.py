@@ -11,6 +11,10 @@ reads the smoothed surface of the strip a quarter depth wide along the tank's ce
 over the bed's section there). A run is done when its breaking point is recorded or its time runs
 out.
 
+A run whose state stops being finite or whose time step collapses stops on the device at the step
+that finds it (sph_run_failures): the request loop ends that request, recording why, and the other
+runs step on; runToBreaking raises NumericalInstabilityError, as the reference's run loop does.
+
 Membership changes between intervals: finished runs leave and admitted runs join, and the fleet
 rebuilds its engine from the runs' current states, each run's particles in the same slot order and
 its clock carried over. Because a run's step depends only on its own state and clock (its cells,
@@ -27,9 +31,10 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from pysurf_amd import readout
+from pysurf_amd import _native, readout
 from pysurf_amd.batch import BatchEngine
 from pysurf_amd.breaking import BreakingTracker, trackInterval
+from pysurf_amd.errors import NumericalInstabilityError
 
 # HBM per particle the admission counts: state 81 B (f64 position, velocity, density; f32 pressure,
 # acceleration, mass, length; a flag), the sorted record and its key, G and L, the neighbor list,
@@ -109,6 +114,7 @@ class Member:
     finishedWall: float = float('nan')
     doneTime: float = float('nan')
     doneSteps: int = 0
+    failure: str = ''           # the device's failure of the run ('non-finite', 'time step collapsed')
 
     def __post_init__(self) -> None:
         if self.particles is None:
@@ -209,10 +215,11 @@ class Fleet:
         clock = self.batch.clocks()
         needed = 1
         for index, member in enumerate(self.members):
-            until = member.time if member.done else member.nextTrack
+            stopped = member.done or bool(member.failure)
+            until = member.time if stopped else member.nextTrack
             self.batch.setClock(index, member.time, until, member.steps, 2**62)
             dt = float(clock['dt'][index])
-            if not member.done and dt > 0.0:
+            if not stopped and dt > 0.0:
                 needed = max(needed, int(math.ceil((until - member.time) / dt)) + 1)
         self.batch.advance(self.chunk * max(1, -(-needed // self.chunk)))
 
@@ -227,9 +234,13 @@ class Fleet:
 
         while not self.reached():
             self.batch.advance(self.chunk)
-        reading = {'offsets': list(self.batch.offsets), 'members': []}
+        reading = {'offsets': list(self.batch.offsets), 'members': [], 'failed': []}
         for index, member in enumerate(self.members):
             if member.done:
+                continue
+            if member.failure:
+                # a failed run is not measured: it ends at the tracking
+                reading['failed'].append((member, member.time, member.steps))
                 continue
             reading['members'].append((index, member, member.time, member.steps))
             member.nextTrack += member.interval
@@ -241,18 +252,30 @@ class Fleet:
         return reading
 
     def reached(self) -> bool:
-        '''Whether every member's clock has reached its reading; refreshes the members' clocks.'''
+        '''Whether every member's clock has reached its reading or stopped on a failure; refreshes the
+        members' clocks and failures.'''
         clock = self.batch.clocks()
         ready = True
         for index, member in enumerate(self.members):
             member.time, member.steps = float(clock['time'][index]), int(clock['steps'][index])
-            if not member.done and member.time < member.nextTrack:
+            code = int(clock['failed'][index])
+            if code and not member.failure:
+                member.failure = _native.failureNames[code]
+            if not member.done and not member.failure and member.time < member.nextTrack:
                 ready = False
         return ready
 
     def track(self, reading: dict, wall: float) -> list:
-        '''Each member's tracker reads its measures of `reading`; returns the members that finished at it.'''
+        '''Each member's tracker reads its measures of `reading`; returns the members that finished at
+        it, a member whose run failed on the device among them.'''
         finished = []
+        for member, time, steps in reading.get('failed', []):
+            if member.done:
+                continue
+            member.done = True
+            member.doneTime, member.doneSteps = time, steps
+            member.finishedWall = wall
+            finished.append(member)
         for index, member, time, steps in reading['members']:
             if member.done:
                 continue
@@ -281,6 +304,11 @@ def runToBreaking(cases: list, precision: str = 'float32', endTime: float = 10.0
     fleet.launch()
     while True:
         reading = fleet.collect()
+        failed = [member for member in members if member.failure]
+        if failed:
+            # as the reference's run loop raises (BreakingWave.py:892-919)
+            raise NumericalInstabilityError(', '.join(f'run {member.key} ({member.case.name}): {member.failure} at '
+                                                      f't = {member.time:.6f} s' for member in failed))
         if not all(member.done for member in members):
             fleet.launch()
         fleet.track(reading, timeModule.perf_counter())
@@ -438,8 +466,9 @@ class Server:
                                               particles = request.case.count, fluid = request.case.fluidCount,
                                               arrival = request.arrival, admitted = admittedAt[request.key],
                                               finished = member.finishedWall, simulated = member.doneTime,
-                                              steps = member.doneSteps, breaking = member.tracker.breakingPoint,
-                                              tracker = member.tracker))
+                                              steps = member.doneSteps,
+                                              breaking = None if member.failure else member.tracker.breakingPoint,
+                                              tracker = member.tracker, refused = member.failure))
             readings = {fleet: fleet.collect() for fleet in active}
             if not pending:
                 break
