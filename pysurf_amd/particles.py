@@ -1,13 +1,20 @@
 '''
 
-Particle state and lattice seeding: the SoA container the engine pushes and pulls, after
-pySurf's `sphParticles` (src/pysurf/sphParticles.py:41-448).
+The host's particle state and the lattice seeding the scenario starts are laid out with.
+
+`ParticleSystem` keeps the field names and shapes of pySurf's container (src/pysurf/sphParticles.py:
+41-283), which the drop-in interface reads and writes in place: vectors (N, dim), scalars (N,),
+masses per unit depth in 2D, fluid first and boundary after. Its operations are written over the
+list of its array fields, so copying, permuting and counting treat every field alike.
+
+Seeding: a square lattice of cell centres, a predicate's share of it, and a seeded jitter that
+moves each point off the lattice while keeping it inside the water.
 
 '''
 
 from __future__ import annotations
 
-from dataclasses import dataclass
+from dataclasses import dataclass, fields
 from typing import Callable
 
 import numpy as np
@@ -18,8 +25,8 @@ class ParticleSystem:
 
     '''
 
-    SPH particle state (sphParticles.py:41-124): vectors shaped (N, dim), scalars (N,); masses
-    per unit depth in 2D.
+    Particle state, structure of arrays (sphParticles.py:41-124). `lengths` is None while every
+    particle has the run's smoothing length, and an (N,) array once some have split.
 
     '''
 
@@ -35,110 +42,130 @@ class ParticleSystem:
     @classmethod
     def assemble(cls, fluidPositions: np.ndarray, boundaryPositions: np.ndarray, spacing: float,
                  referenceDensity: float, fluidVelocities: np.ndarray | None = None) -> ParticleSystem:
-        '''Fluid then boundary particles, each of mass rho0 s^dim, at rest density (sphParticles.py:81-124).'''
-        fluidPositions = np.asarray(fluidPositions, dtype = float)
-        boundaryPositions = np.asarray(boundaryPositions, dtype = float).reshape(-1, fluidPositions.shape[1])
-        nFluid, nBoundary = len(fluidPositions), len(boundaryPositions)
-        nTotal, dimensions = nFluid + nBoundary, fluidPositions.shape[1]
-        velocities = np.zeros((nTotal, dimensions))
+
+        '''
+
+        The state of a start laid on a lattice of `spacing`: the fluid's points, then the
+        boundary's, every particle carrying its lattice cell's mass at rest density, rho0 s^dim,
+        and starting at rest density with no pressure; the fluid moves with `fluidVelocities` (at
+        rest without them), the boundary stays at rest (sphParticles.py:81-124).
+
+        '''
+
+        fluid = np.asarray(fluidPositions, dtype = float)
+        dimensions = fluid.shape[1]
+        walls = np.asarray(boundaryPositions, dtype = float).reshape(-1, dimensions)
+        kinds = np.zeros(len(fluid) + len(walls), dtype = bool)
+        kinds[:len(fluid)] = True
+        count = len(kinds)
+        moving = np.zeros((count, dimensions))
         if fluidVelocities is not None:
-            velocities[:nFluid] = fluidVelocities
-        return cls(positions = np.vstack([fluidPositions, boundaryPositions]), velocities = velocities,
-                   accelerations = np.zeros((nTotal, dimensions)), densities = np.full(nTotal, referenceDensity),
-                   pressures = np.zeros(nTotal), masses = np.full(nTotal, referenceDensity * spacing**dimensions),
-                   isFluid = np.concatenate([np.ones(nFluid, dtype = bool), np.zeros(nBoundary, dtype = bool)]))
+            moving[:len(fluid)] = fluidVelocities
+        return cls(positions = np.concatenate([fluid, walls]), velocities = moving,
+                   accelerations = np.zeros((count, dimensions)), densities = np.full(count, referenceDensity),
+                   pressures = np.zeros(count), masses = np.full(count, referenceDensity * spacing**dimensions),
+                   isFluid = kinds)
+
+    def _arrays(self):
+        '''(name, array) of every field that holds one.'''
+        for entry in fields(self):
+            value = getattr(self, entry.name)
+            if value is not None:
+                yield entry.name, value
 
     def copy(self) -> ParticleSystem:
-        return ParticleSystem(**{name: (None if getattr(self, name) is None else getattr(self, name).copy())
-                                 for name in ('positions', 'velocities', 'accelerations', 'densities', 'pressures',
-                                              'masses', 'isFluid', 'lengths')})
+        '''An independent copy of every field.'''
+        return ParticleSystem(**{entry.name: None for entry in fields(self)} | {name: values.copy()
+                                                                              for name, values in self._arrays()})
 
     def permute(self, order: np.ndarray) -> None:
-        '''Slot k takes what slot order[k] held (sphParticles.py:186-203).'''
-        for name in ('positions', 'velocities', 'accelerations', 'densities', 'pressures', 'masses', 'isFluid'):
-            setattr(self, name, getattr(self, name)[order])
-        if self.lengths is not None:
-            self.lengths = self.lengths[order]
+        '''Reorder every field: slot k takes what slot order[k] held (sphParticles.py:186-203).'''
+        for name, values in list(self._arrays()):
+            setattr(self, name, values[order])
 
     @property
     def nParticles(self) -> int:
-        return self.positions.shape[0]
+        return len(self.positions)
 
     @property
     def nFluid(self) -> int:
-        return int(np.sum(self.isFluid))
+        return int(np.count_nonzero(self.isFluid))
 
     @property
     def dimensions(self) -> int:
         return self.positions.shape[1]
 
     def kineticEnergy(self) -> float:
-        '''sum of m |v|^2 / 2 over the fluid (sphParticles.py:236-242).'''
-        fluid = self.isFluid
-        return float(0.5 * np.sum(self.masses[fluid] * np.sum(self.velocities[fluid]**2, axis = 1)))
+        '''The fluid's kinetic energy, sum m |v|^2 / 2 (sphParticles.py:236-242).'''
+        speed2 = np.einsum('ij,ij->i', self.velocities, self.velocities)
+        return float(0.5 * np.dot(self.masses[self.isFluid], speed2[self.isFluid]))
 
     def potentialEnergy(self, gravity: float) -> float:
-        '''sum of m g z over the fluid (sphParticles.py:244-255).'''
-        fluid = self.isFluid
-        return float(np.sum(self.masses[fluid] * gravity * self.positions[fluid, -1]))
+        '''The fluid's potential energy, sum m g z with g the downward gravity (sphParticles.py:244-255).'''
+        return float(gravity * np.dot(self.masses[self.isFluid], self.positions[self.isFluid, -1]))
 
     def maxSpeed(self) -> float:
-        '''Largest fluid speed (sphParticles.py:257-265).'''
-        fluid = self.isFluid
-        if not np.any(fluid):
-            return 0.0
-        return float(np.max(np.linalg.norm(self.velocities[fluid], axis = 1)))
+        '''The fluid's largest speed, 0 with no fluid (sphParticles.py:257-265).'''
+        speeds = np.linalg.norm(self.velocities[self.isFluid], axis = 1)
+        return float(speeds.max()) if speeds.size else 0.0
 
     def maxDensityError(self, referenceDensity: float) -> float:
-        '''Largest relative fluid density departure (sphParticles.py:267-280).'''
-        fluid = self.isFluid
-        if not np.any(fluid):
-            return 0.0
-        return float(np.max(np.abs(self.densities[fluid] - referenceDensity)) / referenceDensity)
+        '''The fluid's largest density departure from rho0, over rho0 (sphParticles.py:267-280).'''
+        departures = np.abs(self.densities[self.isFluid] - referenceDensity)
+        return float(departures.max()) / referenceDensity if departures.size else 0.0
 
 
 def latticePoints(lower, upper, spacing: float) -> np.ndarray:
-    '''Cell centres of a square lattice in [lower, upper], half a spacing in from each face (sphParticles.py:289-312).'''
-    axes = [np.arange(low + spacing / 2.0, high, spacing) for low, high in zip(lower, upper)]
-    grids = np.meshgrid(*axes, indexing = 'xy')
-    return np.column_stack([grid.ravel() for grid in grids])
+    '''The centres of the lattice cells of `spacing` that fill the box [lower, upper], x fastest (sphParticles.py:289-312).'''
+    centres = [np.arange(low + spacing / 2.0, high, spacing) for low, high in zip(lower, upper)]
+    return np.stack([axis.ravel() for axis in np.meshgrid(*centres, indexing = 'xy')], axis = 1)
 
 
 def fillRegion(inside: Callable[[np.ndarray], np.ndarray], lower, upper, spacing: float) -> np.ndarray:
-    '''The lattice points of a box that a predicate accepts (sphParticles.py:425-448).'''
+    '''The lattice points of the box [lower, upper] that `inside` accepts (sphParticles.py:425-448).'''
     points = latticePoints(lower, upper, spacing)
     return points[np.asarray(inside(points), dtype = bool)]
 
 
+def _mirror(values: np.ndarray, low, high) -> np.ndarray:
+    '''Values folded back into [low, high] by one reflection at each end: below low to 2 low - v, above high to 2 high - v.'''
+    folded = np.where(values < low, 2.0 * low - values, values)
+    return np.where(folded > high, 2.0 * high - folded, folded)
+
+
 def jitterPoints(points: np.ndarray, spacing: float, amplitude: float, seed: int, lower, upper,
                  floor: Callable[..., np.ndarray], ceiling: Callable[..., np.ndarray] | None = None) -> np.ndarray:
-    '''
-
-    Points moved by a seeded uniform offset of up to `amplitude` spacings per axis, reflected back
-    across any face they crossed; a point the reflections leave outside keeps its place
-    (sphParticles.py:357-423).
 
     '''
 
-    points = np.asarray(points, dtype = float)
+    The points moved off their lattice by a seeded uniform offset of up to `amplitude` spacings on
+    every axis, and folded back into the water (sphParticles.py:357-423 gives the rules).
+
+    The water a point may occupy: along each horizontal axis between the walls half a spacing in
+    from `lower` and `upper`; vertically from half a spacing above `floor` (a function of the
+    horizontal coordinates, one array each) up to, and not including, `ceiling` (likewise; None for
+    no surface). A moved point is mirrored once at each face it crossed, horizontally first and then
+    vertically at the floor and ceiling under its new horizontal place. Where water is thinner than
+    the offset a point can still end outside; such a point keeps its lattice place. No point is
+    lost, so the water's volume is the lattice's.
+
+    '''
+
+    start = np.asarray(points, dtype = float)
     if amplitude <= 0.0:
-        return points.copy()
-    moved = points + np.random.default_rng(seed).uniform(-amplitude, amplitude, points.shape) * spacing
-    horizontal = points.shape[1] - 1
+        return start.copy()
+    moved = start + np.random.default_rng(seed).uniform(-amplitude, amplitude, start.shape) * spacing
+    across = start.shape[1] - 1
     lower, upper = np.asarray(lower, dtype = float), np.asarray(upper, dtype = float)
-
-    def reflect(values, low, high):
-        values = np.where(values < low, 2.0 * low - values, values)
-        return np.where(values > high, 2.0 * high - values, values)
-
-    for axis in range(horizontal):
-        moved[:, axis] = reflect(moved[:, axis], lower[axis] + 0.5 * spacing, upper[axis] - 0.5 * spacing)
-    bottom = np.asarray(floor(*moved[:, :horizontal].T), dtype = float) + 0.5 * spacing
-    top = (np.asarray(ceiling(*moved[:, :horizontal].T), dtype = float) if ceiling is not None
-           else np.full(len(moved), np.inf))
-    moved[:, -1] = reflect(moved[:, -1], bottom, top - 1e-9 * spacing)
-    walls = np.all((moved[:, :horizontal] >= lower + (0.5 - 1e-9) * spacing)
-                   & (moved[:, :horizontal] <= upper - (0.5 - 1e-9) * spacing), axis = 1)
-    stray = ~walls | (moved[:, -1] < bottom - 1e-9 * spacing) | (moved[:, -1] >= top)
-    moved[stray] = points[stray]
+    for axis in range(across):
+        moved[:, axis] = _mirror(moved[:, axis], lower[axis] + 0.5 * spacing, upper[axis] - 0.5 * spacing)
+    horizontal = moved[:, :across].T
+    ground = np.asarray(floor(*horizontal), dtype = float) + 0.5 * spacing
+    surface = np.full(len(moved), np.inf) if ceiling is None else np.asarray(ceiling(*horizontal), dtype = float)
+    moved[:, -1] = _mirror(moved[:, -1], ground, surface - 1e-9 * spacing)
+    # what the mirrors could not bring back in, within a billionth of a spacing
+    margin = (0.5 - 1e-9) * spacing
+    between = np.all((moved[:, :across] >= lower + margin) & (moved[:, :across] <= upper - margin), axis = 1)
+    outside = ~between | (moved[:, -1] < ground - 1e-9 * spacing) | (moved[:, -1] >= surface)
+    moved[outside] = start[outside]
     return moved
