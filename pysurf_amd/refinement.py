@@ -1,26 +1,27 @@
 '''
 
-Particle splitting and merging for a refined zone, after pySurf's `sphRefinement`
-(src/pysurf/sphRefinement.py:101-311): a split turns each chosen fluid particle into 2^dim daughters
-at the corners of a square (cube) half its spacing across, each with 1/2^dim of its mass and its
-smoothing length times `lengthRatio`; a merge turns groups of 2^dim split particles of one level,
-binned into cells the size of a particle one level up, back into one.
+Splitting and merging particles in a refined zone, on the host (the rules: pySurf's
+src/pysurf/sphRefinement.py:101-311). BatchEngine.split and BatchEngine.merge do the same on the
+device; the host forms build the refined starts and check the device's.
 
-These are the host's forms; BatchEngine.split and BatchEngine.merge do the same on the device.
+A split replaces a fluid particle by 2^dim daughters on the corners of a square (cube) half its
+spacing across, each with 1/2^dim of its mass and its smoothing length times `lengthRatio`. A merge
+turns 2^dim particles of one level that share a cell the size of their mother back into one. Both
+are written over the state's list of per-particle arrays: a split appends rows copied from their
+mothers, a merge rebuilds the rows through one gather that keeps the boundary's indices.
 
 '''
 
 from __future__ import annotations
 
-import itertools
-from dataclasses import dataclass, field
+from dataclasses import dataclass, field, fields
 
 import numpy as np
 
 
 @dataclass
 class MergeResult:
-    '''What a merge did (sphRefinement.py:101-121): the groups merged, the kinetic energy of the members' motion about their groups' centres of mass it removed, and each remaining particle's index before.'''
+    '''What a merge did (sphRefinement.py:101-121): how many groups merged, the kinetic energy of the members' motion about their groups' centres of mass it took away, and for each remaining particle its index before.'''
 
     groups: int = 0
     removedEnergy: float = 0.0
@@ -28,9 +29,29 @@ class MergeResult:
 
 
 def refinementLevels(particles, referenceMass: float) -> np.ndarray:
-    '''How many times each particle has split: log2(referenceMass / m) / dim, rounded (sphRefinement.py:123-133).'''
-    ratio = referenceMass / np.asarray(particles.masses, dtype = float)
-    return np.rint(np.log2(ratio) / particles.dimensions).astype(int)
+    '''How many times each particle has split, the base-2^dim logarithm of referenceMass / m rounded (sphRefinement.py:123-133).'''
+    return np.rint(np.log2(referenceMass / np.asarray(particles.masses, dtype = float)) / particles.dimensions).astype(int)
+
+
+def _daughterCorners(dimensions: int) -> np.ndarray:
+    '''The 2^dim corner offsets in spacings, daughter k's axis a at +1/4 where bit (dim - 1 - a) of k is set and -1/4 where it is clear.'''
+    bits = (np.arange(2**dimensions)[:, None] >> np.arange(dimensions - 1, -1, -1)[None, :]) & 1
+    return (bits - 0.5) * 0.5
+
+
+def _lengthsOf(particles, referenceLength: float) -> np.ndarray:
+    '''Every particle's smoothing length as a fresh array.'''
+    if particles.lengths is None:
+        return np.full(particles.nParticles, float(referenceLength))
+    return particles.lengths.astype(float, copy = True)
+
+
+def _rows(particles):
+    '''(name, array) of the state's per-particle arrays that are set.'''
+    for entry in fields(particles):
+        values = getattr(particles, entry.name)
+        if values is not None:
+            yield entry.name, values
 
 
 def splitParticles(particles, mask: np.ndarray, referenceDensity: float, referenceLength: float,
@@ -38,80 +59,87 @@ def splitParticles(particles, mask: np.ndarray, referenceDensity: float, referen
 
     '''
 
-    Split the fluid particles `mask` selects, in place; the first daughter takes its mother's slot
-    and the others are appended, mother by mother. Returns how many particles split.
-
-    Parameters:
-    -----------
-    particles : ParticleSystem
-        The state; its arrays are replaced by longer ones
-    mask : np.ndarray
-        Particles to split, shape (N,); boundary particles in it stay whole
-    referenceDensity : float
-        rho0: a particle's spacing is (m / rho0)^(1/dim)
-    referenceLength : float
-        The smoothing length of a particle that has not split
-    lengthRatio : float
-        A daughter's smoothing length over its mother's
+    Split the fluid particles `mask` selects, in place (sphRefinement.py:135-186). Each mother's
+    slot takes its first daughter (the corner at -1/4 on every axis); the other daughters are
+    appended mother by mother, in corner order, carrying their mother's velocity, acceleration,
+    density and pressure. A daughter's spacing is half its mother's, (m / rho0)^(1/dim) / 2, with
+    rho0 `referenceDensity`; `referenceLength` is the length of a particle that never split.
+    Boundary particles in `mask` stay whole. Returns how many particles split.
 
     '''
 
-    chosen = np.flatnonzero(np.asarray(mask, dtype = bool) & particles.isFluid)
-    if chosen.size == 0:
+    mothers = np.flatnonzero(np.asarray(mask, dtype = bool) & particles.isFluid)
+    if mothers.size == 0:
         return 0
     dimensions = particles.dimensions
-    count = 2**dimensions
-    lengths = (particles.lengths.astype(float, copy = True) if particles.lengths is not None
-               else np.full(particles.nParticles, float(referenceLength)))
-    corners = np.array(list(itertools.product((-0.25, 0.25), repeat = dimensions)))
-    spacing = (particles.masses[chosen] / referenceDensity)**(1.0 / dimensions)
-    daughters = particles.positions[chosen][:, None, :] + spacing[:, None, None] * corners[None]
-    mass = particles.masses[chosen] / count
-    length = lengths[chosen] * lengthRatio
-    particles.positions[chosen] = daughters[:, 0]
-    particles.masses[chosen] = mass
-    lengths[chosen] = length
-
-    def repeat(values):
-        return np.repeat(values, count - 1, axis = 0)
-
-    particles.positions = np.vstack([particles.positions, daughters[:, 1:].reshape(-1, dimensions)])
-    particles.velocities = np.vstack([particles.velocities, repeat(particles.velocities[chosen])])
-    particles.accelerations = np.vstack([particles.accelerations, repeat(particles.accelerations[chosen])])
-    particles.densities = np.concatenate([particles.densities, repeat(particles.densities[chosen])])
-    particles.pressures = np.concatenate([particles.pressures, repeat(particles.pressures[chosen])])
-    particles.masses = np.concatenate([particles.masses, repeat(mass)])
-    particles.isFluid = np.concatenate([particles.isFluid, np.ones(chosen.size * (count - 1), dtype = bool)])
-    particles.lengths = np.concatenate([lengths, repeat(length)])
+    corners = _daughterCorners(dimensions)
+    spacing = (particles.masses[mothers] / referenceDensity)**(1.0 / dimensions)
+    places = particles.positions[mothers][:, None, :] + spacing[:, None, None] * corners[None]
+    particles.lengths = _lengthsOf(particles, referenceLength)
+    particles.lengths[mothers] = particles.lengths[mothers] * lengthRatio
+    particles.masses[mothers] = particles.masses[mothers] / len(corners)
+    particles.positions[mothers] = places[:, 0]
+    # the appended daughters are copies of their (already divided) mothers but for their places
+    copies = np.repeat(mothers, len(corners) - 1)
+    for name, values in list(_rows(particles)):
+        setattr(particles, name, np.concatenate([values, values[copies]]))
+    particles.positions[-copies.size:] = places[:, 1:].reshape(-1, dimensions)
     if np.all(particles.lengths == referenceLength):
         particles.lengths = None
-    return int(chosen.size)
+    return int(mothers.size)
 
 
-def cellGroups(positions: np.ndarray, cell: float, offset: float, size: int) -> np.ndarray:
+def _groupsInCells(positions: np.ndarray, cell: float, offset: float, size: int) -> np.ndarray:
 
     '''
 
-    Groups of `size` points binned into cells `cell` across, the grid shifted by `offset` cells: in
-    each cell that holds at least `size`, the `size` nearest its centre, ties by index
-    (sphRefinement.py:188-210). Returns the points' indices, (groups, size), cells in row-major
-    order of their integer coordinates.
+    Groups of `size` points sharing a cell of a grid `cell` across, shifted by `offset` cells
+    (sphRefinement.py:188-210): every cell holding `size` or more gives one group, its `size`
+    points nearest the cell's centre, equal distances by index. Returns the points' indices, shape
+    (groups, size), the cells in row-major order of their integer coordinates.
 
     '''
 
     scaled = positions / cell + offset
-    keys = np.floor(scaled).astype(np.int64)
-    keys -= keys.min(axis = 0)
-    flat = np.ravel_multi_index(keys.T, keys.max(axis = 0) + 1)
-    distance = np.sum((scaled - np.floor(scaled) - 0.5)**2, axis = 1)
-    order = np.lexsort((distance, flat))
-    cells = flat[order]
-    starts = np.flatnonzero(np.r_[True, cells[1:] != cells[:-1]])
-    counts = np.diff(np.r_[starts, len(order)])
-    owner = np.repeat(np.arange(len(starts)), counts)
-    rank = np.arange(len(order)) - starts[owner]
-    taken = (rank < size) & (counts[owner] >= size)
-    return order[taken].reshape(-1, size)
+    corner = np.floor(scaled)
+    inCell = scaled - corner - 0.5
+    distance = np.sum(inCell**2, axis = 1)
+    cells, owner, population = np.unique(corner.astype(np.int64), axis = 0, return_inverse = True,
+                                         return_counts = True)
+    owner = owner.reshape(-1)
+    order = np.lexsort((distance, owner))
+    ranked = owner[order]
+    rank = np.arange(order.size) - np.searchsorted(ranked, ranked, side = 'left')
+    chosen = (rank < size) & (population[ranked] >= size)
+    return order[chosen].reshape(-1, size)
+
+
+def _mergeGroups(particles, eligible: np.ndarray, levels: np.ndarray, referenceDensity: float,
+                 referenceMass: float, offsets) -> np.ndarray | None:
+
+    '''
+
+    The groups a merge takes, shape (groups, 2^dim), each row ascending; None when there are none.
+    Level by level (coarsest first), the level's free particles are grouped in cells the size of a
+    particle one level up, once per grid shift in `offsets`; a particle grouped by one shift is
+    no longer free for the next.
+
+    '''
+
+    dimensions = particles.dimensions
+    size = 2**dimensions
+    found = []
+    for level in np.unique(levels[eligible]):
+        cell = (referenceMass / size**(level - 1) / referenceDensity)**(1.0 / dimensions)
+        free = np.flatnonzero(eligible & (levels == level))
+        for offset in offsets:
+            if free.size < size:
+                break
+            groups = free[_groupsInCells(particles.positions[free], cell, offset, size)]
+            if groups.size:
+                found.append(groups)
+                free = np.setdiff1d(free, groups.ravel(), assume_unique = True)
+    return np.sort(np.concatenate(found), axis = 1) if found else None
 
 
 def mergeParticles(particles, mask: np.ndarray, referenceDensity: float, referenceMass: float, referenceLength: float,
@@ -120,79 +148,64 @@ def mergeParticles(particles, mask: np.ndarray, referenceDensity: float, referen
     '''
 
     Merge the split fluid particles `mask` selects, in place, 2^dim of one level at a time into one
-    particle of the level above (sphRefinement.py:212-311). Level by level, a pass per grid shift in
-    `offsets` groups the level's free particles by cellGroups in cells the size of a particle one
-    level up. A group becomes one particle in its first member's slot: the mass-weighted mean
-    position, velocity, acceleration and pressure, the total mass, the density of the total mass
-    over the members' volumes, and the members' mean length over `lengthRatio`. The other members
-    are removed; fluid particles from the end fill their places among the boundary's, so the
-    boundary keeps its indices.
+    particle of the level above (sphRefinement.py:212-311). The merged particle takes its group's
+    lowest slot: the total mass; the mass-weighted mean position, velocity, acceleration and
+    pressure; the density of the total mass over the members' volumes sum m / rho; the members'
+    mean smoothing length over `lengthRatio`. The members' kinetic energy about the group's mean
+    velocity is what the merge takes away. The other members' slots are closed (compactionSource).
 
     '''
 
-    dimensions = particles.dimensions
-    size = 2**dimensions
     levels = refinementLevels(particles, referenceMass)
     eligible = np.asarray(mask, dtype = bool) & particles.isFluid & (levels > 0)
-    groups = []
-    for level in np.unique(levels[eligible]):
-        cell = (referenceMass / size**(level - 1) / referenceDensity)**(1.0 / dimensions)
-        free = np.flatnonzero(eligible & (levels == level))
-        for offset in offsets:
-            if free.size < size:
-                break
-            members = free[cellGroups(particles.positions[free], cell, offset, size)]
-            if members.size == 0:
-                continue
-            groups.append(members)
-            free = np.setdiff1d(free, members.ravel(), assume_unique = True)
-    if not groups:
+    groups = _mergeGroups(particles, eligible, levels, referenceDensity, referenceMass, offsets)
+    if groups is None:
         return MergeResult(source = np.arange(particles.nParticles))
-    members = np.sort(np.vstack(groups), axis = 1)
-    lengths = (particles.lengths.astype(float, copy = True) if particles.lengths is not None
-               else np.full(particles.nParticles, float(referenceLength)))
-    m = particles.masses[members]
-    total = m.sum(axis = 1)
-    weights = m / total[:, None]
-    velocity = np.einsum('gk,gkd->gd', weights, particles.velocities[members])
-    relative = particles.velocities[members] - velocity[:, None, :]
-    removed = 0.5 * float(np.sum(m * np.sum(relative**2, axis = 2)))
-    first = members[:, 0]
-    particles.positions[first] = np.einsum('gk,gkd->gd', weights, particles.positions[members])
-    particles.accelerations[first] = np.einsum('gk,gkd->gd', weights, particles.accelerations[members])
-    particles.pressures[first] = np.sum(weights * particles.pressures[members], axis = 1)
-    particles.densities[first] = total / np.sum(m / particles.densities[members], axis = 1)
-    particles.velocities[first] = velocity
-    particles.masses[first] = total
-    lengths[first] = lengths[members].mean(axis = 1) / lengthRatio
+    masses = particles.masses[groups]
+    total = masses.sum(axis = 1)
+    weights = masses / total[:, None]
+
+    def weighted(values):
+        return np.einsum('gk,gkd->gd', weights, values[groups])
+
+    mean = weighted(particles.velocities)
+    spread = particles.velocities[groups] - mean[:, None, :]
+    removed = 0.5 * float(np.sum(masses * np.sum(spread**2, axis = 2)))
+    kept = groups[:, 0]
+    lengths = _lengthsOf(particles, referenceLength)
+    lengths[kept] = lengths[groups].mean(axis = 1) / lengthRatio
     particles.lengths = lengths
-    source = compactionSource(particles.nParticles, np.sort(members[:, 1:].ravel()), particles.isFluid)
-    for name in ('positions', 'velocities', 'accelerations', 'densities', 'pressures', 'masses', 'isFluid', 'lengths'):
-        setattr(particles, name, getattr(particles, name)[source])
-    return MergeResult(groups = len(members), removedEnergy = removed, source = source)
+    particles.positions[kept] = weighted(particles.positions)
+    particles.accelerations[kept] = weighted(particles.accelerations)
+    particles.pressures[kept] = np.sum(weights * particles.pressures[groups], axis = 1)
+    particles.densities[kept] = total / np.sum(masses / particles.densities[groups], axis = 1)
+    particles.velocities[kept] = mean
+    particles.masses[kept] = total
+    source = compactionSource(particles.nParticles, np.sort(groups[:, 1:].ravel()), particles.isFluid)
+    particles.permute(source)
+    return MergeResult(groups = len(groups), removedEnergy = removed, source = source)
 
 
 def compactionSource(count: int, removed: np.ndarray, isFluid: np.ndarray) -> np.ndarray:
 
     '''
 
-    Each remaining particle's index before `removed` (sorted) leave (sphRefinement.py:291-306):
-    the holes at or before the last boundary particle take, in order, the last as many remaining
-    particles after it; everything else keeps its order. Raises RuntimeError when too few follow.
+    For each particle left after the sorted slots `removed` close, its slot before
+    (sphRefinement.py:291-306). Up to the last boundary particle every slot stays, a closed one
+    taking, in order, the last as many remaining particles after it; after the last boundary
+    particle the rest follow in order. Raises RuntimeError when too few remain after it.
 
     '''
 
-    alive = np.ones(count, dtype = bool)
-    alive[removed] = False
-    boundary = np.flatnonzero(~np.asarray(isFluid, dtype = bool))
-    last = int(boundary[-1]) if boundary.size else -1
+    kinds = np.asarray(isFluid, dtype = bool)
+    walls = np.flatnonzero(~kinds)
+    last = int(walls[-1]) if walls.size else -1
+    open_ = np.ones(count, dtype = bool)
+    open_[removed] = False
     holes = removed[removed <= last]
-    tail = np.flatnonzero(alive[last + 1:]) + last + 1
-    if tail.size < holes.size:
+    after = np.flatnonzero(open_[last + 1:]) + last + 1
+    if after.size < holes.size:
         raise RuntimeError('Too few fluid particles follow the boundary to fill the places a merge leaves among it.')
-    movers = tail[tail.size - holes.size:]
-    source = np.arange(count)
-    source[holes] = movers
-    alive[holes] = True
-    alive[movers] = False
-    return source[alive]
+    head = np.arange(last + 1)
+    head[holes] = after[after.size - holes.size:]
+    return np.concatenate([head, after[:after.size - holes.size]])
