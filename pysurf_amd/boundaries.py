@@ -1,8 +1,12 @@
 '''
 
-Solid boundaries as fixed particles, after pySurf's `sphBoundaries` (src/pysurf/sphBoundaries.py):
-the geometry of the particle layers the pair sums see, and the description of the clamp the device
-step applies to the fluid (the kernels enforce it; see `pair_sums.boundaryOf`).
+Solid boundaries as layers of fixed particles (the geometry of pySurf's `sphBoundaries`,
+src/pysurf/sphBoundaries.py), and the description of the clamp that keeps the fluid on the water
+side, which the device step applies (`pair_sums.boundaryOf` packs it).
+
+Every solid here is built the same way: lines or surfaces of the fluid lattice carried into the
+solid by `nLayers` layer depths of (k + 1/2) spacings, so the gap between the fluid's lattice and
+the first layer is one spacing on every face.
 
 '''
 
@@ -13,13 +17,29 @@ import numpy as np
 from pysurf_amd.bathymetry import Bathymetry, HeightField
 
 
+def layerDepths(spacing: float, layers: int) -> np.ndarray:
+    '''How far each layer lies inside its solid: (k + 1/2) spacings for k = 0 .. layers - 1.'''
+    return (np.arange(layers) + 0.5) * spacing
+
+
+def _lattice(*axes) -> np.ndarray:
+    '''Every combination of the axes' values, one point per row, the first axis fastest.'''
+    return np.column_stack([grid.ravel() for grid in np.meshgrid(*axes, indexing = 'xy')])
+
+
+def _extrude(profile: np.ndarray, across: np.ndarray) -> np.ndarray:
+    '''A 2D profile's (x, z) points repeated at every `across` position, as (x, y, z) rows point by point.'''
+    return np.column_stack([np.repeat(profile[:, 0], across.size), np.tile(across, len(profile)),
+                            np.repeat(profile[:, 1], across.size)])
+
+
 class ContainerBoundary:
 
-    '''An axis-aligned container, open at the top unless asked otherwise (sphBoundaries.py:61-147).'''
+    '''An axis-aligned box, its faces `nLayers` deep, open at the top unless `openTop` is False (sphBoundaries.py:61-147).'''
 
     def __init__(self, containerMin, containerMax, spacing: float, nLayers: int = 3, openTop: bool = True) -> None:
-        self.containerMin = np.asarray(containerMin, dtype = float).copy()
-        self.containerMax = np.asarray(containerMax, dtype = float).copy()
+        self.containerMin = np.array(containerMin, dtype = float)
+        self.containerMax = np.array(containerMax, dtype = float)
         self.spacing = spacing
         self.nLayers = nLayers
         self.openTop = openTop
@@ -29,32 +49,27 @@ class ContainerBoundary:
         return len(self.containerMin)
 
     def positions(self) -> np.ndarray:
-        '''The shell nLayers deep around the interior, less the top face when open (sphBoundaries.py:95-125).'''
-        s, layers = self.spacing, self.nLayers
-        offsets = (np.arange(layers) + 0.5) * s
+        '''The shell around the box: the lattice extended by the layers past each face, the box's inside taken out (sphBoundaries.py:95-125).'''
+        s, depths = self.spacing, layerDepths(self.spacing, self.nLayers)
+        top = self.dimensions - 1
         axes = []
         for axis, (low, high) in enumerate(zip(self.containerMin, self.containerMax)):
-            pieces = [low - offsets[::-1], np.arange(low + s / 2.0, high, s)]
-            if not (self.openTop and axis == self.dimensions - 1):
-                pieces.append(high + offsets)
-            axes.append(np.concatenate(pieces))
-        grids = np.meshgrid(*axes, indexing = 'xy')
-        points = np.column_stack([grid.ravel() for grid in grids])
-        interior = np.all((points > self.containerMin) & (points < self.containerMax), axis = 1)
+            above = [] if self.openTop and axis == top else [high + depths]
+            axes.append(np.concatenate([low - depths[::-1], np.arange(low + s / 2.0, high, s)] + above))
+        points = _lattice(*axes)
+        within = (points > self.containerMin) & (points < self.containerMax)
         if self.openTop:
-            between = np.all((points[:, :-1] > self.containerMin[:-1]) & (points[:, :-1] < self.containerMax[:-1]),
-                             axis = 1)
-            interior = between & (points[:, -1] > self.containerMin[-1])
-        return points[~interior]
+            within[:, top] = points[:, top] > self.containerMin[top]
+        return points[~within.all(axis = 1)]
 
 
 class BedBoundary:
 
     '''
 
-    A bed of any profile with optional offshore and shoreward walls; 2D, or extruded across a
-    channel in 3D (sphBoundaries.py:149-344). Bed particles sit in layers offset along the bed
-    normal by (layer + 1/2) spacing, one spacing apart along it.
+    A bed of any 2D profile with an optional offshore wall (up to `wallTop`) and shoreward wall (up
+    to `endWallTop`); with a `width`, extruded across a channel in 3D, closed by side walls or
+    `periodic` across it (sphBoundaries.py:149-344). The bed's layers follow its normal.
 
     '''
 
@@ -67,71 +82,73 @@ class BedBoundary:
         self.width = width
         self.endWallTop = endWallTop
         self.periodic = periodic and width is not None
+        # where the offshore wall stands; a piston moves it
         self.wallX = bathymetry.startX
         self._pistonMask: np.ndarray | None = None
 
-    def _profilePositions(self) -> np.ndarray:
-        '''The 2D bed and wall layers in (x, z) (sphBoundaries.py:201-226).'''
-        s = self.spacing
-        bedX, bedZ = self.bathymetry.sample(s)
-        bedNormal = self.bathymetry.normal(bedX)
-        layers = [np.column_stack([bedX, bedZ]) - (layer + 0.5) * s * bedNormal for layer in range(self.nLayers)]
-        for wallX, top, side in ((self.bathymetry.startX, self.wallTop, -1.0),
-                                 (self.bathymetry.endX, self.endWallTop, 1.0)):
+    def _profileBlocks(self):
+        '''The 2D (x, z) blocks: each bed layer along the normal, then each end wall's columns layer by layer (sphBoundaries.py:201-226).'''
+        s, bed = self.spacing, self.bathymetry
+        stations = np.column_stack(bed.sample(s))
+        normals = bed.normal(stations[:, 0])
+        for layer in range(self.nLayers):
+            yield stations - (layer + 0.5) * s * normals
+        for x, top, outward in ((bed.startX, self.wallTop, -1.0), (bed.endX, self.endWallTop, 1.0)):
             if top is None:
                 continue
-            bottom = float(self.bathymetry.elevation(wallX)) - self.nLayers * s
-            heights = np.arange(bottom + s / 2.0, top, s)
+            # from below the deepest bed layer, so the corner is closed
+            heights = np.arange(float(bed.elevation(x)) - self.nLayers * s + s / 2.0, top, s)
             for layer in range(self.nLayers):
-                layers.append(np.column_stack([np.full_like(heights, wallX + side * (layer + 0.5) * s), heights]))
-        return np.vstack(layers)
+                yield np.column_stack([np.full_like(heights, x + outward * (layer + 0.5) * s), heights])
+
+    def _sideWalls(self, depths: np.ndarray) -> list:
+        '''The channel's side walls: at every bed station, the lattice's heights from below the bed's layers up to the wall's top, behind each side.'''
+        s, bed, layers = self.spacing, self.bathymetry, self.nLayers
+        top = self.wallTop if self.wallTop is not None else float(np.max(bed.z)) + layers * s
+        stations, _ = bed.sample(s)
+        heights = np.arange(float(np.min(bed.z)) - layers * s + s / 2.0, top, s)
+        x, z = np.meshgrid(stations, heights, indexing = 'xy')
+        solid = z > bed.elevation(x) - layers * s
+        x, z = x[solid], z[solid]
+        return [np.column_stack([x, np.full_like(x, y), z]) for y in np.concatenate([-depths, self.width + depths])]
 
     def positions(self) -> np.ndarray:
         '''Boundary particle positions, (N, 2), or (N, 3) across a channel (sphBoundaries.py:228-258).'''
-        profile = self._profilePositions()
+        profile = np.vstack(list(self._profileBlocks()))
         if self.width is None:
             return profile
-        s, layers = self.spacing, self.nLayers
-        offsets = (np.arange(layers) + 0.5) * s
+        s, depths = self.spacing, layerDepths(self.spacing, self.nLayers)
+        inside = np.arange(s / 2.0, self.width, s)
         if self.periodic:
-            across = np.arange(s / 2.0, self.width, s)
-            return np.column_stack([np.repeat(profile[:, 0], across.size), np.tile(across, len(profile)),
-                                    np.repeat(profile[:, 1], across.size)])
-        across = np.concatenate([-offsets[::-1], np.arange(s / 2.0, self.width, s), self.width + offsets])
-        extruded = np.column_stack([np.repeat(profile[:, 0], across.size), np.tile(across, len(profile)),
-                                    np.repeat(profile[:, 1], across.size)])
-        top = self.wallTop if self.wallTop is not None else float(np.max(self.bathymetry.z)) + layers * s
-        bedX, _ = self.bathymetry.sample(s)
-        heights = np.arange(float(np.min(self.bathymetry.z)) - layers * s + s / 2.0, top, s)
-        xGrid, zGrid = np.meshgrid(bedX, heights, indexing = 'xy')
-        keep = zGrid > self.bathymetry.elevation(xGrid) - layers * s
-        xWall, zWall = xGrid[keep], zGrid[keep]
-        sides = [np.column_stack([xWall, np.full_like(xWall, y), zWall])
-                 for y in np.concatenate([-offsets, self.width + offsets])]
-        return np.vstack([extruded] + sides)
+            return _extrude(profile, inside)
+        # under the side walls too, then the walls themselves
+        across = np.concatenate([-depths[::-1], inside, self.width + depths])
+        return np.vstack([_extrude(profile, across)] + self._sideWalls(depths))
 
     def attachPiston(self, particles) -> None:
-        '''The offshore wall's particles as a piston (sphBoundaries.py:260-275).'''
+        '''Take the boundary particles behind the offshore wall's face as a piston (sphBoundaries.py:260-275).'''
         if self.wallTop is None:
             raise ValueError('A piston needs the offshore wall; give the boundary a wallTop.')
-        self._pistonMask = (~particles.isFluid) & (particles.positions[:, 0] < self.bathymetry.startX - 0.25 * self.spacing)
+        behind = particles.positions[:, 0] < self.bathymetry.startX - 0.25 * self.spacing
+        self._pistonMask = behind & ~particles.isFluid
 
     def movePiston(self, particles, x: float, velocity: float) -> None:
-        '''Move the offshore wall to x at `velocity` (sphBoundaries.py:277-293).'''
+        '''Carry the piston so the wall stands at x, moving along x at `velocity` (sphBoundaries.py:277-293).'''
         if self._pistonMask is None:
             raise RuntimeError('No piston attached. Call attachPiston first.')
-        particles.positions[self._pistonMask, 0] += x - self.wallX
-        particles.velocities[self._pistonMask] = 0.0
-        particles.velocities[self._pistonMask, 0] = velocity
+        piston = self._pistonMask
+        particles.positions[piston, 0] += x - self.wallX
+        particles.velocities[piston] = 0.0
+        particles.velocities[piston, 0] = velocity
         self.wallX = x
 
     def description(self, pistonSpeed: float = 0.0) -> dict:
-        '''The clamp as the device applies it (sphGpu.py:2141-2158), as a plain dict.'''
+        '''The clamp the device applies (sphGpu.py:2141-2158): a quarter spacing clear of the bed and of each wall.'''
         bed, clearance = self.bathymetry, 0.25 * self.spacing
         return {'kind': 'bed', 'clearance': clearance, 'bedX': bed.x.copy(), 'bedZ': bed.z.copy(),
                 'sideWalls': self.width is not None and not self.periodic, 'width': float(self.width or 0.0),
                 'offshoreWall': self.wallTop is not None, 'wallX': float(self.wallX), 'pistonSpeed': pistonSpeed,
-                'endLimit': bed.endX - clearance if self.endWallTop is not None else bed.endX}
+                'endLimit': bed.endX if self.endWallTop is None else bed.endX - clearance}
 
 
 class HeightFieldBoundary:
@@ -139,11 +156,11 @@ class HeightFieldBoundary:
     '''
 
     A height-field bed in a rectangular tank whose walls stand on the field's bounds: offshore at
-    x = startX, shoreward at x = endX and at y = startY and y = endY (sphBoundaries.py:498-640). Bed
-    particles sit on the lattice's columns, (i + 1/2) s from the walls, in layers below the bed along
-    its local normal at (layer + 1/2) s; each wall's layers stand (layer + 1/2) s behind its plane on
-    the fluid lattice's heights, from below the bed's deepest layer up to `wallTop`, the side walls
-    running past the end walls to close the corners.
+    x = startX and shoreward at x = endX (each optional), across at y = startY and y = endY
+    (sphBoundaries.py:498-640). The bed's particles sit under the lattice's columns, (i + 1/2)
+    spacings in from the walls, their layers along the bed's local normal; each wall's layers
+    stand behind its plane at the fluid lattice's heights, from under the bed's layers up to
+    `wallTop`, the side walls running past the end walls to close the corners.
 
     '''
 
@@ -162,87 +179,87 @@ class HeightFieldBoundary:
 
     @property
     def latticeFloor(self) -> float:
-        '''The level the fluid lattice's heights count from: the bed's lowest point (sphBoundaries.py:545-548).'''
+        '''The level the fluid lattice's heights count from, the bed's lowest grid point (sphBoundaries.py:545-548).'''
         return self.field.lowestElevation()
 
     def columns(self) -> tuple[np.ndarray, np.ndarray]:
-        '''The lattice's column positions along x and y inside the walls (sphBoundaries.py:550-556).'''
+        '''The lattice's column positions along x and along y, inside the walls (sphBoundaries.py:550-556).'''
         s = self.spacing
         startX, endX, startY, endY = self.field.bounds
         return np.arange(startX + 0.5 * s, endX, s), np.arange(startY + 0.5 * s, endY, s)
+
+    def _wallLines(self, xs: np.ndarray, ys: np.ndarray):
+        '''The (x, y) lines the walls' particles stand on: layer by layer the end walls across the tank, then the side walls along it, past the ends.'''
+        startX, endX, startY, endY = self.field.bounds
+        depths = layerDepths(self.spacing, self.nLayers)
+        for depth in depths:
+            if self.offshoreWall:
+                yield np.full(ys.size, startX - depth), ys
+            if self.endWall:
+                yield np.full(ys.size, endX + depth), ys
+        along = np.concatenate([startX - depths[::-1], xs, endX + depths])
+        for depth in depths:
+            yield along, np.full(along.size, startY - depth)
+            yield along, np.full(along.size, endY + depth)
 
     def positions(self) -> np.ndarray:
         '''Boundary particle positions, (N, 3): the bed's layers, then the walls (sphBoundaries.py:558-595).'''
         s, layers, field = self.spacing, self.nLayers, self.field
         startX, endX, startY, endY = field.bounds
         xs, ys = self.columns()
-        gridX, gridY = np.meshgrid(xs, ys, indexing = 'xy')
-        gridX, gridY = gridX.ravel(), gridY.ravel()
-        surface = np.column_stack([gridX, gridY, field.elevation(gridX, gridY)])
-        normal = field.normal(gridX, gridY)
-        blocks = [surface - (layer + 0.5) * s * normal for layer in range(layers)]
+        plan = _lattice(xs, ys)
+        surface = np.column_stack([plan, field.elevation(plan[:, 0], plan[:, 1])])
+        normals = field.normal(plan[:, 0], plan[:, 1])
+        blocks = [surface - (layer + 0.5) * s * normals for layer in range(layers)]
         heights = np.arange(self.latticeFloor - (layers + 1) * s + 0.5 * s, self.wallTop, s)
-
-        def wall(lineX, lineY):
-            bottom = field.elevation(np.clip(lineX, startX, endX), np.clip(lineY, startY, endY)) - layers * s
-            x, y = np.repeat(lineX, heights.size), np.repeat(lineY, heights.size)
-            z = np.tile(heights, lineX.size)
-            keep = z > np.repeat(bottom, heights.size)
-            return np.column_stack([x[keep], y[keep], z[keep]])
-
-        offsets = (np.arange(layers) + 0.5) * s
-        for offset in offsets:
-            if self.offshoreWall:
-                blocks.append(wall(np.full(ys.size, startX - offset), ys))
-            if self.endWall:
-                blocks.append(wall(np.full(ys.size, endX + offset), ys))
-        alongAll = np.concatenate([startX - offsets[::-1], xs, endX + offsets])
-        for offset in offsets:
-            blocks.append(wall(alongAll, np.full(alongAll.size, startY - offset)))
-            blocks.append(wall(alongAll, np.full(alongAll.size, endY + offset)))
+        for lineX, lineY in self._wallLines(xs, ys):
+            # the solid under a wall line starts below the bed's layers where the line meets the tank
+            floor = field.elevation(np.clip(lineX, startX, endX), np.clip(lineY, startY, endY)) - layers * s
+            column = np.column_stack([np.repeat(lineX, heights.size), np.repeat(lineY, heights.size),
+                                      np.tile(heights, lineX.size)])
+            blocks.append(column[column[:, 2] > np.repeat(floor, heights.size)])
         return np.vstack(blocks)
+
+    def _wallClamps(self):
+        '''(axis, limit, side) of each wall's clamp in the order applied: side +1 keeps fluid at or above the limit, -1 at or below.'''
+        clearance = 0.25 * self.spacing
+        startX, endX, startY, endY = self.field.bounds
+        yield 1, startY + clearance, 1.0
+        yield 1, endY - clearance, -1.0
+        if self.offshoreWall:
+            yield 0, startX + clearance, 1.0
+        yield 0, endX - clearance if self.endWall else endX, -1.0
 
     def enforce(self, particles) -> None:
 
         '''
 
-        Return fluid that crossed the bed or a wall a quarter spacing clear of it, removing its
-        velocity into the solid: below the bed a particle rises vertically and loses its velocity
-        into the bed along the bed's normal there (sphBoundaries.py:597-640).
+        Bring fluid that crossed the bed or a wall back a quarter spacing clear of it and take
+        away its velocity into the solid (sphBoundaries.py:597-640): under the bed a particle rises
+        vertically and loses the part of its velocity into the bed along the normal there; past a
+        wall it returns across that wall and keeps only velocity away from it.
 
         '''
 
-        fluid = np.nonzero(particles.isFluid)[0]
-        positions, velocities = particles.positions, particles.velocities
+        fluid = np.flatnonzero(particles.isFluid)
+        x, v = particles.positions, particles.velocities
         clearance = 0.25 * self.spacing
-        field = self.field
-        startX, endX, startY, endY = field.bounds
-        bed = field.elevation(positions[fluid, 0], positions[fluid, 1])
-        below = positions[fluid, 2] < bed + clearance
-        under = fluid[below]
-        if under.size:
-            positions[under, 2] = bed[below] + clearance
-            normal = field.normal(positions[under, 0], positions[under, 1])
-            inward = np.sum(velocities[under] * normal, axis = 1)
-            into = inward < 0.0
-            velocities[under[into]] -= inward[into, None] * normal[into]
-        low = fluid[positions[fluid, 1] < startY + clearance]
-        positions[low, 1] = startY + clearance
-        velocities[low, 1] = np.maximum(velocities[low, 1], 0.0)
-        high = fluid[positions[fluid, 1] > endY - clearance]
-        positions[high, 1] = endY - clearance
-        velocities[high, 1] = np.minimum(velocities[high, 1], 0.0)
-        if self.offshoreWall:
-            behind = fluid[positions[fluid, 0] < startX + clearance]
-            positions[behind, 0] = startX + clearance
-            velocities[behind, 0] = np.maximum(velocities[behind, 0], 0.0)
-        limit = endX - clearance if self.endWall else endX
-        beyond = fluid[positions[fluid, 0] > limit]
-        positions[beyond, 0] = limit
-        velocities[beyond, 0] = np.minimum(velocities[beyond, 0], 0.0)
+        ground = self.field.elevation(x[fluid, 0], x[fluid, 1]) + clearance
+        under = x[fluid, 2] < ground
+        sunk = fluid[under]
+        if sunk.size:
+            x[sunk, 2] = ground[under]
+            normals = self.field.normal(x[sunk, 0], x[sunk, 1])
+            into = np.sum(v[sunk] * normals, axis = 1)
+            entering = into < 0.0
+            v[sunk[entering]] -= into[entering, None] * normals[entering]
+        for axis, limit, side in self._wallClamps():
+            crossed = fluid[side * (x[fluid, axis] - limit) < 0.0]
+            x[crossed, axis] = limit
+            v[crossed, axis] = np.maximum(v[crossed, axis], 0.0) if side > 0 else np.minimum(v[crossed, axis], 0.0)
 
     def description(self) -> dict:
-        '''The clamp as the device applies it (sphGpu.py:2159-2171), as a plain dict.'''
+        '''The clamp the device applies (sphGpu.py:2159-2171): the bed's table and the walls' limits.'''
         field, clearance = self.field, 0.25 * self.spacing
         startX, endX, startY, endY = field.bounds
         return {'kind': 'heightField', 'clearance': clearance, 'table': field.z.copy(), 'originX': float(field.x[0]),
