@@ -184,7 +184,7 @@ def testAVoidInTheWaterIsNotATouchdown():
     tracker.update(2.0, voided)
     assert tracker.touchdownSurface is None
     window = particleSurface(voided, s, np.array([-0.5, 0.9]), np.array([2.0, 2.0]))
-    assert len(window.pockets) == 1 and airShare(window.pockets[0], tracker._lastSurface, s) == 0.0
+    assert len(window.pockets) == 1 and airShare(window.pockets[0], tracker.lastSurface, s) == 0.0
     tracker.update(3.0, np.vstack([voided, tip]))
     assert tracker.touchdownTime == pytest.approx(2.5)
     assert 1.0 < tracker.touchdownPocket[:, 0].mean() < 1.25
