@@ -172,13 +172,11 @@ __device__ __forceinline__ int wrapIndex(int64_t a, int32_t n) {
 // a select instead of a division.
 __device__ __forceinline__ int wrapNear(int c, int32_t n) { return c < 0 ? c + n : (c >= n ? c - n : c); }
 
-// The distinct cell offsets along an axis of n cells; fewer than three cells would repeat one.
-__device__ __forceinline__ int stencilOffsets(int32_t n, int* out) {
-    if (n >= 3) { out[0] = -1; out[1] = 0; out[2] = 1; return 3; }
-    if (n == 2) { out[0] = 0; out[1] = 1; return 2; }
-    out[0] = 0;
-    return 1;
-}
+// The distinct cell offsets along an axis of n cells, b = 0 .. stencilCount(n) - 1: -1, 0, 1, or
+// 0, 1 for two cells and 0 for one (fewer than three cells would repeat one). Computed rather than
+// kept in an array, which the compiler would place in scratch memory.
+__device__ __forceinline__ int stencilCount(int32_t n) { return n >= 3 ? 3 : n; }
+__device__ __forceinline__ int stencilOffset(int32_t n, int b) { return n >= 3 ? b - 1 : b; }
 
 // floor(v / cell) as an integer cell coordinate; a non-finite coordinate reads as cell 0.
 __device__ __forceinline__ int64_t cellCoordinate(double v, double cell) {
@@ -195,22 +193,18 @@ template <int D, class G>
 __device__ __forceinline__ void forStencil(const RunDev& run, const int32_t* __restrict__ start, int cx, int cy, int cz,
                                            G&& g) {
     const int32_t nx = run.nx, ny = run.ny, nz = run.nz;
-    int oy[3], oz[3];
-    const int ky = stencilOffsets(ny, oy);
-    int kz = 1;
-    oz[0] = 0;
-    if (D == 3) kz = stencilOffsets(nz, oz);
+    const int ky = stencilCount(ny);
+    const int kz = D == 3 ? stencilCount(nz) : 1;
     for (int a = 0; a < kz; ++a) {
-        const int64_t plane = D == 3 ? (int64_t)wrapNear(cz + oz[a], nz) : 0;
+        const int64_t plane = D == 3 ? (int64_t)wrapNear(cz + stencilOffset(nz, a), nz) : 0;
         for (int b = 0; b < ky; ++b) {
-            const int64_t row = run.cellFirst + (plane * ny + wrapNear(cy + oy[b], ny)) * nx;
+            const int64_t row = run.cellFirst + (plane * ny + wrapNear(cy + stencilOffset(ny, b), ny)) * nx;
             if (nx >= 3 && cx >= 1 && cx <= nx - 2) {
                 g(start[row + cx - 1], start[row + cx + 2], start[row + cx], start[row + cx + 1]);
             } else {
-                int ox[3];
-                const int kx = stencilOffsets(nx, ox);
+                const int kx = stencilCount(nx);
                 for (int c = 0; c < kx; ++c) {
-                    const int64_t cellIndex = row + wrapNear(cx + ox[c], nx);
+                    const int64_t cellIndex = row + wrapNear(cx + stencilOffset(nx, c), nx);
                     const int32_t begin = start[cellIndex], end = start[cellIndex + 1];
                     g(begin, end, begin, end);
                 }
@@ -332,13 +326,25 @@ __device__ __forceinline__ bool rootBelow(Real r2, Real t) {
     __asm__ volatile("" : "+v"(r2));
     return sqrt(r2) < t;
 }
+// withinSupport for one support t, its band's edges computed once: holds(r2) == withinSupport(r2, t).
+template <typename Real>
+struct SupportBand {
+    Real t, below, above;
+    __device__ __forceinline__ explicit SupportBand(Real support) : t(support) {
+        constexpr Real band = sizeof(Real) == 4 ? Real(9.5367431640625e-07) : Real(3.552713678800501e-15);
+        const Real t2 = support * support;
+        below = t2 * (Real(1) - band);
+        above = t2 * (Real(1) + band);
+    }
+    __device__ __forceinline__ bool holds(Real r2) const {
+        if (r2 < below) return true;
+        if (r2 > above) return false;
+        return rootBelow(r2, t);
+    }
+};
 template <typename Real>
 __device__ __forceinline__ bool withinSupport(Real r2, Real t) {
-    constexpr Real band = sizeof(Real) == 4 ? Real(9.5367431640625e-07) : Real(3.552713678800501e-15);
-    const Real t2 = t * t;
-    if (r2 < t2 * (Real(1) - band)) return true;
-    if (r2 > t2 * (Real(1) + band)) return false;
-    return rootBelow(r2, t);
+    return SupportBand<Real>(t).holds(r2);
 }
 
 // The distance a pair sum uses once its pair is known, from its squared distance: in float32 the
@@ -743,6 +749,16 @@ using LdsReal = __attribute__((address_space(3))) Real;
 template <typename Real>
 __device__ __forceinline__ LdsReal<Real>* packedSums(int32_t cached) {
     return (LdsReal<Real>*)(sweepShared + (size_t)cached * sizeof(Record<Real>));
+}
+
+// The list build's hits kept in LDS after its NearCells copy of `cached` records: bit 30 marks an
+// entry that is the LDS index of the neighbor's record in the copy, else it is the neighbor's
+// sorted position; bit 31 marks the periodic image, as in the list.
+constexpr int32_t kLdsBit = 0x40000000;
+using LdsHits = __attribute__((address_space(3))) int32_t*;
+template <typename Real>
+__device__ __forceinline__ LdsHits ldsHits(int32_t cached) {
+    return (LdsHits)(sweepShared + (size_t)cached * sizeof(Record<Real>));
 }
 
 // The pair sweeps' workgroups: each covers up to blockDim / Lanes consecutive sorted particles of

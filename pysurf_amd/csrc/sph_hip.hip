@@ -31,6 +31,7 @@
 #include <limits>
 #include <map>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "sph_device.hpp"
@@ -625,13 +626,19 @@ __device__ __forceinline__ void sweep1Finish(const RunDev& run, const Sweep1<D, 
 
 // The step's neighbor list: each particle's neighbors {j != i : r_ij < h_i + h_j} in the order the
 // cell walk visits them. A group of `Lanes` lanes tests a particle's candidates `Lanes` at a time
-// and compacts the hits with a ballot, so the list keeps the walk's order. With `WithSweep1` the
-// same walk accumulates sweep 1's sums, each lane over the candidates it tested.
+// and compacts the hits with a ballot, so the list keeps the walk's order. The test reads only the
+// fields the predicate needs. With `WithSweep1` a second pass over the hits adds sweep 1's sums,
+// neighbor m on lane m % Lanes as every list sweep deals them: the sums' arithmetic then runs once
+// per neighbor, not once per batch of candidates of which about a third are neighbors. The first
+// `hitCapacity` hits of each particle are also kept in LDS after the NearCells copy, as the LDS index
+// of the neighbor's record (kLdsBit set) where the copy holds it, so the second pass reads LDS only;
+// later hits are read back from the list, and a particle whose neighbors overflow the list walks its
+// cells for the sums, dealing them alike.
 template <int D, typename Real, int Lanes, bool WithSweep1>
 __global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
     buildNeighbors(const RunDev* __restrict__ runs, Tiles tiles, const int32_t* __restrict__ order, const int32_t* __restrict__ start, Records<Real> rec,
                    NeighborList list, SweepOut<Real> out, Clock clock, const double* __restrict__ y64,
-                   int32_t resident, int32_t cached, int64_t n) {
+                   int32_t resident, int32_t cached, int32_t hitCapacity, int64_t n) {
     const int64_t tile = xcdBlock();
     const int r = tiles.run[tile];
     const int64_t s = tiles.first[tile] + threadIdx.x / Lanes;
@@ -656,26 +663,26 @@ __global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
             cells.fill();
         }
     }
-    Sweep1<D, Real> sweep;
-    sweep.mode = 0;
-    bool fluidI = false;
+    // each particle's kept hits at an odd stride, so the groups of a wave read apart in the banks
+    const int32_t keep = WithSweep1 ? hitCapacity : 0;
+    LdsHits hits = ldsHits<Real>(cached) + (threadIdx.x / Lanes) * (keep | 1);
+    Self<D, Real> me{};
+    double yy = 0.0;
+    int32_t count = 0;
+    const bool uniform = runs[r].uniform != 0;
     if (valid) {
         const RunDev& run = runs[r];
-        const Self<D, Real> me = loadSelf<D, Real>(run, rec, s);
+        me = loadSelf<D, Real>(run, rec, s);
         int cx, cy, cz;
         cellCoordinates(run, me.cell, cx, cy, cz);
-        const bool uniform = run.uniform != 0;
-        PairKernel<Real, D> kernel;
-        kernel.init(run.kernel, run.sigmaUnit, run.hmax, uniform);
-        fluidI = me.fluid;
-        sweep.begin(run, rec, s, WithSweep1 ? sweep1Mode(run, fluidI, resident, 1, 1) : 0);
-        const double yy = run.period > 0.0 ? y64[order[s]] : 0.0;
+        yy = run.period > 0.0 ? y64[order[s]] : 0.0;
         const Real period = (Real)run.period;
         const Real twoHmax = Real(2) * (Real)run.hmax;
         const uint32_t groupMask = Lanes >= 32 ? 0xffffffffu : ((1u << Lanes) - 1u);
+        const uint32_t below = (1u << lane) - 1u;
+        const int32_t self = (int32_t)s;
         int images = 1;
         if (run.period > 0.0 && (me.p[1] < me.reach || me.p[1] > period - me.reach)) images = 2;
-        int32_t count = 0;
         for (int image = 0; image < images; ++image) {
             Real shift = Real(0);
             int oy = cy;
@@ -692,72 +699,108 @@ __global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
                 };
                 const int32_t from = D == 3 ? firstAtLeast(begin, middle, me.p[0] - reach, pointX) : begin;
                 const int32_t to = D == 3 ? firstAbove(right, end, me.p[0] + reach, pointX) : end;
-                for (int32_t base = from; base < to; base += Lanes) {
-                    const int32_t k = base + lane;
-                    bool hit = false;
-                    Real dx, dy, dz, r2;
-                    Record<Real> other;
-                    if (k < to && k != s) {
-                        // the fields the test and sweep 1 read, loaded together
-                        const bool wide = D == 3 || WithSweep1 || !uniform;
-                        if (copied >= 0) {
-                            const int32_t index = copied + (k - begin);
-                            other.q0 = ldsQuad<Real>(index, 0, cached);
-                            if (wide) other.q1 = ldsQuad<Real>(index, 1, cached);
-                            if (WithSweep1) {
-                                if constexpr (D == 2) {
-                                    other.q2.y = ldsValue<Real>(index, 2, 1, cached);
-                                    other.q3.x = ldsValue<Real>(index, 3, 0, cached);
-                                } else {
-                                    other.q2 = ldsQuad<Real>(index, 2, cached);
-                                    other.q3.y = ldsValue<Real>(index, 3, 1, cached);
-                                    other.q3.w = ldsValue<Real>(index, 3, 3, cached);
-                                }
-                            }
-                        } else {
-                            other.q0 = rec.r[k].q0;
-                            if (wide) other.q1 = rec.r[k].q1;
-                            if (WithSweep1) {
-                                if constexpr (D == 2) {
-                                    other.q2.y = rec.r[k].q2.y;
-                                    other.q3.x = rec.r[k].q3.x;
-                                } else {
-                                    other.q2 = rec.r[k].q2;
-                                    other.q3.y = rec.r[k].q3.y;
-                                    other.q3.w = rec.r[k].q3.w;
-                                }
-                            }
-                        }
+                // the predicate's fields only: the point and remainder, and the length when lengths
+                // differ; a lane past the range reads the range's last record and never hits, so
+                // the loop body has no branch but the predicate's rare rounded root. One loop per
+                // source of the records (LDS or the sorted records) and per kind of run (one
+                // smoothing length or several), so neither is decided per candidate.
+                auto scan = [&](auto fetch, auto oneLength) {
+                    constexpr bool Same = decltype(oneLength)::value;
+                    const SupportBand<Real> band(twoHmax);
+                    for (int32_t base = from; base < to; base += Lanes) {
+                        const int32_t k = base + lane;
+                        const bool inside = k < to;
+                        Record<Real> other;
+                        fetch(inside ? k : to - 1, other, D == 3 || !Same);
+                        Real dx, dy, dz;
                         separation<D, Real>(me, other, shift, dx, dy, dz);
-                        r2 = squaredDistance(dx, dy, dz, D);
+                        const Real r2 = squaredDistance(dx, dy, dz, D);
                         // 2 h_ij = h_i + h_j exactly, as 2 (0.5 (h_i + h_j)) is
-                        const Real support = uniform ? twoHmax : me.h + fabs(signedLength<D, Real>(other));
-                        hit = withinSupport(r2, support);
-                    }
-                    const uint32_t mine = (uint32_t)(__ballot(hit) >> groupBase) & groupMask;
-                    if (hit) {
-                        const int32_t slot = count + __popc(mine & ((1u << lane) - 1u));
-                        if (slot < list.capacity) list.index[listOffset(slot, list.stride) + s] = k | tag;
-                        if (WithSweep1) {
-                            const Real h = uniform ? (Real)run.hmax
-                                                   : Real(0.5) * (me.h + fabs(signedLength<D, Real>(other)));
-                            sweep.add(other, kernel, dx, dy, dz, rootOf(r2), h);
+                        const bool near = Same ? band.holds(r2)
+                                               : withinSupport<Real>(r2, me.h + fabs(signedLength<D, Real>(other)));
+                        const bool hit = near && inside && k != self;
+                        const uint32_t mine = (uint32_t)(__ballot(hit) >> groupBase) & groupMask;
+                        if (hit) {
+                            const int32_t slot = count + __popc(mine & below);
+                            if (slot < list.capacity) list.index[listOffset(slot, list.stride) + s] = k | tag;
+                            if (slot < keep) hits[slot] = (copied >= 0 ? (copied + (k - begin)) | kLdsBit : k) | tag;
                         }
+                        count += __popc(mine);
                     }
-                    count += __popc(mine);
+                };
+                auto fromLds = [&](int32_t k, Record<Real>& other, bool both) {
+                    const int32_t index = copied + (k - begin);
+                    other.q0 = ldsQuad<Real>(index, 0, cached);
+                    if (both) other.q1 = ldsQuad<Real>(index, 1, cached);
+                };
+                auto fromRecords = [&](int32_t k, Record<Real>& other, bool both) {
+                    other.q0 = rec.r[k].q0;
+                    if (both) other.q1 = rec.r[k].q1;
+                };
+                using One = std::integral_constant<bool, true>;
+                using Several = std::integral_constant<bool, false>;
+                if (copied >= 0) {
+                    if (uniform) scan(fromLds, One{}); else scan(fromLds, Several{});
+                } else {
+                    if (uniform) scan(fromRecords, One{}); else scan(fromRecords, Several{});
                 }
             });
         }
         if (lane == 0) list.count[s] = count;
     }
     if (!WithSweep1) return;
+    // every thread arrives here: the hits and list entries this workgroup wrote are visible to it
+    // after the barrier
+    __syncthreads();
+    Sweep1<D, Real> sweep;
+    sweep.mode = 0;
+    const bool fluidI = me.fluid;
+    if (valid) {
+        const RunDev& run = runs[r];
+        sweep.begin(run, rec, s, sweep1Mode(run, fluidI, resident, 1, 1));
+        if (sweep.mode) {
+            PairKernel<Real, D> kernel;
+            kernel.init(run.kernel, run.sigmaUnit, run.hmax, uniform);
+            auto add = [&](int32_t, const Record<Real>& other, Real dx, Real dy, Real dz, Real rr, Real h) {
+                sweep.add(other, kernel, dx, dy, dz, rr, h);
+            };
+            if (count <= list.capacity) {
+                const Real shiftImage = me.p[1] < Real(0.5) * (Real)run.period ? (Real)run.period : -(Real)run.period;
+                auto visit = [&](int32_t entry) {
+                    Record<Real> other;
+                    if (entry & kLdsBit) {
+                        const int32_t index = entry & 0x3fffffff;
+                        other.q0 = ldsQuad<Real>(index, 0, cached);
+                        other.q1 = ldsQuad<Real>(index, 1, cached);
+                        other.q2 = ldsQuad<Real>(index, 2, cached);
+                        other.q3 = ldsQuad<Real>(index, 3, cached);
+                    } else {
+                        other = cells(entry & 0x7fffffff);
+                    }
+                    Real dx, dy, dz;
+                    separation<D, Real>(me, other, entry < 0 ? shiftImage : Real(0), dx, dy, dz);
+                    const Real rr = sumDistance(dx, dy, dz, D);
+                    const Real h = uniform ? (Real)run.hmax : Real(0.5) * (me.h + fabs(signedLength<D, Real>(other)));
+                    add(-1, other, dx, dy, dz, rr, h);
+                };
+                // the kept hits from LDS, the rest read back from the list
+                const int32_t kept = min(count, keep);
+                int32_t m = lane;
+                for (; m < kept; m += Lanes) visit(hits[m]);
+                for (; m < count; m += Lanes) visit(list.index[listOffset(m, list.stride) + s]);
+            } else {
+                forListedNeighbors<D, Real, Lanes, false>(run, start, cells, list, s, count, me, yy, lane, uniform, add);
+            }
+        }
+    }
 #pragma unroll
     for (int e = 0; e < Sweep1<D, Real>::K; ++e) sweep.acc[e] = groupSum<Lanes>(sweep.acc[e]);
     if (!valid || lane != 0) return;
     sweep1Finish<D, Real>(runs[r], sweep, rec, out, s, order[s], fluidI, resident, 1, 1);
 }
 
-// Sweep 1 on its own: wall pressure on boundary particles, G on fluid particles, over the cells.
+// Sweep 1 on its own: wall pressure on boundary particles, G on fluid particles, over the list when
+// there is one and the cells otherwise, each neighbor dealt as the fused sweep deals it.
 template <int D, typename Real, int Lanes>
 __global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
     sweepWallAndGradients(const RunDev* __restrict__ runs, Tiles tiles, const int32_t* __restrict__ order, const int32_t* __restrict__ start, Records<Real> rec,
@@ -781,12 +824,12 @@ __global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
             PairKernel<Real, D> kernel;
             kernel.init(run.kernel, run.sigmaUnit, run.hmax, run.uniform != 0);
             const double yy = run.period > 0.0 ? y64[order[s]] : 0.0;
-            forNeighbors<D, Real, Lanes, false>(run, start, GlobalRecords<Real>{rec.r}, s, me, yy, lane, kernel.uniform,
-                                                [&](int32_t, const Record<Real>& other, Real dx, Real dy, Real dz,
-                                                    Real rr, Real h) { sweep.add(other, kernel, dx, dy, dz, rr, h); });
+            forEachNeighbor<D, Real, Lanes, false>(run, start, GlobalRecords<Real>{rec.r}, list, s, me, yy, lane,
+                                                   kernel.uniform,
+                                                   [&](int32_t, const Record<Real>& other, Real dx, Real dy, Real dz,
+                                                       Real rr, Real h) { sweep.add(other, kernel, dx, dy, dz, rr, h); });
         }
     }
-    (void)list;
 #pragma unroll
     for (int e = 0; e < Sweep1<D, Real>::K; ++e) sweep.acc[e] = groupSum<Lanes>(sweep.acc[e]);
     if (!valid || lane != 0) return;
@@ -2552,10 +2595,28 @@ int32_t nearRecords(const sph_context* ctx) {
     const int32_t most = (int32_t)((D == 2 ? 32768 : SPH_NEAR_BYTES_3D) / sizeof(Record<Real>));
     return want < most ? want : most;
 }
+// Hits per particle the fused list build keeps in LDS after its NearCells copy: a 6 KiB (2D) or
+// 16 KiB (3D) share of the workgroup's LDS, and no more than the list holds. A 2D particle has about
+// 20 neighbors at h = 1.3 dx, so at 4 lanes (64 particles a workgroup) nearly all of them fit.
+template <int D>
+int32_t hitCapacity(const sph_context* ctx);
 // A sweep's dynamic LDS: the NearCells copy, then `sums` values per particle of the workgroup.
 template <int D, typename Real>
 size_t sweepLds(const sph_context* ctx, int sums) {
     return (size_t)nearRecords<D, Real>(ctx) * sizeof(Record<Real>) + (size_t)ctx->tilePer * sums * sizeof(Real);
+}
+
+template <int D>
+int32_t hitCapacity(const sph_context* ctx) {
+    const int32_t per = (int32_t)ctx->tilePer;
+    if (per <= 0) return 0;
+    const int32_t keep = (D == 2 ? 6144 : 16384) / (int32_t)sizeof(int32_t) / per;
+    return std::max(0, std::min(keep, ctx->list.capacity));
+}
+// The fused list build's dynamic LDS: the NearCells copy, then the hits it keeps.
+template <int D, typename Real>
+size_t buildLds(const sph_context* ctx) {
+    return sweepLds<D, Real>(ctx, 0) + (size_t)ctx->tilePer * (hitCapacity<D>(ctx) | 1) * sizeof(int32_t);
 }
 
 template <typename Real>
@@ -2727,9 +2788,9 @@ int stepOnce(sph_context* ctx, int phase, int useClock, double forcedDt) {
                                                                              ctx->clock, 1, n)));
         if (ctx->list.capacity > 0) {
             WITH_LANES(ctx, TIMED(ctx, kNeighbors,
-                                  (buildNeighbors<D, Real, L, true><<<tileGrid(ctx), b, sweepLds<D, Real>(ctx, 0), ctx->stream>>>(
+                                  (buildNeighbors<D, Real, L, true><<<tileGrid(ctx), b, buildLds<D, Real>(ctx), ctx->stream>>>(
                             ctx->dRuns, tilesOf(ctx), ctx->order, ctx->cellStart, rec, ctx->list, out,
-                                      ctx->clock, y64, 1, nearRecords<D, Real>(ctx), n))));
+                                      ctx->clock, y64, 1, nearRecords<D, Real>(ctx), hitCapacity<D>(ctx), n))));
         } else {
             WITH_LANES(ctx, TIMED(ctx, kSweepWall,
                                   (sweepWallAndGradients<D, Real, L><<<tileGrid(ctx), b, 0, ctx->stream>>>(
@@ -2837,7 +2898,7 @@ int hostSweep(sph_context* ctx, int which, const double* velocity, const double*
     if (list.capacity > 0)
         WITH_LANES(ctx, (buildNeighbors<D, Real, L, false><<<tileGrid(ctx), b, sweepLds<D, Real>(ctx, 0), ctx->stream>>>(
                             ctx->dRuns, tilesOf(ctx), order, start, rec, list, emptyOut<Real>(ctx), ctx->clock, y64, 0,
-                            nearRecords<D, Real>(ctx), n)));
+                            nearRecords<D, Real>(ctx), 0, n)));
     switch (which) {
         case 0:   // wall pressure
             out.pressure = static_cast<Real*>(out0);
@@ -2952,7 +3013,7 @@ int shiftSweep(sph_context* ctx, const double* velocity, const double* density, 
     if (list.capacity > 0)
         WITH_LANES(ctx, (buildNeighbors<D, Real, L, false><<<tileGrid(ctx), b, sweepLds<D, Real>(ctx, 0), ctx->stream>>>(
                             ctx->dRuns, tilesOf(ctx), ctx->order, ctx->cellStart, rec, ctx->list, emptyOut<Real>(ctx),
-                                ctx->clock, y64, 0, nearRecords<D, Real>(ctx), n)));
+                                ctx->clock, y64, 0, nearRecords<D, Real>(ctx), 0, n)));
     Real* fields = static_cast<Real*>(ctx->shiftFields);
     for (int32_t pass = 0; pass < 2; ++pass)
         WITH_LANES(ctx, (sweepShifting<D, Real, L><<<tileGrid(ctx), b, 0, ctx->stream>>>(
@@ -2989,7 +3050,7 @@ int neighborsImpl(sph_context* ctx, int32_t* counts, const int64_t* offsets, int
     if (ctx->list.capacity > 0)
         WITH_LANES(ctx, (buildNeighbors<D, Real, L, false><<<tileGrid(ctx), b, sweepLds<D, Real>(ctx, 0), ctx->stream>>>(
                             ctx->dRuns, tilesOf(ctx), ctx->order, ctx->cellStart, rec, ctx->list, emptyOut<Real>(ctx),
-                            ctx->clock, y64, 0, nearRecords<D, Real>(ctx), n)));
+                            ctx->clock, y64, 0, nearRecords<D, Real>(ctx), 0, n)));
     neighborExport<D, Real><<<gridFor(n, b), b, 0, ctx->stream>>>(ctx->dRuns, ctx->runOf, ctx->order, ctx->cellStart, rec,
                                                                    ctx->list, y64, counts, offsets, rows, n);
     int status = launchCheck(ctx, "neighbor list");
