@@ -66,12 +66,15 @@ typedef struct {
     double period;         /* periodic y length [m], 0 for none (3D only) */
     double rho0, c0, alpha, boundaryFactor;
     double gravity[3];
-    int32_t diffusion;     /* 0 none, 1 molteni, 2 hydrostatic, 3 antuono (host-step sums only) */
+    int32_t diffusion;     /* 0 none, 1 molteni, 2 hydrostatic, 3 antuono */
     double delta;
     int32_t extrapolatedWalls, negativePressure, correction, limiter;
     double densityFloor;   /* smallest fluid density [kg/m^3], 0 for none */
     double cfl, maxTimeStep;
     sph_boundary boundary;
+    /* the Shepard filter of the fluid density every this many device-clocked steps, at a step count
+       above zero that it divides (densityFilterInterval, sphSolver.py:1979-1981); 0 never */
+    int32_t filterInterval;
 } sph_run;
 
 /* The state of every run, slot order. Replaces PairSums' device arrays (sphGpu.py:111-116). */
@@ -123,12 +126,13 @@ int sph_continuity_rate(sph_context* ctx, const double* velocity, const double* 
 int sph_shepard_density(sph_context* ctx, const double* density, void* out);     /* sphGpu.py:2104 */
 /* The renormalized density gradient of every slot (D rows of `stride`), L_i sum_j V_j (rho_j -
  * rho_i) grad_i W_ij, zero on the boundary: the 'antuono' diffusion's (sph_run.diffusion 3), which
- * sph_continuity_rate subtracts; the device steps refuse that mode. */
+ * sph_continuity_rate and the device steps subtract. */
 int sph_density_gradients(sph_context* ctx, const double* density, void* out);   /* sphGpu.py:1868 */
 /* The delta-LES-SPH closure of the host-step sums (sphGpu.py:1754-1778): while on,
  * sph_pair_accelerations builds G and each slot's eddy alpha and delta first and adds the closure's
  * viscosity on every fluid pair, and sph_continuity_rate's diffusion takes each fluid pair's delta
- * from the deltas the last such sum built (zero before any). The device steps refuse it. */
+ * from the deltas the last such sum built (zero before any). The device steps carry it too: each
+ * step builds G, the coefficients and then the sums with the closure (sphGpu.py:2256-2262). */
 int sph_set_closure(sph_context* ctx, int32_t enabled, double soundSpeed, double referenceDensity,
                     double kinematicViscosity);                                   /* sphGpu.py:1754 */
 /* Particle shifting as a host step (sphGpu.py:1966-1979, runs of one smoothing length): each slot's
@@ -171,8 +175,11 @@ int sph_run_failures(sph_context* ctx, int32_t* failures);
 
 /* `steps` device-resident steps of every active run: PairSums.residentRates + residentIntegrate
  * (sphGpu.py:2237-2315) with the time step reduced on the device instead of cflTimeStep on the
- * host (sphSolver.py:1980-1986). With `useGraph` the steps are captured once into a HIP graph
- * and replayed. Asynchronous. */
+ * host (sphSolver.py:1980-1986). Each run takes the Shepard filter at the steps its filterInterval
+ * names (WcsphSolver.advance, sphSolver.py:1979-1981), its diffusion mode ('antuono' included) and
+ * the context's closure; the energy ledger keeps its terms when on. With `useGraph` the steps are
+ * captured once into a HIP graph and replayed (captured again when the options change).
+ * Asynchronous. */
 int sph_advance(sph_context* ctx, int32_t steps, int32_t useGraph);
 
 /* The two halves of one step with the time step from the host, for step-for-step parity with the
@@ -181,6 +188,10 @@ int sph_advance(sph_context* ctx, int32_t steps, int32_t useGraph);
  * sph_integrate is PairSums.residentIntegrate(dt) (sphGpu.py:2275-2315). Both step every run
  * whatever its clock, as the host loop decides when to stop; sph_integrate advances the clocks. */
 int sph_rates(sph_context* ctx, double* maxima);
+/* sph_rates with PairSums.residentRates' filterDensity argument (sphGpu.py:2237-2254): nonzero
+ * filters every run's fluid density by Shepard's sum first. sph_rates(ctx, m) is
+ * sph_resident_rates(ctx, 0, m). */
+int sph_resident_rates(sph_context* ctx, int32_t filterDensity, double* maxima);
 int sph_integrate(sph_context* ctx, double timeStep);
 
 /* Each run's float64 sums over its fluid particles of the bound state, 7 per run in run order:
@@ -190,6 +201,19 @@ int sph_integrate(sph_context* ctx, double timeStep);
  * sphGpu.py:271-282) and momentum sum m v along x, y, z (z zero in 2D). The summation order is fixed,
  * so the sums repeat bit for bit. Synchronizes. */
 int sph_global_sums(sph_context* ctx, double* sums);
+
+/* The energy budget of the device steps (PairSums.setEnergyBudget, sphGpu.py:1780-1793): on, each
+ * fluid particle keeps a float64 running total of every term of sphEnergy.deviceTerms
+ * (sphEnergy.py:92-98; 14 terms: the pressure and viscosity work from fluid and from boundary
+ * neighbors, gravity's work, the plain continuity over fluid and over boundary neighbors, the
+ * correction's share, the diffusion, the floor, the drift, the enforcement, shifting (zero: no
+ * device step shifts) and the filter), cleared to zero by every call that switches it on. Needs a
+ * bound state. Synchronizes. */
+int sph_set_energy_budget(sph_context* ctx, int32_t enabled);
+/* Each run's total of every term since the budget was switched on [J] (per unit depth in 2D),
+ * 14 per run in run order, summed in a fixed order (PairSums.energySums, sphGpu.py:1805-1825).
+ * SPH_ERR_VALUE while the budget is off. Synchronizes. */
+int sph_energy_sums(sph_context* ctx, double* sums);
 
 /* Permutes the bound state, every field, into the cell order of the current positions, each run
  * within its own slots, carrying `identity` (one int32 per slot; may be null) along:

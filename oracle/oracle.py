@@ -52,6 +52,17 @@ class OracleClosure(ctypes.Structure):
                 ('referenceDensity', ctypes.c_double), ('waterTerm', ctypes.c_double)]
 
 
+class OracleOptions(ctypes.Structure):
+    _fields_ = [('filterInterval', ctypes.c_int32), ('closure', ctypes.c_int32),
+                ('closureSoundSpeed', ctypes.c_double), ('closureDensity', ctypes.c_double),
+                ('closureWater', ctypes.c_double)]
+
+
+# The energy budget's device terms, in the ledger's column order (sphEnergy.py deviceTerms).
+ledgerTerms = ('pressureFluid', 'viscosityFluid', 'pressureWall', 'viscosityWall', 'gravityWork', 'continuityFluid',
+               'continuityWall', 'correction', 'diffusion', 'floor', 'drift', 'clamp', 'shifting', 'filter')
+
+
 def closureStruct(physics, closure: dict | None):
     '''The OracleClosure of {'alphas', 'deltas', 'soundSpeed', 'referenceDensity', 'kinematicViscosity'}, and
     the arrays it points at; (None, None) without a closure.'''
@@ -86,6 +97,7 @@ def library():
         _library = ctypes.CDLL(_libraryPath)
         _library.oracle_cfl_time_step.restype = ctypes.c_double
         _library.oracle_advance.restype = ctypes.c_int
+        _library.oracle_advance_options.restype = ctypes.c_int
         _library.oracle_threads.restype = ctypes.c_int
     return _library
 
@@ -388,12 +400,18 @@ def enforce(physics: Physics, boundary: dict, positions, velocities, isFluid) ->
 
 class OracleRun:
 
-    '''A run stepped by the float64 restatement, state held as padded (N, 3) arrays.'''
+    """
+
+    A run stepped by the float64 restatement, state held as padded (N, 3) arrays. `filterInterval`
+    runs the Shepard filter at every step > 0 that it divides (counted from `steps`); `closure`
+    ({'soundSpeed', 'referenceDensity', 'kinematicViscosity'}) runs the delta-LES-SPH closure;
+    `ledger` keeps the energy budget's 14 device terms (`ledgerTerms`) over every advance.
+
+    """
 
     def __init__(self, physics: Physics, boundary: dict | None, positions, velocities, densities, masses, isFluid,
-                 lengths = None, time: float = 0.0) -> None:
-        if physics.diffusion == 'antuono':
-            raise ValueError("The restated step carries no 'antuono' diffusion; its host-step sums do.")
+                 lengths = None, time: float = 0.0, filterInterval: int = 0, closure: dict | None = None,
+                 ledger: bool = False, steps: int = 0) -> None:
         self.physics = physics
         count, self.pos, self.vel, self.dens, self.mass, self.lens, self.fluid = _state(
             physics, positions, velocities, densities, masses, lengths, isFluid)
@@ -405,9 +423,29 @@ class OracleRun:
         self.lastDt = ctypes.c_double(0.0)
         self.params = physics.params()
         self.boundary, self._keep = boundaryStruct(boundary)
-        self.steps = 0
+        self.steps = steps
+        self.options = None
+        if filterInterval or closure is not None:
+            self.options = OracleOptions(filterInterval = int(filterInterval), closure = int(closure is not None))
+            if closure is not None:
+                self.options.closureSoundSpeed = closure['soundSpeed']
+                self.options.closureDensity = closure['referenceDensity']
+                self.options.closureWater = 2.0 * (physics.dims + 2) * closure['kinematicViscosity'] / closure['soundSpeed']
+        self.ledger = np.zeros(len(ledgerTerms)) if ledger else None
 
     def advance(self, untilTime: float, maxSteps: int, record: bool = False):
+        if self.options is not None or self.ledger is not None or self.physics.diffusion == 'antuono':
+            if record:
+                raise ValueError('The step history is recorded without the options and the ledger only.')
+            counter = ctypes.c_int64(self.steps)
+            taken = library().oracle_advance_options(
+                ctypes.byref(self.params), ctypes.byref(self.boundary),
+                ctypes.byref(self.options) if self.options is not None else None, self.count, _ptr(self.pos),
+                _ptr(self.vel), _ptr(self.dens), _ptr(self.press), _ptr(self.acc), _ptr(self.mass), _ptr(self.lens),
+                _ptr(self.fluid), ctypes.byref(self.time), ctypes.c_double(untilTime), int(maxSteps),
+                ctypes.byref(self.lastDt), ctypes.byref(counter), _ptr(self.ledger))
+            self.steps += taken
+            return taken
         history = np.zeros((maxSteps, 3)) if record else None
         taken = library().oracle_advance(ctypes.byref(self.params), ctypes.byref(self.boundary), self.count,
                                          _ptr(self.pos), _ptr(self.vel), _ptr(self.dens), _ptr(self.press),
@@ -417,6 +455,21 @@ class OracleRun:
         self.steps += taken
         return (taken, history[:taken]) if record else taken
 
+    def budget(self) -> dict:
+        """The ledger's terms by name [J] (per unit depth in 2D)."""
+        if self.ledger is None:
+            raise ValueError('This run keeps no ledger.')
+        return dict(zip(ledgerTerms, self.ledger.tolist()))
+
+    def fluidEnergy(self) -> float:
+        """K + P + E_C of the fluid [J]: sum m (v^2 / 2 - g . x + e(rho))."""
+        fluid = self.fluid != 0
+        g = np.zeros(3)
+        g[:self.physics.dims] = np.asarray(self.physics.gravity, dtype = float)[:self.physics.dims]
+        elastic = elasticEnergy(self.physics, self.dens)
+        m = self.mass[fluid]
+        return float(np.sum(m * (0.5 * np.sum(self.vel[fluid] ** 2, axis = 1) - self.pos[fluid] @ g + elastic[fluid])))
+
     @property
     def positions(self):
         return self.pos[:, :self.physics.dims]
@@ -424,6 +477,15 @@ class OracleRun:
     @property
     def velocities(self):
         return self.vel[:, :self.physics.dims]
+
+
+def elasticEnergy(physics: Physics, densities) -> np.ndarray:
+    """The restatement's elastic energy per unit mass at each density [J/kg]."""
+    dens = np.ascontiguousarray(densities, dtype = float)
+    out = np.zeros(len(dens))
+    params = physics.params()
+    library().oracle_elastic_energy(ctypes.byref(params), len(dens), _ptr(dens), _ptr(out))
+    return out
 
 
 def threads(requested: int = 0) -> int:

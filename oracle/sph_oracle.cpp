@@ -83,6 +83,16 @@ typedef struct {
     double soundSpeed, referenceDensity, waterTerm;
 } OracleClosure;
 
+// The options a device-resident step may carry besides the delta-SPH scheme (sphGpu.py:2237-2315):
+// the Shepard filter every `filterInterval` steps (sphSolver.py:1979-1981: at step > 0 with
+// step % interval == 0) and the delta-LES-SPH closure with its c0, rho0 and K nu / c0. The
+// 'antuono' diffusion comes with the run's diffusion mode.
+typedef struct {
+    int32_t filterInterval;
+    int32_t closure;
+    double closureSoundSpeed, closureDensity, closureWater;
+} OracleOptions;
+
 }  // extern "C"
 
 namespace {
@@ -365,8 +375,12 @@ M3 correctionAt(const OracleParams& p, const CellList& cells, const Fields& s, i
 
 // a_i = -sum_j m_j (p_i / rho_i^2 + p_j / rho_j^2 + Pi_ij) grad W_ij + g (sphGpu.py:303-414;
 // sphSolver.py:1122-1149, 1192-1230, 1328-1356).
+// With `parts` (4 vectors), the pair terms without gravity split for the energy budget: pressure and
+// viscosity from fluid neighbors, then from boundary neighbors (sphGpu.py:396-414).
 V3 accelerationAt(const OracleParams& p, const CellList& cells, const Fields& s, const M3* gradients, int i,
-                  const OracleClosure* closure = nullptr) {
+                  const OracleClosure* closure = nullptr, V3* parts = nullptr) {
+    if (parts)
+        for (int k = 0; k < 4; ++k) parts[k] = {0.0, 0.0, 0.0};
     if (!s.fluid[i]) return {0.0, 0.0, 0.0};
     const V3 vi = vec(s.vel, i);
     const double rhoI = s.dens[i];
@@ -411,15 +425,22 @@ V3 accelerationAt(const OracleParams& p, const CellList& cells, const Fields& s,
         const double coefficient = -(own + s.press[j] / (rhoJ * rhoJ) + viscosity) * s.mass[j];
         const double slope = kernelSlope(p.kernel, q.r, q.h, sigmaAt(p, q.h));
         total = total + (coefficient * slope / q.r) * q.d;
+        if (parts) {
+            const int side = s.fluid[j] ? 0 : 2;
+            parts[side] = parts[side] + (-(own + s.press[j] / (rhoJ * rhoJ)) * s.mass[j] * slope / q.r) * q.d;
+            parts[side + 1] = parts[side + 1] + (-viscosity * s.mass[j] * slope / q.r) * q.d;
+        }
     });
     return total + V3{p.gravity[0], p.gravity[1], p.gravity[2]};
 }
 
 // d rho_i / dt with the delta-SPH diffusive term (sphGpu.py:416-545; sphSolver.py:623-654, 718-822).
 // `deviation` holds rho - rho0.
+// With `parts` (4 values) on a fluid particle, the rate split for the energy budget: the plain sum
+// over fluid and over boundary neighbors, the correction's share and the diffusion (sphGpu.py:479-545).
 double continuityAt(const OracleParams& p, const CellList& cells, const Fields& s, const double* deviation,
                     const M3* corrections, int i, const V3* gradients = nullptr,
-                    const OracleClosure* closure = nullptr) {
+                    const OracleClosure* closure = nullptr, double* parts = nullptr) {
     const V3 vi = vec(s.vel, i);
     const bool boundaryI = !s.fluid[i];
     const bool diffuses = p.diffusion != 0 && !boundaryI;
@@ -427,14 +448,19 @@ double continuityAt(const OracleParams& p, const CellList& cells, const Fields& 
     const int vertical = p.dims - 1;
     const double gOverC2 = std::fabs(p.gravity[vertical]) / (p.c0 * p.c0);
     const double factor = 2.0 * p.delta * p.c0;
-    double total = 0.0;
+    double total = 0.0, continuity = 0.0, plainFluid = 0.0, plainWall = 0.0, diffusive = 0.0;
     forNeighbors(p, cells, s.pos, s.len, i, true, [&](const Pair& q) {
         const int j = q.j;
         if (boundaryI && !s.fluid[j]) return;
         const double slope = kernelSlope(p.kernel, q.r, q.h, sigmaAt(p, q.h));
         const V3 dv = vi - vec(s.vel, j);
         const V3 dd = p.correction ? mul(li, q.d) : q.d;
-        total += s.mass[j] * dot(dv, dd) * slope / q.r;
+        const double increment = s.mass[j] * dot(dv, dd) * slope / q.r;
+        total += increment;
+        if (parts) {
+            continuity += increment;
+            (s.fluid[j] ? plainFluid : plainWall) += s.mass[j] * dot(dv, q.d) * slope / q.r;
+        }
         if (diffuses && s.fluid[j]) {
             double difference = deviation[j] - deviation[i];
             if (p.diffusion == 2) {
@@ -453,11 +479,43 @@ double continuityAt(const OracleParams& p, const CellList& cells, const Fields& 
                 const double both = deltaI + deltaJ;
                 pairFactor = both > 0.0 ? 2.0 * closure->soundSpeed * 2.0 * deltaI * deltaJ / both : 0.0;
             }
-            total += pairFactor * q.h * s.mass[j] / s.dens[j] * difference * (-slope * q.r) /
-                     (q.r * q.r + kSoftening * q.h * q.h);
+            const double increment = pairFactor * q.h * s.mass[j] / s.dens[j] * difference * (-slope * q.r) /
+                                     (q.r * q.r + kSoftening * q.h * q.h);
+            total += increment;
+            diffusive += increment;
         }
     });
+    if (parts) {
+        const bool tracked = !boundaryI;
+        parts[0] = tracked ? plainFluid : 0.0;
+        parts[1] = tracked ? plainWall : 0.0;
+        parts[2] = tracked && p.correction ? continuity - plainFluid - plainWall : 0.0;
+        parts[3] = tracked ? diffusive : 0.0;
+    }
     return total;
+}
+
+// The elastic energy per unit mass of the Tait fluid, e(rho) = integral from rho0 to rho of p / s^2
+// ds = (c0^2 / 7) x^2 (21 + 35 x + 35 x^2 + 21 x^3 + 7 x^4 + x^5) / (6 (1 + x)), x = rho / rho0 - 1,
+// zero below rho0 unless the fluid carries tension (sphEnergy.py:116-160). Checked against a
+// quadrature of p / rho^2 in tests/test_oracle_energy.py.
+double elasticEnergy(const OracleParams& p, double rho) {
+    const double x = rho / p.rho0 - 1.0;
+    if (x < 0.0 && !p.negativePressure) return 0.0;
+    const double polynomial = 21.0 + x * (35.0 + x * (35.0 + x * (21.0 + x * (7.0 + x))));
+    return p.c0 * p.c0 / 7.0 * x * x * polynomial / (6.0 * (1.0 + x));
+}
+
+// (e(after) - e(before)) / (after - before), or p / rho^2 at the midpoint for a change of at most
+// 1e-10 rho0, where the secant would lose its digits (sphEnergy.py:162-183).
+double elasticSlope(const OracleParams& p, double before, double after) {
+    const double change = after - before;
+    if (std::fabs(change) > 1e-10 * p.rho0) return (elasticEnergy(p, after) - elasticEnergy(p, before)) / change;
+    const double middle = 0.5 * (before + after);
+    const double ratio = middle / p.rho0, sq = ratio * ratio;
+    double pressure = p.rho0 * p.c0 * p.c0 / 7.0 * (sq * sq * sq * ratio - 1.0);
+    if (!p.negativePressure) pressure = std::max(pressure, 0.0);
+    return pressure / (middle * middle);
 }
 
 // The renormalized density gradient of fluid particle i, L_i sum_j V_j (rho_j - rho_i) grad_i W_ij
@@ -918,17 +976,59 @@ double oracle_cfl_time_step(double h, double c0, double vmax, double amax, doubl
 // `untilTime` or `maxSteps` steps are taken. State arrays are updated in place; `press` and `acc`
 // hold the last step's pressures and accelerations. Returns the steps taken; `*lastDt` the last
 // step's size. `budget` (may be null) receives per-step (vmax, amax, dt) triples.
-int oracle_advance(const OracleParams* p, const OracleBoundary* b, int n, double* pos, double* vel, double* dens,
-                   double* press, double* acc, const double* mass, const double* len, const uint8_t* fluid,
-                   double* time, double untilTime, int maxSteps, double* lastDt, double* budget) {
+//
+// With `options`: the Shepard filter when due, counted by `*stepCount` (the run's steps before
+// this call, advanced by the steps taken), and the closure. With `ledger` (14 values, added to):
+// the energy budget's device terms over the fluid (sphEnergy.py deviceTerms; sphGpu.py:1084-1419):
+//   0-3   m a_part . (v + v') / 2 dt, a_part the pressure and viscosity from fluid and from
+//         boundary neighbors; 4 the same for gravity
+//   5-8   m e' R_part dt, R_part the plain continuity over fluid and boundary neighbors, the
+//         correction's share and the diffusion, e' the elastic secant slope over the step
+//   9     the floor's change of m e;  10  the drift's -m g . (x' - x);  11  the enforcement's change
+//         of m (v^2 / 2 - g . x);  12  shifting (not a device-resident option: 0);  13  the filter's
+//         change of m e
+static int advanceWith(const OracleParams* p, const OracleBoundary* b, const OracleOptions* options, int n, double* pos,
+                double* vel, double* dens, double* press, double* acc, const double* mass, const double* len,
+                const uint8_t* fluid, double* time, double untilTime, int maxSteps, double* lastDt, double* budget,
+                int64_t* stepCount, double* ledger) {
     CellList cells;
     std::vector<M3> gradients(n), corrections(n);
-    std::vector<double> deviation(n), rate(n);
+    std::vector<double> deviation(n), rate(n), filtered, alphas, deltas, startDensity;
+    std::vector<V3> densityGradients, accParts, startVelocity;
+    std::vector<double> rateParts;
+    const bool closureOn = options && options->closure;
+    const bool antuono = p->diffusion == 3;
+    const int interval = options ? options->filterInterval : 0;
+    OracleClosure closure{};
+    if (closureOn) {
+        alphas.resize(n);
+        deltas.resize(n);
+        closure = OracleClosure{alphas.data(), deltas.data(), options->closureSoundSpeed, options->closureDensity,
+                                options->closureWater};
+    }
+    if (antuono) densityGradients.resize(n);
+    if (ledger) {
+        accParts.resize(4 * (size_t)n);
+        rateParts.resize(4 * (size_t)n);
+        startVelocity.resize(n);
+        startDensity.resize(n);
+    }
+    const V3 g = {p->gravity[0], p->gravity[1], p->gravity[2]};
     int steps = 0;
     while (*time < untilTime && steps < maxSteps) {
         // Rates (residentRates)
         cells.build(*p, n, pos);
         const Fields s{n, pos, vel, dens, press, mass, len, fluid};
+        const int64_t step = stepCount ? *stepCount : steps;
+        if (interval > 0 && step > 0 && step % interval == 0) {
+            filtered.resize(n);
+            oracle_shepard(p, n, pos, dens, mass, len, filtered.data());
+            for (int i = 0; i < n; ++i) {
+                if (!fluid[i]) continue;
+                if (ledger) ledger[13] += mass[i] * (elasticEnergy(*p, filtered[i]) - elasticEnergy(*p, dens[i]));
+                dens[i] = filtered[i];
+            }
+        }
 #pragma omp parallel for schedule(static)
         for (int i = 0; i < n; ++i) press[i] = tait(*p, dens[i], fluid[i] != 0);
         if (p->extrapolatedWalls) {
@@ -937,12 +1037,20 @@ int oracle_advance(const OracleParams* p, const OracleBoundary* b, int n, double
             for (int i = 0; i < n; ++i)
                 if (!fluid[i]) wallPressureAt(*p, cells, s, i, press + i, dens + i);
         }
-        if (p->limiter) {
+        if (p->limiter || closureOn) {
 #pragma omp parallel for schedule(dynamic, 64)
             for (int i = 0; i < n; ++i) gradients[i] = velocityGradientAt(*p, cells, s, i);
         }
+        if (closureOn) {
+            std::vector<double> flat(9 * (size_t)n);
+            for (int i = 0; i < n; ++i) std::memcpy(flat.data() + 9 * (size_t)i, gradients[i].m, sizeof gradients[i].m);
+            oracle_closure_coefficients(p, n, flat.data(), len, fluid, options->closureSoundSpeed, alphas.data(),
+                                        deltas.data());
+        }
 #pragma omp parallel for schedule(dynamic, 64)
-        for (int i = 0; i < n; ++i) store(acc, i, accelerationAt(*p, cells, s, gradients.data(), i));
+        for (int i = 0; i < n; ++i)
+            store(acc, i, accelerationAt(*p, cells, s, gradients.data(), i, closureOn ? &closure : nullptr,
+                                         ledger ? accParts.data() + 4 * (size_t)i : nullptr));
         double vmax = 0.0, amax = 0.0;
         for (int i = 0; i < n; ++i) {
             if (!fluid[i]) continue;
@@ -954,31 +1062,84 @@ int oracle_advance(const OracleParams* p, const OracleBoundary* b, int n, double
         if (budget) { budget[3 * steps] = vmax; budget[3 * steps + 1] = amax; budget[3 * steps + 2] = dt; }
 
         // Integration (residentIntegrate)
-        for (int i = 0; i < n; ++i)
-            if (fluid[i]) store(vel, i, vec(vel, i) + dt * vec(acc, i));
+        for (int i = 0; i < n; ++i) {
+            if (!fluid[i]) continue;
+            const V3 start = vec(vel, i), next = start + dt * vec(acc, i);
+            store(vel, i, next);
+            if (ledger) {
+                const V3 mean = 0.5 * (start + next);
+                for (int k = 0; k < 4; ++k) ledger[k] += mass[i] * dot(accParts[4 * (size_t)i + k], mean) * dt;
+                ledger[4] += mass[i] * dot(g, mean) * dt;
+            }
+        }
         for (int i = 0; i < n; ++i) deviation[i] = dens[i] - p->rho0;
-        if (p->correction) {
+        if (p->correction || antuono) {
 #pragma omp parallel for schedule(dynamic, 64)
             for (int i = 0; i < n; ++i) corrections[i] = correctionAt(*p, cells, s, i);
         }
+        if (antuono) {
 #pragma omp parallel for schedule(dynamic, 64)
-        for (int i = 0; i < n; ++i) rate[i] = continuityAt(*p, cells, s, deviation.data(), corrections.data(), i);
+            for (int i = 0; i < n; ++i)
+                densityGradients[i] = densityGradientAt(*p, cells, s, deviation.data(), corrections[i], i);
+        }
+#pragma omp parallel for schedule(dynamic, 64)
+        for (int i = 0; i < n; ++i)
+            rate[i] = continuityAt(*p, cells, s, deviation.data(), corrections.data(), i,
+                                   antuono ? densityGradients.data() : nullptr, closureOn ? &closure : nullptr,
+                                   ledger ? rateParts.data() + 4 * (size_t)i : nullptr);
         for (int i = 0; i < n; ++i) {
+            const double before = dens[i];
             double rho = dens[i] + rate[i] * dt;
+            const double unfloored = rho;
             if (p->densityFloor > 0.0 && fluid[i]) rho = std::max(rho, p->densityFloor);
             dens[i] = rho;
+            if (ledger && fluid[i]) {
+                const double slope = elasticSlope(*p, before, unfloored);
+                for (int k = 0; k < 4; ++k) ledger[5 + k] += mass[i] * slope * rateParts[4 * (size_t)i + k] * dt;
+                ledger[9] += mass[i] * (elasticEnergy(*p, rho) - elasticEnergy(*p, unfloored));
+            }
         }
         for (int i = 0; i < n; ++i) {
             if (!fluid[i]) continue;
-            store(pos, i, vec(pos, i) + dt * vec(vel, i));
+            const V3 start = vec(pos, i);
+            store(pos, i, start + dt * vec(vel, i));
+            if (ledger) ledger[10] -= mass[i] * dot(g, vec(pos, i) - start);
             if (p->period > 0.0) pos[3 * i + 1] -= p->period * std::floor(pos[3 * i + 1] / p->period);
+            const V3 x0 = vec(pos, i), v0 = vec(vel, i);
             enforceAt(*p, *b, pos + 3 * i, vel + 3 * i);
+            if (ledger) {
+                const V3 x1 = vec(pos, i), v1 = vec(vel, i);
+                ledger[11] += mass[i] * (0.5 * (dot(v1, v1) - dot(v0, v0)) - dot(g, x1 - x0));
+            }
         }
         *time += dt;
         *lastDt = dt;
         ++steps;
+        if (stepCount) ++*stepCount;
     }
     return steps;
+}
+
+
+int oracle_advance(const OracleParams* p, const OracleBoundary* b, int n, double* pos, double* vel, double* dens,
+                   double* press, double* acc, const double* mass, const double* len, const uint8_t* fluid,
+                   double* time, double untilTime, int maxSteps, double* lastDt, double* budget) {
+    return advanceWith(p, b, nullptr, n, pos, vel, dens, press, acc, mass, len, fluid, time, untilTime, maxSteps,
+                       lastDt, budget, nullptr, nullptr);
+}
+
+// oracle_advance with the resident options and the energy ledger (either may be null).
+int oracle_advance_options(const OracleParams* p, const OracleBoundary* b, const OracleOptions* options, int n,
+                           double* pos, double* vel, double* dens, double* press, double* acc, const double* mass,
+                           const double* len, const uint8_t* fluid, double* time, double untilTime, int maxSteps,
+                           double* lastDt, int64_t* stepCount, double* ledger) {
+    return advanceWith(p, b, options, n, pos, vel, dens, press, acc, mass, len, fluid, time, untilTime, maxSteps,
+                       lastDt, nullptr, stepCount, ledger);
+}
+
+// The elastic energy per unit mass at each density (for the ledger's checks).
+void oracle_elastic_energy(const OracleParams* p, int n, const double* dens, double* out) {
+    for (int i = 0; i < n; ++i) out[i] = elasticEnergy(*p, dens[i]);
 }
 
 }  // extern "C"

@@ -46,7 +46,8 @@ class Run(ctypes.Structure):
                 ('gravity', ctypes.c_double * 3), ('diffusion', ctypes.c_int32), ('delta', ctypes.c_double),
                 ('extrapolatedWalls', ctypes.c_int32), ('negativePressure', ctypes.c_int32),
                 ('correction', ctypes.c_int32), ('limiter', ctypes.c_int32), ('densityFloor', ctypes.c_double),
-                ('cfl', ctypes.c_double), ('maxTimeStep', ctypes.c_double), ('boundary', Boundary)]
+                ('cfl', ctypes.c_double), ('maxTimeStep', ctypes.c_double), ('boundary', Boundary),
+                ('filterInterval', ctypes.c_int32)]
 
 
 class State(ctypes.Structure):
@@ -122,6 +123,9 @@ signatures = {
     'sph_run_failures': [ctypes.c_void_p, ctypes.c_void_p],
     'sph_advance': [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32],
     'sph_rates': [ctypes.c_void_p, ctypes.c_void_p],
+    'sph_resident_rates': [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p],
+    'sph_set_energy_budget': [ctypes.c_void_p, ctypes.c_int32],
+    'sph_energy_sums': [ctypes.c_void_p, ctypes.c_void_p],
     'sph_integrate': [ctypes.c_void_p, ctypes.c_double],
     'sph_synchronize': [ctypes.c_void_p],
     'sph_global_sums': [ctypes.c_void_p, ctypes.c_void_p],
@@ -142,6 +146,10 @@ signatures = {
 
 # The columns of sph_global_sums, per run
 globalSumNames = ['mass', 'kinetic', 'potential', 'elastic', 'momentumX', 'momentumY', 'momentumZ']
+
+# The columns of sph_energy_sums, per run: the energy budget's device terms (sphEnergy.py:92-98)
+energyTermNames = ['pressureFluid', 'viscosityFluid', 'pressureWall', 'viscosityWall', 'gravityWork', 'continuityFluid',
+                   'continuityWall', 'correction', 'diffusion', 'floor', 'drift', 'clamp', 'shifting', 'filter']
 
 # The failure codes of sph_run_failures, and what each means
 failureNames = {0: '', 1: 'non-finite', 2: 'time step collapsed'}

@@ -19,7 +19,7 @@ from pysurf_amd import _native
 from pysurf_amd.pair_sums import Engine, boundaryOf, gridDimensions
 from pysurf_amd.particles import ParticleSystem
 
-diffusionModes = {'none': 0, 'molteni': 1, 'hydrostatic': 2}
+diffusionModes = {'none': 0, 'molteni': 1, 'hydrostatic': 2, 'antuono': 3}
 
 
 def runOf(case, first: int, boundary: _native.Boundary) -> _native.Run:
@@ -48,6 +48,7 @@ def runOf(case, first: int, boundary: _native.Boundary) -> _native.Run:
     run.densityFloor = case.densityFloor
     run.cfl, run.maxTimeStep = case.cfl, case.maxTimeStep
     run.boundary = boundary
+    run.filterInterval = int(getattr(case, 'filterInterval', 0))
     return run
 
 
@@ -74,11 +75,14 @@ class BatchEngine:
     tune : bool
         Time the candidate launch shapes on these runs and keep the fastest (autotune.tune); the
         three arguments above are then ignored
+    closure : dict | None
+        The delta-LES-SPH closure of every run: {'soundSpeed', 'referenceDensity',
+        'kinematicViscosity'} (PairSums.setClosure); None for none
 
     '''
 
     def __init__(self, cases: list, precision: str = 'float32', device = None, blockSize: int = 256,
-                 lanes: int = 0, listCapacity: int = -1, tune: bool = False) -> None:
+                 lanes: int = 0, listCapacity: int = -1, tune: bool = False, closure: dict | None = None) -> None:
         if not cases:
             raise ValueError('A batch needs at least one run.')
         dimensions = {case.dimensions for case in cases}
@@ -101,6 +105,9 @@ class BatchEngine:
         self.offsets.append(first)
         self.fluidCounts = [case.fluidCount for case in self.cases]
         self.engine.configure(runs)
+        if closure is not None:
+            self.engine.setClosure(True, closure['soundSpeed'], closure['referenceDensity'],
+                                   closure['kinematicViscosity'])
         self.engine.setLaunch(blockSize, lanes, listCapacity)
         self.identity = None
         self.load()
@@ -112,6 +119,15 @@ class BatchEngine:
     @property
     def count(self) -> int:
         return self.offsets[-1]
+
+    def setEnergyBudget(self, enabled: bool) -> None:
+        '''Every run's per-particle energy ledger on, from zero, or off (PairSums.setEnergyBudget).'''
+        self.engine.setEnergyBudget(enabled)
+
+    def energySums(self) -> list:
+        '''Each run's energy budget terms since the ledger went on [J], a dict per run (PairSums.energySums).'''
+        sums = self.engine.energySums(len(self.cases))
+        return [dict(zip(_native.energyTermNames, map(float, row))) for row in sums]
 
     @property
     def fluidCount(self) -> int:

@@ -63,6 +63,12 @@ class ContainerBoundary:
         return points[~within.all(axis = 1)]
 
 
+    def description(self) -> dict:
+        '''The clamp the device applies (sphGpu.py:1197-1229): the fluid kept inside the box, open at the top when so built.'''
+        return {'kind': 'container', 'clearance': 0.0, 'lower': self.containerMin.copy(),
+                'upper': self.containerMax.copy(), 'openTop': self.openTop}
+
+
 class BedBoundary:
 
     '''

@@ -18,6 +18,8 @@ struct RunDev {
     int32_t diffusion;
     int32_t extrapolatedWalls, negativePressure, correction, limiter;
     int32_t uniform;          // every particle has the longest smoothing length
+    int32_t filterInterval;   // the Shepard filter every this many steps (0 never)
+    int32_t closure;          // the context's delta-LES-SPH closure is on: G on every fluid particle
     double cell, hmax, hmin, period;
     double rho0, c0, alpha, boundaryFactor;
     double gravity[3];
@@ -43,6 +45,16 @@ struct Clock {
     double* speed;                // per sorted slot: |v| of a fluid particle at the gather, else 0
     double* acceleration;         // per sorted slot: |a| of a fluid particle, else 0
 };
+
+// Whether run r takes the Shepard filter this step: `mode` 0 never, 1 every active run (a filtered
+// step asked from the host), 2 a run whose interval divides its step count above zero
+// (sphSolver.py:1979-1981), the count before this step's.
+__device__ __forceinline__ bool filterDue(const RunDev& run, const Clock& clock, int r, int mode) {
+    if (mode == 0 || !clock.active[r]) return false;
+    if (mode == 1) return true;
+    const int64_t steps = clock.steps[r];
+    return run.filterInterval > 0 && steps > 0 && steps % run.filterInterval == 0;
+}
 
 template <typename Real>
 struct alignas(16) Quad {
@@ -123,20 +135,23 @@ __device__ __forceinline__ void velocityOf(const Record<Real>& r, Real (&v)[3]) 
     }
 }
 
-// Field writers of one record: the wall pressure's results on a boundary particle, and the
-// kicked velocity the continuity sweep reads.
+// Field writers of one record: the wall pressure's results on a boundary particle (with rho - rho0
+// of the new density, taken in float64, as the reference's deviations are taken after the wall
+// pressure, sphGpu.py:2287), and the kicked velocity the continuity sweep reads.
 template <int D, typename Real>
-__device__ __forceinline__ void setWall(Record<Real>& r, Real pressure, Real rho) {
+__device__ __forceinline__ void setWall(Record<Real>& r, Real pressure, Real rho, Real deviation) {
     const Real pr2 = pressure / (rho * rho);
     if constexpr (D == 2) {
         r.q1.z = rho;
         r.q2.y = r.q2.x / rho;
+        r.q2.z = deviation;
         r.q2.w = pr2;
         r.q3.x = pressure;
     } else {
         r.q1.w = pr2;
         r.q2.w = rho;
         r.q3.y = r.q3.x / rho;
+        r.q3.z = deviation;
         r.q3.w = pressure;
     }
 }

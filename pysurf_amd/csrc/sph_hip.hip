@@ -359,10 +359,12 @@ struct Sources {
 template <int D, typename Real>
 __global__ void gatherRecords(const RunDev* __restrict__ runs, const int32_t* __restrict__ runOf,
                               const int32_t* __restrict__ order, Sources<Real> src, Records<Real> rec, Clock clock,
-                              int32_t resident, int64_t n) {
+                              int32_t resident, int64_t n, int32_t onlyFiltered = 0) {
     const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool inRange = s < n;
     const int r = inRange ? runOf[s] : 0;
+    // again after the Shepard filter: only the runs it filtered this step
+    if (onlyFiltered && !(inRange && filterDue(runs[r], clock, r, onlyFiltered))) return;
     bool valid = inRange && (!resident || clock.active[r]);
     double speed = 0.0;
     if (valid) {
@@ -505,10 +507,19 @@ struct SweepOut {
     double* vel[3];
     double* density64;
     int64_t stride;
-    // host steps with the delta-LES-SPH closure: alpha_i at eddy[i] and delta_i at eddy[stride + i],
-    // in slot order, the closure's c0 and rho0, and K nu / c0 (null eddy: no closure)
+    // steps with the delta-LES-SPH closure: alpha_i at eddy[i] and delta_i at eddy[stride + i], in
+    // slot order, the closure's c0 and rho0, and K nu / c0 (null eddy: no closure)
     const Real* eddy;
     Real eddyC0, eddyRho0, eddyWater;
+    // resident steps that keep the energy budget (null ledger: none): sweep 2's parts of a, row
+    // (part D + axis) of `parts`, the parts being the pressure and the viscosity from fluid, then from
+    // boundary neighbors (sphGpu.py:396-414); the ledger's 14 rows (sphEnergy.py deviceTerms)
+    Real* parts;
+    double* ledger;
+    int64_t ledgerStride;
+    // sweep 3's parts of the rate, four rows of `stride` (plain over fluid, plain over boundary, the
+    // correction's share, the diffusion), from its parts-only launch before the step's
+    Real* rateParts;
 };
 
 // Sweep 1's sums: the extrapolated wall pressure of a boundary particle (sphGpu.py:804-854, no
@@ -567,7 +578,7 @@ struct Sweep1 {
 __device__ __forceinline__ int sweep1Mode(const RunDev& run, bool fluidI, int32_t resident, int32_t doWall,
                                           int32_t doGradients) {
     if (!fluidI && doWall && (!resident || run.extrapolatedWalls)) return 1;
-    if (fluidI && doGradients && (!resident || run.limiter)) return 2;
+    if (fluidI && doGradients && (!resident || run.limiter || run.closure)) return 2;
     return 0;
 }
 
@@ -587,7 +598,7 @@ __device__ __forceinline__ void sweep1Finish(const RunDev& run, const Sweep1<D, 
         Real pressure = Real(0);
         if (sweep.acc[0] > Real(0)) pressure = fmax(sweep.acc[1] / sweep.acc[0], Real(0));
         const double rho = run.rho0 * pow(1.0 + (double)pressure / run.stiffness, 1.0 / 7.0);
-        setWall<D, Real>(rec.r[s], pressure, (Real)rho);
+        setWall<D, Real>(rec.r[s], pressure, (Real)rho, (Real)(rho - run.rho0));
         if (out.pressure) out.pressure[i] = pressure;
         if (out.density) out.density[i] = rho;
         return;
@@ -638,7 +649,7 @@ template <int D, typename Real, int Lanes, bool WithSweep1>
 __global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
     buildNeighbors(const RunDev* __restrict__ runs, Tiles tiles, const int32_t* __restrict__ order, const int32_t* __restrict__ start, Records<Real> rec,
                    NeighborList list, SweepOut<Real> out, Clock clock, const double* __restrict__ y64,
-                   int32_t resident, int32_t cached, int32_t hitCapacity, int64_t n) {
+                   int32_t resident, int32_t cached, int32_t hitCapacity, int64_t n, int32_t filterMode = 0) {
     const int64_t tile = xcdBlock();
     const int r = tiles.run[tile];
     const int64_t s = tiles.first[tile] + threadIdx.x / Lanes;
@@ -749,6 +760,9 @@ __global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
         if (lane == 0) list.count[s] = count;
     }
     if (!WithSweep1) return;
+    // a run the Shepard filter takes this step has its sweep 1 after the filter, from its new
+    // densities (the workgroup is one run's, so all of it leaves here or none)
+    if (resident && filterDue(runs[r], clock, r, filterMode)) return;
     // every thread arrives here: the hits and list entries this workgroup wrote are visible to it
     // after the barrier
     __syncthreads();
@@ -805,9 +819,11 @@ template <int D, typename Real, int Lanes>
 __global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
     sweepWallAndGradients(const RunDev* __restrict__ runs, Tiles tiles, const int32_t* __restrict__ order, const int32_t* __restrict__ start, Records<Real> rec,
                           NeighborList list, SweepOut<Real> out, Clock clock, const double* __restrict__ y64,
-                          int32_t resident, int32_t doWall, int32_t doGradients, int64_t n) {
+                          int32_t resident, int32_t doWall, int32_t doGradients, int64_t n,
+                          int32_t onlyFiltered = 0) {
     const int64_t tile = xcdBlock();
     const int r = tiles.run[tile];
+    if (onlyFiltered && !filterDue(runs[r], clock, r, onlyFiltered)) return;
     const int64_t s = tiles.first[tile] + threadIdx.x / Lanes;
     const int lane = (int)(threadIdx.x % Lanes);
     Sweep1<D, Real> sweep;
@@ -849,9 +865,11 @@ template <int D, typename Real, int Lanes>
 __global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
     sweepFilters(const RunDev* __restrict__ runs, Tiles tiles, const int32_t* __restrict__ order,
                  const int32_t* __restrict__ start, Records<Real> rec, NeighborList list,
-                 const double* __restrict__ y64, int32_t which, Real epsilon, Real* __restrict__ out, int64_t stride) {
+                 const double* __restrict__ y64, int32_t which, Real epsilon, Real* __restrict__ out, int64_t stride,
+                 Clock clock = Clock{}, int32_t onlyFiltered = 0) {
     const int64_t tile = xcdBlock();
     const int r = tiles.run[tile];
+    if (onlyFiltered && !filterDue(runs[r], clock, r, onlyFiltered)) return;
     const int64_t s = tiles.first[tile] + threadIdx.x / Lanes;
     const int lane = (int)(threadIdx.x % Lanes);
     const bool valid = s < runs[r].first + runs[r].count;
@@ -1138,7 +1156,7 @@ __global__ void closureCoefficients(const int32_t* __restrict__ order, Records<R
 
 // Sweep 2: accelerations with the limited viscosity (sphGpu.py:303-414) and, when asked, the
 // correction matrix L (sphGpu.py:593-641) on fluid particles.
-template <int D, typename Real, int Lanes, bool HostOptions = false>
+template <int D, typename Real, int Lanes, bool Options = false, bool Parts = false>
 __global__ void SWEEP_BOUNDS((accelerationWaves<D, Real>()))
     sweepAccelerations(const RunDev* __restrict__ runs, Tiles tiles, const int32_t* __restrict__ order, const int32_t* __restrict__ start, Records<Real> rec,
                        NeighborList list, SweepOut<Real> out, Clock clock, const double* __restrict__ y64,
@@ -1152,6 +1170,8 @@ __global__ void SWEEP_BOUNDS((accelerationWaves<D, Real>()))
     Real acc[K];
 #pragma unroll
     for (int e = 0; e < K; ++e) acc[e] = Real(0);
+    // with Parts, the energy budget's parts of a instead of a itself
+    Real parts[4][3] = {};
     bool fluidI = false, correction = false;
     const bool runSteps = !resident || clock.active[r];
     const bool activeRun = s < runs[r].first + runs[r].count && runSteps;
@@ -1174,7 +1194,8 @@ __global__ void SWEEP_BOUNDS((accelerationWaves<D, Real>()))
     }
     if (activeRun) {
         const RunDev& run = runs[r];
-        correction = doCorrection && (!resident || run.correction);
+        // resident, L also feeds the renormalized diffusion's density gradients (sphGpu.py:2288-2291)
+        correction = doCorrection && (!resident || run.correction || run.diffusion == 3);
         const bool limiter = resident ? run.limiter != 0 : hostLimiter != 0;
         const Self<D, Real> me = loadSelf<D, Real>(run, rec, s);
         fluidI = me.fluid;
@@ -1194,7 +1215,7 @@ __global__ void SWEEP_BOUNDS((accelerationWaves<D, Real>()))
                     for (int b = 0; b < D; ++b) Gi[a][b] = rec.G[s * (D * D) + a * D + b];
             const Real alphaC0 = (Real)(run.alpha * run.c0), bf = (Real)run.boundaryFactor;
             Real eddyI = Real(0);
-            if constexpr (HostOptions) {
+            if constexpr (Options) {
                 if (out.eddy) eddyI = out.eddy[order[s]];
             }
             forEachNeighbor<D, Real, Lanes, true>(
@@ -1266,7 +1287,7 @@ __global__ void SWEEP_BOUNDS((accelerationWaves<D, Real>()))
                         viscosity = quotient(-alphaC0 * mu, Real(0.5) * (rhoI + rhoJ));
                         if (!fluidJ) viscosity *= bf;
                     }
-                    if constexpr (HostOptions) {
+                    if constexpr (Options) {
                         if (out.eddy && fluidJ) {
                             // the closure on every fluid pair: alpha_ij = K nu / (c0 h_ij) + the harmonic
                             // mean of the eddy alphas, with the unlimited v_ij . r_ij (sphGpu.py:388-393)
@@ -1283,8 +1304,37 @@ __global__ void SWEEP_BOUNDS((accelerationWaves<D, Real>()))
                     const Real coefficient = -(own + pressureRatioOf<D, Real>(other) + viscosity) * massOf<D, Real>(other) * F;
 #pragma unroll
                     for (int a = 0; a < D; ++a) acc[a] += coefficient * d[a];
+                    if constexpr (Parts) {
+                        const Real pressurePart = -(own + pressureRatioOf<D, Real>(other)) * massOf<D, Real>(other) * F;
+                        const Real viscousPart = -viscosity * massOf<D, Real>(other) * F;
+#pragma unroll
+                        for (int a = 0; a < D; ++a) {
+                            if (fluidJ) {
+                                parts[0][a] += pressurePart * d[a];
+                                parts[1][a] += viscousPart * d[a];
+                            } else {
+                                parts[2][a] += pressurePart * d[a];
+                                parts[3][a] += viscousPart * d[a];
+                            }
+                        }
+                    }
                 });
         }
+    }
+    if constexpr (Parts) {
+        // the energy budget's parts of a, sphGpu.py:396-414: this launch writes nothing else
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int a = 0; a < D; ++a) parts[k][a] = groupSum<Lanes>(parts[k][a]);
+        if (activeRun && lane == 0) {
+            const int32_t i = order[s];
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+#pragma unroll
+                for (int a = 0; a < D; ++a) out.parts[(k * D + a) * out.stride + i] = fluidI ? parts[k][a] : Real(0);
+        }
+        return;
     }
 #pragma unroll
     for (int e = 0; e < K; ++e) acc[e] = groupSum<Lanes>(acc[e]);
@@ -1431,7 +1481,7 @@ __device__ __forceinline__ void enforceBoundary(const sph_boundary& b, double* x
 // float32 on the records as they are read), and each particle's float64 kick, density advance
 // with its floor (:1111-1140), drift (:1142-1186) and boundary enforcement follow; the run's first
 // particle advances its clock.
-template <int D, typename Real, int Lanes, bool HostOptions = false>
+template <int D, typename Real, int Lanes, bool Options = false, bool Parts = false>
 __global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
     sweepContinuity(const RunDev* __restrict__ runs, Tiles tiles, const int32_t* __restrict__ order, const int32_t* __restrict__ start, Records<Real> rec,
                     NeighborList list, SweepOut<Real> out, Clock clock, int32_t resident, int32_t hostCorrection,
@@ -1441,6 +1491,9 @@ __global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
     const int64_t s = tiles.first[tile] + threadIdx.x / Lanes;
     const int lane = (int)(threadIdx.x % Lanes);
     Real total = Real(0);
+    // with Parts, the energy budget's parts of the rate of a fluid particle instead of the step: the
+    // plain sum over fluid and over boundary neighbors, the whole continuity sum, and the diffusion
+    Real plainFluid = Real(0), plainWall = Real(0), continuitySum = Real(0), diffusive = Real(0);
     bool boundaryI = false;
     const bool runSteps = !resident || clock.active[r];
     const bool activeRun = s < runs[r].first + runs[r].count && runSteps;
@@ -1487,7 +1540,8 @@ __global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
         const Real devI = deviationOf<D, Real>(rec.r[s]);
         Real gradI[3] = {Real(0), Real(0), Real(0)};
         Real eddyI = Real(0);
-        if constexpr (HostOptions) {
+        const bool tracked = Parts && !boundaryI;
+        if constexpr (Options) {
             if (diffusion == 3)
 #pragma unroll
                 for (int a = 0; a < D; ++a) gradI[a] = rec.G[s * D + a];
@@ -1519,6 +1573,16 @@ __global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
                     for (int a = 0; a < D; ++a) dot += (vi[a] - vj[a]) * d[a];
                 }
                 total += massOf<D, Real>(other) * dot * F;
+                if constexpr (Parts) {
+                    if (tracked) {
+                        continuitySum += massOf<D, Real>(other) * dot * F;
+                        Real plain = Real(0);
+#pragma unroll
+                        for (int a = 0; a < D; ++a) plain += (vi[a] - vj[a]) * d[a];
+                        if (fluidJ) plainFluid += massOf<D, Real>(other) * plain * F;
+                        else plainWall += massOf<D, Real>(other) * plain * F;
+                    }
+                }
                 if (diffuses && fluidJ) {
                     const Real devJ = deviationOf<D, Real>(other);
                     Real difference = devJ - devI;
@@ -1528,7 +1592,7 @@ __global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
                         difference -= quotient(head * d[D - 1], sq * sq * ratio);
                     }
                     Real pairFactor = factor;
-                    if constexpr (HostOptions) {
+                    if constexpr (Options) {
                         if (diffusion == 3) {
                             // less 1/2 (grad rho_i + grad rho_j) . (r_j - r_i), d = r_i - r_j (sphGpu.py:521-524)
                             Real projection = Real(0);
@@ -1546,8 +1610,30 @@ __global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
                     const Real soft = kernel.uniform ? kernel.soft : Real(0.01) * h * h;
                     const Real r2 = rr * rr;
                     total += pairFactor * h * volumeOf<D, Real>(other) * difference * quotient(-F * r2, r2 + soft);
+                    if constexpr (Parts) {
+                        if (tracked)
+                            diffusive += pairFactor * h * volumeOf<D, Real>(other) * difference *
+                                         quotient(-F * r2, r2 + soft);
+                    }
                 }
             });
+    }
+    if constexpr (Parts) {
+        // the energy budget's parts of the rate (sphGpu.py:535-545): this launch writes nothing else
+        plainFluid = groupSum<Lanes>(plainFluid);
+        plainWall = groupSum<Lanes>(plainWall);
+        continuitySum = groupSum<Lanes>(continuitySum);
+        diffusive = groupSum<Lanes>(diffusive);
+        if (activeRun && lane == 0) {
+            const int32_t i = order[s];
+            const bool tracked = !boundaryI;
+            const bool corrected = resident ? runs[r].correction != 0 : hostCorrection != 0;
+            out.rateParts[i] = tracked ? plainFluid : Real(0);
+            out.rateParts[out.stride + i] = tracked ? plainWall : Real(0);
+            out.rateParts[2 * out.stride + i] = tracked && corrected ? continuitySum - plainFluid - plainWall : Real(0);
+            out.rateParts[3 * out.stride + i] = tracked ? diffusive : Real(0);
+        }
+        return;
     }
     total = groupSum<Lanes>(total);
     LdsReal<Real>* packed = packedSums<Real>(cached);
@@ -1568,7 +1654,9 @@ __global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
         clock.steps[r] += 1;
     }
     const bool boundaryOwn = !(signedLength<D, Real>(rec.r[own]) > Real(0));
-    double rho = out.density64[i] + (double)total * dt;
+    const double before = out.density64[i];
+    double rho = before + (double)total * dt;
+    const double unfloored = rho;
     if (run.densityFloor > 0.0 && !boundaryOwn) rho = fmax(rho, run.densityFloor);
     out.density64[i] = rho;
     if (boundaryOwn) return;
@@ -1579,8 +1667,67 @@ __global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
         v[a] = out.vel[a][i] + (double)out.acceleration[a * out.stride + i] * dt;
         x[a] = out.pos[a][i] + v[a] * dt;
     }
+    // the energy budget: each term's change of K + P + E_C over this step (sphGpu.py:1084-1186,
+    // 1191-1419), summed per fluid particle
+    // (it reads the step's results, never its intermediate products, so keeping it leaves the
+    // step's arithmetic alone; the pair parts come from the parts-only launches before this one)
+    double* L = nullptr;
+    double startV[3] = {0.0, 0.0, 0.0}, kicked[3] = {0.0, 0.0, 0.0}, g[3] = {0.0, 0.0, 0.0};
+    double m = 0.0;
+    {
+        if (out.ledger) {
+            L = out.ledger + i;
+            const int64_t ls = out.ledgerStride;
+            m = (double)massOf<D, Real>(rec.r[own]);
+            const double scale = run.c0 * run.c0 / 7.0;
+            const double slope = elasticSlope(before, unfloored, run.rho0, scale, run.negativePressure);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) L[(5 + k) * ls] += m * slope * (double)out.rateParts[k * out.stride + i] * dt;
+            L[9 * ls] += m * (elasticEnergy(rho, run.rho0, scale, run.negativePressure) -
+                              elasticEnergy(unfloored, run.rho0, scale, run.negativePressure));
+            double mean[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+            for (int a = 0; a < D; ++a) {
+                startV[a] = out.vel[a][i];
+                mean[a] = 0.5 * (startV[a] + v[a]);
+                g[a] = run.gravity[a];
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                double work = 0.0;
+#pragma unroll
+                for (int a = 0; a < D; ++a) work += (double)out.parts[(k * D + a) * out.stride + i] * mean[a];
+                L[k * ls] += m * work * dt;
+            }
+            double fall = 0.0, rise = 0.0;
+#pragma unroll
+            for (int a = 0; a < D; ++a) {
+                fall += g[a] * mean[a];
+                rise += g[a] * (x[a] - out.pos[a][i]);
+            }
+            L[4 * ls] += m * fall * dt;
+            L[10 * ls] -= m * rise;
+        }
+    }
     if (run.period > 0.0) x[1] = x[1] - run.period * floor(x[1] / run.period);
+    if (L)
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            startV[a] = x[a];
+            kicked[a] = v[a];
+        }
     enforceBoundary<D>(run.boundary, x, v);
+    {
+        if (L) {
+            double kinetic = 0.0, potential = 0.0;
+#pragma unroll
+            for (int a = 0; a < D; ++a) {
+                kinetic += v[a] * v[a] - kicked[a] * kicked[a];
+                potential += g[a] * (x[a] - startV[a]);
+            }
+            L[11 * out.ledgerStride] += m * (0.5 * kinetic - potential);
+        }
+    }
 #pragma unroll
     for (int a = 0; a < D; ++a) {
         out.pos[a][i] = x[a];
@@ -1952,6 +2099,71 @@ __global__ void sumRuns(const int64_t* __restrict__ runChunk, const double* __re
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= runs) return;
     addChunks(partial, runChunk[r], runChunk[r + 1], out + (int64_t)r * kSums);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Shepard filter and the energy budget's ledger
+// ---------------------------------------------------------------------------------------------
+
+// A fluid particle of a run the filter takes this step gets its Shepard-filtered density (slot
+// order, `filtered`), the boundary keeps its own (sphGpu.py:1040-1056); with a ledger the change of
+// elastic energy goes to its filter row, 13.
+template <typename Real>
+__global__ void applyShepard(const RunDev* __restrict__ runs, const int32_t* __restrict__ runOf, sph_state state,
+                             const Real* __restrict__ filtered, double* __restrict__ ledger, int64_t ledgerStride,
+                             Clock clock, int32_t mode, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int r = runOf[i];
+    const RunDev& run = runs[r];
+    if (!filterDue(run, clock, r, mode) || !state.fluid[i]) return;
+    const double before = state.density[i], after = (double)filtered[i];
+    state.density[i] = after;
+    if (ledger) {
+        const double scale = run.c0 * run.c0 / 7.0;
+        const double m = (double)static_cast<const Real*>(state.mass)[i];
+        ledger[13 * ledgerStride + i] += m * (elasticEnergy(after, run.rho0, scale, run.negativePressure) -
+                                              elasticEnergy(before, run.rho0, scale, run.negativePressure));
+    }
+}
+
+constexpr int kLedgerTerms = 14;   // sphEnergy.py deviceTerms
+
+// One partial of each ledger row over chunk b's slots, each thread over a fixed stride and then a
+// fixed tree, so a run's totals do not depend on the other runs or on the launch.
+__global__ void __launch_bounds__(kSumThreads)
+    ledgerChunks(const RunDev* __restrict__ runs, const int32_t* __restrict__ chunkRun,
+                 const int64_t* __restrict__ chunkFirst, const double* __restrict__ ledger, int64_t ledgerStride,
+                 double* __restrict__ partial) {
+    __shared__ double tree[kLedgerTerms][kSumThreads];
+    const int64_t b = blockIdx.x;
+    const RunDev& run = runs[chunkRun[b]];
+    const int64_t first = chunkFirst[b];
+    const int64_t end = min(first + kSumChunk, run.first + run.count);
+    double acc[kLedgerTerms] = {};
+    for (int64_t i = first + threadIdx.x; i < end; i += kSumThreads)
+#pragma unroll
+        for (int t = 0; t < kLedgerTerms; ++t) acc[t] += ledger[t * ledgerStride + i];
+#pragma unroll
+    for (int t = 0; t < kLedgerTerms; ++t) tree[t][threadIdx.x] = acc[t];
+    __syncthreads();
+    for (int width = kSumThreads / 2; width > 0; width >>= 1) {
+        if ((int)threadIdx.x < width)
+            for (int t = 0; t < kLedgerTerms; ++t) tree[t][threadIdx.x] += tree[t][threadIdx.x + width];
+        __syncthreads();
+    }
+    if (threadIdx.x < kLedgerTerms) partial[b * kLedgerTerms + threadIdx.x] = tree[threadIdx.x][0];
+}
+
+// Each run's ledger totals from its chunks' partials, in chunk order.
+__global__ void ledgerRuns(const int64_t* __restrict__ runChunk, const double* __restrict__ partial,
+                           double* __restrict__ out, int32_t runs) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= runs) return;
+    double acc[kLedgerTerms] = {};
+    for (int64_t b = runChunk[r]; b < runChunk[r + 1]; ++b)
+        for (int t = 0; t < kLedgerTerms; ++t) acc[t] += partial[b * kLedgerTerms + t];
+    for (int t = 0; t < kLedgerTerms; ++t) out[(int64_t)r * kLedgerTerms + t] = acc[t];
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -2346,6 +2558,19 @@ struct sph_context {
     void* eddy = nullptr;
     size_t eddyBytes = 0;
     int64_t eddyStride = 0;
+    // The energy budget's ledger (sphEnergy.py deviceTerms): 14 float64 rows of the state's stride,
+    // each fluid particle's running total of each term; the acceleration sweep's four parts of a (4 D
+    // Real rows); the per-chunk partials and per-run totals of sph_energy_sums
+    int32_t ledgerOn = 0;
+    double* ledger = nullptr;
+    void* ledgerParts = nullptr;
+    double* ledgerPartial = nullptr;
+    double* ledgerOut = nullptr;
+    int64_t ledgerStride = 0;
+    // The Shepard filter's densities, one Real per slot; a filtered step asked from the host
+    void* filtered = nullptr;
+    int64_t filteredStride = 0;
+    int32_t filterRequest = 0;
     // particle shifting's per-slot fields of the host step, 3 D + 1 values per sorted slot
     void* shiftFields = nullptr;
     size_t shiftBytes = 0;
@@ -2397,6 +2622,7 @@ struct sph_context {
     hipGraphExec_t graphExec = nullptr;
     hipGraph_t graph = nullptr;
     int32_t graphSteps = 0;
+    int graphKey = 0;   // the step options the graph's launches carry
 
     // Per-kernel timing with HIP events on the context's stream, while profiling
     bool profiling = false;
@@ -2449,6 +2675,18 @@ void dropGraph(sph_context* ctx) {
     ctx->graphSteps = 0;
 }
 
+void releaseLedger(sph_context* ctx) {
+    if (ctx->ledger) (void)hipFree(ctx->ledger);
+    if (ctx->ledgerParts) (void)hipFree(ctx->ledgerParts);
+    if (ctx->ledgerPartial) (void)hipFree(ctx->ledgerPartial);
+    if (ctx->ledgerOut) (void)hipFree(ctx->ledgerOut);
+    ctx->ledger = nullptr;
+    ctx->ledgerParts = nullptr;
+    ctx->ledgerPartial = nullptr;
+    ctx->ledgerOut = nullptr;
+    ctx->ledgerStride = 0;
+}
+
 void releaseScratch(sph_context* ctx) {
     dropGraph(ctx);
     if (ctx->list.index) (void)hipFree(ctx->list.index);
@@ -2483,6 +2721,10 @@ void releaseScratch(sph_context* ctx) {
     if (ctx->mergeCells) (void)hipFree(ctx->mergeCells);
     ctx->mergeCells = nullptr;
     ctx->mergeCellBytes = 0;
+    releaseLedger(ctx);
+    if (ctx->filtered) (void)hipFree(ctx->filtered);
+    ctx->filtered = nullptr;
+    ctx->filteredStride = 0;
     for (void* p : ctx->allocations) (void)hipFree(p);
     ctx->allocations.clear();
     ctx->dRuns = nullptr;
@@ -2523,6 +2765,7 @@ int checkRuns(sph_context* ctx, const sph_run* runs, int32_t count, int64_t capa
         if (run.diffusion < 0 || run.diffusion > 3)
             return fail(ctx, SPH_ERR_VALUE,
                         "Density diffusion is not 0 (none), 1 (molteni), 2 (hydrostatic) or 3 (antuono).");
+        if (run.filterInterval < 0) return fail(ctx, SPH_ERR_VALUE, "The filter interval cannot be negative.");
         const sph_boundary& b = run.boundary;
         if (b.kind < 0 || b.kind > 3) return fail(ctx, SPH_ERR_VALUE, "Unknown boundary kind.");
         if (b.kind == 2 && (b.nBed < 2 || !b.bedX || !b.bedZ)) return fail(ctx, SPH_ERR_VALUE, "A bed needs at least two vertices.");
@@ -2741,28 +2984,59 @@ int requireBound(sph_context* ctx) {
     return ensureTiles(ctx);
 }
 
-// The two halves of a device-resident step of every active run. `phase` 1 is the rates
-// (residentRates), 2 the integration (residentIntegrate) with `forcedDt` when positive and the
-// run's own time step otherwise, 3 both. `useClock` 0 steps every run whatever its clock.
-// The device step runs the delta-SPH scheme's options only: a run with the renormalized diffusion
-// is refused before anything is launched or captured.
-int refuseHostOnly(sph_context* ctx) {
-    if (ctx->closureOn)
-        return fail(ctx, SPH_ERR_VALUE, "The device step does not carry the delta-LES-SPH closure; its host-step sums do.");
-    for (const RunDev& run : ctx->runsHost)
-        if (run.diffusion == 3)
-            return fail(ctx, SPH_ERR_VALUE,
-                        "The device step does not carry the renormalized 'antuono' diffusion; its host-step sums do.");
+// The options a context's device step carries besides the delta-SPH scheme, which pick its
+// launches: the Shepard filter (some run has an interval, or the host asked for a filtered step),
+// the renormalized 'antuono' diffusion (some run has it), the delta-LES-SPH closure and the energy
+// budget's ledger. Without any of them the step runs the lean sweeps.
+struct StepOptions {
+    bool filter = false, antuono = false, closure = false, ledger = false;
+    bool any() const { return antuono || closure || ledger; }
+};
+StepOptions stepOptions(const sph_context* ctx, int filterMode) {
+    StepOptions o;
+    for (const RunDev& run : ctx->runsHost) {
+        if (run.diffusion == 3) o.antuono = true;
+        if (run.filterInterval > 0 && filterMode == 2) o.filter = true;
+    }
+    if (filterMode == 1) o.filter = true;
+    o.closure = ctx->closureOn != 0;
+    o.ledger = ctx->ledgerOn != 0 && ctx->ledger != nullptr;
+    return o;
+}
+
+// The option buffers a step reads: the closure's eddy coefficients, the filter's densities and the
+// ledger's parts. Allocated here, before any launch or graph capture.
+template <typename Real>
+int ensureEddy(sph_context* ctx);
+int ensureStepBuffers(sph_context* ctx, int filterMode) {
+    const StepOptions o = stepOptions(ctx, filterMode);
+    const size_t real = ctx->precision == 0 ? sizeof(float) : sizeof(double);
+    if (o.closure) {
+        const int status = ctx->precision == 0 ? ensureEddy<float>(ctx) : ensureEddy<double>(ctx);
+        if (status != SPH_OK) return status;
+    }
+    if (o.filter && (!ctx->filtered || ctx->filteredStride != ctx->capacity)) {
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        if (ctx->filtered) (void)hipFree(ctx->filtered);
+        ctx->filtered = nullptr;
+        HIP_TRY(hipMalloc(&ctx->filtered, (size_t)ctx->capacity * real));
+        ctx->filteredStride = ctx->capacity;
+    }
     return SPH_OK;
 }
 
+// The two halves of a device-resident step of every active run. `phase` 1 is the rates
+// (residentRates), 2 the integration (residentIntegrate) with `forcedDt` when positive and the
+// run's own time step otherwise, 3 both. `useClock` 0 steps every run whatever its clock.
+// `filterMode` picks the runs the Shepard filter takes (filterDue).
 template <int D, typename Real>
-int stepOnce(sph_context* ctx, int phase, int useClock, double forcedDt) {
+int stepOnce(sph_context* ctx, int phase, int useClock, double forcedDt, int filterMode) {
     const int64_t n = ctx->total;
     const int b = ctx->block;
     const int32_t nruns = (int32_t)ctx->runs.size();
     Records<Real>& rec = recordsOf<Real>(ctx);
     const int64_t st = ctx->state.stride;
+    const StepOptions o = stepOptions(ctx, filterMode);
     if (phase & 1) {
         // A run without particles has no slot to open its step; the kernels skip it either way
         int status = buildCells<D>(ctx, useClock ? 1 : 2);
@@ -2781,6 +3055,22 @@ int stepOnce(sph_context* ctx, int phase, int useClock, double forcedDt) {
         out.vel[a] = a < D ? ctx->state.velocity + a * st : nullptr;
     }
     out.density64 = ctx->state.density;
+    if (o.closure) {
+        out.eddy = static_cast<Real*>(ctx->eddy);
+        out.eddyC0 = (Real)ctx->closureC0;
+        out.eddyRho0 = (Real)ctx->closureRho0;
+        out.eddyWater = (Real)ctx->closureWater;
+    }
+    if (o.ledger) {
+        out.ledger = ctx->ledger;
+        out.ledgerStride = ctx->ledgerStride;
+        out.parts = static_cast<Real*>(ctx->ledgerParts);
+        out.rateParts = out.parts + (int64_t)4 * D * st;
+    }
+    // the sweeps' options instantiation carries the closure and the renormalized diffusion; the
+    // ledger adds launches of its own and leaves the step's kernels as they are
+    const bool physicsOptions = o.antuono || o.closure;
+    const int mode = o.filter ? filterMode : 0;
     const double* y64 = ctx->state.position + st;
     if (phase & 1) {
         TIMED(ctx, kGather,
@@ -2790,17 +3080,49 @@ int stepOnce(sph_context* ctx, int phase, int useClock, double forcedDt) {
             WITH_LANES(ctx, TIMED(ctx, kNeighbors,
                                   (buildNeighbors<D, Real, L, true><<<tileGrid(ctx), b, buildLds<D, Real>(ctx), ctx->stream>>>(
                             ctx->dRuns, tilesOf(ctx), ctx->order, ctx->cellStart, rec, ctx->list, out,
-                                      ctx->clock, y64, 1, nearRecords<D, Real>(ctx), hitCapacity<D>(ctx), n))));
+                                      ctx->clock, y64, 1, nearRecords<D, Real>(ctx), hitCapacity<D>(ctx), n, mode))));
         } else {
             WITH_LANES(ctx, TIMED(ctx, kSweepWall,
                                   (sweepWallAndGradients<D, Real, L><<<tileGrid(ctx), b, 0, ctx->stream>>>(
                             ctx->dRuns, tilesOf(ctx), ctx->order, ctx->cellStart, rec, ctx->list, out,
                                       ctx->clock, y64, 1, 1, 1, n))));
         }
-        WITH_LANES(ctx, TIMED(ctx, kSweepAcceleration,
-                              (sweepAccelerations<D, Real, L><<<tileGrid(ctx), b, sweepLds<D, Real>(ctx, 0), ctx->stream>>>(
+        if (o.filter) {
+            // the runs the filter takes this step: their fluid takes the Shepard density of the
+            // gathered records, then their records and sweep 1 are taken again (sphGpu.py:2251-2254)
+            Real* filtered = static_cast<Real*>(ctx->filtered);
+            WITH_LANES(ctx, (sweepFilters<D, Real, L><<<tileGrid(ctx), b, 0, ctx->stream>>>(
+                            ctx->dRuns, tilesOf(ctx), ctx->order, ctx->cellStart, rec, ctx->list, y64, 0, Real(0),
+                            filtered, st, ctx->clock, mode)));
+            applyShepard<Real><<<gridFor(n, b), b, 0, ctx->stream>>>(ctx->dRuns, ctx->runOf, ctx->state, filtered,
+                                                                     o.ledger ? ctx->ledger : nullptr,
+                                                                     ctx->ledgerStride, ctx->clock, mode, n);
+            gatherRecords<D, Real><<<gridFor(n, b), b, 0, ctx->stream>>>(ctx->dRuns, ctx->runOf, ctx->order, src, rec,
+                                                                         ctx->clock, 1, n, mode);
+            WITH_LANES(ctx, (sweepWallAndGradients<D, Real, L><<<tileGrid(ctx), b, 0, ctx->stream>>>(
                             ctx->dRuns, tilesOf(ctx), ctx->order, ctx->cellStart, rec, ctx->list, out, ctx->clock,
-                                  y64, 1, 1, 1, 0, nearRecords<D, Real>(ctx), n))));
+                            y64, 1, 1, 1, n, mode)));
+        }
+        if (o.closure)
+            closureCoefficients<D, Real><<<gridFor(n, b), b, 0, ctx->stream>>>(
+                ctx->order, rec, static_cast<Real*>(ctx->eddy), st, (Real)ctx->closureC0, n);
+        if (o.ledger)
+            WITH_LANES(ctx, (sweepAccelerations<D, Real, L, true, true><<<tileGrid(ctx), b, sweepLds<D, Real>(ctx, 0),
+                                                                          ctx->stream>>>(
+                            ctx->dRuns, tilesOf(ctx), ctx->order, ctx->cellStart, rec, ctx->list, out, ctx->clock,
+                            y64, 1, 1, 1, 0, nearRecords<D, Real>(ctx), n)));
+        if (physicsOptions) {
+            WITH_LANES(ctx, TIMED(ctx, kSweepAcceleration,
+                                  (sweepAccelerations<D, Real, L, true><<<tileGrid(ctx), b, sweepLds<D, Real>(ctx, 0),
+                                                                          ctx->stream>>>(
+                            ctx->dRuns, tilesOf(ctx), ctx->order, ctx->cellStart, rec, ctx->list, out, ctx->clock,
+                                      y64, 1, 1, 1, 0, nearRecords<D, Real>(ctx), n))));
+        } else {
+            WITH_LANES(ctx, TIMED(ctx, kSweepAcceleration,
+                                  (sweepAccelerations<D, Real, L><<<tileGrid(ctx), b, sweepLds<D, Real>(ctx, 0), ctx->stream>>>(
+                            ctx->dRuns, tilesOf(ctx), ctx->order, ctx->cellStart, rec, ctx->list, out, ctx->clock,
+                                      y64, 1, 1, 1, 0, nearRecords<D, Real>(ctx), n))));
+        }
         if (ctx->sumChunkCount > 0) {
             const int slot = timeBegin(ctx, kMaxima);
             chunkMaxima<<<(unsigned)ctx->sumChunkCount, 256, 0, ctx->stream>>>(
@@ -2812,11 +3134,30 @@ int stepOnce(sph_context* ctx, int phase, int useClock, double forcedDt) {
         }
     }
     if (phase & 2) {
-        WITH_LANES(ctx, TIMED(ctx, kSweepContinuity,
-                              (sweepContinuity<D, Real, L><<<tileGrid(ctx), b,
-                                                              sweepLds<D, Real>(ctx, 1), ctx->stream>>>(
+        if (o.antuono)
+            // the renormalized density gradients from the records' deviations and sweep 2's L, kept
+            // in the records' G rows for the continuity sums (sphGpu.py:2288-2291)
+            WITH_LANES(ctx, (sweepFilters<D, Real, L><<<tileGrid(ctx), b, 0, ctx->stream>>>(
+                            ctx->dRuns, tilesOf(ctx), ctx->order, ctx->cellStart, rec, ctx->list, y64, 2, Real(0),
+                            nullptr, st)));
+        if (o.ledger)
+            WITH_LANES(ctx, (sweepContinuity<D, Real, L, true, true><<<tileGrid(ctx), b, sweepLds<D, Real>(ctx, 1),
+                                                                       ctx->stream>>>(
+                            ctx->dRuns, tilesOf(ctx), ctx->order, ctx->cellStart, rec, ctx->list, out, ctx->clock,
+                            1, 0, 0, forcedDt, nearRecords<D, Real>(ctx), n)));
+        if (physicsOptions) {
+            WITH_LANES(ctx, TIMED(ctx, kSweepContinuity,
+                                  (sweepContinuity<D, Real, L, true><<<tileGrid(ctx), b, sweepLds<D, Real>(ctx, 1),
+                                                                       ctx->stream>>>(
                             ctx->dRuns, tilesOf(ctx), ctx->order, ctx->cellStart, rec, ctx->list, out, ctx->clock,
                                   1, 0, 0, forcedDt, nearRecords<D, Real>(ctx), n))));
+        } else {
+            WITH_LANES(ctx, TIMED(ctx, kSweepContinuity,
+                                  (sweepContinuity<D, Real, L><<<tileGrid(ctx), b,
+                                                                  sweepLds<D, Real>(ctx, 1), ctx->stream>>>(
+                            ctx->dRuns, tilesOf(ctx), ctx->order, ctx->cellStart, rec, ctx->list, out, ctx->clock,
+                                  1, 0, 0, forcedDt, nearRecords<D, Real>(ctx), n))));
+        }
     }
     return launchCheck(ctx, "device step");
 }
@@ -2824,7 +3165,7 @@ int stepOnce(sph_context* ctx, int phase, int useClock, double forcedDt) {
 template <int D, typename Real>
 int advanceSteps(sph_context* ctx, int32_t steps) {
     for (int32_t k = 0; k < steps; ++k) {
-        const int status = stepOnce<D, Real>(ctx, 3, 1, 0.0);
+        const int status = stepOnce<D, Real>(ctx, 3, 1, 0.0, 2);
         if (status != SPH_OK) return status;
     }
     return SPH_OK;
@@ -2835,12 +3176,14 @@ int dispatchAdvance(sph_context* ctx, int32_t steps) {
     return ctx->precision == 0 ? advanceSteps<3, float>(ctx, steps) : advanceSteps<3, double>(ctx, steps);
 }
 
-int dispatchPhase(sph_context* ctx, int phase, double forcedDt) {
-    const int refused = refuseHostOnly(ctx);
-    if (refused != SPH_OK) return refused;
+int dispatchPhase(sph_context* ctx, int phase, double forcedDt, int filterMode) {
+    const int status = ensureStepBuffers(ctx, filterMode);
+    if (status != SPH_OK) return status;
     if (ctx->dims == 2)
-        return ctx->precision == 0 ? stepOnce<2, float>(ctx, phase, 0, forcedDt) : stepOnce<2, double>(ctx, phase, 0, forcedDt);
-    return ctx->precision == 0 ? stepOnce<3, float>(ctx, phase, 0, forcedDt) : stepOnce<3, double>(ctx, phase, 0, forcedDt);
+        return ctx->precision == 0 ? stepOnce<2, float>(ctx, phase, 0, forcedDt, filterMode)
+                                   : stepOnce<2, double>(ctx, phase, 0, forcedDt, filterMode);
+    return ctx->precision == 0 ? stepOnce<3, float>(ctx, phase, 0, forcedDt, filterMode)
+                               : stepOnce<3, double>(ctx, phase, 0, forcedDt, filterMode);
 }
 
 // The closure's eddy alphas and deltas, two rows of the state's stride, zeroed when first made or
@@ -3175,6 +3518,8 @@ int sph_configure(sph_context* ctx, const sph_run* runs, int32_t count, int64_t 
         d.correction = run.correction;
         d.limiter = run.limiter;
         d.uniform = run.hmin == run.hmax;
+        d.filterInterval = run.filterInterval;
+        d.closure = ctx->closureOn;
         d.cell = run.cell;
         d.hmax = run.hmax;
         d.hmin = run.hmin;
@@ -3358,6 +3703,13 @@ int sph_set_closure(sph_context* ctx, int32_t enabled, double soundSpeed, double
         ctx->closureRho0 = referenceDensity;
         ctx->closureWater = 2.0 * (ctx->dims + 2) * kinematicViscosity / soundSpeed;
     }
+    // the run descriptors carry the flag, so sweep 1 builds G for the closure
+    for (RunDev& run : ctx->runsHost) run.closure = ctx->closureOn;
+    if (ctx->dRuns) {
+        HIP_TRY(hipMemcpyAsync(ctx->dRuns, ctx->runsHost.data(), ctx->runsHost.size() * sizeof(RunDev),
+                               hipMemcpyHostToDevice, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+    }
     return SPH_OK;
 }
 
@@ -3468,11 +3820,16 @@ int sph_advance(sph_context* ctx, int32_t steps, int32_t useGraph) {
     if (status != SPH_OK) return status;
     if (steps < 0) return fail(ctx, SPH_ERR_VALUE, "The step count cannot be negative.");
     if (steps == 0) return SPH_OK;
-    status = refuseHostOnly(ctx);
+    status = ensureStepBuffers(ctx, 2);
     if (status != SPH_OK) return status;
     if (!useGraph || ctx->profiling) return dispatchAdvance(ctx, steps);
     if (ctx->ownsStream == false && ctx->stream == nullptr)
         return fail(ctx, SPH_ERR_VALUE, "A graph needs a stream other than the null stream.");
+    // a graph holds the launches of the options it was captured with
+    const StepOptions o = stepOptions(ctx, 2);
+    const int key = (o.filter ? 1 : 0) | (o.antuono ? 2 : 0) | (o.closure ? 4 : 0) | (o.ledger ? 8 : 0);
+    if (ctx->graphSteps == steps && ctx->graphKey != key) dropGraph(ctx);
+    ctx->graphKey = key;
     if (ctx->graphSteps != steps) {
         dropGraph(ctx);
         HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
@@ -3492,10 +3849,12 @@ int sph_advance(sph_context* ctx, int32_t steps, int32_t useGraph) {
     return SPH_OK;
 }
 
-int sph_rates(sph_context* ctx, double* maxima) {
+int sph_rates(sph_context* ctx, double* maxima) { return sph_resident_rates(ctx, 0, maxima); }
+
+int sph_resident_rates(sph_context* ctx, int32_t filterDensity, double* maxima) {
     int status = requireBound(ctx);
     if (status != SPH_OK) return status;
-    status = dispatchPhase(ctx, 1, 0.0);
+    status = dispatchPhase(ctx, 1, 0.0, filterDensity ? 1 : 0);
     if (status != SPH_OK) return status;
     if (maxima) {
         HIP_TRY(hipMemcpyAsync(maxima, ctx->clock.maxima, 2 * ctx->runs.size() * sizeof(double), hipMemcpyDeviceToHost,
@@ -3860,6 +4219,48 @@ int sph_merge(sph_context* ctx, const sph_merge_spec* specs, const uint8_t* mask
     return SPH_OK;
 }
 
+int sph_set_energy_budget(sph_context* ctx, int32_t enabled) {
+    int status = requireBound(ctx);
+    if (status != SPH_OK) return status;
+    ctx->ledgerOn = enabled ? 1 : 0;
+    if (!enabled) return SPH_OK;
+    const int64_t stride = ctx->capacity;
+    const size_t real = ctx->precision == 0 ? sizeof(float) : sizeof(double);
+    if (!ctx->ledger || ctx->ledgerStride != stride) {
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        releaseLedger(ctx);
+        HIP_TRY(hipMalloc(&ctx->ledger, (size_t)kLedgerTerms * stride * sizeof(double)));
+        HIP_TRY(hipMalloc(&ctx->ledgerParts, (size_t)(4 * ctx->dims + 4) * stride * real));
+        HIP_TRY(hipMalloc(&ctx->ledgerPartial, (size_t)std::max<int64_t>(ctx->sumChunkCount, 1) * kLedgerTerms *
+                                                   sizeof(double)));
+        HIP_TRY(hipMalloc(&ctx->ledgerOut, ctx->runs.size() * kLedgerTerms * sizeof(double)));
+        ctx->ledgerStride = stride;
+    }
+    // every term starts from zero, as the reference's push clears its ledger (sphGpu.py:2218-2219)
+    HIP_TRY(hipMemsetAsync(ctx->ledger, 0, (size_t)kLedgerTerms * stride * sizeof(double), ctx->stream));
+    HIP_TRY(hipMemsetAsync(ctx->ledgerParts, 0, (size_t)(4 * ctx->dims + 4) * stride * real, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return SPH_OK;
+}
+
+int sph_energy_sums(sph_context* ctx, double* sums) {
+    int status = requireBound(ctx);
+    if (status != SPH_OK) return status;
+    if (!sums) return fail(ctx, SPH_ERR_VALUE, "The energy sums need an output array.");
+    if (!ctx->ledgerOn || !ctx->ledger) return fail(ctx, SPH_ERR_VALUE, "The energy budget is off; see sph_set_energy_budget.");
+    const int32_t runs = (int32_t)ctx->runs.size();
+    if (ctx->sumChunkCount > 0)
+        ledgerChunks<<<(unsigned)ctx->sumChunkCount, kSumThreads, 0, ctx->stream>>>(
+            ctx->dRuns, ctx->sumChunkRun, ctx->sumChunkFirst, ctx->ledger, ctx->ledgerStride, ctx->ledgerPartial);
+    ledgerRuns<<<gridFor(runs, 64), 64, 0, ctx->stream>>>(ctx->sumRunChunk, ctx->ledgerPartial, ctx->ledgerOut, runs);
+    status = launchCheck(ctx, "energy sums");
+    if (status != SPH_OK) return status;
+    HIP_TRY(hipMemcpyAsync(sums, ctx->ledgerOut, (size_t)runs * kLedgerTerms * sizeof(double), hipMemcpyDeviceToHost,
+                           ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return SPH_OK;
+}
+
 int sph_global_sums(sph_context* ctx, double* sums) {
     int status = requireBound(ctx);
     if (status != SPH_OK) return status;
@@ -4108,7 +4509,7 @@ int sph_integrate(sph_context* ctx, double timeStep) {
     int status = requireBound(ctx);
     if (status != SPH_OK) return status;
     if (!(timeStep > 0.0)) return fail(ctx, SPH_ERR_VALUE, "The time step must be positive.");
-    return dispatchPhase(ctx, 2, timeStep);
+    return dispatchPhase(ctx, 2, timeStep, 0);
 }
 
 int sph_synchronize(sph_context* ctx) {

@@ -26,6 +26,19 @@ __device__ __forceinline__ double elasticEnergy(double rho, double rho0, double 
     return (!tension && x < 0.0) ? 0.0 : energy;
 }
 
+// The secant slope of e between two densities, (e(after) - e(before)) / (after - before), or, for a
+// change of at most 1e-10 rho0 where the secant would lose its digits, p / rho^2 at the midpoint
+// (sphEnergy.py:162-183, sphGpu.py:284-301): a density change times it is the change of e.
+__device__ __forceinline__ double elasticSlope(double before, double after, double rho0, double scale, int tension) {
+    const double change = after - before;
+    if (fabs(change) > 1e-10 * rho0)
+        return (elasticEnergy(after, rho0, scale, tension) - elasticEnergy(before, rho0, scale, tension)) / change;
+    const double middle = 0.5 * (before + after), ratio = middle / rho0, square = ratio * ratio;
+    double pressure = scale * rho0 * (square * square * square * ratio - 1.0);
+    if (!tension) pressure = fmax(pressure, 0.0);
+    return pressure / (middle * middle);
+}
+
 // The physics one chunk's sums read.
 struct SumPhysics {
     double rho0, c0;

@@ -21,9 +21,11 @@ The options the delta-SPH scheme leaves off are built as host-step sums: XSPH, t
 the renormalized 'antuono' diffusion (`densityGradients`, and `continuityRate` under
 setDiffusion('antuono')), the delta-LES-SPH closure (`setClosure`, `closureCoefficients`, and the
 closure's terms in `pairAccelerations` and `continuityRate`) and particle shifting (`setShifting`,
-`shiftingDisplacement`). The device step runs without them, and setPhysics, residentRates and the
-device steps raise ValueError when asked for one. The energy budget's per-particle ledger is not
-built: its setter accepts it switched off and raises ValueError when asked to switch it on.
+`shiftingDisplacement`). The device steps carry the Shepard filter (`residentRates(True)`, or
+`setFilterInterval` for `advance`), the 'antuono' diffusion and the closure as the reference's do
+(sphGpu.py:2237-2315), and the energy budget's per-particle ledger (`setEnergyBudget`,
+`energySums`, sphGpu.py:1780-1825). XSPH and shifting stay host steps: setPhysics and the device
+steps raise ValueError when asked for either.
 
 '''
 
@@ -258,6 +260,9 @@ class Engine:
         self.check(self.library.sph_configure(self.handle, array, len(runs), self.capacity), 'sph_configure')
         state = self.stateOf(self.tensors, self.capacity)
         self.check(self.library.sph_bind(self.handle, ctypes.byref(state)), 'sph_bind')
+        # a new layout of the runs starts the ledger afresh
+        if getattr(self, 'energyBudget', False):
+            self.setEnergyBudget(True)
 
     def upload(self, name: str, values, count: int | None = None) -> torch.Tensor:
         '''Copy host values (N,), or (N, dim) into the rows of the field, into a state tensor.'''
@@ -307,6 +312,24 @@ class Engine:
 
     def advance(self, steps: int, graph: bool = True) -> None:
         self.check(self.library.sph_advance(self.handle, int(steps), 1 if graph else 0), 'sph_advance')
+
+    # -- Options of the device step -- #
+    def setClosure(self, enabled: bool, soundSpeed: float = 1.0, referenceDensity: float = 1000.0,
+                   kinematicViscosity: float = 1e-6) -> None:
+        '''The delta-LES-SPH closure of every run (sph_set_closure).'''
+        self.check(self.library.sph_set_closure(self.handle, 1 if enabled else 0, float(soundSpeed),
+                                                float(referenceDensity), float(kinematicViscosity)), 'sph_set_closure')
+
+    def setEnergyBudget(self, enabled: bool) -> None:
+        '''Switch the per-particle energy ledger on (cleared to zero) or off (sph_set_energy_budget).'''
+        self.energyBudget = bool(enabled)
+        self.check(self.library.sph_set_energy_budget(self.handle, 1 if enabled else 0), 'sph_set_energy_budget')
+
+    def energySums(self, runs: int) -> np.ndarray:
+        '''Each run's ledger totals (sph_energy_sums), shape (runs, 14) in _native.energyTermNames order.'''
+        sums = np.zeros((runs, len(_native.energyTermNames)))
+        self.check(self.library.sph_energy_sums(self.handle, sums.ctypes.data_as(ctypes.c_void_p)), 'sph_energy_sums')
+        return sums
 
     def sortState(self, identity: torch.Tensor | None = None) -> None:
         '''Permute the state into cell order within each run, carrying `identity` (int32 per slot) along (sph_sort_state).'''
@@ -509,6 +532,7 @@ class PairSums:
         self.closure = {'enabled': False, 'soundSpeed': 1.0, 'referenceDensity': 1000.0, 'kinematicViscosity': 0.0}
         self.shifting = {'enabled': False, 'mach': 0.0, 'spacingKernel': 0.0, 'soundSpeed': 1.0, 'local': False}
         self.trackEnergy = False
+        self.filterInterval = 0
         self.engine = Engine(dimensions, pairPrecision, device)
         self.device = self.engine.device
         self._boundary = _native.Boundary()
@@ -543,6 +567,7 @@ class PairSums:
         run.cfl = physics.get('cfl', 0.25)
         run.maxTimeStep = physics.get('maxTimeStep', 1.0)
         run.boundary = self._boundary
+        run.filterInterval = self.filterInterval
         return run
 
     def _configure(self) -> None:
@@ -641,18 +666,16 @@ class PairSums:
 
         '''
 
-        Whether the host-step accelerations and diffusive term carry the delta-LES-SPH closure
+        Whether the accelerations and diffusive term carry the delta-LES-SPH closure
         (sphGpu.py:1754-1778): each fluid pair's viscosity adds K nu / (c0 h) plus the harmonic mean
-        of the eddy alphas, and the diffusive term takes the harmonic mean of the eddy deltas. The
-        device steps refuse it; the delta-SPH scheme runs without it.
+        of the eddy alphas, and the diffusive term takes the harmonic mean of the eddy deltas, in
+        the host steps and in the device steps alike.
 
         '''
 
         self.closure = {'enabled': bool(enabled), 'soundSpeed': float(soundSpeed),
                         'referenceDensity': float(referenceDensity), 'kinematicViscosity': float(kinematicViscosity)}
-        self.check(self.engine.library.sph_set_closure(self.engine.handle, 1 if enabled else 0, float(soundSpeed),
-                                                       float(referenceDensity), float(kinematicViscosity)),
-                   'sph_set_closure')
+        self.engine.setClosure(enabled, soundSpeed, referenceDensity, kinematicViscosity)
 
     def closureCoefficients(self, velocities: np.ndarray, densities: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
         '''Each particle's eddy alpha and delta at the current positions (sphGpu.py:1841-1853).'''
@@ -671,9 +694,25 @@ class PairSums:
         return engine.download('scalar'), deltas[:self.count].double().cpu().numpy()
 
     def setEnergyBudget(self, enabled: bool) -> None:
-        '''The per-particle energy ledger (sphGpu.py:1777-1789), not built: `globalSums` gives the float64 totals.'''
-        if enabled:
-            raise ValueError('The per-particle energy ledger is not built.')
+
+        '''
+
+        Whether the device steps keep the energy budget (sphGpu.py:1780-1793): each fluid particle's
+        running float64 total of every term in `_native.energyTermNames` (sphEnergy.deviceTerms),
+        from `push` to `energySums`.
+
+        '''
+
+        self.trackEnergy = bool(enabled)
+        if self.count:
+            self.engine.setEnergyBudget(self.trackEnergy)
+
+    def setFilterInterval(self, interval: int) -> None:
+        '''The Shepard filter every `interval` steps of `advance` (densityFilterInterval, sphSolver.py:1979-1981); 0 never.'''
+        if interval < 0:
+            raise ValueError('The filter interval cannot be negative.')
+        self.filterInterval = int(interval)
+        self._configured = None
 
     # -- Host steps -- #
     def update(self, positions: np.ndarray, masses: np.ndarray, isFluid: np.ndarray,
@@ -906,6 +945,9 @@ class PairSums:
             return
         self.engine.upload('velocity', np.asarray(particles.velocities, dtype = float))
         self.engine.upload('density', np.asarray(particles.densities, dtype = float))
+        # the ledger counts from the state pushed (sphGpu.py:2218-2219)
+        if self.trackEnergy:
+            self.engine.setEnergyBudget(True)
         self.engine.synchronize()
 
     def pull(self, particles) -> None:
@@ -920,8 +962,23 @@ class PairSums:
         particles.accelerations[:] = engine.download('acceleration')[:, :dim]
 
     def energySums(self) -> np.ndarray:
-        '''The per-particle energy ledger's sums (sphGpu.py:1805-1825); the ledger is not built, so always raises.'''
-        raise ValueError('The energy budget is off; see setEnergyBudget.')
+
+        '''
+
+        Each term's energy over the device steps since `push` [J], summed over the particles, in
+        `_native.energyTermNames` order (sphGpu.py:1805-1825).
+
+        Raises:
+        -------
+        ValueError : While the budget is off.
+
+        '''
+
+        if not self.trackEnergy:
+            raise ValueError('The energy budget is off; see setEnergyBudget.')
+        if self.count == 0:
+            return np.zeros(len(_native.energyTermNames))
+        return self.engine.energySums(1)[0]
 
     def globalSums(self) -> dict:
 
@@ -940,15 +997,21 @@ class PairSums:
         return values
 
     def residentRates(self, filterDensity: bool = False) -> tuple[float, float]:
-        '''The first half of a device step; returns the largest fluid speed and acceleration (sphGpu.py:2237-2273).'''
-        if filterDensity:
-            raise ValueError('The device step does not filter the density; the delta-SPH scheme runs without it. '
-                             'shepardDensity gives the filtered density as a host step.')
+
+        '''
+
+        The first half of a device step: the cells, the Shepard filter of the fluid density when
+        `filterDensity`, then the pressure and the accelerations; returns the largest fluid speed
+        and acceleration (sphGpu.py:2237-2273).
+
+        '''
+
         self._refuseShifting()
         self._sync()
         maxima = np.zeros(2)
-        self.check(self.engine.library.sph_rates(self.engine.handle, maxima.ctypes.data_as(ctypes.c_void_p)),
-                   'sph_rates')
+        self.check(self.engine.library.sph_resident_rates(self.engine.handle, 1 if filterDensity else 0,
+                                                          maxima.ctypes.data_as(ctypes.c_void_p)),
+                   'sph_resident_rates')
         return float(maxima[0]), float(maxima[1])
 
     def _refuseShifting(self) -> None:

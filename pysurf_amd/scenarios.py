@@ -16,7 +16,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from pysurf_amd.bathymetry import Bathymetry, HeightField
-from pysurf_amd.boundaries import BedBoundary, HeightFieldBoundary
+from pysurf_amd.boundaries import BedBoundary, ContainerBoundary, HeightFieldBoundary
 from pysurf_amd.particles import ParticleSystem, jitterPoints
 from pysurf_amd.refinement import splitParticles
 from pysurf_amd.waves import freshwaterDensity, gravity, solitaryWaveFluid, solitaryWaveFluid3D, solitaryWaveProfile
@@ -56,6 +56,7 @@ class Case:
     densityFloor: float = 0.9 * freshwaterDensity
     cfl: float = 0.25
     maxTimeStep: float = 1.0
+    filterInterval: int = 0      # the Shepard filter every this many steps (densityFilterInterval); 0 never
     parameters: dict = field(default_factory = dict)
 
     @property
@@ -104,6 +105,41 @@ def solitaryOnSlope(particlesPerDepth: int = 16, depth: float = 1.0, waveHeight:
                 soundSpeed = c0, densityFloor = 0.9 * freshwaterDensity,
                 parameters = dict(particlesPerDepth = particlesPerDepth, waveHeight = waveHeight, slope = slope,
                                   crestX = crestX, toeX = toeX, length = length, seed = seed, depth = d))
+
+
+def stillTank(spacing: float = 0.005, width: float = 0.2, height: float = 0.15, depth: float = 0.075,
+              jitter: float = 0.0, seed: int = 0, smoothingRatio: float = 1.3, kernelType: str = 'cubicSpline',
+              **physics) -> Case:
+
+    '''
+
+    Water at rest, `depth` deep, in an open box `width` x `height` (ContainerBoundary, 3 layers) at
+    hydrostatic densities, c0 = 10 sqrt(g height); each fluid point jittered by up to `jitter`
+    spacings with `seed`. By default the reference's small tank of its at-rest checks
+    (tests/testSphGpu.py:440-457, 557-575: 0.2 x 0.15 m half full, 5 mm spacing, cubic spline),
+    with `physics` setting the scheme (Case's fields).
+
+    '''
+
+    s = spacing
+    layers = 3
+    boundary = ContainerBoundary([0.0, 0.0], [width, height], s, nLayers = layers, openTop = True)
+    xs = np.arange(0.5 * s, width, s)
+    zs = np.arange(0.5 * s, depth, s)
+    fluid = np.array([(x, z) for z in zs for x in xs])
+    if jitter > 0.0:
+        fluid = jitterPoints(fluid, s, jitter, seed, [0.0], [width], lambda x: np.zeros_like(x),
+                             lambda x: np.full_like(x, depth))
+    particles = ParticleSystem.assemble(fluid, boundary.positions(), s, freshwaterDensity, np.zeros_like(fluid))
+    c0 = 10.0 * math.sqrt(gravity * height)
+    depthBelow = np.maximum(depth - particles.positions[:, -1], 0.0)
+    particles.densities = taitDensity(freshwaterDensity * gravity * depthBelow, freshwaterDensity, c0)
+    options = dict(kernelType = kernelType, soundSpeed = c0, densityFloor = 0.9 * freshwaterDensity)
+    options.update(physics)
+    return Case(name = 'tank', particles = particles, boundary = boundary, dimensions = 2, spacing = s,
+                smoothingLength = smoothingRatio * s, domainMin = np.array([-(layers + 1.0) * s, -(layers + 1.0) * s]),
+                domainMax = np.array([width + (layers + 1.0) * s, height]),
+                parameters = dict(depth = depth, width = width, height = height, seed = seed), **options)
 
 
 def singleRun(seed: int = 0, particlesPerDepth: int = 64) -> Case:

@@ -27,6 +27,20 @@ from pysurf_amd.waves import gravity as standardGravity
 taitExponent = 7.0
 
 
+def elasticEnergy(densities, referenceDensity: float, soundSpeed: float, tension: bool = False) -> np.ndarray:
+    '''
+
+    The elastic energy per unit mass the Tait fluid stores at each density [J/kg]: the integral of
+    p / s^2 from rho0, (c0^2 / 7) x^2 (21 + 35 x + 35 x^2 + 21 x^3 + 7 x^4 + x^5) / (6 (1 + x)) with
+    x = rho / rho0 - 1, zero below rho0 unless the fluid carries `tension` (sphEnergy.py:116-160).
+
+    '''
+    x = np.asarray(densities, dtype = float) / referenceDensity - 1.0
+    polynomial = 21.0 + x * (35.0 + x * (35.0 + x * (21.0 + x * (7.0 + x))))
+    energy = soundSpeed**2 / 7.0 * x * x * polynomial / (6.0 * (1.0 + x))
+    return energy if tension else np.where(x < 0.0, 0.0, energy)
+
+
 @dataclass
 class SimulationConfig:
 
@@ -167,21 +181,14 @@ class WcsphSolver:
             raise ValueError(f"Wall pressure '{config.wallPressure}' is not 'continuity' or 'extrapolated'.")
         if config.turbulenceClosure not in ('none', 'deltaLes'):
             raise ValueError(f"Turbulence closure '{config.turbulenceClosure}' is not 'none' or 'deltaLes'.")
-        if config.densityFilterInterval and config.densityDiffusion == 'none':
-            raise ValueError('The device step does not run the Shepard filter (PairSums.shepardDensity gives it as a '
-                             'host step); set densityFilterInterval to 0 with a diffusive term.')
+        if config.densityFilterInterval < 0:
+            raise ValueError('The density filter interval cannot be negative.')
         if config.xsphEpsilon > 0.0:
             raise ValueError('The device step does not move the fluid with XSPH (PairSums.xsphVelocity gives it as a '
                              'host step); the delta-SPH scheme sets xsphEpsilon to 0.')
         if getattr(config, 'particleShifting', False):
             raise ValueError('The device step does not shift the particles (PairSums.shiftingDisplacement gives the '
                              'shift as a host step); the delta-SPH scheme sets particleShifting to False.')
-        if config.turbulenceClosure == 'deltaLes':
-            raise ValueError("The device step does not carry the delta-LES-SPH closure (PairSums.setClosure gives it "
-                             "to the host-step sums); the delta-SPH scheme sets turbulenceClosure to 'none'.")
-        if config.densityDiffusion == 'antuono':
-            raise ValueError("The device step does not carry the renormalized 'antuono' diffusion (PairSums."
-                             "continuityRate gives it as a host step); the delta-SPH scheme uses 'hydrostatic'.")
         self._config = config
         self._boundary = boundary
         self._particles: ParticleSystem | None = None
@@ -201,7 +208,9 @@ class WcsphSolver:
         gpu.setContinuityCorrection(config.continuityCorrection)
         gpu.setViscosityLimiter(config.viscosityLimiter)
         gpu.setClosure(config.turbulenceClosure == 'deltaLes', self._speedOfSound, config.referenceDensity, 1.0034e-6)
+        gpu.setFilterInterval(config.densityFilterInterval)
         self._gpu = gpu
+        self._energy: dict | None = None
 
     def _pressure(self, densities, isFluid) -> np.ndarray:
         config = self._config
@@ -238,14 +247,44 @@ class WcsphSolver:
         '''One step on the device, the time step from the host (sphSolver.py:1899-1939 with 1976-1988).'''
         self._prepare()
         gpu, config = self._gpu, self._config
-        maxSpeed, maxAcceleration = gpu.residentRates(False)
+        interval = config.densityFilterInterval
+        maxSpeed, maxAcceleration = gpu.residentRates(bool(interval) and self._step > 0 and self._step % interval == 0)
         self._dt = cflTimeStep(self.shortestLength, self._speedOfSound, maxSpeed, maxAcceleration, config.cflNumber,
                                config.maxTimeStep)
         gpu.residentIntegrate(self._dt)
         self._time += self._dt
         self._step += 1
+        self._collectEnergy()
         gpu.pull(self._particles)
         return self.currentState
+
+    def _collectEnergy(self) -> None:
+        '''Add the device ledger's totals since the last push to the budget (sphSolver.py:1987-1989).'''
+        if self._energy is None:
+            return
+        from pysurf_amd._native import energyTermNames
+        for name, value in zip(energyTermNames, self._gpu.energySums()):
+            self._energy[name] += float(value)
+
+    def trackEnergy(self, enabled: bool = True) -> None:
+        '''Keep the energy budget from here on (sphSolver.py:2128-2149): every term starts at zero.'''
+        from pysurf_amd._native import energyTermNames
+        self._energy = {name: 0.0 for name in energyTermNames + ['refinement']} if enabled else None
+        self._gpu.setEnergyBudget(enabled)
+
+    @property
+    def energyBudget(self) -> dict | None:
+        '''Each term's energy since trackEnergy [J], or None while the budget is off (sphSolver.py:2151-2154).'''
+        return None if self._energy is None else dict(self._energy)
+
+    @property
+    def elasticEnergy(self) -> float:
+        '''The fluid's elastic energy, sum m e(rho) [J] (sphSolver.py:2156-2161).'''
+        p = self._particles
+        fluid = p.isFluid
+        energy = elasticEnergy(p.densities[fluid], self._config.referenceDensity, self._speedOfSound,
+                               self._config.negativePressure)
+        return float(np.sum(p.masses[fluid] * energy))
 
     def advance(self, untilTime: float, maxSteps: int | None = None, chunk: int = 64) -> SimulationState:
 
@@ -266,6 +305,7 @@ class WcsphSolver:
         self._prepare()
         clock = self._gpu.advance(self._time, untilTime, self._step, lastStep, chunk = chunk)
         self._time, self._step, self._dt = clock['time'], clock['steps'], clock['dt']
+        self._collectEnergy()
         self._gpu.pull(self._particles)
         if not np.all(np.isfinite(self._particles.positions)):
             raise NumericalInstabilityError(f'Particle positions stopped being finite by t = {self._time:.6f} s.')

@@ -37,7 +37,8 @@ def testStructuresMatchTheHeaderLayout():
     from pysurf_amd import _native
     # sizeof and offsetof as gcc lays out include/sph_hip.h on x86-64
     assert ctypes.sizeof(_native.Boundary) == 216
-    assert ctypes.sizeof(_native.Run) == 392
+    assert ctypes.sizeof(_native.Run) == 400
+    assert _native.Run.filterInterval.offset == 392
     assert ctypes.sizeof(_native.State) == 72
     assert _native.Run.boundary.offset == 176
     assert ctypes.sizeof(_native.ProfileSpec) == 40

@@ -122,18 +122,28 @@ def testRunsStopAtTheirOwnClocks():
     assert list(clock['steps']) == [7, 12]
 
 
-def testSplitStepsWithTheHostTimeStepMatchTheDeviceTimeStep():
-    # residentRates, cflTimeStep on the host and residentIntegrate(dt), against the fused device step
+@pytest.mark.parametrize('options', ['delta-SPH', 'antuono-closure', 'filtered'])
+def testSplitStepsWithTheHostTimeStepMatchTheDeviceTimeStep(options):
+    # residentRates (filtering when the host's step count is due), cflTimeStep on the host and
+    # residentIntegrate(dt), against the fused device step that decides both on the device; with the
+    # renormalized diffusion and the closure, and with the Shepard filter, as the reference's loop
+    # runs them (sphSolver.py:1976-1988)
     from pysurf_amd.pair_sums import PairSums
     case = smallCase(seed = 7)
     steps = 20
+    interval = 6 if options == 'filtered' else 0
+    print(f'split vs fused steps, {options}', flush = True)
 
     def make():
         sums = PairSums(2, case.smoothingLength, case.domainMin, case.domainMax, kernelType = case.kernelType)
-        sums.setDiffusion(case.diffusion, case.delta, case.soundSpeed, case.referenceDensity, 9.80665)
+        diffusion = 'antuono' if options == 'antuono-closure' else case.diffusion
+        sums.setDiffusion(diffusion, case.delta, case.soundSpeed, case.referenceDensity, 9.80665)
         sums.setWallPressure(case.extrapolatedWalls)
         sums.setContinuityCorrection(case.correction)
         sums.setViscosityLimiter(case.limiter)
+        if options == 'antuono-closure':
+            sums.setClosure(True, case.soundSpeed, case.referenceDensity, 1e-6)
+        sums.setFilterInterval(interval)
         sums.setPhysics(case.referenceDensity, case.soundSpeed, case.alpha, case.boundaryViscosityFactor, case.gravity,
                         0.0, case.densityFloor)
         particles = case.particles.copy()
@@ -143,8 +153,8 @@ def testSplitStepsWithTheHostTimeStepMatchTheDeviceTimeStep():
 
     split, splitParticles = make()
     time = 0.0
-    for _ in range(steps):
-        vmax, amax = split.residentRates(False)
+    for step in range(steps):
+        vmax, amax = split.residentRates(bool(interval) and step > 0 and step % interval == 0)
         dt = oracle.cflTimeStep(case.smoothingLength, case.soundSpeed, vmax, amax, 0.25, 1.0)
         split.residentIntegrate(dt)
         time += dt

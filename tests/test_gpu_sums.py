@@ -165,10 +165,6 @@ def testClosureMatchesTheOracle(precision, dimensions, periodic, mode):
     assert result == pytest.approx(expected, abs = scale * np.abs(expected).max())
     rate = sums.continuityRate(velocities, case['densities'])
     assert rate == pytest.approx(expectedRate, abs = scale * np.abs(expectedRate).max())
-    # The device step refuses the closure before it launches anything
-    sums.setPhysics(rho0, c0, alpha, 0.3, gravityOf(dimensions), 0.0, 0.0)
-    with pytest.raises(ValueError, match = 'closure'):
-        sums.residentRates()
 
 
 @pytest.mark.parametrize('precision', precisions)
@@ -201,20 +197,6 @@ def testShiftsMatchTheOracle(precision, dimensions, kernelType, local):
     sums.setPhysics(rho0, c0, alpha, 0.0, gravityOf(dimensions), 0.0, 0.0)
     with pytest.raises(ValueError, match = 'shift'):
         sums.residentRates()
-
-
-def testTheDeviceStepRefusesTheRenormalizedDiffusion():
-    case = cloud(2, False)
-    sums = pairSums(case, 'float32')
-    sums.setDiffusion('antuono', 0.1, c0, rho0, 9.81)
-    sums.setPhysics(rho0, c0, alpha, 0.0, gravityOf(2), 0.0, 0.0)
-    # Refused before anything is launched, on the phases and on the graph-captured steps alike
-    with pytest.raises(ValueError, match = 'antuono'):
-        sums.residentRates()
-    with pytest.raises(ValueError, match = 'antuono'):
-        sums.residentIntegrate(1e-5)
-    with pytest.raises(ValueError, match = 'antuono'):
-        sums.engine.advance(2, graph = True)
 
 
 @pytest.mark.parametrize('precision', precisions)

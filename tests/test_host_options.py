@@ -1,8 +1,9 @@
 '''
 
 The solver shell's handling of the options the delta-SPH scheme leaves off (sphSolver.py:320-380):
-the device step carries none of them, so WcsphSolver refuses each before touching a device, naming
-the PairSums host step that gives it. CPU only.
+the device step carries the Shepard filter, the 'antuono' diffusion and the closure, but not XSPH
+or shifting, so WcsphSolver refuses those two before touching a device, naming the PairSums host
+step that gives each. CPU only.
 
 '''
 
@@ -24,9 +25,6 @@ def deltaSph(**changes) -> SimulationConfig:
 
 @pytest.mark.parametrize('changes, method', [
     (dict(xsphEpsilon = 0.5), 'xsphVelocity'),
-    (dict(densityFilterInterval = 30, densityDiffusion = 'none'), 'shepardDensity'),
-    (dict(densityDiffusion = 'antuono'), 'continuityRate'),
-    (dict(turbulenceClosure = 'deltaLes'), 'setClosure'),
     (dict(particleShifting = True), 'shiftingDisplacement'),
 ])
 def testTheSolverRefusesEachHostOnlyOption(changes, method):
@@ -39,3 +37,5 @@ def testTheSolverStillRefusesUnknownOptions():
         WcsphSolver(deltaSph(turbulenceClosure = 'smagorinsky'))
     with pytest.raises(ValueError, match = 'diffusion'):
         WcsphSolver(deltaSph(densityDiffusion = 'laplacian'))
+    with pytest.raises(ValueError, match = 'filter interval'):
+        WcsphSolver(deltaSph(densityFilterInterval = -1))
