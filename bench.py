@@ -102,8 +102,12 @@ def parse():
     parser.add_argument('--no-single', action = 'store_true', help = 'skip the single 64 ppd run')
     parser.add_argument('--ridge-steps', type = int, default = 10, help = 'timed steps of the 3D ridge run')
     parser.add_argument('--no-ridge', action = 'store_true', help = 'skip the 3D ridge run')
-    parser.add_argument('--serving-requests', type = int, default = 24, help = 'requests of the serving stream')
-    parser.add_argument('--serving-wall', type = float, default = 150.0, help = 'wall seconds the serving stream may take')
+    parser.add_argument('--serving-window', type = float, default = 4.0, help = 'wall seconds of Poisson arrivals')
+    parser.add_argument('--serving-pool', type = int, default = 40, help = 'distinct runs the requests draw from')
+    parser.add_argument('--serving-wall', type = float, default = 120.0, help = 'wall seconds the serving stream may take')
+    parser.add_argument('--serving-slots-2d', type = int, default = 4000000, help = '2D particles in flight')
+    parser.add_argument('--serving-slots-3d', type = int, default = 3000000, help = '3D particles in flight')
+    parser.add_argument('--serving-3d-end', type = float, default = 0.1, help = 'simulated seconds of a 3D request')
     parser.add_argument('--no-serving', action = 'store_true', help = 'skip the serving stream')
     parser.add_argument('--no-gate', action = 'store_true', help = "skip Grilli's 1:35 case from the exact start")
     parser.add_argument('--gate-only', action = 'store_true',
@@ -288,39 +292,58 @@ def servingRun(args) -> dict:
 
     '''
 
-    The serving stream: `args.serving_requests` requests arriving as a Poisson stream of 40 per
-    second, 90% 2D runs of the batched sweep and 10% small 3D ridge runs at 16 particles per depth,
-    admitted under a 200 GB HBM budget and served by continuous batching until each run's breaking
-    point is recorded (2D runs, at most 8 s simulated) or its time runs out (3D runs: 1 s simulated).
-    Latency is each request's arrival to its breaking point's recording on the server's wall clock.
+    The serving stream: requests arriving as a Poisson stream of 40 per second over a window of
+    `args.serving_window` wall seconds, each a run drawn from a pool of `args.serving_pool` seeded
+    runs (90% 2D runs of the batched sweep, 10% small 3D ridge runs at 16 particles per depth),
+    admitted under a 200 GB HBM budget and the fleets' slots (`args.serving_slots_2d` and
+    `_3d` particles in flight), and served by continuous batching until each run's breaking point
+    is recorded (2D runs, at most 8 s simulated) or its time runs out (3D runs: `args.serving_3d_end`
+    s simulated), for at most `args.serving_wall` wall seconds. The offered load is above one GPU's
+    capacity, so requests queue for slots and the latencies include the queueing. Latency is each
+    request's arrival to its breaking point's recording on the server's wall clock; requests
+    unfinished at the wall limit are counted apart.
 
     '''
 
     import numpy as np
 
     from pysurf_amd.scenarios import servingMix
-    from pysurf_amd.serving import Server, latencyPercentiles, poissonRequests
-    cases = servingMix(args.serving_requests, seed = 0)
-    requests = poissonRequests(cases, 40.0, seed = 0, endTime = 8.0)
-    for request in requests:
-        if request.case.dimensions == 3:
-            request.endTime = 1.0
-    server = Server(budgetBytes = 200e9)
+    from pysurf_amd.serving import Request, Server, latencyPercentiles
+    pool = servingMix(args.serving_pool, seed = 0)
+    kinds = {2: [case for case in pool if case.dimensions == 2], 3: [case for case in pool if case.dimensions == 3]}
+    rng = np.random.default_rng(0)
+    requests, arrival, rate = [], 0.0, 40.0
+    while arrival < args.serving_window:
+        dimensions = 3 if (rng.uniform() < 0.1 and kinds[3]) else 2
+        case = kinds[dimensions][int(rng.integers(len(kinds[dimensions])))]
+        requests.append(Request(key = len(requests), arrival = arrival, case = case,
+                                endTime = args.serving_3d_end if dimensions == 3 else 8.0))
+        arrival += float(rng.exponential(1.0 / rate))
+    server = Server(budgetBytes = 200e9, slots2D = args.serving_slots_2d, slots3D = args.serving_slots_3d)
     start = time.perf_counter()
     served = server.serve(requests, wallLimit = args.serving_wall)
     wall = time.perf_counter() - start
-    broken = [record for record in served if record.breaking is not None]
+    finished = [record for record in served if not record.refused]
+    broken = [record for record in finished if record.breaking is not None]
     updates = sum(record.fluid * record.steps for record in served)
     latency = latencyPercentiles(broken)
-    log(f'serving: {len(served)} of {len(requests)} served in {wall:.1f} s, {len(broken)} broke; time to breaking '
-        f'p50 {latency["p50"]:.2f} s, p99 {latency["p99"]:.2f} s')
-    return {'requests': len(requests), 'served': len(served), 'broke': len(broken), 'rate_per_s': 40.0,
-            'share_3d': float(np.mean([case.dimensions == 3 for case in cases])), 'budget_GB': 200.0,
-            'wall_s': wall, 'time_to_breaking_p50_s': latency['p50'], 'time_to_breaking_p99_s': latency['p99'],
-            'completion_p50_s': latencyPercentiles(served)['p50'], 'completion_p99_s': latencyPercentiles(served)['p99'],
+    depths = [depth for _, depth in server.queueDepth]
+    log(f'serving: {len(requests)} requests over {args.serving_window:.0f} s, {len(finished)} finished in '
+        f'{wall:.1f} s, {len(broken)} broke; time to breaking p50 {latency["p50"]:.2f} s, p99 {latency["p99"]:.2f} s; '
+        f'queue depth max {max(depths, default = 0)}')
+    return {'requests': len(requests), 'window_s': args.serving_window, 'pool': len(pool), 'finished': len(finished),
+            'unfinished_at_wall_limit': len(requests) - len(served), 'refused': len(served) - len(finished),
+            'broke': len(broken), 'rate_per_s': rate,
+            'share_3d': float(np.mean([request.case.dimensions == 3 for request in requests])), 'budget_GB': 200.0,
+            'wall_s': wall, 'wall_limit_s': args.serving_wall,
+            'time_to_breaking_p50_s': latency['p50'], 'time_to_breaking_p99_s': latency['p99'],
+            'completion_p50_s': latencyPercentiles(finished)['p50'],
+            'completion_p99_s': latencyPercentiles(finished)['p99'],
             'throughput': updates / wall, 'unit': 'particle-updates/s',
-            'max_queue_depth': max((depth for _, depth in server.queueDepth), default = 0),
-            'particles_max': max(case.count for case in cases), 'particles_min': min(case.count for case in cases)}
+            'slots_2d': args.serving_slots_2d, 'slots_3d': args.serving_slots_3d, 'end_3d_s': args.serving_3d_end,
+            'max_queue_depth': max(depths, default = 0),
+            'mean_queue_depth': float(np.mean(depths)) if depths else 0.0,
+            'particles_max': max(case.count for case in pool), 'particles_min': min(case.count for case in pool)}
 
 
 def gateRun(args) -> dict:

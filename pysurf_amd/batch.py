@@ -52,6 +52,10 @@ def runOf(case, first: int, boundary: _native.Boundary) -> _native.Run:
     return run
 
 
+# The state fields that make a run's state between steps, the ones BatchEngine's carry takes
+stateFields = ('position', 'velocity', 'density', 'mass', 'length', 'fluid')
+
+
 class BatchEngine:
 
     '''
@@ -78,11 +82,18 @@ class BatchEngine:
     closure : dict | None
         The delta-LES-SPH closure of every run: {'soundSpeed', 'referenceDensity',
         'kinematicViscosity'} (PairSums.setClosure); None for none
+    carry : list | None
+        Per case, None to load its particles from the host, or (fields, start) to take its state from
+        device tensors: `fields` maps each state field ('position', 'velocity', 'density', 'mass',
+        'length', 'fluid') to a tensor laid out as the engine's (rows by slots) and the run's slots
+        start at `start` there (the request loop's membership changes: the runs that stay never
+        cross PCIe)
 
     '''
 
     def __init__(self, cases: list, precision: str = 'float32', device = None, blockSize: int = 256,
-                 lanes: int = 0, listCapacity: int = -1, tune: bool = False, closure: dict | None = None) -> None:
+                 lanes: int = 0, listCapacity: int = -1, tune: bool = False, closure: dict | None = None,
+                 carry: list | None = None) -> None:
         if not cases:
             raise ValueError('A batch needs at least one run.')
         dimensions = {case.dimensions for case in cases}
@@ -110,7 +121,7 @@ class BatchEngine:
                                    closure['kinematicViscosity'])
         self.engine.setLaunch(blockSize, lanes, listCapacity)
         self.identity = None
-        self.load()
+        self.load(carry)
         self.launch = {'lanes': lanes, 'block': blockSize, 'listCapacity': listCapacity}
         if tune:
             from pysurf_amd import autotune
@@ -133,19 +144,43 @@ class BatchEngine:
     def fluidCount(self) -> int:
         return sum(self.fluidCounts)
 
-    def load(self) -> None:
-        '''Copy every run's particle state into the engine, in the cases' own order.'''
+    def load(self, carry: list | None = None) -> None:
+        '''Copy every run's particle state into the engine, in the cases' own order: from the host, or
+        for a run that `carry` names, from another engine's slots on the device.'''
         engine = self.engine
         self.identity = None
-        parts = [case.particles for case in self.cases]
-        engine.upload('position', np.vstack([p.positions for p in parts]))
-        engine.upload('velocity', np.vstack([p.velocities for p in parts]))
-        engine.upload('density', np.concatenate([p.densities for p in parts]))
-        engine.upload('mass', np.concatenate([p.masses for p in parts]))
-        lengths = [p.lengths if p.lengths is not None else np.full(p.nParticles, case.smoothingLength)
-                   for p, case in zip(parts, self.cases)]
-        engine.upload('length', np.concatenate(lengths))
-        engine.upload('fluid', np.concatenate([p.isFluid for p in parts]).astype(np.uint8))
+        carry = carry or [None] * len(self.cases)
+        if len(carry) != len(self.cases):
+            raise ValueError('The carried runs need one entry per case.')
+        fields = stateFields
+        hosts = [index for index, source in enumerate(carry) if source is None]
+        if len(hosts) == len(self.cases):
+            parts = [case.particles for case in self.cases]
+            engine.upload('position', np.vstack([p.positions for p in parts]))
+            engine.upload('velocity', np.vstack([p.velocities for p in parts]))
+            engine.upload('density', np.concatenate([p.densities for p in parts]))
+            engine.upload('mass', np.concatenate([p.masses for p in parts]))
+            lengths = [p.lengths if p.lengths is not None else np.full(p.nParticles, case.smoothingLength)
+                       for p, case in zip(parts, self.cases)]
+            engine.upload('length', np.concatenate(lengths))
+            engine.upload('fluid', np.concatenate([p.isFluid for p in parts]).astype(np.uint8))
+        else:
+            for index, source in enumerate(carry):
+                first = self.offsets[index]
+                if source is None:
+                    p, case = self.cases[index].particles, self.cases[index]
+                    lengths = p.lengths if p.lengths is not None else np.full(p.nParticles, case.smoothingLength)
+                    for name, values in (('position', p.positions), ('velocity', p.velocities),
+                                         ('density', p.densities), ('mass', p.masses), ('length', lengths),
+                                         ('fluid', p.isFluid.astype(np.uint8))):
+                        engine.uploadSlice(name, values, first)
+                    continue
+                tensors, start = source
+                count = self.cases[index].count
+                for name in fields:
+                    if tensors[name].shape[-1] < start + count:
+                        raise ValueError(f'The carried {name} holds fewer than the run\'s {count} slots.')
+                    engine.copySlice(name, tensors[name], start, first, count)
         engine.synchronize()
         # runs whose masses, lengths and kinds the device changed (a split or a merge)
         self.refined = [False] * len(self.cases)

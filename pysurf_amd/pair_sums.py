@@ -277,6 +277,30 @@ class Engine:
                 tensor[:count].copy_(source.to(tensor.dtype), non_blocking = False)
         return tensor
 
+    def uploadSlice(self, name: str, values, first: int) -> None:
+        '''Copy host values (n,) or (n, dim) into slots [first, first + n) of a state tensor.'''
+        tensor = self.tensors[name]
+        values = np.asarray(values)
+        count = values.shape[0]
+        with torch.cuda.device(self.device), torch.cuda.stream(self.stream):
+            source = torch.from_numpy(np.ascontiguousarray(values.T if values.ndim == 2 else values))
+            if tensor.dim() == 2:
+                rows = values.shape[1] if values.ndim == 2 else 1
+                tensor[:rows, first:first + count].copy_(source.to(tensor.dtype), non_blocking = False)
+            else:
+                tensor[first:first + count].copy_(source.to(tensor.dtype), non_blocking = False)
+
+    def copySlice(self, name: str, tensor: torch.Tensor, source: int, first: int, count: int) -> None:
+        '''Copy slots [source, source + count) of a device tensor laid out as this field's into slots
+        [first, first + count) of the field, on the device.'''
+        with torch.cuda.device(self.device), torch.cuda.stream(self.stream):
+            target = self.tensors[name]
+            rows = tensor[..., source:source + count]
+            if target.dim() == 2:
+                target[:rows.shape[0], first:first + count].copy_(rows)
+            else:
+                target[first:first + count].copy_(rows)
+
     def download(self, name: str, rows: int | None = None, count: int | None = None) -> np.ndarray:
         '''A state tensor's first `count` slots as float64, (N, rows) for a field of rows, (N,) otherwise.'''
         count = self.count if count is None else count

@@ -17,7 +17,7 @@ runs step on; runToBreaking raises NumericalInstabilityError, as the reference's
 
 Membership changes between intervals: finished runs leave and admitted runs join, and the fleet
 rebuilds its engine from the runs' current states, each run's particles in the same slot order and
-its clock carried over. Because a run's step depends only on its own state and clock (its cells,
+its clock carried over; the runs that stay are copied from the old engine on the device. Because a run's step depends only on its own state and clock (its cells,
 its time step and the fixed launch shape), a run's history is the same bit for bit alone and in any
 mix.
 
@@ -30,9 +30,10 @@ import time as timeModule
 from dataclasses import dataclass, field
 
 import numpy as np
+import torch
 
 from pysurf_amd import _native, readout
-from pysurf_amd.batch import BatchEngine
+from pysurf_amd.batch import BatchEngine, stateFields
 from pysurf_amd.breaking import BreakingTracker, trackInterval
 from pysurf_amd.errors import NumericalInstabilityError
 
@@ -99,12 +100,11 @@ def readingEntry(index: int, member) -> tuple:
 @dataclass
 class Member:
 
-    '''One run in a fleet: its case, its current state and clock, its tracker and its deadlines.'''
+    '''One run in a fleet: its case (its start), its clock, its tracker and its deadlines; its state lives in the fleet's engine.'''
 
     key: object
     case: object
     endTime: float
-    particles: object = None
     time: float = 0.0
     steps: int = 0
     tracker: BreakingTracker = None
@@ -117,8 +117,6 @@ class Member:
     failure: str = ''           # the device's failure of the run ('non-finite', 'time step collapsed')
 
     def __post_init__(self) -> None:
-        if self.particles is None:
-            self.particles = self.case.particles.copy()
         if self.tracker is None:
             self.tracker = trackerOf(self.case)
         self.interval = trackInterval(self.case.parameters.get('depth', 1.0))
@@ -163,38 +161,47 @@ class Fleet:
         self.batch: BatchEngine | None = None
         self.updates = 0
 
-    def _harvest(self) -> None:
-        '''Copy every member's state and clock back from the engine.'''
-        if self.batch is None:
-            return
-        batch = self.batch
-        positions = batch.engine.download('position', self.dimensions)
-        velocities = batch.engine.download('velocity', self.dimensions)
-        densities = batch.engine.download('density')
-        clock = batch.clocks()
-        for index, member in enumerate(self.members):
-            a, b = batch.offsets[index], batch.offsets[index + 1]
-            member.particles.positions = positions[a:b].copy()
-            member.particles.velocities = velocities[a:b].copy()
-            member.particles.densities = densities[a:b].copy()
-            member.time, member.steps = float(clock['time'][index]), int(clock['steps'][index])
-
     def rebuild(self, leaving: list, joining: list) -> None:
-        '''Drop `leaving` members and add `joining` ones, rebuilding the engine from the members' current states.'''
-        self._harvest()
+
+        '''
+
+        Drop `leaving` members and add `joining` ones, building the new engine from the members'
+        current states: the members that stay are copied slot for slot from the old engine on the
+        device (their state never crosses PCIe), the joining ones load from their cases.
+
+        '''
+
+        old = self.batch
+        place = {id(member): index for index, member in enumerate(self.members)}
         keep = [m for m in self.members if m not in leaving]
+        stash, starts = None, {}
+        if old is not None:
+            clock = old.clocks()
+            for index, member in enumerate(self.members):
+                member.time, member.steps = float(clock['time'][index]), int(clock['steps'][index])
+            if keep:
+                # the staying runs' state, gathered on the device into tensors just their size, so the
+                # old engine (its neighbor lists and scratch) is freed before the new one is made
+                old.synchronize()
+                ranges = [(old.offsets[place[id(m)]], old.offsets[place[id(m)] + 1]) for m in keep]
+                with torch.cuda.device(old.device):
+                    stash = {name: torch.cat([old.engine.tensors[name][..., a:b] for a, b in ranges], dim = -1)
+                             for name in stateFields}
+                    torch.cuda.synchronize(old.device)
+                start = 0
+                for member, (a, b) in zip(keep, ranges):
+                    starts[id(member)] = start
+                    start += b - a
         self.members = keep + list(joining)
         self.batch = None
+        del old
         if not self.members:
             return
         block, lanes, capacity = self.shape
-        cases = []
-        for member in self.members:
-            case = member.case
-            snapshot = type(case)(**{**case.__dict__, 'particles': member.particles})
-            cases.append(snapshot)
+        cases = [member.case for member in self.members]
+        carry = [(stash, starts[id(member)]) if id(member) in starts else None for member in self.members]
         self.batch = BatchEngine(cases, self.precision, self.device, blockSize = block, lanes = lanes,
-                                 listCapacity = capacity)
+                                 listCapacity = capacity, carry = carry)
         for index, member in enumerate(self.members):
             self.batch.setClock(index, member.time, member.nextTrack, member.steps, 2**62)
 
@@ -365,6 +372,11 @@ class Server:
     '''
 
     budgetBytes: float = 200e9
+    # the slots of each fleet: at most this many particles in flight in the 2D and in the 3D fleet
+    # (0: no limit), so the runs on the device finish at the pace one batch steps and the rest queue;
+    # a request larger than its fleet's slots runs alone once its fleet is empty
+    slots2D: int = 0
+    slots3D: int = 0
     precision: str = 'float32'
     lanes: int = 4
     chunk: int = 8
@@ -401,9 +413,10 @@ class Server:
         interval's positions on the host while the device steps, so the trackers overlap the GPU.
 
         Admission takes the queue in arrival order and admits every request whose state fits the
-        budget left, so a large request waiting for room does not hold back smaller ones behind it
-        (it keeps its place and is admitted as soon as room frees up). A request whose state alone
-        exceeds the budget can never be admitted: it is refused on arrival.
+        budget left and whose fleet has slots for its particles, so a large request waiting for room
+        does not hold back smaller ones behind it (it keeps its place and is admitted as soon as room
+        frees up). A request whose state alone exceeds the budget can never be admitted: it is
+        refused on arrival.
 
         '''
 
@@ -427,9 +440,14 @@ class Server:
             # Membership changes while every fleet is idle: finished runs leave, admitted runs join
             joining: dict = {}
             still: list = []
+            busy = {d: sum(m.case.count for m in fleet.members if not m.done) for d, fleet in self.fleets.items()}
             for request in queued:
-                if inUse + self.footprint(request.case) <= self.budgetBytes:
+                d = request.case.dimensions
+                slots = self.slots2D if d == 2 else self.slots3D
+                fits = not slots or busy.get(d, 0) == 0 or busy.get(d, 0) + request.case.count <= slots
+                if fits and inUse + self.footprint(request.case) <= self.budgetBytes:
                     inUse += self.footprint(request.case)
+                    busy[d] = busy.get(d, 0) + request.case.count
                     admittedAt[request.key] = now
                     joining.setdefault(request.case.dimensions, []).append(
                         Member(key = request, case = request.case, endTime = request.endTime))
