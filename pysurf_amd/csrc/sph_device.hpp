@@ -200,10 +200,11 @@ __device__ __forceinline__ int64_t cellCoordinate(double v, double cell) {
     return (int64_t)c;
 }
 
-// Calls g(begin, end, middle, right) for the sorted ranges of the 3x3 (2D) or 3x3x3 (3D) cells
-// around cell (cx, cy, cz), x fastest, the three cells of an unwrapped row as one range: [begin,
-// middle) its left cell, [middle, right) its middle one and [right, end) its right one, each
-// ascending in x. A cell taken alone comes as g(begin, end, begin, end).
+// Calls g(begin, end, middle, right, rowY, rowZ, wrapped) for the sorted ranges of the 3x3 (2D) or
+// 3x3x3 (3D) cells around cell (cx, cy, cz), x fastest, the three cells of an unwrapped row as one
+// range: [begin, middle) its left cell, [middle, right) its middle one and [right, end) its right
+// one, each ascending in x. A cell taken alone comes as g(begin, end, begin, end, ...). (rowY,
+// rowZ) are the row's cell coordinates before wrapping, and `wrapped` whether either wrapped.
 template <int D, class G>
 __device__ __forceinline__ void forStencil(const RunDev& run, const int32_t* __restrict__ start, int cx, int cy, int cz,
                                            G&& g) {
@@ -211,17 +212,21 @@ __device__ __forceinline__ void forStencil(const RunDev& run, const int32_t* __r
     const int ky = stencilCount(ny);
     const int kz = D == 3 ? stencilCount(nz) : 1;
     for (int a = 0; a < kz; ++a) {
-        const int64_t plane = D == 3 ? (int64_t)wrapNear(cz + stencilOffset(nz, a), nz) : 0;
+        const int rowZ = D == 3 ? cz + stencilOffset(nz, a) : 0;
+        const int64_t plane = D == 3 ? (int64_t)wrapNear(rowZ, nz) : 0;
         for (int b = 0; b < ky; ++b) {
-            const int64_t row = run.cellFirst + (plane * ny + wrapNear(cy + stencilOffset(ny, b), ny)) * nx;
+            const int rowY = cy + stencilOffset(ny, b);
+            const int wrappedY = wrapNear(rowY, ny);
+            const bool wrapped = wrappedY != rowY || (D == 3 && plane != rowZ);
+            const int64_t row = run.cellFirst + (plane * ny + wrappedY) * nx;
             if (nx >= 3 && cx >= 1 && cx <= nx - 2) {
-                g(start[row + cx - 1], start[row + cx + 2], start[row + cx], start[row + cx + 1]);
+                g(start[row + cx - 1], start[row + cx + 2], start[row + cx], start[row + cx + 1], rowY, rowZ, wrapped);
             } else {
                 const int kx = stencilCount(nx);
                 for (int c = 0; c < kx; ++c) {
                     const int64_t cellIndex = row + wrapNear(cx + stencilOffset(nx, c), nx);
                     const int32_t begin = start[cellIndex], end = start[cellIndex + 1];
-                    g(begin, end, begin, end);
+                    g(begin, end, begin, end, rowY, rowZ, wrapped);
                 }
             }
         }
@@ -235,6 +240,44 @@ __device__ __forceinline__ void forStencil(const RunDev& run, const int32_t* __r
 template <typename Real>
 __device__ __forceinline__ Real rowReach(Real twoHmax, Real px) {
     return twoHmax * Real(1.0 + 1.0 / 1024.0) + fabs(px) * Real(1.0 / 2097152.0);
+}
+
+// The reach along x left in a 3D row of cells at cell coordinates (rowY, rowZ), for a particle at
+// point (py, pz) of cell (cy, cz): the particle's distance across to the row's cells takes its share,
+// reach^2 - gap_y^2 - gap_z^2, each gap less a margin over the cell assignment's rounding and the
+// float32 points' (so no pair within reach is lost); below zero no candidate of the row can be a
+// neighbor, and it returns -1. A wrapped row or a periodic run keeps the whole reach.
+template <typename Real>
+__device__ __forceinline__ Real rowReachAcross(const RunDev& run, Real reach, Real py, Real pz, int cy, int cz, int rowY,
+                                               int rowZ, bool wrapped) {
+    if (wrapped || run.period > 0.0 || (rowY == cy && rowZ == cz)) return reach;
+    const Real cell = (Real)run.cell;
+    const Real margin = cell * Real(1.0 / 4096.0) + (fabs(py) + fabs(pz)) * Real(1.0 / 1048576.0);
+    Real gy = Real(0), gz = Real(0);
+    if (rowY < cy) gy = py - (Real)cy * cell;
+    else if (rowY > cy) gy = (Real)(cy + 1) * cell - py;
+    if (rowZ < cz) gz = pz - (Real)cz * cell;
+    else if (rowZ > cz) gz = (Real)(cz + 1) * cell - pz;
+    gy = fmax(gy - margin, Real(0));
+    gz = fmax(gz - margin, Real(0));
+    const Real left = reach * reach - gy * gy - gz * gz;
+    return left > Real(0) ? sqrt(left) : Real(-1);
+}
+
+// A 3D row's candidates [from, to) within `across` of x (the row ascending in x): the left cell's
+// and right cell's ends searched while the row keeps the whole reach, the whole row otherwise.
+template <typename Real, class X>
+__device__ __forceinline__ void rowCandidates(int32_t begin, int32_t end, int32_t middle, int32_t right, Real px,
+                                              Real reach, Real across, X&& pointX, int32_t& from, int32_t& to) {
+    if (across < Real(0)) {
+        from = to = begin;
+    } else if (across == reach) {
+        from = firstAtLeast(begin, middle, px - reach, pointX);
+        to = firstAbove(right, end, px + reach, pointX);
+    } else {
+        from = firstAtLeast(begin, end, px - across, pointX);
+        to = firstAbove(from, end, px + across, pointX);
+    }
 }
 
 template <typename Real, class X>
@@ -651,10 +694,14 @@ __device__ __forceinline__ void forNeighbors(const RunDev& run, const int32_t* _
             oy = wrapIndex(cellCoordinate(y64 + (double)shift, run.cell), run.ny);
         }
         const Real reach = rowReach(twoHmax, me.p[0]);
-        forStencil<D>(run, start, cx, oy, cz, [&](int32_t begin, int32_t end, int32_t middle, int32_t right) {
+        forStencil<D>(run, start, cx, oy, cz, [&](int32_t begin, int32_t end, int32_t middle, int32_t right, int rowY,
+                                                   int rowZ, bool wrapped) {
             auto pointX = [&](int32_t k) { return src.pointX(k); };
-            const int32_t from = D == 3 ? firstAtLeast(begin, middle, me.p[0] - reach, pointX) : begin;
-            const int32_t to = D == 3 ? firstAbove(right, end, me.p[0] + reach, pointX) : end;
+            int32_t from = begin, to = end;
+            if (D == 3)
+                rowCandidates(begin, end, middle, right, me.p[0], reach,
+                              rowReachAcross(run, reach, me.p[1], me.p[2], cy, cz, rowY, rowZ, wrapped || image == 1),
+                              pointX, from, to);
             for (int32_t k = from + lane; k < to; k += Lanes) {
                 if (k == s) continue;
                 const Record<Real> other = src(k);

@@ -703,13 +703,17 @@ __global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
             }
             const int32_t tag = image ? kImageBit : 0;
             const Real reach = rowReach(twoHmax, me.p[0]);
-            forStencil<D>(run, start, cx, oy, cz, [&](int32_t begin, int32_t end, int32_t middle, int32_t right) {
+            forStencil<D>(run, start, cx, oy, cz, [&](int32_t begin, int32_t end, int32_t middle, int32_t right,
+                                                       int rowY, int rowZ, bool wrapped) {
                 const int32_t copied = cells.rangeBase(begin, end);
                 auto pointX = [&](int32_t k) {
                     return copied >= 0 ? ldsValue<Real>(copied + (k - begin), 0, 0, cached) : rec.r[k].q0.x;
                 };
-                const int32_t from = D == 3 ? firstAtLeast(begin, middle, me.p[0] - reach, pointX) : begin;
-                const int32_t to = D == 3 ? firstAbove(right, end, me.p[0] + reach, pointX) : end;
+                int32_t from = begin, to = end;
+                if (D == 3)
+                    rowCandidates(begin, end, middle, right, me.p[0], reach,
+                                  rowReachAcross(run, reach, me.p[1], me.p[2], cy, cz, rowY, rowZ, wrapped || image == 1),
+                                  pointX, from, to);
                 // the predicate's fields only: the point and remainder, and the length when lengths
                 // differ; a lane past the range reads the range's last record and never hits, so
                 // the loop body has no branch but the predicate's rare rounded root. One loop per
