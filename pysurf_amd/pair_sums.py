@@ -881,7 +881,7 @@ class PairSums:
 
     def velocityGradients(self, velocities: np.ndarray, densities: np.ndarray) -> np.ndarray:
         '''G for every particle, shape (N, dim, dim) (sphGpu.py:1827-1839).'''
-        if not (self.limitViscosity or self.closure['enabled']):
+        if not self.needsVelocityGradients:
             raise ValueError('Neither the viscosity limiter nor the closure is on; see setViscosityLimiter, setClosure.')
         self._sync()
         engine = self.engine
@@ -898,9 +898,19 @@ class PairSums:
         '''Whether the diffusive term subtracts the renormalized density gradients (sphGpu.py:1654-1657).'''
         return self.diffusion['mode'] == diffusionModes['antuono']
 
+    @property
+    def needsCorrections(self) -> bool:
+        '''Whether a step builds L: for the continuity correction or the density gradients (sphGpu.py:1659-1662).'''
+        return self.correctContinuity or self.renormalizedDiffusion
+
+    @property
+    def needsVelocityGradients(self) -> bool:
+        '''Whether a step builds G: for the viscosity limiter or the closure (sphGpu.py:1664-1667).'''
+        return self.limitViscosity or self.closure['enabled']
+
     def correctionMatrices(self, densities: np.ndarray) -> np.ndarray:
         '''L for every particle, shape (N, dim, dim) (sphGpu.py:1855-1866).'''
-        if not (self.correctContinuity or self.renormalizedDiffusion):
+        if not self.needsCorrections:
             raise ValueError('Neither the continuity correction nor the renormalized diffusion is on.')
         self._sync()
         engine = self.engine

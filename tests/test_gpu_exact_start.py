@@ -4,7 +4,9 @@ Grilli's 1:35 case from the exact solitary wave (pysurf_amd.scenarios.exactSolit
 reference's BreakingWave 'deltaSph' start: bed-fitted columns, Wendland kernel at h = 3 spacings),
 stepped on the GPU in float32 pairs at 64 particles per depth and read by the smoothed-surface
 tracker until the jet's touchdown fixes the breaking point. It must break where the reference's own
-GPU run recorded it (tests/golden/grilli_breaking_64.json, from tests/golden/make_grilli_record.py):
+GPU run recorded it (tests/golden/grilli_breaking_64.json, from tests/golden/make_grilli_record.py),
+and at 128 particles per depth with the renormalized 'antuono' diffusion, where the reference's run
+with it recorded it (tests/golden/grilli_breaking_128_antuono.json):
 the time, crest height and depths within 1% (the recorded run's own scatter under rounding-level
 changes is a few tenths of a percent on this measure), and within the reference's criteria against
 Grilli et al. (1997): Hb/h0 within 10%, hb/h0 within 15%.
@@ -19,16 +21,21 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'golden', 'grilli_breaking_64.json')
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'golden')
+# (particles per depth, density diffusion, the recorded run)
+RECORDS = [(64, 'hydrostatic', 'grilli_breaking_64.json'), (128, 'antuono', 'grilli_breaking_128_antuono.json')]
 
 
-@pytest.mark.timeout(600)
-def testTheGrilliCaseBreaksWhereTheReferenceRecordedIt():
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize('ppd, diffusion, record', RECORDS)
+def testTheGrilliCaseBreaksWhereTheReferenceRecordedIt(ppd, diffusion, record):
     from pysurf_amd.scenarios import exactSolitaryOnSlope
     from pysurf_amd.serving import Fleet, Member
-    golden = json.load(open(GOLDEN))
+    golden = json.load(open(os.path.join(GOLDEN, record)))
     recorded, grilli = golden['recorded'], golden['grilli1997']
-    case = exactSolitaryOnSlope(64)
+    print(f'Grilli 1:35 at {ppd} ppd, {diffusion} diffusion', flush = True)
+    case = exactSolitaryOnSlope(ppd)
+    case.diffusion = diffusion
     assert case.fluidCount == recorded['fluidParticles']
     fleet = Fleet(2, 'float32', lanes = 4, listCapacity = 160, chunk = 8)
     member = Member(key = 0, case = case, endTime = 12.0)

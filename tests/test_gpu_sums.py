@@ -344,3 +344,26 @@ def testHostStepSumsRefuseRunsWithOtherOptions():
     assert engine.library.sph_build_cells(engine.handle) == _native.SPH_OK
     assert engine.library.sph_velocity_gradients(engine.handle, engine.pointer('velocity'), engine.pointer('density'),
                                                  engine.pointer('matrix')) == _native.SPH_OK
+
+
+def testWhichMatricesAStepNeeds():
+    # needsCorrections: the continuity correction or the 'antuono' diffusion; needsVelocityGradients:
+    # the viscosity limiter or the closure (sphGpu.py:1659-1667); the host steps refuse G and L when
+    # neither is on
+    from pysurf_amd.pair_sums import PairSums
+    sums = PairSums(2, 0.05, np.zeros(2), np.ones(2))
+    assert not sums.needsCorrections and not sums.needsVelocityGradients
+    sums.setContinuityCorrection(True)
+    assert sums.needsCorrections
+    sums.setContinuityCorrection(False)
+    sums.setDiffusion('antuono', 0.1, 20.0, 1000.0, 9.81)
+    assert sums.needsCorrections
+    sums.setViscosityLimiter(True)
+    assert sums.needsVelocityGradients
+    sums.setViscosityLimiter(False)
+    sums.setClosure(True, soundSpeed = 20.0)
+    assert sums.needsVelocityGradients
+    sums.setClosure(False)
+    assert not sums.needsVelocityGradients
+    with pytest.raises(ValueError):
+        sums.velocityGradients(np.zeros((0, 2)), np.zeros(0))
