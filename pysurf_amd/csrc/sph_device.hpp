@@ -242,44 +242,6 @@ __device__ __forceinline__ Real rowReach(Real twoHmax, Real px) {
     return twoHmax * Real(1.0 + 1.0 / 1024.0) + fabs(px) * Real(1.0 / 2097152.0);
 }
 
-// The reach along x left in a 3D row of cells at cell coordinates (rowY, rowZ), for a particle at
-// point (py, pz) of cell (cy, cz): the particle's distance across to the row's cells takes its share,
-// reach^2 - gap_y^2 - gap_z^2, each gap less a margin over the cell assignment's rounding and the
-// float32 points' (so no pair within reach is lost); below zero no candidate of the row can be a
-// neighbor, and it returns -1. A wrapped row or a periodic run keeps the whole reach.
-template <typename Real>
-__device__ __forceinline__ Real rowReachAcross(const RunDev& run, Real reach, Real py, Real pz, int cy, int cz, int rowY,
-                                               int rowZ, bool wrapped) {
-    if (wrapped || run.period > 0.0 || (rowY == cy && rowZ == cz)) return reach;
-    const Real cell = (Real)run.cell;
-    const Real margin = cell * Real(1.0 / 4096.0) + (fabs(py) + fabs(pz)) * Real(1.0 / 1048576.0);
-    Real gy = Real(0), gz = Real(0);
-    if (rowY < cy) gy = py - (Real)cy * cell;
-    else if (rowY > cy) gy = (Real)(cy + 1) * cell - py;
-    if (rowZ < cz) gz = pz - (Real)cz * cell;
-    else if (rowZ > cz) gz = (Real)(cz + 1) * cell - pz;
-    gy = fmax(gy - margin, Real(0));
-    gz = fmax(gz - margin, Real(0));
-    const Real left = reach * reach - gy * gy - gz * gz;
-    return left > Real(0) ? sqrt(left) : Real(-1);
-}
-
-// A 3D row's candidates [from, to) within `across` of x (the row ascending in x): the left cell's
-// and right cell's ends searched while the row keeps the whole reach, the whole row otherwise.
-template <typename Real, class X>
-__device__ __forceinline__ void rowCandidates(int32_t begin, int32_t end, int32_t middle, int32_t right, Real px,
-                                              Real reach, Real across, X&& pointX, int32_t& from, int32_t& to) {
-    if (across < Real(0)) {
-        from = to = begin;
-    } else if (across == reach) {
-        from = firstAtLeast(begin, middle, px - reach, pointX);
-        to = firstAbove(right, end, px + reach, pointX);
-    } else {
-        from = firstAtLeast(begin, end, px - across, pointX);
-        to = firstAbove(from, end, px + across, pointX);
-    }
-}
-
 template <typename Real, class X>
 __device__ __forceinline__ int32_t firstAtLeast(int32_t lo, int32_t hi, Real bound, X&& x) {
     while (lo < hi) {
@@ -296,6 +258,64 @@ __device__ __forceinline__ int32_t firstAbove(int32_t lo, int32_t hi, Real bound
         if (x(mid) <= bound) lo = mid + 1; else hi = mid;
     }
     return lo;
+}
+
+// The lower edge of a particle's own cell along one axis: its cell coordinate c is the unwrapped
+// floor(p / cell) taken modulo the n cells of the axis, so the unwrapped one is c + k n with k the
+// nearest integer to (p / cell - c - 1/2) / n (within 1/2 - 1/(2n) of k for n >= 3).
+template <typename Real>
+__device__ __forceinline__ Real cellLowerEdge(Real p, int c, int32_t n, Real cell) {
+    const Real turns = rint((p / cell - (Real)c - Real(0.5)) / (Real)n);
+    return ((Real)c + turns * (Real)n) * cell;
+}
+
+// The reach along x left in a 3D row of cells at cell coordinates (rowY, rowZ), for a particle at
+// point (py, pz) of cell (cy, cz): the particle's distance across to the row's cells takes its share,
+// reach^2 - gap_y^2 - gap_z^2, each gap less a margin over the float32 points' rounding (so no pair
+// within reach is lost); below zero no candidate of the row can be a neighbor, and it returns -1. A
+// wrapped row, a periodic run or an axis of fewer than three cells (where one row of cells lies on
+// both sides of the particle) keeps the whole reach.
+template <typename Real>
+__device__ __forceinline__ Real rowReachAcross(const RunDev& run, Real reach, Real py, Real pz, int cy, int cz, int rowY,
+                                               int rowZ, bool wrapped) {
+    if (wrapped || run.period > 0.0 || run.ny < 3 || run.nz < 3 || (rowY == cy && rowZ == cz)) return reach;
+    const Real cell = (Real)run.cell;
+    const Real margin = cell * Real(1.0 / 4096.0) + (fabs(py) + fabs(pz)) * Real(1.0 / 1048576.0);
+    Real gy = Real(0), gz = Real(0);
+    if (rowY != cy) {
+        const Real low = cellLowerEdge(py, cy, run.ny, cell);
+        gy = rowY < cy ? py - low : low + cell - py;
+    }
+    if (rowZ != cz) {
+        const Real low = cellLowerEdge(pz, cz, run.nz, cell);
+        gz = rowZ < cz ? pz - low : low + cell - pz;
+    }
+    gy = fmax(gy - margin, Real(0));
+    gz = fmax(gz - margin, Real(0));
+    const Real left = reach * reach - gy * gy - gz * gz;
+    return left > Real(0) ? sqrt(left) : Real(-1);
+}
+
+// A 3D row's candidates [from, to) within `across` of x: the left cell's from the first whose point
+// x is at least x_i - across and the right cell's up to the last at most x_i + across (each cell
+// ascending in x), the middle cell whole; with a reduced reach the whole row is searched when it
+// ascends across its cells too, as it does while the particles lie within the grid's extent.
+template <typename Real, class X>
+__device__ __forceinline__ void rowCandidates(int32_t begin, int32_t end, int32_t middle, int32_t right, Real px,
+                                              Real reach, Real across, X&& pointX, int32_t& from, int32_t& to) {
+    if (across < Real(0)) {
+        from = to = begin;
+        return;
+    }
+    const bool ascending = across < reach && (middle == begin || middle == end || pointX(middle - 1) <= pointX(middle)) &&
+                           (right == begin || right == end || pointX(right - 1) <= pointX(right));
+    if (ascending) {
+        from = firstAtLeast(begin, end, px - across, pointX);
+        to = firstAbove(from, end, px + across, pointX);
+    } else {
+        from = firstAtLeast(begin, middle, px - across, pointX);
+        to = firstAbove(right, end, px + across, pointX);
+    }
 }
 
 template <int D, typename Real>

@@ -298,6 +298,29 @@ def testNeighborSetsAreBitExact(precision, dimensions, periodic):
 
 
 @pytest.mark.parametrize('precision', precisions)
+@pytest.mark.parametrize('placement', ['straddling the origin', 'beyond the grid'])
+def testThreeDimensionalRowsKeepEveryNeighbor(precision, placement):
+    # 3D rows are cut along x by the particle's distance across to each row, from its unwrapped cell,
+    # and searched as one ascending range only where the row ascends: a cloud across the origin (cell
+    # coordinates of both signs) and one given a grid narrower than itself (cells holding particles
+    # of several wrapped cells) keep the brute-force neighbor sets
+    print(f'3D rows {placement} {precision}', flush = True)
+    case = cloud(3)
+    positions = case['positions'] - 25.0 - 0.5 * (case['positions'].max(axis = 0) - 25.0)
+    from pysurf_amd.pair_sums import PairSums
+    low, high = positions.min(axis = 0), positions.max(axis = 0)
+    if placement == 'beyond the grid':
+        high = low + 0.4 * (high - low)
+    sums = PairSums(3, case['h'], low, high, 0.0, 'float64', 'cuda:0', case['kernelType'], pairPrecision = precision)
+    sums.update(positions, case['masses'], case['isFluid'], None)
+    offsets, rows = sums.neighborLists()
+    expectedOffsets, expectedRows = oracle.neighbors(physicsFor(dict(case, positions = positions)), positions,
+                                                     np.full(len(positions), case['h']), precision)
+    assert np.array_equal(offsets, expectedOffsets)
+    assert np.array_equal(rows, expectedRows)
+
+
+@pytest.mark.parametrize('precision', precisions)
 def testMixedLengthsMatchTheOracle(precision):
     # Particles of two smoothing lengths pair at h_ij = (h_i + h_j) / 2 (sphGpu.py:52-56)
     case = cloud(2, False, kernelType = 'wendlandC2')
