@@ -84,6 +84,16 @@ def kernelBytes(dimensions: int, real: int, entries: float) -> dict:
     }
 
 
+# Useful float32 operations per neighbor-list entry of the 2D list build: the predicate on the pair
+# that makes the entry (the separation from points and remainders 6, the squared distance 3, the
+# compare 1) and sweep 1's sums of a fluid particle on it (the root 4, the Wendland slope over r 8,
+# the volume 1, the velocity difference and the weighted separation 6, G's rises and offsets 16), as
+# read from buildNeighbors and Sweep1::add in pysurf_amd/csrc/sph_hip.hip. Candidates that miss are
+# not counted: their tests are the work the eighths tables exist to avoid.
+FLOP_PER_ENTRY = {('buildNeighbors', 2): 51.0}
+FP32_PEAK_TFLOPS = 157.3
+
+
 def parse():
     parser = argparse.ArgumentParser(description = __doc__.split('\n\n')[1])
     parser.add_argument('--gpus', type = int, default = 1)
@@ -100,7 +110,7 @@ def parse():
     parser.add_argument('--plain-steps', type = int, default = 200)
     parser.add_argument('--single-steps', type = int, default = 1000, help = 'steps of the single 64 ppd run')
     parser.add_argument('--no-single', action = 'store_true', help = 'skip the single 64 ppd run')
-    parser.add_argument('--ridge-steps', type = int, default = 10, help = 'timed steps of the 3D ridge run')
+    parser.add_argument('--ridge-steps', type = int, default = 100, help = 'timed steps of the 3D ridge run')
     parser.add_argument('--no-ridge', action = 'store_true', help = 'skip the 3D ridge run')
     parser.add_argument('--serving-window', type = float, default = 4.0, help = 'wall seconds of Poisson arrivals')
     parser.add_argument('--serving-pool', type = int, default = 40, help = 'distinct runs the requests draw from')
@@ -112,6 +122,10 @@ def parse():
     parser.add_argument('--no-gate', action = 'store_true', help = "skip Grilli's 1:35 case from the exact start")
     parser.add_argument('--gate-only', action = 'store_true',
                         help = "run only Grilli's case and print its record (for profiling the breaking read-out)")
+    parser.add_argument('--single-only', action = 'store_true',
+                        help = 'run only the single 64 ppd run and print its record (for profiling its launches)')
+    parser.add_argument('--ridge-only', action = 'store_true',
+                        help = 'run only the 3D ridge run and print its record (for profiling the 3D step)')
     parser.add_argument('--gate-end', type = float, default = 12.0,
                         help = "simulated seconds Grilli's case may run before its touchdown")
     parser.add_argument('--cpu-seconds', type = float, default = 15.0)
@@ -210,8 +224,14 @@ def roofline(batch, profileSteps: int, precision: str) -> dict:
     stepMs = sum(entry['ms'] for entry in profile.values()) / profileSteps
     log(f'dominant kernel {dominant}: {launchMs * 1000:.1f} us/launch, {achieved:.1f} GB/s of algorithmic bytes')
     measured = MEASURED_TRAFFIC.get((dominant, batch.launch.get('lanes'), batch.count)) if precision == 'float32' else None
+    flops = FLOP_PER_ENTRY.get((dominant, batch.dimensions)) if precision == 'float32' else None
+    valu = None
+    if flops:
+        tflops = flops * entries * batch.count / (launchMs / 1000.0) / 1e12
+        valu = {'flop_per_entry': flops, 'achieved': tflops, 'peak': FP32_PEAK_TFLOPS, 'unit': 'TFLOP/s',
+                'frac': tflops / FP32_PEAK_TFLOPS}
     return {'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS,
-            'traffic': measured, 'kernel': dominant, 'launch_us': launchMs * 1000.0,
+            'traffic': measured, 'valu': valu, 'kernel': dominant, 'launch_us': launchMs * 1000.0,
             'algorithmic_bytes_per_launch': traffic, 'bytes_per_particle': perParticle,
             'mean_list_entries': entries, 'particles_per_launch': batch.count,
             'step_algorithmic_bytes_per_particle': sum(table.values()),
@@ -248,8 +268,11 @@ def singleRun(args) -> dict:
         batch = BatchEngine([case], precision, tune = not args.no_tune)
         warm(batch, args.warmup, args.chunk)
         wall, _ = timedSteps(batch, args.single_steps, args.chunk)
+        profile = kernelProfile(batch, 20)
         result[precision] = {'value': case.fluidCount * args.single_steps / wall,
-                             'ms_per_step': 1000.0 * wall / args.single_steps, 'lanes': batch.launch['lanes']}
+                             'ms_per_step': 1000.0 * wall / args.single_steps, 'lanes': batch.launch['lanes'],
+                             'kernel_us_per_step': {name: 1000.0 * entry['ms'] / 20 for name, entry in profile.items()
+                                                    if entry['launches']}}
         log(f'single run {precision}: {result[precision]["ms_per_step"]:.4f} ms/step')
         del batch
     result['speedup_of_f32'] = result['float32']['value'] / result['float64']['value']
@@ -485,6 +508,12 @@ def main() -> int:
         return copyCalibration(args.copy_calibration)
     if args.gate_only:
         print(json.dumps(gateRun(args)))
+        return 0
+    if args.single_only:
+        print(json.dumps(singleRun(args)))
+        return 0
+    if args.ridge_only:
+        print(json.dumps(ridgeRun(args)))
         return 0
 
     import numpy as np

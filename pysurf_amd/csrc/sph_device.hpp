@@ -269,16 +269,16 @@ __device__ __forceinline__ Real cellLowerEdge(Real p, int c, int32_t n, Real cel
     return ((Real)c + turns * (Real)n) * cell;
 }
 
-// The reach along x left in a 3D row of cells at cell coordinates (rowY, rowZ), for a particle at
+// The reach along x left in a row of cells at cell coordinates (rowY, rowZ), for a particle at
 // point (py, pz) of cell (cy, cz): the particle's distance across to the row's cells takes its share,
 // reach^2 - gap_y^2 - gap_z^2, each gap less a margin over the float32 points' rounding (so no pair
 // within reach is lost); below zero no candidate of the row can be a neighbor, and it returns -1. A
 // wrapped row, a periodic run or an axis of fewer than three cells (where one row of cells lies on
-// both sides of the particle) keeps the whole reach.
-template <typename Real>
+// both sides of the particle) keeps the whole reach. In 2D rows differ in y alone.
+template <int D, typename Real>
 __device__ __forceinline__ Real rowReachAcross(const RunDev& run, Real reach, Real py, Real pz, int cy, int cz, int rowY,
                                                int rowZ, bool wrapped) {
-    if (wrapped || run.period > 0.0 || run.ny < 3 || run.nz < 3 || (rowY == cy && rowZ == cz)) return reach;
+    if (wrapped || run.period > 0.0 || run.ny < 3 || (D == 3 && run.nz < 3) || (rowY == cy && rowZ == cz)) return reach;
     const Real cell = (Real)run.cell;
     const Real margin = cell * Real(1.0 / 4096.0) + (fabs(py) + fabs(pz)) * Real(1.0 / 1048576.0);
     Real gy = Real(0), gz = Real(0);
@@ -286,7 +286,7 @@ __device__ __forceinline__ Real rowReachAcross(const RunDev& run, Real reach, Re
         const Real low = cellLowerEdge(py, cy, run.ny, cell);
         gy = rowY < cy ? py - low : low + cell - py;
     }
-    if (rowZ != cz) {
+    if (D == 3 && rowZ != cz) {
         const Real low = cellLowerEdge(pz, cz, run.nz, cell);
         gz = rowZ < cz ? pz - low : low + cell - pz;
     }
@@ -316,6 +316,77 @@ __device__ __forceinline__ void rowCandidates(int32_t begin, int32_t end, int32_
         from = firstAtLeast(begin, middle, px - across, pointX);
         to = firstAbove(right, end, px + across, pointX);
     }
+}
+
+// EighthTable: each cell's slots, ordered by x, split by the eighth of the cell's width along x
+// their x lies in. Byte k (k = 1 .. 7) of a cell's table is how many of its slots lie in its first
+// k eighths, so eighth k's slots are [start + byte k, start + byte k+1) with byte 0 = 0 and byte 8
+// the cell's count; kEighthsUnknown (byte 0 nonzero) marks a cell whose table does not hold: one of
+// its slots lies outside the grid's cells along x (its eighth is not ordered with the others') or
+// it holds more than 255. Built by adding kEighthsAbove << 8e for each slot of eighth e.
+constexpr uint8_t kOutsideX = 0x80;
+constexpr unsigned long long kEighthsAbove = 0x0101010101010100ull;
+constexpr unsigned long long kEighthsUnknown = 0xffull;
+
+// The sorted position where eighth k (0 .. 8) of a cell [begin, end) starts; an unknown table
+// answers the cell's begin (`late` false) or its end (`late` true), the safe side for each use.
+__device__ __forceinline__ int32_t eighthStart(unsigned long long table, int32_t begin, int32_t end, int k, bool late) {
+    if (k <= 0) return begin;
+    if (k >= 8) return end;
+    if (table & 0xffull) return late ? end : begin;
+    const int32_t at = begin + (int32_t)((table >> (8 * k)) & 0xffull);
+    return at < end ? at : end;
+}
+
+// Where eighth e (0 .. 24) of a row of three cells starts: eighths 0-7 are the left cell's
+// [begin, middle), 8-15 the middle cell's [middle, right), 16-23 the right cell's [right, end), and
+// 24 is the row's end; `tables` holds the three cells' tables in that order.
+__device__ __forceinline__ int32_t rowEighthStart(int e, bool late, int32_t begin, int32_t middle, int32_t right,
+                                                  int32_t end, const unsigned long long (&tables)[3]) {
+    if (e >= 24) return end;
+    const int c = e >> 3;
+    const int32_t b = c == 0 ? begin : (c == 1 ? middle : right);
+    const int32_t f = c == 0 ? middle : (c == 1 ? right : end);
+    const unsigned long long table = c == 0 ? tables[0] : (c == 1 ? tables[1] : tables[2]);
+    return eighthStart(table, b, f, e & 7, late);
+}
+
+// The candidates of an unwrapped row of three cells (left cell [begin, middle), middle
+// [middle, right), right [right, end), x ascending within each) that can lie within `across` of x
+// along x, from whole eighths of the cells: every slot of an eighth that ends before px - across
+// or starts after px + across is left out, and nothing else. Slots of the left cell lie left of
+// the middle cell's and those of the right cell right of it, except slots from outside the grid
+// along x, which lie at least the grid's width minus two cells away and are no neighbors. `rowLeft`
+// is the left cell's lower edge as the particle's own cell coordinate places it, `tables` the three
+// cells' tables. With `Refine`, a binary search within the first and last eighths then bounds the
+// range exactly as firstAtLeast and firstAbove would over the row (the 3D list build refines).
+template <bool Refine, typename Real, class X>
+__device__ __forceinline__ void rowEighths(int32_t begin, int32_t middle, int32_t right, int32_t end, Real px,
+                                           Real across, Real rowLeft, Real cell, const unsigned long long (&tables)[3],
+                                           X&& pointX, int32_t& from, int32_t& to) {
+    if (across < Real(0)) {
+        from = to = begin;
+        return;
+    }
+    // a sixteenth of an eighth, and the float rounding of x, keep the eighths' cuts on the safe side
+    const Real scale = Real(8) / cell;
+    const Real margin = Real(1.0 / 16.0) + (fabs(px) + cell) * scale * Real(1.0 / 1048576.0);
+    const Real low = px - across, high = px + across;
+    const Real lowAt = floor((low - rowLeft) * scale - margin);
+    const Real highAt = floor((high - rowLeft) * scale + margin);
+    const int first = lowAt < Real(0) ? 0 : (lowAt > Real(24) ? 24 : (int)lowAt);
+    const int last = highAt < Real(-1) ? -1 : (highAt > Real(23) ? 23 : (int)highAt);
+    int32_t lo = rowEighthStart(first, false, begin, middle, right, end, tables);
+    int32_t hi = last < first ? lo : rowEighthStart(last + 1, true, begin, middle, right, end, tables);
+    if constexpr (Refine) {
+        if (hi > lo) {
+            lo = firstAtLeast(lo, rowEighthStart(first + 1, true, begin, middle, right, end, tables), low, pointX);
+            const int32_t lastFrom = rowEighthStart(last, false, begin, middle, right, end, tables);
+            hi = firstAbove(lastFrom > lo ? lastFrom : lo, hi, high, pointX);
+        }
+    }
+    from = lo;
+    to = hi < lo ? lo : hi;
 }
 
 template <int D, typename Real>
@@ -720,7 +791,7 @@ __device__ __forceinline__ void forNeighbors(const RunDev& run, const int32_t* _
             int32_t from = begin, to = end;
             if (D == 3)
                 rowCandidates(begin, end, middle, right, me.p[0], reach,
-                              rowReachAcross(run, reach, me.p[1], me.p[2], cy, cz, rowY, rowZ, wrapped || image == 1),
+                              rowReachAcross<D>(run, reach, me.p[1], me.p[2], cy, cz, rowY, rowZ, wrapped || image == 1),
                               pointX, from, to);
             for (int32_t k = from + lane; k < to; k += Lanes) {
                 if (k == s) continue;

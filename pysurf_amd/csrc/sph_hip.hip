@@ -179,12 +179,16 @@ __device__ __forceinline__ double runTimeStep(const RunDev& run, const Clock& cl
 // Each run's maxima from the per-slot magnitudes the step wrote (|v| at the gather, |a| in the
 // acceleration sweep; zero on boundary particles): a maximum over each chunk of a run's slots,
 // then over the run's chunks, written as the run's float64 maxima. No atomics, so no waves queue on
-// one address, and a maximum does not depend on the order it is taken in.
-__global__ void __launch_bounds__(256)
+// one address, and a maximum does not depend on the order it is taken in. A chunk takes a workgroup
+// of kMaximaThreads, so each thread reads 8 slots of the chunk's 8192: a small run has few chunks,
+// and the loop's length is the kernel's.
+constexpr int kMaximaThreads = 1024;
+
+__global__ void __launch_bounds__(kMaximaThreads)
     chunkMaxima(const int32_t* __restrict__ chunkRun, const int64_t* __restrict__ chunkFirst, const RunDev* __restrict__ runs,
                 const double* __restrict__ speed, const double* __restrict__ acceleration, Clock clock,
                 double* __restrict__ partial) {
-    __shared__ double top[2][256];
+    __shared__ double top[2][kMaximaThreads];
     const int64_t b = blockIdx.x;
     const int r = chunkRun[b];
     if (!clock.active[r]) return;
@@ -258,13 +262,14 @@ __device__ __forceinline__ void equalRuns(int32_t key, bool valid, int& head, in
     rank = lane - head;
 }
 
-// Each slot's cell and the cells' counts. With `clockMode` the first slot of each run also opens the
-// run's step: 1 sets its active flag from its clock, 2 makes it active whatever the clock; both
-// clear its maxima.
+// Each slot's cell and the cells' counts, and with `eighth` the slot's eighth of its cell along x
+// (0 .. 7, with kOutsideX set when x lies outside the grid's cells or is not finite). With `clockMode` the first
+// slot of each run also opens the run's step: 1 sets its active flag from its clock, 2 makes it
+// active whatever the clock; both clear its maxima.
 template <int D>
 __global__ void cellKeys(const RunDev* __restrict__ runs, const int32_t* __restrict__ runOf, const double* __restrict__ x,
                          const double* __restrict__ y, const double* __restrict__ z, int64_t n, int32_t* __restrict__ key,
-                         int32_t* __restrict__ count, Clock clock, int32_t clockMode) {
+                         int32_t* __restrict__ count, uint8_t* __restrict__ eighth, Clock clock, int32_t clockMode) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = i < n;
     int32_t k = 0;
@@ -277,11 +282,17 @@ __global__ void cellKeys(const RunDev* __restrict__ runs, const int32_t* __restr
             clock.maxima[2 * r] = 0ull;
             clock.maxima[2 * r + 1] = 0ull;
         }
-        const int cx = wrapIndex(cellCoordinate(x[i], run.cell), run.nx);
+        const int64_t ux = cellCoordinate(x[i], run.cell);
+        const int cx = wrapIndex(ux, run.nx);
         const int cy = wrapIndex(cellCoordinate(y[i], run.cell), run.ny);
         const int cz = D == 3 ? wrapIndex(cellCoordinate(z[i], run.cell), run.nz) : 0;
         k = (int32_t)(run.cellFirst + ((int64_t)cz * run.ny + cy) * run.nx + cx);
         key[i] = k;
+        if (eighth) {
+            const double within = x[i] / run.cell - (double)ux;   // in [0, 1) for a finite x
+            const bool inside = ux == (int64_t)cx && within >= 0.0 && within < 1.0;
+            eighth[i] = inside ? (uint8_t)min((int)(within * 8.0), 7) : kOutsideX;
+        }
     }
     int head, length, rank;
     equalRuns(k, valid, head, length, rank);
@@ -303,13 +314,16 @@ __global__ void cellScatter(const int32_t* __restrict__ key, const int32_t* __re
 
 // Orders each cell's slots by x, ties by slot, or by slot alone when `x` is null: the slot at each
 // scattered place goes to its cell's start plus the number of the cell's slots before it, one
-// thread per place, so a cell of k slots costs k reads per slot rather than a serial sort.
-// Ascending x lets a 3D walk bound its candidates along x by a binary search (firstAtLeast,
-// firstAbove); a 2D cell holds too few for that to pay, and keeps slot order. The first `cells`
-// threads also clear the cells' counters for the next build.
+// thread per place, so a cell of k slots costs k reads per slot rather than a serial sort. With
+// `eighths`, the cell's first slot in that order also writes the cell's eighths table (EighthTable):
+// how many of the cell's slots lie in its first k eighths along x, k = 1 .. 7, one byte each, which
+// bounds a row's candidates along x before a search within one eighth. 3D cells are ordered by x and
+// take tables; a 2D cell holds about 7 slots, too few for bounds along x to repay their loads, and
+// keeps slot order. The first `cells` threads also clear the cells' counters for the next build.
 __global__ void cellRank(const int32_t* __restrict__ start, const int32_t* __restrict__ key, const double* __restrict__ x,
-                         const int32_t* __restrict__ scattered, int32_t* __restrict__ order, int32_t* __restrict__ count,
-                         int32_t* __restrict__ fill, int64_t n, int64_t cells) {
+                         const uint8_t* __restrict__ eighth, const int32_t* __restrict__ scattered,
+                         int32_t* __restrict__ order, unsigned long long* __restrict__ eighths,
+                         int32_t* __restrict__ count, int32_t* __restrict__ fill, int64_t n, int64_t cells) {
     const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (a < n) {
         const int32_t v = scattered[a];
@@ -318,11 +332,20 @@ __global__ void cellRank(const int32_t* __restrict__ start, const int32_t* __res
         int32_t below = 0;
         if (x) {
             const double xv = x[v];
+            unsigned long long table = 0ull;
+            uint32_t outside = 0u;
             for (int32_t b = begin; b < end; ++b) {
                 const int32_t u = scattered[b];
                 const double xu = x[u];
                 below += xu < xv || (xu == xv && u < v);
+                if (eighths) {
+                    const uint32_t e = eighth[u];
+                    outside |= e & kOutsideX;
+                    table += kEighthsAbove << (8u * (e & 7u));
+                }
             }
+            if (eighths && below == 0)
+                eighths[c] = outside || end - begin > 255 ? kEighthsUnknown : table;
         } else {
             for (int32_t b = begin; b < end; ++b) below += scattered[b] < v;
         }
@@ -647,7 +670,8 @@ __device__ __forceinline__ void sweep1Finish(const RunDev& run, const Sweep1<D, 
 // cells for the sums, dealing them alike.
 template <int D, typename Real, int Lanes, bool WithSweep1>
 __global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
-    buildNeighbors(const RunDev* __restrict__ runs, Tiles tiles, const int32_t* __restrict__ order, const int32_t* __restrict__ start, Records<Real> rec,
+    buildNeighbors(const RunDev* __restrict__ runs, Tiles tiles, const int32_t* __restrict__ order, const int32_t* __restrict__ start,
+                   const unsigned long long* __restrict__ eighths, Records<Real> rec,
                    NeighborList list, SweepOut<Real> out, Clock clock, const double* __restrict__ y64,
                    int32_t resident, int32_t cached, int32_t hitCapacity, int64_t n, int32_t filterMode = 0) {
     const int64_t tile = xcdBlock();
@@ -703,6 +727,12 @@ __global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
             }
             const int32_t tag = image ? kImageBit : 0;
             const Real reach = rowReach(twoHmax, me.p[0]);
+            // 3D rows of three cells take their candidates from the cells' eighths tables, refined
+            // by a search within the first and last eighth; the single cells at the grid's ends
+            // along x by the search alone; 2D rows are walked whole
+            const bool threeCells = D == 3 && run.nx >= 3 && cx >= 1 && cx <= run.nx - 2;
+            const Real cell = (Real)run.cell;
+            const Real rowLeft = threeCells ? cellLowerEdge(me.p[0], cx, run.nx, cell) - cell : Real(0);
             forStencil<D>(run, start, cx, oy, cz, [&](int32_t begin, int32_t end, int32_t middle, int32_t right,
                                                        int rowY, int rowZ, bool wrapped) {
                 const int32_t copied = cells.rangeBase(begin, end);
@@ -710,10 +740,18 @@ __global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
                     return copied >= 0 ? ldsValue<Real>(copied + (k - begin), 0, 0, cached) : rec.r[k].q0.x;
                 };
                 int32_t from = begin, to = end;
-                if (D == 3)
-                    rowCandidates(begin, end, middle, right, me.p[0], reach,
-                                  rowReachAcross(run, reach, me.p[1], me.p[2], cy, cz, rowY, rowZ, wrapped || image == 1),
-                                  pointX, from, to);
+                const Real across = D == 3 ? rowReachAcross<D>(run, reach, me.p[1], me.p[2], cy, cz, rowY, rowZ,
+                                                               wrapped || image == 1)
+                                           : reach;
+                if (threeCells) {
+                    const int64_t plane = D == 3 ? (int64_t)wrapNear(rowZ, run.nz) : 0;
+                    const int64_t left = run.cellFirst + (plane * run.ny + wrapNear(rowY, run.ny)) * run.nx + cx - 1;
+                    const unsigned long long tables[3] = {eighths[left], eighths[left + 1], eighths[left + 2]};
+                    rowEighths<true>(begin, middle, right, end, me.p[0], across, rowLeft, cell, tables, pointX, from,
+                                     to);
+                } else if (D == 3) {
+                    rowCandidates(begin, end, middle, right, me.p[0], reach, across, pointX, from, to);
+                }
                 // the predicate's fields only: the point and remainder, and the length when lengths
                 // differ; a lane past the range reads the range's last record and never hits, so
                 // the loop body has no branch but the predicate's rare rounded root. One loop per
@@ -2602,6 +2640,8 @@ struct sph_context {
     int32_t* cellCount = nullptr;
     int32_t* cellFill = nullptr;
     int32_t* cellStart = nullptr;
+    uint8_t* eighth = nullptr;                     // per slot: its eighth of its cell along x
+    unsigned long long* cellEighths = nullptr;     // per cell: its eighths table (EighthTable)
     int32_t* scanPartial = nullptr;
     int64_t scanTiles = 0;
 
@@ -2936,8 +2976,9 @@ int buildCells(sph_context* ctx, int clockMode = 0) {
     if (n > 0)
         TIMED(ctx, kCellKeys, (cellKeys<D><<<gridFor(n, b), b, 0, ctx->stream>>>(ctx->dRuns, ctx->runOf, P, P + st,
                                                                                  D == 3 ? P + 2 * st : P, n, ctx->key,
-                                                                                 ctx->cellCount, ctx->clock,
-                                                                                 clockMode)));
+                                                                                 ctx->cellCount,
+                                                                                 D == 3 ? ctx->eighth : nullptr,
+                                                                                 ctx->clock, clockMode)));
     const int64_t m = ctx->cells;
     {
         const int slot = timeBegin(ctx, kScan);
@@ -2951,8 +2992,8 @@ int buildCells(sph_context* ctx, int clockMode = 0) {
         TIMED(ctx, kCellScatter, (cellScatter<<<gridFor(n, b), b, 0, ctx->stream>>>(ctx->key, ctx->cellStart,
                                                                                     ctx->cellFill, ctx->scattered, n)));
     TIMED(ctx, kCellOrder, (cellRank<<<gridFor(std::max<int64_t>(n, m), b), b, 0, ctx->stream>>>(
-                               ctx->cellStart, ctx->key, D == 3 ? P : nullptr, ctx->scattered, ctx->order, ctx->cellCount,
-                               ctx->cellFill, n, m)));
+                               ctx->cellStart, ctx->key, D == 3 ? P : nullptr, ctx->eighth, ctx->scattered, ctx->order,
+                               D == 3 ? ctx->cellEighths : nullptr, ctx->cellCount, ctx->cellFill, n, m)));
     return launchCheck(ctx, "cell build");
 }
 
@@ -3083,7 +3124,7 @@ int stepOnce(sph_context* ctx, int phase, int useClock, double forcedDt, int fil
         if (ctx->list.capacity > 0) {
             WITH_LANES(ctx, TIMED(ctx, kNeighbors,
                                   (buildNeighbors<D, Real, L, true><<<tileGrid(ctx), b, buildLds<D, Real>(ctx), ctx->stream>>>(
-                            ctx->dRuns, tilesOf(ctx), ctx->order, ctx->cellStart, rec, ctx->list, out,
+                            ctx->dRuns, tilesOf(ctx), ctx->order, ctx->cellStart, ctx->cellEighths, rec, ctx->list, out,
                                       ctx->clock, y64, 1, nearRecords<D, Real>(ctx), hitCapacity<D>(ctx), n, mode))));
         } else {
             WITH_LANES(ctx, TIMED(ctx, kSweepWall,
@@ -3129,7 +3170,7 @@ int stepOnce(sph_context* ctx, int phase, int useClock, double forcedDt, int fil
         }
         if (ctx->sumChunkCount > 0) {
             const int slot = timeBegin(ctx, kMaxima);
-            chunkMaxima<<<(unsigned)ctx->sumChunkCount, 256, 0, ctx->stream>>>(
+            chunkMaxima<<<(unsigned)ctx->sumChunkCount, kMaximaThreads, 0, ctx->stream>>>(
                 ctx->sumChunkRun, ctx->sumChunkFirst, ctx->dRuns, ctx->clock.speed, ctx->clock.acceleration, ctx->clock,
                 ctx->sumPartial);
             runMaxima<<<gridFor(nruns, 64), 64, 0, ctx->stream>>>(ctx->dRuns, ctx->sumRunChunk, ctx->sumPartial,
@@ -3244,7 +3285,7 @@ int hostSweep(sph_context* ctx, int which, const double* velocity, const double*
     const NeighborList list = ctx->list;
     if (list.capacity > 0)
         WITH_LANES(ctx, (buildNeighbors<D, Real, L, false><<<tileGrid(ctx), b, sweepLds<D, Real>(ctx, 0), ctx->stream>>>(
-                            ctx->dRuns, tilesOf(ctx), order, start, rec, list, emptyOut<Real>(ctx), ctx->clock, y64, 0,
+                            ctx->dRuns, tilesOf(ctx), order, start, ctx->cellEighths, rec, list, emptyOut<Real>(ctx), ctx->clock, y64, 0,
                             nearRecords<D, Real>(ctx), 0, n)));
     switch (which) {
         case 0:   // wall pressure
@@ -3359,7 +3400,7 @@ int shiftSweep(sph_context* ctx, const double* velocity, const double* density, 
     const NeighborList list = ctx->list;
     if (list.capacity > 0)
         WITH_LANES(ctx, (buildNeighbors<D, Real, L, false><<<tileGrid(ctx), b, sweepLds<D, Real>(ctx, 0), ctx->stream>>>(
-                            ctx->dRuns, tilesOf(ctx), ctx->order, ctx->cellStart, rec, ctx->list, emptyOut<Real>(ctx),
+                            ctx->dRuns, tilesOf(ctx), ctx->order, ctx->cellStart, ctx->cellEighths, rec, ctx->list, emptyOut<Real>(ctx),
                                 ctx->clock, y64, 0, nearRecords<D, Real>(ctx), 0, n)));
     Real* fields = static_cast<Real*>(ctx->shiftFields);
     for (int32_t pass = 0; pass < 2; ++pass)
@@ -3396,7 +3437,7 @@ int neighborsImpl(sph_context* ctx, int32_t* counts, const int64_t* offsets, int
     const double* y64 = ctx->state.position + ctx->state.stride;
     if (ctx->list.capacity > 0)
         WITH_LANES(ctx, (buildNeighbors<D, Real, L, false><<<tileGrid(ctx), b, sweepLds<D, Real>(ctx, 0), ctx->stream>>>(
-                            ctx->dRuns, tilesOf(ctx), ctx->order, ctx->cellStart, rec, ctx->list, emptyOut<Real>(ctx),
+                            ctx->dRuns, tilesOf(ctx), ctx->order, ctx->cellStart, ctx->cellEighths, rec, ctx->list, emptyOut<Real>(ctx),
                             ctx->clock, y64, 0, nearRecords<D, Real>(ctx), 0, n)));
     neighborExport<D, Real><<<gridFor(n, b), b, 0, ctx->stream>>>(ctx->dRuns, ctx->runOf, ctx->order, ctx->cellStart, rec,
                                                                    ctx->list, y64, counts, offsets, rows, n);
@@ -3565,6 +3606,8 @@ int sph_configure(sph_context* ctx, const sph_run* runs, int32_t count, int64_t 
     if (status == SPH_OK) status = allocate(ctx, &ctx->cellCount, cells);
     if (status == SPH_OK) status = allocate(ctx, &ctx->cellFill, cells);
     if (status == SPH_OK) status = allocate(ctx, &ctx->cellStart, cells + 1);
+    if (status == SPH_OK) status = allocate(ctx, &ctx->eighth, cap);
+    if (status == SPH_OK) status = allocate(ctx, &ctx->cellEighths, cells);
     if (status == SPH_OK) status = allocate(ctx, &ctx->scanPartial, ctx->scanTiles);
     if (status == SPH_OK) status = allocate(ctx, &ctx->clock.time, count);
     if (status == SPH_OK) status = allocate(ctx, &ctx->clock.until, count);

@@ -39,11 +39,11 @@ from pysurf_amd.errors import NumericalInstabilityError
 
 # HBM per particle the admission counts: state 81 B (f64 position, velocity, density; f32 pressure,
 # acceleration, mass, length; a flag), the sorted record and its key, G and L, the neighbor list,
-# the run id, cell key, order and scattered order, and the resident accelerations
+# the run id, cell key, order, scattered order and (3D) eighth along x, and the resident accelerations
 def bytesPerParticle(dimensions: int, listCapacity: int) -> int:
     d = dimensions
     state = 8 * d * 2 + 8 + 4 + 4 * d + 4 + 4 + 1
-    return state + 64 + 4 + 8 * d * d + 4 * (listCapacity + 1) + 16 + 4 * d
+    return state + 64 + 4 + 8 * d * d + 4 * (listCapacity + 1) + 16 + 1 + 4 * d
 
 
 def trackerOf(case) -> BreakingTracker:

@@ -298,26 +298,33 @@ def testNeighborSetsAreBitExact(precision, dimensions, periodic):
 
 
 @pytest.mark.parametrize('precision', precisions)
-@pytest.mark.parametrize('placement', ['straddling the origin', 'beyond the grid'])
-def testThreeDimensionalRowsKeepEveryNeighbor(precision, placement):
-    # 3D rows are cut along x by the particle's distance across to each row, from its unwrapped cell,
-    # and searched as one ascending range only where the row ascends: a cloud across the origin (cell
-    # coordinates of both signs) and one given a grid narrower than itself (cells holding particles
-    # of several wrapped cells) keep the brute-force neighbor sets
-    print(f'3D rows {placement} {precision}', flush = True)
-    case = cloud(3)
+@pytest.mark.parametrize('dimensions', [2, 3])
+@pytest.mark.parametrize('placement', ['straddling the origin', 'partly beyond the grid', 'beyond the grid'])
+def testRowsCutAlongXKeepEveryNeighbor(precision, dimensions, placement):
+    # 3D rows are cut along x by the particle's distance across to each row, from its unwrapped
+    # cell, in whole eighths of each cell and then by a search within the first and last eighth, and
+    # a cell holding a particle from outside the grid along x is taken whole; 2D rows are walked
+    # whole: a cloud across the origin (cell coordinates of both signs) and ones given a grid
+    # narrower than themselves (cells holding particles of several wrapped cells, some or most of
+    # them) keep the brute-force neighbor sets
+    print(f'rows cut along x {dimensions}D {placement} {precision}', flush = True)
+    case = cloud(dimensions)
     positions = case['positions'] - 25.0 - 0.5 * (case['positions'].max(axis = 0) - 25.0)
     from pysurf_amd.pair_sums import PairSums
     low, high = positions.min(axis = 0), positions.max(axis = 0)
-    if placement == 'beyond the grid':
-        high = low + 0.4 * (high - low)
-    sums = PairSums(3, case['h'], low, high, 0.0, 'float64', 'cuda:0', case['kernelType'], pairPrecision = precision)
+    if placement != 'straddling the origin':
+        high = low + (0.4 if placement == 'beyond the grid' else 0.8) * (high - low)
+    sums = PairSums(dimensions, case['h'], low, high, 0.0, 'float64', 'cuda:0', case['kernelType'],
+                    pairPrecision = precision)
     sums.update(positions, case['masses'], case['isFluid'], None)
     offsets, rows = sums.neighborLists()
     expectedOffsets, expectedRows = oracle.neighbors(physicsFor(dict(case, positions = positions)), positions,
                                                      np.full(len(positions), case['h']), precision)
     assert np.array_equal(offsets, expectedOffsets)
     assert np.array_equal(rows, expectedRows)
+    # the float64 sets are the brute-force sets
+    if precision == 'float64':
+        assert len(rows) == 2 * len(case['i'])
 
 
 @pytest.mark.parametrize('precision', precisions)

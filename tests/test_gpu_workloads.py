@@ -10,6 +10,12 @@ GPU tests step:
   the 3D crest zone   daughters that keep the mother's smoothing length (length ratio 1), so a
                       crest particle has more neighbors than the list holds and walks its cells:
                       the steps follow the restatement
+  the 3D ridge        all 12.4 M particles of the 32 particles-per-depth ridge with its split crest
+                      zone (daughters keep h, up to about 600 neighbors, more than the list's 768
+                      slots hold at the densest), stepped at the bench's launch shape: the state stays
+                      finite, the mass is the start's, the energy and the momentum along the wave
+                      move by no more than the start's relaxation allows, and the fluid stays in the
+                      tank
   the request loop    a full-size 2D sweep run and a 16 particles-per-depth 3D ridge run of 0.73 M
                       particles under an HBM budget that holds one of them at a time, so the second
                       queues: both are served, and each history is the run's alone, bit for bit
@@ -65,6 +71,41 @@ def testTheBatchedSweepOf256RunsFollowsTheOracle():
     mixed = batch.state(index)
     for name in ('positions', 'velocities', 'densities', 'pressures'):
         assert np.array_equal(getattr(alone, name), getattr(mixed, name)), name
+
+
+@pytest.mark.timeout(600)
+def testTheFullSizeRidgeStaysPhysical():
+    from pysurf_amd.batch import BatchEngine
+    from pysurf_amd.scenarios import ridge3D
+    case = ridge3D()
+    steps = 20
+    print(f'3D ridge at full size: {case.count} particles, {steps} steps', flush = True)
+    batch = BatchEngine([case], 'float32', blockSize = 256, lanes = 8, listCapacity = 768)
+    before = batch.globalSums()[0]
+    batch.setClocks(maxSteps = steps)
+    batch.advance(steps)
+    batch.synchronize()
+    clock = batch.clocks()
+    after = batch.globalSums()[0]
+    assert int(clock['steps'][0]) == steps and clock['time'][0] > 0.0
+    assert clock['failed'][0] == 0
+    assert np.all(np.isfinite(after))
+    # mass is carried by the particles, each keeping its own
+    assert after[0] == pytest.approx(before[0], rel = 1e-12)
+    # kinetic + potential + elastic: over the first 20 steps the start's pressure field relaxes
+    # (elastic energy turns kinetic) and the delta-SPH diffusion dissipates a little; the total moves
+    # by about 2e-4 of itself
+    energy = lambda sums: sums[1] + sums[2] + sums[3]
+    assert abs(energy(after) - energy(before)) < 1e-3 * abs(energy(before))
+    # the wave's momentum along x moves only by what the bed and walls push in a few steps
+    assert abs(after[4] - before[4]) < 0.02 * abs(before[4])
+    state = batch.state(0)
+    fluid = state.isFluid
+    assert np.all(np.isfinite(state.positions)) and np.all(np.isfinite(state.velocities))
+    low, high = np.asarray(case.domainMin), np.asarray(case.domainMax)
+    margin = 2.0 * case.smoothingLength
+    assert np.all(state.positions[fluid] > low - margin) and np.all(state.positions[fluid] < high + margin)
+    assert state.densities[fluid].min() >= case.densityFloor * (1.0 - 1e-12)
 
 
 @pytest.mark.timeout(300)
