@@ -496,6 +496,40 @@ __device__ __forceinline__ bool withinSupport(Real r2, Real t) {
     return SupportBand<Real>(t).holds(r2);
 }
 
+// The 3D float32 predicate from the points alone, for candidates of one length: the separation
+// without the neighbor's remainder, d' = (p_i - p_j) + shift + l_i, differs from the exact one,
+// d = (p_i - p_j) + shift + (l_i - l_j), by |l_j| (at most an ulp of |p_j| / 2, below 2^-23 P with
+// P = |p_i|_1 + |shift| + 8 t bounding |p_j| on every axis for a candidate within 4 t) and the
+// roundings of the two sums, so for r' = |d'| >= r_t one has |r^2 - r'^2| <= (7 e P / r' + 6 e) r'^2
+// with e = 2^-24, and for r' < t, |r^2 - r'^2| <= 7 e P t + 6 e t^2. `slack` (2^-16 + 16 e P / t)
+// covers both with room. decide() answers 1 (r^2 is below the band, a neighbor), -1 (above it, not
+// one) or 0 (too close to call: the caller reads the remainder and tests exactly), so its answers
+// are the exact predicate's wherever it gives one.
+template <typename Real>
+struct CoarseSupport {
+    Real in, outScale, above;
+    template <int D>
+    __device__ __forceinline__ CoarseSupport(const SupportBand<Real>& band, const Self<D, Real>& me, Real shift) {
+        const Real t = band.t;
+        const Real reachOf = fabs(me.p[0]) + fabs(me.p[1]) + fabs(me.p[2]) + fabs(shift) + Real(8) * t;
+        const Real slack = Real(1.0 / 65536.0) + Real(16.0 / 16777216.0) * reachOf / t;
+        in = band.below - slack * t * t;
+        outScale = Real(1) - slack;
+        above = band.above;
+    }
+    template <int D>
+    __device__ __forceinline__ int decide(const Self<D, Real>& me, const Record<Real>& other, Real shift) const {
+        const Quad<Real> a = other.q0;
+        const Real dx = (me.p[0] - a.x) + (Real)me.l[0];
+        const Real dy = ((me.p[1] - a.y) + shift) + (Real)me.l[1];
+        const Real dz = (me.p[2] - a.z) + (Real)me.l[2];
+        const Real r2 = squaredDistance(dx, dy, dz, 3);
+        if (r2 < in) return 1;
+        if (r2 * outScale > above) return -1;
+        return 0;
+    }
+};
+
 // The distance a pair sum uses once its pair is known, from its squared distance: in float32 the
 // hardware square root (within 1 ulp; the sums are float32 anyway), in float64 the correctly
 // rounded one. Every sum takes its pair's distance from here, so the list, the cell walk and the

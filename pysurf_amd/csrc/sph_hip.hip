@@ -760,17 +760,34 @@ __global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
                 auto scan = [&](auto fetch, auto oneLength) {
                     constexpr bool Same = decltype(oneLength)::value;
                     const SupportBand<Real> band(twoHmax);
+                    // 3D float32 with one length: the point alone decides all but a thin shell of
+                    // candidates (CoarseSupport), and only those read the remainder
+                    constexpr bool Coarse = D == 3 && Same && sizeof(Real) == 4;
+                    const CoarseSupport<Real> coarse(band, me, shift);
                     for (int32_t base = from; base < to; base += Lanes) {
                         const int32_t k = base + lane;
                         const bool inside = k < to;
+                        const int32_t at = inside ? k : to - 1;
                         Record<Real> other;
-                        fetch(inside ? k : to - 1, other, D == 3 || !Same);
-                        Real dx, dy, dz;
-                        separation<D, Real>(me, other, shift, dx, dy, dz);
-                        const Real r2 = squaredDistance(dx, dy, dz, D);
-                        // 2 h_ij = h_i + h_j exactly, as 2 (0.5 (h_i + h_j)) is
-                        const bool near = Same ? band.holds(r2)
-                                               : withinSupport<Real>(r2, me.h + fabs(signedLength<D, Real>(other)));
+                        fetch(at, other, !Coarse && (D == 3 || !Same));
+                        bool near;
+                        if constexpr (Coarse) {
+                            const int verdict = coarse.decide(me, other, shift);
+                            near = verdict > 0;
+                            if (verdict == 0) {
+                                fetch(at, other, true);
+                                Real dx, dy, dz;
+                                separation<D, Real>(me, other, shift, dx, dy, dz);
+                                near = band.holds(squaredDistance(dx, dy, dz, D));
+                            }
+                        } else {
+                            Real dx, dy, dz;
+                            separation<D, Real>(me, other, shift, dx, dy, dz);
+                            const Real r2 = squaredDistance(dx, dy, dz, D);
+                            // 2 h_ij = h_i + h_j exactly, as 2 (0.5 (h_i + h_j)) is
+                            near = Same ? band.holds(r2)
+                                        : withinSupport<Real>(r2, me.h + fabs(signedLength<D, Real>(other)));
+                        }
                         const bool hit = near && inside && k != self;
                         const uint32_t mine = (uint32_t)(__ballot(hit) >> groupBase) & groupMask;
                         if (hit) {
