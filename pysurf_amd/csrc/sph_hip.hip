@@ -63,6 +63,12 @@ constexpr int accelerationWaves() { return (D == 2 && sizeof(Real) == 4) ? SPH_W
 #endif
 #define SWEEP_BOUNDS(waves) __attribute__((amdgpu_flat_work_group_size(1, 256), amdgpu_waves_per_eu(waves)))
 
+// Whether a 3D row's range from whole eighths is then cut exactly by a search within its first
+// and last eighth (rowEighths).
+#ifndef SPH_REFINE_3D
+#define SPH_REFINE_3D 1
+#endif
+
 // Whether the sweeps copy the records around their particles into LDS (NearCells).
 #ifndef SPH_NEAR_CELLS
 #define SPH_NEAR_CELLS 1
@@ -747,8 +753,8 @@ __global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
                     const int64_t plane = D == 3 ? (int64_t)wrapNear(rowZ, run.nz) : 0;
                     const int64_t left = run.cellFirst + (plane * run.ny + wrapNear(rowY, run.ny)) * run.nx + cx - 1;
                     const unsigned long long tables[3] = {eighths[left], eighths[left + 1], eighths[left + 2]};
-                    rowEighths<true>(begin, middle, right, end, me.p[0], across, rowLeft, cell, tables, pointX, from,
-                                     to);
+                    rowEighths<SPH_REFINE_3D != 0>(begin, middle, right, end, me.p[0], across, rowLeft, cell, tables,
+                                                   pointX, from, to);
                 } else if (D == 3) {
                     rowCandidates(begin, end, middle, right, me.p[0], reach, across, pointX, from, to);
                 }
