@@ -52,10 +52,10 @@ HBM_PEAK_GBS = 8000.0
 # what the counter reads for a 1 GiB device copy (x 2.000 and x 1.000), as recorded in
 # profiles/batched_sweep_256runs_counters.md. A kernel, shape or size without a measurement has none.
 MEASURED_TRAFFIC = {
-    ('buildNeighbors', 4, 5073680): 1.1592e+09,
-    ('sweepContinuity', 4, 5073680): 1.3340e+09,
+    ('buildNeighbors', 4, 5073680): 9.4210e+08,
+    ('sweepContinuity', 4, 5073680): 1.3341e+09,
     ('sweepAccelerations', 4, 5073680): 1.0547e+09,
-    ('gatherRecords', 4, 5073680): 7.3172e+08,
+    ('gatherRecords', 4, 5073680): 7.3165e+08,
 }
 
 

@@ -32,11 +32,18 @@ def cacheKey(dimensions: int, precision: str, particles: int) -> tuple:
     return (dimensions, precision, int(math.log2(max(particles, 1))))
 
 
+# Steps timed per candidate for a context of fewer particles than this, at the least: a small
+# context's step is a fraction of a millisecond, and the timing needs several of them
+SMALL_CONTEXT, SMALL_CONTEXT_STEPS = 1_000_000, 64
+
+
 def timeSteps(batch, steps: int) -> float:
     '''Milliseconds per step of `steps` graph-replayed steps, between events on the engine's stream.'''
     import torch
     stream = batch.engine.stream
-    batch.advance(2, graph = True)                 # capture and warm the shape
+    # capture and warm the graph of `steps` steps first: a graph of another length would be captured
+    # inside the timed interval, and for a small context the capture outlasts the steps
+    batch.advance(steps, graph = True)
     batch.synchronize()
     start, end = torch.cuda.Event(enable_timing = True), torch.cuda.Event(enable_timing = True)
     start.record(stream)
@@ -77,6 +84,8 @@ def tune(batch, steps: int = 8, candidates = None, reuse: bool = True) -> dict:
         batch.engine.setLaunch(chosen['block'], chosen['lanes'], chosen['listCapacity'])
         return chosen
     shapes = candidates if candidates is not None else CANDIDATES[batch.dimensions]
+    if batch.count < SMALL_CONTEXT:
+        steps = max(steps, SMALL_CONTEXT_STEPS)
     timings = []
     for lanes, block, capacity in shapes:
         batch.engine.setLaunch(block, lanes, capacity)
