@@ -364,6 +364,10 @@ def servingRun(args) -> dict:
             'completion_p99_s': latencyPercentiles(finished)['p99'],
             'throughput': updates / wall, 'unit': 'particle-updates/s',
             'slots_2d': args.serving_slots_2d, 'slots_3d': args.serving_slots_3d, 'end_3d_s': args.serving_3d_end,
+            'fleets': {f'{d}d': {'fluid_updates': fleet.fluidUpdates, 'device_s': fleet.deviceSeconds,
+                                 'updates_per_device_s': fleet.fluidUpdates / fleet.deviceSeconds
+                                 if fleet.deviceSeconds > 0 else None}
+                       for d, fleet in sorted(server.fleets.items())},
             'max_queue_depth': max(depths, default = 0),
             'mean_queue_depth': float(np.mean(depths)) if depths else 0.0,
             'particles_max': max(case.count for case in pool), 'particles_min': min(case.count for case in pool)}

@@ -160,6 +160,11 @@ class Fleet:
         self.members: list[Member] = []
         self.batch: BatchEngine | None = None
         self.updates = 0
+        # device time of the fleet's steps (HIP events on its engine's stream around each interval's
+        # launches), and the fluid updates those steps made
+        self.deviceSeconds = 0.0
+        self.fluidUpdates = 0
+        self._interval = None
 
     def rebuild(self, leaving: list, joining: list) -> None:
 
@@ -228,7 +233,14 @@ class Fleet:
             dt = float(clock['dt'][index])
             if not stopped and dt > 0.0:
                 needed = max(needed, int(math.ceil((until - member.time) / dt)) + 1)
-        self.batch.advance(self.chunk * max(1, -(-needed // self.chunk)))
+        stream = self.batch.engine.stream
+        begin = torch.cuda.Event(enable_timing = True)
+        begin.record(stream)
+        self._interval = (begin, {index: member.steps for index, member in enumerate(self.members)})
+        # the one graph of `chunk` steps, replayed: the engine keeps a single captured graph, so asking
+        # for another step count would capture anew while the device waits
+        for _ in range(max(1, -(-needed // self.chunk))):
+            self.batch.advance(self.chunk)
 
     def collect(self) -> dict:
 
@@ -241,6 +253,15 @@ class Fleet:
 
         while not self.reached():
             self.batch.advance(self.chunk)
+        if self._interval is not None:
+            begin, before = self._interval
+            end = torch.cuda.Event(enable_timing = True)
+            end.record(self.batch.engine.stream)
+            end.synchronize()
+            self.deviceSeconds += begin.elapsed_time(end) / 1000.0
+            self.fluidUpdates += sum(member.case.fluidCount * (member.steps - before[index])
+                                     for index, member in enumerate(self.members))
+            self._interval = None
         reading = {'offsets': list(self.batch.offsets), 'members': [], 'failed': []}
         for index, member in enumerate(self.members):
             if member.done:
