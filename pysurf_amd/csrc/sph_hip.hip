@@ -159,6 +159,23 @@ __global__ void __launch_bounds__(kScanThreads) scanAdd(int32_t* __restrict__ ou
         if (base + k < n) out[base + k] += add;
 }
 
+// scanPartials and scanAdd in one launch, for at most kScanThreads tiles: each block sums the tile
+// totals before its own (one per thread, the block's reduction), adds that to its tile, and the last
+// block writes the grand total to out[n]. Integer sums, so the scan is the three-kernel one's.
+__global__ void __launch_bounds__(kScanThreads) scanAddFew(int32_t* __restrict__ out, const int32_t* __restrict__ partial,
+                                                           int64_t tiles, int64_t n) {
+    __shared__ int32_t waves[kScanThreads / 64];
+    const int64_t t = threadIdx.x;
+    const int32_t before = t < tiles && t < (int64_t)blockIdx.x ? partial[t] : 0;
+    int32_t add;
+    (void)blockExclusiveScan(before, waves, add);
+    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k)
+        if (base + k < n) out[base + k] += add;
+    if ((int64_t)blockIdx.x == tiles - 1 && threadIdx.x == 0) out[n] = add + partial[blockIdx.x];
+}
+
 // ---------------------------------------------------------------------------------------------
 // Clock
 // ---------------------------------------------------------------------------------------------
@@ -326,36 +343,47 @@ __global__ void cellScatter(const int32_t* __restrict__ key, const int32_t* __re
 // bounds a row's candidates along x before a search within one eighth. 3D cells are ordered by x and
 // take tables; a 2D cell holds about 7 slots, too few for bounds along x to repay their loads, and
 // keeps slot order. The first `cells` threads also clear the cells' counters for the next build.
+// The sorted place of the slot at scattered place a (its cell's start plus the cell's slots before
+// it), writing the cell's eighths table when asked (see cellRank); `slot` receives the slot.
+__device__ __forceinline__ int32_t rankInCell(const int32_t* __restrict__ start, const int32_t* __restrict__ key,
+                                              const double* __restrict__ x, const uint8_t* __restrict__ eighth,
+                                              const int32_t* __restrict__ scattered,
+                                              unsigned long long* __restrict__ eighths, int64_t a, int32_t& slot) {
+    const int32_t v = scattered[a];
+    const int32_t c = key[v];
+    const int32_t begin = start[c], end = start[c + 1];
+    int32_t below = 0;
+    if (x) {
+        const double xv = x[v];
+        unsigned long long table = 0ull;
+        uint32_t outside = 0u;
+        for (int32_t b = begin; b < end; ++b) {
+            const int32_t u = scattered[b];
+            const double xu = x[u];
+            below += xu < xv || (xu == xv && u < v);
+            if (eighths) {
+                const uint32_t e = eighth[u];
+                outside |= e & kOutsideX;
+                table += kEighthsAbove << (8u * (e & 7u));
+            }
+        }
+        if (eighths && below == 0) eighths[c] = outside || end - begin > 255 ? kEighthsUnknown : table;
+    } else {
+        for (int32_t b = begin; b < end; ++b) below += scattered[b] < v;
+    }
+    slot = v;
+    return begin + below;
+}
+
 __global__ void cellRank(const int32_t* __restrict__ start, const int32_t* __restrict__ key, const double* __restrict__ x,
                          const uint8_t* __restrict__ eighth, const int32_t* __restrict__ scattered,
                          int32_t* __restrict__ order, unsigned long long* __restrict__ eighths,
                          int32_t* __restrict__ count, int32_t* __restrict__ fill, int64_t n, int64_t cells) {
     const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (a < n) {
-        const int32_t v = scattered[a];
-        const int32_t c = key[v];
-        const int32_t begin = start[c], end = start[c + 1];
-        int32_t below = 0;
-        if (x) {
-            const double xv = x[v];
-            unsigned long long table = 0ull;
-            uint32_t outside = 0u;
-            for (int32_t b = begin; b < end; ++b) {
-                const int32_t u = scattered[b];
-                const double xu = x[u];
-                below += xu < xv || (xu == xv && u < v);
-                if (eighths) {
-                    const uint32_t e = eighth[u];
-                    outside |= e & kOutsideX;
-                    table += kEighthsAbove << (8u * (e & 7u));
-                }
-            }
-            if (eighths && below == 0)
-                eighths[c] = outside || end - begin > 255 ? kEighthsUnknown : table;
-        } else {
-            for (int32_t b = begin; b < end; ++b) below += scattered[b] < v;
-        }
-        order[begin + below] = v;
+        int32_t v;
+        const int32_t at = rankInCell(start, key, x, eighth, scattered, eighths, a, v);
+        order[at] = v;
     }
     if (a < cells) {
         count[a] = 0;
@@ -381,10 +409,67 @@ struct Sources {
     Real* pressureOut;       // the Tait pressure per slot, written when non-null
 };
 
-// Each sorted particle's record from its slot's state: the float32 point and remainder of its
-// float64 position (sphGpu.py:1010-1019), its velocity and density as the pair sums read them
+// Sorted particle s's record from slot i's state: the float32 point and remainder of its float64
+// position (sphGpu.py:1010-1019), its velocity and density as the pair sums read them
 // (sphGpu.py:1021-1029), rho - rho0 taken in float64 (sphGpu.py:1031-1038), the Tait pressure
-// (sphGpu.py:1058-1070), p / rho^2 and m / rho; resident, each run's |v|max.
+// (sphGpu.py:1058-1070), p / rho^2 and m / rho; returns |v| of a fluid particle (infinite when its
+// velocity or position is not finite), 0 on the boundary.
+template <int D, typename Real>
+__device__ __forceinline__ double gatherOne(const RunDev& run, int32_t i, int64_t s, const Sources<Real>& src,
+                                            Records<Real>& rec) {
+    Fields<Real> f;
+    Real p[3] = {Real(0), Real(0), Real(0)}, v[3] = {Real(0), Real(0), Real(0)};
+    float l[3] = {0.0f, 0.0f, 0.0f};
+    double v2 = 0.0;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        const double x = src.pos[a][i];
+        if constexpr (sizeof(Real) == 4) {
+            const float point = (float)x;
+            p[a] = point;
+            l[a] = (float)(x - (double)point);
+        } else {
+            p[a] = x;
+        }
+        const double u = src.vel[a][i];
+        v[a] = (Real)u;
+        v2 += u * u;
+    }
+    const double rho = src.density[i];
+    const bool isFluid = src.fluid[i] != 0;
+    Real pressure;
+    if (src.pressure) {
+        pressure = src.pressure[i];
+    } else {
+        const double ratio = rho / run.rho0, sq = ratio * ratio;
+        double value = run.stiffness * (sq * sq * sq * ratio - 1.0);
+        if (!run.negativePressure || !isFluid) value = fmax(value, 0.0);
+        pressure = (Real)value;
+        if (src.pressureOut) src.pressureOut[i] = pressure;
+    }
+    const Real rhoR = (Real)rho, massR = src.mass[i], h = src.len[i];
+    for (int a = 0; a < 3; ++a) {
+        f.p[a] = p[a];
+        f.l[a] = l[a];
+        f.v[a] = v[a];
+    }
+    f.sh = isFluid ? h : -h;
+    f.rho = rhoR;
+    f.mass = massR;
+    f.vol = massR / rhoR;
+    f.dev = (Real)(rho - run.rho0);
+    f.pr2 = pressure / (rhoR * rhoR);
+    f.press = pressure;
+    rec.r[s] = pack<D, Real>(f);
+    rec.key[s] = src.key[i];
+    bool finite = v2 < INFINITY;
+#pragma unroll
+    for (int a = 0; a < D; ++a) finite = finite && fabs(src.pos[a][i]) < INFINITY;
+    return isFluid ? (finite ? sqrt(v2) : INFINITY) : 0.0;
+}
+
+// Each sorted particle's record from its slot's state (gatherOne); resident, each slot's |v| for
+// the run's |v|max.
 template <int D, typename Real>
 __global__ void gatherRecords(const RunDev* __restrict__ runs, const int32_t* __restrict__ runOf,
                               const int32_t* __restrict__ order, Sources<Real> src, Records<Real> rec, Clock clock,
@@ -394,63 +479,34 @@ __global__ void gatherRecords(const RunDev* __restrict__ runs, const int32_t* __
     const int r = inRange ? runOf[s] : 0;
     // again after the Shepard filter: only the runs it filtered this step
     if (onlyFiltered && !(inRange && filterDue(runs[r], clock, r, onlyFiltered))) return;
-    bool valid = inRange && (!resident || clock.active[r]);
+    const bool valid = inRange && (!resident || clock.active[r]);
     double speed = 0.0;
-    if (valid) {
-        const RunDev& run = runs[r];
-        const int32_t i = order[s];
-        Fields<Real> f;
-        Real p[3] = {Real(0), Real(0), Real(0)}, v[3] = {Real(0), Real(0), Real(0)};
-        float l[3] = {0.0f, 0.0f, 0.0f};
-        double v2 = 0.0;
-#pragma unroll
-        for (int a = 0; a < D; ++a) {
-            const double x = src.pos[a][i];
-            if constexpr (sizeof(Real) == 4) {
-                const float point = (float)x;
-                p[a] = point;
-                l[a] = (float)(x - (double)point);
-            } else {
-                p[a] = x;
-            }
-            const double u = src.vel[a][i];
-            v[a] = (Real)u;
-            v2 += u * u;
-        }
-        const double rho = src.density[i];
-        const bool isFluid = src.fluid[i] != 0;
-        Real pressure;
-        if (src.pressure) {
-            pressure = src.pressure[i];
-        } else {
-            const double ratio = rho / run.rho0, sq = ratio * ratio;
-            double value = run.stiffness * (sq * sq * sq * ratio - 1.0);
-            if (!run.negativePressure || !isFluid) value = fmax(value, 0.0);
-            pressure = (Real)value;
-            if (src.pressureOut) src.pressureOut[i] = pressure;
-        }
-        const Real rhoR = (Real)rho, massR = src.mass[i], h = src.len[i];
-        for (int a = 0; a < 3; ++a) {
-            f.p[a] = p[a];
-            f.l[a] = l[a];
-            f.v[a] = v[a];
-        }
-        f.sh = isFluid ? h : -h;
-        f.rho = rhoR;
-        f.mass = massR;
-        f.vol = massR / rhoR;
-        f.dev = (Real)(rho - run.rho0);
-        f.pr2 = pressure / (rhoR * rhoR);
-        f.press = pressure;
-        rec.r[s] = pack<D, Real>(f);
-        rec.key[s] = src.key[i];
-        bool finite = v2 < INFINITY;
-#pragma unroll
-        for (int a = 0; a < D; ++a) finite = finite && fabs(src.pos[a][i]) < INFINITY;
-        speed = finite ? sqrt(v2) : INFINITY;
-        valid = isFluid;
+    if (valid) speed = gatherOne<D, Real>(runs[r], order[s], s, src, rec);
+    if (resident && inRange && clock.active[r]) clock.speed[s] = speed;
+}
+
+// cellRank and the resident gatherRecords in one launch: the thread that places a slot in cell order
+// also writes the slot's record there, so the step's gather neither waits for a launch of its own
+// nor reads the order back.
+template <int D, typename Real>
+__global__ void cellRankGather(const int32_t* __restrict__ start, const int32_t* __restrict__ key,
+                               const double* __restrict__ x, const uint8_t* __restrict__ eighth,
+                               const int32_t* __restrict__ scattered, int32_t* __restrict__ order,
+                               unsigned long long* __restrict__ eighths, int32_t* __restrict__ count,
+                               int32_t* __restrict__ fill, int64_t n, int64_t cells, const RunDev* __restrict__ runs,
+                               const int32_t* __restrict__ runOf, Sources<Real> src, Records<Real> rec, Clock clock) {
+    const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a < n) {
+        int32_t v;
+        const int32_t at = rankInCell(start, key, x, eighth, scattered, eighths, a, v);
+        order[at] = v;
+        const int r = runOf[v];
+        if (clock.active[r]) clock.speed[at] = gatherOne<D, Real>(runs[r], v, at, src, rec);
     }
-    if (resident && inRange && clock.active[r]) clock.speed[s] = valid ? speed : 0.0;
+    if (a < cells) {
+        count[a] = 0;
+        fill[a] = 0;
+    }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -2989,9 +3045,10 @@ int collectTimes(sph_context* ctx) {
         timeEnd((ctx), slot_);                    \
     } while (0)
 
-// The cell list over the bound positions.
-template <int D>
-int buildCells(sph_context* ctx, int clockMode = 0) {
+// The cell list over the bound positions. `rank` launches the ordering within cells on the
+// scattered slots (cellRank, or the step's cellRankGather), over max(n, cells) threads.
+template <int D, class Rank>
+int buildCellsWith(sph_context* ctx, int clockMode, Rank&& rank) {
     const int64_t n = ctx->total;
     const int b = 256;
     double* P = ctx->state.position;
@@ -3007,17 +3064,30 @@ int buildCells(sph_context* ctx, int clockMode = 0) {
         const int slot = timeBegin(ctx, kScan);
         scanTiles<<<(unsigned)ctx->scanTiles, kScanThreads, 0, ctx->stream>>>(ctx->cellCount, ctx->cellStart,
                                                                                ctx->scanPartial, m);
-        scanPartials<<<1, kScanThreads, 0, ctx->stream>>>(ctx->scanPartial, ctx->scanTiles, ctx->cellStart, m);
-        scanAdd<<<(unsigned)ctx->scanTiles, kScanThreads, 0, ctx->stream>>>(ctx->cellStart, ctx->scanPartial, m);
+        if (ctx->scanTiles > 0 && ctx->scanTiles <= kScanThreads) {
+            scanAddFew<<<(unsigned)ctx->scanTiles, kScanThreads, 0, ctx->stream>>>(ctx->cellStart, ctx->scanPartial,
+                                                                                   ctx->scanTiles, m);
+        } else {
+            scanPartials<<<1, kScanThreads, 0, ctx->stream>>>(ctx->scanPartial, ctx->scanTiles, ctx->cellStart, m);
+            scanAdd<<<(unsigned)ctx->scanTiles, kScanThreads, 0, ctx->stream>>>(ctx->cellStart, ctx->scanPartial, m);
+        }
         timeEnd(ctx, slot);
     }
     if (n > 0)
         TIMED(ctx, kCellScatter, (cellScatter<<<gridFor(n, b), b, 0, ctx->stream>>>(ctx->key, ctx->cellStart,
                                                                                     ctx->cellFill, ctx->scattered, n)));
-    TIMED(ctx, kCellOrder, (cellRank<<<gridFor(std::max<int64_t>(n, m), b), b, 0, ctx->stream>>>(
-                               ctx->cellStart, ctx->key, D == 3 ? P : nullptr, ctx->eighth, ctx->scattered, ctx->order,
-                               D == 3 ? ctx->cellEighths : nullptr, ctx->cellCount, ctx->cellFill, n, m)));
+    TIMED(ctx, kCellOrder, (rank(gridFor(std::max<int64_t>(n, m), b), b, D == 3 ? P : nullptr,
+                                 D == 3 ? ctx->cellEighths : nullptr, n, m)));
     return launchCheck(ctx, "cell build");
+}
+
+template <int D>
+int buildCells(sph_context* ctx, int clockMode = 0) {
+    return buildCellsWith<D>(ctx, clockMode, [&](unsigned grid, int block, const double* x,
+                                                 unsigned long long* eighths, int64_t n, int64_t m) {
+        cellRank<<<grid, block, 0, ctx->stream>>>(ctx->cellStart, ctx->key, x, ctx->eighth, ctx->scattered, ctx->order,
+                                                 eighths, ctx->cellCount, ctx->cellFill, n, m);
+    });
 }
 
 template <typename Real>
@@ -3105,15 +3175,22 @@ int stepOnce(sph_context* ctx, int phase, int useClock, double forcedDt, int fil
     Records<Real>& rec = recordsOf<Real>(ctx);
     const int64_t st = ctx->state.stride;
     const StepOptions o = stepOptions(ctx, filterMode);
+    Sources<Real> src = sourcesOf<Real>(ctx, ctx->state.velocity, ctx->state.density, nullptr,
+                                        static_cast<Real*>(ctx->state.pressure));
     if (phase & 1) {
-        // A run without particles has no slot to open its step; the kernels skip it either way
-        int status = buildCells<D>(ctx, useClock ? 1 : 2);
+        // A run without particles has no slot to open its step; the kernels skip it either way. The
+        // ordering within cells also gathers the records (cellRankGather).
+        int status = buildCellsWith<D>(ctx, useClock ? 1 : 2, [&](unsigned grid, int block, const double* x,
+                                                                   unsigned long long* eighths, int64_t count,
+                                                                   int64_t cells) {
+            cellRankGather<D, Real><<<grid, block, 0, ctx->stream>>>(
+                ctx->cellStart, ctx->key, x, ctx->eighth, ctx->scattered, ctx->order, eighths, ctx->cellCount,
+                ctx->cellFill, count, cells, ctx->dRuns, ctx->runOf, src, rec, ctx->clock);
+        });
         if (status != SPH_OK) return status;
     }
     (void)nruns;
     if (n == 0) return launchCheck(ctx, "device step");
-    Sources<Real> src = sourcesOf<Real>(ctx, ctx->state.velocity, ctx->state.density, nullptr,
-                                        static_cast<Real*>(ctx->state.pressure));
     SweepOut<Real> out = emptyOut<Real>(ctx);
     out.pressure = static_cast<Real*>(ctx->state.pressure);
     out.density = ctx->state.density;
@@ -3141,9 +3218,6 @@ int stepOnce(sph_context* ctx, int phase, int useClock, double forcedDt, int fil
     const int mode = o.filter ? filterMode : 0;
     const double* y64 = ctx->state.position + st;
     if (phase & 1) {
-        TIMED(ctx, kGather,
-              (gatherRecords<D, Real><<<gridFor(n, b), b, 0, ctx->stream>>>(ctx->dRuns, ctx->runOf, ctx->order, src, rec,
-                                                                             ctx->clock, 1, n)));
         if (ctx->list.capacity > 0) {
             WITH_LANES(ctx, TIMED(ctx, kNeighbors,
                                   (buildNeighbors<D, Real, L, true><<<tileGrid(ctx), b, buildLds<D, Real>(ctx), ctx->stream>>>(
