@@ -289,6 +289,8 @@ def ridgeRun(args) -> dict:
 
     '''
 
+    import numpy as np
+
     from pysurf_amd import autotune
     from pysurf_amd.batch import BatchEngine
     from pysurf_amd.scenarios import ridge3D
@@ -296,11 +298,26 @@ def ridgeRun(args) -> dict:
     log(f'3D ridge: {case.count} particles ({case.fluidCount} fluid)')
     batch = BatchEngine([case], 'float32')
     launch = autotune.tune(batch, steps = 3, candidates = [(lanes, 256, 768) for lanes in (4, 8, 16)])
+    before = batch.globalSums()[0]
     warm(batch, 2, args.chunk)
-    wall, events = timedSteps(batch, args.ridge_steps, args.chunk)
+    # timed in segments of 100 steps (about 20 s), each reported as it ends
+    wall, events, done = 0.0, 0.0, 0
+    while done < args.ridge_steps:
+        take = min(100, args.ridge_steps - done)
+        segment = timedSteps(batch, take, args.chunk)
+        wall, events, done = wall + segment[0], events + segment[1], done + take
+        log(f'3D ridge: {done} of {args.ridge_steps} timed steps, {1000.0 * wall / done:.1f} ms/step')
+    after = batch.globalSums()[0]
+    clock = batch.clocks()
     profile = kernelProfile(batch, 2)
     entries = meanNeighbors(batch)
     value = case.fluidCount * args.ridge_steps / wall
+    energy = lambda sums: float(sums[1] + sums[2] + sums[3])
+    checks = {'steps': int(clock['steps'][0]), 'simulated_s': float(clock['time'][0]),
+              'failed': int(clock['failed'][0]), 'finite': bool(np.all(np.isfinite(after))),
+              'mass_relative_change': float(after[0] / before[0] - 1.0),
+              'energy_relative_change': float(energy(after) / energy(before) - 1.0),
+              'momentum_x_relative_change': float(after[4] / before[4] - 1.0)}
     log(f'3D ridge float32: {1000.0 * wall / args.ridge_steps:.1f} ms/step, {value:.4e} fluid updates/s')
     del batch
     return {'workload': f'{case.name}: Gaussian ridge 0.5 d tall, 2 d wide, 1:15, turned 25 degrees; 20 x 10 d tank; '
@@ -308,6 +325,7 @@ def ridgeRun(args) -> dict:
             'fluid_particles': case.fluidCount, 'steps': args.ridge_steps, 'value': value,
             'unit': 'particle-updates/s', 'ms_per_step': 1000.0 * wall / args.ridge_steps,
             'mean_neighbors': entries, 'launch': {key: launch[key] for key in ('lanes', 'block', 'listCapacity')},
+            'checks': checks,
             'kernel_ms_per_step': {name: entry['ms'] / 2 for name, entry in profile.items() if entry['launches']}}
 
 
