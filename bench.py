@@ -297,7 +297,8 @@ def ridgeRun(args) -> dict:
     case = ridge3D()
     log(f'3D ridge: {case.count} particles ({case.fluidCount} fluid)')
     batch = BatchEngine([case], 'float32')
-    launch = autotune.tune(batch, steps = 3, candidates = [(lanes, 256, 768) for lanes in (4, 8, 16)])
+    launch = autotune.tune(batch, steps = 3, candidates = [(lanes, 256, 768) for lanes in (4, 8, 16)] +
+                                                        [(8, 256, 768, cap) for cap in (16384, 24576)])
     before = batch.globalSums()[0]
     warm(batch, 2, args.chunk)
     # timed in segments of 100 steps (about 20 s), each reported as it ends
@@ -324,7 +325,8 @@ def ridgeRun(args) -> dict:
                         f'crest zone |x - 4 d| < 0.75 d split once, daughters keep h', 'particles': case.count,
             'fluid_particles': case.fluidCount, 'steps': args.ridge_steps, 'value': value,
             'unit': 'particle-updates/s', 'ms_per_step': 1000.0 * wall / args.ridge_steps,
-            'mean_neighbors': entries, 'launch': {key: launch[key] for key in ('lanes', 'block', 'listCapacity')},
+            'mean_neighbors': entries,
+            'launch': {key: launch[key] for key in ('lanes', 'block', 'listCapacity', 'nearCells')},
             'checks': checks,
             'kernel_ms_per_step': {name: entry['ms'] / 2 for name, entry in profile.items() if entry['launches']}}
 

@@ -122,7 +122,7 @@ class BatchEngine:
         self.engine.setLaunch(blockSize, lanes, listCapacity)
         self.identity = None
         self.load(carry)
-        self.launch = {'lanes': lanes, 'block': blockSize, 'listCapacity': listCapacity}
+        self.launch = {'lanes': lanes, 'block': blockSize, 'listCapacity': listCapacity, 'nearCells': -1}
         if tune:
             from pysurf_amd import autotune
             self.launch = autotune.tune(self)

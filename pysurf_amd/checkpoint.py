@@ -134,7 +134,7 @@ def save(batch, path: str, extra: dict | None = None) -> None:
         'dimensions': batch.dimensions, 'precision': batch.precision, 'count': count,
         'runs': [{'first': batch.offsets[r], 'count': batch.offsets[r + 1] - batch.offsets[r]} for r in range(runs)],
         'digests': [runDigest(case) for case in batch.cases],
-        'launch': {key: int(batch.launch[key]) for key in ('lanes', 'block', 'listCapacity')},
+        'launch': {key: int(batch.launch.get(key, -1)) for key in ('lanes', 'block', 'listCapacity', 'nearCells')},
         'clock': {'time': [float(t).hex() for t in clock['time']], 'steps': [int(s) for s in clock['steps']],
                   'until': [float(u).hex() for u in batch.until], 'maxSteps': [int(m) for m in batch.maxSteps]},
         'extra': extra if extra is not None else {},
@@ -163,9 +163,11 @@ def restore(batch, path: str) -> dict:
     if differ:
         raise ValueError(f'The checkpoint was written for other physics or another bed: runs {differ} differ from '
                          'the saved runs (settings or bed hash).')
-    launch = header['launch']
+    launch = dict(header['launch'])
+    launch.setdefault('nearCells', -1)
     batch.engine.setLaunch(launch['block'], launch['lanes'], launch['listCapacity'])
-    batch.launch = dict(launch)
+    batch.engine.setNearCells(launch['nearCells'])
+    batch.launch = launch
     engine, count = batch.engine, header['count']
     with torch.cuda.device(engine.device), torch.cuda.stream(engine.stream):
         for name in STATE:

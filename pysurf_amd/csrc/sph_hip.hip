@@ -2662,6 +2662,7 @@ struct sph_context {
     std::string error;
     int block = 256;
     int lanes = 0;           // lanes per particle in the sweeps; 0 picks by dimension
+    int32_t nearBytes = -1;  // the cap of a sweep workgroup's LDS copy of records; -1 picks by dimension
     int listCapacity = -1;   // neighbor-list entries per particle; -1 picks by dimension, 0 walks the cells
     NeighborList list{};
     // A copy of the bound state the state sort permutes from, allocated on first use
@@ -2951,14 +2952,15 @@ inline Tiles tilesOf(const sph_context* ctx) { return Tiles{ctx->tileRun, ctx->t
 inline unsigned tileGrid(const sph_context* ctx) { return (unsigned)ctx->tileCount; }
 
 // Records a sweep workgroup copies into LDS: the cells around its particles hold about 3 records
-// per particle in 2D and 9 in 3D plus the ends of the ranges; the copy is capped at 32 KiB so
-// that LDS leaves the waves' occupancy alone.
+// per particle in 2D and 9 in 3D plus the ends of the ranges; the copy is capped (32 KiB unless
+// sph_set_near_cells says otherwise) so that LDS leaves the waves' occupancy alone.
 template <int D, typename Real>
 int32_t nearRecords(const sph_context* ctx) {
     if (!SPH_NEAR_CELLS) return 0;
     const int32_t per = (int32_t)ctx->tilePer;
     const int32_t want = D == 2 ? 4 * per + 64 : SPH_NEAR_PER_3D * per + 128;
-    const int32_t most = (int32_t)((D == 2 ? 32768 : SPH_NEAR_BYTES_3D) / sizeof(Record<Real>));
+    const int32_t bytes = ctx->nearBytes >= 0 ? ctx->nearBytes : (D == 2 ? 32768 : SPH_NEAR_BYTES_3D);
+    const int32_t most = (int32_t)(bytes / sizeof(Record<Real>));
     return want < most ? want : most;
 }
 // Hits per particle the fused list build keeps in LDS after its NearCells copy: a 6 KiB (2D) or
@@ -4705,6 +4707,17 @@ int sph_set_launch(sph_context* ctx, int32_t blockSize, int32_t lanes, int32_t l
             HIP_TRY(hipStreamSynchronize(ctx->stream));
             return allocateList(ctx);
         }
+    }
+    return SPH_OK;
+}
+
+int sph_set_near_cells(sph_context* ctx, int32_t bytes) {
+    if (!ctx) return SPH_ERR_VALUE;
+    if (bytes < -1 || bytes > 98304)
+        return fail(ctx, SPH_ERR_VALUE, "The LDS copy's cap must be from 0 to 98304 bytes (-1 picks by dimension).");
+    if (bytes != ctx->nearBytes) {
+        ctx->nearBytes = bytes;
+        dropGraph(ctx);
     }
     return SPH_OK;
 }
