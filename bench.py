@@ -298,7 +298,7 @@ def ridgeRun(args) -> dict:
     log(f'3D ridge: {case.count} particles ({case.fluidCount} fluid)')
     batch = BatchEngine([case], 'float32')
     launch = autotune.tune(batch, steps = 3, candidates = [(lanes, 256, 768) for lanes in (4, 8, 16)] +
-                                                        [(8, 256, 768, cap) for cap in (16384, 24576)])
+                                                        [(8, 256, 768, cap) for cap in (0, 8192, 16384, 24576)])
     before = batch.globalSums()[0]
     warm(batch, 2, args.chunk)
     # timed in segments of 100 steps (about 20 s), each reported as it ends

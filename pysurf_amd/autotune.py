@@ -23,12 +23,14 @@ from __future__ import annotations
 import math
 
 # (lanes, workgroup threads, list capacity[, LDS copy cap]); capacity -1 is 64 entries in 2D and
-# 160 in 3D, the cap -1 32 KiB. A 2D workgroup's neighborhood fits 32 KiB at every lane count; a 3D
-# one outgrows it, so 3D also tries smaller copies that leave room for more workgroups.
+# 160 in 3D, the cap -1 32 KiB. A 2D workgroup's neighborhood fits 32 KiB at every lane count, but a
+# smaller copy leaves room for one more workgroup on a CU; a 3D one outgrows it, so 3D also tries
+# smaller copies for the same reason.
 CANDIDATES = {
-    2: [(lanes, block, -1) for lanes in (1, 2, 4, 8, 16) for block in (128, 256)],
+    2: [(lanes, block, -1) for lanes in (1, 2, 4, 8, 16) for block in (128, 256)] +
+       [(lanes, 256, -1, cap) for lanes in (4, 8) for cap in (12288, 16384)],
     3: [(lanes, block, -1) for lanes in (2, 4, 8, 16) for block in (128, 256)] +
-       [(lanes, 256, -1, cap) for lanes in (4, 8) for cap in (16384, 24576)],
+       [(lanes, 256, -1, cap) for lanes in (4, 8) for cap in (0, 8192, 16384, 24576)],
 }
 
 _cache: dict = {}

@@ -27,6 +27,7 @@ def testLongLengthsTryLongerLists():
     shapes = autotune.candidatesFor(standIn(2, [1.3, 3.0]))
     capacities = {shape[2] for shape in shapes}
     assert capacities == {-1, 192, 288}
+    assert all(len(shape) == 3 or shape[2] == -1 for shape in shapes)
     assert all(capacity % 32 == 0 for capacity in capacities if capacity > 0)
 
 
