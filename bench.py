@@ -119,6 +119,8 @@ def parse():
     parser.add_argument('--serving-slots-3d', type = int, default = 3000000, help = '3D particles in flight')
     parser.add_argument('--serving-3d-end', type = float, default = 0.1, help = 'simulated seconds of a 3D request')
     parser.add_argument('--no-serving', action = 'store_true', help = 'skip the serving stream')
+    parser.add_argument('--serving-one-fleet', action = 'store_true',
+                        help = 'serve every 2D run in one fleet rather than a fleet per particle spacing')
     parser.add_argument('--no-gate', action = 'store_true', help = "skip Grilli's 1:35 case from the exact start")
     parser.add_argument('--gate-only', action = 'store_true',
                         help = "run only Grilli's case and print its record (for profiling the breaking read-out)")
@@ -362,7 +364,8 @@ def servingRun(args) -> dict:
         requests.append(Request(key = len(requests), arrival = arrival, case = case,
                                 endTime = args.serving_3d_end if dimensions == 3 else 8.0))
         arrival += float(rng.exponential(1.0 / rate))
-    server = Server(budgetBytes = 200e9, slots2D = args.serving_slots_2d, slots3D = args.serving_slots_3d)
+    server = Server(budgetBytes = 200e9, slots2D = args.serving_slots_2d, slots3D = args.serving_slots_3d,
+                    fleetsBySpacing = not args.serving_one_fleet)
     start = time.perf_counter()
     served = server.serve(requests, wallLimit = args.serving_wall)
     wall = time.perf_counter() - start
@@ -384,10 +387,11 @@ def servingRun(args) -> dict:
             'completion_p99_s': latencyPercentiles(finished)['p99'],
             'throughput': updates / wall, 'unit': 'particle-updates/s',
             'slots_2d': args.serving_slots_2d, 'slots_3d': args.serving_slots_3d, 'end_3d_s': args.serving_3d_end,
-            'fleets': {f'{d}d': {'fluid_updates': fleet.fluidUpdates, 'device_s': fleet.deviceSeconds,
-                                 'updates_per_device_s': fleet.fluidUpdates / fleet.deviceSeconds
-                                 if fleet.deviceSeconds > 0 else None}
-                       for d, fleet in sorted(server.fleets.items())},
+            'fleets_by_spacing': server.fleetsBySpacing,
+            'fleets': sorted(({'fleet': str(key), 'fluid_updates': fleet.fluidUpdates, 'device_s': fleet.deviceSeconds,
+                               'updates_per_device_s': fleet.fluidUpdates / fleet.deviceSeconds
+                               if fleet.deviceSeconds > 0 else None}
+                              for key, fleet in server.fleets.items()), key = lambda entry: entry['fleet']),
             'max_queue_depth': max(depths, default = 0),
             'mean_queue_depth': float(np.mean(depths)) if depths else 0.0,
             'particles_max': max(case.count for case in pool), 'particles_min': min(case.count for case in pool)}

@@ -393,11 +393,14 @@ class Server:
     '''
 
     budgetBytes: float = 200e9
-    # the slots of each fleet: at most this many particles in flight in the 2D and in the 3D fleet
-    # (0: no limit), so the runs on the device finish at the pace one batch steps and the rest queue;
-    # a request larger than its fleet's slots runs alone once its fleet is empty
+    # the slots of each dimension: at most this many particles in flight in its fleets (0: no limit),
+    # so the runs on the device finish at the pace one batch steps and the rest queue; a request
+    # larger than its dimension's slots runs alone once that dimension's fleets are empty
     slots2D: int = 0
     slots3D: int = 0
+    # 2D runs in a fleet per particle spacing: a fleet steps until its every member reaches its next
+    # reading, and runs of one spacing take about as many steps to get there, so few stop and wait
+    fleetsBySpacing: bool = False
     precision: str = 'float32'
     lanes: int = 4
     chunk: int = 8
@@ -412,12 +415,23 @@ class Server:
         capacity = self.listCapacity2D if case.dimensions == 2 else self.listCapacity3D
         return case.count * bytesPerParticle(case.dimensions, capacity)
 
-    def fleet(self, dimensions: int) -> Fleet:
-        if dimensions not in self.fleets:
+    def fleetKey(self, case):
+        '''The fleet a run joins: its dimension, or with fleetsBySpacing a 2D run's (2, spacing).'''
+        if self.fleetsBySpacing and case.dimensions == 2:
+            return (2, round(float(case.spacing), 9))
+        return case.dimensions
+
+    @staticmethod
+    def dimensionOf(key) -> int:
+        return key[0] if isinstance(key, tuple) else key
+
+    def fleet(self, key) -> Fleet:
+        if key not in self.fleets:
+            dimensions = self.dimensionOf(key)
             capacity = self.listCapacity2D if dimensions == 2 else self.listCapacity3D
-            self.fleets[dimensions] = Fleet(dimensions, self.precision, lanes = self.lanes, listCapacity = capacity,
-                                            chunk = self.chunk, measure = self.measure)
-        return self.fleets[dimensions]
+            self.fleets[key] = Fleet(dimensions, self.precision, lanes = self.lanes, listCapacity = capacity,
+                                     chunk = self.chunk, measure = self.measure)
+        return self.fleets[key]
 
     def refuse(self, request, now: float, reason: str) -> None:
         '''Record `request` as ended at `now` without a run, for `reason`.'''
@@ -461,7 +475,10 @@ class Server:
             # Membership changes while every fleet is idle: finished runs leave, admitted runs join
             joining: dict = {}
             still: list = []
-            busy = {d: sum(m.case.count for m in fleet.members if not m.done) for d, fleet in self.fleets.items()}
+            busy: dict = {}
+            for key, fleet in self.fleets.items():
+                d = self.dimensionOf(key)
+                busy[d] = busy.get(d, 0) + sum(m.case.count for m in fleet.members if not m.done)
             for request in queued:
                 d = request.case.dimensions
                 slots = self.slots2D if d == 2 else self.slots3D
@@ -470,17 +487,17 @@ class Server:
                     inUse += self.footprint(request.case)
                     busy[d] = busy.get(d, 0) + request.case.count
                     admittedAt[request.key] = now
-                    joining.setdefault(request.case.dimensions, []).append(
+                    joining.setdefault(self.fleetKey(request.case), []).append(
                         Member(key = request, case = request.case, endTime = request.endTime))
                 else:
                     still.append(request)
             queued = still
             self.queueDepth.append((now, len(queued)))
-            for dimensions in set(joining) | set(self.fleets):
-                fleet = self.fleet(dimensions)
+            for key in set(joining) | set(self.fleets):
+                fleet = self.fleet(key)
                 leaving = [member for member in fleet.members if member.done]
-                if leaving or joining.get(dimensions):
-                    fleet.rebuild(leaving, joining.get(dimensions, []))
+                if leaving or joining.get(key):
+                    fleet.rebuild(leaving, joining.get(key, []))
             active = [fleet for fleet in self.fleets.values() if fleet.batch is not None
                       and not all(member.done for member in fleet.members)]
             if not active and not readings:

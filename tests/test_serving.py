@@ -88,3 +88,22 @@ def testASmallerRequestPassesOneWaitingForRoom():
     assert records['b'].admitted > records['a'].admitted
     assert max(depth for _, depth in server.queueDepth) == 1
     assert all(record.refused == '' for record in served)
+
+
+@pytest.mark.timeout(60)
+def testFleetsBySpacingShareTheSlots():
+    # With a fleet per 2D spacing, runs of two spacings join two fleets; the dimension's slots hold
+    # the particles of both, so a third run that would overflow them waits for one to finish
+    a, b, c = (solitaryOnSlope(particlesPerDepth = ppd, seed = seed) for ppd, seed in ((8, 1), (10, 2), (8, 3)))
+    server, fleets = standInServer(budget = 1e12)
+    server.fleetsBySpacing = True
+    server.slots2D = a.count + b.count
+    served = server.serve([Request(key = 'a', arrival = 0.0, case = a, endTime = 2.0),
+                           Request(key = 'b', arrival = 0.0, case = b, endTime = 3.0),
+                           Request(key = 'c', arrival = 0.0, case = c, endTime = 1.0)])
+    assert sorted(record.key for record in served) == ['a', 'b', 'c']
+    assert set(fleets) == {server.fleetKey(a), server.fleetKey(b)} and server.fleetKey(a) == server.fleetKey(c)
+    assert fleets[server.fleetKey(a)].joined == ['a', 'c'] and fleets[server.fleetKey(b)].joined == ['b']
+    records = {record.key: record for record in served}
+    assert records['c'].admitted > records['a'].admitted
+    assert max(depth for _, depth in server.queueDepth) == 1
