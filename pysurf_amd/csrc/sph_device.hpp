@@ -511,8 +511,8 @@ struct CoarseSupport {
     template <int D>
     __device__ __forceinline__ CoarseSupport(const SupportBand<Real>& band, const Self<D, Real>& me, Real shift) {
         const Real t = band.t;
-        const Real reachOf = fabs(me.p[0]) + fabs(me.p[1]) + fabs(me.p[2]) + fabs(shift) + Real(8) * t;
-        const Real slack = Real(1.0 / 65536.0) + Real(16.0 / 16777216.0) * reachOf / t;
+        const Real bound = fabs(me.p[0]) + fabs(me.p[1]) + fabs(me.p[2]) + fabs(shift) + Real(8) * t;
+        const Real slack = Real(1.0 / 65536.0) + Real(16.0 / 16777216.0) * bound / t;
         in = band.below - slack * t * t;
         outScale = Real(1) - slack;
         above = band.above;
