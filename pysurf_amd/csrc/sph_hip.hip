@@ -409,21 +409,48 @@ struct Sources {
     Real* pressureOut;       // the Tait pressure per slot, written when non-null
 };
 
-// Sorted particle s's record from slot i's state: the float32 point and remainder of its float64
-// position (sphGpu.py:1010-1019), its velocity and density as the pair sums read them
-// (sphGpu.py:1021-1029), rho - rho0 taken in float64 (sphGpu.py:1031-1038), the Tait pressure
-// (sphGpu.py:1058-1070), p / rho^2 and m / rho; returns |v| of a fluid particle (infinite when its
-// velocity or position is not finite), 0 on the boundary.
+// What a gather reads of slot i's state, loaded before the record is made, so that a kernel with
+// other work to do first (cellRankGather) has the loads in flight meanwhile.
+template <typename Real>
+struct GatherIn {
+    double x[3], u[3], rho;
+    Real pressure, mass, length;
+    int32_t key;
+    bool fluid;
+};
+
 template <int D, typename Real>
-__device__ __forceinline__ double gatherOne(const RunDev& run, int32_t i, int64_t s, const Sources<Real>& src,
-                                            Records<Real>& rec) {
+__device__ __forceinline__ GatherIn<Real> gatherLoad(int32_t i, const Sources<Real>& src) {
+    GatherIn<Real> in;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        in.x[a] = a < D ? src.pos[a][i] : 0.0;
+        in.u[a] = a < D ? src.vel[a][i] : 0.0;
+    }
+    in.rho = src.density[i];
+    in.pressure = src.pressure ? src.pressure[i] : Real(0);
+    in.mass = src.mass[i];
+    in.length = src.len[i];
+    in.key = src.key[i];
+    in.fluid = src.fluid[i] != 0;
+    return in;
+}
+
+// Sorted particle s's record from slot i's state as gatherLoad read it: the float32 point and
+// remainder of its float64 position (sphGpu.py:1010-1019), its velocity and density as the pair
+// sums read them (sphGpu.py:1021-1029), rho - rho0 taken in float64 (sphGpu.py:1031-1038), the
+// Tait pressure (sphGpu.py:1058-1070), p / rho^2 and m / rho; returns |v| of a fluid particle
+// (infinite when its velocity or position is not finite), 0 on the boundary.
+template <int D, typename Real>
+__device__ __forceinline__ double gatherStore(const RunDev& run, const GatherIn<Real>& in, int32_t i, int64_t s,
+                                              const Sources<Real>& src, Records<Real>& rec) {
     Fields<Real> f;
     Real p[3] = {Real(0), Real(0), Real(0)}, v[3] = {Real(0), Real(0), Real(0)};
     float l[3] = {0.0f, 0.0f, 0.0f};
     double v2 = 0.0;
 #pragma unroll
     for (int a = 0; a < D; ++a) {
-        const double x = src.pos[a][i];
+        const double x = in.x[a];
         if constexpr (sizeof(Real) == 4) {
             const float point = (float)x;
             p[a] = point;
@@ -431,15 +458,15 @@ __device__ __forceinline__ double gatherOne(const RunDev& run, int32_t i, int64_
         } else {
             p[a] = x;
         }
-        const double u = src.vel[a][i];
+        const double u = in.u[a];
         v[a] = (Real)u;
         v2 += u * u;
     }
-    const double rho = src.density[i];
-    const bool isFluid = src.fluid[i] != 0;
+    const double rho = in.rho;
+    const bool isFluid = in.fluid;
     Real pressure;
     if (src.pressure) {
-        pressure = src.pressure[i];
+        pressure = in.pressure;
     } else {
         const double ratio = rho / run.rho0, sq = ratio * ratio;
         double value = run.stiffness * (sq * sq * sq * ratio - 1.0);
@@ -447,7 +474,7 @@ __device__ __forceinline__ double gatherOne(const RunDev& run, int32_t i, int64_
         pressure = (Real)value;
         if (src.pressureOut) src.pressureOut[i] = pressure;
     }
-    const Real rhoR = (Real)rho, massR = src.mass[i], h = src.len[i];
+    const Real rhoR = (Real)rho, massR = in.mass, h = in.length;
     for (int a = 0; a < 3; ++a) {
         f.p[a] = p[a];
         f.l[a] = l[a];
@@ -461,11 +488,17 @@ __device__ __forceinline__ double gatherOne(const RunDev& run, int32_t i, int64_
     f.pr2 = pressure / (rhoR * rhoR);
     f.press = pressure;
     rec.r[s] = pack<D, Real>(f);
-    rec.key[s] = src.key[i];
+    rec.key[s] = in.key;
     bool finite = v2 < INFINITY;
 #pragma unroll
-    for (int a = 0; a < D; ++a) finite = finite && fabs(src.pos[a][i]) < INFINITY;
+    for (int a = 0; a < D; ++a) finite = finite && fabs(in.x[a]) < INFINITY;
     return isFluid ? (finite ? sqrt(v2) : INFINITY) : 0.0;
+}
+
+template <int D, typename Real>
+__device__ __forceinline__ double gatherOne(const RunDev& run, int32_t i, int64_t s, const Sources<Real>& src,
+                                            Records<Real>& rec) {
+    return gatherStore<D, Real>(run, gatherLoad<D, Real>(i, src), i, s, src, rec);
 }
 
 // Each sorted particle's record from its slot's state (gatherOne); resident, each slot's |v| for
@@ -497,11 +530,16 @@ __global__ void cellRankGather(const int32_t* __restrict__ start, const int32_t*
                                const int32_t* __restrict__ runOf, Sources<Real> src, Records<Real> rec, Clock clock) {
     const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (a < n) {
+        // the slot's state is read while its place in the cell is counted
+        const int32_t slot = scattered[a];
+        const int r = runOf[slot];
+        const bool active = clock.active[r] != 0;
+        GatherIn<Real> in{};
+        if (active) in = gatherLoad<D, Real>(slot, src);
         int32_t v;
         const int32_t at = rankInCell(start, key, x, eighth, scattered, eighths, a, v);
         order[at] = v;
-        const int r = runOf[v];
-        if (clock.active[r]) clock.speed[at] = gatherOne<D, Real>(runs[r], v, at, src, rec);
+        if (active) clock.speed[at] = gatherStore<D, Real>(runs[r], in, v, at, src, rec);
     }
     if (a < cells) {
         count[a] = 0;
