@@ -97,6 +97,28 @@ def testTheNeighborListAndTheCellWalkAgreeBitForBit(precision):
             assert np.array_equal(states[0].__dict__[name], other.__dict__[name]), name
 
 
+@pytest.mark.parametrize('dimensions', [2, 3])
+def testTheLdsCopysCapLeavesTheStepAlone(dimensions):
+    # A sweep workgroup reads the records it copied into LDS and the rest from HBM, with the same
+    # values: the default copy, a 4 KiB one (most neighbors from HBM) and none step identically, in
+    # the list build, the three sweeps and the kick the continuity sweep applies to what it reads
+    from pysurf_amd.batch import BatchEngine
+    print(f'LDS copy cap {dimensions}D', flush = True)
+    case = smallCase(seed = 9) if dimensions == 2 else smallRidge(seed = 3, lengthRatio = 1.0)
+    states = []
+    for cap in (-1, 4096, 0):
+        batch = BatchEngine([case], 'float32', lanes = 4 if dimensions == 2 else 8,
+                            listCapacity = -1 if dimensions == 2 else 160)
+        batch.engine.setNearCells(cap)
+        batch.setClocks(maxSteps = 20)
+        batch.advance(20)
+        batch.synchronize()
+        states.append(batch.state(0))
+    for other in states[1:]:
+        for name in ('positions', 'velocities', 'densities', 'pressures'):
+            assert np.array_equal(states[0].__dict__[name], other.__dict__[name]), name
+
+
 def testARunAloneAndInABatchAgreeBitForBit():
     # The same run stepped alone and packed between two others of different sizes and physics
     print('batched vs solo', flush = True)
