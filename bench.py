@@ -300,7 +300,8 @@ def ridgeRun(args) -> dict:
     log(f'3D ridge: {case.count} particles ({case.fluidCount} fluid)')
     batch = BatchEngine([case], 'float32')
     launch = autotune.tune(batch, steps = 3, candidates = [(lanes, 256, 768) for lanes in (4, 8, 16)] +
-                                                        [(8, 256, 768, cap) for cap in (0, 8192, 16384, 24576)])
+                                                        [(8, 256, 768, cap) for cap in (0, 8192, 16384, 24576)] +
+                                                        [(8, 256, 768, -1, cap) for cap in (8192, 16384)])
     before = batch.globalSums()[0]
     warm(batch, 2, args.chunk)
     # timed in segments of 100 steps (about 20 s), each reported as it ends
@@ -328,7 +329,7 @@ def ridgeRun(args) -> dict:
             'fluid_particles': case.fluidCount, 'steps': args.ridge_steps, 'value': value,
             'unit': 'particle-updates/s', 'ms_per_step': 1000.0 * wall / args.ridge_steps,
             'mean_neighbors': entries,
-            'launch': {key: launch[key] for key in ('lanes', 'block', 'listCapacity', 'nearCells')},
+            'launch': {key: launch[key] for key in ('lanes', 'block', 'listCapacity', 'nearCells', 'nearCellsBuild')},
             'checks': checks,
             'kernel_ms_per_step': {name: entry['ms'] / 2 for name, entry in profile.items() if entry['launches']}}
 

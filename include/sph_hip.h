@@ -367,10 +367,11 @@ int sph_kernel_times(sph_context* ctx, double* milliseconds, int64_t* launches, 
 int sph_set_launch(sph_context* ctx, int32_t blockSize, int32_t lanes, int32_t listCapacity);
 
 /* The cap of the records a pair-sweep workgroup copies into LDS around its particles, in bytes
- * (0 to 98304; 0 reads every neighbor from HBM; -1 picks 32 KiB). No counterpart in the reference:
- * a launch-shape parameter for the autotuner (pysurf_amd/autotune.py). A neighbor outside the copy
- * is read from HBM with the same value, so results do not depend on it. */
-int sph_set_near_cells(sph_context* ctx, int32_t bytes);
+ * (0 to 98304; 0 reads every neighbor from HBM; -1 picks 32 KiB), and the same for the device
+ * step's list build (-1 takes `bytes`). No counterpart in the reference: launch-shape parameters for
+ * the autotuner (pysurf_amd/autotune.py). A neighbor outside the copy is read from HBM with the same
+ * value, so results do not depend on them. */
+int sph_set_near_cells(sph_context* ctx, int32_t bytes, int32_t buildBytes);
 
 #ifdef __cplusplus
 }

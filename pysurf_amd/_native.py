@@ -137,7 +137,7 @@ signatures = {
     'sph_split_apply': [ctypes.c_void_p, ctypes.POINTER(State), ctypes.c_double, ctypes.c_double],
     'sph_merge': [ctypes.c_void_p, ctypes.POINTER(MergeSpec), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
     'sph_set_launch': [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32],
-    'sph_set_near_cells': [ctypes.c_void_p, ctypes.c_int32],
+    'sph_set_near_cells': [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32],
     'sph_profile': [ctypes.c_void_p, ctypes.c_int32],
     'sph_kernel_times': [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32],
     'sph_state_sums': [ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(Run), ctypes.c_int32, ctypes.POINTER(State),

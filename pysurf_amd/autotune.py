@@ -3,7 +3,8 @@
 Launch shapes for the pair sweeps, chosen by measurement on the device at hand.
 
 A shape is (lanes per particle, threads per workgroup, neighbor-list entries per particle, and
-optionally the cap in bytes of a sweep workgroup's LDS copy of the records around its particles).
+optionally the cap in bytes of a sweep workgroup's LDS copy of the records around its particles and
+the same cap for the step's list build apart).
 More lanes per particle split each particle's neighbors over more lanes, which fills the GPU when a
 context holds few particles and costs shuffles and idle lanes when it holds many; the list
 capacity trades the list's HBM footprint against the particles that overflow it and walk their
@@ -22,15 +23,17 @@ from __future__ import annotations
 
 import math
 
-# (lanes, workgroup threads, list capacity[, LDS copy cap]); capacity -1 is 64 entries in 2D and
-# 160 in 3D, the cap -1 32 KiB. A 2D workgroup's neighborhood fits 32 KiB at every lane count, but a
-# smaller copy leaves room for one more workgroup on a CU; a 3D one outgrows it, so 3D also tries
-# smaller copies for the same reason.
+# (lanes, workgroup threads, list capacity[, LDS copy cap[, the list build's cap]]); capacity -1 is
+# 64 entries in 2D and 160 in 3D, a cap -1 32 KiB (the list build's: the sweeps'). A 2D workgroup's
+# neighborhood fits 32 KiB at every lane count, but a smaller copy leaves room for one more workgroup
+# on a CU; a 3D one outgrows it, so 3D also tries smaller copies for the same reason, for every
+# kernel or for the list build alone.
 CANDIDATES = {
     2: [(lanes, block, -1) for lanes in (1, 2, 4, 8, 16) for block in (128, 256)] +
        [(lanes, 256, -1, cap) for lanes in (4, 8) for cap in (12288, 16384)],
     3: [(lanes, block, -1) for lanes in (2, 4, 8, 16) for block in (128, 256)] +
-       [(lanes, 256, -1, cap) for lanes in (4, 8) for cap in (0, 8192, 16384, 24576)],
+       [(lanes, 256, -1, cap) for lanes in (4, 8) for cap in (0, 8192, 16384, 24576)] +
+       [(lanes, 256, -1, -1, cap) for lanes in (4, 8) for cap in (8192, 16384)],
 }
 
 _cache: dict = {}
@@ -67,11 +70,16 @@ def candidatesFor(batch) -> list:
     return shapes
 
 
+def full(shape) -> tuple:
+    '''A shape with its caps: (lanes, block, capacity, cap, the list build's cap).'''
+    return tuple(shape) + (-1,) * (5 - len(shape))
+
+
 def apply(batch, shape) -> None:
-    '''Set a (lanes, block, capacity[, cap]) shape on the batch's engine.'''
-    lanes, block, capacity = shape[:3]
+    '''Set a (lanes, block, capacity[, cap[, build cap]]) shape on the batch's engine.'''
+    lanes, block, capacity, cap, build = full(shape)
     batch.engine.setLaunch(block, lanes, capacity)
-    batch.engine.setNearCells(shape[3] if len(shape) > 3 else -1)
+    batch.engine.setNearCells(cap, build)
 
 
 # Steps timed per candidate for a context of fewer particles than this, at the least: a small
@@ -123,24 +131,26 @@ def tune(batch, steps: int = 8, candidates = None, reuse: bool = True) -> dict:
     key = cacheKey(batch.dimensions, batch.precision, batch.count, lengthRatios(batch))
     if reuse and key in _cache and candidates is None:
         chosen = _cache[key]
-        apply(batch, (chosen['lanes'], chosen['block'], chosen['listCapacity'], chosen['nearCells']))
+        apply(batch, (chosen['lanes'], chosen['block'], chosen['listCapacity'], chosen['nearCells'],
+                      chosen['nearCellsBuild']))
         return chosen
     shapes = candidates if candidates is not None else candidatesFor(batch)
     if batch.count < SMALL_CONTEXT:
         steps = max(steps, SMALL_CONTEXT_STEPS)
     timings = []
     for shape in shapes:
-        shape = tuple(shape) + ((-1,) if len(shape) == 3 else ())
+        shape = full(shape)
         apply(batch, shape)
         batch.load()
         batch.setClocks(maxSteps = 2**62)
         timings.append((timeSteps(batch, steps),) + shape)
     best = min(timings)
     chosen = {'lanes': best[1], 'block': best[2], 'listCapacity': best[3], 'nearCells': best[4],
-              'ms_per_step': best[0],
-              'timings': [{'lanes': l, 'block': b, 'listCapacity': c, 'nearCells': n, 'ms_per_step': t}
-                          for t, l, b, c, n in timings]}
-    apply(batch, (chosen['lanes'], chosen['block'], chosen['listCapacity'], chosen['nearCells']))
+              'nearCellsBuild': best[5], 'ms_per_step': best[0],
+              'timings': [{'lanes': l, 'block': b, 'listCapacity': c, 'nearCells': n, 'nearCellsBuild': nb,
+                           'ms_per_step': t} for t, l, b, c, n, nb in timings]}
+    apply(batch, (chosen['lanes'], chosen['block'], chosen['listCapacity'], chosen['nearCells'],
+                  chosen['nearCellsBuild']))
     batch.load()
     if candidates is None:
         _cache[key] = chosen

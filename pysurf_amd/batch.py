@@ -122,7 +122,8 @@ class BatchEngine:
         self.engine.setLaunch(blockSize, lanes, listCapacity)
         self.identity = None
         self.load(carry)
-        self.launch = {'lanes': lanes, 'block': blockSize, 'listCapacity': listCapacity, 'nearCells': -1}
+        self.launch = {'lanes': lanes, 'block': blockSize, 'listCapacity': listCapacity, 'nearCells': -1,
+                       'nearCellsBuild': -1}
         if tune:
             from pysurf_amd import autotune
             self.launch = autotune.tune(self)

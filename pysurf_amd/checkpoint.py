@@ -134,7 +134,8 @@ def save(batch, path: str, extra: dict | None = None) -> None:
         'dimensions': batch.dimensions, 'precision': batch.precision, 'count': count,
         'runs': [{'first': batch.offsets[r], 'count': batch.offsets[r + 1] - batch.offsets[r]} for r in range(runs)],
         'digests': [runDigest(case) for case in batch.cases],
-        'launch': {key: int(batch.launch.get(key, -1)) for key in ('lanes', 'block', 'listCapacity', 'nearCells')},
+        'launch': {key: int(batch.launch.get(key, -1))
+                   for key in ('lanes', 'block', 'listCapacity', 'nearCells', 'nearCellsBuild')},
         'clock': {'time': [float(t).hex() for t in clock['time']], 'steps': [int(s) for s in clock['steps']],
                   'until': [float(u).hex() for u in batch.until], 'maxSteps': [int(m) for m in batch.maxSteps]},
         'extra': extra if extra is not None else {},
@@ -165,8 +166,9 @@ def restore(batch, path: str) -> dict:
                          'the saved runs (settings or bed hash).')
     launch = dict(header['launch'])
     launch.setdefault('nearCells', -1)
+    launch.setdefault('nearCellsBuild', -1)
     batch.engine.setLaunch(launch['block'], launch['lanes'], launch['listCapacity'])
-    batch.engine.setNearCells(launch['nearCells'])
+    batch.engine.setNearCells(launch['nearCells'], launch['nearCellsBuild'])
     batch.launch = launch
     engine, count = batch.engine, header['count']
     with torch.cuda.device(engine.device), torch.cuda.stream(engine.stream):

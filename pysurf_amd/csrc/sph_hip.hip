@@ -2701,6 +2701,7 @@ struct sph_context {
     int block = 256;
     int lanes = 0;           // lanes per particle in the sweeps; 0 picks by dimension
     int32_t nearBytes = -1;  // the cap of a sweep workgroup's LDS copy of records; -1 picks by dimension
+    int32_t nearBytesBuild = -1;   // the same for the step's list build; -1 takes nearBytes
     int listCapacity = -1;   // neighbor-list entries per particle; -1 picks by dimension, 0 walks the cells
     NeighborList list{};
     // A copy of the bound state the state sort permutes from, allocated on first use
@@ -2993,11 +2994,12 @@ inline unsigned tileGrid(const sph_context* ctx) { return (unsigned)ctx->tileCou
 // per particle in 2D and 9 in 3D plus the ends of the ranges; the copy is capped (32 KiB unless
 // sph_set_near_cells says otherwise) so that LDS leaves the waves' occupancy alone.
 template <int D, typename Real>
-int32_t nearRecords(const sph_context* ctx) {
+int32_t nearRecords(const sph_context* ctx, bool build = false) {
     if (!SPH_NEAR_CELLS) return 0;
     const int32_t per = (int32_t)ctx->tilePer;
     const int32_t want = D == 2 ? 4 * per + 64 : SPH_NEAR_PER_3D * per + 128;
-    const int32_t bytes = ctx->nearBytes >= 0 ? ctx->nearBytes : (D == 2 ? 32768 : SPH_NEAR_BYTES_3D);
+    const int32_t chosen = build && ctx->nearBytesBuild >= 0 ? ctx->nearBytesBuild : ctx->nearBytes;
+    const int32_t bytes = chosen >= 0 ? chosen : (D == 2 ? 32768 : SPH_NEAR_BYTES_3D);
     const int32_t most = (int32_t)(bytes / sizeof(Record<Real>));
     return want < most ? want : most;
 }
@@ -3019,10 +3021,12 @@ int32_t hitCapacity(const sph_context* ctx) {
     const int32_t keep = (D == 2 ? 6144 : 16384) / (int32_t)sizeof(int32_t) / per;
     return std::max(0, std::min(keep, ctx->list.capacity));
 }
-// The fused list build's dynamic LDS: the NearCells copy, then the hits it keeps.
+// The fused list build's dynamic LDS: the NearCells copy (at the list build's own cap), then the
+// hits it keeps.
 template <int D, typename Real>
 size_t buildLds(const sph_context* ctx) {
-    return sweepLds<D, Real>(ctx, 0) + (size_t)ctx->tilePer * (hitCapacity<D>(ctx) | 1) * sizeof(int32_t);
+    return (size_t)nearRecords<D, Real>(ctx, true) * sizeof(Record<Real>) +
+           (size_t)ctx->tilePer * (hitCapacity<D>(ctx) | 1) * sizeof(int32_t);
 }
 
 template <typename Real>
@@ -3262,7 +3266,7 @@ int stepOnce(sph_context* ctx, int phase, int useClock, double forcedDt, int fil
             WITH_LANES(ctx, TIMED(ctx, kNeighbors,
                                   (buildNeighbors<D, Real, L, true><<<tileGrid(ctx), b, buildLds<D, Real>(ctx), ctx->stream>>>(
                             ctx->dRuns, tilesOf(ctx), ctx->order, ctx->cellStart, ctx->cellEighths, rec, ctx->list, out,
-                                      ctx->clock, y64, 1, nearRecords<D, Real>(ctx), hitCapacity<D>(ctx), n, mode))));
+                                      ctx->clock, y64, 1, nearRecords<D, Real>(ctx, true), hitCapacity<D>(ctx), n, mode))));
         } else {
             WITH_LANES(ctx, TIMED(ctx, kSweepWall,
                                   (sweepWallAndGradients<D, Real, L><<<tileGrid(ctx), b, 0, ctx->stream>>>(
@@ -4749,12 +4753,13 @@ int sph_set_launch(sph_context* ctx, int32_t blockSize, int32_t lanes, int32_t l
     return SPH_OK;
 }
 
-int sph_set_near_cells(sph_context* ctx, int32_t bytes) {
+int sph_set_near_cells(sph_context* ctx, int32_t bytes, int32_t buildBytes) {
     if (!ctx) return SPH_ERR_VALUE;
-    if (bytes < -1 || bytes > 98304)
+    if (bytes < -1 || bytes > 98304 || buildBytes < -1 || buildBytes > 98304)
         return fail(ctx, SPH_ERR_VALUE, "The LDS copy's cap must be from 0 to 98304 bytes (-1 picks by dimension).");
-    if (bytes != ctx->nearBytes) {
+    if (bytes != ctx->nearBytes || buildBytes != ctx->nearBytesBuild) {
         ctx->nearBytes = bytes;
+        ctx->nearBytesBuild = buildBytes;
         dropGraph(ctx);
     }
     return SPH_OK;

@@ -430,9 +430,10 @@ class Engine:
         self.check(self.library.sph_set_launch(self.handle, int(blockSize), int(lanes), int(listCapacity)),
                    'sph_set_launch')
 
-    def setNearCells(self, bytes: int = -1) -> None:
-        '''The cap of a sweep workgroup's LDS copy of the records around its particles (-1: 32 KiB).'''
-        self.check(self.library.sph_set_near_cells(self.handle, int(bytes)), 'sph_set_near_cells')
+    def setNearCells(self, bytes: int = -1, buildBytes: int = -1) -> None:
+        '''The cap of a sweep workgroup's LDS copy of the records around its particles (-1: 32 KiB), and
+        the same for the device step's list build (-1: `bytes`).'''
+        self.check(self.library.sph_set_near_cells(self.handle, int(bytes), int(buildBytes)), 'sph_set_near_cells')
 
 
 def boundaryOf(boundary, dimensions: int, device: torch.device, pistonSpeed: float = 0.0) -> tuple:

@@ -62,4 +62,4 @@ def testBadArgumentsFailWithoutADevice():
     assert library.sph_create(ctypes.byref(handle), 4, 0, None) == _native.SPH_ERR_VALUE
     assert library.sph_create(ctypes.byref(handle), 2, 7, None) == _native.SPH_ERR_VALUE
     assert library.sph_destroy(None) == _native.SPH_OK
-    assert library.sph_set_near_cells(None, 0) == _native.SPH_ERR_VALUE
+    assert library.sph_set_near_cells(None, 0, -1) == _native.SPH_ERR_VALUE

@@ -28,6 +28,7 @@ def testLongLengthsTryLongerLists():
     capacities = {shape[2] for shape in shapes}
     assert capacities == {-1, 192, 288}
     assert all(len(shape) == 3 or shape[2] == -1 for shape in shapes)
+    assert all(len(autotune.full(shape)) == 5 for shape in shapes)
     assert all(capacity % 32 == 0 for capacity in capacities if capacity > 0)
 
 
