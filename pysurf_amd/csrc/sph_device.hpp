@@ -676,6 +676,10 @@ struct NearCells {
     bool kicked;
     const Real* A;
     Real dt;
+    // With `withG` (2D), the copy also holds each record's velocity gradient G (4 values, from `G`)
+    // as a fifth quad, which the acceleration sweep's limiter reads with the record.
+    bool withG = false;
+    const Real* G = nullptr;
 
     __device__ __forceinline__ void kick(Record<Real>& record, int32_t k) const {
         if (!kicked || !(signedLength<D, Real>(record) > Real(0))) return;
@@ -738,17 +742,24 @@ struct NearCells {
             Record<Real> b = r[fromB];
             kick(a, fromA);
             kick(b, fromB);
+            Quad<Real> ga{}, gb{};
+            if (withG) {
+                ga = *(const Quad<Real>*)(G + (int64_t)fromA * 4);
+                gb = *(const Quad<Real>*)(G + (int64_t)fromB * 4);
+            }
             LdsVec4<Real>* to = shared + first;
             storeLds(to, a.q0);
             storeLds(to + stride, a.q1);
             storeLds(to + 2 * stride, a.q2);
             storeLds(to + 3 * stride, a.q3);
+            if (withG) storeLds(to + 4 * stride, ga);
             if (both) {
                 to = shared + second;
                 storeLds(to, b.q0);
                 storeLds(to + stride, b.q1);
                 storeLds(to + 2 * stride, b.q2);
                 storeLds(to + 3 * stride, b.q3);
+                if (withG) storeLds(to + 4 * stride, gb);
             }
         }
         __syncthreads();
@@ -770,6 +781,21 @@ struct NearCells {
         for (int i = 0; i < kRows; ++i)
             if (k >= lo[i] && k < hi[i]) index = at[i] + (k - lo[i]);
         return index >= 0 && index < held ? ldsValue<Real>(index, 0, 0, stride) : r[k].q0.x;
+    }
+
+    // The 2D velocity gradient G of sorted particle k, from the copy when it holds it.
+    __device__ __forceinline__ void gradient(int32_t k, Real (&g)[4]) const {
+        int32_t index = -1;
+#pragma unroll
+        for (int i = 0; i < kRows; ++i)
+            if (k >= lo[i] && k < hi[i]) index = at[i] + (k - lo[i]);
+        Quad<Real> q;
+        if (withG && index >= 0 && index < held) q = ldsQuad<Real>(index, 4, stride);
+        else q = *(const Quad<Real>*)(G + (int64_t)k * 4);
+        g[0] = q.x;
+        g[1] = q.y;
+        g[2] = q.z;
+        g[3] = q.w;
     }
 
     // The record of sorted particle k, from LDS when copied there.

@@ -1340,6 +1340,9 @@ __global__ void SWEEP_BOUNDS((accelerationWaves<D, Real>()))
     cells.held = 0;
     cells.stride = cached;
     cells.kicked = false;
+    // 2D: the copy also holds each record's G for the limiter (the launch's LDS has room for it)
+    cells.G = rec.G;
+    cells.withG = D == 2 && sizeof(Real) == 4 && doAcceleration && (resident ? runs[r].limiter != 0 : hostLimiter != 0);
 #pragma unroll
     for (int i = 0; i < NearCells<D, Real>::kRows; ++i) cells.lo[i] = cells.hi[i] = cells.at[i] = 0;
     if (runSteps && cached > 0) {
@@ -1380,11 +1383,16 @@ __global__ void SWEEP_BOUNDS((accelerationWaves<D, Real>()))
             forEachNeighbor<D, Real, Lanes, true>(
                 run, start, cells, list, s, me, yy, lane, kernel.uniform,
                 [&](int32_t k, const Record<Real>& other, Real dx, Real dy, Real dz, Real rr, Real h) {
-                    // G_j with the record, before the sums need it
+                    // G_j with the record: from the copy in 2D, else before the sums need it
                     Real Gj[D * D];
-                    if (limiter && doAcceleration)
+                    if (limiter && doAcceleration) {
+                        if constexpr (D == 2) {
+                            cells.gradient(k, Gj);
+                        } else {
 #pragma unroll
-                        for (int e = 0; e < D * D; ++e) Gj[e] = rec.G[(int64_t)k * (D * D) + e];
+                            for (int e = 0; e < D * D; ++e) Gj[e] = rec.G[(int64_t)k * (D * D) + e];
+                        }
+                    }
                     const Real F = kernel.slopeOverR(rr, h);
                     const Real d[3] = {dx, dy, dz};
                     if (correction) {
@@ -3013,6 +3021,12 @@ template <int D, typename Real>
 size_t sweepLds(const sph_context* ctx, int sums) {
     return (size_t)nearRecords<D, Real>(ctx) * sizeof(Record<Real>) + (size_t)ctx->tilePer * sums * sizeof(Real);
 }
+// The acceleration sweep's: the NearCells copy with, in 2D float32, each record's G (a fifth quad).
+template <int D, typename Real>
+size_t accelerationLds(const sph_context* ctx) {
+    const bool withG = D == 2 && sizeof(Real) == 4;
+    return (size_t)nearRecords<D, Real>(ctx) * (sizeof(Record<Real>) + (withG ? sizeof(Quad<Real>) : 0));
+}
 
 template <int D>
 int32_t hitCapacity(const sph_context* ctx) {
@@ -3293,19 +3307,19 @@ int stepOnce(sph_context* ctx, int phase, int useClock, double forcedDt, int fil
             closureCoefficients<D, Real><<<gridFor(n, b), b, 0, ctx->stream>>>(
                 ctx->order, rec, static_cast<Real*>(ctx->eddy), st, (Real)ctx->closureC0, n);
         if (o.ledger)
-            WITH_LANES(ctx, (sweepAccelerations<D, Real, L, true, true><<<tileGrid(ctx), b, sweepLds<D, Real>(ctx, 0),
+            WITH_LANES(ctx, (sweepAccelerations<D, Real, L, true, true><<<tileGrid(ctx), b, accelerationLds<D, Real>(ctx),
                                                                           ctx->stream>>>(
                             ctx->dRuns, tilesOf(ctx), ctx->order, ctx->cellStart, rec, ctx->list, out, ctx->clock,
                             y64, 1, 1, 1, 0, nearRecords<D, Real>(ctx), n)));
         if (physicsOptions) {
             WITH_LANES(ctx, TIMED(ctx, kSweepAcceleration,
-                                  (sweepAccelerations<D, Real, L, true><<<tileGrid(ctx), b, sweepLds<D, Real>(ctx, 0),
+                                  (sweepAccelerations<D, Real, L, true><<<tileGrid(ctx), b, accelerationLds<D, Real>(ctx),
                                                                           ctx->stream>>>(
                             ctx->dRuns, tilesOf(ctx), ctx->order, ctx->cellStart, rec, ctx->list, out, ctx->clock,
                                       y64, 1, 1, 1, 0, nearRecords<D, Real>(ctx), n))));
         } else {
             WITH_LANES(ctx, TIMED(ctx, kSweepAcceleration,
-                                  (sweepAccelerations<D, Real, L><<<tileGrid(ctx), b, sweepLds<D, Real>(ctx, 0), ctx->stream>>>(
+                                  (sweepAccelerations<D, Real, L><<<tileGrid(ctx), b, accelerationLds<D, Real>(ctx), ctx->stream>>>(
                             ctx->dRuns, tilesOf(ctx), ctx->order, ctx->cellStart, rec, ctx->list, out, ctx->clock,
                                       y64, 1, 1, 1, 0, nearRecords<D, Real>(ctx), n))));
         }
@@ -3442,7 +3456,7 @@ int hostSweep(sph_context* ctx, int which, const double* velocity, const double*
             break;
         case 2:   // correction matrices
             out.corrections = static_cast<Real*>(out0);
-            WITH_LANES(ctx, (sweepAccelerations<D, Real, L><<<tileGrid(ctx), b, sweepLds<D, Real>(ctx, 0), ctx->stream>>>(
+            WITH_LANES(ctx, (sweepAccelerations<D, Real, L><<<tileGrid(ctx), b, accelerationLds<D, Real>(ctx), ctx->stream>>>(
                             ctx->dRuns, tilesOf(ctx), order, start, rec, list, out, ctx->clock, y64, 0, 0, 1, 0, nearRecords<D, Real>(ctx), n)));
             break;
         case 3:    // accelerations, with G first when the limiter or the closure needs it
@@ -3458,11 +3472,11 @@ int hostSweep(sph_context* ctx, int which, const double* velocity, const double*
             if (which == 8) break;
             out.acceleration = static_cast<Real*>(out0);
             if (closure) {
-                WITH_LANES(ctx, (sweepAccelerations<D, Real, L, true><<<tileGrid(ctx), b, sweepLds<D, Real>(ctx, 0),
+                WITH_LANES(ctx, (sweepAccelerations<D, Real, L, true><<<tileGrid(ctx), b, accelerationLds<D, Real>(ctx),
                                                                         ctx->stream>>>(
                             ctx->dRuns, tilesOf(ctx), order, start, rec, list, out, ctx->clock, y64, 0, 1, 0, limiter, nearRecords<D, Real>(ctx), n)));
             } else {
-                WITH_LANES(ctx, (sweepAccelerations<D, Real, L><<<tileGrid(ctx), b, sweepLds<D, Real>(ctx, 0), ctx->stream>>>(
+                WITH_LANES(ctx, (sweepAccelerations<D, Real, L><<<tileGrid(ctx), b, accelerationLds<D, Real>(ctx), ctx->stream>>>(
                             ctx->dRuns, tilesOf(ctx), order, start, rec, list, out, ctx->clock, y64, 0, 1, 0, limiter, nearRecords<D, Real>(ctx), n)));
             }
             break;
@@ -3473,7 +3487,7 @@ int hostSweep(sph_context* ctx, int which, const double* velocity, const double*
             const int correction = run0.correction;
             const bool renormalized = run0.diffusion == 3;
             if (correction || renormalized)
-                WITH_LANES(ctx, (sweepAccelerations<D, Real, L><<<tileGrid(ctx), b, sweepLds<D, Real>(ctx, 0), ctx->stream>>>(
+                WITH_LANES(ctx, (sweepAccelerations<D, Real, L><<<tileGrid(ctx), b, accelerationLds<D, Real>(ctx), ctx->stream>>>(
                             ctx->dRuns, tilesOf(ctx), order, start, rec, list, out, ctx->clock, y64, 0, 0, 1, 0, nearRecords<D, Real>(ctx), n)));
             if (renormalized)
                 WITH_LANES(ctx, (sweepFilters<D, Real, L><<<tileGrid(ctx), b, 0, ctx->stream>>>(
@@ -3495,7 +3509,7 @@ int hostSweep(sph_context* ctx, int which, const double* velocity, const double*
             break;
         }
         case 7:   // the renormalized density gradients, after the correction matrices
-            WITH_LANES(ctx, (sweepAccelerations<D, Real, L><<<tileGrid(ctx), b, sweepLds<D, Real>(ctx, 0), ctx->stream>>>(
+            WITH_LANES(ctx, (sweepAccelerations<D, Real, L><<<tileGrid(ctx), b, accelerationLds<D, Real>(ctx), ctx->stream>>>(
                             ctx->dRuns, tilesOf(ctx), order, start, rec, list, out, ctx->clock, y64, 0, 0, 1, 0, nearRecords<D, Real>(ctx), n)));
             WITH_LANES(ctx, (sweepFilters<D, Real, L><<<tileGrid(ctx), b, 0, ctx->stream>>>(
                             ctx->dRuns, tilesOf(ctx), order, start, rec, list, y64, 2, Real(0),
