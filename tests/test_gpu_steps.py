@@ -117,6 +117,11 @@ def testTheLdsCopysCapLeavesTheStepAlone(dimensions):
     for other in states[1:]:
         for name in ('positions', 'velocities', 'densities', 'pressures'):
             assert np.array_equal(states[0].__dict__[name], other.__dict__[name]), name
+    # a cap beyond the LDS a workgroup may take is refused, naming the range
+    with pytest.raises(ValueError, match = '98304'):
+        batch.engine.setNearCells(200000)
+    with pytest.raises(ValueError, match = '98304'):
+        batch.engine.setNearCells(-1, -5)
 
 
 def testARunAloneAndInABatchAgreeBitForBit():
