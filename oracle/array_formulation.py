@@ -5,8 +5,8 @@ The array formulation of pySurf's SPH equations, restated in numpy over explicit
 TEST INFRASTRUCTURE, NOT PRODUCT CODE: only tests/, `__graft_entry__.smoke()` and bench.py's
 cpu_baseline leg may import anything under oracle/, and only as the checker.
 
-Each function restates one function of the reference's numpy backend, which the survey names the
-float64 oracle of the step. Every pair appears once as (i, j) with i < j and is scattered to both
+Each function restates one function of the reference's numpy backend (sphSolver.py), the float64
+form of the step. Every pair appears once as (i, j) with i < j and is scattered to both
 members, the structure of sphSolver.py, which differs from the per-particle gathers of the C++
 restatement (oracle/sph_oracle.cpp) and of the device. Comparing the two therefore checks index,
 sign and symmetry mistakes that a retyped formula would repeat.

@@ -872,9 +872,9 @@ __global__ void SWEEP_BOUNDS((sweepWaves<D, Real>()))
                         fetch(at, other, !Coarse && (D == 3 || !Same));
                         bool near;
                         if constexpr (Coarse) {
-                            const int verdict = coarse.decide(me, other, shift);
-                            near = verdict > 0;
-                            if (verdict == 0) {
+                            const int decided = coarse.decide(me, other, shift);
+                            near = decided > 0;
+                            if (decided == 0) {
                                 fetch(at, other, true);
                                 Real dx, dy, dz;
                                 separation<D, Real>(me, other, shift, dx, dy, dz);
